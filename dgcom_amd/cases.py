@@ -1,0 +1,270 @@
+"""Workload generators: quadrilateral meshes, beds, forcing and initial states.
+
+Everything here is plain input data (numpy), shared by the HIP path and by its
+checker. The mesh connectivity uses the reference's 4-byte face link
+(`src/mesh/core/mesh2d.rs:54-107`): ``element << 4 | shared << 3 | left << 2 |
+face`` for an interior face, ``0xFFFFFFFF`` for a boundary face, whose
+boundary kind sits in a separate byte per face (wall 0, tidal open boundary 1).
+
+Structured grids follow `Mesh2D::uniform_rectangle_with_sides` /
+`uniform_periodic` (`mesh2d.rs:216-243,309-400`): vertices row by row,
+element ``k = j*nx + i`` counter-clockwise from its south-west corner, faces
+0..3 = south, east, north, west; the left side of an edge is the face that
+appears first in (element, face) order, and of a periodic pair the east /
+north face.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+BOUNDARY = 0xFFFFFFFF
+WALL = 0
+TIDE = 1
+
+# M2 period of the workload definitions: 12.42 h. (The reference's own
+# Doodson-number M2 is 12.4206012 h, `src/tides/astronomy.rs:360-363`.)
+M2_PERIOD_S = 12.42 * 3600.0
+
+
+def gll_nodes(order: int) -> np.ndarray:
+    """Closed-form Gauss-Lobatto-Legendre points on [-1, 1] for N = 1..3."""
+    if order == 1:
+        return np.array([-1.0, 1.0])
+    if order == 2:
+        return np.array([-1.0, 0.0, 1.0])
+    if order == 3:
+        a = math.sqrt(0.2)
+        return np.array([-1.0, -a, a, 1.0])
+    raise ValueError("order must be 1, 2 or 3")
+
+
+def link(element: int, face: int, left: bool, shared: bool) -> int:
+    return (element << 4) | (int(shared) << 3) | (int(left) << 2) | face
+
+
+@dataclasses.dataclass
+class Mesh:
+    corners: np.ndarray  # (K, 4, 2) float64, counter-clockwise from (r, s) = (-1, -1)
+    links: np.ndarray    # (K, 4) uint32
+    tags: np.ndarray     # (K, 4) uint8, boundary kind of boundary faces
+
+    @property
+    def n_elements(self) -> int:
+        return self.corners.shape[0]
+
+
+def structured_mesh(x0, x1, y0, y1, nx, ny, sides=(WALL, WALL, WALL, WALL), periodic_x=False,
+                    periodic_y=False, jitter=0.0, seed=0, distort=None) -> Mesh:
+    """An nx x ny grid of quads. ``sides`` = boundary kinds of [south, east, north,
+    west]. ``jitter``: interior vertices moved by independent uniform offsets of up
+    to ``jitter`` times the cell size in x and y (seeded). ``distort``: a callable
+    (x, y) -> (x', y') applied to every vertex (smooth periodic distortions)."""
+    dx = (x1 - x0) / nx
+    dy = (y1 - y0) / ny
+    jj, ii = np.meshgrid(np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    vx = x0 + ii * dx
+    vy = y0 + jj * dy
+    if jitter > 0.0:
+        rng = np.random.default_rng(seed)
+        ox = rng.uniform(-jitter, jitter, size=vx.shape) * dx
+        oy = rng.uniform(-jitter, jitter, size=vy.shape) * dy
+        interior = np.zeros(vx.shape, dtype=bool)
+        interior[1:-1, 1:-1] = True
+        vx = np.where(interior, vx + ox, vx)
+        vy = np.where(interior, vy + oy, vy)
+    if distort is not None:
+        vx, vy = distort(vx, vy)
+    K = nx * ny
+    j, i = np.divmod(np.arange(K), nx)
+    corners = np.empty((K, 4, 2))
+    for c, (di, dj) in enumerate(((0, 0), (1, 0), (1, 1), (0, 1))):
+        corners[:, c, 0] = vx[j + dj, i + di]
+        corners[:, c, 1] = vy[j + dj, i + di]
+    links = np.full((K, 4), BOUNDARY, dtype=np.uint64)
+    tags = np.zeros((K, 4), dtype=np.uint8)
+    k = np.arange(K, dtype=np.uint64)
+
+    def put(mask, face, nb, nb_face, left, shared):
+        links[mask, face] = (nb[mask] << 4) | (int(shared) << 3) | (int(left) << 2) | nb_face
+
+    # south (face 0) <-> north (face 2) of the element below; east (1) <-> west (3)
+    put(j > 0, 0, k - nx, 2, False, True)
+    put(j < ny - 1, 2, k + nx, 0, True, True)
+    put(i < nx - 1, 1, k + 1, 3, True, True)
+    put(i > 0, 3, k - 1, 1, False, True)
+    if periodic_x:
+        put(i == nx - 1, 1, k - (nx - 1), 3, True, False)
+        put(i == 0, 3, k + (nx - 1), 1, False, False)
+    if periodic_y:
+        put(j == ny - 1, 2, k - (ny - 1) * nx, 0, True, False)
+        put(j == 0, 0, k + (ny - 1) * nx, 2, False, False)
+    for face, (mask) in enumerate((j == 0, i == nx - 1, j == ny - 1, i == 0)):
+        b = mask & (links[:, face] == BOUNDARY)
+        tags[b, face] = sides[face]
+    return Mesh(corners=corners, links=links.astype(np.uint32), tags=tags)
+
+
+def node_coordinates(mesh: Mesh, order: int):
+    """Physical (x, y) of every GLL node, (K, NN) each, node i = j*(N+1) + a
+    (r fastest), by the bilinear map of `Mesh2D::bilinear` (`mesh2d.rs:578-595`)."""
+    x1 = gll_nodes(order)
+    s, r = np.meshgrid(x1, x1, indexing="ij")
+    r = r.ravel()
+    s = s.ravel()
+    n0 = (1.0 - r) * (1.0 - s) / 4.0
+    n1 = (1.0 + r) * (1.0 - s) / 4.0
+    n2 = (1.0 + r) * (1.0 + s) / 4.0
+    n3 = (1.0 - r) * (1.0 + s) / 4.0
+    c = mesh.corners
+    x = n0 * c[:, 0, 0:1] + n1 * c[:, 1, 0:1] + n2 * c[:, 2, 0:1] + n3 * c[:, 3, 0:1]
+    y = n0 * c[:, 0, 1:2] + n1 * c[:, 1, 1:2] + n2 * c[:, 2, 1:2] + n3 * c[:, 3, 1:2]
+    return x, y
+
+
+def ramp_depth(x, y, from_m, to_m, depth_m):
+    """`Ramp::depth` (`src/case/schema.rs:656-663`)."""
+    x0, y0 = from_m
+    dx, dy = to_m[0] - x0, to_m[1] - y0
+    s = np.clip(((x - x0) * dx + (y - y0) * dy) / (dx * dx + dy * dy), 0.0, 1.0)
+    return depth_m[0] + s * (depth_m[1] - depth_m[0])
+
+
+def gaussian_random_field(x, y, extent, sd, correlation, seed, spacing=None):
+    """A seeded, zero-mean Gaussian random field with a squared-exponential
+    covariance ``sd^2 exp(-r^2 / (2 correlation^2))``, by spectral synthesis on a
+    periodic grid twice the domain, sampled at (x, y) by bilinear interpolation.
+    ``extent`` = (x0, x1, y0, y1)."""
+    x0, x1, y0, y1 = extent
+    size = max(x1 - x0, y1 - y0) * 2.0
+    if spacing is None:
+        spacing = correlation / 8.0
+    m = int(2 ** math.ceil(math.log2(size / spacing)))
+    h = size / m
+    rng = np.random.default_rng(seed)
+    noise = rng.standard_normal((m, m))
+    k = np.fft.fftfreq(m, d=h) * 2.0 * math.pi
+    kx, ky = np.meshgrid(k, k, indexing="ij")
+    psd = np.exp(-0.5 * (kx * kx + ky * ky) * correlation * correlation)
+    field = np.real(np.fft.ifft2(np.fft.fft2(noise) * np.sqrt(psd)))
+    field *= sd / field.std()
+    gx = (np.asarray(x) - x0) / h
+    gy = (np.asarray(y) - y0) / h
+    ix = np.floor(gx).astype(np.int64)
+    iy = np.floor(gy).astype(np.int64)
+    fx = gx - ix
+    fy = gy - iy
+    ix %= m
+    iy %= m
+    ix1 = (ix + 1) % m
+    iy1 = (iy + 1) % m
+    return ((1 - fx) * (1 - fy) * field[ix, iy] + fx * (1 - fy) * field[ix1, iy]
+            + (1 - fx) * fy * field[ix, iy1] + fx * fy * field[ix1, iy1])
+
+
+@dataclasses.dataclass
+class Tide:
+    """Uniform harmonic tide `eta(t) = mean + R(t) sum A cos(w t + phase)`, with the
+    smooth ramp `R = 3 tau^2 - 2 tau^3`, `tau = t / ramp` (`harmonic_tide.rs:24-32,124-136`)."""
+    amplitude: tuple = ()
+    omega: tuple = ()
+    phase: tuple = ()
+    mean: float = 0.0
+    ramp: float = 3600.0
+
+    def elevation(self, t: float) -> float:
+        tide = 0.0
+        for a, w, p in zip(self.amplitude, self.omega, self.phase):
+            tide += 1.0 * a * math.cos(w * t + p + 0.0)
+        rf = 1.0
+        if self.ramp > 0.0:
+            tau = min(max(t / self.ramp, 0.0), 1.0)
+            rf = tau * tau * (3.0 - 2.0 * tau)
+        return self.mean + rf * tide
+
+
+def m2_tide(amplitude, phase=0.0, ramp=3600.0) -> Tide:
+    return Tide(amplitude=(amplitude,), omega=(2.0 * math.pi / M2_PERIOD_S,), phase=(phase,), ramp=ramp)
+
+
+@dataclasses.dataclass
+class Case:
+    """One 2D shallow-water problem: mesh, order, bed, friction, rotation, tide,
+    initial state and stepping."""
+    name: str
+    order: int
+    mesh: Mesh
+    bed: np.ndarray            # (K*NN,)
+    manning: np.ndarray        # (K,) per-element Manning n
+    f0: float
+    tide: Tide
+    h: np.ndarray
+    hu: np.ndarray
+    hv: np.ndarray
+    cfl: float = 0.5
+    scheme: int = 0            # 0 SSP-RK3, 1 SSP-RK(4,3)
+    n_steps: int = 1000
+    g: float = 9.81
+    h_min: float = 1e-6
+    h_dry: float = 1e-3
+
+    @property
+    def n_elements(self) -> int:
+        return self.mesh.n_elements
+
+    @property
+    def nn(self) -> int:
+        return (self.order + 1) ** 2
+
+    def dofs(self) -> int:
+        return 3 * self.n_elements * self.nn
+
+
+def still_water(bed, eta0=0.0):
+    """`still_water` (`src/case/run.rs:1158-1168`): h = max(0, eta0 - B), at rest."""
+    h = np.maximum(eta0 - bed, 0.0)
+    return h, np.zeros_like(h), np.zeros_like(h)
+
+
+def tidal_estuary(nx=100, ny=20, order=2, n_steps=1000) -> Case:
+    """Idealised tidal estuary: 10 km x 2 km, 100 x 20 affine quads, N = 2, bed depth
+    linear from 10 m at the open (west) end to 2 m at the closed end, 0.5 m M2 tide
+    on the west boundary, Manning 0.025, f = 1.2e-4, SSP-RK3 at CFL 0.5, from rest."""
+    mesh = structured_mesh(0.0, 10000.0, 0.0, 2000.0, nx, ny, sides=(WALL, WALL, WALL, TIDE))
+    x, y = node_coordinates(mesh, order)
+    bed = -ramp_depth(x, y, (0.0, 1000.0), (10000.0, 1000.0), (10.0, 2.0)).ravel()
+    h, hu, hv = still_water(bed)
+    return Case("tidal_estuary", order, mesh, bed, np.full(mesh.n_elements, 0.025), 1.2e-4, m2_tide(0.5),
+                h, hu, hv, cfl=0.5, scheme=0, n_steps=n_steps)
+
+
+def large_unstructured(n=1000, length=100_000.0, order=3, seed=1234, n_steps=1000, correlation=5000.0,
+                       mean_depth=40.0, sd=25.0, amplitude=1.0) -> Case:
+    """Large unstructured: an n x n grid over `length` square with interior vertices
+    jittered by up to 20 % of the cell size (seeded), N = 3, bed = -(mean_depth +
+    GRF(sd, correlation)) so part of the domain is dry or intertidal, an M2 tide on
+    the west boundary, Manning 0.025, f = 1.3e-4, SSP-RK3. The defaults are the
+    1,000,000-element workload; smaller `n` keeps the same physics."""
+    mesh = structured_mesh(0.0, length, 0.0, length, n, n, sides=(WALL, WALL, WALL, TIDE), jitter=0.2, seed=seed)
+    x, y = node_coordinates(mesh, order)
+    field = gaussian_random_field(x, y, (0.0, length, 0.0, length), sd, correlation, seed + 1)
+    bed = -(mean_depth + field).ravel()
+    h, hu, hv = still_water(bed)
+    return Case("large_unstructured", order, mesh, bed, np.full(mesh.n_elements, 0.025), 1.3e-4, m2_tide(amplitude),
+                h, hu, hv, cfl=0.5, scheme=0, n_steps=n_steps)
+
+
+def drying_beach(nx=24, ny=6, order=2, slope=0.01, length=2000.0, width=500.0, eta_amp=0.5, jitter=0.0,
+                 seed=7, n_steps=200) -> Case:
+    """A walled sloping beach (deep at the open west end, rising above the mean
+    level to the east) forced by a tide: it wets and dries, exercising the
+    subcells, the wet/dry correction and the tidal boundary."""
+    mesh = structured_mesh(0.0, length, 0.0, width, nx, ny, sides=(WALL, WALL, WALL, TIDE), jitter=jitter, seed=seed)
+    x, y = node_coordinates(mesh, order)
+    bed = (-5.0 + slope * x + 0.3 * np.sin(2 * math.pi * y / width)).ravel()
+    h, hu, hv = still_water(bed)
+    tide = Tide(amplitude=(eta_amp,), omega=(2 * math.pi / 600.0,), phase=(0.0,), ramp=60.0)
+    return Case("drying_beach", order, mesh, bed, np.full(mesh.n_elements, 0.025), 1.0e-4, tide, h, hu, hv,
+                cfl=0.5, scheme=0, n_steps=n_steps)

@@ -1,0 +1,175 @@
+// ORACLE — test infrastructure only (see swe2d_oracle.hpp): a C ABI over the
+// CPU restatement for the Python tests (ctypes), the smoke check and the
+// benchmark's CPU-baseline leg.
+#include <chrono>
+#include <cstdio>
+#include <string>
+
+#include "swe2d_solver.hpp"
+
+using oracle::Dual;
+using oracle::Solver;
+
+namespace {
+thread_local std::string g_err;
+struct Handle {
+    Solver<double>* s = nullptr;
+    std::vector<double> work;
+};
+}  // namespace
+
+extern "C" {
+
+const char* oc_last_error(void) { return g_err.c_str(); }
+
+void oc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+int oc_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+// flags: bit0 subcells, bit1 friction, bit2 wet_dry, bit3 manufactured source
+// surface: 0 hydrostatic HLL (WetDry), 1 entropy stable, 2 entropy conservative
+void* oc_create(int order, int64_t K, const double* corners, const uint32_t* links, const uint8_t* tags,
+                const double* bed, const double* manning, double g, double h_min, double h_dry, double f0,
+                int surface, int flags) {
+    if (order < 1 || order > 3) { g_err = "order must be 1..3"; return nullptr; }
+    Handle* h = new Handle;
+    h->s = new Solver<double>(order, K, corners, links, tags);
+    std::string e = h->s->init_geometry();
+    if (!e.empty()) { g_err = e; delete h->s; delete h; return nullptr; }
+    auto& c = h->s->cfg;
+    c.g = g; c.h_min = h_min; c.h_dry = h_dry; c.subcell_depth = h_dry; c.f0 = f0; c.surface = surface;
+    c.subcells = flags & 1; c.friction = flags & 2; c.wet_dry = flags & 4; c.manufactured = flags & 8;
+    size_t n = (size_t)K * h->s->ops.nn;
+    std::copy(bed, bed + n, h->s->bed.begin());
+    if (manning) std::copy(manning, manning + K, h->s->manning.begin());
+    return h;
+}
+
+void oc_destroy(void* p) {
+    Handle* h = (Handle*)p;
+    if (!h) return;
+    delete h->s;
+    delete h;
+}
+
+void oc_set_tide(void* p, double mean, double ramp, int n, const double* amp, const double* omega, const double* phase) {
+    auto& t = ((Handle*)p)->s->tide;
+    t.mean = mean; t.ramp = ramp;
+    t.amplitude.assign(amp, amp + n); t.omega.assign(omega, omega + n); t.phase.assign(phase, phase + n);
+}
+
+double oc_tide_elevation(void* p, double t) { return ((Handle*)p)->s->tide.elevation(t); }
+
+void oc_rhs(void* p, const double* h, const double* hu, const double* hv, double t, double* oh, double* ohu, double* ohv) {
+    ((Handle*)p)->s->rhs(h, hu, hv, t, oh, ohu, ohv);
+}
+
+// F* mass component at every face node from each element's side ([K][4][n1]); boundary faces included
+void oc_face_mass(void* p, const double* h, const double* hu, const double* hv, double t, double* out) {
+    Solver<double>& s = *((Handle*)p)->s;
+    s.face_pass(h, hu, hv);
+    int n1 = s.ops.n1;
+    double eta = s.tide.elevation(t);
+    for (int64_t k = 0; k < s.K; ++k)
+        for (int f = 0; f < 4; ++f)
+            for (int fi = 0; fi < n1; ++fi) {
+                size_t idx = ((size_t)k * 4 + f) * n1 + fi;
+                uint32_t l = s.links[4 * k + f];
+                if (oracle::link_is_boundary(l)) {
+                    double nx, ny, sj;
+                    s.normal(k, f, fi, nx, ny, sj);
+                    out[idx] = s.boundary_flux(k, f, s.node(h, hu, hv, k, s.ops.face_nodes[f][fi]), nx, ny, eta).h;
+                } else out[idx] = s.faces[idx].h;
+            }
+}
+
+void oc_stage(void* p, const double* bh, const double* bhu, const double* bhv, const double* ih, const double* ihu,
+              const double* ihv, double* oh, double* ohu, double* ohv, double a, double b, double c, double time, int last) {
+    ((Handle*)p)->s->stage(bh, bhu, bhv, ih, ihu, ihv, oh, ohu, ohv, a, b, c, time, last != 0);
+}
+
+void oc_step(void* p, double* h, double* hu, double* hv, double t, double dt, int scheme) {
+    Handle* hd = (Handle*)p;
+    hd->s->step(h, hu, hv, t, dt, scheme, hd->work);
+}
+
+double oc_dt(void* p, const double* h, const double* hu, const double* hv, double cfl, int scheme) {
+    return ((Handle*)p)->s->dt_ssp(h, hu, hv, cfl, scheme);
+}
+
+void oc_element_dt(void* p, const double* h, const double* hu, const double* hv, double cfl, int scheme, double* out) {
+    Solver<double>& s = *((Handle*)p)->s;
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < s.K; ++k) out[k] = s.element_dt(h, hu, hv, k, cfl, scheme, s.cfg.wet_dry);
+}
+
+// n_steps SSP steps from t at the adaptive dt (no landing); returns the end time.
+// dts (optional, n_steps) receives each step's dt.
+double oc_run(void* p, double* h, double* hu, double* hv, double t, int n_steps, double cfl, int scheme, double* dts) {
+    Handle* hd = (Handle*)p;
+    for (int i = 0; i < n_steps; ++i) {
+        double dt = hd->s->dt_ssp(h, hu, hv, cfl, scheme);
+        if (dts) dts[i] = dt;
+        hd->s->step(h, hu, hv, t, dt, scheme, hd->work);
+        t += dt;
+    }
+    return t;
+}
+
+// The same, timed: wall seconds of the step loop alone
+double oc_run_timed(void* p, double* h, double* hu, double* hv, double t, int n_steps, double cfl, int scheme, double* seconds) {
+    auto t0 = std::chrono::steady_clock::now();
+    double te = oc_run(p, h, hu, hv, t, n_steps, cfl, scheme, nullptr);
+    auto t1 = std::chrono::steady_clock::now();
+    *seconds = std::chrono::duration<double>(t1 - t0).count();
+    return te;
+}
+
+uint64_t oc_clips(void* p) { return ((Handle*)p)->s->clips; }
+
+// Integer wet/dry masks: node h < h_dry, and the element takes the subcells
+void oc_masks(void* p, const double* h, uint8_t* node_dry, uint8_t* elem_subcell) {
+    Solver<double>& s = *((Handle*)p)->s;
+    int nn = s.ops.nn;
+    for (int64_t k = 0; k < s.K; ++k) {
+        for (int i = 0; i < nn; ++i) node_dry[(size_t)k * nn + i] = h[(size_t)k * nn + i] < s.cfg.h_dry ? 1 : 0;
+        elem_subcell[k] = s.takes_subcells(h, k) ? 1 : 0;
+    }
+}
+
+// Operators and geometry, for the closed-form and metric-identity checks
+void oc_operators(int order, double* x1d, double* w1d, double* d1) {
+    oracle::Operators ops(order);
+    std::copy(ops.x1d.begin(), ops.x1d.end(), x1d);
+    std::copy(ops.w1d.begin(), ops.w1d.end(), w1d);
+    std::copy(ops.d1.begin(), ops.d1.end(), d1);
+}
+
+void oc_geometry(void* p, double* det_j, double* mass, double* jrx, double* jry, double* jsx, double* jsy) {
+    Solver<double>& s = *((Handle*)p)->s;
+    size_t n = (size_t)s.K * s.ops.nn;
+    for (size_t i = 0; i < n; ++i) {
+        det_j[i] = s.geom.det_j[i];
+        mass[i] = s.geom.mass[i];
+        jrx[i] = s.geom.det_j[i] * s.geom.rx[i];
+        jry[i] = s.geom.det_j[i] * s.geom.ry[i];
+        jsx[i] = s.geom.det_j[i] * s.geom.sx[i];
+        jsy[i] = s.geom.det_j[i] * s.geom.sy[i];
+    }
+}
+
+double oc_cbrt(double x) { return oracle::friction_cbrt(x); }
+
+}  // extern "C"

@@ -1,0 +1,535 @@
+// ORACLE — test infrastructure only. A CPU restatement of the reference's
+// 2D nodal-DG shallow-water time step (EmilLindfors/dgcom, `WetDry` split
+// form), used by the test suite, the smoke check and the CPU-baseline leg of
+// the benchmark as the checker. Nothing in the product (dgcom_amd/) may
+// include, link or call it.
+//
+// Each function cites the reference file:line it restates. The arithmetic
+// follows the reference's scalar kernel (src/solver/rhs/swe_2d_split_form.rs)
+// operation by operation where it is cheap to do so, so that a product built
+// without floating-point contraction can be compared bit for bit; the
+// differentiation matrix is built from the closed-form GLL formula rather
+// than the reference's faer `V_r V^-1` product (bits unpinned, values within
+// 1e-16; pinned by the metric identities and lake-at-rest tests).
+//
+// Everything is templated on the scalar `T` (double, or oracle::Dual for the
+// tangent-linear check of gradients); geometry stays in double.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+namespace oracle {
+
+// ---------------------------------------------------------------------------
+// Scalar helpers. Comparisons act on the value; a Dual carries a tangent.
+// ---------------------------------------------------------------------------
+struct Dual {
+    double v = 0.0, d = 0.0;
+    Dual() = default;
+    Dual(double v_) : v(v_), d(0.0) {}
+    Dual(double v_, double d_) : v(v_), d(d_) {}
+};
+inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+inline Dual operator-(Dual a) { return {-a.v, -a.d}; }
+inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+inline Dual operator/(Dual a, Dual b) {
+    double q = a.v / b.v;
+    return {q, (a.d - q * b.d) / b.v};
+}
+inline Dual& operator+=(Dual& a, Dual b) { return a = a + b; }
+inline Dual& operator-=(Dual& a, Dual b) { return a = a - b; }
+inline Dual& operator*=(Dual& a, Dual b) { return a = a * b; }
+inline bool operator<(Dual a, Dual b) { return a.v < b.v; }
+inline bool operator>(Dual a, Dual b) { return a.v > b.v; }
+inline bool operator<=(Dual a, Dual b) { return a.v <= b.v; }
+inline bool operator>=(Dual a, Dual b) { return a.v >= b.v; }
+inline double val(double x) { return x; }
+inline double val(Dual x) { return x.v; }
+inline double sq_sqrt(double x) { return std::sqrt(x); }
+inline Dual sq_sqrt(Dual x) {
+    double s = std::sqrt(x.v);
+    return {s, s > 0.0 ? x.d / (2.0 * s) : 0.0};
+}
+inline double fabs_(double x) { return std::fabs(x); }
+inline Dual fabs_(Dual x) { return x.v < 0.0 ? -x : x; }
+// f64::max / f64::min of the reference (NaN-ignoring is not relied on)
+template <class T> inline T vmax(T a, T b) { return a > b ? a : (b > a ? b : a); }
+template <class T> inline T vmin(T a, T b) { return a < b ? a : (b < a ? b : a); }
+inline double vmax(double a, double b) { return std::fmax(a, b); }
+inline double vmin(double a, double b) { return std::fmin(a, b); }
+
+// ---------------------------------------------------------------------------
+// GLL operators: src/polynomial/nodes.rs:17-107, src/polynomial/legendre.rs:62-93,
+// src/operators/operators_2d.rs:114-200,345-483
+// ---------------------------------------------------------------------------
+inline void legendre_and_derivative(int n, double x, double& p_n, double& dp_n) {
+    // src/polynomial/legendre.rs:62-93
+    if (n == 0) { p_n = 1.0; dp_n = 0.0; return; }
+    if (n == 1) { p_n = x; dp_n = 1.0; return; }
+    double p_prev = 1.0, p_curr = x;
+    for (int k = 1; k < n; ++k) {
+        double p_next = ((double)(2 * k + 1) * x * p_curr - (double)k * p_prev) / (double)(k + 1);
+        p_prev = p_curr;
+        p_curr = p_next;
+    }
+    p_n = p_curr;
+    if (std::fabs(x - 1.0) < 1e-14) dp_n = (double)(n * (n + 1)) / 2.0;
+    else if (std::fabs(x + 1.0) < 1e-14) dp_n = ((n % 2 == 0) ? -1.0 : 1.0) * (double)(n * (n + 1)) / 2.0;
+    else dp_n = (double)n * (x * p_curr - p_prev) / (x * x - 1.0);
+}
+
+inline std::vector<double> gll_nodes(int n) {
+    // Newton iteration from the Chebyshev-Lobatto points (src/polynomial/nodes.rs:17-86)
+    if (n == 1) return {-1.0, 1.0};
+    std::vector<double> x(n + 1);
+    for (int j = 0; j <= n; ++j) x[j] = -std::cos(M_PI * (double)j / (double)n);
+    x[0] = -1.0;
+    x[n] = 1.0;
+    for (int j = 1; j < n; ++j) {
+        double xj = x[j];
+        for (int it = 0; it < 100; ++it) {
+            double p, dp;
+            legendre_and_derivative(n, xj, p, dp);
+            double update = (1.0 - xj * xj) * dp / ((double)n * (double)(n + 1) * p);
+            if (std::fabs(update) < 1e-15) break;
+            xj += update;
+        }
+        x[j] = xj;
+    }
+    return x;
+}
+
+inline std::vector<double> gll_weights(int n, const std::vector<double>& x) {
+    // w_j = 2 / (N(N+1) P_N(x_j)^2)  (src/polynomial/nodes.rs:88-107)
+    std::vector<double> w;
+    double denom = (double)(n * (n + 1));
+    for (double xj : x) {
+        double p, dp;
+        legendre_and_derivative(n, xj, p, dp);
+        w.push_back(2.0 / (denom * p * p));
+    }
+    return w;
+}
+
+struct Operators {
+    int order = 0, n1 = 0, nn = 0;
+    std::vector<double> x1d, w1d, d1;     // d1 row-major n1*n1
+    std::vector<double> r, s, w2d;        // tensor nodes, r fastest (legendre_2d.rs:108-144)
+    std::vector<int> face_nodes[4];       // operators_2d.rs:345-377
+    std::vector<double> lift[4];          // [node*n1 + fi]  operators_2d.rs:445-483
+
+    explicit Operators(int order_) : order(order_), n1(order_ + 1), nn((order_ + 1) * (order_ + 1)) {
+        x1d = gll_nodes(order);
+        w1d = gll_weights(order, x1d);
+        // Differentiation matrix from the Lagrange basis on the GLL points:
+        // D_ij = P_N(x_i) / (P_N(x_j) (x_i - x_j)), D_00 = -N(N+1)/4, D_NN = N(N+1)/4.
+        d1.assign(n1 * n1, 0.0);
+        std::vector<double> p(n1);
+        for (int i = 0; i < n1; ++i) {
+            double dp;
+            legendre_and_derivative(order, x1d[i], p[i], dp);
+        }
+        for (int i = 0; i < n1; ++i)
+            for (int j = 0; j < n1; ++j) {
+                if (i != j) d1[i * n1 + j] = p[i] / (p[j] * (x1d[i] - x1d[j]));
+            }
+        d1[0] = -(double)(order * (order + 1)) / 4.0;
+        d1[n1 * n1 - 1] = (double)(order * (order + 1)) / 4.0;
+        for (int j = 0; j < n1; ++j)
+            for (int i = 0; i < n1; ++i) {
+                r.push_back(x1d[i]);
+                s.push_back(x1d[j]);
+                w2d.push_back(w1d[i] * w1d[j]);
+            }
+        for (int i = 0; i < n1; ++i) face_nodes[0].push_back(i);                        // s = -1
+        for (int j = 0; j < n1; ++j) face_nodes[1].push_back(j * n1 + n1 - 1);          // r = +1
+        for (int i = n1 - 1; i >= 0; --i) face_nodes[2].push_back((n1 - 1) * n1 + i);   // s = +1, reversed
+        for (int j = n1 - 1; j >= 0; --j) face_nodes[3].push_back(j * n1);              // r = -1, reversed
+        for (int f = 0; f < 4; ++f) {
+            lift[f].assign(nn * n1, 0.0);
+            for (int i = 0; i < nn; ++i)
+                for (int fi = 0; fi < n1; ++fi) {
+                    double e = face_nodes[f][fi] == i ? 1.0 : 0.0;
+                    lift[f][i * n1 + fi] = (1.0 / w2d[i]) * (e * w1d[fi]);
+                }
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Face links: the reference's 4-byte FaceLink (src/mesh/core/mesh2d.rs:54-107)
+// ---------------------------------------------------------------------------
+constexpr uint32_t LINK_BOUNDARY = 0xFFFFFFFFu;
+inline bool link_is_boundary(uint32_t l) { return l == LINK_BOUNDARY; }
+inline int64_t link_element(uint32_t l) { return (int64_t)(l >> 4); }
+inline int link_face(uint32_t l) { return (int)(l & 3u); }
+inline bool link_is_left(uint32_t l) { return (l & 4u) != 0; }
+
+// Boundary kinds of a boundary face (the boundary tag's condition)
+enum BoundaryKind : uint8_t { BC_WALL = 0, BC_TIDE = 1 };
+
+// ---------------------------------------------------------------------------
+// Geometry: src/operators/geometric.rs:126-221,438-460
+// ---------------------------------------------------------------------------
+struct Bilinear {
+    double x_r, x_s, y_r, y_s, det;
+};
+inline Bilinear bilinear_metric(const double* v, double r, double s) {
+    // geometric.rs:438-460; v = x0,y0,x1,y1,x2,y2,x3,y3
+    double x0 = v[0], y0 = v[1], x1 = v[2], y1 = v[3], x2 = v[4], y2 = v[5], x3 = v[6], y3 = v[7];
+    Bilinear m;
+    m.x_r = 0.25 * ((1.0 - s) * (x1 - x0) + (1.0 + s) * (x2 - x3));
+    m.y_r = 0.25 * ((1.0 - s) * (y1 - y0) + (1.0 + s) * (y2 - y3));
+    m.x_s = 0.25 * ((1.0 - r) * (x3 - x0) + (1.0 + r) * (x2 - x1));
+    m.y_s = 0.25 * ((1.0 - r) * (y3 - y0) + (1.0 + r) * (y2 - y1));
+    m.det = m.x_r * m.y_s - m.x_s * m.y_r;
+    return m;
+}
+
+inline bool is_parallelogram(const double* v) {
+    // geometric.rs:464-475
+    double bx = v[0] - v[2] + v[4] - v[6], by = v[1] - v[3] + v[5] - v[7];
+    double scale = 0.0;
+    for (int i = 0; i < 4; ++i) {
+        int j = (i + 1) % 4;
+        scale = std::fmax(scale, std::hypot(v[2 * j] - v[2 * i], v[2 * j + 1] - v[2 * i + 1]));
+    }
+    return std::hypot(bx, by) <= 1.0e-12 * scale;
+}
+
+inline void bilinear_point(const double* v, double r, double s, double& x, double& y) {
+    // mesh2d.rs:578-595
+    double n0 = (1.0 - r) * (1.0 - s) / 4.0, n1 = (1.0 + r) * (1.0 - s) / 4.0;
+    double n2 = (1.0 + r) * (1.0 + s) / 4.0, n3 = (1.0 - r) * (1.0 + s) / 4.0;
+    x = n0 * v[0] + n1 * v[2] + n2 * v[4] + n3 * v[6];
+    y = n0 * v[1] + n1 * v[3] + n2 * v[5] + n3 * v[7];
+}
+
+struct ElementGeometry {
+    bool affine;
+    double rx, ry, sx, sy, det_j, det_j_inv;  // node 0
+    double fnx[4], fny[4], fsj[4];            // face node 0 of each face
+};
+
+struct Geometry {
+    int64_t K = 0;
+    int nn = 0, n1 = 0;
+    std::vector<double> rx, ry, sx, sy, det_j, det_j_inv, mass, area, surface_j, nx, ny;
+    std::vector<ElementGeometry> elements;
+
+    // Returns an empty string on success, else the reason (geometric.rs:163-170)
+    std::string compute(const Operators& ops, int64_t K_, const double* corners) {
+        K = K_; nn = ops.nn; n1 = ops.n1;
+        size_t nv = (size_t)K * nn, nf = (size_t)K * 4 * n1;
+        rx.resize(nv); ry.resize(nv); sx.resize(nv); sy.resize(nv);
+        det_j.resize(nv); det_j_inv.resize(nv); mass.resize(nv); area.resize(K);
+        surface_j.resize(nf); nx.resize(nf); ny.resize(nf);
+        elements.resize(K);
+        for (int64_t k = 0; k < K; ++k) {
+            const double* v = corners + 8 * k;
+            bool affine = is_parallelogram(v);
+            double a = 0.0;
+            for (int i = 0; i < nn; ++i) {
+                Bilinear m = bilinear_metric(v, ops.r[i], ops.s[i]);
+                if (!(std::isfinite(m.det) && m.det > 0.0))
+                    return "element " + std::to_string(k) + " has a non-positive Jacobian at node " + std::to_string(i);
+                size_t n = (size_t)k * nn + i;
+                rx[n] = m.y_s / m.det;
+                ry[n] = -m.x_s / m.det;
+                sx[n] = -m.y_r / m.det;
+                sy[n] = m.x_r / m.det;
+                det_j[n] = m.det;
+                det_j_inv[n] = 1.0 / m.det;
+                mass[n] = ops.w2d[i] * m.det;
+                a += mass[n];
+            }
+            area[k] = a;
+            for (int f = 0; f < 4; ++f)
+                for (int fi = 0; fi < n1; ++fi) {
+                    int node = ops.face_nodes[f][fi];
+                    Bilinear m = bilinear_metric(v, ops.r[node], ops.s[node]);
+                    double vx, vy;
+                    switch (f) {
+                        case 0: vx = m.y_r; vy = -m.x_r; break;
+                        case 1: vx = m.y_s; vy = -m.x_s; break;
+                        case 2: vx = -m.y_r; vy = m.x_r; break;
+                        default: vx = -m.y_s; vy = m.x_s; break;
+                    }
+                    double sj = std::sqrt(vx * vx + vy * vy);
+                    size_t fidx = ((size_t)k * 4 + f) * n1 + fi;
+                    surface_j[fidx] = sj;
+                    nx[fidx] = vx / sj;
+                    ny[fidx] = vy / sj;
+                }
+            ElementGeometry& e = elements[k];
+            size_t n0 = (size_t)k * nn;
+            e.affine = affine;
+            e.rx = rx[n0]; e.ry = ry[n0]; e.sx = sx[n0]; e.sy = sy[n0];
+            e.det_j = det_j[n0]; e.det_j_inv = det_j_inv[n0];
+            for (int f = 0; f < 4; ++f) {
+                size_t f0 = ((size_t)k * 4 + f) * n1;
+                e.fnx[f] = nx[f0]; e.fny[f] = ny[f0]; e.fsj[f] = surface_j[f0];
+            }
+        }
+        return "";
+    }
+    // geometric.rs:299-307: (J grad r, J grad s)
+    void contravariant(int64_t k, int i, double& arx, double& ary, double& asx, double& asy) const {
+        size_t n = (size_t)k * nn + i;
+        double j = det_j[n];
+        arx = j * rx[n]; ary = j * ry[n]; asx = j * sx[n]; asy = j * sy[n];
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Pointwise physics
+// ---------------------------------------------------------------------------
+template <class T> struct State { T h, hu, hv; };
+template <class T> inline State<T> operator+(State<T> a, State<T> b) { return {a.h + b.h, a.hu + b.hu, a.hv + b.hv}; }
+template <class T> inline State<T> operator-(State<T> a, State<T> b) { return {a.h - b.h, a.hu - b.hu, a.hv - b.hv}; }
+template <class T> inline State<T> operator-(State<T> a) { return {-a.h, -a.hu, -a.hv}; }
+template <class T, class S> inline State<T> scale(S c, State<T> a) { return {T(c) * a.h, T(c) * a.hu, T(c) * a.hv}; }
+
+template <class T> struct NodeState { T h, hu, hv, u, v, b; };
+template <class T> struct Side { T h, u, v, b; };
+
+// flux/swe_2d_entropy.rs:97-124
+template <class T> inline NodeState<T> node_state(T h, T hu, T hv, T b, double h_min) {
+    bool wet = h > T(h_min);
+    T inv_h = T(1.0) / h;
+    NodeState<T> q{h, hu, hv, T(0.0), T(0.0), b};
+    if (wet) { q.u = hu * inv_h; q.v = hv * inv_h; }
+    return q;
+}
+template <class T> inline Side<T> side_of(const NodeState<T>& q) { return {q.h, q.u, q.v, q.b}; }
+
+// flux/swe_2d_entropy.rs:140-158
+template <class T> inline State<T> wintermeyer_flux(const NodeState<T>& l, const NodeState<T>& r, T mx, T my, double g) {
+    T half(0.5);
+    T mass = half * ((l.hu + r.hu) * mx + (l.hv + r.hv) * my);
+    T pressure = T(0.5 * g) * (l.h * r.h);
+    T u_avg = half * (l.u + r.u);
+    T v_avg = half * (l.v + r.v);
+    return {mass, mass * u_avg + pressure * mx, mass * v_avg + pressure * my};
+}
+
+// flux/swe_2d_entropy.rs:166-203 (EntropyStable interface; oracle-only, for the manufactured KAT)
+template <class T> inline State<T> wintermeyer_bed_interface(T h_int, T b_int, T b_ext, T nx, T ny, double g) {
+    T force = T(0.5 * g) * h_int * (b_ext - b_int);
+    return {T(0.0), force * nx, force * ny};
+}
+template <class T> inline State<T> es_dissipation(const NodeState<T>& qi, const NodeState<T>& qe, T nx, T ny, double g) {
+    T speed_int = fabs_(qi.u * nx + qi.v * ny) + sq_sqrt(T(g) * vmax(qi.h, T(0.0)));
+    T speed_ext = fabs_(qe.u * nx + qe.v * ny) + sq_sqrt(T(g) * vmax(qe.h, T(0.0)));
+    T sc = T(-0.5) * vmax(speed_int, speed_ext);
+    T jump_eta = (qe.h + qe.b) - (qi.h + qi.b);
+    T h_avg = T(0.5) * (qi.h + qe.h);
+    T u_avg = T(0.5) * (qi.u + qe.u);
+    T v_avg = T(0.5) * (qi.v + qe.v);
+    return {sc * jump_eta, sc * (u_avg * jump_eta + h_avg * (qe.u - qi.u)), sc * (v_avg * jump_eta + h_avg * (qe.v - qi.v))};
+}
+
+// flux/swe_2d.rs:234-331 HllSide / hll_flux_face_aligned, :337-374 einfeldt
+template <class T> struct HllSide { T h, hun, hut, un, ut; };
+
+template <class T> inline State<T> hll_face_aligned(const HllSide<T>& l, const HllSide<T>& r, double g, double h_min) {
+    T h_l = l.h, h_r = r.h, zero(0.0), hm(h_min);
+    bool dry = (h_l <= hm) && (h_r <= hm);
+    T sqrt_h_l = sq_sqrt(vmax(h_l, zero)), sqrt_h_r = sq_sqrt(vmax(h_r, zero));
+    T sqrt_g = sq_sqrt(T(g));
+    T c_l = sqrt_g * sqrt_h_l, c_r = sqrt_g * sqrt_h_r;
+    // Einfeldt speeds
+    bool roe = (sqrt_h_l + sqrt_h_r) > T(1e-10);
+    T h_roe = T(0.5) * (h_l + h_r);
+    T un_roe = roe ? (sqrt_h_l * l.un + sqrt_h_r * r.un) / (sqrt_h_l + sqrt_h_r) : zero;
+    T c_roe = roe ? sq_sqrt(T(g) * h_roe) : zero;
+    T s_l = (h_l > hm) ? vmin(l.un - c_l, un_roe - c_roe) : r.un - T(2.0) * c_r;
+    T s_r = (h_r > hm) ? vmax(r.un + c_r, un_roe + c_roe) : l.un + T(2.0) * c_l;
+    T half_g(0.5 * g);
+    State<T> f_l{l.h * l.un, l.h * l.un * l.un + half_g * l.h * l.h, l.h * l.un * l.ut};
+    State<T> f_r{r.h * r.un, r.h * r.un * r.un + half_g * r.h * r.h, r.h * r.un * r.ut};
+    T inv_ds = T(1.0) / (s_r - s_l);
+    T s_lr = s_l * s_r;
+    State<T> star{inv_ds * (s_r * f_l.h - s_l * f_r.h + s_lr * (h_r - h_l)),
+                  inv_ds * (s_r * f_l.hu - s_l * f_r.hu + s_lr * (r.hun - l.hun)),
+                  inv_ds * (s_r * f_l.hv - s_l * f_r.hv + s_lr * (r.hut - l.hut))};
+    bool from_l = s_l >= zero, from_r = s_r <= zero;
+    auto pick = [&](T fl, T fr, T st) { return dry ? zero : (from_l ? fl : (from_r ? fr : st)); };
+    return {pick(f_l.h, f_r.h, star.h), pick(f_l.hu, f_r.hu, star.hu), pick(f_l.hv, f_r.hv, star.hv)};
+}
+
+// swe_2d_split_form.rs:147-156
+template <class T> inline T wall_ram_pressure(T h, T un, T sill, double g) {
+    T zero(0.0);
+    un = vmax(un, zero);
+    T head = un * un * T(0.5 / g);
+    T ram = (T(1.0) - head / sill) * h * un * (T(2.0) * un + sq_sqrt(T(g) * h));
+    return (sill > head) ? ram : zero;
+}
+
+// swe_2d_split_form.rs:336-402: (flux seen from a, flux seen from b), both along +m
+template <class T> inline void hydrostatic_hll(const Side<T>& qa, const Side<T>& qb, T mx, T my, double g, double h_min,
+                                               State<T>& out_a, State<T>& out_b) {
+    T zero(0.0), hm(h_min);
+    T norm = sq_sqrt(mx * mx + my * my);
+    T inv_norm = T(1.0) / norm;
+    T nx = mx * inv_norm, ny = my * inv_norm;
+    T b_face = vmax(qa.b, qb.b);
+    auto side = [&](const Side<T>& q) {
+        T h_star = vmax(q.h + q.b - b_face, zero);
+        bool wet = q.h > hm;
+        T u = wet ? q.u : zero, v = wet ? q.v : zero;
+        T un = u * nx + v * ny, ut = -u * ny + v * nx;
+        T hun = h_star * un, hut = h_star * ut;
+        bool star_wet = h_star > hm;
+        return HllSide<T>{h_star, hun, hut, star_wet ? un : zero, star_wet ? ut : zero};
+    };
+    HllSide<T> a = side(qa), b = side(qb);
+    State<T> rot = hll_face_aligned(a, b, g, h_min);
+    // rotate_from_normal (flux/swe_2d.rs:38-44)
+    State<T> phys{rot.h, rot.hu * nx - rot.hv * ny, rot.hu * ny + rot.hv * nx};
+    State<T> flux{norm * phys.h, norm * phys.hu, norm * phys.hv};
+    bool hll_dry = (a.h <= hm) && (b.h <= hm);
+    T h_star_a = hll_dry ? zero : a.h, h_star_b = hll_dry ? zero : b.h;
+    T wall_a = zero, wall_b = zero;
+    if (hll_dry) {
+        auto wall = [&](const Side<T>& q, double towards) {
+            T un = T(towards) * (q.u * nx + q.v * ny);
+            T sill = b_face - (q.h + q.b);
+            T ram = wall_ram_pressure(q.h, un, sill, g);
+            return (q.h > hm) ? ram : zero;
+        };
+        wall_a = wall(qa, 1.0);
+        wall_b = wall(qb, -1.0);
+    }
+    T half_g(0.5 * g);
+    T dp_a = half_g * (qa.h * qa.h - h_star_a * h_star_a) + wall_a;
+    T dp_b = half_g * (qb.h * qb.h - h_star_b * h_star_b) + wall_b;
+    out_a = {flux.h + zero, flux.hu + dp_a * mx, flux.hv + dp_a * my};
+    out_b = {flux.h + zero, flux.hu + dp_b * mx, flux.hv + dp_b * my};
+}
+
+// swe_2d_split_form.rs:168-203 limited_slope / SlopeDistances
+template <class T> struct SlopeDistances { T central, minus, plus; };
+template <class T> inline SlopeDistances<T> slope_distances(const T xi[3], T x_l, T x_r) {
+    T zero(0.0), one(1.0), half(0.5);
+    auto side = [&](T face, T node) { return one / ((face > zero) ? face : half * node); };
+    return {one / (xi[2] - xi[0]), side(xi[1] - x_l, xi[1] - xi[0]), side(x_r - xi[1], xi[2] - xi[1])};
+}
+template <class T> inline T limited_slope(const T q[3], const SlopeDistances<T>& inv) {
+    T zero(0.0), one(1.0);
+    T dm = q[1] - q[0], dp = q[2] - q[1];
+    bool extremum = (dm * dp) <= zero;
+    T central = (q[2] - q[0]) * inv.central;
+    T bound = vmin(fabs_(dm * inv.minus), fabs_(dp * inv.plus));
+    T sign = (central < zero) ? -one : one;
+    return extremum ? zero : sign * vmin(fabs_(central), bound);
+}
+
+// swe_2d_split_form.rs:279-317
+template <class T> inline void reconstruct_subcell(const Side<T> nodes[3], const T xi[3], T x_l, T x_r, double h_dry,
+                                                   Side<T>& left, Side<T>& right) {
+    const Side<T>& m = nodes[0];
+    const Side<T>& c = nodes[1];
+    const Side<T>& p = nodes[2];
+    T zero(0.0), hd(h_dry);
+    T eta[3] = {m.h + m.b, c.h + c.b, p.h + p.b};
+    T h[3] = {m.h, c.h, p.h};
+    T wet_surface(-std::numeric_limits<double>::infinity());
+    for (int i = 0; i < 3; ++i) {
+        bool higher = eta[i] > wet_surface;
+        if ((h[i] >= hd) && higher) wet_surface = eta[i];
+    }
+    bool flooding = false;
+    for (int i = 0; i < 3; ++i) flooding = flooding || ((h[i] < hd) && (eta[i] < wet_surface));
+    SlopeDistances<T> inv = slope_distances(xi, x_l, x_r);
+    T s_h = limited_slope(h, inv);
+    T s_eta = flooding ? zero : limited_slope(eta, inv);
+    T uu[3] = {m.u, c.u, p.u}, vv[3] = {m.v, c.v, p.v};
+    T s_u = limited_slope(uu, inv);
+    T s_v = limited_slope(vv, inv);
+    auto at = [&](T x) {
+        T d = x - xi[1];
+        T hh = vmax(c.h + s_h * d, zero);
+        return Side<T>{hh, c.u + s_u * d, c.v + s_v * d, eta[1] + s_eta * d - hh};
+    };
+    left = at(x_l);
+    right = at(x_r);
+}
+
+// boundary/characteristic.rs:481-520
+template <class T> inline State<T> characteristic_state(const State<T>& interior, T h_e, T un_e, T ut_e, T nx, T ny,
+                                                        double g, double h_min) {
+    T zero(0.0), hm(h_min);
+    T h_i = vmax(interior.h, zero);
+    T un_i = zero, ut_i = zero;
+    if (h_i > hm) {
+        T u = interior.hu / h_i, v = interior.hv / h_i;
+        un_i = u * nx + v * ny;
+        ut_i = -u * ny + v * nx;
+    }
+    T c_i = sq_sqrt(T(g) * h_i), c_e = sq_sqrt(T(g) * h_e);
+    T w_out = un_i + T(2.0) * c_i, w_in = un_e - T(2.0) * c_e;
+    bool wi = h_i > hm, we = h_e > hm;
+    T h_b, un_b;
+    if (!wi && !we) { h_b = zero; un_b = zero; }
+    else if (wi && un_i >= c_i) { h_b = h_i; un_b = un_i; }
+    else if (we && un_e <= -c_e) { h_b = h_e; un_b = un_e; }
+    else if (wi && !we) { T c = vmax(w_out / T(3.0), zero); h_b = c * c / T(g); un_b = c; }
+    else if (!wi && we) { T c = vmax(-w_in / T(3.0), zero); h_b = c * c / T(g); un_b = -c; }
+    else { T c = vmax(T(0.25) * (w_out - w_in), zero); h_b = c * c / T(g); un_b = T(0.5) * (w_out + w_in); }
+    T ut_b = (un_b > zero) ? ut_i : ut_e;
+    // SWEState2D::from_primitives (state/swe_2d.rs:41-47)
+    T u_b = un_b * nx - ut_b * ny, v_b = un_b * ny + ut_b * nx;
+    return {h_b, h_b * u_b, h_b * v_b};
+}
+
+// source/swe_2d/friction.rs:39-64: cube root, Halley iterations from a bit-pattern guess
+inline double cbrt_guess(double x) {
+    uint64_t bits;
+    std::memcpy(&bits, &x, 8);
+    bits = bits / 3 + 0x2A9F7893782DA1CEull;
+    double y;
+    std::memcpy(&y, &bits, 8);
+    return y;
+}
+template <class T> inline T cbrt_refine(T x, T y) {
+    for (int i = 0; i < 3; ++i) {
+        T y3 = y * y * y;
+        y += y * ((x - y3) / (T(2.0) * y3 + x));
+    }
+    return y;
+}
+template <class T> inline T friction_cbrt(T x) {
+    double xv = val(x);
+    // zero, subnormal, negative and non-finite inputs: the library cube root
+    // (no tangent: depths reaching it are floored at h_min first)
+    if (!(xv >= std::numeric_limits<double>::min() && std::isfinite(xv))) return T(std::cbrt(xv));
+    return cbrt_refine(x, T(cbrt_guess(xv)));
+}
+
+// tides: harmonic_tide.rs:24-32,124-136 (identity nodal corrections)
+struct Tide {
+    double mean = 0.0, ramp = 0.0;  // ramp <= 0: none
+    std::vector<double> amplitude, omega, phase;
+    double elevation(double t) const {
+        double tide = 0.0;
+        for (size_t i = 0; i < amplitude.size(); ++i) tide += 1.0 * amplitude[i] * std::cos(omega[i] * t + phase[i] + 0.0);
+        double rf = 1.0;
+        if (ramp > 0.0) {
+            double tau = std::clamp(t / ramp, 0.0, 1.0);
+            rf = tau * tau * (3.0 - 2.0 * tau);
+        }
+        return mean + rf * tide;
+    }
+};
+
+}  // namespace oracle
