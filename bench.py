@@ -1,0 +1,157 @@
+"""Benchmark of the 2D shallow-water `WetDry` SSP-RK3 time step on one MI355X.
+
+Workload (the large unstructured configuration): 1,000,000 quads from a
+1000 x 1000 grid over 100 km x 100 km with interior vertices jittered by up to
+20 % of the cell size (seeded), order N = 3 (16 nodes per element), bed =
+-(40 m + a seeded Gaussian random field of sd 25 m and 5 km correlation), a
+1 m M2 tide on the west boundary, Manning n = 0.025, f = 1.3e-4 s^-1, SSP-RK3
+at CFL 0.5 with the positivity-relaxed time step, from still water.
+
+A step is one adaptive SSP-RK3 time step (the time-step kernel, its global
+minimum read back to the host, three fused stage kernels), inputs resident in
+HBM. value = DOF-updates/s = 3 * K * (N+1)^2 * 3 stages * steps / wall time.
+
+Besides the timed loop: the stage kernel's average launch duration from HIP
+events on the library's stream gives the roofline point (algorithmic bytes per
+element-stage, SoA FP64: 8*NN*(3 input + 3 base if base != input + 1 bed +
+3 out) + 64 corners + 16 face links + 8 Manning n); rank 0 also times the CPU
+restatement of the reference (OpenMP, all host threads) on a bounded sample of
+the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from dgcom_amd import cases, swe2d  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+STAGES = 3
+
+
+def stage_bytes(nn: int, base_differs: bool) -> int:
+    """Algorithmic HBM bytes of one element-stage."""
+    return 8 * nn * (3 + (3 if base_differs else 0) + 1 + 3) + 64 + 16 + 8
+
+
+def read_traffic(workload: str):
+    """HBM bytes per stage launch measured with rocprofv3 counters, if a summary
+    for this workload has been committed (profiles/traffic_<workload>.json)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("bytes_per_stage_launch")
+    return None
+
+
+def cpu_baseline(case, seconds_target: float):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as checker
+
+    threads = checker.max_threads()
+    orc = checker.Oracle.from_case(case)
+    h, hu, hv = case.h.copy(), case.hu.copy(), case.hv.copy()
+    # one warm step, then as many as fit the target (at least one)
+    t, sec = orc.run_timed(h, hu, hv, 0.0, 1, case.cfl)
+    n = max(1, min(50, int(seconds_target / max(sec, 1e-9))))
+    t, sec = orc.run_timed(h, hu, hv, t, n, case.cfl)
+    dofs = 3 * case.n_elements * case.nn * STAGES * n
+    try:
+        model = [line.split(":", 1)[1].strip() for line in open("/proc/cpuinfo") if line.startswith("model name")][0]
+    except Exception:
+        model = "unknown"
+    return {"value": dofs / sec, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{n} SSP-RK3 steps of the full workload after 1 warm step, {sec:.1f} s, OpenMP "
+                      f"{threads} threads on {model}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=1000, help="grid side (1000: the 1M-element workload)")
+    ap.add_argument("--fma", action="store_true", help="the FMA-contracted variant of the library")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--profile-steps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the time step on a single device "
+              "(see the sharding design in DESIGN.md); --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.time()
+    case = cases.large_unstructured(n=args.n)
+    gen_s = time.time() - t0
+    K, nn = case.n_elements, case.nn
+    phys = swe2d.SWEPhysics2D.from_case(case, fma=args.fma)
+    state = phys.state(case.h, case.hu, case.hv)
+
+    t = phys.run(state, 0.0, args.warmup, case.cfl) if args.warmup > 0 else 0.0
+    phys.sync()
+    w0 = time.perf_counter()
+    t = phys.run(state, t, args.steps, case.cfl)
+    phys.sync()
+    wall = time.perf_counter() - w0
+    assert state.is_finite(), "the state is not finite"
+    dofs = 3 * K * nn * STAGES * args.steps
+    value = dofs / wall
+
+    # Roofline point of the stage kernel: HIP events around each launch (separate pass)
+    phys.timing(True)
+    t = phys.run(state, t, args.profile_steps, case.cfl)
+    stage_ms, launches, dt_ms, dt_launches = phys.timing_read()
+    phys.timing(False)
+    steps_p = launches // STAGES
+    bytes_total = steps_p * K * (stage_bytes(nn, False) + 2 * stage_bytes(nn, True))
+    achieved = bytes_total / (stage_ms * 1e-3) / 1e9
+    copy_bytes = 2 * 3 * K * nn * 8
+    copy_ms = phys.copy_bench(3 * K * nn * 8, reps=20)
+    copy_gbs = copy_bytes / (copy_ms * 1e-3) / 1e9
+    nd, es = phys.masks(state)
+    workload = f"large_unstructured_n{args.n}_N3"
+    traffic = read_traffic(workload)
+
+    out = {
+        "metric": "SWE-2D RK-stage DOF-updates/s + %HBM roofline; adjoint/forward time ratio",
+        "value": value,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded jittered mesh and Gaussian-random-field bed)",
+        "config": {"workload": f"large unstructured: {K} jittered quads ({args.n}x{args.n} over 100 km), N=3, "
+                               f"GRF bed, 1 m M2 tide, Manning 0.025, f=1.3e-4, SSP-RK3 CFL 0.5",
+                   "elements": K, "order": 3, "nodes": K * nn, "scheme": "ssp_rk3", "stages_per_step": STAGES,
+                   "library": "fma" if args.fma else "exact",
+                   "subcell_elements_at_end": int(es.sum()), "dry_nodes_at_end": int(nd.sum()),
+                   "model_time_s": t},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "stage_kernel<3>", "avg_launch_ms": stage_ms / max(launches, 1),
+                     "bytes_per_launch_avg": bytes_total / max(launches, 1),
+                     "measured_copy_GBps": copy_gbs, "frac_of_copy": achieved / copy_gbs,
+                     "dt_kernel_avg_ms": dt_ms / max(dt_launches, 1)},
+        "setup_s": gen_s,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

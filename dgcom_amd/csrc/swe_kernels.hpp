@@ -1,0 +1,703 @@
+// Device code of the 2D shallow-water stage: the fused SSP stage kernel, the
+// time-step kernel and the state utilities. Included by dgswe.hip (and by a
+// host-thread emulation of workgroups used to check the kernels on a CPU).
+#pragma once
+
+#include <cmath>
+#include <cstring>
+
+#include "../../include/dgswe.h"
+#include "swe_physics.hpp"
+
+namespace dgswe {
+
+// ---------------------------------------------------------------------------
+// GLL tables per order (set once from the host)
+// ---------------------------------------------------------------------------
+struct GllTables {
+    double x1[4], w1[4], d1[16], w2[16];
+    double lift[4][4];       // LIFT value at face node fi of face f (its node's entry)
+    double xl[4], xr[4];     // subcell a spans [xl[a], xr[a]]
+};
+__constant__ GllTables c_gll[4];
+
+constexpr uint32_t LINK_BOUNDARY = 0xFFFFFFFFu;
+
+template <int N> __device__ __forceinline__ int face_node(int f, int fi) {
+    constexpr int N1 = N + 1;
+    return f == 0 ? fi : f == 1 ? fi * N1 + N1 - 1 : f == 2 ? (N1 - 1) * N1 + (N1 - 1 - fi) : (N1 - 1 - fi) * N1;
+}
+
+// Bilinear map derivatives at (r, s) (operators/geometric.rs:438-460); c = x0,y0,..,x3,y3
+struct Metric { double x_r, x_s, y_r, y_s, det; };
+__device__ __forceinline__ Metric bilinear(const double* c, double r, double s) {
+    Metric m;
+    m.x_r = 0.25 * ((1.0 - s) * (c[2] - c[0]) + (1.0 + s) * (c[4] - c[6]));
+    m.y_r = 0.25 * ((1.0 - s) * (c[3] - c[1]) + (1.0 + s) * (c[5] - c[7]));
+    m.x_s = 0.25 * ((1.0 - r) * (c[6] - c[0]) + (1.0 + r) * (c[4] - c[2]));
+    m.y_s = 0.25 * ((1.0 - r) * (c[7] - c[1]) + (1.0 + r) * (c[5] - c[3]));
+    m.det = m.x_r * m.y_s - m.x_s * m.y_r;
+    return m;
+}
+
+// sJ n of face f: the outward contravariant vector of the bilinear map at its
+// face nodes (geometric.rs:188-205). On the face the map's derivative along it
+// is half the edge vector from vertex f to vertex f+1, exactly (the other
+// corner pair enters with a factor 0 and the rest are powers of two), so the
+// vector is that half edge turned clockwise, bit for bit the reference's at
+// every node of the face.
+template <int N> __device__ __forceinline__ void face_vector(const double* c, int f, double& vx, double& vy) {
+    double xa = f == 0 ? c[0] : f == 1 ? c[2] : f == 2 ? c[4] : c[6];
+    double ya = f == 0 ? c[1] : f == 1 ? c[3] : f == 2 ? c[5] : c[7];
+    double xb = f == 0 ? c[2] : f == 1 ? c[4] : f == 2 ? c[6] : c[0];
+    double yb = f == 0 ? c[3] : f == 1 ? c[5] : f == 2 ? c[7] : c[1];
+    vx = 0.5 * (yb - ya);
+    vy = -0.5 * (xb - xa);
+}
+
+struct StageParams {
+    const double *bh, *bhu, *bhv;
+    const double *ih, *ihu, *ihv;
+    double *oh, *ohu, *ohv;
+    const double* bed;
+    const double* corners;
+    const double* manning;
+    const uint32_t* links;
+    const uint8_t* tags;
+    const uint8_t* affine;
+    int64_t n_elements;
+    double a, b, c;
+    double eta_e;
+    double g, h_min, h_dry, u_max, tau, f0;
+    int rhs_only;
+    int flags;
+    unsigned long long* clips;
+    double* fstar_out;   // optional [K][4][N+1][3]: F* at every face node, outward (this element's side)
+};
+
+// One line's flux-differencing volume term with the bed slope, for the node at
+// position p (swe_2d_split_form.rs:843-894): the pair terms where an earlier
+// node of the line is `a` and this node `c`, then this node's own sum.
+template <int N>
+__device__ __forceinline__ void volume_line(Cons& rhs, int p, const int* idx, const double* sh, const double* shu,
+                                            const double* shv, const double* su, const double* sv, const double* sb,
+                                            const double* smx, const double* smy, bool affine, double g) {
+    constexpr int N1 = N + 1;
+    const GllTables& T = c_gll[N];
+    Node me = {sh[idx[p]], shu[idx[p]], shv[idx[p]], su[idx[p]], sv[idx[p]], sb[idx[p]]};
+    double mpx = smx[idx[p]], mpy = smy[idx[p]];
+#pragma unroll
+    for (int q = 0; q < N1; ++q) {
+        if (q >= p) break;
+        int iq = idx[q];
+        Node o = {sh[iq], shu[iq], shv[iq], su[iq], sv[iq], sb[iq]};
+        double mx = affine ? mpx : 0.5 * (smx[iq] + mpx);
+        double my = affine ? mpy : 0.5 * (smy[iq] + mpy);
+        Cons f = ec_flux(o, me, mx, my, g);
+        rhs = csub(rhs, cscale(2.0 * T.d1[p * N1 + q], f));
+    }
+    Cons acc = cscale(2.0 * T.d1[p * N1 + p], ec_flux(me, me, mpx, mpy, g));
+#pragma unroll
+    for (int q = 0; q < N1; ++q) {
+        if (q <= p) continue;
+        int iq = idx[q];
+        Node o = {sh[iq], shu[iq], shv[iq], su[iq], sv[iq], sb[iq]};
+        double mx = affine ? mpx : 0.5 * (mpx + smx[iq]);
+        double my = affine ? mpy : 0.5 * (mpy + smy[iq]);
+        Cons f = ec_flux(me, o, mx, my, g);
+        acc = cadd(acc, cscale(2.0 * T.d1[p * N1 + q], f));
+    }
+    double db = 0.0, dbx = 0.0, dby = 0.0;
+#pragma unroll
+    for (int q = 0; q < N1; ++q) {
+        double d_b = T.d1[p * N1 + q] * sb[idx[q]];
+        db += d_b;
+        if (!affine) {
+            dbx += d_b * smx[idx[q]];
+            dby += d_b * smy[idx[q]];
+        }
+    }
+    double bx, by;
+    if (affine) { bx = db * mpx; by = db * mpy; }
+    else { bx = 0.5 * (db * mpx + dbx); by = 0.5 * (db * mpy + dby); }
+    double force = g * me.h;
+    acc = cadd(acc, Cons{0.0, force * bx, force * by});
+    rhs = csub(rhs, acc);
+}
+
+// Subcell interface directions of a line (subcells.rs:63-90): im[0..N1]
+template <int N>
+__device__ __forceinline__ void interfaces(const int* idx, const double* smx, const double* smy, bool affine,
+                                           double* imx, double* imy) {
+    constexpr int N1 = N + 1;
+    const GllTables& T = c_gll[N];
+    if (affine) {
+#pragma unroll
+        for (int a = 0; a <= N1; ++a) { imx[a] = smx[idx[0]]; imy[a] = smy[idx[0]]; }
+        return;
+    }
+    imx[0] = smx[idx[0]];
+    imy[0] = smy[idx[0]];
+    double m0 = 0.0, m1 = 0.0;
+#pragma unroll
+    for (int a = 0; a < N1 - 1; ++a) {
+#pragma unroll
+        for (int c = 0; c < N1; ++c) {
+            if (c == a) continue;
+            double q2 = 2.0 * T.w1[a] * T.d1[a * N1 + c];
+            m0 += q2 * 0.5 * (smx[idx[a]] + smx[idx[c]]);
+            m1 += q2 * 0.5 * (smy[idx[a]] + smy[idx[c]]);
+        }
+        imx[a + 1] = m0;
+        imy[a + 1] = m1;
+    }
+    imx[N1] = smx[idx[N1 - 1]];
+    imy[N1] = smy[idx[N1 - 1]];
+}
+
+// The subcell update's contributions to the node at position p of a line
+// (swe_2d_split_form.rs:938-1035), from the reconstructed face states of the
+// line's subcells (rl, rr) and its interface fluxes (fa, fb) in LDS.
+template <int N>
+__device__ __forceinline__ void subcell_line(Cons& rhs, int p, const int* idx, const double* sh, const double* shu,
+                                             const double* shv, const double* su, const double* sv, const double* sb,
+                                             const double* smx, const double* smy, bool affine, double g,
+                                             const Side* rl, const Side* rr, const Cons* ifa, const Cons* ifb) {
+    constexpr int N1 = N + 1;
+    const GllTables& T = c_gll[N];
+    int ip = idx[p];
+    Node me = {sh[ip], shu[ip], shv[ip], su[ip], sv[ip], sb[ip]};
+    if (p == 0) rhs = cadd(rhs, cscale(1.0 / T.w1[0], ec_flux(me, me, smx[ip], smy[ip], g)));
+    if (p == N1 - 1) rhs = csub(rhs, cscale(1.0 / T.w1[N1 - 1], ec_flux(me, me, smx[ip], smy[ip], g)));
+    if (p > 0) rhs = cadd(rhs, cscale(1.0 / T.w1[p], ifb[idx[p - 1]]));
+    if (p < N1 - 1) rhs = csub(rhs, cscale(1.0 / T.w1[p], ifa[ip]));
+    double imx[N1 + 1], imy[N1 + 1];
+    interfaces<N>(idx, smx, smy, affine, imx, imy);
+    const Side& left = rl[ip];
+    const Side& right = rr[ip];
+    double bed = -0.5 * g * (left.h + right.h) * (right.b - left.b) / T.w1[p];
+    double fx, fy;
+    if (affine) { fx = bed * smx[idx[0]]; fy = bed * smy[idx[0]]; }
+    else {
+        double metric = 0.25 * g * ((right.h + me.h) * (me.b - right.b) + (left.h + me.h) * (me.b - left.b)) / T.w1[p];
+        fx = bed * 0.5 * (imx[p] + imx[p + 1]) + metric * (imx[p + 1] - imx[p]);
+        fy = bed * 0.5 * (imy[p] + imy[p + 1]) + metric * (imy[p + 1] - imy[p]);
+    }
+    rhs = cadd(rhs, Cons{0.0, fx, fy});
+}
+
+// Element height normal to face f at face node fi (J / sJ), affine: node-0 values
+template <int N>
+__device__ __forceinline__ double face_height(const double* c, bool affine, int f, int fi) {
+    const GllTables& T = c_gll[N];
+    constexpr int N1 = N + 1;
+    double vx, vy;
+    face_vector<N>(c, f, vx, vy);
+    double sj = sqrt(vx * vx + vy * vy);
+    int node = affine ? 0 : face_node<N>(f, fi);
+    Metric m = bilinear(c, T.x1[node % N1], T.x1[node / N1]);
+    return m.det / sj;
+}
+
+// The outer node of face node fi of face f (swe_2d_split_form.rs:1038-1085): the
+// neighbour's first interior node on the GLL line and its reference distance
+template <int N>
+__device__ __forceinline__ bool outer_node(const StageParams& p, int64_t k, const double* c, bool affine, int f, int fi,
+                                           Side& out, double& dist) {
+    constexpr int N1 = N + 1, NN = N1 * N1;
+    const GllTables& T = c_gll[N];
+    uint32_t l = p.links[4 * k + f];
+    if (l == LINK_BOUNDARY) return false;
+    int64_t nb = (int64_t)(l >> 4);
+    int nbf = (int)(l & 3u);
+    int nb_node = face_node<N>(nbf, N1 - 1 - fi);
+    int inner = nbf == 0 ? nb_node + N1 : nbf == 1 ? nb_node - 1 : nbf == 2 ? nb_node - N1 : nb_node + 1;
+    double cn[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) cn[q] = p.corners[8 * nb + q];
+    bool nb_affine = p.affine[nb] != 0;
+    dist = (T.x1[1] - T.x1[0]) * face_height<N>(cn, nb_affine, nbf, N1 - 1 - fi) / face_height<N>(c, affine, f, fi);
+    size_t n = (size_t)nb * NN + inner;
+    Node q = make_node(p.ih[n], p.ihu[n], p.ihv[n], p.bed[n], p.h_min);
+    out = side(q);
+    return true;
+}
+
+template <int N>
+__global__ __launch_bounds__(64) void stage_kernel(StageParams p) {
+    constexpr int N1 = N + 1, NN = N1 * N1, EPB = 64 / NN, NF = 4 * N1;
+    const GllTables& T = c_gll[N];
+    __shared__ double sh[64], shu[64], shv[64], su[64], sv[64], sb[64];
+    __shared__ double srx[64], sry[64], ssx[64], ssy[64];
+    __shared__ Cons sfd[EPB * NF];
+    __shared__ double ssj[EPB * 4];
+    __shared__ Side rl[2][64], rr[2][64];
+    __shared__ Cons ifa[2][64], ifb[2][64];
+    __shared__ double pw[64];
+    __shared__ int sflag[EPB], wflag[EPB];
+    __shared__ double savg[EPB * 4];
+
+    const int t = threadIdx.x;
+    const int e = t / NN;
+    const int i = t - e * NN;
+    const int64_t k = (int64_t)blockIdx.x * EPB + e;
+    const bool active = (e < EPB) && (k < p.n_elements);
+    const int a = i % N1, j = i / N1;
+    const size_t n = (size_t)k * NN + i;
+    const int eb = e * NN;  // first lane of this element
+
+    double c[8];
+    bool affine = false;
+    double hin = 0.0, huin = 0.0, hvin = 0.0;
+    double det = 1.0, det_inv = 1.0;
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) c[q] = p.corners[8 * k + q];
+        affine = p.affine[k] != 0;
+        hin = p.ih[n]; huin = p.ihu[n]; hvin = p.ihv[n];
+        double b = p.bed[n];
+        Node q = make_node(hin, huin, hvin, b, p.h_min);
+        sh[t] = q.h; shu[t] = q.hu; shv[t] = q.hv; su[t] = q.u; sv[t] = q.v; sb[t] = q.b;
+        Metric m = bilinear(c, T.x1[a], T.x1[j]);
+        det = m.det;
+        if (affine) {
+            Metric m0 = bilinear(c, T.x1[0], T.x1[0]);
+            srx[t] = m0.det * (m0.y_s / m0.det);
+            sry[t] = m0.det * (-m0.x_s / m0.det);
+            ssx[t] = m0.det * (-m0.y_r / m0.det);
+            ssy[t] = m0.det * (m0.x_r / m0.det);
+            det_inv = 1.0 / m0.det;
+        } else {
+            srx[t] = m.det * (m.y_s / m.det);
+            sry[t] = m.det * (-m.x_s / m.det);
+            ssx[t] = m.det * (-m.y_r / m.det);
+            ssy[t] = m.det * (m.x_r / m.det);
+            det_inv = 1.0 / m.det;
+        }
+    }
+    if (t < EPB) { sflag[t] = 0; wflag[t] = 0; }
+    __syncthreads();
+    const bool subcells_on = (p.flags & DG_SUBCELLS) != 0;
+    if (active && subcells_on && hin < p.h_dry) sflag[e] = 1;
+    __syncthreads();
+    const bool sub = active && sflag[e] != 0;
+
+    // Line node indices (lane numbers): r-line j and s-line a
+    int ridx[N1], sidx[N1];
+#pragma unroll
+    for (int q = 0; q < N1; ++q) { ridx[q] = eb + j * N1 + q; sidx[q] = eb + q * N1 + a; }
+
+    // Subcells: reconstructed face states of this node's subcell on both lines
+    if (sub) {
+#pragma unroll
+        for (int dir = 0; dir < 2; ++dir) {
+            const int* idx = dir == 0 ? ridx : sidx;
+            int pos = dir == 0 ? a : j;
+            Side me = {sh[t], su[t], sv[t], sb[t]};
+            Side lft, rgt;
+            double xlf = 0.0, xrt = 0.0;
+            bool has_l = true, has_r = true;
+            if (pos > 0) { int q = idx[pos - 1]; lft = {sh[q], su[q], sv[q], sb[q]}; xlf = T.x1[pos - 1]; }
+            else {
+                // r-line start: face 3 at fi = N1-1-j; s-line start: face 0 at fi = a
+                double d;
+                has_l = dir == 0 ? outer_node<N>(p, k, c, affine, 3, N1 - 1 - j, lft, d)
+                                 : outer_node<N>(p, k, c, affine, 0, a, lft, d);
+                xlf = -1.0 - d;
+            }
+            if (pos < N1 - 1) { int q = idx[pos + 1]; rgt = {sh[q], su[q], sv[q], sb[q]}; xrt = T.x1[pos + 1]; }
+            else {
+                double d;
+                has_r = dir == 0 ? outer_node<N>(p, k, c, affine, 1, j, rgt, d)
+                                 : outer_node<N>(p, k, c, affine, 2, N1 - 1 - a, rgt, d);
+                xrt = 1.0 + d;
+            }
+            if (has_l && has_r) reconstruct(lft, me, rgt, xlf, T.x1[pos], xrt, T.xl[pos], T.xr[pos], p.h_dry, rl[dir][t], rr[dir][t]);
+            else { rl[dir][t] = me; rr[dir][t] = me; }
+        }
+    }
+    __syncthreads();
+    if (sub) {
+#pragma unroll
+        for (int dir = 0; dir < 2; ++dir) {
+            const int* idx = dir == 0 ? ridx : sidx;
+            int pos = dir == 0 ? a : j;
+            if (pos < N1 - 1) {
+                double imx[N1 + 1], imy[N1 + 1];
+                interfaces<N>(idx, dir == 0 ? srx : ssx, dir == 0 ? sry : ssy, affine, imx, imy);
+                hydro_hll(rr[dir][t], rl[dir][idx[pos + 1]], imx[pos + 1], imy[pos + 1], p.g, p.h_min, ifa[dir][t], ifb[dir][t]);
+            }
+        }
+    }
+
+    // Faces: the 4(N+1) face nodes of the element spread over its lanes
+    if (active) {
+        for (int task = i; task < NF; task += NN) {
+            int f = task / N1, fi = task - f * N1;
+            int nd = face_node<N>(f, fi);
+            int li = eb + nd;
+            Node q = {sh[li], shu[li], shv[li], su[li], sv[li], sb[li]};
+            double vx, vy;
+            face_vector<N>(c, f, vx, vy);
+            double sj = sqrt(vx * vx + vy * vy);
+            double nx = vx / sj, ny = vy / sj;
+            uint32_t l = p.links[4 * k + f];
+            Cons fstar;
+            if (l == LINK_BOUNDARY) {
+                if (p.tags[4 * k + f] == DG_BC_TIDE) {
+                    Cons qb = characteristic(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min);
+                    Node nb = make_node(qb.h, qb.hu, qb.hv, q.b, p.h_min);
+                    fstar = ec_flux(nb, nb, nx, ny, p.g);
+                } else {
+                    Cons gs = wall_ghost(q.h, q.hu, q.hv, nx, ny, p.h_min);
+                    Node qe = make_node(gs.h, gs.hu, gs.hv, q.b, p.h_min);
+                    Cons fb;
+                    hydro_hll(side(q), side(qe), nx, ny, p.g, p.h_min, fstar, fb);
+                }
+            } else {
+                int64_t nbk = (int64_t)(l >> 4);
+                int nbf = (int)(l & 3u);
+                size_t m = (size_t)nbk * NN + face_node<N>(nbf, N1 - 1 - fi);
+                Node qn = make_node(p.ih[m], p.ihu[m], p.ihv[m], p.bed[m], p.h_min);
+                Cons fa, fb;
+                if (l & 4u) {
+                    hydro_hll(side(q), side(qn), nx, ny, p.g, p.h_min, fa, fb);
+                    fstar = fa;
+                } else {
+                    double lx, ly;
+                    if (l & 8u) { lx = -nx; ly = -ny; }
+                    else {
+                        double cn[8];
+#pragma unroll
+                        for (int r = 0; r < 8; ++r) cn[r] = p.corners[8 * nbk + r];
+                        double wx, wy;
+                        face_vector<N>(cn, nbf, wx, wy);
+                        double s2 = sqrt(wx * wx + wy * wy);
+                        lx = wx / s2; ly = wy / s2;
+                    }
+                    hydro_hll(side(qn), side(q), lx, ly, p.g, p.h_min, fa, fb);
+                    fstar = cneg(fb);
+                }
+            }
+            if (p.fstar_out) {
+                double* o = p.fstar_out + ((size_t)k * NF + task) * 3;
+                o[0] = fstar.h; o[1] = fstar.hu; o[2] = fstar.hv;
+            }
+            sfd[e * NF + task] = csub(ec_flux(q, q, nx, ny, p.g), fstar);
+            if (fi == 0) ssj[e * 4 + f] = sj;
+        }
+    }
+    __syncthreads();
+
+    Cons rhs = {0.0, 0.0, 0.0};
+    if (active) {
+        const bool r_first = j <= a;
+#pragma unroll
+        for (int pass = 0; pass < 2; ++pass) {
+            bool rline = (pass == 0) == r_first;
+            if (sub) {
+                if (rline) subcell_line<N>(rhs, a, ridx, sh, shu, shv, su, sv, sb, srx, sry, affine, p.g, rl[0], rr[0], ifa[0], ifb[0]);
+                else subcell_line<N>(rhs, j, sidx, sh, shu, shv, su, sv, sb, ssx, ssy, affine, p.g, rl[1], rr[1], ifa[1], ifb[1]);
+            } else {
+                if (rline) volume_line<N>(rhs, a, ridx, sh, shu, shv, su, sv, sb, srx, sry, affine, p.g);
+                else volume_line<N>(rhs, j, sidx, sh, shu, shv, su, sv, sb, ssx, ssy, affine, p.g);
+            }
+        }
+        // Surface terms in face order: LIFT sJ (F(q).n - F*)
+        if (j == 0) rhs = cadd(rhs, cscale(ssj[e * 4 + 0] * T.lift[0][a], sfd[e * NF + 0 * N1 + a]));
+        if (a == N1 - 1) rhs = cadd(rhs, cscale(ssj[e * 4 + 1] * T.lift[1][j], sfd[e * NF + 1 * N1 + j]));
+        if (j == N1 - 1) rhs = cadd(rhs, cscale(ssj[e * 4 + 2] * T.lift[2][N1 - 1 - a], sfd[e * NF + 2 * N1 + (N1 - 1 - a)]));
+        if (a == 0) rhs = cadd(rhs, cscale(ssj[e * 4 + 3] * T.lift[3][N1 - 1 - j], sfd[e * NF + 3 * N1 + (N1 - 1 - j)]));
+        rhs = {det_inv * rhs.h, det_inv * rhs.hu, det_inv * rhs.hv};
+        if (p.f0 != 0.0) {
+            rhs.hu += p.f0 * hvin;
+            rhs.hv += (-p.f0) * huin;
+        }
+    }
+    if (p.rhs_only) {
+        if (active) { p.oh[n] = rhs.h; p.ohu[n] = rhs.hu; p.ohv[n] = rhs.hv; }
+        return;
+    }
+
+    // Shu-Osher combination and point-implicit damping (builder.rs:668-690; wetting_drying.rs:324-363)
+    double vh = 0.0, vhu = 0.0, vhv = 0.0;
+    const bool friction = (p.flags & DG_FRICTION) != 0, wet_dry = (p.flags & DG_WET_DRY) != 0;
+    if (wet_dry) __syncthreads();  // the node states in LDS are reused below
+    if (active) {
+        if (p.a != 0.0) { vh = p.a * p.bh[n]; vhu = p.a * p.bhu[n]; vhv = p.a * p.bhv[n]; }
+        if (p.b != 0.0) { vh += p.b * hin; vhu += p.b * huin; vhv += p.b * hvin; }
+        if (p.c != 0.0) { vh += p.c * rhs.h; vhu += p.c * rhs.hu; vhv += p.c * rhs.hv; }
+        if (friction || wet_dry) {
+            if (vh <= 0.0) { vhu = 0.0; vhv = 0.0; }
+            else {
+                double rate = 0.0;
+                if (friction) {
+                    double u, v;
+                    if (wet_dry) desing_velocity(hin, huin, hvin, p.h_dry, u, v);
+                    else { u = 0.0; v = 0.0; if (hin > p.h_min) { double inv = 1.0 / hin; u = huin * inv; v = hvin * inv; } }
+                    double speed = sqrt(u * u + v * v);
+                    if (speed > 0.0) {
+                        double mn = p.manning[k];
+                        double gn2 = p.g * mn * mn;
+                        double hh = fmax_(vh, p.h_min);
+                        rate += gn2 * speed / (manning_cbrt(hh) * hh);
+                    }
+                }
+                if (wet_dry && vh < p.h_dry) {
+                    double ex = p.h_dry / vh - 1.0;
+                    rate += ex * ex / p.tau;
+                }
+                double factor = 1.0 / (1.0 + p.c * rate);
+                vhu *= factor;
+                vhv *= factor;
+            }
+        }
+        if (wet_dry) {
+            pw[t] = T.w2[i] * det;
+            sh[t] = vh; shu[t] = vhu; shv[t] = vhv;
+            if (vh < p.h_dry) wflag[e] = 1;
+        }
+    }
+    // Wet/dry correction (wetting_drying.rs:194-213; limiters/swe_2d.rs:104-197)
+    if (wet_dry) {
+        __syncthreads();
+        const bool needs = active && wflag[e] != 0;
+        if (needs && i == 0) {
+            double area = 0.0;
+            for (int q = 0; q < NN; ++q) area += pw[eb + q];
+            double inv_total = 1.0 / area;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, hmin = INFINITY;
+            for (int q = 0; q < NN; ++q) {
+                double w = pw[eb + q];
+                s0 += w * sh[eb + q];
+                s1 += w * shu[eb + q];
+                s2 += w * shv[eb + q];
+            }
+            for (int q = 0; q < NN; ++q) hmin = fmin_(hmin, sh[eb + q]);
+            savg[e * 4 + 0] = s0 * inv_total;
+            savg[e * 4 + 1] = s1 * inv_total;
+            savg[e * 4 + 2] = s2 * inv_total;
+            savg[e * 4 + 3] = hmin;
+        }
+        __syncthreads();
+        if (needs) {
+            double ah = savg[e * 4 + 0], ahu = savg[e * 4 + 1], ahv = savg[e * 4 + 2], hmin = savg[e * 4 + 3];
+            if (ah <= 0.0) {
+                vh = 0.0; vhu = 0.0; vhv = 0.0;
+                if (i == 0 && ah < 0.0) atomicAdd(p.clips, 1ull);
+            } else {
+                if (hmin < 0.0) {
+                    double theta = fmax_(0.0, fmin_(1.0, ah / (ah - hmin)));
+                    vh = theta * (vh - ah) + ah;
+                    vhu = theta * (vhu - ahu) + ahu;
+                    vhv = theta * (vhv - ahv) + ahv;
+                    vh = fmax_(vh, 0.0);
+                }
+                if (ah < p.h_dry) { vhu = 0.0; vhv = 0.0; }
+            }
+        }
+        if (active) correct_node(vh, vhu, vhv, p.h_dry, p.u_max);
+    }
+    if (active) { p.oh[n] = vh; p.ohu[n] = vhu; p.ohv[n] = vhv; }
+}
+
+// ---------------------------------------------------------------------------
+// Time step: per element bound and the global minimum (swe_2d.rs:2072-2258)
+// ---------------------------------------------------------------------------
+struct DtParams {
+    const double *h, *hu, *hv;
+    const double* corners;
+    const uint32_t* links;
+    const uint8_t* affine;
+    int64_t n_elements;
+    double cfl, g, h_min, h_dry;
+    double pcfl, max_cfl, max_cfl_sub;
+    int positivity;
+    double* element_dt;                 // optional
+    unsigned long long* dt_min_bits;    // global minimum as ordered bits (dt >= 0)
+};
+
+__device__ __forceinline__ double node_rate(double h, double hu, double hv, double rx, double ry, double sx, double sy,
+                                            double g, double h_min) {
+    double u = 0.0, v = 0.0;
+    if (h > h_min) { double inv = 1.0 / h; u = hu * inv; v = hv * inv; }
+    double nr = sqrt(rx * rx + ry * ry), ns = sqrt(sx * sx + sy * sy);
+    double cc = sqrt(g * fmax_(h, 0.0));
+    double lr = fabs(u * rx + v * ry) + cc * nr;
+    double ls = fabs(u * sx + v * sy) + cc * ns;
+    return 0.25 * (lr + ls);
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void dt_kernel(DtParams p) {
+    constexpr int N1 = N + 1, NN = N1 * N1;
+    const GllTables& T = c_gll[N];
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double dt = INFINITY;
+    if (k < p.n_elements) {
+        double c[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) c[q] = p.corners[8 * k + q];
+        bool affine = p.affine[k] != 0;
+        Metric m0 = bilinear(c, T.x1[0], T.x1[0]);
+        double rx0 = m0.y_s / m0.det, ry0 = -m0.x_s / m0.det, sx0 = -m0.y_r / m0.det, sy0 = m0.x_r / m0.det;
+        double rate = 0.0;
+        double water = 0.0, water_b = 0.0, weight = 0.0, weight_b = 0.0;
+        bool shallow = false;
+        const double shallow_depth = 100.0 * p.h_dry;
+#pragma unroll
+        for (int i = 0; i < NN; ++i) {
+            size_t n = (size_t)k * NN + i;
+            double h = p.h[n];
+            int a = i % N1, j = i / N1;
+            Metric m = bilinear(c, T.x1[a], T.x1[j]);
+            double r = 0.0;
+            if (h > p.h_min) {
+                if (affine) r = node_rate(h, p.hu[n], p.hv[n], rx0, ry0, sx0, sy0, p.g, p.h_min);
+                else r = node_rate(h, p.hu[n], p.hv[n], m.y_s / m.det, -m.x_s / m.det, -m.y_r / m.det, m.x_r / m.det, p.g, p.h_min);
+            }
+            rate = fmax_(rate, r);
+            double mass = T.w2[i] * m.det;
+            shallow = shallow || (h <= shallow_depth);
+            double wi = mass * fmax_(h, 0.0);
+            water += wi;
+            weight += mass;
+            if (a == 0 || a == N1 - 1 || j == 0 || j == N1 - 1) { water_b += wi; weight_b += mass; }
+        }
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            uint32_t l = p.links[4 * k + f];
+            if (l == LINK_BOUNDARY) continue;
+            int64_t nb = (int64_t)(l >> 4);
+            int nbf = (int)(l & 3u);
+#pragma unroll
+            for (int fi = 0; fi < N1; ++fi) {
+                size_t m = (size_t)nb * NN + face_node<N>(nbf, N1 - 1 - fi);
+                double h = p.h[m];
+                if (h <= p.h_min) continue;
+                int i = face_node<N>(f, fi);
+                Metric mm = bilinear(c, T.x1[i % N1], T.x1[i / N1]);
+                rate = fmax_(rate, node_rate(h, p.hu[m], p.hv[m], mm.y_s / mm.det, -mm.x_s / mm.det, -mm.y_r / mm.det,
+                                             mm.x_r / mm.det, p.g, p.h_min));
+            }
+        }
+        double cfl = p.cfl;
+        if (p.positivity) {
+            double ratio = fmin_(water / water_b, weight / weight_b);
+            double rho = fmax_(0.9 * ratio, 1.0);
+            double cap = shallow ? p.max_cfl_sub : p.max_cfl;
+            cfl = fmin_(cfl, fmin_(p.pcfl * rho, cap));
+        }
+        dt = rate < 1e-14 ? INFINITY : cfl / ((2.0 * N + 1.0) * rate);
+        if (p.element_dt) p.element_dt[k] = dt;
+    }
+    // block minimum, then one atomic per block
+    __shared__ double red[256];
+    red[threadIdx.x] = dt;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmin_(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMin(p.dt_min_bits, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+// ---------------------------------------------------------------------------
+// Utilities
+// ---------------------------------------------------------------------------
+__global__ void fill_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+
+__global__ void finite_kernel(const double* x, size_t n, int* bad) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    int local = 0;
+    for (; i < n; i += stride) local |= !isfinite(x[i]);
+    if (local) atomicOr(bad, 1);
+}
+
+template <int N>
+__global__ void masks_kernel(const double* h, int64_t K, double h_dry, int subcells, uint8_t* node_dry, uint8_t* elem_sub) {
+    constexpr int NN = (N + 1) * (N + 1);
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    uint8_t any = 0;
+    for (int i = 0; i < NN; ++i) {
+        uint8_t d = h[(size_t)k * NN + i] < h_dry ? 1 : 0;
+        node_dry[(size_t)k * NN + i] = d;
+        any |= d;
+    }
+    elem_sub[k] = subcells ? any : 0;
+}
+
+__global__ void copy_kernel(const double4* __restrict__ src, double4* __restrict__ dst, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+
+// Host: the tables of order n
+inline void legendre(int n, double x, double& p_n, double& dp_n) {
+    // three-term recurrence and P'_N from P_N, P_{N-1} (polynomial/legendre.rs:62-93)
+    if (n == 1) { p_n = x; dp_n = 1.0; return; }
+    double pp = 1.0, pc = x;
+    for (int k = 1; k < n; ++k) {
+        double nx = ((double)(2 * k + 1) * x * pc - (double)k * pp) / (double)(k + 1);
+        pp = pc;
+        pc = nx;
+    }
+    p_n = pc;
+    if (std::fabs(x - 1.0) < 1e-14) dp_n = (double)(n * (n + 1)) / 2.0;
+    else if (std::fabs(x + 1.0) < 1e-14) dp_n = ((n % 2 == 0) ? -1.0 : 1.0) * (double)(n * (n + 1)) / 2.0;
+    else dp_n = (double)n * (x * pc - pp) / (x * x - 1.0);
+}
+
+inline GllTables make_tables(int n) {
+    // GLL points by Newton's method on (1 - x^2) P_N' from the Chebyshev-Lobatto
+    // points (polynomial/nodes.rs:17-107), weights 2 / (N(N+1) P_N^2), and the
+    // Lagrange differentiation matrix on them.
+    GllTables t;
+    std::memset(&t, 0, sizeof(t));
+    const int n1 = n + 1;
+    for (int j = 0; j <= n; ++j) t.x1[j] = -std::cos(M_PI * (double)j / (double)n);
+    t.x1[0] = -1.0;
+    t.x1[n] = 1.0;
+    for (int j = 1; j < n; ++j) {
+        double x = t.x1[j];
+        for (int it = 0; it < 100; ++it) {
+            double p, dp;
+            legendre(n, x, p, dp);
+            double upd = (1.0 - x * x) * dp / ((double)n * (double)(n + 1) * p);
+            if (std::fabs(upd) < 1e-15) break;
+            x += upd;
+        }
+        t.x1[j] = x;
+    }
+    double pn[4];
+    for (int j = 0; j < n1; ++j) {
+        double dp;
+        legendre(n, t.x1[j], pn[j], dp);
+        t.w1[j] = 2.0 / ((double)(n * (n + 1)) * pn[j] * pn[j]);
+    }
+    for (int r = 0; r < n1; ++r)
+        for (int c = 0; c < n1; ++c) t.d1[r * n1 + c] = r == c ? 0.0 : pn[r] / (pn[c] * (t.x1[r] - t.x1[c]));
+    t.d1[0] = -(double)(n * (n + 1)) / 4.0;
+    t.d1[n1 * n1 - 1] = (double)(n * (n + 1)) / 4.0;
+    for (int j = 0; j < n1; ++j)
+        for (int a = 0; a < n1; ++a) t.w2[j * n1 + a] = t.w1[a] * t.w1[j];
+    auto fnode = [&](int f, int fi) {
+        return f == 0 ? fi : f == 1 ? fi * n1 + n1 - 1 : f == 2 ? (n1 - 1) * n1 + (n1 - 1 - fi) : (n1 - 1 - fi) * n1;
+    };
+    for (int f = 0; f < 4; ++f)
+        for (int fi = 0; fi < n1; ++fi) t.lift[f][fi] = (1.0 / t.w2[fnode(f, fi)]) * (1.0 * t.w1[fi]);
+    double xl = -1.0;
+    for (int a = 0; a < n1; ++a) {
+        double xr = xl + t.w1[a];
+        t.xl[a] = xl;
+        t.xr[a] = xr;
+        xl = xr;
+    }
+    return t;
+}
+
+}  // namespace dgswe

@@ -1,0 +1,252 @@
+// Pointwise physics of the 2D shallow-water `WetDry` split form, for the HIP
+// kernels (host + device). Each function names the reference routine it
+// computes; the operation order follows the reference so that the kernels
+// can be compared with a CPU restatement to the last bit when both are built
+// without floating-point contraction.
+#pragma once
+
+#ifndef DGSWE_EMULATE
+#include <hip/hip_runtime.h>
+#endif
+#include <math.h>
+#include <stdint.h>
+
+#define DG_HD __host__ __device__ __forceinline__
+
+namespace dgswe {
+
+struct Cons { double h, hu, hv; };            // SWEState2D
+struct Node { double h, hu, hv, u, v, b; };   // SWENodeState2D
+struct Side { double h, u, v, b; };           // SWESide2D
+
+DG_HD Cons cadd(Cons a, Cons b) { return {a.h + b.h, a.hu + b.hu, a.hv + b.hv}; }
+DG_HD Cons csub(Cons a, Cons b) { return {a.h - b.h, a.hu - b.hu, a.hv - b.hv}; }
+DG_HD Cons cneg(Cons a) { return {-a.h, -a.hu, -a.hv}; }
+DG_HD Cons cscale(double c, Cons a) { return {c * a.h, c * a.hu, c * a.hv}; }
+
+// Rust's f64::max / f64::min on non-NaN operands
+DG_HD double fmax_(double a, double b) { return fmax(a, b); }
+DG_HD double fmin_(double a, double b) { return fmin(a, b); }
+
+// SWENodeState2D::new / node_velocity (flux/swe_2d_entropy.rs:97-124): one reciprocal,
+// zero velocity at or below h_min
+DG_HD Node make_node(double h, double hu, double hv, double b, double h_min) {
+    double inv_h = 1.0 / h;
+    bool wet = h > h_min;
+    Node q;
+    q.h = h; q.hu = hu; q.hv = hv; q.b = b;
+    q.u = wet ? hu * inv_h : 0.0;
+    q.v = wet ? hv * inv_h : 0.0;
+    return q;
+}
+DG_HD Side side(const Node& q) { return {q.h, q.u, q.v, q.b}; }
+
+// wintermeyer_flux_2d (flux/swe_2d_entropy.rs:140-158): F#(l, r)·m, bitwise symmetric
+DG_HD Cons ec_flux(const Node& l, const Node& r, double mx, double my, double g) {
+    double mass = 0.5 * ((l.hu + r.hu) * mx + (l.hv + r.hv) * my);
+    double pressure = (0.5 * g) * (l.h * r.h);
+    double ua = 0.5 * (l.u + r.u);
+    double va = 0.5 * (l.v + r.v);
+    return {mass, mass * ua + pressure * mx, mass * va + pressure * my};
+}
+
+// hll_flux_face_aligned + einfeldt_speeds_2d (flux/swe_2d.rs:277-374), with the
+// sides in face-aligned components (depth, normal / tangential momentum and velocity)
+struct Hll { double h, hun, hut, un, ut; };
+
+DG_HD Cons hll_aligned(const Hll& l, const Hll& r, double g, double h_min) {
+    const double hl = l.h, hr = r.h;
+    bool dry = (hl <= h_min) && (hr <= h_min);
+    double sql = sqrt(fmax_(hl, 0.0)), sqr = sqrt(fmax_(hr, 0.0));
+    double sqg = sqrt(g);
+    double cl = sqg * sql, cr = sqg * sqr;
+    bool roe = (sql + sqr) > 1e-10;
+    double h_roe = 0.5 * (hl + hr);
+    double un_roe = roe ? (sql * l.un + sqr * r.un) / (sql + sqr) : 0.0;
+    double c_roe = roe ? sqrt(g * h_roe) : 0.0;
+    double s_l = (hl > h_min) ? fmin_(l.un - cl, un_roe - c_roe) : r.un - 2.0 * cr;
+    double s_r = (hr > h_min) ? fmax_(r.un + cr, un_roe + c_roe) : l.un + 2.0 * cl;
+    double hg = 0.5 * g;
+    Cons fl = {l.h * l.un, l.h * l.un * l.un + hg * l.h * l.h, l.h * l.un * l.ut};
+    Cons fr = {r.h * r.un, r.h * r.un * r.un + hg * r.h * r.h, r.h * r.un * r.ut};
+    double inv_ds = 1.0 / (s_r - s_l);
+    double slr = s_l * s_r;
+    Cons st = {inv_ds * (s_r * fl.h - s_l * fr.h + slr * (hr - hl)),
+               inv_ds * (s_r * fl.hu - s_l * fr.hu + slr * (r.hun - l.hun)),
+               inv_ds * (s_r * fl.hv - s_l * fr.hv + slr * (r.hut - l.hut))};
+    bool from_l = s_l >= 0.0, from_r = s_r <= 0.0;
+    Cons o;
+    o.h = dry ? 0.0 : (from_l ? fl.h : (from_r ? fr.h : st.h));
+    o.hu = dry ? 0.0 : (from_l ? fl.hu : (from_r ? fr.hu : st.hu));
+    o.hv = dry ? 0.0 : (from_l ? fl.hv : (from_r ? fr.hv : st.hv));
+    return o;
+}
+
+// wall_ram_pressure (swe_2d_split_form.rs:147-156)
+DG_HD double ram_pressure(double h, double un, double sill, double g) {
+    un = fmax_(un, 0.0);
+    double head = un * un * (0.5 / g);
+    double ram = (1.0 - head / sill) * h * un * (2.0 * un + sqrt(g * h));
+    return (sill > head) ? ram : 0.0;
+}
+
+// hydrostatic_hll (swe_2d_split_form.rs:336-402): Audusse states, HLL along m,
+// the hydrostatic correction per side; fa as seen from a, fb from b (both along +m)
+DG_HD void hydro_hll(const Side& qa, const Side& qb, double mx, double my, double g, double h_min, Cons& fa, Cons& fb) {
+    double norm = sqrt(mx * mx + my * my);
+    double inv_norm = 1.0 / norm;
+    double nx = mx * inv_norm, ny = my * inv_norm;
+    double b_face = fmax_(qa.b, qb.b);
+    Hll s[2];
+    const Side* q[2] = {&qa, &qb};
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        double h_star = fmax_(q[i]->h + q[i]->b - b_face, 0.0);
+        bool wet = q[i]->h > h_min;
+        double u = wet ? q[i]->u : 0.0, v = wet ? q[i]->v : 0.0;
+        double un = u * nx + v * ny, ut = -u * ny + v * nx;
+        bool star_wet = h_star > h_min;
+        s[i] = {h_star, h_star * un, h_star * ut, star_wet ? un : 0.0, star_wet ? ut : 0.0};
+    }
+    Cons rot = hll_aligned(s[0], s[1], g, h_min);
+    double px = rot.hu * nx - rot.hv * ny, py = rot.hu * ny + rot.hv * nx;
+    double fh = norm * rot.h, fhu = norm * px, fhv = norm * py;
+    bool hll_dry = (s[0].h <= h_min) && (s[1].h <= h_min);
+    double hsa = hll_dry ? 0.0 : s[0].h, hsb = hll_dry ? 0.0 : s[1].h;
+    double wall_a = 0.0, wall_b = 0.0;
+    if (hll_dry) {
+        double una = 1.0 * (qa.u * nx + qa.v * ny);
+        double ra = ram_pressure(qa.h, una, b_face - (qa.h + qa.b), g);
+        wall_a = (qa.h > h_min) ? ra : 0.0;
+        double unb = -1.0 * (qb.u * nx + qb.v * ny);
+        double rb = ram_pressure(qb.h, unb, b_face - (qb.h + qb.b), g);
+        wall_b = (qb.h > h_min) ? rb : 0.0;
+    }
+    double hg = 0.5 * g;
+    double dpa = hg * (qa.h * qa.h - hsa * hsa) + wall_a;
+    double dpb = hg * (qb.h * qb.h - hsb * hsb) + wall_b;
+    fa = {fh + 0.0, fhu + dpa * mx, fhv + dpa * my};
+    fb = {fh + 0.0, fhu + dpb * mx, fhv + dpb * my};
+}
+
+// limited_slope with SlopeDistances (swe_2d_split_form.rs:168-203)
+struct Inv3 { double central, minus, plus; };
+DG_HD Inv3 slope_inv(double xm, double xc, double xp, double x_l, double x_r) {
+    double fm = xc - x_l, fp = x_r - xc;
+    Inv3 r;
+    r.central = 1.0 / (xp - xm);
+    r.minus = 1.0 / ((fm > 0.0) ? fm : 0.5 * (xc - xm));
+    r.plus = 1.0 / ((fp > 0.0) ? fp : 0.5 * (xp - xc));
+    return r;
+}
+DG_HD double lim_slope(double q0, double q1, double q2, const Inv3& inv) {
+    double dm = q1 - q0, dp = q2 - q1;
+    bool extremum = (dm * dp) <= 0.0;
+    double central = (q2 - q0) * inv.central;
+    double bound = fmin_(fabs(dm * inv.minus), fabs(dp * inv.plus));
+    double sign = (central < 0.0) ? -1.0 : 1.0;
+    return extremum ? 0.0 : sign * fmin_(fabs(central), bound);
+}
+
+// reconstruct_subcell (swe_2d_split_form.rs:279-317): face states at x_l and x_r
+DG_HD void reconstruct(const Side& m, const Side& c, const Side& p, double xm, double xc, double xp, double x_l,
+                       double x_r, double h_dry, Side& left, Side& right) {
+    double eta0 = m.h + m.b, eta1 = c.h + c.b, eta2 = p.h + p.b;
+    double ws = -INFINITY;
+    if ((m.h >= h_dry) && (eta0 > ws)) ws = eta0;
+    if ((c.h >= h_dry) && (eta1 > ws)) ws = eta1;
+    if ((p.h >= h_dry) && (eta2 > ws)) ws = eta2;
+    bool flooding = ((m.h < h_dry) && (eta0 < ws)) || ((c.h < h_dry) && (eta1 < ws)) || ((p.h < h_dry) && (eta2 < ws));
+    Inv3 inv = slope_inv(xm, xc, xp, x_l, x_r);
+    double s_h = lim_slope(m.h, c.h, p.h, inv);
+    double s_eta = flooding ? 0.0 : lim_slope(eta0, eta1, eta2, inv);
+    double s_u = lim_slope(m.u, c.u, p.u, inv);
+    double s_v = lim_slope(m.v, c.v, p.v, inv);
+    double d = x_l - xc;
+    double hh = fmax_(c.h + s_h * d, 0.0);
+    left = {hh, c.u + s_u * d, c.v + s_v * d, eta1 + s_eta * d - hh};
+    d = x_r - xc;
+    hh = fmax_(c.h + s_h * d, 0.0);
+    right = {hh, c.u + s_u * d, c.v + s_v * d, eta1 + s_eta * d - hh};
+}
+
+// characteristic_state (boundary/characteristic.rs:481-520) for elevation-only
+// data at rest, whose tangential velocity is the interior's (characteristic.rs:411-438)
+DG_HD Cons characteristic(double h, double hu, double hv, double bed, double eta_e, double nx, double ny, double g,
+                          double h_min) {
+    double h_e = fmax_(eta_e - bed, 0.0);
+    double ui = 0.0, vi = 0.0;
+    if (h > h_min) { double inv = 1.0 / h; ui = hu * inv; vi = hv * inv; }
+    double ut_e = -ui * ny + vi * nx, un_e = 0.0;
+    double h_i = fmax_(h, 0.0);
+    double un_i = 0.0, ut_i = 0.0;
+    if (h_i > h_min) {
+        double u = hu / h_i, v = hv / h_i;
+        un_i = u * nx + v * ny;
+        ut_i = -u * ny + v * nx;
+    }
+    double c_i = sqrt(g * h_i), c_e = sqrt(g * h_e);
+    double w_out = un_i + 2.0 * c_i, w_in = un_e - 2.0 * c_e;
+    bool wi = h_i > h_min, we = h_e > h_min;
+    double h_b, un_b;
+    if (!wi && !we) { h_b = 0.0; un_b = 0.0; }
+    else if (wi && un_i >= c_i) { h_b = h_i; un_b = un_i; }
+    else if (we && un_e <= -c_e) { h_b = h_e; un_b = un_e; }
+    else if (wi && !we) { double c = fmax_(w_out / 3.0, 0.0); h_b = c * c / g; un_b = c; }
+    else if (!wi && we) { double c = fmax_(-w_in / 3.0, 0.0); h_b = c * c / g; un_b = -c; }
+    else { double c = fmax_(0.25 * (w_out - w_in), 0.0); h_b = c * c / g; un_b = 0.5 * (w_out + w_in); }
+    double ut_b = (un_b > 0.0) ? ut_i : ut_e;
+    double ub = un_b * nx - ut_b * ny, vb = un_b * ny + ut_b * nx;
+    return {h_b, h_b * ub, h_b * vb};
+}
+
+// Reflective2D ghost state (boundary/boundary_2d.rs:271-285)
+DG_HD Cons wall_ghost(double h, double hu, double hv, double nx, double ny, double h_min) {
+    if (h <= h_min) return {h, 0.0, 0.0};
+    double inv = 1.0 / h;
+    double u = hu * inv, v = hv * inv;
+    double un = u * nx + v * ny;
+    double ug = u - 2.0 * un * nx, vg = v - 2.0 * un * ny;
+    return {h, h * ug, h * vg};
+}
+
+// Cube root of the Manning rate (source/swe_2d/friction.rs:39-64): Halley from a
+// bit-pattern guess; only called on normal positive inputs (depths floored at h_min)
+DG_HD double manning_cbrt(double x) {
+    uint64_t bits = __double_as_longlong(x);
+    double y = __longlong_as_double((long long)(bits / 3 + 0x2A9F7893782DA1CEull));
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        double y3 = y * y * y;
+        y += y * ((x - y3) / (2.0 * y3 + x));
+    }
+    return y;
+}
+
+// Kurganov-Petrova desingularised velocity (wetting_drying.rs:115-128)
+DG_HD void desing_velocity(double h, double hu, double hv, double eps, double& u, double& v) {
+    if (h <= 0.0) { u = 0.0; v = 0.0; return; }
+    if (h >= eps) { double inv = 1.0 / h; u = hu * inv; v = hv * inv; return; }
+    double r = h / eps;
+    double factor = 1.4142135623730951 * r / (eps * sqrt(1.0 + r * r * r * r));
+    u = factor * hu; v = factor * hv;
+}
+
+// correct_node (wetting_drying.rs:148-167): desingularise below h_dry, cap the speed
+DG_HD void correct_node(double h, double& hu, double& hv, double h_dry, double u_max) {
+    if (h <= 0.0) { hu = 0.0; hv = 0.0; return; }
+    if (h < h_dry) {
+        double u, v;
+        desing_velocity(h, hu, hv, h_dry, u, v);
+        hu = h * u;
+        hv = h * v;
+    }
+    double momentum = sqrt(hu * hu + hv * hv);
+    if (momentum > u_max * h) {
+        double sc = u_max * h / momentum;
+        hu *= sc;
+        hv *= sc;
+    }
+}
+
+}  // namespace dgswe

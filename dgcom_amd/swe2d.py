@@ -1,0 +1,315 @@
+"""Host-side mirror of the reference's device interface for the 2D shallow-water
+`WetDry` time step, over the C ABI of `include/dgswe.h` (ctypes).
+
+The names follow the reference: `SWEPhysics2D` (`src/physics/builder.rs:83`),
+whose `compute_rhs_into`, `compute_dt_ssp`, `element_dt`, `stage_where`,
+`finish_where` and `post_process` are those of `PhysicsModule` /
+`LocalTimeStepping` (`src/physics/traits.rs:61-163`,
+`src/time/multirate.rs:296-391`); `DeviceState` is the device `Integrable`
+(`gpu/src/swe2d.rs:2384-2396`); `Simulation.run` is the dt loop of
+`Simulation::run_with_callback` (`src/simulation/runner.rs:264-483`).
+
+There is no CPU fallback: without the built HIP library, or without a GPU,
+constructing `SWEPhysics2D` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_dp = ctypes.POINTER(ctypes.c_double)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+
+SSP_RK3, SSP_RK43 = 0, 1
+FRICTION, WET_DRY, SUBCELLS = 1, 2, 4
+BC_WALL, BC_TIDE = 0, 1
+
+# Every entry point of include/dgswe.h
+EXPORTS = (
+    "dg_last_error", "dg_create", "dg_destroy", "dg_upload_form", "dg_set_tide", "dg_state_alloc",
+    "dg_state_free", "dg_state_upload", "dg_state_download", "dg_state_copy", "dg_rhs", "dg_face_fluxes", "dg_stage",
+    "dg_element_dt", "dg_dt_ssp", "dg_step", "dg_run", "dg_is_finite", "dg_clip_count", "dg_masks",
+    "dg_timing_enable", "dg_timing_read", "dg_sync", "dg_copy_bench",
+)
+
+
+class Form(ctypes.Structure):
+    _fields_ = [("order", ctypes.c_int), ("n_elements", ctypes.c_int64), ("corners", _dp),
+                ("face_links", ctypes.POINTER(ctypes.c_uint32)), ("face_tags", _u8p), ("bed", _dp),
+                ("manning", _dp), ("manning_uniform", ctypes.c_double), ("coriolis_f0", ctypes.c_double),
+                ("g", ctypes.c_double), ("h_min", ctypes.c_double), ("h_dry", ctypes.c_double),
+                ("relaxation_time", ctypes.c_double), ("max_velocity", ctypes.c_double), ("flags", ctypes.c_int)]
+
+
+class TideDesc(ctypes.Structure):
+    _fields_ = [("mean", ctypes.c_double), ("ramp", ctypes.c_double), ("n", ctypes.c_int),
+                ("amplitude", ctypes.c_double * 8), ("omega", ctypes.c_double * 8), ("phase", ctypes.c_double * 8)]
+
+
+class StageDesc(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_double), ("b", ctypes.c_double), ("c", ctypes.c_double), ("time", ctypes.c_double)]
+
+
+_libs: dict = {}
+
+
+def library_path(fma: bool = False) -> str:
+    return os.path.join(HERE, "libdgswe_fma.so" if fma else "libdgswe.so")
+
+
+def load(fma: bool = False):
+    """The HIP library (raises if it was not built): the product build, or with
+    `fma` the variant that contracts multiply-adds."""
+    key = bool(fma)
+    if key in _libs:
+        return _libs[key]
+    path = library_path(fma)
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m dgcom_amd.build` (hipcc, gfx950)")
+    L = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    L.dg_last_error.restype = ctypes.c_char_p
+    L.dg_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    L.dg_destroy.argtypes = [vp]
+    L.dg_upload_form.argtypes = [vp, ctypes.POINTER(Form)]
+    L.dg_set_tide.argtypes = [vp, ctypes.POINTER(TideDesc)]
+    L.dg_state_alloc.argtypes = [vp, ctypes.POINTER(vp)]
+    L.dg_state_free.argtypes = [vp, vp]
+    L.dg_state_upload.argtypes = [vp, vp, _dp, _dp, _dp]
+    L.dg_state_download.argtypes = [vp, vp, _dp, _dp, _dp]
+    L.dg_state_copy.argtypes = [vp, vp, vp]
+    L.dg_rhs.argtypes = [vp, vp, ctypes.c_double, vp]
+    L.dg_face_fluxes.argtypes = [vp, vp, ctypes.c_double, _dp]
+    L.dg_stage.argtypes = [vp, vp, vp, vp, ctypes.POINTER(StageDesc)]
+    L.dg_element_dt.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int, _dp]
+    L.dg_dt_ssp.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int, _dp]
+    L.dg_step.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    L.dg_run.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp]
+    L.dg_is_finite.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int)]
+    L.dg_clip_count.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
+    L.dg_masks.argtypes = [vp, vp, _u8p, _u8p]
+    L.dg_timing_enable.argtypes = [vp, ctypes.c_int]
+    L.dg_timing_read.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_int64), _dp, ctypes.POINTER(ctypes.c_int64)]
+    L.dg_sync.argtypes = [vp]
+    L.dg_copy_bench.argtypes = [vp, ctypes.c_size_t, ctypes.c_int, _dp]
+    _libs[key] = L
+    return L
+
+
+def _check(L, status, what):
+    if status != 0:
+        raise RuntimeError(f"{what}: {L.dg_last_error().decode()} (status {status})")
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+class DeviceState:
+    """A device-resident state (h, hu, hv), SoA FP64."""
+
+    def __init__(self, physics: "SWEPhysics2D"):
+        self.physics = physics
+        self.handle = ctypes.c_void_p()
+        _check(physics.lib, physics.lib.dg_state_alloc(physics.ctx, ctypes.byref(self.handle)), "dg_state_alloc")
+
+    def upload(self, h, hu, hv):
+        L = self.physics.lib
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (h, hu, hv)]
+        n = self.physics.n_nodes_total
+        if any(a.size != n for a in arrs):
+            raise ValueError(f"state arrays must have {n} values each")
+        _check(L, L.dg_state_upload(self.physics.ctx, self.handle, *(_p(a) for a in arrs)), "dg_state_upload")
+        return self
+
+    def download(self):
+        L = self.physics.lib
+        out = [np.empty(self.physics.n_nodes_total) for _ in range(3)]
+        _check(L, L.dg_state_download(self.physics.ctx, self.handle, *(_p(a) for a in out)), "dg_state_download")
+        return out
+
+    def copy_from(self, other: "DeviceState"):
+        L = self.physics.lib
+        _check(L, L.dg_state_copy(self.physics.ctx, self.handle, other.handle), "dg_state_copy")
+
+    def is_finite(self) -> bool:
+        L = self.physics.lib
+        f = ctypes.c_int()
+        _check(L, L.dg_is_finite(self.physics.ctx, self.handle, ctypes.byref(f)), "dg_is_finite")
+        return bool(f.value)
+
+    def free(self):
+        if self.handle and self.physics.ctx:
+            self.physics.lib.dg_state_free(self.physics.ctx, self.handle)
+        self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class SWEPhysics2D:
+    """The `WetDry` 2D shallow-water physics on one HIP device."""
+
+    def __init__(self, order, mesh, bed, manning=None, manning_uniform=0.025, f0=0.0, tide=None, g=9.81,
+                 h_min=1e-6, h_dry=1e-3, relaxation_time=1.0, max_velocity=20.0, friction=True, wet_dry=True,
+                 subcells=True, device=0, fma=False):
+        self.lib = L = load(fma)
+        self.order = order
+        self.nn = (order + 1) ** 2
+        self.K = mesh.n_elements
+        self.n_nodes_total = self.K * self.nn
+        self.ctx = ctypes.c_void_p()
+        _check(L, L.dg_create(device, ctypes.byref(self.ctx)), "dg_create")
+        corners = np.ascontiguousarray(mesh.corners, dtype=np.float64)
+        links = np.ascontiguousarray(mesh.links, dtype=np.uint32)
+        tags = np.ascontiguousarray(mesh.tags, dtype=np.uint8)
+        bed = np.ascontiguousarray(bed, dtype=np.float64)
+        if bed.size != self.n_nodes_total:
+            raise ValueError("bed must have K*(N+1)^2 values")
+        man = None if manning is None else np.ascontiguousarray(manning, dtype=np.float64)
+        flags = (FRICTION if friction else 0) | (WET_DRY if wet_dry else 0) | (SUBCELLS if subcells else 0)
+        form = Form(order, self.K, _p(corners), links.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                    tags.ctypes.data_as(_u8p), _p(bed), None if man is None else _p(man), manning_uniform, f0, g,
+                    h_min, h_dry, relaxation_time, max_velocity, flags)
+        _check(L, L.dg_upload_form(self.ctx, ctypes.byref(form)), "dg_upload_form")
+        if tide is not None:
+            self.set_tide(tide)
+
+    @classmethod
+    def from_case(cls, case, **kw):
+        return cls(case.order, case.mesh, case.bed, manning=case.manning, f0=case.f0, tide=case.tide, g=case.g,
+                   h_min=case.h_min, h_dry=case.h_dry, **kw)
+
+    def set_tide(self, tide):
+        n = len(tide.amplitude)
+        if n > 8:
+            raise ValueError("at most 8 tidal constituents")
+        d = TideDesc()
+        d.mean, d.ramp, d.n = tide.mean, tide.ramp, n
+        for q in range(n):
+            d.amplitude[q], d.omega[q], d.phase[q] = tide.amplitude[q], tide.omega[q], tide.phase[q]
+        _check(self.lib, self.lib.dg_set_tide(self.ctx, ctypes.byref(d)), "dg_set_tide")
+
+    def close(self):
+        if self.ctx:
+            self.lib.dg_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- states ---
+    def state(self, h=None, hu=None, hv=None) -> DeviceState:
+        s = DeviceState(self)
+        if h is not None:
+            s.upload(h, hu, hv)
+        return s
+
+    # --- PhysicsModule ---
+    def compute_rhs_into(self, state: DeviceState, t: float, out: DeviceState):
+        _check(self.lib, self.lib.dg_rhs(self.ctx, state.handle, t, out.handle), "dg_rhs")
+
+    def face_fluxes(self, state: DeviceState, t: float) -> np.ndarray:
+        """F* at every face node, (K, 4, N+1, 3), each element's own outward side."""
+        out = np.empty(self.K * 4 * (self.order + 1) * 3)
+        _check(self.lib, self.lib.dg_face_fluxes(self.ctx, state.handle, t, _p(out)), "dg_face_fluxes")
+        return out.reshape(self.K, 4, self.order + 1, 3)
+
+    def compute_dt_ssp(self, state: DeviceState, cfl: float, scheme: int = SSP_RK3) -> float:
+        dt = ctypes.c_double()
+        _check(self.lib, self.lib.dg_dt_ssp(self.ctx, state.handle, cfl, scheme, ctypes.byref(dt)), "dg_dt_ssp")
+        return dt.value
+
+    # --- LocalTimeStepping ---
+    def element_dt(self, state: DeviceState, cfl: float, scheme: int = SSP_RK3) -> np.ndarray:
+        out = np.empty(self.K)
+        _check(self.lib, self.lib.dg_element_dt(self.ctx, state.handle, cfl, scheme, _p(out)), "dg_element_dt")
+        return out
+
+    def stage_where(self, base: DeviceState, input: DeviceState, out: DeviceState, a, b, c, time):
+        d = StageDesc(a, b, c, time)
+        _check(self.lib, self.lib.dg_stage(self.ctx, base.handle, input.handle, out.handle, ctypes.byref(d)),
+               "dg_stage")
+
+    def step(self, state: DeviceState, t: float, dt: float, scheme: int = SSP_RK3):
+        _check(self.lib, self.lib.dg_step(self.ctx, state.handle, t, dt, scheme), "dg_step")
+
+    def run(self, state: DeviceState, t: float, n_steps: int, cfl: float, scheme: int = SSP_RK3,
+            record_dt: bool = False):
+        te = ctypes.c_double()
+        dts = np.empty(n_steps) if record_dt else None
+        _check(self.lib, self.lib.dg_run(self.ctx, state.handle, t, n_steps, cfl, scheme, ctypes.byref(te),
+                                         None if dts is None else _p(dts)), "dg_run")
+        return (te.value, dts) if record_dt else te.value
+
+    def negative_depth_clips(self) -> int:
+        v = ctypes.c_uint64()
+        _check(self.lib, self.lib.dg_clip_count(self.ctx, ctypes.byref(v)), "dg_clip_count")
+        return int(v.value)
+
+    def masks(self, state: DeviceState):
+        nd = np.empty(self.n_nodes_total, dtype=np.uint8)
+        es = np.empty(self.K, dtype=np.uint8)
+        _check(self.lib, self.lib.dg_masks(self.ctx, state.handle, nd.ctypes.data_as(_u8p), es.ctypes.data_as(_u8p)),
+               "dg_masks")
+        return nd, es
+
+    # --- measurement ---
+    def timing(self, on: bool):
+        _check(self.lib, self.lib.dg_timing_enable(self.ctx, int(on)), "dg_timing_enable")
+
+    def timing_read(self):
+        sm, dm = ctypes.c_double(), ctypes.c_double()
+        sn, dn = ctypes.c_int64(), ctypes.c_int64()
+        _check(self.lib, self.lib.dg_timing_read(self.ctx, ctypes.byref(sm), ctypes.byref(sn), ctypes.byref(dm),
+                                                 ctypes.byref(dn)), "dg_timing_read")
+        return sm.value, sn.value, dm.value, dn.value
+
+    def sync(self):
+        _check(self.lib, self.lib.dg_sync(self.ctx), "dg_sync")
+
+    def copy_bench(self, nbytes: int, reps: int = 20) -> float:
+        ms = ctypes.c_double()
+        _check(self.lib, self.lib.dg_copy_bench(self.ctx, nbytes, reps, ctypes.byref(ms)), "dg_copy_bench")
+        return ms.value
+
+
+class Simulation:
+    """The dt loop of `Simulation::run_with_callback` over a device physics
+    (`src/simulation/runner.rs:264-483`): SSP-capped CFL, landing on `t_end`
+    within LANDING_SLACK, a finiteness check per step."""
+
+    LANDING_SLACK = 1e-3
+
+    def __init__(self, physics: SWEPhysics2D, cfl: float, scheme: int = SSP_RK3, max_steps=None):
+        self.physics = physics
+        self.cfl = cfl
+        self.scheme = scheme
+        self.max_steps = max_steps
+
+    def run(self, state: DeviceState, t: float, t_end: float, callback=None):
+        n = 0
+        while t < t_end:
+            if self.max_steps is not None and n >= self.max_steps:
+                raise RuntimeError(f"Maximum step limit ({self.max_steps}) reached")
+            dt = self.physics.compute_dt_ssp(state, self.cfl, self.scheme)
+            landed = t + dt * (1.0 + self.LANDING_SLACK) >= t_end
+            if landed:
+                dt = t_end - t
+            self.physics.step(state, t, dt, self.scheme)
+            t = t_end if landed else t + dt
+            n += 1
+            if not state.is_finite():
+                raise RuntimeError(f"The state is not finite at t = {t:.1f} s: it has blown up")
+            if callback is not None:
+                callback(state, t)
+        return t, n

@@ -1,0 +1,143 @@
+/*
+ * dgswe — MI355X (gfx950) implementation of the 2D nodal-DG shallow-water
+ * `WetDry` time step of EmilLindfors/dgcom, behind a C ABI.
+ *
+ * The entry points replace the reference's device boundary for this path:
+ *   - `PhysicsModule<SWESolution2D>`            src/physics/traits.rs:61-163
+ *   - `LocalTimeStepping<SWESolution2D>`        src/time/multirate.rs:296-391
+ *   - `Integrable` (the device state)           src/time/integrator.rs:54-106
+ *   - `CaseDevice::run_2d` and its physics-as-data input `SWEDeviceForm2D`
+ *                                               src/case/run.rs:179-191, src/physics/device.rs:27-143
+ * Conventions (the reference's, src/physics/device.rs:79-143, src/case/run.rs:528):
+ * every call returns 0 on success or a nonzero status whose reason is
+ * `dg_last_error()` (the reference's `Result<_, String>`); no exception
+ * crosses the ABI; calls on one context are issued by one host thread in
+ * order and run on one HIP stream. Numerical failure is not a call error:
+ * `dg_is_finite` and `dg_clip_count` report it (src/simulation/runner.rs:441-449,
+ * src/physics/builder.rs:199-206).
+ *
+ * Layout: SoA FP64, node i of element k at k*NN + i, NN = (N+1)^2, node
+ * i = j*(N+1) + a with r fastest (src/solver/state/system_solution.rs:135-147).
+ */
+#ifndef DGSWE_H
+#define DGSWE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dg_ctx dg_ctx;
+typedef struct dg_state dg_state;
+
+/* Boundary kinds of a boundary face (face_tags). */
+enum { DG_BC_WALL = 0, DG_BC_TIDE = 1 };
+
+/* Flags of dg_form.flags */
+enum { DG_FRICTION = 1, DG_WET_DRY = 2, DG_SUBCELLS = 4 };
+
+/* Schemes (src/time/integrator.rs:226-263) */
+enum { DG_SSP_RK3 = 0, DG_SSP_RK43 = 1 };
+
+/*
+ * The physics as data: replaces `SWEDeviceForm2D` (src/physics/device.rs:27-71).
+ * face_links uses the reference's 4-byte `FaceLink` (src/mesh/core/mesh2d.rs:54-107):
+ * element << 4 | shared << 3 | left << 2 | face, 0xFFFFFFFF at a boundary, whose
+ * kind is face_tags (the boundary tag's condition, src/case/run.rs:653-737,892).
+ */
+typedef struct dg_form {
+    int order;                    /* N in 1..3 */
+    int64_t n_elements;           /* K */
+    const double* corners;        /* [K][4][2], counter-clockwise from (r,s) = (-1,-1) */
+    const uint32_t* face_links;   /* [K][4] */
+    const uint8_t* face_tags;     /* [K][4] */
+    const double* bed;            /* [K*NN] nodal bed elevation B */
+    const double* manning;        /* [K] per-element Manning n, or NULL: manning_uniform */
+    double manning_uniform;
+    double coriolis_f0;           /* f-plane */
+    double g, h_min, h_dry;       /* 9.81, 1e-6, 1e-3 */
+    double relaxation_time;       /* thin-layer tau, 1 s */
+    double max_velocity;          /* 20 m/s */
+    int flags;                    /* DG_FRICTION | DG_WET_DRY | DG_SUBCELLS */
+} dg_form;
+
+/* Uniform harmonic tide of the open boundary (src/boundary/harmonic_tide.rs:24-136). */
+typedef struct dg_tide {
+    double mean, ramp;            /* ramp <= 0: none */
+    int n;                        /* constituents, at most 8 */
+    double amplitude[8], omega[8], phase[8];
+} dg_tide;
+
+/* One element stage: out = a*base + b*input + c*L(input, time), then damping over c
+ * and the wet/dry post-process (src/time/multirate.rs:137-200 ElementStage). */
+typedef struct dg_stage_desc {
+    double a, b, c, time;
+} dg_stage_desc;
+
+const char* dg_last_error(void);
+
+/* Context on HIP device `device` (replaces the reference's `Gpu` + `GpuSWE2D`). */
+int dg_create(int device, dg_ctx** ctx);
+int dg_destroy(dg_ctx* ctx);
+
+/* Upload the statics (GpuSWE2D::from_form, gpu/src/swe2d.rs:3360). */
+int dg_upload_form(dg_ctx* ctx, const dg_form* form);
+int dg_set_tide(dg_ctx* ctx, const dg_tide* tide);
+
+/* Device states (DeviceState::upload / download, gpu/src/swe2d.rs:2384-2396). */
+int dg_state_alloc(dg_ctx* ctx, dg_state** state);
+int dg_state_free(dg_ctx* ctx, dg_state* state);
+int dg_state_upload(dg_ctx* ctx, dg_state* state, const double* h, const double* hu, const double* hv);
+int dg_state_download(dg_ctx* ctx, const dg_state* state, double* h, double* hu, double* hv);
+int dg_state_copy(dg_ctx* ctx, dg_state* dst, const dg_state* src);
+
+/* PhysicsModule::compute_rhs_into (src/physics/traits.rs:61-163): out = L(in, t). */
+int dg_rhs(dg_ctx* ctx, const dg_state* in, double t, dg_state* out);
+
+/* F* at every face node [K][4][N+1][3] (outward, each element's own side, face
+ * nodes in the reference's face order), as the stage kernel evaluates it: the
+ * face fluxes that compute_rhs_swe_2d_mass_fluxes_into reports
+ * (src/solver/rhs/swe_2d.rs:1005). */
+int dg_face_fluxes(dg_ctx* ctx, const dg_state* in, double t, double* out);
+
+/* LocalTimeStepping::stage_where over all elements (src/physics/builder.rs:652-737),
+ * with the post-process of finish_where fused into every stage (one level). */
+int dg_stage(dg_ctx* ctx, const dg_state* base, const dg_state* input, dg_state* out, const dg_stage_desc* stage);
+
+/* LocalTimeStepping::element_dt per element (src/solver/rhs/swe_2d.rs:2018-2115) into
+ * the host array out[K], and PhysicsModule::compute_dt_ssp (builder.rs:464-485). */
+int dg_element_dt(dg_ctx* ctx, const dg_state* state, double cfl, int scheme, double* out);
+int dg_dt_ssp(dg_ctx* ctx, const dg_state* state, double cfl, int scheme, double* dt);
+
+/* One SSP step of dt from t in place (MultirateStepper::step, one level,
+ * src/time/multirate.rs:1285-1311). */
+int dg_step(dg_ctx* ctx, dg_state* state, double t, double dt, int scheme);
+
+/* n_steps adaptive SSP steps from t (the dt loop of Simulation::run_with_callback,
+ * src/simulation/runner.rs:264-483, without output landing); t_end receives the
+ * end time, dts (optional, n_steps) each step's dt. */
+int dg_run(dg_ctx* ctx, dg_state* state, double t, int n_steps, double cfl, int scheme, double* t_end, double* dts);
+
+/* Integrable::is_finite (integrator.rs:54-106) and the negative-depth clips. */
+int dg_is_finite(dg_ctx* ctx, const dg_state* state, int* finite);
+int dg_clip_count(dg_ctx* ctx, uint64_t* clips);
+
+/* Integer wet/dry masks: node_dry[K*NN] = h < h_dry, elem_subcell[K] = the element
+ * takes the subcell volume term (swe_2d_split_form.rs:711-713). */
+int dg_masks(dg_ctx* ctx, const dg_state* state, uint8_t* node_dry, uint8_t* elem_subcell);
+
+/* Measurement: per-launch HIP-event timing of the stage kernel on the context's
+ * stream. dg_timing_read returns the summed kernel milliseconds and launches since
+ * the last reset. dg_copy_bench times a plain device copy of `bytes` (reps times). */
+int dg_timing_enable(dg_ctx* ctx, int on);
+int dg_timing_read(dg_ctx* ctx, double* stage_ms, int64_t* stage_launches, double* dt_ms, int64_t* dt_launches);
+int dg_sync(dg_ctx* ctx);
+int dg_copy_bench(dg_ctx* ctx, size_t bytes, int reps, double* ms_per_copy);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DGSWE_H */

@@ -33,7 +33,38 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
+        _lib = _bind(ctypes.CDLL(LIB_PATH))
+    return _lib
+
+
+def mutant(name: str):
+    """A planted-mistake build of the restatement (NO_BED, LIFT, TRANSPOSE), bound
+    like lib(); use with `using(...)`."""
+    path = os.path.join(HERE, f"liboracle_swe2d_mutant_{name.lower()}.so")
+    subprocess.run(["make", "-s", "-C", HERE, "mutant", f"MUTANT={name}"], check=True)
+    return _bind(ctypes.CDLL(path))
+
+
+class using:
+    """Context manager: route Oracle objects created inside to another build."""
+
+    def __init__(self, L):
+        self.L = L
+
+    def __enter__(self):
+        global _lib
+        self.saved = lib()
+        _lib = self.L
+        return self.L
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.saved
+
+
+def _bind(L):
+    """Declare the C signatures on a loaded build."""
+    if True:
         L.oc_create.restype = ctypes.c_void_p
         L.oc_create.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, _u32p, _u8p, _dp, _dp, ctypes.c_double,
                                 ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int]
@@ -64,8 +95,8 @@ def lib():
         L.oc_cbrt.restype = ctypes.c_double
         L.oc_set_threads.argtypes = [ctypes.c_int]
         L.oc_max_threads.restype = ctypes.c_int
-        _lib = L
-    return _lib
+        L.oc_face_mass.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, _dp]
+    return L
 
 
 def _p(a):
@@ -86,7 +117,7 @@ class Oracle:
 
     def __init__(self, order, mesh, bed, manning=None, f0=0.0, g=9.81, h_min=1e-6, h_dry=1e-3, surface=SURFACE_HLL,
                  subcells=True, friction=True, wet_dry=True, manufactured=False, tide=None):
-        L = lib()
+        self._L = L = lib()
         self.order = order
         self.nn = (order + 1) ** 2
         self.K = mesh.n_elements
@@ -112,7 +143,7 @@ class Oracle:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().oc_destroy(self.h)
+            self._L.oc_destroy(self.h)
             self.h = None
 
     def set_tide(self, tide):
@@ -120,63 +151,63 @@ class Oracle:
         a = np.array(tide.amplitude, dtype=np.float64)
         w = np.array(tide.omega, dtype=np.float64)
         p = np.array(tide.phase, dtype=np.float64)
-        lib().oc_set_tide(self.h, tide.mean, tide.ramp, n, _p(a), _p(w), _p(p))
+        self._L.oc_set_tide(self.h, tide.mean, tide.ramp, n, _p(a), _p(w), _p(p))
 
     def tide_elevation(self, t):
-        return lib().oc_tide_elevation(self.h, t)
+        return self._L.oc_tide_elevation(self.h, t)
 
     def rhs(self, h, hu, hv, t=0.0):
         h, hu, hv = (np.ascontiguousarray(a, dtype=np.float64) for a in (h, hu, hv))
         out = [np.empty_like(h) for _ in range(3)]
-        lib().oc_rhs(self.h, _p(h), _p(hu), _p(hv), t, *(_p(o) for o in out))
+        self._L.oc_rhs(self.h, _p(h), _p(hu), _p(hv), t, *(_p(o) for o in out))
         return out
 
     def face_mass(self, h, hu, hv, t=0.0):
         out = np.empty(self.K * 4 * (self.order + 1))
-        lib().oc_face_mass(self.h, _p(h), _p(hu), _p(hv), t, _p(out))
+        self._L.oc_face_mass(self.h, _p(h), _p(hu), _p(hv), t, _p(out))
         return out.reshape(self.K, 4, self.order + 1)
 
     def stage(self, base, inp, a, b, c, time, last):
         out = [np.empty_like(base[0]) for _ in range(3)]
-        lib().oc_stage(self.h, *(_p(x) for x in base), *(_p(x) for x in inp), *(_p(o) for o in out), a, b, c, time,
+        self._L.oc_stage(self.h, *(_p(x) for x in base), *(_p(x) for x in inp), *(_p(o) for o in out), a, b, c, time,
                        int(last))
         return out
 
     def step(self, h, hu, hv, t, dt, scheme=0):
         """One SSP step in place."""
-        lib().oc_step(self.h, _p(h), _p(hu), _p(hv), t, dt, scheme)
+        self._L.oc_step(self.h, _p(h), _p(hu), _p(hv), t, dt, scheme)
 
     def dt(self, h, hu, hv, cfl, scheme=0):
-        return lib().oc_dt(self.h, _p(h), _p(hu), _p(hv), cfl, scheme)
+        return self._L.oc_dt(self.h, _p(h), _p(hu), _p(hv), cfl, scheme)
 
     def element_dt(self, h, hu, hv, cfl, scheme=0):
         out = np.empty(self.K)
-        lib().oc_element_dt(self.h, _p(h), _p(hu), _p(hv), cfl, scheme, _p(out))
+        self._L.oc_element_dt(self.h, _p(h), _p(hu), _p(hv), cfl, scheme, _p(out))
         return out
 
     def run(self, h, hu, hv, t, n_steps, cfl, scheme=0, record_dt=False):
         """n_steps adaptive SSP steps in place; returns the end time (and the dts)."""
         dts = np.empty(n_steps) if record_dt else None
-        te = lib().oc_run(self.h, _p(h), _p(hu), _p(hv), t, n_steps, cfl, scheme, None if dts is None else _p(dts))
+        te = self._L.oc_run(self.h, _p(h), _p(hu), _p(hv), t, n_steps, cfl, scheme, None if dts is None else _p(dts))
         return (te, dts) if record_dt else te
 
     def run_timed(self, h, hu, hv, t, n_steps, cfl, scheme=0):
         sec = ctypes.c_double()
-        te = lib().oc_run_timed(self.h, _p(h), _p(hu), _p(hv), t, n_steps, cfl, scheme, ctypes.byref(sec))
+        te = self._L.oc_run_timed(self.h, _p(h), _p(hu), _p(hv), t, n_steps, cfl, scheme, ctypes.byref(sec))
         return te, sec.value
 
     def clips(self):
-        return int(lib().oc_clips(self.h))
+        return int(self._L.oc_clips(self.h))
 
     def masks(self, h):
         nd = np.empty(self.K * self.nn, dtype=np.uint8)
         es = np.empty(self.K, dtype=np.uint8)
-        lib().oc_masks(self.h, _p(np.ascontiguousarray(h)), nd.ctypes.data_as(_u8p), es.ctypes.data_as(_u8p))
+        self._L.oc_masks(self.h, _p(np.ascontiguousarray(h)), nd.ctypes.data_as(_u8p), es.ctypes.data_as(_u8p))
         return nd, es
 
     def geometry(self):
         out = [np.empty(self.K * self.nn) for _ in range(6)]
-        lib().oc_geometry(self.h, *(_p(o) for o in out))
+        self._L.oc_geometry(self.h, *(_p(o) for o in out))
         return dict(zip(("det_j", "mass", "jrx", "jry", "jsx", "jsy"), out))
 
 

@@ -174,7 +174,11 @@ struct Solver {
                 else { m0 = 0.5 * (ma0 + mx[c]); m1 = 0.5 * (ma1 + my[c]); }
                 State<T> fl = wintermeyer_flux(qa, nodes[ic], T(m0), T(m1), g);
                 acc = acc + scale(2.0 * d1[a * n1 + c], fl);
+#ifdef ORACLE_MUTANT_TRANSPOSE
+                rhs[ic] = rhs[ic] - scale(2.0 * d1[a * n1 + c], fl);  // planted mistake: D transposed
+#else
                 rhs[ic] = rhs[ic] - scale(2.0 * d1[c * n1 + a], fl);
+#endif
             }
             // bathymetry is always set on this path
             T db(0.0), db_x(0.0), db_y(0.0);
@@ -187,6 +191,9 @@ struct Solver {
             if (affine) { bx = db * T(ma0); by = db * T(ma1); }
             else { bx = T(0.5) * (db * T(ma0) + db_x); by = T(0.5) * (db * T(ma1) + db_y); }
             T force = T(g) * qa.h;
+#ifdef ORACLE_MUTANT_NO_BED
+            force = T(0.0);  // planted mistake: the bed-slope term dropped
+#endif
             acc = acc + State<T>{T(0.0), force * bx, force * by};
             rhs[ia] = rhs[ia] - acc;
         }
@@ -281,8 +288,8 @@ struct Solver {
             else {
                 T h_a = nodes[ia].h, b_a = nodes[ia].b;
                 T metric = T(0.25 * g) * ((right.h + h_a) * (b_a - right.b) + (left.h + h_a) * (b_a - left.b)) / T(w[a]);
-                fx = bedt * T(0.5 * (ix[a] + ix[a + 1])) + metric * T(ix[a + 1] - ix[a]);
-                fy = bedt * T(0.5 * (iy[a] + iy[a + 1])) + metric * T(iy[a + 1] - iy[a]);
+                fx = bedt * T(0.5) * T(ix[a] + ix[a + 1]) + metric * T(ix[a + 1] - ix[a]);
+                fy = bedt * T(0.5) * T(iy[a] + iy[a + 1]) + metric * T(iy[a + 1] - iy[a]);
             }
             rhs[ia] = rhs[ia] + State<T>{T(0.0), fx, fy};
         }
@@ -344,6 +351,9 @@ struct Solver {
                 State<T> fstar = link_is_boundary(l) ? boundary_flux(k, f, q, nx, ny, eta_e)
                                                      : faces[((size_t)k * 4 + f) * n1 + fi];
                 State<T> diff = wintermeyer_flux(q, q, T(nx), T(ny), g) - fstar;
+#ifdef ORACLE_MUTANT_LIFT
+                if (f == 1) diff = -diff;  // planted mistake: one face lifted with the wrong sign
+#endif
                 rhs[nd] = rhs[nd] + scale(sj * ops.lift[f][nd * n1 + fi], diff);
             }
         }

@@ -1,0 +1,60 @@
+// C ABI over the emulated product kernels (tests only).
+#include "hip_emu.hpp"
+
+#include "../../dgcom_amd/csrc/swe_kernels.hpp"
+
+using namespace dgswe;
+
+extern "C" {
+
+void emu_init(void) {
+    for (int n = 1; n <= 3; ++n) c_gll[n] = make_tables(n);
+}
+
+unsigned long long emu_stage(int order, int64_t K, const double* corners, const uint32_t* links, const uint8_t* tags,
+                             const uint8_t* affine, const double* bed, const double* manning, const double* base,
+                             const double* input, double* out, double a, double b, double c, double eta_e, double g,
+                             double h_min, double h_dry, double u_max, double tau, double f0, int rhs_only, int flags,
+                             double* fstar_out) {
+    size_t nn = (size_t)(order + 1) * (order + 1), nt = (size_t)K * nn;
+    unsigned long long clips = 0;
+    StageParams p;
+    p.bh = base; p.bhu = base + nt; p.bhv = base + 2 * nt;
+    p.ih = input; p.ihu = input + nt; p.ihv = input + 2 * nt;
+    p.oh = out; p.ohu = out + nt; p.ohv = out + 2 * nt;
+    p.bed = bed; p.corners = corners; p.manning = manning; p.links = links; p.tags = tags; p.affine = affine;
+    p.n_elements = K; p.a = a; p.b = b; p.c = c; p.eta_e = eta_e;
+    p.g = g; p.h_min = h_min; p.h_dry = h_dry; p.u_max = u_max; p.tau = tau; p.f0 = f0;
+    p.rhs_only = rhs_only; p.flags = flags; p.clips = &clips; p.fstar_out = fstar_out;
+    int epb = 64 / (int)nn;
+    unsigned grid = (unsigned)((K + epb - 1) / epb);
+    switch (order) {
+        case 1: emu_launch(grid, 64, [&] { stage_kernel<1>(p); }); break;
+        case 2: emu_launch(grid, 64, [&] { stage_kernel<2>(p); }); break;
+        default: emu_launch(grid, 64, [&] { stage_kernel<3>(p); }); break;
+    }
+    return clips;
+}
+
+double emu_dt(int order, int64_t K, const double* state, const double* corners, const uint32_t* links,
+              const uint8_t* affine, double cfl, double g, double h_min, double h_dry, double pcfl, double max_cfl,
+              double max_cfl_sub, int positivity, double* element_dt) {
+    size_t nn = (size_t)(order + 1) * (order + 1), nt = (size_t)K * nn;
+    unsigned long long bits = 0x7FF0000000000000ull;
+    DtParams p;
+    p.h = state; p.hu = state + nt; p.hv = state + 2 * nt;
+    p.corners = corners; p.links = links; p.affine = affine; p.n_elements = K;
+    p.cfl = cfl; p.g = g; p.h_min = h_min; p.h_dry = h_dry; p.pcfl = pcfl; p.max_cfl = max_cfl;
+    p.max_cfl_sub = max_cfl_sub; p.positivity = positivity; p.element_dt = element_dt; p.dt_min_bits = &bits;
+    unsigned grid = (unsigned)((K + 255) / 256);
+    switch (order) {
+        case 1: emu_launch(grid, 256, [&] { dt_kernel<1>(p); }); break;
+        case 2: emu_launch(grid, 256, [&] { dt_kernel<2>(p); }); break;
+        default: emu_launch(grid, 256, [&] { dt_kernel<3>(p); }); break;
+    }
+    double dt;
+    std::memcpy(&dt, &bits, 8);
+    return dt;
+}
+
+}  // extern "C"

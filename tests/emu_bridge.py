@@ -1,0 +1,90 @@
+"""ctypes access to the CPU emulation of the product's HIP kernels (tests only)."""
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "emu")
+_dp = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+        L = ctypes.CDLL(os.path.join(HERE, "libswe_emu.so"))
+        L.emu_stage.restype = ctypes.c_ulonglong
+        L.emu_stage.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                _dp, _dp, _dp, _dp, _dp] + [ctypes.c_double] * 10 + [ctypes.c_int, ctypes.c_int, _dp]
+        L.emu_dt.restype = ctypes.c_double
+        L.emu_dt.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, _dp, ctypes.c_void_p, ctypes.c_void_p] + \
+            [ctypes.c_double] * 7 + [ctypes.c_int, _dp]
+        L.emu_init()
+        _lib = L
+    return _lib
+
+
+def parallelogram(c):
+    v = c.reshape(-1, 8)
+    bx = v[:, 0] - v[:, 2] + v[:, 4] - v[:, 6]
+    by = v[:, 1] - v[:, 3] + v[:, 5] - v[:, 7]
+    scale = np.zeros(len(v))
+    for i in range(4):
+        j = (i + 1) % 4
+        scale = np.fmax(scale, np.hypot(v[:, 2 * j] - v[:, 2 * i], v[:, 2 * j + 1] - v[:, 2 * i + 1]))
+    return (np.hypot(bx, by) <= 1.0e-12 * scale).astype(np.uint8)
+
+
+class EmulatedKernels:
+    """The product's stage and time-step kernels run on host threads."""
+
+    def __init__(self, case, flags=7):
+        self.case = case
+        self.order = case.order
+        self.K = case.n_elements
+        self.corners = np.ascontiguousarray(case.mesh.corners, dtype=np.float64).ravel()
+        self.links = np.ascontiguousarray(case.mesh.links, dtype=np.uint32).ravel()
+        self.tags = np.ascontiguousarray(case.mesh.tags, dtype=np.uint8).ravel()
+        self.affine = parallelogram(self.corners)
+        self.bed = np.ascontiguousarray(case.bed, dtype=np.float64)
+        self.manning = np.ascontiguousarray(case.manning, dtype=np.float64)
+        self.flags = flags
+
+    def _p(self, a):
+        return a.ctypes.data_as(_dp)
+
+    def stage(self, base, inp, a, b, c, time, rhs_only=False, fstar=None):
+        c_ = self.case
+        B = np.concatenate(base)
+        I = np.concatenate(inp)
+        out = np.empty_like(I)
+        eta = c_.tide.elevation(time)
+        lib().emu_stage(self.order, self.K, self._p(self.corners), self.links.ctypes.data, self.tags.ctypes.data,
+                        self.affine.ctypes.data, self._p(self.bed), self._p(self.manning), self._p(B), self._p(I),
+                        self._p(out), a, b, c, eta, c_.g, c_.h_min, c_.h_dry, 20.0, 1.0, c_.f0, int(rhs_only),
+                        self.flags, None if fstar is None else self._p(fstar))
+        return np.split(out, 3)
+
+    def face_fluxes(self, h, hu, hv, t):
+        out = np.empty(self.K * 4 * (self.order + 1) * 3)
+        self.stage((h, hu, hv), (h, hu, hv), 0.0, 0.0, 0.0, t, rhs_only=True, fstar=out)
+        return out.reshape(self.K, 4, self.order + 1, 3)
+
+    def rhs(self, h, hu, hv, t):
+        return self.stage((h, hu, hv), (h, hu, hv), 0.0, 0.0, 0.0, t, rhs_only=True)
+
+    def element_dt(self, h, hu, hv, cfl, scheme=0):
+        n = self.order
+        ssp = 1.0 if scheme == 0 else 2.0
+        lin = ([1.6, 1.04, 0.73, 0.56] if scheme == 0 else [2.2, 1.51, 1.15, 0.93])[n - 1]
+        sub = ([0.94, 0.65, 0.47, 0.36] if scheme == 0 else [1.5, 1.1, 0.97, 0.75])[n - 1]
+        pcfl = ssp * ((2.0 * n + 1.0) / (2.0 * n * (n + 1.0)))
+        S = np.concatenate((h, hu, hv))
+        out = np.empty(self.K)
+        dt = lib().emu_dt(n, self.K, self._p(S), self._p(self.corners), self.links.ctypes.data,
+                          self.affine.ctypes.data, cfl, self.case.g, self.case.h_min, self.case.h_dry, pcfl,
+                          max(pcfl, 0.9 * lin), max(pcfl, 0.9 * sub), int(bool(self.flags & 2)), self._p(out))
+        return out, dt
