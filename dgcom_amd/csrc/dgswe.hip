@@ -54,7 +54,12 @@ struct dg_ctx {
     double *corners = nullptr, *bed = nullptr, *manning = nullptr, *element_dt = nullptr;
     uint32_t* links = nullptr;
     uint8_t *tags = nullptr, *affine = nullptr;
-    unsigned long long *clips = nullptr, *dt_bits = nullptr;
+    unsigned long long* clips = nullptr;
+    uint32_t* queue = nullptr;
+    uint8_t* queue_count = nullptr;
+    double *block_min = nullptr, *dt_value = nullptr;
+    uint32_t* sub_list = nullptr;
+    unsigned int *chunk_total = nullptr, *sub_count = nullptr;
     int* flag = nullptr;
     dg_state* work[4] = {nullptr, nullptr, nullptr, nullptr};
     bool timing = false;
@@ -111,12 +116,18 @@ static double tide_elevation(const dg_tide& t, double time) {
 }
 
 static int free_statics(dg_ctx* c) {
-    void* ptrs[] = {c->corners, c->bed, c->manning, c->element_dt, c->links, c->tags, c->affine};
+    void* ptrs[] = {c->corners, c->bed, c->manning, c->element_dt, c->links, c->tags, c->affine, c->queue,
+                    c->queue_count, c->block_min, c->sub_list, c->chunk_total};
     for (void* p : ptrs)
         if (p) DG_TRY(hipFree(p));
     c->corners = c->bed = c->manning = c->element_dt = nullptr;
     c->links = nullptr;
     c->tags = c->affine = nullptr;
+    c->queue = nullptr;
+    c->queue_count = nullptr;
+    c->block_min = nullptr;
+    c->sub_list = nullptr;
+    c->chunk_total = nullptr;
     for (auto& w : c->work) {
         if (w) {
             DG_TRY(hipFree(w->d));
@@ -144,13 +155,26 @@ static int launch_stage(dg_ctx* c, const double* base, const double* input, doub
     p.flags = c->flags;
     p.clips = c->clips;
     p.fstar_out = fstar_out;
-    int epb = 64 / c->nn;
-    dim3 grid((unsigned)((c->K + epb - 1) / epb));
+    p.queue = c->queue;
+    p.queue_count = c->queue_count;
+    const int epb = 64 / c->nn;
+    const unsigned int nblk = (unsigned int)((c->K + epb - 1) / epb);
+    const unsigned int nchunk = (nblk + 1023) / 1024;
+    p.xcd_stride = (nblk + 7) / 8;
+    dim3 grid(8 * p.xcd_stride), sub_grid(std::min(nblk, 2048u));
     if (c->timing) DG_TRY(hipEventRecord(c->ev[0], c->stream));
     switch (c->order) {
-        case 1: hipLaunchKernelGGL(stage_kernel<1>, grid, dim3(64), 0, c->stream, p); break;
-        case 2: hipLaunchKernelGGL(stage_kernel<2>, grid, dim3(64), 0, c->stream, p); break;
-        default: hipLaunchKernelGGL(stage_kernel<3>, grid, dim3(64), 0, c->stream, p); break;
+        case 1: hipLaunchKernelGGL(wet_stage_kernel<1>, grid, dim3(64), 0, c->stream, p); break;
+        case 2: hipLaunchKernelGGL(wet_stage_kernel<2>, grid, dim3(64), 0, c->stream, p); break;
+        default: hipLaunchKernelGGL(wet_stage_kernel<3>, grid, dim3(64), 0, c->stream, p); break;
+    }
+    hipLaunchKernelGGL(chunk_count_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, nblk, c->chunk_total);
+    hipLaunchKernelGGL(compact_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, c->queue, nblk,
+                       c->chunk_total, epb, c->sub_list, c->sub_count);
+    switch (c->order) {
+        case 1: hipLaunchKernelGGL(subcell_stage_kernel<1>, sub_grid, dim3(64), 0, c->stream, p, c->sub_list, c->sub_count); break;
+        case 2: hipLaunchKernelGGL(subcell_stage_kernel<2>, sub_grid, dim3(64), 0, c->stream, p, c->sub_list, c->sub_count); break;
+        default: hipLaunchKernelGGL(subcell_stage_kernel<3>, sub_grid, dim3(64), 0, c->stream, p, c->sub_list, c->sub_count); break;
     }
     DG_TRY(hipGetLastError());
     if (c->timing) {
@@ -181,20 +205,22 @@ static int launch_dt(dg_ctx* c, const double* state, double cfl, int scheme, boo
     p.max_cfl_sub = std::fmax(p.pcfl, 0.9 * (scheme == DG_SSP_RK3 ? sub3 : sub43)[c->order - 1]);
     p.positivity = (c->flags & DG_WET_DRY) ? 1 : 0;
     p.element_dt = element_out ? c->element_dt : nullptr;
-    p.dt_min_bits = c->dt_bits;
-    hipLaunchKernelGGL(fill_u64, dim3(1), dim3(1), 0, c->stream, c->dt_bits, 0x7FF0000000000000ull);
-    DG_TRY(hipGetLastError());
-    dim3 grid((unsigned)((c->K + 255) / 256));
+    p.block_min = c->block_min;
+    const int epb = 64 / c->nn;
+    const unsigned int nblk = (unsigned int)((c->K + epb - 1) / epb);
+    p.xcd_stride = (nblk + 7) / 8;
+    dim3 grid(8 * p.xcd_stride);
     if (c->timing) DG_TRY(hipEventRecord(c->ev[0], c->stream));
     switch (c->order) {
-        case 1: hipLaunchKernelGGL(dt_kernel<1>, grid, dim3(256), 0, c->stream, p); break;
-        case 2: hipLaunchKernelGGL(dt_kernel<2>, grid, dim3(256), 0, c->stream, p); break;
-        default: hipLaunchKernelGGL(dt_kernel<3>, grid, dim3(256), 0, c->stream, p); break;
+        case 1: hipLaunchKernelGGL(dt_kernel<1>, grid, dim3(64), 0, c->stream, p); break;
+        case 2: hipLaunchKernelGGL(dt_kernel<2>, grid, dim3(64), 0, c->stream, p); break;
+        default: hipLaunchKernelGGL(dt_kernel<3>, grid, dim3(64), 0, c->stream, p); break;
     }
+    hipLaunchKernelGGL(min_kernel, dim3(1), dim3(1024), 0, c->stream, c->block_min, (size_t)nblk, c->dt_value);
     DG_TRY(hipGetLastError());
     if (c->timing) DG_TRY(hipEventRecord(c->ev[1], c->stream));
-    unsigned long long bits = 0;
-    DG_TRY(hipMemcpyAsync(&bits, c->dt_bits, 8, hipMemcpyDeviceToHost, c->stream));
+    double value = 0.0;
+    DG_TRY(hipMemcpyAsync(&value, c->dt_value, 8, hipMemcpyDeviceToHost, c->stream));
     DG_TRY(hipStreamSynchronize(c->stream));
     if (c->timing) {
         float ms = 0.f;
@@ -202,7 +228,7 @@ static int launch_dt(dg_ctx* c, const double* state, double cfl, int scheme, boo
         c->dt_ms += ms;
         c->dt_n += 1;
     }
-    std::memcpy(dt, &bits, 8);
+    *dt = value;
     return 0;
 }
 
@@ -256,7 +282,8 @@ int dg_create(int device, dg_ctx** out) {
         tables = true;
     }
     DG_TRY(hipMalloc(&c->clips, sizeof(unsigned long long)));
-    DG_TRY(hipMalloc(&c->dt_bits, sizeof(unsigned long long)));
+    DG_TRY(hipMalloc(&c->dt_value, sizeof(double)));
+    DG_TRY(hipMalloc(&c->sub_count, sizeof(unsigned int)));
     DG_TRY(hipMalloc(&c->flag, sizeof(int)));
     DG_TRY(hipMemset(c->clips, 0, sizeof(unsigned long long)));
     DG_TRY(hipEventCreate(&c->ev[0]));
@@ -271,7 +298,8 @@ int dg_destroy(dg_ctx* c) {
     DG_TRY(hipStreamSynchronize(c->stream));
     if (int r = free_statics(c)) return r;
     DG_TRY(hipFree(c->clips));
-    DG_TRY(hipFree(c->dt_bits));
+    DG_TRY(hipFree(c->dt_value));
+    DG_TRY(hipFree(c->sub_count));
     DG_TRY(hipFree(c->flag));
     DG_TRY(hipEventDestroy(c->ev[0]));
     DG_TRY(hipEventDestroy(c->ev[1]));
@@ -326,6 +354,14 @@ int dg_upload_form(dg_ctx* c, const dg_form* f) {
     DG_TRY(hipMalloc(&c->links, 4 * K * sizeof(uint32_t)));
     DG_TRY(hipMalloc(&c->tags, 4 * K));
     DG_TRY(hipMalloc(&c->affine, K));
+    {
+        const int64_t epb = 64 / nn, nblk = (K + epb - 1) / epb;
+        DG_TRY(hipMalloc(&c->queue, nblk * epb * sizeof(uint32_t)));
+        DG_TRY(hipMalloc(&c->queue_count, nblk));
+        DG_TRY(hipMalloc(&c->block_min, nblk * sizeof(double)));
+        DG_TRY(hipMalloc(&c->sub_list, K * sizeof(uint32_t)));
+        DG_TRY(hipMalloc(&c->chunk_total, ((nblk + 1023) / 1024) * sizeof(unsigned int)));
+    }
     DG_TRY(hipMemcpy(c->corners, f->corners, 8 * K * sizeof(double), hipMemcpyHostToDevice));
     DG_TRY(hipMemcpy(c->bed, f->bed, (size_t)K * nn * sizeof(double), hipMemcpyHostToDevice));
     DG_TRY(hipMemcpy(c->manning, man.data(), K * sizeof(double), hipMemcpyHostToDevice));

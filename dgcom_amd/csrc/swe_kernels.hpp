@@ -73,54 +73,84 @@ struct StageParams {
     int flags;
     unsigned long long* clips;
     double* fstar_out;   // optional [K][4][N+1][3]: F* at every face node, outward (this element's side)
+    uint32_t* queue;     // [blocks][EPB]: the block's elements that take the subcell volume term
+    uint8_t* queue_count;  // [blocks]: how many
+    unsigned int xcd_stride;  // workgroups per XCD share of the wet kernel's grid (0: no remap)
 };
 
+// LDS exchange among the lanes of a workgroup that is one wavefront: the
+// wavefront's LDS operations complete in issue order, so only the compiler's
+// ordering needs fencing (no s_barrier, and no wait on outstanding global loads).
+__device__ __forceinline__ void lds_sync() {
+#ifdef DGSWE_EMULATE
+    __syncthreads();
+#else
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#endif
+}
+
+// Workgroup b of a grid of 8*stride runs on XCD b % 8; give each XCD a contiguous
+// eighth of the element blocks, so that neighbour gathers hit its own L2.
+__device__ __forceinline__ unsigned int xcd_block(unsigned int b, unsigned int stride) {
+    return stride ? (b & 7u) * stride + (b >> 3) : b;
+}
+
 // One line's flux-differencing volume term with the bed slope, for the node at
-// position p (swe_2d_split_form.rs:843-894): the pair terms where an earlier
-// node of the line is `a` and this node `c`, then this node's own sum.
+// position p (swe_2d_split_form.rs:843-894). Every pair flux of the node is
+// formed first (independent, so they overlap), then they are summed in the
+// reference's order: the pairs whose first node is an earlier one (q < p)
+// straight into the node, then the node's own sum from the diagonal on. The
+// two-point flux and the metric average are bitwise symmetric in the pair.
 template <int N>
 __device__ __forceinline__ void volume_line(Cons& rhs, int p, const int* idx, const double* sh, const double* shu,
                                             const double* shv, const double* su, const double* sv, const double* sb,
                                             const double* smx, const double* smy, bool affine, double g) {
     constexpr int N1 = N + 1;
     const GllTables& T = c_gll[N];
-    Node me = {sh[idx[p]], shu[idx[p]], shv[idx[p]], su[idx[p]], sv[idx[p]], sb[idx[p]]};
-    double mpx = smx[idx[p]], mpy = smy[idx[p]];
+    const int ip = idx[p];
+    const Node me = {sh[ip], shu[ip], shv[ip], su[ip], sv[ip], sb[ip]};
+    const double mpx = smx[ip], mpy = smy[ip];
+    double d[N1];
+#pragma unroll
+    for (int q = 0; q < N1; ++q) d[q] = T.d1[p * N1 + q];
+    Cons f[N1];
+    double bq[N1], mqx[N1], mqy[N1];
 #pragma unroll
     for (int q = 0; q < N1; ++q) {
-        if (q >= p) break;
-        int iq = idx[q];
-        Node o = {sh[iq], shu[iq], shv[iq], su[iq], sv[iq], sb[iq]};
-        double mx = affine ? mpx : 0.5 * (smx[iq] + mpx);
-        double my = affine ? mpy : 0.5 * (smy[iq] + mpy);
-        Cons f = ec_flux(o, me, mx, my, g);
-        rhs = csub(rhs, cscale(2.0 * T.d1[p * N1 + q], f));
+        const int iq = idx[q];
+        const Node o = {sh[iq], shu[iq], shv[iq], su[iq], sv[iq], sb[iq]};
+        mqx[q] = smx[iq];
+        mqy[q] = smy[iq];
+        bq[q] = o.b;
+        const bool own = q == p;
+        const double mx = (own || affine) ? mpx : 0.5 * (mqx[q] + mpx);
+        const double my = (own || affine) ? mpy : 0.5 * (mqy[q] + mpy);
+        f[q] = ec_flux(o, me, mx, my, g);
     }
-    Cons acc = cscale(2.0 * T.d1[p * N1 + p], ec_flux(me, me, mpx, mpy, g));
+    Cons acc = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < N1; ++q) {
-        if (q <= p) continue;
-        int iq = idx[q];
-        Node o = {sh[iq], shu[iq], shv[iq], su[iq], sv[iq], sb[iq]};
-        double mx = affine ? mpx : 0.5 * (mpx + smx[iq]);
-        double my = affine ? mpy : 0.5 * (mpy + smy[iq]);
-        Cons f = ec_flux(me, o, mx, my, g);
-        acc = cadd(acc, cscale(2.0 * T.d1[p * N1 + q], f));
+        const Cons t = cscale(2.0 * d[q], f[q]);
+        if (q < p) rhs = csub(rhs, t);
+        if (q == p) acc = t;
     }
+#pragma unroll
+    for (int q = 0; q < N1; ++q)
+        if (q > p) acc = cadd(acc, cscale(2.0 * d[q], f[q]));
     double db = 0.0, dbx = 0.0, dby = 0.0;
 #pragma unroll
     for (int q = 0; q < N1; ++q) {
-        double d_b = T.d1[p * N1 + q] * sb[idx[q]];
+        double d_b = d[q] * bq[q];
         db += d_b;
-        if (!affine) {
-            dbx += d_b * smx[idx[q]];
-            dby += d_b * smy[idx[q]];
-        }
+        dbx += d_b * mqx[q];
+        dby += d_b * mqy[q];
     }
     double bx, by;
     if (affine) { bx = db * mpx; by = db * mpy; }
     else { bx = 0.5 * (db * mpx + dbx); by = 0.5 * (db * mpy + dby); }
-    double force = g * me.h;
+    const double force = g * me.h;
     acc = cadd(acc, Cons{0.0, force * bx, force * by});
     rhs = csub(rhs, acc);
 }
@@ -223,161 +253,202 @@ __device__ __forceinline__ bool outer_node(const StageParams& p, int64_t k, cons
     return true;
 }
 
-template <int N>
-__global__ __launch_bounds__(64) void stage_kernel(StageParams p) {
+// One workgroup-wavefront of EPB elements through a fused stage: the element
+// RHS (volume term, Riemann fluxes at the 4(N+1) face nodes from the left side
+// of each edge, LIFT, 1/J, Coriolis), then out = a*base + b*input + c*RHS, the
+// point-implicit damping and the wet/dry correction (builder.rs:652-737).
+// SUB: the subcell volume term (elements with a node below h_dry); otherwise
+// the flux-differencing one, and an element with a shallow node is handed to
+// the subcell queue instead.
+template <int N, bool SUB>
+__device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, bool active) {
     constexpr int N1 = N + 1, NN = N1 * N1, EPB = 64 / NN, NF = 4 * N1;
+    constexpr int NT = (NF + NN - 1) / NN;  // face tasks per lane
     const GllTables& T = c_gll[N];
     __shared__ double sh[64], shu[64], shv[64], su[64], sv[64], sb[64];
     __shared__ double srx[64], sry[64], ssx[64], ssy[64];
     __shared__ Cons sfd[EPB * NF];
     __shared__ double ssj[EPB * 4];
-    __shared__ Side rl[2][64], rr[2][64];
-    __shared__ Cons ifa[2][64], ifb[2][64];
     __shared__ double pw[64];
     __shared__ int sflag[EPB], wflag[EPB];
     __shared__ double savg[EPB * 4];
+    __shared__ Side rl[2][SUB ? 64 : 1], rr[2][SUB ? 64 : 1];
+    __shared__ Cons ifa[2][SUB ? 64 : 1], ifb[2][SUB ? 64 : 1];
 
     const int t = threadIdx.x;
     const int e = t / NN;
     const int i = t - e * NN;
-    const int64_t k = (int64_t)blockIdx.x * EPB + e;
-    const bool active = (e < EPB) && (k < p.n_elements);
     const int a = i % N1, j = i / N1;
     const size_t n = (size_t)k * NN + i;
-    const int eb = e * NN;  // first lane of this element
+    const int eb = e * NN;
 
+    // Loads first: the face tasks' links, this node, the corners, then the
+    // neighbours' face nodes, so that their latency overlaps the volume term.
+    uint32_t lk[NT];
+    uint8_t tg[NT];
+    double nh[NT], nhu[NT], nhv[NT], nbd[NT];
     double c[8];
-    bool affine = false;
-    double hin = 0.0, huin = 0.0, hvin = 0.0;
-    double det = 1.0, det_inv = 1.0;
+    double hin = 0.0, huin = 0.0, hvin = 0.0, bin = 0.0;
     if (active) {
 #pragma unroll
+        for (int r = 0; r < NT; ++r) {
+            int task = i + r * NN;
+            lk[r] = LINK_BOUNDARY;
+            tg[r] = 0;
+            if (task < NF) {
+                lk[r] = p.links[4 * k + task / N1];
+                tg[r] = p.tags[4 * k + task / N1];
+            }
+        }
+        hin = p.ih[n]; huin = p.ihu[n]; hvin = p.ihv[n]; bin = p.bed[n];
+#pragma unroll
         for (int q = 0; q < 8; ++q) c[q] = p.corners[8 * k + q];
-        affine = p.affine[k] != 0;
-        hin = p.ih[n]; huin = p.ihu[n]; hvin = p.ihv[n];
-        double b = p.bed[n];
-        Node q = make_node(hin, huin, hvin, b, p.h_min);
-        sh[t] = q.h; shu[t] = q.hu; shv[t] = q.hv; su[t] = q.u; sv[t] = q.v; sb[t] = q.b;
-        Metric m = bilinear(c, T.x1[a], T.x1[j]);
-        det = m.det;
-        if (affine) {
-            Metric m0 = bilinear(c, T.x1[0], T.x1[0]);
-            srx[t] = m0.det * (m0.y_s / m0.det);
-            sry[t] = m0.det * (-m0.x_s / m0.det);
-            ssx[t] = m0.det * (-m0.y_r / m0.det);
-            ssy[t] = m0.det * (m0.x_r / m0.det);
-            det_inv = 1.0 / m0.det;
-        } else {
-            srx[t] = m.det * (m.y_s / m.det);
-            sry[t] = m.det * (-m.x_s / m.det);
-            ssx[t] = m.det * (-m.y_r / m.det);
-            ssy[t] = m.det * (m.x_r / m.det);
-            det_inv = 1.0 / m.det;
+#pragma unroll
+        for (int r = 0; r < NT; ++r) {
+            int task = i + r * NN;
+            nh[r] = nhu[r] = nhv[r] = nbd[r] = 0.0;
+            if (task < NF && lk[r] != LINK_BOUNDARY) {
+                int fi = task - (task / N1) * N1;
+                uint32_t l = lk[r];
+                size_t m = (size_t)(l >> 4) * NN + face_node<N>((int)(l & 3u), N1 - 1 - fi);
+                nh[r] = p.ih[m]; nhu[r] = p.ihu[m]; nhv[r] = p.ihv[m]; nbd[r] = p.bed[m];
+            }
         }
     }
     if (t < EPB) { sflag[t] = 0; wflag[t] = 0; }
-    __syncthreads();
+    lds_sync();
     const bool subcells_on = (p.flags & DG_SUBCELLS) != 0;
-    if (active && subcells_on && hin < p.h_dry) sflag[e] = 1;
-    __syncthreads();
-    const bool sub = active && sflag[e] != 0;
+    if (!SUB && active && subcells_on && hin < p.h_dry) sflag[e] = 1;
+    lds_sync();
+    if (!SUB) {
+        // Elements with a shallow node take the subcell volume term: listed per
+        // block (in element order) for the subcell kernel
+        if (t == 0) {
+            int cnt = 0;
+            for (int q = 0; q < EPB; ++q) {
+                const int64_t kq = k - e + q;
+                if (kq < p.n_elements && sflag[q] != 0) p.queue[(size_t)(k / EPB) * EPB + cnt++] = (uint32_t)kq;
+            }
+            p.queue_count[k / EPB] = (uint8_t)cnt;
+        }
+        if (active && sflag[e] != 0) active = false;
+    }
 
-    // Line node indices (lane numbers): r-line j and s-line a
+    bool affine = false;
+    double det = 1.0, det_inv = 1.0;
+    if (active) {
+        affine = p.affine[k] != 0;
+        Node q = make_node(hin, huin, hvin, bin, p.h_min);
+        sh[t] = q.h; shu[t] = q.hu; shv[t] = q.hv; su[t] = q.u; sv[t] = q.v; sb[t] = q.b;
+        Metric m = bilinear(c, T.x1[a], T.x1[j]);
+        det = m.det;
+        Metric mm = affine ? bilinear(c, T.x1[0], T.x1[0]) : m;
+        srx[t] = mm.det * (mm.y_s / mm.det);
+        sry[t] = mm.det * (-mm.x_s / mm.det);
+        ssx[t] = mm.det * (-mm.y_r / mm.det);
+        ssy[t] = mm.det * (mm.x_r / mm.det);
+        det_inv = 1.0 / mm.det;
+    }
+    lds_sync();
+
     int ridx[N1], sidx[N1];
 #pragma unroll
     for (int q = 0; q < N1; ++q) { ridx[q] = eb + j * N1 + q; sidx[q] = eb + q * N1 + a; }
 
-    // Subcells: reconstructed face states of this node's subcell on both lines
-    if (sub) {
+    if (SUB) {
+        // Reconstructed face states of this node's subcell on its r- and s-line
+        if (active) {
 #pragma unroll
-        for (int dir = 0; dir < 2; ++dir) {
-            const int* idx = dir == 0 ? ridx : sidx;
-            int pos = dir == 0 ? a : j;
-            Side me = {sh[t], su[t], sv[t], sb[t]};
-            Side lft, rgt;
-            double xlf = 0.0, xrt = 0.0;
-            bool has_l = true, has_r = true;
-            if (pos > 0) { int q = idx[pos - 1]; lft = {sh[q], su[q], sv[q], sb[q]}; xlf = T.x1[pos - 1]; }
-            else {
-                // r-line start: face 3 at fi = N1-1-j; s-line start: face 0 at fi = a
-                double d;
-                has_l = dir == 0 ? outer_node<N>(p, k, c, affine, 3, N1 - 1 - j, lft, d)
-                                 : outer_node<N>(p, k, c, affine, 0, a, lft, d);
-                xlf = -1.0 - d;
+            for (int dir = 0; dir < 2; ++dir) {
+                const int* idx = dir == 0 ? ridx : sidx;
+                int pos = dir == 0 ? a : j;
+                Side me = {sh[t], su[t], sv[t], sb[t]};
+                Side lft, rgt;
+                double xlf = 0.0, xrt = 0.0;
+                bool has_l = true, has_r = true;
+                if (pos > 0) { int q = idx[pos - 1]; lft = {sh[q], su[q], sv[q], sb[q]}; xlf = T.x1[pos - 1]; }
+                else {
+                    double d;
+                    has_l = dir == 0 ? outer_node<N>(p, k, c, affine, 3, N1 - 1 - j, lft, d)
+                                     : outer_node<N>(p, k, c, affine, 0, a, lft, d);
+                    xlf = -1.0 - d;
+                }
+                if (pos < N1 - 1) { int q = idx[pos + 1]; rgt = {sh[q], su[q], sv[q], sb[q]}; xrt = T.x1[pos + 1]; }
+                else {
+                    double d;
+                    has_r = dir == 0 ? outer_node<N>(p, k, c, affine, 1, j, rgt, d)
+                                     : outer_node<N>(p, k, c, affine, 2, N1 - 1 - a, rgt, d);
+                    xrt = 1.0 + d;
+                }
+                if (has_l && has_r)
+                    reconstruct(lft, me, rgt, xlf, T.x1[pos], xrt, T.xl[pos], T.xr[pos], p.h_dry, rl[dir][t], rr[dir][t]);
+                else { rl[dir][t] = me; rr[dir][t] = me; }
             }
-            if (pos < N1 - 1) { int q = idx[pos + 1]; rgt = {sh[q], su[q], sv[q], sb[q]}; xrt = T.x1[pos + 1]; }
-            else {
-                double d;
-                has_r = dir == 0 ? outer_node<N>(p, k, c, affine, 1, j, rgt, d)
-                                 : outer_node<N>(p, k, c, affine, 2, N1 - 1 - a, rgt, d);
-                xrt = 1.0 + d;
-            }
-            if (has_l && has_r) reconstruct(lft, me, rgt, xlf, T.x1[pos], xrt, T.xl[pos], T.xr[pos], p.h_dry, rl[dir][t], rr[dir][t]);
-            else { rl[dir][t] = me; rr[dir][t] = me; }
         }
-    }
-    __syncthreads();
-    if (sub) {
+        lds_sync();
+        if (active) {
 #pragma unroll
-        for (int dir = 0; dir < 2; ++dir) {
-            const int* idx = dir == 0 ? ridx : sidx;
-            int pos = dir == 0 ? a : j;
-            if (pos < N1 - 1) {
-                double imx[N1 + 1], imy[N1 + 1];
-                interfaces<N>(idx, dir == 0 ? srx : ssx, dir == 0 ? sry : ssy, affine, imx, imy);
-                hydro_hll(rr[dir][t], rl[dir][idx[pos + 1]], imx[pos + 1], imy[pos + 1], p.g, p.h_min, ifa[dir][t], ifb[dir][t]);
+            for (int dir = 0; dir < 2; ++dir) {
+                const int* idx = dir == 0 ? ridx : sidx;
+                int pos = dir == 0 ? a : j;
+                if (pos < N1 - 1) {
+                    double imx[N1 + 1], imy[N1 + 1];
+                    interfaces<N>(idx, dir == 0 ? srx : ssx, dir == 0 ? sry : ssy, affine, imx, imy);
+                    hydro_hll(rr[dir][t], rl[dir][idx[pos + 1]], imx[pos + 1], imy[pos + 1], p.g, p.h_min,
+                              ifa[dir][t], ifb[dir][t]);
+                }
             }
         }
     }
 
-    // Faces: the 4(N+1) face nodes of the element spread over its lanes
+    // Faces: F* at the element's face nodes, spread over its lanes
     if (active) {
-        for (int task = i; task < NF; task += NN) {
-            int f = task / N1, fi = task - f * N1;
-            int nd = face_node<N>(f, fi);
-            int li = eb + nd;
+#pragma unroll
+        for (int r = 0; r < NT; ++r) {
+            const int task = i + r * NN;
+            if (task >= NF) break;
+            const int f = task / N1, fi = task - f * N1;
+            const int li = eb + face_node<N>(f, fi);
             Node q = {sh[li], shu[li], shv[li], su[li], sv[li], sb[li]};
             double vx, vy;
             face_vector<N>(c, f, vx, vy);
             double sj = sqrt(vx * vx + vy * vy);
             double nx = vx / sj, ny = vy / sj;
-            uint32_t l = p.links[4 * k + f];
+            const uint32_t l = lk[r];
             Cons fstar;
-            if (l == LINK_BOUNDARY) {
-                if (p.tags[4 * k + f] == DG_BC_TIDE) {
-                    Cons qb = characteristic(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min);
-                    Node nb = make_node(qb.h, qb.hu, qb.hv, q.b, p.h_min);
-                    fstar = ec_flux(nb, nb, nx, ny, p.g);
-                } else {
-                    Cons gs = wall_ghost(q.h, q.hu, q.hv, nx, ny, p.h_min);
-                    Node qe = make_node(gs.h, gs.hu, gs.hv, q.b, p.h_min);
-                    Cons fb;
-                    hydro_hll(side(q), side(qe), nx, ny, p.g, p.h_min, fstar, fb);
-                }
+            if (l == LINK_BOUNDARY && tg[r] == DG_BC_TIDE) {
+                Cons qb = characteristic(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min);
+                Node nb = make_node(qb.h, qb.hu, qb.hv, q.b, p.h_min);
+                fstar = ec_flux(nb, nb, nx, ny, p.g);
             } else {
-                int64_t nbk = (int64_t)(l >> 4);
-                int nbf = (int)(l & 3u);
-                size_t m = (size_t)nbk * NN + face_node<N>(nbf, N1 - 1 - fi);
-                Node qn = make_node(p.ih[m], p.ihu[m], p.ihv[m], p.bed[m], p.h_min);
-                Cons fa, fb;
-                if (l & 4u) {
-                    hydro_hll(side(q), side(qn), nx, ny, p.g, p.h_min, fa, fb);
-                    fstar = fa;
-                } else {
-                    double lx, ly;
-                    if (l & 8u) { lx = -nx; ly = -ny; }
-                    else {
+                // One Riemann solve per face node, as the left side of the edge
+                // sees it: an interior face's left side (this element, or the
+                // neighbour with this face's normal reversed), or a wall's
+                // reflected ghost on the right
+                const bool wall = l == LINK_BOUNDARY;
+                const bool left = wall || (l & 4u);
+                Cons gs = wall_ghost(q.h, q.hu, q.hv, nx, ny, p.h_min);
+                Node qo = wall ? make_node(gs.h, gs.hu, gs.hv, q.b, p.h_min)
+                               : make_node(nh[r], nhu[r], nhv[r], nbd[r], p.h_min);
+                double lx = nx, ly = ny;
+                if (!left) {
+                    lx = -nx; ly = -ny;
+                    if (!(l & 8u)) {
+                        // a periodic pair: the left side's own normal
+                        int64_t nbk = (int64_t)(l >> 4);
                         double cn[8];
 #pragma unroll
-                        for (int r = 0; r < 8; ++r) cn[r] = p.corners[8 * nbk + r];
+                        for (int q2 = 0; q2 < 8; ++q2) cn[q2] = p.corners[8 * nbk + q2];
                         double wx, wy;
-                        face_vector<N>(cn, nbf, wx, wy);
+                        face_vector<N>(cn, (int)(l & 3u), wx, wy);
                         double s2 = sqrt(wx * wx + wy * wy);
                         lx = wx / s2; ly = wy / s2;
                     }
-                    hydro_hll(side(qn), side(q), lx, ly, p.g, p.h_min, fa, fb);
-                    fstar = cneg(fb);
                 }
+                const Side sa = left ? side(q) : side(qo), sbb = left ? side(qo) : side(q);
+                Cons fa, fb;
+                hydro_hll(sa, sbb, lx, ly, p.g, p.h_min, fa, fb);
+                fstar = left ? fa : cneg(fb);
             }
             if (p.fstar_out) {
                 double* o = p.fstar_out + ((size_t)k * NF + task) * 3;
@@ -387,7 +458,7 @@ __global__ __launch_bounds__(64) void stage_kernel(StageParams p) {
             if (fi == 0) ssj[e * 4 + f] = sj;
         }
     }
-    __syncthreads();
+    lds_sync();
 
     Cons rhs = {0.0, 0.0, 0.0};
     if (active) {
@@ -395,7 +466,7 @@ __global__ __launch_bounds__(64) void stage_kernel(StageParams p) {
 #pragma unroll
         for (int pass = 0; pass < 2; ++pass) {
             bool rline = (pass == 0) == r_first;
-            if (sub) {
+            if (SUB) {
                 if (rline) subcell_line<N>(rhs, a, ridx, sh, shu, shv, su, sv, sb, srx, sry, affine, p.g, rl[0], rr[0], ifa[0], ifb[0]);
                 else subcell_line<N>(rhs, j, sidx, sh, shu, shv, su, sv, sb, ssx, ssy, affine, p.g, rl[1], rr[1], ifa[1], ifb[1]);
             } else {
@@ -422,7 +493,6 @@ __global__ __launch_bounds__(64) void stage_kernel(StageParams p) {
     // Shu-Osher combination and point-implicit damping (builder.rs:668-690; wetting_drying.rs:324-363)
     double vh = 0.0, vhu = 0.0, vhv = 0.0;
     const bool friction = (p.flags & DG_FRICTION) != 0, wet_dry = (p.flags & DG_WET_DRY) != 0;
-    if (wet_dry) __syncthreads();  // the node states in LDS are reused below
     if (active) {
         if (p.a != 0.0) { vh = p.a * p.bh[n]; vhu = p.a * p.bhu[n]; vhv = p.a * p.bhv[n]; }
         if (p.b != 0.0) { vh += p.b * hin; vhu += p.b * huin; vhv += p.b * hvin; }
@@ -452,15 +522,16 @@ __global__ __launch_bounds__(64) void stage_kernel(StageParams p) {
                 vhv *= factor;
             }
         }
-        if (wet_dry) {
+    }
+    // Wet/dry correction (wetting_drying.rs:194-213; limiters/swe_2d.rs:104-197)
+    if (wet_dry) {
+        lds_sync();  // the node states in LDS are reused here
+        if (active) {
             pw[t] = T.w2[i] * det;
             sh[t] = vh; shu[t] = vhu; shv[t] = vhv;
             if (vh < p.h_dry) wflag[e] = 1;
         }
-    }
-    // Wet/dry correction (wetting_drying.rs:194-213; limiters/swe_2d.rs:104-197)
-    if (wet_dry) {
-        __syncthreads();
+        lds_sync();
         const bool needs = active && wflag[e] != 0;
         if (needs && i == 0) {
             double area = 0.0;
@@ -479,7 +550,7 @@ __global__ __launch_bounds__(64) void stage_kernel(StageParams p) {
             savg[e * 4 + 2] = s2 * inv_total;
             savg[e * 4 + 3] = hmin;
         }
-        __syncthreads();
+        lds_sync();
         if (needs) {
             double ah = savg[e * 4 + 0], ahu = savg[e * 4 + 1], ahv = savg[e * 4 + 2], hmin = savg[e * 4 + 3];
             if (ah <= 0.0) {
@@ -501,8 +572,81 @@ __global__ __launch_bounds__(64) void stage_kernel(StageParams p) {
     if (active) { p.oh[n] = vh; p.ohu[n] = vhu; p.ohv[n] = vhv; }
 }
 
+// Elements with every node at or above h_dry, in mesh order (XCD-aware block
+// mapping); the others are listed for the subcell kernel.
+template <int N>
+__global__ __launch_bounds__(64) void wet_stage_kernel(StageParams p) {
+    constexpr int NN = (N + 1) * (N + 1), EPB = 64 / NN;
+    const int64_t blk = xcd_block(blockIdx.x, p.xcd_stride);
+    if (blk * EPB >= p.n_elements) return;
+    const int e = threadIdx.x / NN;
+    const int64_t k = blk * EPB + (e < EPB ? e : 0);
+    stage_block<N, false>(p, k, e < EPB && blk * EPB + e < p.n_elements);
+}
+
+// Compaction of the per-block subcell lists into one list in element order:
+// chunk_count_kernel sums the counts of each chunk of 1024 blocks, then
+// compact_kernel scans them and scatters the element indices.
+__global__ __launch_bounds__(256) void chunk_count_kernel(const uint8_t* counts, unsigned int n_blocks,
+                                                          unsigned int* chunk_total) {
+    __shared__ unsigned int red[256];
+    const unsigned int b0 = blockIdx.x * 1024u + threadIdx.x * 4u;
+    unsigned int s = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s += b0 + q < n_blocks ? counts[b0 + q] : 0u;
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) chunk_total[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void compact_kernel(const uint8_t* counts, const uint32_t* queue, unsigned int n_blocks,
+                                                      const unsigned int* chunk_total, int epb, uint32_t* list,
+                                                      unsigned int* list_count) {
+    __shared__ unsigned int scan[257];
+    __shared__ unsigned int base;
+    if (threadIdx.x == 0) {
+        unsigned int b = 0;
+        for (unsigned int c = 0; c < blockIdx.x; ++c) b += chunk_total[c];
+        base = b;
+    }
+    const unsigned int b0 = blockIdx.x * 1024u + threadIdx.x * 4u;
+    unsigned int cnt[4], s = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { cnt[q] = b0 + q < n_blocks ? counts[b0 + q] : 0u; s += cnt[q]; }
+    scan[threadIdx.x + 1] = s;
+    if (threadIdx.x == 0) scan[0] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 256; ++q) scan[q + 1] += scan[q];
+    __syncthreads();
+    unsigned int off = base + scan[threadIdx.x];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        for (unsigned int r = 0; r < cnt[q]; ++r) list[off++] = queue[(size_t)(b0 + q) * epb + r];
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *list_count = base + scan[256];
+}
+
+// The listed elements with a shallow node (subcell volume term), EPB per
+// workgroup pass, grid-stride over the compacted list.
+template <int N>
+__global__ __launch_bounds__(64) void subcell_stage_kernel(StageParams p, const uint32_t* list, const unsigned int* count_ptr) {
+    constexpr int NN = (N + 1) * (N + 1), EPB = 64 / NN;
+    const unsigned int count = *count_ptr;
+    const int e = threadIdx.x / NN;
+    for (size_t first = (size_t)blockIdx.x * EPB; first < count; first += (size_t)gridDim.x * EPB) {
+        const bool active = e < EPB && first + e < count;
+        stage_block<N, true>(p, active ? (int64_t)list[first + e] : 0, active);
+        lds_sync();
+    }
+}
+
 // ---------------------------------------------------------------------------
-// Time step: per element bound and the global minimum (swe_2d.rs:2072-2258)
+// Time step: per element bound and the global minimum (swe_2d.rs:2072-2258),
+// one lane per node
 // ---------------------------------------------------------------------------
 struct DtParams {
     const double *h, *hu, *hv;
@@ -514,7 +658,8 @@ struct DtParams {
     double pcfl, max_cfl, max_cfl_sub;
     int positivity;
     double* element_dt;                 // optional
-    unsigned long long* dt_min_bits;    // global minimum as ordered bits (dt >= 0)
+    double* block_min;                  // [blocks]: the minimum of each block's elements
+    unsigned int xcd_stride;
 };
 
 __device__ __forceinline__ double node_rate(double h, double hu, double hv, double rx, double ry, double sx, double sy,
@@ -529,83 +674,124 @@ __device__ __forceinline__ double node_rate(double h, double hu, double hv, doub
 }
 
 template <int N>
-__global__ __launch_bounds__(256) void dt_kernel(DtParams p) {
-    constexpr int N1 = N + 1, NN = N1 * N1;
+__global__ __launch_bounds__(64) void dt_kernel(DtParams p) {
+    constexpr int N1 = N + 1, NN = N1 * N1, EPB = 64 / NN, NF = 4 * N1;
+    constexpr int NT = (NF + NN - 1) / NN;
     const GllTables& T = c_gll[N];
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double dt = INFINITY;
-    if (k < p.n_elements) {
-        double c[8];
+    __shared__ double mrx[64], mry[64], msx[64], msy[64], sr[64], smass[64], swater[64], shh[64];
+    __shared__ double sdt[EPB];
+    const int64_t blk = xcd_block(blockIdx.x, p.xcd_stride);
+    if (blk * EPB >= p.n_elements) return;
+    const int t = threadIdx.x, e = t / NN, i = t - e * NN, a = i % N1, j = i / N1, eb = e * NN;
+    const int64_t k = blk * EPB + e;
+    const bool active = e < EPB && k < p.n_elements;
+    const size_t n = (size_t)k * NN + i;
+    uint32_t lk[NT];
+    double nh[NT], nhu[NT], nhv[NT];
+    double h = 0.0, hu = 0.0, hv = 0.0;
+    double c[8];
+    if (active) {
+#pragma unroll
+        for (int r = 0; r < NT; ++r) {
+            int task = i + r * NN;
+            lk[r] = task < NF ? p.links[4 * k + task / N1] : LINK_BOUNDARY;
+        }
+        h = p.h[n]; hu = p.hu[n]; hv = p.hv[n];
 #pragma unroll
         for (int q = 0; q < 8; ++q) c[q] = p.corners[8 * k + q];
+#pragma unroll
+        for (int r = 0; r < NT; ++r) {
+            int task = i + r * NN;
+            nh[r] = nhu[r] = nhv[r] = 0.0;
+            if (task < NF && lk[r] != LINK_BOUNDARY) {
+                int fi = task - (task / N1) * N1;
+                size_t m = (size_t)(lk[r] >> 4) * NN + face_node<N>((int)(lk[r] & 3u), N1 - 1 - fi);
+                nh[r] = p.h[m]; nhu[r] = p.hu[m]; nhv[r] = p.hv[m];
+            }
+        }
+    }
+    double rate = 0.0;
+    if (active) {
         bool affine = p.affine[k] != 0;
-        Metric m0 = bilinear(c, T.x1[0], T.x1[0]);
-        double rx0 = m0.y_s / m0.det, ry0 = -m0.x_s / m0.det, sx0 = -m0.y_r / m0.det, sy0 = m0.x_r / m0.det;
-        double rate = 0.0;
-        double water = 0.0, water_b = 0.0, weight = 0.0, weight_b = 0.0;
-        bool shallow = false;
-        const double shallow_depth = 100.0 * p.h_dry;
-#pragma unroll
-        for (int i = 0; i < NN; ++i) {
-            size_t n = (size_t)k * NN + i;
-            double h = p.h[n];
-            int a = i % N1, j = i / N1;
-            Metric m = bilinear(c, T.x1[a], T.x1[j]);
-            double r = 0.0;
-            if (h > p.h_min) {
-                if (affine) r = node_rate(h, p.hu[n], p.hv[n], rx0, ry0, sx0, sy0, p.g, p.h_min);
-                else r = node_rate(h, p.hu[n], p.hv[n], m.y_s / m.det, -m.x_s / m.det, -m.y_r / m.det, m.x_r / m.det, p.g, p.h_min);
-            }
-            rate = fmax_(rate, r);
-            double mass = T.w2[i] * m.det;
-            shallow = shallow || (h <= shallow_depth);
-            double wi = mass * fmax_(h, 0.0);
-            water += wi;
-            weight += mass;
-            if (a == 0 || a == N1 - 1 || j == 0 || j == N1 - 1) { water_b += wi; weight_b += mass; }
+        Metric m = bilinear(c, T.x1[a], T.x1[j]);
+        double rx = m.y_s / m.det, ry = -m.x_s / m.det, sx = -m.y_r / m.det, sy = m.x_r / m.det;
+        mrx[t] = rx; mry[t] = ry; msx[t] = sx; msy[t] = sy;
+        if (h > p.h_min) {
+            if (affine) {
+                Metric m0 = bilinear(c, T.x1[0], T.x1[0]);
+                rate = node_rate(h, hu, hv, m0.y_s / m0.det, -m0.x_s / m0.det, -m0.y_r / m0.det, m0.x_r / m0.det, p.g, p.h_min);
+            } else rate = node_rate(h, hu, hv, rx, ry, sx, sy, p.g, p.h_min);
         }
+        smass[t] = T.w2[i] * m.det;
+        swater[t] = smass[t] * fmax_(h, 0.0);
+        shh[t] = h;
+    }
+    lds_sync();
+    if (active) {
 #pragma unroll
-        for (int f = 0; f < 4; ++f) {
-            uint32_t l = p.links[4 * k + f];
-            if (l == LINK_BOUNDARY) continue;
-            int64_t nb = (int64_t)(l >> 4);
-            int nbf = (int)(l & 3u);
-#pragma unroll
-            for (int fi = 0; fi < N1; ++fi) {
-                size_t m = (size_t)nb * NN + face_node<N>(nbf, N1 - 1 - fi);
-                double h = p.h[m];
-                if (h <= p.h_min) continue;
-                int i = face_node<N>(f, fi);
-                Metric mm = bilinear(c, T.x1[i % N1], T.x1[i / N1]);
-                rate = fmax_(rate, node_rate(h, p.hu[m], p.hv[m], mm.y_s / mm.det, -mm.x_s / mm.det, -mm.y_r / mm.det,
-                                             mm.x_r / mm.det, p.g, p.h_min));
-            }
+        for (int r = 0; r < NT; ++r) {
+            int task = i + r * NN;
+            if (task >= NF || lk[r] == LINK_BOUNDARY || nh[r] <= p.h_min) continue;
+            int f = task / N1, fi = task - f * N1;
+            int li = eb + face_node<N>(f, fi);
+            rate = fmax_(rate, node_rate(nh[r], nhu[r], nhv[r], mrx[li], mry[li], msx[li], msy[li], p.g, p.h_min));
         }
+        sr[t] = rate;
+    }
+    lds_sync();
+    if (active && i == 0) {
+        double rmax = 0.0;
+        for (int q = 0; q < NN; ++q) rmax = fmax_(rmax, sr[eb + q]);
         double cfl = p.cfl;
         if (p.positivity) {
+            double water = 0.0, water_b = 0.0, weight = 0.0, weight_b = 0.0;
+            bool shallow = false;
+            const double shallow_depth = 100.0 * p.h_dry;
+            for (int q = 0; q < NN; ++q) {
+                int qa = q % N1, qj = q / N1;
+                double mass = smass[eb + q], wi = swater[eb + q];
+                shallow = shallow || (shh[eb + q] <= shallow_depth);
+                water += wi;
+                weight += mass;
+                if (qa == 0 || qa == N1 - 1 || qj == 0 || qj == N1 - 1) { water_b += wi; weight_b += mass; }
+            }
             double ratio = fmin_(water / water_b, weight / weight_b);
             double rho = fmax_(0.9 * ratio, 1.0);
             double cap = shallow ? p.max_cfl_sub : p.max_cfl;
             cfl = fmin_(cfl, fmin_(p.pcfl * rho, cap));
         }
-        dt = rate < 1e-14 ? INFINITY : cfl / ((2.0 * N + 1.0) * rate);
+        double dt = rmax < 1e-14 ? INFINITY : cfl / ((2.0 * N + 1.0) * rmax);
         if (p.element_dt) p.element_dt[k] = dt;
+        sdt[e] = dt;
     }
-    // block minimum, then one atomic per block
-    __shared__ double red[256];
-    red[threadIdx.x] = dt;
+    if (!active && i == 0 && e < EPB) sdt[e] = INFINITY;
+    lds_sync();
+    if (t == 0) {
+        double m = INFINITY;
+        for (int q = 0; q < EPB; ++q) m = fmin_(m, sdt[q]);
+        p.block_min[blk] = m;
+    }
+}
+
+// The minimum of n values (one workgroup of 1024 lanes), into *out
+__global__ __launch_bounds__(1024) void min_kernel(const double* v, size_t n, double* out) {
+    __shared__ double red[1024];
+    double m = INFINITY;
+    for (size_t q = threadIdx.x; q < n; q += 1024) m = fmin_(m, v[q]);
+    red[threadIdx.x] = m;
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    for (int s = 512; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) red[threadIdx.x] = fmin_(red[threadIdx.x], red[threadIdx.x + s]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicMin(p.dt_min_bits, (unsigned long long)__double_as_longlong(red[0]));
+    if (threadIdx.x == 0) *out = red[0];
 }
 
 // ---------------------------------------------------------------------------
 // Utilities
 // ---------------------------------------------------------------------------
 __global__ void fill_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+__global__ void fill_u32(unsigned int* p, unsigned int v) { *p = v; }
 
 __global__ void finite_kernel(const double* x, size_t n, int* bad) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

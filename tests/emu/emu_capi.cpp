@@ -1,6 +1,8 @@
 // C ABI over the emulated product kernels (tests only).
 #include "hip_emu.hpp"
 
+#include <algorithm>
+
 #include "../../dgcom_amd/csrc/swe_kernels.hpp"
 
 using namespace dgswe;
@@ -27,11 +29,27 @@ unsigned long long emu_stage(int order, int64_t K, const double* corners, const 
     p.g = g; p.h_min = h_min; p.h_dry = h_dry; p.u_max = u_max; p.tau = tau; p.f0 = f0;
     p.rhs_only = rhs_only; p.flags = flags; p.clips = &clips; p.fstar_out = fstar_out;
     int epb = 64 / (int)nn;
-    unsigned grid = (unsigned)((K + epb - 1) / epb);
+    unsigned nblk = (unsigned)((K + epb - 1) / epb);
+    unsigned nchunk = (nblk + 1023) / 1024;
+    std::vector<uint32_t> queue((size_t)nblk * epb), list((size_t)K);
+    std::vector<uint8_t> qcount(nblk);
+    std::vector<unsigned int> ctot(nchunk);
+    unsigned int lcount = 0;
+    p.queue = queue.data();
+    p.queue_count = qcount.data();
+    p.xcd_stride = (nblk + 7) / 8;
+    unsigned grid = 8 * p.xcd_stride, sub_grid = std::min(nblk, 5u);
     switch (order) {
-        case 1: emu_launch(grid, 64, [&] { stage_kernel<1>(p); }); break;
-        case 2: emu_launch(grid, 64, [&] { stage_kernel<2>(p); }); break;
-        default: emu_launch(grid, 64, [&] { stage_kernel<3>(p); }); break;
+        case 1: emu_launch(grid, 64, [&] { wet_stage_kernel<1>(p); }); break;
+        case 2: emu_launch(grid, 64, [&] { wet_stage_kernel<2>(p); }); break;
+        default: emu_launch(grid, 64, [&] { wet_stage_kernel<3>(p); }); break;
+    }
+    emu_launch(nchunk, 256, [&] { chunk_count_kernel(qcount.data(), nblk, ctot.data()); });
+    emu_launch(nchunk, 256, [&] { compact_kernel(qcount.data(), queue.data(), nblk, ctot.data(), epb, list.data(), &lcount); });
+    switch (order) {
+        case 1: emu_launch(sub_grid, 64, [&] { subcell_stage_kernel<1>(p, list.data(), &lcount); }); break;
+        case 2: emu_launch(sub_grid, 64, [&] { subcell_stage_kernel<2>(p, list.data(), &lcount); }); break;
+        default: emu_launch(sub_grid, 64, [&] { subcell_stage_kernel<3>(p, list.data(), &lcount); }); break;
     }
     return clips;
 }
@@ -40,21 +58,24 @@ double emu_dt(int order, int64_t K, const double* state, const double* corners, 
               const uint8_t* affine, double cfl, double g, double h_min, double h_dry, double pcfl, double max_cfl,
               double max_cfl_sub, int positivity, double* element_dt) {
     size_t nn = (size_t)(order + 1) * (order + 1), nt = (size_t)K * nn;
-    unsigned long long bits = 0x7FF0000000000000ull;
+    int epb = 64 / (int)nn;
+    unsigned nblk = (unsigned)((K + epb - 1) / epb);
+    std::vector<double> bmin(nblk);
     DtParams p;
     p.h = state; p.hu = state + nt; p.hv = state + 2 * nt;
     p.corners = corners; p.links = links; p.affine = affine; p.n_elements = K;
     p.cfl = cfl; p.g = g; p.h_min = h_min; p.h_dry = h_dry; p.pcfl = pcfl; p.max_cfl = max_cfl;
-    p.max_cfl_sub = max_cfl_sub; p.positivity = positivity; p.element_dt = element_dt; p.dt_min_bits = &bits;
-    unsigned grid = (unsigned)((K + 255) / 256);
+    p.max_cfl_sub = max_cfl_sub; p.positivity = positivity; p.element_dt = element_dt; p.block_min = bmin.data();
+    p.xcd_stride = (nblk + 7) / 8;
+    unsigned grid = 8 * p.xcd_stride;
     switch (order) {
-        case 1: emu_launch(grid, 256, [&] { dt_kernel<1>(p); }); break;
-        case 2: emu_launch(grid, 256, [&] { dt_kernel<2>(p); }); break;
-        default: emu_launch(grid, 256, [&] { dt_kernel<3>(p); }); break;
+        case 1: emu_launch(grid, 64, [&] { dt_kernel<1>(p); }); break;
+        case 2: emu_launch(grid, 64, [&] { dt_kernel<2>(p); }); break;
+        default: emu_launch(grid, 64, [&] { dt_kernel<3>(p); }); break;
     }
-    double dt;
-    std::memcpy(&dt, &bits, 8);
-    return dt;
+    double out = 0.0;
+    emu_launch(1, 1024, [&] { min_kernel(bmin.data(), nblk, &out); });
+    return out;
 }
 
 }  // extern "C"
