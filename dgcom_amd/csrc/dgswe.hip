@@ -39,6 +39,11 @@ struct DeviceArray {
 
 }  // namespace
 
+// Workgroups of the time-step kernel (a multiple of 8: XCDs)
+constexpr unsigned int DT_GROUPS = 8 * 4096;
+// Workgroups of the subcell kernel (a grid-stride loop over its list)
+constexpr unsigned int SUB_GROUPS = 16384;
+
 struct dg_state {
     double* d = nullptr;  // 3*K*NN: h, hu, hv
 };
@@ -161,7 +166,7 @@ static int launch_stage(dg_ctx* c, const double* base, const double* input, doub
     const unsigned int nblk = (unsigned int)((c->K + epb - 1) / epb);
     const unsigned int nchunk = (nblk + 1023) / 1024;
     p.xcd_stride = (nblk + 7) / 8;
-    dim3 grid(8 * p.xcd_stride), sub_grid(std::min(nblk, 2048u));
+    dim3 grid(8 * p.xcd_stride), sub_grid(std::min(nblk, SUB_GROUPS));
     if (c->timing) DG_TRY(hipEventRecord(c->ev[0], c->stream));
     switch (c->order) {
         case 1: hipLaunchKernelGGL(wet_stage_kernel<1>, grid, dim3(64), 0, c->stream, p); break;
@@ -206,17 +211,16 @@ static int launch_dt(dg_ctx* c, const double* state, double cfl, int scheme, boo
     p.positivity = (c->flags & DG_WET_DRY) ? 1 : 0;
     p.element_dt = element_out ? c->element_dt : nullptr;
     p.block_min = c->block_min;
-    const int epb = 64 / c->nn;
-    const unsigned int nblk = (unsigned int)((c->K + epb - 1) / epb);
-    p.xcd_stride = (nblk + 7) / 8;
-    dim3 grid(8 * p.xcd_stride);
+    const unsigned int groups = DT_GROUPS;
+    p.xcd_stride = 0;
+    dim3 grid(groups);
     if (c->timing) DG_TRY(hipEventRecord(c->ev[0], c->stream));
     switch (c->order) {
         case 1: hipLaunchKernelGGL(dt_kernel<1>, grid, dim3(64), 0, c->stream, p); break;
         case 2: hipLaunchKernelGGL(dt_kernel<2>, grid, dim3(64), 0, c->stream, p); break;
         default: hipLaunchKernelGGL(dt_kernel<3>, grid, dim3(64), 0, c->stream, p); break;
     }
-    hipLaunchKernelGGL(min_kernel, dim3(1), dim3(1024), 0, c->stream, c->block_min, (size_t)nblk, c->dt_value);
+    hipLaunchKernelGGL(min_kernel, dim3(1), dim3(1024), 0, c->stream, c->block_min, (size_t)groups, c->dt_value);
     DG_TRY(hipGetLastError());
     if (c->timing) DG_TRY(hipEventRecord(c->ev[1], c->stream));
     double value = 0.0;
@@ -358,7 +362,7 @@ int dg_upload_form(dg_ctx* c, const dg_form* f) {
         const int64_t epb = 64 / nn, nblk = (K + epb - 1) / epb;
         DG_TRY(hipMalloc(&c->queue, nblk * epb * sizeof(uint32_t)));
         DG_TRY(hipMalloc(&c->queue_count, nblk));
-        DG_TRY(hipMalloc(&c->block_min, nblk * sizeof(double)));
+        DG_TRY(hipMalloc(&c->block_min, DT_GROUPS * sizeof(double)));
         DG_TRY(hipMalloc(&c->sub_list, K * sizeof(uint32_t)));
         DG_TRY(hipMalloc(&c->chunk_total, ((nblk + 1023) / 1024) * sizeof(unsigned int)));
     }

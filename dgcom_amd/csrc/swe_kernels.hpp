@@ -28,14 +28,22 @@ template <int N> __device__ __forceinline__ int face_node(int f, int fi) {
     return f == 0 ? fi : f == 1 ? fi * N1 + N1 - 1 : f == 2 ? (N1 - 1) * N1 + (N1 - 1 - fi) : (N1 - 1 - fi) * N1;
 }
 
-// Bilinear map derivatives at (r, s) (operators/geometric.rs:438-460); c = x0,y0,..,x3,y3
+// The four corners of a quadrilateral, counter-clockwise from (r, s) = (-1, -1),
+// held as named scalars (kept in registers)
+struct Quad { double x0, y0, x1, y1, x2, y2, x3, y3; };
+__device__ __forceinline__ Quad load_quad(const double* corners, int64_t k) {
+    const double* c = corners + 8 * k;
+    return {c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7]};
+}
+
+// Bilinear map derivatives at (r, s) (operators/geometric.rs:438-460)
 struct Metric { double x_r, x_s, y_r, y_s, det; };
-__device__ __forceinline__ Metric bilinear(const double* c, double r, double s) {
+__device__ __forceinline__ Metric bilinear(const Quad& c, double r, double s) {
     Metric m;
-    m.x_r = 0.25 * ((1.0 - s) * (c[2] - c[0]) + (1.0 + s) * (c[4] - c[6]));
-    m.y_r = 0.25 * ((1.0 - s) * (c[3] - c[1]) + (1.0 + s) * (c[5] - c[7]));
-    m.x_s = 0.25 * ((1.0 - r) * (c[6] - c[0]) + (1.0 + r) * (c[4] - c[2]));
-    m.y_s = 0.25 * ((1.0 - r) * (c[7] - c[1]) + (1.0 + r) * (c[5] - c[3]));
+    m.x_r = 0.25 * ((1.0 - s) * (c.x1 - c.x0) + (1.0 + s) * (c.x2 - c.x3));
+    m.y_r = 0.25 * ((1.0 - s) * (c.y1 - c.y0) + (1.0 + s) * (c.y2 - c.y3));
+    m.x_s = 0.25 * ((1.0 - r) * (c.x3 - c.x0) + (1.0 + r) * (c.x2 - c.x1));
+    m.y_s = 0.25 * ((1.0 - r) * (c.y3 - c.y0) + (1.0 + r) * (c.y2 - c.y1));
     m.det = m.x_r * m.y_s - m.x_s * m.y_r;
     return m;
 }
@@ -46,13 +54,13 @@ __device__ __forceinline__ Metric bilinear(const double* c, double r, double s) 
 // corner pair enters with a factor 0 and the rest are powers of two), so the
 // vector is that half edge turned clockwise, bit for bit the reference's at
 // every node of the face.
-template <int N> __device__ __forceinline__ void face_vector(const double* c, int f, double& vx, double& vy) {
-    double xa = f == 0 ? c[0] : f == 1 ? c[2] : f == 2 ? c[4] : c[6];
-    double ya = f == 0 ? c[1] : f == 1 ? c[3] : f == 2 ? c[5] : c[7];
-    double xb = f == 0 ? c[2] : f == 1 ? c[4] : f == 2 ? c[6] : c[0];
-    double yb = f == 0 ? c[3] : f == 1 ? c[5] : f == 2 ? c[7] : c[1];
-    vx = 0.5 * (yb - ya);
-    vy = -0.5 * (xb - xa);
+template <int N> __device__ __forceinline__ void face_vector(const Quad& c, int f, double& vx, double& vy) {
+    const double v0x = 0.5 * (c.y1 - c.y0), v0y = -0.5 * (c.x1 - c.x0);
+    const double v1x = 0.5 * (c.y2 - c.y1), v1y = -0.5 * (c.x2 - c.x1);
+    const double v2x = 0.5 * (c.y3 - c.y2), v2y = -0.5 * (c.x3 - c.x2);
+    const double v3x = 0.5 * (c.y0 - c.y3), v3y = -0.5 * (c.x0 - c.x3);
+    vx = f == 0 ? v0x : f == 1 ? v1x : f == 2 ? v2x : v3x;
+    vy = f == 0 ? v0y : f == 1 ? v1y : f == 2 ? v2y : v3y;
 }
 
 struct StageParams {
@@ -104,12 +112,12 @@ __device__ __forceinline__ unsigned int xcd_block(unsigned int b, unsigned int s
 // straight into the node, then the node's own sum from the diagonal on. The
 // two-point flux and the metric average are bitwise symmetric in the pair.
 template <int N>
-__device__ __forceinline__ void volume_line(Cons& rhs, int p, const int* idx, const double* sh, const double* shu,
+__device__ __forceinline__ void volume_line(Cons& rhs, int p, int lb, int ls, const double* sh, const double* shu,
                                             const double* shv, const double* su, const double* sv, const double* sb,
                                             const double* smx, const double* smy, bool affine, double g) {
     constexpr int N1 = N + 1;
     const GllTables& T = c_gll[N];
-    const int ip = idx[p];
+    const int ip = lb + p * ls;
     const Node me = {sh[ip], shu[ip], shv[ip], su[ip], sv[ip], sb[ip]};
     const double mpx = smx[ip], mpy = smy[ip];
     double d[N1];
@@ -119,7 +127,7 @@ __device__ __forceinline__ void volume_line(Cons& rhs, int p, const int* idx, co
     double bq[N1], mqx[N1], mqy[N1];
 #pragma unroll
     for (int q = 0; q < N1; ++q) {
-        const int iq = idx[q];
+        const int iq = lb + q * ls;
         const Node o = {sh[iq], shu[iq], shv[iq], su[iq], sv[iq], sb[iq]};
         mqx[q] = smx[iq];
         mqy[q] = smy[iq];
@@ -157,17 +165,17 @@ __device__ __forceinline__ void volume_line(Cons& rhs, int p, const int* idx, co
 
 // Subcell interface directions of a line (subcells.rs:63-90): im[0..N1]
 template <int N>
-__device__ __forceinline__ void interfaces(const int* idx, const double* smx, const double* smy, bool affine,
+__device__ __forceinline__ void interfaces(int lb, int ls, const double* smx, const double* smy, bool affine,
                                            double* imx, double* imy) {
     constexpr int N1 = N + 1;
     const GllTables& T = c_gll[N];
     if (affine) {
 #pragma unroll
-        for (int a = 0; a <= N1; ++a) { imx[a] = smx[idx[0]]; imy[a] = smy[idx[0]]; }
+        for (int a = 0; a <= N1; ++a) { imx[a] = smx[lb]; imy[a] = smy[lb]; }
         return;
     }
-    imx[0] = smx[idx[0]];
-    imy[0] = smy[idx[0]];
+    imx[0] = smx[lb];
+    imy[0] = smy[lb];
     double m0 = 0.0, m1 = 0.0;
 #pragma unroll
     for (int a = 0; a < N1 - 1; ++a) {
@@ -175,50 +183,54 @@ __device__ __forceinline__ void interfaces(const int* idx, const double* smx, co
         for (int c = 0; c < N1; ++c) {
             if (c == a) continue;
             double q2 = 2.0 * T.w1[a] * T.d1[a * N1 + c];
-            m0 += q2 * 0.5 * (smx[idx[a]] + smx[idx[c]]);
-            m1 += q2 * 0.5 * (smy[idx[a]] + smy[idx[c]]);
+            m0 += q2 * 0.5 * (smx[lb + a * ls] + smx[lb + c * ls]);
+            m1 += q2 * 0.5 * (smy[lb + a * ls] + smy[lb + c * ls]);
         }
         imx[a + 1] = m0;
         imy[a + 1] = m1;
     }
-    imx[N1] = smx[idx[N1 - 1]];
-    imy[N1] = smy[idx[N1 - 1]];
+    imx[N1] = smx[lb + (N1 - 1) * ls];
+    imy[N1] = smy[lb + (N1 - 1) * ls];
 }
 
 // The subcell update's contributions to the node at position p of a line
 // (swe_2d_split_form.rs:938-1035), from the reconstructed face states of the
 // line's subcells (rl, rr) and its interface fluxes (fa, fb) in LDS.
 template <int N>
-__device__ __forceinline__ void subcell_line(Cons& rhs, int p, const int* idx, const double* sh, const double* shu,
+__device__ __forceinline__ void subcell_line(Cons& rhs, int p, int lb, int ls, const double* sh, const double* shu,
                                              const double* shv, const double* su, const double* sv, const double* sb,
                                              const double* smx, const double* smy, bool affine, double g,
                                              const Side* rl, const Side* rr, const Cons* ifa, const Cons* ifb) {
     constexpr int N1 = N + 1;
     const GllTables& T = c_gll[N];
-    int ip = idx[p];
+    const int ip = lb + p * ls;
     Node me = {sh[ip], shu[ip], shv[ip], su[ip], sv[ip], sb[ip]};
     if (p == 0) rhs = cadd(rhs, cscale(1.0 / T.w1[0], ec_flux(me, me, smx[ip], smy[ip], g)));
     if (p == N1 - 1) rhs = csub(rhs, cscale(1.0 / T.w1[N1 - 1], ec_flux(me, me, smx[ip], smy[ip], g)));
-    if (p > 0) rhs = cadd(rhs, cscale(1.0 / T.w1[p], ifb[idx[p - 1]]));
+    if (p > 0) rhs = cadd(rhs, cscale(1.0 / T.w1[p], ifb[ip - ls]));
     if (p < N1 - 1) rhs = csub(rhs, cscale(1.0 / T.w1[p], ifa[ip]));
     double imx[N1 + 1], imy[N1 + 1];
-    interfaces<N>(idx, smx, smy, affine, imx, imy);
+    interfaces<N>(lb, ls, smx, smy, affine, imx, imy);
+    double mlx = imx[0], mly = imy[0], mrx = imx[1], mry = imy[1];
+#pragma unroll
+    for (int q = 1; q < N1; ++q)
+        if (q == p) { mlx = imx[q]; mly = imy[q]; mrx = imx[q + 1]; mry = imy[q + 1]; }
     const Side& left = rl[ip];
     const Side& right = rr[ip];
     double bed = -0.5 * g * (left.h + right.h) * (right.b - left.b) / T.w1[p];
     double fx, fy;
-    if (affine) { fx = bed * smx[idx[0]]; fy = bed * smy[idx[0]]; }
+    if (affine) { fx = bed * smx[lb]; fy = bed * smy[lb]; }
     else {
         double metric = 0.25 * g * ((right.h + me.h) * (me.b - right.b) + (left.h + me.h) * (me.b - left.b)) / T.w1[p];
-        fx = bed * 0.5 * (imx[p] + imx[p + 1]) + metric * (imx[p + 1] - imx[p]);
-        fy = bed * 0.5 * (imy[p] + imy[p + 1]) + metric * (imy[p + 1] - imy[p]);
+        fx = bed * 0.5 * (mlx + mrx) + metric * (mrx - mlx);
+        fy = bed * 0.5 * (mly + mry) + metric * (mry - mly);
     }
     rhs = cadd(rhs, Cons{0.0, fx, fy});
 }
 
 // Element height normal to face f at face node fi (J / sJ), affine: node-0 values
 template <int N>
-__device__ __forceinline__ double face_height(const double* c, bool affine, int f, int fi) {
+__device__ __forceinline__ double face_height(const Quad& c, bool affine, int f, int fi) {
     const GllTables& T = c_gll[N];
     constexpr int N1 = N + 1;
     double vx, vy;
@@ -232,7 +244,7 @@ __device__ __forceinline__ double face_height(const double* c, bool affine, int 
 // The outer node of face node fi of face f (swe_2d_split_form.rs:1038-1085): the
 // neighbour's first interior node on the GLL line and its reference distance
 template <int N>
-__device__ __forceinline__ bool outer_node(const StageParams& p, int64_t k, const double* c, bool affine, int f, int fi,
+__device__ __forceinline__ bool outer_node(const StageParams& p, int64_t k, const Quad& c, bool affine, int f, int fi,
                                            Side& out, double& dist) {
     constexpr int N1 = N + 1, NN = N1 * N1;
     const GllTables& T = c_gll[N];
@@ -242,9 +254,7 @@ __device__ __forceinline__ bool outer_node(const StageParams& p, int64_t k, cons
     int nbf = (int)(l & 3u);
     int nb_node = face_node<N>(nbf, N1 - 1 - fi);
     int inner = nbf == 0 ? nb_node + N1 : nbf == 1 ? nb_node - 1 : nbf == 2 ? nb_node - N1 : nb_node + 1;
-    double cn[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) cn[q] = p.corners[8 * nb + q];
+    const Quad cn = load_quad(p.corners, nb);
     bool nb_affine = p.affine[nb] != 0;
     dist = (T.x1[1] - T.x1[0]) * face_height<N>(cn, nb_affine, nbf, N1 - 1 - fi) / face_height<N>(c, affine, f, fi);
     size_t n = (size_t)nb * NN + inner;
@@ -287,7 +297,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
     uint32_t lk[NT];
     uint8_t tg[NT];
     double nh[NT], nhu[NT], nhv[NT], nbd[NT];
-    double c[8];
+    Quad c = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     double hin = 0.0, huin = 0.0, hvin = 0.0, bin = 0.0;
     if (active) {
 #pragma unroll
@@ -301,8 +311,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             }
         }
         hin = p.ih[n]; huin = p.ihu[n]; hvin = p.ihv[n]; bin = p.bed[n];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) c[q] = p.corners[8 * k + q];
+        c = load_quad(p.corners, k);
 #pragma unroll
         for (int r = 0; r < NT; ++r) {
             int task = i + r * NN;
@@ -351,29 +360,28 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
     }
     lds_sync();
 
-    int ridx[N1], sidx[N1];
-#pragma unroll
-    for (int q = 0; q < N1; ++q) { ridx[q] = eb + j * N1 + q; sidx[q] = eb + q * N1 + a; }
+    // Lines through this node: the r-line (lanes eb + j*N1 + q) and the s-line (eb + q*N1 + a)
+    const int rlb = eb + j * N1, slb = eb + a;
 
     if (SUB) {
         // Reconstructed face states of this node's subcell on its r- and s-line
         if (active) {
 #pragma unroll
             for (int dir = 0; dir < 2; ++dir) {
-                const int* idx = dir == 0 ? ridx : sidx;
+                const int lb = dir == 0 ? rlb : slb, ls = dir == 0 ? 1 : N1;
                 int pos = dir == 0 ? a : j;
                 Side me = {sh[t], su[t], sv[t], sb[t]};
                 Side lft, rgt;
                 double xlf = 0.0, xrt = 0.0;
                 bool has_l = true, has_r = true;
-                if (pos > 0) { int q = idx[pos - 1]; lft = {sh[q], su[q], sv[q], sb[q]}; xlf = T.x1[pos - 1]; }
+                if (pos > 0) { int q = t - ls; lft = {sh[q], su[q], sv[q], sb[q]}; xlf = T.x1[pos - 1]; }
                 else {
                     double d;
                     has_l = dir == 0 ? outer_node<N>(p, k, c, affine, 3, N1 - 1 - j, lft, d)
                                      : outer_node<N>(p, k, c, affine, 0, a, lft, d);
                     xlf = -1.0 - d;
                 }
-                if (pos < N1 - 1) { int q = idx[pos + 1]; rgt = {sh[q], su[q], sv[q], sb[q]}; xrt = T.x1[pos + 1]; }
+                if (pos < N1 - 1) { int q = t + ls; rgt = {sh[q], su[q], sv[q], sb[q]}; xrt = T.x1[pos + 1]; }
                 else {
                     double d;
                     has_r = dir == 0 ? outer_node<N>(p, k, c, affine, 1, j, rgt, d)
@@ -389,13 +397,16 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
         if (active) {
 #pragma unroll
             for (int dir = 0; dir < 2; ++dir) {
-                const int* idx = dir == 0 ? ridx : sidx;
+                const int lb = dir == 0 ? rlb : slb, ls = dir == 0 ? 1 : N1;
                 int pos = dir == 0 ? a : j;
                 if (pos < N1 - 1) {
                     double imx[N1 + 1], imy[N1 + 1];
-                    interfaces<N>(idx, dir == 0 ? srx : ssx, dir == 0 ? sry : ssy, affine, imx, imy);
-                    hydro_hll(rr[dir][t], rl[dir][idx[pos + 1]], imx[pos + 1], imy[pos + 1], p.g, p.h_min,
-                              ifa[dir][t], ifb[dir][t]);
+                    interfaces<N>(lb, ls, dir == 0 ? srx : ssx, dir == 0 ? sry : ssy, affine, imx, imy);
+                    double mx = imx[1], my = imy[1];
+#pragma unroll
+                    for (int q = 1; q < N1; ++q)
+                        if (q == pos + 1) { mx = imx[q]; my = imy[q]; }
+                    hydro_hll(rr[dir][t], rl[dir][t + ls], mx, my, p.g, p.h_min, ifa[dir][t], ifb[dir][t]);
                 }
             }
         }
@@ -435,10 +446,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
                     lx = -nx; ly = -ny;
                     if (!(l & 8u)) {
                         // a periodic pair: the left side's own normal
-                        int64_t nbk = (int64_t)(l >> 4);
-                        double cn[8];
-#pragma unroll
-                        for (int q2 = 0; q2 < 8; ++q2) cn[q2] = p.corners[8 * nbk + q2];
+                        const Quad cn = load_quad(p.corners, (int64_t)(l >> 4));
                         double wx, wy;
                         face_vector<N>(cn, (int)(l & 3u), wx, wy);
                         double s2 = sqrt(wx * wx + wy * wy);
@@ -467,11 +475,11 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
         for (int pass = 0; pass < 2; ++pass) {
             bool rline = (pass == 0) == r_first;
             if (SUB) {
-                if (rline) subcell_line<N>(rhs, a, ridx, sh, shu, shv, su, sv, sb, srx, sry, affine, p.g, rl[0], rr[0], ifa[0], ifb[0]);
-                else subcell_line<N>(rhs, j, sidx, sh, shu, shv, su, sv, sb, ssx, ssy, affine, p.g, rl[1], rr[1], ifa[1], ifb[1]);
+                if (rline) subcell_line<N>(rhs, a, rlb, 1, sh, shu, shv, su, sv, sb, srx, sry, affine, p.g, rl[0], rr[0], ifa[0], ifb[0]);
+                else subcell_line<N>(rhs, j, slb, N1, sh, shu, shv, su, sv, sb, ssx, ssy, affine, p.g, rl[1], rr[1], ifa[1], ifb[1]);
             } else {
-                if (rline) volume_line<N>(rhs, a, ridx, sh, shu, shv, su, sv, sb, srx, sry, affine, p.g);
-                else volume_line<N>(rhs, j, sidx, sh, shu, shv, su, sv, sb, ssx, ssy, affine, p.g);
+                if (rline) volume_line<N>(rhs, a, rlb, 1, sh, shu, shv, su, sv, sb, srx, sry, affine, p.g);
+                else volume_line<N>(rhs, j, slb, N1, sh, shu, shv, su, sv, sb, ssx, ssy, affine, p.g);
             }
         }
         // Surface terms in face order: LIFT sJ (F(q).n - F*)
@@ -674,14 +682,12 @@ __device__ __forceinline__ double node_rate(double h, double hu, double hv, doub
 }
 
 template <int N>
-__global__ __launch_bounds__(64) void dt_kernel(DtParams p) {
+__device__ __forceinline__ double dt_block(const DtParams& p, int64_t blk) {
     constexpr int N1 = N + 1, NN = N1 * N1, EPB = 64 / NN, NF = 4 * N1;
     constexpr int NT = (NF + NN - 1) / NN;
     const GllTables& T = c_gll[N];
     __shared__ double mrx[64], mry[64], msx[64], msy[64], sr[64], smass[64], swater[64], shh[64];
     __shared__ double sdt[EPB];
-    const int64_t blk = xcd_block(blockIdx.x, p.xcd_stride);
-    if (blk * EPB >= p.n_elements) return;
     const int t = threadIdx.x, e = t / NN, i = t - e * NN, a = i % N1, j = i / N1, eb = e * NN;
     const int64_t k = blk * EPB + e;
     const bool active = e < EPB && k < p.n_elements;
@@ -689,7 +695,7 @@ __global__ __launch_bounds__(64) void dt_kernel(DtParams p) {
     uint32_t lk[NT];
     double nh[NT], nhu[NT], nhv[NT];
     double h = 0.0, hu = 0.0, hv = 0.0;
-    double c[8];
+    Quad c = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (active) {
 #pragma unroll
         for (int r = 0; r < NT; ++r) {
@@ -697,8 +703,7 @@ __global__ __launch_bounds__(64) void dt_kernel(DtParams p) {
             lk[r] = task < NF ? p.links[4 * k + task / N1] : LINK_BOUNDARY;
         }
         h = p.h[n]; hu = p.hu[n]; hv = p.hv[n];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) c[q] = p.corners[8 * k + q];
+        c = load_quad(p.corners, k);
 #pragma unroll
         for (int r = 0; r < NT; ++r) {
             int task = i + r * NN;
@@ -766,11 +771,25 @@ __global__ __launch_bounds__(64) void dt_kernel(DtParams p) {
     }
     if (!active && i == 0 && e < EPB) sdt[e] = INFINITY;
     lds_sync();
-    if (t == 0) {
-        double m = INFINITY;
+    double m = INFINITY;
+    if (t == 0)
         for (int q = 0; q < EPB; ++q) m = fmin_(m, sdt[q]);
-        p.block_min[blk] = m;
-    }
+    lds_sync();
+    return m;
+}
+
+// Grid-stride over element blocks (each XCD a contiguous share); one minimum
+// per workgroup into block_min[blockIdx.x].
+template <int N>
+__global__ __launch_bounds__(64) void dt_kernel(DtParams p) {
+    constexpr int NN = (N + 1) * (N + 1), EPB = 64 / NN;
+    const int64_t nblk = (p.n_elements + EPB - 1) / EPB;
+    const unsigned int per_xcd = gridDim.x / 8;
+    const int64_t share = (nblk + 7) / 8;
+    const int64_t lo = (int64_t)(blockIdx.x & 7u) * share, hi = lo + share < nblk ? lo + share : nblk;
+    double m = INFINITY;
+    for (int64_t blk = lo + (blockIdx.x >> 3); blk < hi; blk += per_xcd) m = fmin_(m, dt_block<N>(p, blk));
+    if (threadIdx.x == 0) p.block_min[blockIdx.x] = m;
 }
 
 // The minimum of n values (one workgroup of 1024 lanes), into *out

@@ -90,6 +90,18 @@ DG_HD double ram_pressure(double h, double un, double sill, double g) {
     return (sill > head) ? ram : 0.0;
 }
 
+// One side of hydrostatic_hll in face-aligned components: the Audusse depth
+// h* = max(0, eta - B_face), the velocity (zero at or below h_min), and the
+// velocity HLL's speeds use (zero where h* is at or below h_min)
+DG_HD Hll hll_side(const Side& q, double b_face, double nx, double ny, double h_min) {
+    double h_star = fmax_(q.h + q.b - b_face, 0.0);
+    bool wet = q.h > h_min;
+    double u = wet ? q.u : 0.0, v = wet ? q.v : 0.0;
+    double un = u * nx + v * ny, ut = -u * ny + v * nx;
+    bool star_wet = h_star > h_min;
+    return {h_star, h_star * un, h_star * ut, star_wet ? un : 0.0, star_wet ? ut : 0.0};
+}
+
 // hydrostatic_hll (swe_2d_split_form.rs:336-402): Audusse states, HLL along m,
 // the hydrostatic correction per side; fa as seen from a, fb from b (both along +m)
 DG_HD void hydro_hll(const Side& qa, const Side& qb, double mx, double my, double g, double h_min, Cons& fa, Cons& fb) {
@@ -97,22 +109,12 @@ DG_HD void hydro_hll(const Side& qa, const Side& qb, double mx, double my, doubl
     double inv_norm = 1.0 / norm;
     double nx = mx * inv_norm, ny = my * inv_norm;
     double b_face = fmax_(qa.b, qb.b);
-    Hll s[2];
-    const Side* q[2] = {&qa, &qb};
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        double h_star = fmax_(q[i]->h + q[i]->b - b_face, 0.0);
-        bool wet = q[i]->h > h_min;
-        double u = wet ? q[i]->u : 0.0, v = wet ? q[i]->v : 0.0;
-        double un = u * nx + v * ny, ut = -u * ny + v * nx;
-        bool star_wet = h_star > h_min;
-        s[i] = {h_star, h_star * un, h_star * ut, star_wet ? un : 0.0, star_wet ? ut : 0.0};
-    }
-    Cons rot = hll_aligned(s[0], s[1], g, h_min);
+    const Hll sa = hll_side(qa, b_face, nx, ny, h_min), sb = hll_side(qb, b_face, nx, ny, h_min);
+    Cons rot = hll_aligned(sa, sb, g, h_min);
     double px = rot.hu * nx - rot.hv * ny, py = rot.hu * ny + rot.hv * nx;
     double fh = norm * rot.h, fhu = norm * px, fhv = norm * py;
-    bool hll_dry = (s[0].h <= h_min) && (s[1].h <= h_min);
-    double hsa = hll_dry ? 0.0 : s[0].h, hsb = hll_dry ? 0.0 : s[1].h;
+    bool hll_dry = (sa.h <= h_min) && (sb.h <= h_min);
+    double hsa = hll_dry ? 0.0 : sa.h, hsb = hll_dry ? 0.0 : sb.h;
     double wall_a = 0.0, wall_b = 0.0;
     if (hll_dry) {
         double una = 1.0 * (qa.u * nx + qa.v * ny);

@@ -60,21 +60,22 @@ double emu_dt(int order, int64_t K, const double* state, const double* corners, 
     size_t nn = (size_t)(order + 1) * (order + 1), nt = (size_t)K * nn;
     int epb = 64 / (int)nn;
     unsigned nblk = (unsigned)((K + epb - 1) / epb);
-    std::vector<double> bmin(nblk);
+    const unsigned groups = 16;
+    std::vector<double> bmin(groups);
     DtParams p;
     p.h = state; p.hu = state + nt; p.hv = state + 2 * nt;
     p.corners = corners; p.links = links; p.affine = affine; p.n_elements = K;
     p.cfl = cfl; p.g = g; p.h_min = h_min; p.h_dry = h_dry; p.pcfl = pcfl; p.max_cfl = max_cfl;
     p.max_cfl_sub = max_cfl_sub; p.positivity = positivity; p.element_dt = element_dt; p.block_min = bmin.data();
-    p.xcd_stride = (nblk + 7) / 8;
-    unsigned grid = 8 * p.xcd_stride;
+    p.xcd_stride = 0;
+    unsigned grid = groups;
     switch (order) {
         case 1: emu_launch(grid, 64, [&] { dt_kernel<1>(p); }); break;
         case 2: emu_launch(grid, 64, [&] { dt_kernel<2>(p); }); break;
         default: emu_launch(grid, 64, [&] { dt_kernel<3>(p); }); break;
     }
     double out = 0.0;
-    emu_launch(1, 1024, [&] { min_kernel(bmin.data(), nblk, &out); });
+    emu_launch(1, 1024, [&] { min_kernel(bmin.data(), groups, &out); });
     return out;
 }
 
