@@ -43,8 +43,9 @@ def stage_bytes(nn: int, base_differs: bool) -> int:
 
 
 def read_traffic(workload: str):
-    """HBM bytes per stage launch measured with rocprofv3 counters, if a summary
-    for this workload has been committed (profiles/traffic_<workload>.json)."""
+    """HBM bytes per stage (wet + subcell kernels) measured with rocprofv3's
+    FETCH_SIZE and WRITE_SIZE counters, scaled by what they read for a plain copy
+    of known size, if that summary is in profiles/traffic_<workload>.json."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if os.path.exists(path):
         with open(path) as f:
@@ -85,8 +86,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the time step on a single device "
-              "(see the sharding design in DESIGN.md); --gpus must be 1", file=sys.stderr)
+        print("bench.py measures one GPU: the time step runs on a single device here "
+              "(element partitions with a face-trace halo per stage are the multi-GPU design); "
+              "--gpus must be 1", file=sys.stderr)
         sys.exit(2)
 
     t0 = time.time()
@@ -142,7 +144,8 @@ def main():
                    "model_time_s": t},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "stage_kernel<3>", "avg_launch_ms": stage_ms / max(launches, 1),
+                     "kernel": "one SSP-RK3 stage: wet_stage_kernel<3> + list compaction + subcell_stage_kernel<3>",
+                     "avg_launch_ms": stage_ms / max(launches, 1),
                      "bytes_per_launch_avg": bytes_total / max(launches, 1),
                      "measured_copy_GBps": copy_gbs, "frac_of_copy": achieved / copy_gbs,
                      "dt_kernel_avg_ms": dt_ms / max(dt_launches, 1)},
