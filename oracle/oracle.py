@@ -96,6 +96,17 @@ def _bind(L):
         L.oc_set_threads.argtypes = [ctypes.c_int]
         L.oc_max_threads.restype = ctypes.c_int
         L.oc_face_mass.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, _dp]
+        L.oc_tlm_create.restype = ctypes.c_void_p
+        L.oc_tlm_create.argtypes = L.oc_create.argtypes
+        L.oc_tlm_destroy.argtypes = [ctypes.c_void_p]
+        L.oc_tlm_set_tide.argtypes = L.oc_set_tide.argtypes
+        L.oc_tlm_set_param_tangents.argtypes = [ctypes.c_void_p, _dp, _dp]
+        _pp = ctypes.POINTER(_dp)
+        L.oc_tlm_stage.argtypes = [ctypes.c_void_p, _pp, _pp, _pp, _pp] + [ctypes.c_double] * 4 + [ctypes.c_int, _pp, _pp]
+        L.oc_tlm_run.argtypes = [ctypes.c_void_p, _pp, _pp, ctypes.c_double, ctypes.c_int, _dp, ctypes.c_int]
+        L.oc_tlm_run.restype = ctypes.c_double
+        L.oc_run_dts.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, ctypes.c_int]
+        L.oc_run_dts.restype = ctypes.c_double
     return L
 
 
@@ -191,6 +202,11 @@ class Oracle:
         te = self._L.oc_run(self.h, _p(h), _p(hu), _p(hv), t, n_steps, cfl, scheme, None if dts is None else _p(dts))
         return (te, dts) if record_dt else te
 
+    def run_dts(self, h, hu, hv, t, dts, scheme=0):
+        """Steps at the prescribed dts, in place."""
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        return self._L.oc_run_dts(self.h, _p(h), _p(hu), _p(hv), t, len(dts), _p(dts), scheme)
+
     def run_timed(self, h, hu, hv, t, n_steps, cfl, scheme=0):
         sec = ctypes.c_double()
         te = self._L.oc_run_timed(self.h, _p(h), _p(hu), _p(hv), t, n_steps, cfl, scheme, ctypes.byref(sec))
@@ -209,6 +225,80 @@ class Oracle:
         out = [np.empty(self.K * self.nn) for _ in range(6)]
         self._L.oc_geometry(self.h, *(_p(o) for o in out))
         return dict(zip(("det_j", "mass", "jrx", "jry", "jsx", "jsy"), out))
+
+
+def _ptrs(arrs):
+    """A C array of double pointers to three contiguous float64 arrays."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in arrs]
+    return (_dp * 3)(*(_p(a) for a in arrs)), arrs
+
+
+class TangentLinear:
+    """The restatement on dual numbers: directional derivatives of stages and runs
+    with respect to the states, the nodal bed and per-element Manning n (time steps
+    prescribed, as a discrete adjoint freezes them)."""
+
+    def __init__(self, order, mesh, bed, manning=None, f0=0.0, g=9.81, h_min=1e-6, h_dry=1e-3, tide=None,
+                 subcells=True, friction=True, wet_dry=True):
+        self._L = L = lib()
+        self.K = mesh.n_elements
+        self.nn = (order + 1) ** 2
+        self._keep = [np.ascontiguousarray(mesh.corners, dtype=np.float64),
+                      np.ascontiguousarray(mesh.links, dtype=np.uint32),
+                      np.ascontiguousarray(mesh.tags, dtype=np.uint8),
+                      np.ascontiguousarray(bed, dtype=np.float64),
+                      None if manning is None else np.ascontiguousarray(manning, dtype=np.float64)]
+        c, l, t, b, m = self._keep
+        flags = (FLAG_SUBCELLS if subcells else 0) | (FLAG_FRICTION if friction else 0) | (FLAG_WET_DRY if wet_dry else 0)
+        self.h = L.oc_tlm_create(order, self.K, _p(c), l.ctypes.data_as(_u32p), t.ctypes.data_as(_u8p), _p(b),
+                                 None if m is None else _p(m), g, h_min, h_dry, f0, SURFACE_HLL, flags)
+        if not self.h:
+            raise RuntimeError(L.oc_last_error().decode())
+        if tide is not None:
+            a = np.array(tide.amplitude, dtype=np.float64)
+            w = np.array(tide.omega, dtype=np.float64)
+            ph = np.array(tide.phase, dtype=np.float64)
+            L.oc_tlm_set_tide(self.h, tide.mean, tide.ramp, len(a), _p(a), _p(w), _p(ph))
+
+    @classmethod
+    def from_case(cls, case, **kw):
+        return cls(case.order, case.mesh, case.bed, manning=case.manning, f0=case.f0, g=case.g, h_min=case.h_min,
+                   h_dry=case.h_dry, tide=case.tide, **kw)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._L.oc_tlm_destroy(self.h)
+            self.h = None
+
+    def set_param_tangents(self, dbed=None, dmanning=None):
+        self._dp_keep = [None if dbed is None else np.ascontiguousarray(dbed, dtype=np.float64),
+                         None if dmanning is None else np.ascontiguousarray(dmanning, dtype=np.float64)]
+        a, b = self._dp_keep
+        self._L.oc_tlm_set_param_tangents(self.h, None if a is None else _p(a), None if b is None else _p(b))
+
+    def stage(self, base, dbase, inp, dinp, a, b, c, time, last):
+        pb, kb = _ptrs(base)
+        pdb, kdb = _ptrs(dbase)
+        pi, ki = _ptrs(inp)
+        pdi, kdi = _ptrs(dinp)
+        out = [np.empty(self.K * self.nn) for _ in range(3)]
+        dout = [np.empty(self.K * self.nn) for _ in range(3)]
+        po, _ = _ptrs(out)
+        pdo, _ = _ptrs(dout)
+        self._L.oc_tlm_stage(self.h, pb, pdb, pi, pdi, a, b, c, time, int(last), po, pdo)
+        return out, dout
+
+    def run(self, x, dx, t, dts, scheme=0):
+        """Steps at the prescribed dts, in place on the value and tangent arrays."""
+        px, kx = _ptrs(x)
+        pdx, kdx = _ptrs(dx)
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        te = self._L.oc_tlm_run(self.h, px, pdx, t, len(dts), _p(dts), scheme)
+        for a, b in zip(x, kx):
+            a[...] = b
+        for a, b in zip(dx, kdx):
+            a[...] = b
+        return te
 
 
 def set_threads(n):

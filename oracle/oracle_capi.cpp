@@ -139,6 +139,96 @@ double oc_run_timed(void* p, double* h, double* hu, double* hv, double t, int n_
 
 uint64_t oc_clips(void* p) { return ((Handle*)p)->s->clips; }
 
+// ---- Tangent-linear model: the same restatement on dual numbers ----
+struct TlmHandle {
+    Solver<Dual>* s = nullptr;
+    std::vector<Dual> work;
+};
+
+void* oc_tlm_create(int order, int64_t K, const double* corners, const uint32_t* links, const uint8_t* tags,
+                    const double* bed, const double* manning, double g, double h_min, double h_dry, double f0,
+                    int surface, int flags) {
+    if (order < 1 || order > 3) { g_err = "order must be 1..3"; return nullptr; }
+    TlmHandle* h = new TlmHandle;
+    h->s = new Solver<Dual>(order, K, corners, links, tags);
+    std::string e = h->s->init_geometry();
+    if (!e.empty()) { g_err = e; delete h->s; delete h; return nullptr; }
+    auto& c = h->s->cfg;
+    c.g = g; c.h_min = h_min; c.h_dry = h_dry; c.subcell_depth = h_dry; c.f0 = f0; c.surface = surface;
+    c.subcells = flags & 1; c.friction = flags & 2; c.wet_dry = flags & 4; c.manufactured = flags & 8;
+    size_t n = (size_t)K * h->s->ops.nn;
+    for (size_t i = 0; i < n; ++i) h->s->bed[i] = Dual(bed[i]);
+    if (manning) for (int64_t k = 0; k < K; ++k) h->s->manning[k] = Dual(manning[k]);
+    return h;
+}
+
+void oc_tlm_destroy(void* p) {
+    TlmHandle* h = (TlmHandle*)p;
+    if (!h) return;
+    delete h->s;
+    delete h;
+}
+
+void oc_tlm_set_tide(void* p, double mean, double ramp, int n, const double* amp, const double* omega, const double* phase) {
+    auto& t = ((TlmHandle*)p)->s->tide;
+    t.mean = mean; t.ramp = ramp;
+    t.amplitude.assign(amp, amp + n); t.omega.assign(omega, omega + n); t.phase.assign(phase, phase + n);
+}
+
+// Tangents of the bed and of Manning n (directions of the parameters)
+void oc_tlm_set_param_tangents(void* p, const double* dbed, const double* dmanning) {
+    Solver<Dual>& s = *((TlmHandle*)p)->s;
+    size_t n = (size_t)s.K * s.ops.nn;
+    for (size_t i = 0; i < n; ++i) s.bed[i].d = dbed ? dbed[i] : 0.0;
+    for (int64_t k = 0; k < s.K; ++k) s.manning[k].d = dmanning ? dmanning[k] : 0.0;
+}
+
+static void pack(std::vector<Dual>& v, const double* x, const double* dx, size_t n) {
+    v.resize(n);
+    for (size_t i = 0; i < n; ++i) v[i] = Dual(x[i], dx ? dx[i] : 0.0);
+}
+static void unpack(const std::vector<Dual>& v, double* x, double* dx) {
+    for (size_t i = 0; i < v.size(); ++i) { x[i] = v[i].v; dx[i] = v[i].d; }
+}
+
+// One fused stage on dual numbers: states (value, tangent) in, (value, tangent) out
+void oc_tlm_stage(void* p, const double* const* base, const double* const* dbase, const double* const* input,
+                  const double* const* dinput, double a, double b, double c, double time, int last,
+                  double* const* out, double* const* dout) {
+    Solver<Dual>& s = *((TlmHandle*)p)->s;
+    size_t n = (size_t)s.K * s.ops.nn;
+    std::vector<Dual> B[3], I[3], O[3];
+    for (int v = 0; v < 3; ++v) { pack(B[v], base[v], dbase[v], n); pack(I[v], input[v], dinput[v], n); O[v].resize(n); }
+    s.stage(B[0].data(), B[1].data(), B[2].data(), I[0].data(), I[1].data(), I[2].data(), O[0].data(), O[1].data(),
+            O[2].data(), a, b, c, time, last != 0);
+    for (int v = 0; v < 3; ++v) unpack(O[v], out[v], dout[v]);
+}
+
+// n_steps SSP steps at the prescribed dts, in place on (value, tangent) states
+double oc_tlm_run(void* p, double* const* x, double* const* dx, double t, int n_steps, const double* dts, int scheme) {
+    TlmHandle* hd = (TlmHandle*)p;
+    Solver<Dual>& s = *hd->s;
+    size_t n = (size_t)s.K * s.ops.nn;
+    std::vector<Dual> S[3];
+    for (int v = 0; v < 3; ++v) pack(S[v], x[v], dx[v], n);
+    for (int i = 0; i < n_steps; ++i) {
+        s.step(S[0].data(), S[1].data(), S[2].data(), t, dts[i], scheme, hd->work);
+        t += dts[i];
+    }
+    for (int v = 0; v < 3; ++v) unpack(S[v], x[v], dx[v]);
+    return t;
+}
+
+// n_steps SSP steps at prescribed dts on plain doubles (the forward of a frozen-dt adjoint)
+double oc_run_dts(void* p, double* h, double* hu, double* hv, double t, int n_steps, const double* dts, int scheme) {
+    Handle* hd = (Handle*)p;
+    for (int i = 0; i < n_steps; ++i) {
+        hd->s->step(h, hu, hv, t, dts[i], scheme, hd->work);
+        t += dts[i];
+    }
+    return t;
+}
+
 // Integer wet/dry masks: node h < h_dry, and the element takes the subcells
 void oc_masks(void* p, const double* h, uint8_t* node_dry, uint8_t* elem_subcell) {
     Solver<double>& s = *((Handle*)p)->s;
