@@ -46,11 +46,13 @@ constexpr unsigned int SUB_GROUPS = 16384;
 
 struct dg_state {
     double* d = nullptr;  // 3*K*NN: h, hu, hv
+    bool owned = true;    // false: caller memory (dg_state_wrap)
 };
 
 struct dg_ctx {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // where the work runs (the context's own, or the caller's)
+    hipStream_t own_stream = nullptr;
     int order = 0, nn = 0;
     int64_t K = 0;
     int flags = 0;
@@ -71,6 +73,15 @@ struct dg_ctx {
     hipEvent_t ev[2] = {nullptr, nullptr};
     double stage_ms = 0.0, dt_ms = 0.0;
     int64_t stage_n = 0, dt_n = 0;
+    // discrete adjoint
+    double *gbed = nullptr, *gman = nullptr;          // parameter cotangents [K*NN], [K]
+    double *facebuf = nullptr, *innerbuf = nullptr;  // [K][4(N+1)][4] neighbour cotangents
+    uint8_t* kind = nullptr;                         // [K] subcell elements of the last VJP
+    dg_state* adj[2] = {nullptr, nullptr};           // stage cotangents of dg_step_vjp
+    // stations
+    int n_st = 0;
+    int64_t* st_elem = nullptr;
+    double *st_w = nullptr, *st_eta = nullptr;
 };
 
 #define DG_TRY(expr)                                                                              \
@@ -122,9 +133,22 @@ static double tide_elevation(const dg_tide& t, double time) {
 
 static int free_statics(dg_ctx* c) {
     void* ptrs[] = {c->corners, c->bed, c->manning, c->element_dt, c->links, c->tags, c->affine, c->queue,
-                    c->queue_count, c->block_min, c->sub_list, c->chunk_total};
+                    c->queue_count, c->block_min, c->sub_list, c->chunk_total, c->gbed, c->gman, c->facebuf,
+                    c->innerbuf, c->kind, c->st_elem, c->st_w, c->st_eta};
     for (void* p : ptrs)
         if (p) DG_TRY(hipFree(p));
+    c->gbed = c->gman = c->facebuf = c->innerbuf = nullptr;
+    c->kind = nullptr;
+    c->st_elem = nullptr;
+    c->st_w = c->st_eta = nullptr;
+    c->n_st = 0;
+    for (auto& w : c->adj) {
+        if (w) {
+            DG_TRY(hipFree(w->d));
+            delete w;
+            w = nullptr;
+        }
+    }
     c->corners = c->bed = c->manning = c->element_dt = nullptr;
     c->links = nullptr;
     c->tags = c->affine = nullptr;
@@ -247,23 +271,295 @@ static int ensure_work(dg_ctx* c) {
     return 0;
 }
 
+// Shu-Osher stages (a, b, c, time) per scheme (integrator.rs:239-263)
+static const double RK3[3][4] = {{1.0, 0.0, 1.0, 0.0}, {0.75, 0.25, 0.25, 1.0}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, 0.5}};
+static const double RK43[4][4] = {{1.0, 0.0, 0.5, 0.0}, {0.0, 1.0, 0.5, 0.5}, {2.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0, 1.0},
+                                  {0.0, 1.0, 0.5, 0.5}};
+
 static int step_impl(dg_ctx* c, dg_state* s, double t, double dt, int scheme) {
-    // Shu-Osher stages (integrator.rs:239-263); stage values in their own buffers,
-    // the last one the new state (multirate.rs:1285-1311 with one level)
-    static const double rk3[3][4] = {{1.0, 0.0, 1.0, 0.0}, {0.75, 0.25, 0.25, 1.0}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, 0.5}};
-    static const double rk43[4][4] = {{1.0, 0.0, 0.5, 0.0}, {0.0, 1.0, 0.5, 0.5}, {2.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0, 1.0},
-                                      {0.0, 1.0, 0.5, 0.5}};
+    // stage values in their own buffers, the last one the new state
+    // (multirate.rs:1285-1311 with one level)
     if (int r = ensure_work(c)) return r;
     int ns = scheme == DG_SSP_RK3 ? 3 : 4;
-    const double(*st)[4] = scheme == DG_SSP_RK3 ? rk3 : rk43;
+    const double(*st)[4] = scheme == DG_SSP_RK3 ? RK3 : RK43;
     const double* input = s->d;
     for (int q = 0; q < ns; ++q) {
         double* out = c->work[q]->d;
         if (int r = launch_stage(c, s->d, input, out, st[q][0], st[q][1], st[q][2] * dt, t + st[q][3] * dt, 0)) return r;
         input = out;
     }
-    std::swap(s->d, c->work[ns - 1]->d);
+    if (s->owned) std::swap(s->d, c->work[ns - 1]->d);
+    else DG_TRY(hipMemcpyAsync(s->d, c->work[ns - 1]->d, (size_t)3 * c->K * c->nn * 8, hipMemcpyDeviceToDevice, c->stream));
     return 0;
+}
+
+static int zero_async(dg_ctx* c, double* p, size_t n) {
+    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192)), dim3(256), 0, c->stream, p, n);
+    DG_TRY(hipGetLastError());
+    return 0;
+}
+
+static int ensure_adjoint(dg_ctx* c) {
+    if (c->facebuf) return 0;
+    const size_t nf = 4 * (size_t)(c->order + 1), nt = (size_t)c->K * c->nn;
+    DG_TRY(hipMalloc(&c->facebuf, (size_t)c->K * nf * 4 * sizeof(double)));
+    DG_TRY(hipMalloc(&c->innerbuf, (size_t)c->K * nf * 4 * sizeof(double)));
+    DG_TRY(hipMalloc(&c->kind, (size_t)c->K));
+    DG_TRY(hipMalloc(&c->gbed, nt * sizeof(double)));
+    DG_TRY(hipMalloc(&c->gman, (size_t)c->K * sizeof(double)));
+    for (auto& w : c->adj) {
+        w = new dg_state;
+        DG_TRY(hipMalloc(&w->d, 3 * nt * sizeof(double)));
+    }
+    if (int r = zero_async(c, c->gbed, nt)) return r;
+    return zero_async(c, c->gman, (size_t)c->K);
+}
+
+// The VJP of one stage: gbase (may be null) and ginput are added to, and the
+// parameter cotangents; the neighbour parts of ginput and the bed's are
+// gathered after the element kernels.
+static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, const double* w, double* gbase,
+                            double* ginput, double a, double b, double cc, double time, int rhs_only) {
+    if (int r = ensure_adjoint(c)) return r;
+    size_t nt = (size_t)c->K * c->nn;
+    VjpParams P;
+    StageParams& p = P.s;
+    p.bh = base; p.bhu = base + nt; p.bhv = base + 2 * nt;
+    p.ih = input; p.ihu = input + nt; p.ihv = input + 2 * nt;
+    p.oh = p.ohu = p.ohv = nullptr;
+    p.bed = c->bed; p.corners = c->corners; p.manning = c->manning;
+    p.links = c->links; p.tags = c->tags; p.affine = c->affine;
+    p.n_elements = c->K;
+    p.a = a; p.b = b; p.c = cc;
+    p.eta_e = tide_elevation(c->tide, time);
+    p.g = c->g; p.h_min = c->h_min; p.h_dry = c->h_dry; p.u_max = c->u_max; p.tau = c->tau; p.f0 = c->f0;
+    p.rhs_only = rhs_only;
+    p.flags = c->flags;
+    p.clips = c->clips;
+    p.fstar_out = nullptr;
+    p.queue = c->queue;
+    p.queue_count = c->queue_count;
+    P.wh = w; P.whu = w + nt; P.whv = w + 2 * nt;
+    P.gbh = gbase; P.gbhu = gbase ? gbase + nt : nullptr; P.gbhv = gbase ? gbase + 2 * nt : nullptr;
+    P.gih = ginput; P.gihu = ginput + nt; P.gihv = ginput + 2 * nt;
+    P.gbed = c->gbed;
+    P.gman = c->gman;
+    P.facebuf = c->facebuf;
+    P.innerbuf = c->innerbuf;
+    P.kind = c->kind;
+    const int epb = 64 / c->nn;
+    const unsigned int nblk = (unsigned int)((c->K + epb - 1) / epb);
+    const unsigned int nchunk = (nblk + 1023) / 1024;
+    p.xcd_stride = (nblk + 7) / 8;
+    dim3 grid(8 * p.xcd_stride), sub_grid(std::min(nblk, SUB_GROUPS)), ggrid((unsigned)((nt + 255) / 256));
+    switch (c->order) {
+        case 1: hipLaunchKernelGGL(wet_vjp_kernel<1>, grid, dim3(64), 0, c->stream, P); break;
+        case 2: hipLaunchKernelGGL(wet_vjp_kernel<2>, grid, dim3(64), 0, c->stream, P); break;
+        default: hipLaunchKernelGGL(wet_vjp_kernel<3>, grid, dim3(64), 0, c->stream, P); break;
+    }
+    hipLaunchKernelGGL(chunk_count_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, nblk, c->chunk_total);
+    hipLaunchKernelGGL(compact_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, c->queue, nblk,
+                       c->chunk_total, epb, c->sub_list, c->sub_count);
+    switch (c->order) {
+        case 1: hipLaunchKernelGGL(subcell_vjp_kernel<1>, sub_grid, dim3(64), 0, c->stream, P, c->sub_list, c->sub_count); break;
+        case 2: hipLaunchKernelGGL(subcell_vjp_kernel<2>, sub_grid, dim3(64), 0, c->stream, P, c->sub_list, c->sub_count); break;
+        default: hipLaunchKernelGGL(subcell_vjp_kernel<3>, sub_grid, dim3(64), 0, c->stream, P, c->sub_list, c->sub_count); break;
+    }
+    double *gh = ginput, *ghu = ginput + nt, *ghv = ginput + 2 * nt;
+    switch (c->order) {
+        case 1: hipLaunchKernelGGL(vjp_gather_kernel<1>, ggrid, dim3(256), 0, c->stream, c->links, c->facebuf, c->innerbuf, c->kind, c->K, gh, ghu, ghv, c->gbed); break;
+        case 2: hipLaunchKernelGGL(vjp_gather_kernel<2>, ggrid, dim3(256), 0, c->stream, c->links, c->facebuf, c->innerbuf, c->kind, c->K, gh, ghu, ghv, c->gbed); break;
+        default: hipLaunchKernelGGL(vjp_gather_kernel<3>, ggrid, dim3(256), 0, c->stream, c->links, c->facebuf, c->innerbuf, c->kind, c->K, gh, ghu, ghv, c->gbed); break;
+    }
+    DG_TRY(hipGetLastError());
+    return 0;
+}
+
+// The VJP of one SSP step from state X: the stage values are recomputed, then
+// the stages are reversed; lam = (dX'/dX)^T lam_next (overwritten).
+static int step_vjp_impl(dg_ctx* c, const double* X, double t, double dt, int scheme, const double* lam_next,
+                         double* lam) {
+    if (int r = ensure_work(c)) return r;
+    if (int r = ensure_adjoint(c)) return r;
+    const int ns = scheme == DG_SSP_RK3 ? 3 : 4;
+    const double(*st)[4] = scheme == DG_SSP_RK3 ? RK3 : RK43;
+    const size_t n3 = (size_t)3 * c->K * c->nn;
+    const double* input = X;
+    for (int q = 0; q + 1 < ns; ++q) {
+        if (int r = launch_stage(c, X, input, c->work[q]->d, st[q][0], st[q][1], st[q][2] * dt, t + st[q][3] * dt, 0))
+            return r;
+        input = c->work[q]->d;
+    }
+    if (int r = zero_async(c, lam, n3)) return r;
+    const double* w = lam_next;
+    int tb = 0;
+    for (int q = ns - 1; q >= 0; --q) {
+        const double* inq = q == 0 ? X : c->work[q - 1]->d;
+        double* gin = lam;
+        if (q > 0) {
+            gin = c->adj[tb]->d;
+            if (int r = zero_async(c, gin, n3)) return r;
+        }
+        if (int r = launch_stage_vjp(c, X, inq, w, lam, gin, st[q][0], st[q][1], st[q][2] * dt, t + st[q][3] * dt, 0))
+            return r;
+        w = gin;
+        tb ^= 1;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Binomial checkpointing. reverse(a, b) holds the state of step a in a slot
+// and reverses steps b-1 .. a with `free` more slots: it advances to a split
+// a+m, stores that state, reverses [a+m, b) with one slot fewer, then [a, a+m).
+// The splits minimise the forward steps: T(l, s) = min_m m + T(l-m, s-1) + T(m, s)
+// with T(1, s) = 0 and T(l, 0) = l(l-1)/2 (restarting from the slot for every
+// step); G(l, s) = min_m T(m, s) + G(l-m, s-1) is the same for the right-most
+// descent, whose checkpoints the first forward sweep takes (it runs anyway for
+// the misfit). T and G are convex in l, so the best split does not move left
+// as l grows and one pointer per s finds it: O(s n) set-up and memory (the
+// slots the schedule uses are capped so that the tables stay under 64 M
+// entries). With 20 slots over 2000 steps it takes 5980 forward steps, the
+// binomial bound of C(s+r, s) steps at r repetitions (r = 4 here).
+// ---------------------------------------------------------------------------
+struct Revolve {
+    int n = 0, S = 0;             // steps, free slots beyond the initial state's
+    std::vector<int64_t> T, G;    // costs [s][l]
+    std::vector<int32_t> AT, AG;  // best splits [s][l]
+    size_t idx(int s, int l) const { return (size_t)s * (n + 1) + l; }
+
+    void init(int n_, int S_) {
+        n = n_;
+        S = std::min(S_, std::max(n_ - 1, 0));  // more slots than steps change nothing
+        while (S > 1 && (size_t)(S + 1) * (size_t)(n + 1) > (size_t)64 << 20) --S;
+        T.assign((size_t)(S + 1) * (n + 1), 0);
+        G = T;
+        AT.assign(T.size(), 1);
+        AG = AT;
+        for (int l = 1; l <= n; ++l) T[idx(0, l)] = G[idx(0, l)] = (int64_t)l * (l - 1) / 2;
+        for (int s = 1; s <= S; ++s) {
+            int mt = 1, mg = 1;
+            for (int l = 2; l <= n; ++l) {
+                auto ft = [&](int m) { return m + T[idx(s - 1, l - m)] + T[idx(s, m)]; };
+                auto fg = [&](int m) { return T[idx(s, m)] + G[idx(s - 1, l - m)]; };
+                mt = std::min(mt, l - 1);
+                mg = std::min(mg, l - 1);
+                while (mt + 1 < l && ft(mt + 1) <= ft(mt)) ++mt;
+                while (mg + 1 < l && fg(mg + 1) <= fg(mg)) ++mg;
+                T[idx(s, l)] = ft(mt); AT[idx(s, l)] = mt;
+                G[idx(s, l)] = fg(mg); AG[idx(s, l)] = mg;
+            }
+        }
+    }
+    int slots(int free) const { return std::min(free, S); }
+    int split(int l, int s, bool prestored) const {
+        s = std::min(s, S);
+        return prestored ? AG[idx(s, l)] : AT[idx(s, l)];
+    }
+};
+
+struct dg_tape {
+    int snapshots = 0;
+    std::vector<dg_state*> slot;
+    std::vector<char> used;
+    dg_state* cur = nullptr;
+    dg_state* lam[2] = {nullptr, nullptr};
+    std::vector<double> dts, t_at;
+    int n = 0, scheme = 0, every = 0, n_samples = 0, n_st = 0;
+    double* eta = nullptr;  // device [n_samples][n_st]
+    double* obs = nullptr;  // device [n_samples][n_st]
+    int64_t fwd = 0, vjps = 0;
+    bool ready = false;
+    int64_t K = 0;
+    int nn = 0;
+    Revolve rv;
+    std::vector<std::pair<int, int>> pre;  // (step, slot) kept by the forward sweep
+    int lam_cur = 0;
+    double scale = 1.0;
+};
+
+static int tape_free_slot(dg_tape* tp) {
+    for (size_t i = 0; i < tp->slot.size(); ++i)
+        if (!tp->used[i]) return (int)i;
+    return -1;
+}
+
+static int tape_copy(dg_ctx* c, dg_state* dst, const dg_state* src) {
+    DG_TRY(hipMemcpyAsync(dst->d, src->d, (size_t)3 * c->K * c->nn * 8, hipMemcpyDeviceToDevice, c->stream));
+    return 0;
+}
+
+// Advance tp->cur from step a to step b
+static int tape_advance(dg_ctx* c, dg_tape* tp, int a, int b) {
+    for (int i = a; i < b; ++i) {
+        if (int r = step_impl(c, tp->cur, tp->t_at[i], tp->dts[i], tp->scheme)) return r;
+        tp->fwd += 1;
+    }
+    return 0;
+}
+
+// Reverse step i from the state X_i in `xs`: seed the cotangent of X_{i+1} with
+// the misfit's sample there, then lam <- (dX_{i+1}/dX_i)^T lam
+static int tape_step_back(dg_ctx* c, dg_tape* tp, const dg_state* xs, int i) {
+    const int after = i + 1;
+    double* lam = tp->lam[tp->lam_cur]->d;
+    if (after % tp->every == 0) {
+        const int m = after / tp->every - 1;
+        if (m < tp->n_samples) {
+            hipLaunchKernelGGL(station_seed_kernel, dim3(1), dim3(1), 0, c->stream, c->st_elem, c->st_w, tp->n_st,
+                               c->nn, tp->eta + (size_t)m * tp->n_st, tp->obs + (size_t)m * tp->n_st, tp->scale, lam,
+                               c->gbed);
+            DG_TRY(hipGetLastError());
+        }
+    }
+    double* out = tp->lam[tp->lam_cur ^ 1]->d;
+    if (int r = step_vjp_impl(c, xs->d, tp->t_at[i], tp->dts[i], tp->scheme, lam, out)) return r;
+    tp->lam_cur ^= 1;
+    tp->vjps += 1;
+    return 0;
+}
+
+static int tape_reverse(dg_ctx* c, dg_tape* tp, int a, int b, int slot_a, int free, bool prestored) {
+    const int l = b - a;
+    if (l <= 0) return 0;
+    if (l == 1) return tape_step_back(c, tp, tp->slot[slot_a], a);
+    if (free == 0) {
+        for (int i = b - 1; i >= a; --i) {
+            if (i == a) {
+                if (int r = tape_step_back(c, tp, tp->slot[slot_a], a)) return r;
+            } else {
+                if (int r = tape_copy(c, tp->cur, tp->slot[slot_a])) return r;
+                if (int r = tape_advance(c, tp, a, i)) return r;
+                if (int r = tape_step_back(c, tp, tp->cur, i)) return r;
+            }
+        }
+        return 0;
+    }
+    const int m = tp->rv.split(l, free, prestored);
+    int sm = -1;
+    if (prestored) {
+        for (auto& ps : tp->pre)
+            if (ps.first == a + m) sm = ps.second;
+        DG_REQUIRE(sm >= 0, "dg_misfit_backward: checkpoint schedule mismatch");
+    } else {
+        sm = tape_free_slot(tp);
+        DG_REQUIRE(sm >= 0, "dg_misfit_backward: no free checkpoint slot");
+        tp->used[sm] = 1;
+        if (int r = tape_copy(c, tp->cur, tp->slot[slot_a])) return r;
+        if (int r = tape_advance(c, tp, a, a + m)) return r;
+        if (int r = tape_copy(c, tp->slot[sm], tp->cur)) return r;
+    }
+    if (int r = tape_reverse(c, tp, a + m, b, sm, free - 1, prestored)) return r;
+    tp->used[sm] = 0;
+    return tape_reverse(c, tp, a, a + m, slot_a, free, false);
+}
+
+// Forward steps a schedule takes (first sweep included), without a device
+static int64_t revolve_count_rec(const Revolve& rv, int l, int free, bool prestored) {
+    if (l <= 1) return 0;
+    if (free == 0) return (int64_t)l * (l - 1) / 2;
+    const int m = rv.split(l, free, prestored);
+    return (prestored ? 0 : m) + revolve_count_rec(rv, l - m, free - 1, prestored) + revolve_count_rec(rv, m, free, false);
 }
 
 extern "C" {
@@ -275,8 +571,9 @@ int dg_create(int device, dg_ctx** out) {
     DG_TRY(hipSetDevice(device));
     dg_ctx* c = new dg_ctx;
     c->device = device;
-    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; g_err = hipGetErrorString(e); return 1; }
+    c->stream = c->own_stream;
     static bool tables = false;
     if (!tables) {
         GllTables t[4];
@@ -307,7 +604,7 @@ int dg_destroy(dg_ctx* c) {
     DG_TRY(hipFree(c->flag));
     DG_TRY(hipEventDestroy(c->ev[0]));
     DG_TRY(hipEventDestroy(c->ev[1]));
-    DG_TRY(hipStreamDestroy(c->stream));
+    DG_TRY(hipStreamDestroy(c->own_stream));
     delete c;
     return 0;
 }
@@ -396,7 +693,7 @@ int dg_state_free(dg_ctx* c, dg_state* s) {
     if (!s) return 0;
     DG_TRY(hipSetDevice(c->device));
     DG_TRY(hipStreamSynchronize(c->stream));
-    DG_TRY(hipFree(s->d));
+    if (s->owned) DG_TRY(hipFree(s->d));
     delete s;
     return 0;
 }
@@ -589,6 +886,289 @@ int dg_copy_bench(dg_ctx* c, size_t bytes, int reps, double* ms_per_copy) {
     *ms_per_copy = ms / reps;
     DG_TRY(hipFree(src));
     DG_TRY(hipFree(dst));
+    return 0;
+}
+
+int dg_state_zero(dg_ctx* c, dg_state* s) {
+    DG_REQUIRE(c && s && c->K > 0, "dg_state_zero: null argument or no form");
+    DG_TRY(hipSetDevice(c->device));
+    return zero_async(c, s->d, (size_t)3 * c->K * c->nn);
+}
+
+int dg_state_wrap(dg_ctx* c, double* device_ptr, dg_state** s) {
+    DG_REQUIRE(c && device_ptr && s && c->K > 0, "dg_state_wrap: null argument or no form");
+    dg_state* st = new dg_state;
+    st->d = device_ptr;
+    st->owned = false;
+    *s = st;
+    return 0;
+}
+
+int dg_set_stream(dg_ctx* c, void* stream) {
+    DG_REQUIRE(c, "dg_set_stream: null context");
+    DG_TRY(hipStreamSynchronize(c->stream));
+    c->stream = stream ? (hipStream_t)stream : c->own_stream;
+    return 0;
+}
+
+int dg_set_bed(dg_ctx* c, const double* bed, int on_device) {
+    DG_REQUIRE(c && bed && c->K > 0, "dg_set_bed: null argument or no form");
+    DG_TRY(hipSetDevice(c->device));
+    DG_TRY(hipMemcpyAsync(c->bed, bed, (size_t)c->K * c->nn * 8, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                          c->stream));
+    DG_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dg_set_manning(dg_ctx* c, const double* manning, int on_device) {
+    DG_REQUIRE(c && manning && c->K > 0, "dg_set_manning: null argument or no form");
+    DG_TRY(hipSetDevice(c->device));
+    DG_TRY(hipMemcpyAsync(c->manning, manning, (size_t)c->K * 8,
+                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+    DG_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dg_param_grads_zero(dg_ctx* c) {
+    DG_REQUIRE(c && c->K > 0, "dg_param_grads_zero: no form");
+    DG_TRY(hipSetDevice(c->device));
+    if (int r = ensure_adjoint(c)) return r;
+    if (int r = zero_async(c, c->gbed, (size_t)c->K * c->nn)) return r;
+    return zero_async(c, c->gman, (size_t)c->K);
+}
+
+int dg_param_grads_read(dg_ctx* c, double* gbed, double* gman, int on_device) {
+    DG_REQUIRE(c && c->K > 0, "dg_param_grads_read: no form");
+    DG_TRY(hipSetDevice(c->device));
+    if (int r = ensure_adjoint(c)) return r;
+    const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    if (gbed) DG_TRY(hipMemcpyAsync(gbed, c->gbed, (size_t)c->K * c->nn * 8, kind, c->stream));
+    if (gman) DG_TRY(hipMemcpyAsync(gman, c->gman, (size_t)c->K * 8, kind, c->stream));
+    DG_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dg_rhs_vjp(dg_ctx* c, const dg_state* in, double t, const dg_state* cot_out, dg_state* cot_in) {
+    DG_REQUIRE(c && in && cot_out && cot_in && c->K > 0, "dg_rhs_vjp: null argument or no form");
+    DG_REQUIRE(cot_in != cot_out, "dg_rhs_vjp: cot_in must differ from cot_out");
+    DG_TRY(hipSetDevice(c->device));
+    if (int r = launch_stage_vjp(c, in->d, in->d, cot_out->d, nullptr, cot_in->d, 0.0, 0.0, 0.0, t, 1)) return r;
+    DG_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dg_stage_vjp(dg_ctx* c, const dg_state* base, const dg_state* input, const dg_stage_desc* st,
+                 const dg_state* cot_out, dg_state* cot_base, dg_state* cot_input) {
+    DG_REQUIRE(c && base && input && st && cot_out && cot_input && c->K > 0, "dg_stage_vjp: null argument or no form");
+    DG_REQUIRE(cot_input != cot_out && cot_base != cot_out, "dg_stage_vjp: the cotangent outputs must differ from cot_out");
+    DG_TRY(hipSetDevice(c->device));
+    if (int r = launch_stage_vjp(c, base->d, input->d, cot_out->d, cot_base ? cot_base->d : nullptr, cot_input->d, st->a,
+                                 st->b, st->c, st->time, 0))
+        return r;
+    DG_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dg_step_vjp(dg_ctx* c, const dg_state* s, double t, double dt, int scheme, const dg_state* cot_next,
+                dg_state* cot_state) {
+    DG_REQUIRE(c && s && cot_next && cot_state && c->K > 0, "dg_step_vjp: null argument or no form");
+    DG_REQUIRE(scheme == DG_SSP_RK3 || scheme == DG_SSP_RK43, "dg_step_vjp: unknown scheme");
+    DG_REQUIRE(cot_next != cot_state && cot_next->d != cot_state->d, "dg_step_vjp: cot_next must differ from cot_state");
+    DG_TRY(hipSetDevice(c->device));
+    if (int r = step_vjp_impl(c, s->d, t, dt, scheme, cot_next->d, cot_state->d)) return r;
+    DG_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dg_set_stations(dg_ctx* c, int n, const int64_t* element, const double* weights) {
+    DG_REQUIRE(c && c->K > 0 && n >= 0 && (n == 0 || (element && weights)), "dg_set_stations: bad argument or no form");
+    for (int s = 0; s < n; ++s)
+        DG_REQUIRE(element[s] >= 0 && element[s] < c->K, "dg_set_stations: station element out of range");
+    DG_TRY(hipSetDevice(c->device));
+    DG_TRY(hipStreamSynchronize(c->stream));
+    if (c->st_elem) DG_TRY(hipFree(c->st_elem));
+    if (c->st_w) DG_TRY(hipFree(c->st_w));
+    if (c->st_eta) DG_TRY(hipFree(c->st_eta));
+    c->st_elem = nullptr;
+    c->st_w = c->st_eta = nullptr;
+    c->n_st = n;
+    if (n == 0) return 0;
+    DG_TRY(hipMalloc(&c->st_elem, (size_t)n * sizeof(int64_t)));
+    DG_TRY(hipMalloc(&c->st_w, (size_t)n * c->nn * sizeof(double)));
+    DG_TRY(hipMalloc(&c->st_eta, (size_t)n * sizeof(double)));
+    DG_TRY(hipMemcpy(c->st_elem, element, (size_t)n * sizeof(int64_t), hipMemcpyHostToDevice));
+    DG_TRY(hipMemcpy(c->st_w, weights, (size_t)n * c->nn * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int dg_sample_stations(dg_ctx* c, const dg_state* s, double* eta) {
+    DG_REQUIRE(c && s && eta && c->n_st > 0, "dg_sample_stations: null argument or no stations");
+    DG_TRY(hipSetDevice(c->device));
+    hipLaunchKernelGGL(station_sample_kernel, dim3((c->n_st + 63) / 64), dim3(64), 0, c->stream, s->d, c->bed,
+                       c->st_elem, c->st_w, c->n_st, c->nn, c->st_eta);
+    DG_TRY(hipGetLastError());
+    DG_TRY(hipMemcpyAsync(eta, c->st_eta, (size_t)c->n_st * 8, hipMemcpyDeviceToHost, c->stream));
+    DG_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dg_tape_create(dg_ctx* c, int snapshots, dg_tape** out) {
+    DG_REQUIRE(c && out && c->K > 0, "dg_tape_create: null argument or no form");
+    DG_REQUIRE(snapshots >= 1, "dg_tape_create: at least one snapshot (the initial state)");
+    DG_TRY(hipSetDevice(c->device));
+    dg_tape* tp = new dg_tape;
+    tp->snapshots = snapshots;
+    tp->K = c->K;
+    tp->nn = c->nn;
+    const size_t bytes = (size_t)3 * c->K * c->nn * sizeof(double);
+    auto alloc = [&](dg_state*& st) -> int {
+        st = new dg_state;
+        DG_TRY(hipMalloc(&st->d, bytes));
+        return 0;
+    };
+    tp->slot.resize(snapshots, nullptr);
+    tp->used.assign(snapshots, 0);
+    for (auto& st : tp->slot)
+        if (int r = alloc(st)) return r;
+    if (int r = alloc(tp->cur)) return r;
+    if (int r = alloc(tp->lam[0])) return r;
+    if (int r = alloc(tp->lam[1])) return r;
+    *out = tp;
+    return 0;
+}
+
+int dg_tape_destroy(dg_ctx* c, dg_tape* tp) {
+    if (!tp) return 0;
+    DG_REQUIRE(c, "dg_tape_destroy: null context");
+    DG_TRY(hipSetDevice(c->device));
+    DG_TRY(hipStreamSynchronize(c->stream));
+    auto fr = [&](dg_state* st) -> int {
+        if (st) {
+            DG_TRY(hipFree(st->d));
+            delete st;
+        }
+        return 0;
+    };
+    for (auto* st : tp->slot)
+        if (int r = fr(st)) return r;
+    if (int r = fr(tp->cur)) return r;
+    if (int r = fr(tp->lam[0])) return r;
+    if (int r = fr(tp->lam[1])) return r;
+    if (tp->eta) DG_TRY(hipFree(tp->eta));
+    if (tp->obs) DG_TRY(hipFree(tp->obs));
+    delete tp;
+    return 0;
+}
+
+int dg_misfit_forward(dg_ctx* c, dg_tape* tp, const dg_state* x0, double t0, int n_steps, const double* dts, int scheme,
+                      int every, const double* obs, double* misfit) {
+    DG_REQUIRE(c && tp && x0 && dts && obs && misfit && c->K > 0, "dg_misfit_forward: null argument or no form");
+    DG_REQUIRE(tp->K == c->K && tp->nn == c->nn, "dg_misfit_forward: the tape belongs to another form");
+    DG_REQUIRE(n_steps >= 1 && every >= 1 && every <= n_steps, "dg_misfit_forward: need 1 <= every <= n_steps");
+    DG_REQUIRE(scheme == DG_SSP_RK3 || scheme == DG_SSP_RK43, "dg_misfit_forward: unknown scheme");
+    DG_REQUIRE(c->n_st > 0, "dg_misfit_forward: no stations (dg_set_stations)");
+    DG_TRY(hipSetDevice(c->device));
+    tp->ready = false;
+    tp->n = n_steps;
+    tp->scheme = scheme;
+    tp->every = every;
+    tp->n_st = c->n_st;
+    tp->n_samples = n_steps / every;
+    tp->dts.assign(dts, dts + n_steps);
+    tp->t_at.resize(n_steps);
+    double t = t0;
+    for (int i = 0; i < n_steps; ++i) {
+        tp->t_at[i] = t;
+        t += dts[i];
+    }
+    if (tp->eta) DG_TRY(hipFree(tp->eta));
+    if (tp->obs) DG_TRY(hipFree(tp->obs));
+    tp->eta = tp->obs = nullptr;
+    const size_t ns = (size_t)tp->n_samples * tp->n_st;
+    DG_TRY(hipMalloc(&tp->eta, ns * sizeof(double)));
+    DG_TRY(hipMalloc(&tp->obs, ns * sizeof(double)));
+    DG_TRY(hipMemcpy(tp->obs, obs, ns * sizeof(double), hipMemcpyHostToDevice));
+    // the schedule: the initial state in slot 0, the right-most descent's checkpoints
+    // taken by this sweep
+    tp->rv.init(n_steps, tp->snapshots - 1);
+    std::fill(tp->used.begin(), tp->used.end(), 0);
+    tp->pre.clear();
+    {
+        int a = 0, free = tp->snapshots - 1, sl = 1;
+        while (n_steps - a >= 2 && free > 0) {
+            a += tp->rv.split(n_steps - a, free, true);
+            tp->pre.push_back({a, sl});
+            tp->used[sl] = 1;
+            ++sl;
+            --free;
+        }
+    }
+    tp->used[0] = 1;
+    if (int r = tape_copy(c, tp->slot[0], x0)) return r;
+    if (int r = tape_copy(c, tp->cur, x0)) return r;
+    tp->fwd = 0;
+    tp->vjps = 0;
+    size_t pi = 0;
+    for (int i = 0; i < n_steps; ++i) {
+        if (int r = step_impl(c, tp->cur, tp->t_at[i], tp->dts[i], scheme)) return r;
+        tp->fwd += 1;
+        const int after = i + 1;
+        if (after % every == 0 && after / every <= tp->n_samples) {
+            hipLaunchKernelGGL(station_sample_kernel, dim3((tp->n_st + 63) / 64), dim3(64), 0, c->stream, tp->cur->d,
+                               c->bed, c->st_elem, c->st_w, tp->n_st, c->nn,
+                               tp->eta + (size_t)(after / every - 1) * tp->n_st);
+            DG_TRY(hipGetLastError());
+        }
+        if (pi < tp->pre.size() && tp->pre[pi].first == after) {
+            if (int r = tape_copy(c, tp->slot[tp->pre[pi].second], tp->cur)) return r;
+            ++pi;
+        }
+    }
+    std::vector<double> eta(ns);
+    DG_TRY(hipMemcpyAsync(eta.data(), tp->eta, ns * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    DG_TRY(hipStreamSynchronize(c->stream));
+    double J = 0.0;
+    for (size_t q = 0; q < ns; ++q) {
+        const double d = eta[q] - obs[q];
+        J += d * d;
+    }
+    *misfit = J;
+    tp->ready = true;
+    return 0;
+}
+
+int dg_misfit_backward(dg_ctx* c, dg_tape* tp, double scale, dg_state* cot_x0) {
+    DG_REQUIRE(c && tp && c->K > 0, "dg_misfit_backward: null argument or no form");
+    DG_REQUIRE(tp->ready, "dg_misfit_backward: run dg_misfit_forward first (once per backward)");
+    DG_REQUIRE(c->n_st == tp->n_st, "dg_misfit_backward: the stations changed since the forward");
+    DG_TRY(hipSetDevice(c->device));
+    tp->ready = false;
+    if (int r = ensure_adjoint(c)) return r;
+    if (int r = zero_async(c, c->gbed, (size_t)c->K * c->nn)) return r;
+    if (int r = zero_async(c, c->gman, (size_t)c->K)) return r;
+    tp->lam_cur = 0;
+    if (int r = zero_async(c, tp->lam[0]->d, (size_t)3 * c->K * c->nn)) return r;
+    tp->scale = scale;
+    if (int r = tape_reverse(c, tp, 0, tp->n, 0, tp->snapshots - 1, true)) return r;
+    if (cot_x0)
+        DG_TRY(hipMemcpyAsync(cot_x0->d, tp->lam[tp->lam_cur]->d, (size_t)3 * c->K * c->nn * 8,
+                              hipMemcpyDeviceToDevice, c->stream));
+    DG_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dg_tape_stats(const dg_tape* tp, int64_t* forward_steps, int64_t* step_vjps) {
+    DG_REQUIRE(tp, "dg_tape_stats: null tape");
+    if (forward_steps) *forward_steps = tp->fwd;
+    if (step_vjps) *step_vjps = tp->vjps;
+    return 0;
+}
+
+int dg_revolve_count(int n_steps, int snapshots, int64_t* forward_steps) {
+    DG_REQUIRE(n_steps >= 1 && snapshots >= 1 && forward_steps, "dg_revolve_count: bad argument");
+    Revolve rv;
+    rv.init(n_steps, snapshots - 1);
+    *forward_steps = n_steps + revolve_count_rec(rv, n_steps, snapshots - 1, true);
     return 0;
 }
 

@@ -270,8 +270,17 @@ __device__ __forceinline__ bool outer_node(const StageParams& p, int64_t k, cons
 // SUB: the subcell volume term (elements with a node below h_dry); otherwise
 // the flux-differencing one, and an element with a shallow node is handed to
 // the subcell queue instead.
-template <int N, bool SUB>
-__device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, bool active) {
+}  // namespace dgswe
+
+#include "swe_vjp.hpp"
+
+namespace dgswe {
+
+// VJP: the same forward up to the element RHS, then the reverse (vjp_tail)
+// instead of the post-processing and the stores.
+template <int N, bool SUB, bool VJP = false>
+__device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, bool active,
+                                            const VjpParams* vp = nullptr) {
     constexpr int N1 = N + 1, NN = N1 * N1, EPB = 64 / NN, NF = 4 * N1;
     constexpr int NT = (NF + NN - 1) / NN;  // face tasks per lane
     const GllTables& T = c_gll[N];
@@ -339,6 +348,9 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
                 if (kq < p.n_elements && sflag[q] != 0) p.queue[(size_t)(k / EPB) * EPB + cnt++] = (uint32_t)kq;
             }
             p.queue_count[k / EPB] = (uint8_t)cnt;
+        }
+        if constexpr (VJP) {
+            if (active && i == 0) vp->kind[k] = (uint8_t)(sflag[e] != 0);
         }
         if (active && sflag[e] != 0) active = false;
     }
@@ -492,6 +504,11 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             rhs.hu += p.f0 * hvin;
             rhs.hv += (-p.f0) * huin;
         }
+    }
+    if constexpr (VJP) {
+        vjp_tail<N, SUB>(*vp, k, active, c, affine, det, det_inv, hin, huin, hvin, bin, lk, tg, nh, nhu, nhv, nbd, rhs,
+                         sh, shu, shv, su, sv, sb, srx, sry, ssx, ssy, rl, rr);
+        return;
     }
     if (p.rhs_only) {
         if (active) { p.oh[n] = rhs.h; p.ohu[n] = rhs.hu; p.ohv[n] = rhs.hv; }
@@ -648,6 +665,30 @@ __global__ __launch_bounds__(64) void subcell_stage_kernel(StageParams p, const 
     for (size_t first = (size_t)blockIdx.x * EPB; first < count; first += (size_t)gridDim.x * EPB) {
         const bool active = e < EPB && first + e < count;
         stage_block<N, true>(p, active ? (int64_t)list[first + e] : 0, active);
+        lds_sync();
+    }
+}
+
+// The VJP of a stage: wet elements first (the others are listed as in the
+// forward), then the listed elements, grid-stride.
+template <int N>
+__global__ __launch_bounds__(64) void wet_vjp_kernel(VjpParams P) {
+    constexpr int NN = (N + 1) * (N + 1), EPB = 64 / NN;
+    const int64_t blk = xcd_block(blockIdx.x, P.s.xcd_stride);
+    if (blk * EPB >= P.s.n_elements) return;
+    const int e = threadIdx.x / NN;
+    const int64_t k = blk * EPB + (e < EPB ? e : 0);
+    stage_block<N, false, true>(P.s, k, e < EPB && blk * EPB + e < P.s.n_elements, &P);
+}
+
+template <int N>
+__global__ __launch_bounds__(64) void subcell_vjp_kernel(VjpParams P, const uint32_t* list, const unsigned int* count_ptr) {
+    constexpr int NN = (N + 1) * (N + 1), EPB = 64 / NN;
+    const unsigned int count = *count_ptr;
+    const int e = threadIdx.x / NN;
+    for (size_t first = (size_t)blockIdx.x * EPB; first < count; first += (size_t)gridDim.x * EPB) {
+        const bool active = e < EPB && first + e < count;
+        stage_block<N, true, true>(P.s, active ? (int64_t)list[first + e] : 0, active, &P);
         lds_sync();
     }
 }
@@ -838,6 +879,41 @@ __global__ void copy_kernel(const double4* __restrict__ src, double4* __restrict
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) dst[i] = src[i];
+}
+
+__global__ void zero_kernel(double* x, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) x[i] = 0.0;
+}
+
+// Water level at stations (Probe2D, src/solver/probe.rs:1-30): eta_s = sum_i w_si (h + B)
+// over the nodes of the station's element, one thread per station
+__global__ void station_sample_kernel(const double* h, const double* bed, const int64_t* elem, const double* w,
+                                      int n_st, int nn, double* eta) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_st) return;
+    const size_t n0 = (size_t)elem[s] * nn;
+    double v = 0.0;
+    for (int i = 0; i < nn; ++i) v += w[(size_t)s * nn + i] * (h[n0 + i] + bed[n0 + i]);
+    eta[s] = v;
+}
+
+// The misfit's cotangent at a sample: 2 scale (eta_s - obs_s) w_si into the depth and
+// the bed of each station's nodes, stations in order by one thread (stations may
+// share an element)
+__global__ void station_seed_kernel(const int64_t* elem, const double* w, int n_st, int nn, const double* eta,
+                                    const double* obs, double scale, double* gh, double* gbed) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int s = 0; s < n_st; ++s) {
+        const double r = 2.0 * scale * (eta[s] - obs[s]);
+        const size_t n0 = (size_t)elem[s] * nn;
+        for (int i = 0; i < nn; ++i) {
+            const double v = r * w[(size_t)s * nn + i];
+            gh[n0 + i] += v;
+            gbed[n0 + i] += v;
+        }
+    }
 }
 
 

@@ -33,6 +33,10 @@ EXPORTS = (
     "dg_state_free", "dg_state_upload", "dg_state_download", "dg_state_copy", "dg_rhs", "dg_face_fluxes", "dg_stage",
     "dg_element_dt", "dg_dt_ssp", "dg_step", "dg_run", "dg_is_finite", "dg_clip_count", "dg_masks",
     "dg_timing_enable", "dg_timing_read", "dg_sync", "dg_copy_bench",
+    "dg_state_zero", "dg_state_wrap", "dg_set_stream", "dg_set_bed", "dg_set_manning", "dg_param_grads_zero",
+    "dg_param_grads_read", "dg_rhs_vjp", "dg_stage_vjp", "dg_step_vjp", "dg_set_stations", "dg_sample_stations",
+    "dg_tape_create", "dg_tape_destroy", "dg_misfit_forward", "dg_misfit_backward", "dg_tape_stats",
+    "dg_revolve_count",
 )
 
 
@@ -95,6 +99,26 @@ def load(fma: bool = False):
     L.dg_timing_read.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_int64), _dp, ctypes.POINTER(ctypes.c_int64)]
     L.dg_sync.argtypes = [vp]
     L.dg_copy_bench.argtypes = [vp, ctypes.c_size_t, ctypes.c_int, _dp]
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    L.dg_state_zero.argtypes = [vp, vp]
+    L.dg_state_wrap.argtypes = [vp, vp, ctypes.POINTER(vp)]
+    L.dg_set_stream.argtypes = [vp, vp]
+    L.dg_set_bed.argtypes = [vp, vp, ctypes.c_int]
+    L.dg_set_manning.argtypes = [vp, vp, ctypes.c_int]
+    L.dg_param_grads_zero.argtypes = [vp]
+    L.dg_param_grads_read.argtypes = [vp, vp, vp, ctypes.c_int]
+    L.dg_rhs_vjp.argtypes = [vp, vp, ctypes.c_double, vp, vp]
+    L.dg_stage_vjp.argtypes = [vp, vp, vp, ctypes.POINTER(StageDesc), vp, vp, vp]
+    L.dg_step_vjp.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, vp]
+    L.dg_set_stations.argtypes = [vp, ctypes.c_int, i64p, _dp]
+    L.dg_sample_stations.argtypes = [vp, vp, _dp]
+    L.dg_tape_create.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
+    L.dg_tape_destroy.argtypes = [vp, vp]
+    L.dg_misfit_forward.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp,
+                                    _dp]
+    L.dg_misfit_backward.argtypes = [vp, vp, ctypes.c_double, vp]
+    L.dg_tape_stats.argtypes = [vp, i64p, i64p]
+    L.dg_revolve_count.argtypes = [ctypes.c_int, ctypes.c_int, i64p]
     _libs[key] = L
     return L
 
@@ -108,13 +132,31 @@ def _p(a):
     return a.ctypes.data_as(_dp)
 
 
-class DeviceState:
-    """A device-resident state (h, hu, hv), SoA FP64."""
+def revolve_count(n_steps: int, snapshots: int, fma: bool = False) -> int:
+    """Forward steps (the first sweep included) of the checkpoint schedule for
+    n_steps with `snapshots` stored states (host only, no device needed)."""
+    L = load(fma)
+    v = ctypes.c_int64()
+    _check(L, L.dg_revolve_count(n_steps, snapshots, ctypes.byref(v)), "dg_revolve_count")
+    return int(v.value)
 
-    def __init__(self, physics: "SWEPhysics2D"):
+
+class DeviceState:
+    """A device-resident state (h, hu, hv), SoA FP64; with `device_ptr`, a view of
+    caller memory (3*K*NN doubles, e.g. a torch tensor) that is not freed here."""
+
+    def __init__(self, physics: "SWEPhysics2D", device_ptr=None):
         self.physics = physics
         self.handle = ctypes.c_void_p()
-        _check(physics.lib, physics.lib.dg_state_alloc(physics.ctx, ctypes.byref(self.handle)), "dg_state_alloc")
+        if device_ptr is None:
+            _check(physics.lib, physics.lib.dg_state_alloc(physics.ctx, ctypes.byref(self.handle)), "dg_state_alloc")
+        else:
+            _check(physics.lib, physics.lib.dg_state_wrap(physics.ctx, ctypes.c_void_p(device_ptr),
+                                                          ctypes.byref(self.handle)), "dg_state_wrap")
+
+    def zero(self):
+        _check(self.physics.lib, self.physics.lib.dg_state_zero(self.physics.ctx, self.handle), "dg_state_zero")
+        return self
 
     def upload(self, h, hu, hv):
         L = self.physics.lib
@@ -281,6 +323,129 @@ class SWEPhysics2D:
         ms = ctypes.c_double()
         _check(self.lib, self.lib.dg_copy_bench(self.ctx, nbytes, reps, ctypes.byref(ms)), "dg_copy_bench")
         return ms.value
+
+    # --- discrete adjoint ---
+    def wrap(self, device_ptr: int) -> DeviceState:
+        """A state over caller device memory of 3*K*NN doubles (h, hu, hv)."""
+        return DeviceState(self, device_ptr)
+
+    def set_stream(self, stream_handle):
+        """Run on the caller's HIP stream (an integer handle), or the own stream (None)."""
+        _check(self.lib, self.lib.dg_set_stream(self.ctx, ctypes.c_void_p(stream_handle or 0)), "dg_set_stream")
+
+    def set_bed(self, bed=None, device_ptr=None):
+        if device_ptr is not None:
+            _check(self.lib, self.lib.dg_set_bed(self.ctx, ctypes.c_void_p(device_ptr), 1), "dg_set_bed")
+            return
+        b = np.ascontiguousarray(bed, dtype=np.float64)
+        if b.size != self.n_nodes_total:
+            raise ValueError("bed must have K*(N+1)^2 values")
+        _check(self.lib, self.lib.dg_set_bed(self.ctx, b.ctypes.data, 0), "dg_set_bed")
+
+    def set_manning(self, manning=None, device_ptr=None):
+        if device_ptr is not None:
+            _check(self.lib, self.lib.dg_set_manning(self.ctx, ctypes.c_void_p(device_ptr), 1), "dg_set_manning")
+            return
+        m = np.ascontiguousarray(manning, dtype=np.float64)
+        if m.size != self.K:
+            raise ValueError("manning must have K values")
+        _check(self.lib, self.lib.dg_set_manning(self.ctx, m.ctypes.data, 0), "dg_set_manning")
+
+    def param_grads_zero(self):
+        _check(self.lib, self.lib.dg_param_grads_zero(self.ctx), "dg_param_grads_zero")
+
+    def param_grads(self):
+        """The accumulated (bed, Manning n) cotangents, on the host."""
+        gb = np.empty(self.n_nodes_total)
+        gm = np.empty(self.K)
+        _check(self.lib, self.lib.dg_param_grads_read(self.ctx, gb.ctypes.data, gm.ctypes.data, 0),
+               "dg_param_grads_read")
+        return gb, gm
+
+    def param_grads_to_device(self, bed_ptr: int, manning_ptr: int):
+        _check(self.lib, self.lib.dg_param_grads_read(self.ctx, ctypes.c_void_p(bed_ptr), ctypes.c_void_p(manning_ptr),
+                                                      1), "dg_param_grads_read")
+
+    def rhs_vjp(self, state: DeviceState, t: float, cot_out: DeviceState, cot_in: DeviceState):
+        _check(self.lib, self.lib.dg_rhs_vjp(self.ctx, state.handle, t, cot_out.handle, cot_in.handle), "dg_rhs_vjp")
+
+    def stage_vjp(self, base: DeviceState, input: DeviceState, a, b, c, time, cot_out: DeviceState,
+                  cot_base, cot_input: DeviceState):
+        d = StageDesc(a, b, c, time)
+        _check(self.lib, self.lib.dg_stage_vjp(self.ctx, base.handle, input.handle, ctypes.byref(d), cot_out.handle,
+                                               None if cot_base is None else cot_base.handle, cot_input.handle),
+               "dg_stage_vjp")
+
+    def step_vjp(self, state: DeviceState, t: float, dt: float, cot_next: DeviceState, cot_state: DeviceState,
+                 scheme: int = SSP_RK3):
+        _check(self.lib, self.lib.dg_step_vjp(self.ctx, state.handle, t, dt, scheme, cot_next.handle,
+                                              cot_state.handle), "dg_step_vjp")
+
+    def set_stations(self, elements, weights):
+        e = np.ascontiguousarray(elements, dtype=np.int64)
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        if w.size != e.size * self.nn:
+            raise ValueError("weights must be [stations][(N+1)^2]")
+        self.n_stations = e.size
+        _check(self.lib, self.lib.dg_set_stations(self.ctx, e.size, e.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                                  _p(w)), "dg_set_stations")
+
+    def sample_stations(self, state: DeviceState) -> np.ndarray:
+        out = np.empty(self.n_stations)
+        _check(self.lib, self.lib.dg_sample_stations(self.ctx, state.handle, _p(out)), "dg_sample_stations")
+        return out
+
+    def tape(self, snapshots: int) -> "Tape":
+        return Tape(self, snapshots)
+
+
+class Tape:
+    """Checkpoints of a station-misfit run for its reverse sweep (binomial schedule)."""
+
+    def __init__(self, physics: SWEPhysics2D, snapshots: int):
+        self.physics = physics
+        self.snapshots = snapshots
+        self.handle = ctypes.c_void_p()
+        _check(physics.lib, physics.lib.dg_tape_create(physics.ctx, snapshots, ctypes.byref(self.handle)),
+               "dg_tape_create")
+
+    def forward(self, x0: DeviceState, t0: float, dts, every: int, obs, scheme: int = SSP_RK3) -> float:
+        """J = sum (eta - obs)^2 over the samples after every `every`-th step; obs is
+        [n_steps // every][stations]."""
+        L = self.physics.lib
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        obs = np.ascontiguousarray(obs, dtype=np.float64)
+        if obs.size != (dts.size // every) * self.physics.n_stations:
+            raise ValueError("obs must be [n_steps // every][stations]")
+        J = ctypes.c_double()
+        _check(L, L.dg_misfit_forward(self.physics.ctx, self.handle, x0.handle, t0, dts.size, _p(dts), scheme, every,
+                                      _p(obs), ctypes.byref(J)), "dg_misfit_forward")
+        return J.value
+
+    def backward(self, scale: float = 1.0, cot_x0: DeviceState = None):
+        """Fills the parameter cotangents with scale * dJ/d(bed, n) (and cot_x0 with
+        scale * dJ/dX0); returns (bed gradient, Manning gradient) on the host."""
+        L = self.physics.lib
+        _check(L, L.dg_misfit_backward(self.physics.ctx, self.handle, scale, None if cot_x0 is None else cot_x0.handle),
+               "dg_misfit_backward")
+        return self.physics.param_grads()
+
+    def stats(self):
+        f, v = ctypes.c_int64(), ctypes.c_int64()
+        _check(self.physics.lib, self.physics.lib.dg_tape_stats(self.handle, ctypes.byref(f), ctypes.byref(v)),
+               "dg_tape_stats")
+        return int(f.value), int(v.value)
+
+    def close(self):
+        if self.handle and self.physics.ctx:
+            self.physics.lib.dg_tape_destroy(self.physics.ctx, self.handle)
+        self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Simulation:

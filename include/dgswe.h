@@ -136,6 +136,69 @@ int dg_timing_read(dg_ctx* ctx, double* stage_ms, int64_t* stage_launches, doubl
 int dg_sync(dg_ctx* ctx);
 int dg_copy_bench(dg_ctx* ctx, size_t bytes, int reps, double* ms_per_copy);
 
+/* ------------------------------------------------------------------------
+ * Discrete adjoint. The reference has no adjoint of this path: these calls are
+ * the vector-Jacobian products of the calls above, for a torch
+ * autograd.Function (forward = dg_stage / dg_step, backward = the VJP) and for
+ * gradients of a station misfit with respect to the nodal bed and per-element
+ * Manning n. They are exact for the arithmetic as written: at a kink (a
+ * max/min tie, a branch of the HLL speeds, the limiter, the wet/dry switches)
+ * the derivative is that of the branch the forward takes, and time steps are
+ * held at the values given. Cotangents are device states; the parameter
+ * cotangents accumulate in the context until dg_param_grads_zero.
+ * ------------------------------------------------------------------------ */
+typedef struct dg_tape dg_tape;
+
+/* state = 0 */
+int dg_state_zero(dg_ctx* ctx, dg_state* state);
+/* A state over caller-owned device memory of 3*K*NN doubles (h, hu, hv), for
+ * tensors of a framework; dg_state_free releases the handle, not the memory. */
+int dg_state_wrap(dg_ctx* ctx, double* device_ptr, dg_state** state);
+/* Run the context's work on the caller's HIP stream (NULL: its own stream). */
+int dg_set_stream(dg_ctx* ctx, void* hip_stream);
+/* Replace the bed [K*NN] or the per-element Manning n [K] (host or device memory). */
+int dg_set_bed(dg_ctx* ctx, const double* bed, int on_device);
+int dg_set_manning(dg_ctx* ctx, const double* manning, int on_device);
+/* The accumulated cotangents of the bed [K*NN] and of Manning n [K]. */
+int dg_param_grads_zero(dg_ctx* ctx);
+int dg_param_grads_read(dg_ctx* ctx, double* bed_grad, double* manning_grad, int on_device);
+
+/* VJP of dg_rhs: cot_in += (dL/din)^T cot_out; the bed's part accumulates. */
+int dg_rhs_vjp(dg_ctx* ctx, const dg_state* in, double t, const dg_state* cot_out, dg_state* cot_in);
+/* VJP of dg_stage: cot_base += (dout/dbase)^T cot_out (cot_base may be NULL),
+ * cot_input += (dout/dinput)^T cot_out; cot_base may be cot_input (a stage
+ * whose base is its input); neither may be cot_out. Bed and Manning parts
+ * accumulate in the context. */
+int dg_stage_vjp(dg_ctx* ctx, const dg_state* base, const dg_state* input, const dg_stage_desc* stage,
+                 const dg_state* cot_out, dg_state* cot_base, dg_state* cot_input);
+/* VJP of dg_step (the stage values are recomputed from state): cot_state =
+ * (dX_{n+1}/dX_n)^T cot_next, overwritten; cot_next may not be cot_state. */
+int dg_step_vjp(dg_ctx* ctx, const dg_state* state, double t, double dt, int scheme, const dg_state* cot_next,
+                dg_state* cot_state);
+
+/* Stations (Probe2D, src/solver/probe.rs:1-30): the water level
+ * eta = sum_i w_i (h_i + B_i) over the nodes of a station's element, with w the
+ * element polynomial's Lagrange weights at the station point. */
+int dg_set_stations(dg_ctx* ctx, int n, const int64_t* element, const double* weights /* [n][NN] */);
+int dg_sample_stations(dg_ctx* ctx, const dg_state* state, double* eta /* host [n] */);
+
+/* Station misfit J = sum over samples m and stations s of (eta_ms - obs_ms)^2,
+ * sampled after every `every`-th step of an n_steps run at the given dts, and its
+ * gradient by reverse sweeps over binomial checkpoints: the tape holds
+ * `snapshots` states (the initial one included). dg_misfit_forward runs the
+ * forward, keeping the checkpoints of the schedule; dg_misfit_backward zeroes the
+ * parameter cotangents and fills them with scale * dJ/d(bed, n) (read them with
+ * dg_param_grads_read); cot_x0 (optional) receives scale * dJ/dX_0. */
+int dg_tape_create(dg_ctx* ctx, int snapshots, dg_tape** tape);
+int dg_tape_destroy(dg_ctx* ctx, dg_tape* tape);
+int dg_misfit_forward(dg_ctx* ctx, dg_tape* tape, const dg_state* x0, double t0, int n_steps, const double* dts,
+                      int scheme, int every, const double* obs /* [n_steps/every][stations] */, double* misfit);
+int dg_misfit_backward(dg_ctx* ctx, dg_tape* tape, double scale, dg_state* cot_x0);
+/* Forward steps and step VJPs the last forward/backward pair took, and the
+ * forward steps a schedule takes (host only). */
+int dg_tape_stats(const dg_tape* tape, int64_t* forward_steps, int64_t* step_vjps);
+int dg_revolve_count(int n_steps, int snapshots, int64_t* forward_steps);
+
 #ifdef __cplusplus
 }
 #endif

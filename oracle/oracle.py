@@ -103,6 +103,7 @@ def _bind(L):
         L.oc_tlm_set_param_tangents.argtypes = [ctypes.c_void_p, _dp, _dp]
         _pp = ctypes.POINTER(_dp)
         L.oc_tlm_stage.argtypes = [ctypes.c_void_p, _pp, _pp, _pp, _pp] + [ctypes.c_double] * 4 + [ctypes.c_int, _pp, _pp]
+        L.oc_tlm_rhs.argtypes = [ctypes.c_void_p, _pp, _pp, ctypes.c_double, _pp, _pp]
         L.oc_tlm_run.argtypes = [ctypes.c_void_p, _pp, _pp, ctypes.c_double, ctypes.c_int, _dp, ctypes.c_int]
         L.oc_tlm_run.restype = ctypes.c_double
         L.oc_run_dts.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, ctypes.c_int]
@@ -286,6 +287,16 @@ class TangentLinear:
         po, _ = _ptrs(out)
         pdo, _ = _ptrs(dout)
         self._L.oc_tlm_stage(self.h, pb, pdb, pi, pdi, a, b, c, time, int(last), po, pdo)
+        return out, dout
+
+    def rhs(self, inp, dinp, time):
+        pi, ki = _ptrs(inp)
+        pdi, kdi = _ptrs(dinp)
+        out = [np.empty(self.K * self.nn) for _ in range(3)]
+        dout = [np.empty(self.K * self.nn) for _ in range(3)]
+        po, _ = _ptrs(out)
+        pdo, _ = _ptrs(dout)
+        self._L.oc_tlm_rhs(self.h, pi, pdi, time, po, pdo)
         return out, dout
 
     def run(self, x, dx, t, dts, scheme=0):

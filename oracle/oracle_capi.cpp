@@ -204,6 +204,17 @@ void oc_tlm_stage(void* p, const double* const* base, const double* const* dbase
     for (int v = 0; v < 3; ++v) unpack(O[v], out[v], dout[v]);
 }
 
+// The element RHS on dual numbers
+void oc_tlm_rhs(void* p, const double* const* input, const double* const* dinput, double time, double* const* out,
+                double* const* dout) {
+    Solver<Dual>& s = *((TlmHandle*)p)->s;
+    size_t n = (size_t)s.K * s.ops.nn;
+    std::vector<Dual> I[3], O[3];
+    for (int v = 0; v < 3; ++v) { pack(I[v], input[v], dinput[v], n); O[v].resize(n); }
+    s.rhs(I[0].data(), I[1].data(), I[2].data(), time, O[0].data(), O[1].data(), O[2].data());
+    for (int v = 0; v < 3; ++v) unpack(O[v], out[v], dout[v]);
+}
+
 // n_steps SSP steps at the prescribed dts, in place on (value, tangent) states
 double oc_tlm_run(void* p, double* const* x, double* const* dx, double t, int n_steps, const double* dts, int scheme) {
     TlmHandle* hd = (TlmHandle*)p;

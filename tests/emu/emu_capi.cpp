@@ -54,6 +54,61 @@ unsigned long long emu_stage(int order, int64_t K, const double* corners, const 
     return clips;
 }
 
+// The stage VJP: gbase (if not null) and ginput are added to; gbed and gman too.
+void emu_stage_vjp(int order, int64_t K, const double* corners, const uint32_t* links, const uint8_t* tags,
+                   const uint8_t* affine, const double* bed, const double* manning, const double* base,
+                   const double* input, const double* w, double* gbase, double* ginput, double* gbed, double* gman,
+                   double a, double b, double c, double eta_e, double g, double h_min, double h_dry, double u_max,
+                   double tau, double f0, int rhs_only, int flags) {
+    size_t nn = (size_t)(order + 1) * (order + 1), nt = (size_t)K * nn, nf = 4 * (size_t)(order + 1);
+    unsigned long long clips = 0;
+    VjpParams P;
+    StageParams& p = P.s;
+    p.bh = base; p.bhu = base + nt; p.bhv = base + 2 * nt;
+    p.ih = input; p.ihu = input + nt; p.ihv = input + 2 * nt;
+    p.oh = p.ohu = p.ohv = nullptr;
+    p.bed = bed; p.corners = corners; p.manning = manning; p.links = links; p.tags = tags; p.affine = affine;
+    p.n_elements = K; p.a = a; p.b = b; p.c = c; p.eta_e = eta_e;
+    p.g = g; p.h_min = h_min; p.h_dry = h_dry; p.u_max = u_max; p.tau = tau; p.f0 = f0;
+    p.rhs_only = rhs_only; p.flags = flags; p.clips = &clips; p.fstar_out = nullptr;
+    P.wh = w; P.whu = w + nt; P.whv = w + 2 * nt;
+    P.gbh = gbase; P.gbhu = gbase ? gbase + nt : nullptr; P.gbhv = gbase ? gbase + 2 * nt : nullptr;
+    P.gih = ginput; P.gihu = ginput + nt; P.gihv = ginput + 2 * nt;
+    P.gbed = gbed; P.gman = gman;
+    std::vector<double> fb((size_t)K * nf * 4, 0.0), ib((size_t)K * nf * 4, 0.0);
+    std::vector<uint8_t> kind((size_t)K, 0);
+    P.facebuf = fb.data(); P.innerbuf = ib.data(); P.kind = kind.data();
+    int epb = 64 / (int)nn;
+    unsigned nblk = (unsigned)((K + epb - 1) / epb);
+    unsigned nchunk = (nblk + 1023) / 1024;
+    std::vector<uint32_t> queue((size_t)nblk * epb), list((size_t)K);
+    std::vector<uint8_t> qcount(nblk);
+    std::vector<unsigned int> ctot(nchunk);
+    unsigned int lcount = 0;
+    p.queue = queue.data();
+    p.queue_count = qcount.data();
+    p.xcd_stride = (nblk + 7) / 8;
+    unsigned grid = 8 * p.xcd_stride, sub_grid = std::min(nblk, 5u);
+    switch (order) {
+        case 1: emu_launch(grid, 64, [&] { wet_vjp_kernel<1>(P); }); break;
+        case 2: emu_launch(grid, 64, [&] { wet_vjp_kernel<2>(P); }); break;
+        default: emu_launch(grid, 64, [&] { wet_vjp_kernel<3>(P); }); break;
+    }
+    emu_launch(nchunk, 256, [&] { chunk_count_kernel(qcount.data(), nblk, ctot.data()); });
+    emu_launch(nchunk, 256, [&] { compact_kernel(qcount.data(), queue.data(), nblk, ctot.data(), epb, list.data(), &lcount); });
+    switch (order) {
+        case 1: emu_launch(sub_grid, 64, [&] { subcell_vjp_kernel<1>(P, list.data(), &lcount); }); break;
+        case 2: emu_launch(sub_grid, 64, [&] { subcell_vjp_kernel<2>(P, list.data(), &lcount); }); break;
+        default: emu_launch(sub_grid, 64, [&] { subcell_vjp_kernel<3>(P, list.data(), &lcount); }); break;
+    }
+    unsigned ggrid = (unsigned)((nt + 255) / 256);
+    switch (order) {
+        case 1: emu_launch(ggrid, 256, [&] { vjp_gather_kernel<1>(links, fb.data(), ib.data(), kind.data(), K, ginput, ginput + nt, ginput + 2 * nt, gbed); }); break;
+        case 2: emu_launch(ggrid, 256, [&] { vjp_gather_kernel<2>(links, fb.data(), ib.data(), kind.data(), K, ginput, ginput + nt, ginput + 2 * nt, gbed); }); break;
+        default: emu_launch(ggrid, 256, [&] { vjp_gather_kernel<3>(links, fb.data(), ib.data(), kind.data(), K, ginput, ginput + nt, ginput + 2 * nt, gbed); }); break;
+    }
+}
+
 double emu_dt(int order, int64_t K, const double* state, const double* corners, const uint32_t* links,
               const uint8_t* affine, double cfl, double g, double h_min, double h_dry, double pcfl, double max_cfl,
               double max_cfl_sub, int positivity, double* element_dt) {

@@ -22,6 +22,10 @@ def lib():
         L.emu_dt.restype = ctypes.c_double
         L.emu_dt.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, _dp, ctypes.c_void_p, ctypes.c_void_p] + \
             [ctypes.c_double] * 7 + [ctypes.c_int, _dp]
+        L.emu_stage_vjp.restype = None
+        L.emu_stage_vjp.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp] + \
+            [ctypes.c_double] * 10 + [ctypes.c_int, ctypes.c_int]
         L.emu_init()
         _lib = L
     return _lib
@@ -88,3 +92,20 @@ class EmulatedKernels:
                           self.affine.ctypes.data, cfl, self.case.g, self.case.h_min, self.case.h_dry, pcfl,
                           max(pcfl, 0.9 * lin), max(pcfl, 0.9 * sub), int(bool(self.flags & 2)), self._p(out))
         return out, dt
+
+    def stage_vjp(self, base, inp, w, a, b, c, time, rhs_only=False):
+        """Cotangents of (base, input, bed, Manning n) of a stage for the output cotangent w."""
+        c_ = self.case
+        B = np.concatenate(base)
+        I = np.concatenate(inp)
+        W = np.concatenate(w)
+        gb = np.zeros_like(B)
+        gi = np.zeros_like(I)
+        gbed = np.zeros(self.bed.size)
+        gman = np.zeros(self.K)
+        eta = c_.tide.elevation(time)
+        lib().emu_stage_vjp(self.order, self.K, self._p(self.corners), self.links.ctypes.data, self.tags.ctypes.data,
+                            self.affine.ctypes.data, self._p(self.bed), self._p(self.manning), self._p(B), self._p(I),
+                            self._p(W), self._p(gb), self._p(gi), self._p(gbed), self._p(gman), a, b, c, eta, c_.g,
+                            c_.h_min, c_.h_dry, 20.0, 1.0, c_.f0, int(rhs_only), self.flags)
+        return np.split(gb, 3), np.split(gi, 3), gbed, gman

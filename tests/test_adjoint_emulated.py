@@ -1,0 +1,95 @@
+"""The stage kernel's VJP instantiation, run on host threads, against the
+oracle's tangent-linear model (its solver on dual numbers): the dot-product
+test <w, J d> = <J^T w, d> for random output cotangents w and random
+directions d of the stage's base, input, nodal bed and per-element Manning n.
+The two sides are computed by different code (hand-written reverse kernels;
+forward-mode dual arithmetic of the CPU restatement), so agreement to rounding
+pins the adjoint of every branch the inputs reach: shallow (subcell) elements,
+the tide and wall faces, friction and the thin-layer rate, and the theta
+limiter of the wet/dry correction (exercised by the drained-shore inputs).
+"""
+import numpy as np
+import pytest
+
+from dgcom_amd import cases
+import oracle as O
+from cases_for_tests import perturbed, small_unstructured
+from emu_bridge import EmulatedKernels
+
+TOL = 1e-12
+
+STAGE_CASES = [
+    ("estuary_n2", lambda: cases.tidal_estuary(nx=6, ny=3)),
+    ("beach_n1", lambda: cases.drying_beach(order=1, nx=8, ny=3)),
+    ("beach_n2", lambda: cases.drying_beach(order=2, nx=8, ny=3)),
+    ("beach_n3_jitter", lambda: cases.drying_beach(order=3, nx=8, ny=3, jitter=0.2)),
+    ("unstructured_n3", lambda: cases.large_unstructured(n=6, length=600.0, order=3, correlation=300.0, sd=20.0,
+                                                         mean_depth=15.0)),
+]
+
+
+def _dot(a, b):
+    return sum(float(np.dot(x, y)) for x, y in zip(a, b))
+
+
+def dot_test(c, base, inp, a, b, cc, time, rhs_only, seed):
+    emu = EmulatedKernels(c)
+    rng = np.random.default_rng(seed)
+    n = base[0].size
+    w = [rng.standard_normal(n) for _ in range(3)]
+    gb, gi, gbed, gman = emu.stage_vjp(base, inp, w, a, b, cc, time, rhs_only=rhs_only)
+    zero = [np.zeros(n)] * 3
+    out = {}
+    for which in ("base", "input", "bed", "manning"):
+        if rhs_only and which == "base":
+            continue
+        dxb = [rng.standard_normal(n) for _ in range(3)] if which == "base" else zero
+        dxi = [rng.standard_normal(n) for _ in range(3)] if which == "input" else zero
+        dbed = rng.standard_normal(n) if which == "bed" else np.zeros(n)
+        dman = rng.standard_normal(c.n_elements) if which == "manning" else np.zeros(c.n_elements)
+        tlm = O.TangentLinear.from_case(c)
+        tlm.set_param_tangents(dbed, dman)
+        if rhs_only:
+            _, dout = tlm.rhs(inp, dxi, time)
+        else:
+            _, dout = tlm.stage(base, dxb, inp, dxi, a, b, cc, time, False)
+        lhs = _dot(w, dout)
+        rhs = _dot(gb, dxb) + _dot(gi, dxi) + float(np.dot(gbed, dbed)) + float(np.dot(gman, dman))
+        out[which] = (lhs, rhs)
+    return out
+
+
+@pytest.mark.parametrize("name,make", STAGE_CASES)
+@pytest.mark.parametrize("mode", ["rhs", "stage"])
+def test_stage_vjp_dot_product(name, make, mode):
+    c = make()
+    orc = O.Oracle.from_case(c)
+    base = perturbed(c, seed=1)
+    dt = orc.dt(*base, c.cfl)
+    if mode == "rhs":
+        res = dot_test(c, base, base, 0.0, 0.0, 0.0, 0.0, True, 11)
+    else:
+        inp = orc.stage(base, base, 1.0, 0.0, dt, 0.0, False)
+        res = dot_test(c, base, inp, 0.75, 0.25, 0.25 * dt, 0.5 * dt, False, 11)
+    for which, (lhs, rhs) in res.items():
+        scale = max(abs(lhs), 1e-300)
+        assert abs(lhs - rhs) <= TOL * scale or (lhs == 0.0 and rhs == 0.0), (which, lhs, rhs)
+
+
+def test_theta_limiter_adjoint():
+    """A draining shore with a long stage: elements whose combined depth has a
+    negative node and a positive mean take the theta limiter."""
+    c = cases.drying_beach(order=2, nx=8, ny=3)
+    n = c.h.size
+    rng = np.random.default_rng(0)
+    h = np.maximum(c.h + 0.2 * rng.standard_normal(n) * (c.h > 0), 0.0)
+    base = (h, (-4.0 + rng.standard_normal(n)) * h, rng.standard_normal(n) * h)
+    dt = O.Oracle.from_case(c).dt(*base, c.cfl)
+    cc = 3.0 * dt
+    # the limiter is reached: a negative node under a positive element mean (the
+    # depth of the combination does not depend on the wet/dry switch)
+    pre = O.Oracle.from_case(c, wet_dry=False).stage(base, base, 1.0, 0.0, cc, 0.0, False)[0].reshape(c.n_elements, -1)
+    assert ((pre < c.h_dry).any(1) & (pre.min(1) < 0) & (pre.mean(1) > 0)).sum() >= 1
+    res = dot_test(c, base, base, 1.0, 0.0, cc, 0.0, False, 5)
+    for which, (lhs, rhs) in res.items():
+        assert abs(lhs - rhs) <= TOL * abs(lhs), (which, lhs, rhs)
