@@ -74,6 +74,57 @@ def cpu_baseline(case, seconds_target: float):
                       f"{threads} threads on {model}"}
 
 
+def adjoint_bench(n, n_steps, snapshots, every, n_stations):
+    """The adjoint workload: wall time of the station-misfit gradient (forward
+    sweep + reverse sweep with recomputation from `snapshots` checkpoints) over
+    the wall time of one plain forward run of the same steps, and the reverse
+    sweep alone over the forward when every state is kept (per-step VJP cost)."""
+    from dgcom_amd import inversion
+
+    t0 = time.time()
+    inv = inversion.workload(n=n, n_steps=n_steps, every=every, n_stations=n_stations)
+    c = inv.case
+    phys = swe2d.SWEPhysics2D.from_case(c)
+    inversion.observe(inv, phys)
+    setup = time.time() - t0
+    x0 = phys.state(c.h, c.hu, c.hv)
+
+    def timed(fn):
+        phys.sync()
+        w0 = time.perf_counter()
+        r = fn()
+        phys.sync()
+        return r, time.perf_counter() - w0
+
+    plain = phys.tape(1)
+    J, fwd_s = timed(lambda: plain.forward(x0, 0.0, inv.dts, every, inv.obs))
+    J, fwd_s = timed(lambda: plain.forward(x0, 0.0, inv.dts, every, inv.obs))
+    plain.close()
+    tape = phys.tape(snapshots)
+    Jg, f1 = timed(lambda: tape.forward(x0, 0.0, inv.dts, every, inv.obs))
+    (gbed, gman), b1 = timed(lambda: tape.backward(1.0))
+    fsteps, vjps = tape.stats()
+    tape.close()
+    per_step = None
+    if n_steps <= 4000:
+        full = phys.tape(n_steps + 1)
+        _, f2 = timed(lambda: full.forward(x0, 0.0, inv.dts, every, inv.obs))
+        _, b2 = timed(lambda: full.backward(1.0))
+        full.close()
+        per_step = {"forward_with_all_checkpoints_s": f2, "reverse_sweep_s": b2, "ratio": b2 / fwd_s,
+                    "gradient_over_forward_keeping_every_state": (f2 + b2) / fwd_s}
+    K, nn = c.n_elements, c.nn
+    return {"workload": f"bathymetry inversion: {K} jittered quads ({n}x{n} over 50 km), N={c.order}, GRF truth bed, "
+                        f"prior = truth + GRF(sd 2 m), {n_stations} stations every {every} steps, {n_steps} SSP-RK3 "
+                        f"steps, {snapshots} checkpoints",
+            "elements": K, "order": c.order, "steps": n_steps, "snapshots": snapshots,
+            "forward_s": fwd_s, "gradient_s": f1 + b1, "ratio": (f1 + b1) / fwd_s,
+            "gradient_forward_sweep_s": f1, "gradient_reverse_sweep_s": b1,
+            "forward_steps": fsteps, "step_vjps": vjps, "misfit": Jg,
+            "grad_norms": {"bed": float(np.linalg.norm(gbed)), "manning": float(np.linalg.norm(gman))},
+            "per_step": per_step, "setup_s": setup}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +135,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--adjoint-n", type=int, default=500, help="grid side of the adjoint workload (500: 250,000 quads)")
+    ap.add_argument("--adjoint-steps", type=int, default=2000)
+    ap.add_argument("--adjoint-snapshots", type=int, default=20)
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the time step runs on a single device here "
@@ -151,6 +206,9 @@ def main():
                      "dt_kernel_avg_ms": dt_ms / max(dt_launches, 1)},
         "setup_s": gen_s,
     }
+    if not args.no_adjoint:
+        phys.close()
+        out["adjoint"] = adjoint_bench(args.adjoint_n, args.adjoint_steps, args.adjoint_snapshots, 10, 20)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
     print(json.dumps(out), flush=True)

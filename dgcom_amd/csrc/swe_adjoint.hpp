@@ -444,19 +444,17 @@ DG_HD void lim_slope_adj(double q0, double q1, double q2, const Inv3& inv, doubl
     const bool first = takes_first_min(a1, a2);
     const double bound = first ? a1 : a2;
     const double mb = ((central < 0.0) ? -1.0 : 1.0) * sb;
-    if (takes_first_min(fabs(central), bound)) {
-        const double cb = (central < 0.0) ? -mb : mb;
-        b2 += cb * inv.central;
-        b0 -= cb * inv.central;
-    } else if (first) {
-        const double tb = (t1 < 0.0) ? -mb : mb;
-        b1 += tb * inv.minus;
-        b0 -= tb * inv.minus;
-    } else {
-        const double tb = (t2 < 0.0) ? -mb : mb;
-        b2 += tb * inv.plus;
-        b1 -= tb * inv.plus;
-    }
+    // the slope is c * (q_hi - q_lo) for the chosen candidate; values, not
+    // references, are selected (selecting a reference would put it in memory)
+    const bool use_c = takes_first_min(fabs(central), bound);
+    const double cc = use_c ? ((central < 0.0) ? -mb : mb) * inv.central
+                            : first ? ((t1 < 0.0) ? -mb : mb) * inv.minus : ((t2 < 0.0) ? -mb : mb) * inv.plus;
+    const double w0 = (use_c || first) ? -cc : 0.0;
+    const double w1 = use_c ? 0.0 : (first ? cc : -cc);
+    const double w2 = (use_c || !first) ? cc : 0.0;
+    b0 += w0;
+    b1 += w1;
+    b2 += w2;
 }
 
 // reconstruct reverse: cotangents of the three sides (added) from those of the

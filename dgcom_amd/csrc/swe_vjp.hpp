@@ -527,11 +527,18 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
                     }
                 }
                 const Side sa = left ? side(q) : side(qo), sbs = left ? side(qo) : side(q);
-                const Cons zero3 = {0.0, 0.0, 0.0};
+                // per-field selects (a conditional between two struct lvalues would
+                // put them in memory)
+                const double fl = left ? 1.0 : 0.0, fr = left ? 0.0 : -1.0;
+                const Cons fab = {fl * fsb.h, fl * fsb.hu, fl * fsb.hv};
+                const Cons fbb = {fr * fsb.h, fr * fsb.hu, fr * fsb.hv};
                 SideBar sab = sbar_zero(), sbb = sbar_zero();
-                hydro_hll_adj(sa, sbs, lx, ly, p.g, p.h_min, left ? fsb : zero3, left ? zero3 : cneg(fsb), sab, sbb);
-                nbar_add_side(qb, left ? sab : sbb);
-                const SideBar os = left ? sbb : sab;
+                hydro_hll_adj(sa, sbs, lx, ly, p.g, p.h_min, fab, fbb, sab, sbb);
+                const SideBar mine = {left ? sab.h : sbb.h, left ? sab.u : sbb.u, left ? sab.v : sbb.v,
+                                      left ? sab.b : sbb.b};
+                const SideBar os = {left ? sbb.h : sab.h, left ? sbb.u : sab.u, left ? sbb.v : sab.v,
+                                    left ? sbb.b : sab.b};
+                nbar_add_side(qb, mine);
                 const NodeBar ob = {os.h, 0.0, 0.0, os.u, os.v, os.b};
                 double oh = 0.0, ohu = 0.0, ohv = 0.0, obb = 0.0;
                 if (wall) {
