@@ -76,8 +76,10 @@ struct dg_ctx {
     // discrete adjoint
     double *gbed = nullptr, *gman = nullptr;          // parameter cotangents [K*NN], [K]
     double *facebuf = nullptr, *innerbuf = nullptr;  // [K][4(N+1)][4] neighbour cotangents
+    double *facecot = nullptr, *ownbuf = nullptr;    // [K][4(N+1)][3], [K][4(N+1)][4] F* cotangents, own sides
     uint8_t* kind = nullptr;                         // [K] subcell elements of the last VJP
     dg_state* adj[2] = {nullptr, nullptr};           // stage cotangents of dg_step_vjp
+    dg_state* rhsbuf[4] = {nullptr, nullptr, nullptr, nullptr};  // stage RHS saved by the forward for the VJP
     // stations
     int n_st = 0;
     int64_t* st_elem = nullptr;
@@ -134,15 +136,22 @@ static double tide_elevation(const dg_tide& t, double time) {
 static int free_statics(dg_ctx* c) {
     void* ptrs[] = {c->corners, c->bed, c->manning, c->element_dt, c->links, c->tags, c->affine, c->queue,
                     c->queue_count, c->block_min, c->sub_list, c->chunk_total, c->gbed, c->gman, c->facebuf,
-                    c->innerbuf, c->kind, c->st_elem, c->st_w, c->st_eta};
+                    c->innerbuf, c->facecot, c->ownbuf, c->kind, c->st_elem, c->st_w, c->st_eta};
     for (void* p : ptrs)
         if (p) DG_TRY(hipFree(p));
-    c->gbed = c->gman = c->facebuf = c->innerbuf = nullptr;
+    c->gbed = c->gman = c->facebuf = c->innerbuf = c->facecot = c->ownbuf = nullptr;
     c->kind = nullptr;
     c->st_elem = nullptr;
     c->st_w = c->st_eta = nullptr;
     c->n_st = 0;
     for (auto& w : c->adj) {
+        if (w) {
+            DG_TRY(hipFree(w->d));
+            delete w;
+            w = nullptr;
+        }
+    }
+    for (auto& w : c->rhsbuf) {
         if (w) {
             DG_TRY(hipFree(w->d));
             delete w;
@@ -168,7 +177,7 @@ static int free_statics(dg_ctx* c) {
 }
 
 static int launch_stage(dg_ctx* c, const double* base, const double* input, double* out, double a, double b, double cc,
-                        double time, int rhs_only, double* fstar_out = nullptr) {
+                        double time, int rhs_only, double* fstar_out = nullptr, double* rhs_save = nullptr) {
     size_t nt = (size_t)c->K * c->nn;
     StageParams p;
     p.bh = base; p.bhu = base + nt; p.bhv = base + 2 * nt;
@@ -184,6 +193,9 @@ static int launch_stage(dg_ctx* c, const double* base, const double* input, doub
     p.flags = c->flags;
     p.clips = c->clips;
     p.fstar_out = fstar_out;
+    p.rh = rhs_save;
+    p.rhu = rhs_save ? rhs_save + nt : nullptr;
+    p.rhv = rhs_save ? rhs_save + 2 * nt : nullptr;
     p.queue = c->queue;
     p.queue_count = c->queue_count;
     const int epb = 64 / c->nn;
@@ -304,10 +316,16 @@ static int ensure_adjoint(dg_ctx* c) {
     const size_t nf = 4 * (size_t)(c->order + 1), nt = (size_t)c->K * c->nn;
     DG_TRY(hipMalloc(&c->facebuf, (size_t)c->K * nf * 4 * sizeof(double)));
     DG_TRY(hipMalloc(&c->innerbuf, (size_t)c->K * nf * 4 * sizeof(double)));
+    DG_TRY(hipMalloc(&c->facecot, (size_t)c->K * nf * 3 * sizeof(double)));
+    DG_TRY(hipMalloc(&c->ownbuf, (size_t)c->K * nf * 4 * sizeof(double)));
     DG_TRY(hipMalloc(&c->kind, (size_t)c->K));
     DG_TRY(hipMalloc(&c->gbed, nt * sizeof(double)));
     DG_TRY(hipMalloc(&c->gman, (size_t)c->K * sizeof(double)));
     for (auto& w : c->adj) {
+        w = new dg_state;
+        DG_TRY(hipMalloc(&w->d, 3 * nt * sizeof(double)));
+    }
+    for (auto& w : c->rhsbuf) {
         w = new dg_state;
         DG_TRY(hipMalloc(&w->d, 3 * nt * sizeof(double)));
     }
@@ -317,9 +335,10 @@ static int ensure_adjoint(dg_ctx* c) {
 
 // The VJP of one stage: gbase (may be null) and ginput are added to, and the
 // parameter cotangents; the neighbour parts of ginput and the bed's are
-// gathered after the element kernels.
-static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, const double* w, double* gbase,
-                            double* ginput, double a, double b, double cc, double time, int rhs_only) {
+// gathered after the element kernels. rhs: the stage's RHS as its forward
+// saved it (not read for rhs_only).
+static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, const double* rhs, const double* w,
+                            double* gbase, double* ginput, double a, double b, double cc, double time, int rhs_only) {
     if (int r = ensure_adjoint(c)) return r;
     size_t nt = (size_t)c->K * c->nn;
     VjpParams P;
@@ -339,6 +358,9 @@ static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, 
     p.fstar_out = nullptr;
     p.queue = c->queue;
     p.queue_count = c->queue_count;
+    p.rh = const_cast<double*>(rhs);
+    p.rhu = rhs ? p.rh + nt : nullptr;
+    p.rhv = rhs ? p.rh + 2 * nt : nullptr;
     P.wh = w; P.whu = w + nt; P.whv = w + 2 * nt;
     P.gbh = gbase; P.gbhu = gbase ? gbase + nt : nullptr; P.gbhv = gbase ? gbase + 2 * nt : nullptr;
     P.gih = ginput; P.gihu = ginput + nt; P.gihv = ginput + 2 * nt;
@@ -346,12 +368,16 @@ static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, 
     P.gman = c->gman;
     P.facebuf = c->facebuf;
     P.innerbuf = c->innerbuf;
+    P.facecot = c->facecot;
+    P.ownbuf = c->ownbuf;
     P.kind = c->kind;
     const int epb = 64 / c->nn;
     const unsigned int nblk = (unsigned int)((c->K + epb - 1) / epb);
     const unsigned int nchunk = (nblk + 1023) / 1024;
     p.xcd_stride = (nblk + 7) / 8;
-    dim3 grid(8 * p.xcd_stride), sub_grid(std::min(nblk, SUB_GROUPS)), ggrid((unsigned)((nt + 255) / 256));
+    const size_t nft = (size_t)c->K * 4 * (c->order + 1);
+    dim3 grid(8 * p.xcd_stride), sub_grid(std::min(nblk, SUB_GROUPS)), ggrid((unsigned)((nt + 255) / 256)),
+        fgrid((unsigned)((nft + 255) / 256));
     switch (c->order) {
         case 1: hipLaunchKernelGGL(wet_vjp_kernel<1>, grid, dim3(64), 0, c->stream, P); break;
         case 2: hipLaunchKernelGGL(wet_vjp_kernel<2>, grid, dim3(64), 0, c->stream, P); break;
@@ -365,18 +391,23 @@ static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, 
         case 2: hipLaunchKernelGGL(subcell_vjp_kernel<2>, sub_grid, dim3(64), 0, c->stream, P, c->sub_list, c->sub_count); break;
         default: hipLaunchKernelGGL(subcell_vjp_kernel<3>, sub_grid, dim3(64), 0, c->stream, P, c->sub_list, c->sub_count); break;
     }
+    switch (c->order) {
+        case 1: hipLaunchKernelGGL(face_vjp_kernel<1>, fgrid, dim3(256), 0, c->stream, P); break;
+        case 2: hipLaunchKernelGGL(face_vjp_kernel<2>, fgrid, dim3(256), 0, c->stream, P); break;
+        default: hipLaunchKernelGGL(face_vjp_kernel<3>, fgrid, dim3(256), 0, c->stream, P); break;
+    }
     double *gh = ginput, *ghu = ginput + nt, *ghv = ginput + 2 * nt;
     switch (c->order) {
-        case 1: hipLaunchKernelGGL(vjp_gather_kernel<1>, ggrid, dim3(256), 0, c->stream, c->links, c->facebuf, c->innerbuf, c->kind, c->K, gh, ghu, ghv, c->gbed); break;
-        case 2: hipLaunchKernelGGL(vjp_gather_kernel<2>, ggrid, dim3(256), 0, c->stream, c->links, c->facebuf, c->innerbuf, c->kind, c->K, gh, ghu, ghv, c->gbed); break;
-        default: hipLaunchKernelGGL(vjp_gather_kernel<3>, ggrid, dim3(256), 0, c->stream, c->links, c->facebuf, c->innerbuf, c->kind, c->K, gh, ghu, ghv, c->gbed); break;
+        case 1: hipLaunchKernelGGL(vjp_gather_kernel<1>, ggrid, dim3(256), 0, c->stream, c->links, c->ownbuf, c->facebuf, c->innerbuf, c->kind, c->K, gh, ghu, ghv, c->gbed); break;
+        case 2: hipLaunchKernelGGL(vjp_gather_kernel<2>, ggrid, dim3(256), 0, c->stream, c->links, c->ownbuf, c->facebuf, c->innerbuf, c->kind, c->K, gh, ghu, ghv, c->gbed); break;
+        default: hipLaunchKernelGGL(vjp_gather_kernel<3>, ggrid, dim3(256), 0, c->stream, c->links, c->ownbuf, c->facebuf, c->innerbuf, c->kind, c->K, gh, ghu, ghv, c->gbed); break;
     }
     DG_TRY(hipGetLastError());
     return 0;
 }
 
-// The VJP of one SSP step from state X: the stage values are recomputed, then
-// the stages are reversed; lam = (dX'/dX)^T lam_next (overwritten).
+// The VJP of one SSP step from state X: the stages are recomputed, saving
+// their RHS, then reversed; lam = (dX'/dX)^T lam_next (overwritten).
 static int step_vjp_impl(dg_ctx* c, const double* X, double t, double dt, int scheme, const double* lam_next,
                          double* lam) {
     if (int r = ensure_work(c)) return r;
@@ -385,8 +416,9 @@ static int step_vjp_impl(dg_ctx* c, const double* X, double t, double dt, int sc
     const double(*st)[4] = scheme == DG_SSP_RK3 ? RK3 : RK43;
     const size_t n3 = (size_t)3 * c->K * c->nn;
     const double* input = X;
-    for (int q = 0; q + 1 < ns; ++q) {
-        if (int r = launch_stage(c, X, input, c->work[q]->d, st[q][0], st[q][1], st[q][2] * dt, t + st[q][3] * dt, 0))
+    for (int q = 0; q < ns; ++q) {
+        if (int r = launch_stage(c, X, input, c->work[q]->d, st[q][0], st[q][1], st[q][2] * dt, t + st[q][3] * dt, 0,
+                                 nullptr, c->rhsbuf[q]->d))
             return r;
         input = c->work[q]->d;
     }
@@ -400,7 +432,8 @@ static int step_vjp_impl(dg_ctx* c, const double* X, double t, double dt, int sc
             gin = c->adj[tb]->d;
             if (int r = zero_async(c, gin, n3)) return r;
         }
-        if (int r = launch_stage_vjp(c, X, inq, w, lam, gin, st[q][0], st[q][1], st[q][2] * dt, t + st[q][3] * dt, 0))
+        if (int r = launch_stage_vjp(c, X, inq, c->rhsbuf[q]->d, w, lam, gin, st[q][0], st[q][1], st[q][2] * dt,
+                                     t + st[q][3] * dt, 0))
             return r;
         w = gin;
         tb ^= 1;
@@ -952,7 +985,7 @@ int dg_rhs_vjp(dg_ctx* c, const dg_state* in, double t, const dg_state* cot_out,
     DG_REQUIRE(c && in && cot_out && cot_in && c->K > 0, "dg_rhs_vjp: null argument or no form");
     DG_REQUIRE(cot_in != cot_out, "dg_rhs_vjp: cot_in must differ from cot_out");
     DG_TRY(hipSetDevice(c->device));
-    if (int r = launch_stage_vjp(c, in->d, in->d, cot_out->d, nullptr, cot_in->d, 0.0, 0.0, 0.0, t, 1)) return r;
+    if (int r = launch_stage_vjp(c, in->d, in->d, nullptr, cot_out->d, nullptr, cot_in->d, 0.0, 0.0, 0.0, t, 1)) return r;
     DG_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -962,8 +995,13 @@ int dg_stage_vjp(dg_ctx* c, const dg_state* base, const dg_state* input, const d
     DG_REQUIRE(c && base && input && st && cot_out && cot_input && c->K > 0, "dg_stage_vjp: null argument or no form");
     DG_REQUIRE(cot_input != cot_out && cot_base != cot_out, "dg_stage_vjp: the cotangent outputs must differ from cot_out");
     DG_TRY(hipSetDevice(c->device));
-    if (int r = launch_stage_vjp(c, base->d, input->d, cot_out->d, cot_base ? cot_base->d : nullptr, cot_input->d, st->a,
-                                 st->b, st->c, st->time, 0))
+    if (int r = ensure_work(c)) return r;
+    if (int r = ensure_adjoint(c)) return r;
+    if (int r = launch_stage(c, base->d, input->d, c->work[0]->d, st->a, st->b, st->c, st->time, 0, nullptr,
+                             c->rhsbuf[0]->d))
+        return r;
+    if (int r = launch_stage_vjp(c, base->d, input->d, c->rhsbuf[0]->d, cot_out->d, cot_base ? cot_base->d : nullptr,
+                                 cot_input->d, st->a, st->b, st->c, st->time, 0))
         return r;
     DG_TRY(hipStreamSynchronize(c->stream));
     return 0;

@@ -138,8 +138,9 @@ DG_HD void hll_aligned_adj(const Hll& l, const Hll& r, double g, double h_min, c
             lb.h += 0.5 * h_roeb;
             rb.h += 0.5 * h_roeb;
         }
-        const double numb = un_roeb / den;
-        const double denb = -un_roeb * un_roe / den;
+        const double iden = 1.0 / den;
+        const double numb = un_roeb * iden;
+        const double denb = -un_roeb * un_roe * iden;
         sqlb += numb * l.un + denb;
         sqrb += numb * r.un + denb;
         lb.un += numb * sql;

@@ -81,6 +81,7 @@ struct StageParams {
     int flags;
     unsigned long long* clips;
     double* fstar_out;   // optional [K][4][N+1][3]: F* at every face node, outward (this element's side)
+    double *rh, *rhu, *rhv;  // optional: the stage's RHS, written by the forward, read by the VJP
     uint32_t* queue;     // [blocks][EPB]: the block's elements that take the subcell volume term
     uint8_t* queue_count;  // [blocks]: how many
     unsigned int xcd_stride;  // workgroups per XCD share of the wet kernel's grid (0: no remap)
@@ -406,7 +407,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             }
         }
         lds_sync();
-        if (active) {
+        if (active && !VJP) {
 #pragma unroll
             for (int dir = 0; dir < 2; ++dir) {
                 const int lb = dir == 0 ? rlb : slb, ls = dir == 0 ? 1 : N1;
@@ -424,8 +425,9 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
         }
     }
 
-    // Faces: F* at the element's face nodes, spread over its lanes
-    if (active) {
+    // Faces: F* at the element's face nodes, spread over its lanes (the VJP
+    // reads the RHS that its forward saved instead)
+    if (active && !VJP) {
 #pragma unroll
         for (int r = 0; r < NT; ++r) {
             const int task = i + r * NN;
@@ -481,7 +483,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
     lds_sync();
 
     Cons rhs = {0.0, 0.0, 0.0};
-    if (active) {
+    if (active && !VJP) {
         const bool r_first = j <= a;
 #pragma unroll
         for (int pass = 0; pass < 2; ++pass) {
@@ -504,10 +506,12 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             rhs.hu += p.f0 * hvin;
             rhs.hv += (-p.f0) * huin;
         }
+        if (p.rh) { p.rh[n] = rhs.h; p.rhu[n] = rhs.hu; p.rhv[n] = rhs.hv; }
     }
+    if (VJP && active && p.rh) rhs = {p.rh[n], p.rhu[n], p.rhv[n]};
     if constexpr (VJP) {
-        vjp_tail<N, SUB>(*vp, k, active, c, affine, det, det_inv, hin, huin, hvin, bin, lk, tg, nh, nhu, nhv, nbd, rhs,
-                         sh, shu, shv, su, sv, sb, srx, sry, ssx, ssy, rl, rr);
+        vjp_tail<N, SUB>(*vp, k, active, c, affine, det, det_inv, hin, huin, hvin, bin, rhs, sh, shu, shv, su, sv, sb,
+                         srx, sry, ssx, ssy, rl, rr);
         return;
     }
     if (p.rhs_only) {

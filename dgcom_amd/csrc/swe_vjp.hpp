@@ -7,10 +7,12 @@
 //   out cotangent -> correct_node -> theta limiter -> damping -> combination
 //   -> Coriolis, 1/J -> volume (or subcell) term and surface term -> node states.
 // Cotangents of this element's own nodes are added to the input's cotangent and
-// to the bed's; the parts that belong to a neighbour's nodes (the neighbour side
-// of every face Riemann solve, and the neighbour's inner node that a subcell
-// element reconstructs from) are written per face node to a face buffer and
-// added by vjp_gather_kernel afterwards, so that no two workgroups write one
+// to the bed's, except those through the face Riemann fluxes F*: the element
+// writes the cotangent of F* per face node, face_vjp_kernel runs each Riemann
+// solve backwards (one thread per face node, which keeps the element kernels'
+// registers low) and writes the cotangents of both sides per face node, and
+// vjp_gather_kernel adds them, with the neighbour's inner node that a subcell
+// element reconstructs from, to the nodes afterwards. No two threads write one
 // address and the sums do not depend on scheduling. Included by
 // swe_kernels.hpp before stage_block.
 #pragma once
@@ -26,6 +28,8 @@ struct VjpParams {
     double *gih, *gihu, *gihv;     // += the element's own part of the input's cotangent
     double* gbed;                  // += the element's own part of the bed's cotangent
     double* gman;                  // += per element Manning n cotangent (null: not wanted)
+    double* facecot;               // [K][4(N+1)][3]: cotangent of F* at each face node (element kernels)
+    double* ownbuf;                // [K][4(N+1)][4]: of the face node's own (h, hu, hv, B) through F*
     double* facebuf;               // [K][4(N+1)][4]: cotangent of the neighbour's face node (h, hu, hv, B)
     double* innerbuf;              // [K][4(N+1)][4]: of the neighbour's inner node (subcell elements)
     uint8_t* kind;                 // [K]: 1 where the element takes the subcell volume term
@@ -65,8 +69,7 @@ __device__ __forceinline__ void outer_node_adj(const VjpParams& P, int64_t k, in
 template <int N, bool SUB>
 __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool active, const Quad& c, bool affine,
                                          double det, double det_inv, double hin, double huin, double hvin, double bin,
-                                         const uint32_t* lk, const uint8_t* tg, const double* nh, const double* nhu,
-                                         const double* nhv, const double* nbd, const Cons& rhs, const double* sh,
+                                         const Cons& rhs, const double* sh,
                                          const double* shu, const double* shv, const double* su, const double* sv,
                                          const double* sb, const double* srx, const double* sry, const double* ssx,
                                          const double* ssy, const Side (*rl)[SUB ? 64 : 1],
@@ -75,14 +78,20 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
     constexpr int N1 = N + 1, NN = N1 * N1, EPB = 64 / NN, NF = 4 * N1;
     constexpr int NT = (NF + NN - 1) / NN;
     const GllTables& T = c_gll[N];
-    __shared__ double xh[64], xhu[64], xhv[64], pw[64];
-    __shared__ double gyh[64], gyhu[64], gyhv[64], gtb[64];
+    // One LDS region for the phases' own arrays (separated by lds_sync): the
+    // wet/dry recompute (xh..) and its reverse (gy.., gtb), pw through both, the
+    // bed term's line cotangents (sdb), the face nodes' own cotangents (tcot),
+    // the Manning reduction (sman). Less LDS per workgroup, more workgroups per CU.
+    constexpr int NREG = EPB * NF * 6 > 6 * 64 ? EPB * NF * 6 : 6 * 64;
+    __shared__ double region[NREG];
+    double *xh = region, *xhu = region + 64, *xhv = region + 128, *pw = region + 256;
+    double *gyh = region, *gyhu = region + 64, *gyhv = region + 128, *gtb = region + 192;
+    double(*sdb)[3][64] = reinterpret_cast<double(*)[3][64]>(region);
+    NodeBar* tcot = reinterpret_cast<NodeBar*>(region);
+    double* sman = region;
     __shared__ double red[EPB * 8];
     __shared__ int wfl[EPB];
     __shared__ double swh[64], swhu[64], swhv[64];
-    __shared__ double sdb[2][3][64];
-    __shared__ NodeBar tcot[EPB * NF];
-    __shared__ double sman[64];
     __shared__ SideBar rlb[2][SUB ? 64 : 1], rrb[2][SUB ? 64 : 1], rlx[2][SUB ? 64 : 1];
     __shared__ SideBar tol[2][SUB ? 64 : 1], tor[2][SUB ? 64 : 1];
     __shared__ Cons ifab[2][SUB ? 64 : 1], ifbb[2][SUB ? 64 : 1];
@@ -263,14 +272,18 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
                 if (fric) {
                     const double gn2 = p.g * mn * mn;
                     const double num = gn2 * speed, den = cbr * hh;
-                    const double term = num / den;
-                    const double numb = rateb / den, denb = -rateb * term / den;
+                    const double iden = 1.0 / den;
+                    const double term = num * iden;
+                    const double numb = rateb * iden, denb = -rateb * term * iden;
                     const double gn2b = numb * speed, speedb = numb * gn2;
                     const double cbb = denb * hh;
-                    const double hhb = denb * cbr + manning_cbrt_adj(hh, cbb);
+                    // the Halley iterate's derivative is that of the cube root, 1/(3 y^2),
+                    // to rounding (the iteration converges cubically)
+                    const double hhb = denb * cbr + cbb / (3.0 * cbr * cbr);
                     if (takes_first_max(vh, p.h_min)) gh += hhb;
                     gman += gn2b * (p.g * mn + p.g * mn);
-                    const double ub = speedb * uf / speed, vb = speedb * vf / speed;
+                    const double isp = 1.0 / speed;
+                    const double ub = speedb * uf * isp, vb = speedb * vf * isp;
                     if (wet_dry) desing_velocity_adj(hin, huin, hvin, p.h_dry, ub, vb, gi_h, gi_hu, gi_hv);
                     else if (hin > p.h_min) {
                         const double inv = 1.0 / hin;
@@ -480,7 +493,11 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
         }
     }
 
-    // Faces: F* at the element's face nodes, spread over its lanes (as the forward)
+    lds_sync();  // the region's next use (tcot) overlays the line cotangents
+    // Faces: rhs += sJ LIFT (F(q).n - F*) at the element's face nodes, spread
+    // over its lanes. The F(q).n part is this node's own; the cotangent of F*
+    // goes to the face buffer for face_vjp_kernel, which runs the Riemann
+    // solve backwards for both sides of the face.
     if (active) {
 #pragma unroll
         for (int r = 0; r < NT; ++r) {
@@ -493,64 +510,11 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
             face_vector<N>(c, f, vx, vy);
             const double sj = sqrt(vx * vx + vy * vy);
             const double nx = vx / sj, ny = vy / sj;
-            const uint32_t l = lk[r];
-            // rhs += sJ LIFT (F(q).n - F*)
             const Cons sfdb = cscale(sj * T.lift[f][fi], Cons{swh[li], swhu[li], swhv[li]});
             NodeBar qb = nodebar_zero();
             ec_flux_adj(q, q, nx, ny, p.g, sfdb, qb, qb);
-            const Cons fsb = cneg(sfdb);
-            if (l == LINK_BOUNDARY && tg[r] == DG_BC_TIDE) {
-                const Cons qbd = characteristic(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min);
-                const Node nbn = make_node(qbd.h, qbd.hu, qbd.hv, q.b, p.h_min);
-                NodeBar nbb = nodebar_zero();
-                ec_flux_adj(nbn, nbn, nx, ny, p.g, fsb, nbb, nbb);
-                double bh = 0.0, bhu = 0.0, bhv = 0.0, bbb = 0.0;
-                make_node_adj(qbd.h, qbd.hu, qbd.hv, p.h_min, nbb, bh, bhu, bhv, bbb);
-                qb.b += bbb;
-                characteristic_adj(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min, Cons{bh, bhu, bhv}, qb.h, qb.hu,
-                                   qb.hv, qb.b);
-            } else {
-                const bool wall = l == LINK_BOUNDARY;
-                const bool left = wall || (l & 4u);
-                const Cons gs = wall_ghost(q.h, q.hu, q.hv, nx, ny, p.h_min);
-                const Node qo = wall ? make_node(gs.h, gs.hu, gs.hv, q.b, p.h_min)
-                                     : make_node(nh[r], nhu[r], nhv[r], nbd[r], p.h_min);
-                double lx = nx, ly = ny;
-                if (!left) {
-                    lx = -nx; ly = -ny;
-                    if (!(l & 8u)) {
-                        const Quad cn = load_quad(p.corners, (int64_t)(l >> 4));
-                        double wx, wy;
-                        face_vector<N>(cn, (int)(l & 3u), wx, wy);
-                        double s2 = sqrt(wx * wx + wy * wy);
-                        lx = wx / s2; ly = wy / s2;
-                    }
-                }
-                const Side sa = left ? side(q) : side(qo), sbs = left ? side(qo) : side(q);
-                // per-field selects (a conditional between two struct lvalues would
-                // put them in memory)
-                const double fl = left ? 1.0 : 0.0, fr = left ? 0.0 : -1.0;
-                const Cons fab = {fl * fsb.h, fl * fsb.hu, fl * fsb.hv};
-                const Cons fbb = {fr * fsb.h, fr * fsb.hu, fr * fsb.hv};
-                SideBar sab = sbar_zero(), sbb = sbar_zero();
-                hydro_hll_adj(sa, sbs, lx, ly, p.g, p.h_min, fab, fbb, sab, sbb);
-                const SideBar mine = {left ? sab.h : sbb.h, left ? sab.u : sbb.u, left ? sab.v : sbb.v,
-                                      left ? sab.b : sbb.b};
-                const SideBar os = {left ? sbb.h : sab.h, left ? sbb.u : sab.u, left ? sbb.v : sab.v,
-                                    left ? sbb.b : sab.b};
-                nbar_add_side(qb, mine);
-                const NodeBar ob = {os.h, 0.0, 0.0, os.u, os.v, os.b};
-                double oh = 0.0, ohu = 0.0, ohv = 0.0, obb = 0.0;
-                if (wall) {
-                    make_node_adj(gs.h, gs.hu, gs.hv, p.h_min, ob, oh, ohu, ohv, obb);
-                    qb.b += obb;
-                    wall_ghost_adj(q.h, q.hu, q.hv, nx, ny, p.h_min, Cons{oh, ohu, ohv}, qb.h, qb.hu, qb.hv);
-                } else {
-                    make_node_adj(nh[r], nhu[r], nhv[r], p.h_min, ob, oh, ohu, ohv, obb);
-                    double* o = P.facebuf + ((size_t)k * NF + task) * 4;
-                    o[0] = oh; o[1] = ohu; o[2] = ohv; o[3] = obb;
-                }
-            }
+            double* o = P.facecot + ((size_t)k * NF + task) * 3;
+            o[0] = -sfdb.h; o[1] = -sfdb.hu; o[2] = -sfdb.hv;
             tcot[e * NF + task] = qb;
         }
     }
@@ -568,6 +532,7 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
         P.gbed[n] += gb;
     }
     if (P.gman) {
+        lds_sync();  // sman overlays tcot
         sman[t] = gman;
         lds_sync();
         if (active && i == 0) {
@@ -579,14 +544,101 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
     (void)bin;
 }
 
-// The neighbour parts of the input's and the bed's cotangents: every node adds
-// what its element's neighbours wrote for it, face by face in face order (the
-// face node written by the neighbour's face task; the inner node written by a
-// neighbour that took the subcell volume term). One thread per node.
+// The Riemann solves backwards: one thread per face node of every element
+// (the element's own side of the edge, or its boundary condition), from the
+// cotangent of F* that the element kernels wrote. The cotangent of the
+// element's own face node (h, hu, hv, B) goes to ownbuf, the neighbour's to
+// facebuf (both per face node, so that no two threads write one address).
 template <int N>
-__global__ __launch_bounds__(256) void vjp_gather_kernel(const uint32_t* links, const double* facebuf,
-                                                         const double* innerbuf, const uint8_t* kind, int64_t K,
-                                                         double* gh, double* ghu, double* ghv, double* gbed) {
+__global__ __launch_bounds__(256) void face_vjp_kernel(VjpParams P) {
+    constexpr int N1 = N + 1, NN = N1 * N1, NF = 4 * N1;
+    const StageParams& p = P.s;
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (size_t)p.n_elements * NF) return;
+    const int64_t k = (int64_t)(gid / NF);
+    const int task = (int)(gid - (size_t)k * NF);
+    const int f = task / N1, fi = task - f * N1;
+    const size_t n = (size_t)k * NN + face_node<N>(f, fi);
+    const double h = p.ih[n], hu = p.ihu[n], hv = p.ihv[n], bd = p.bed[n];
+    const Node q = make_node(h, hu, hv, bd, p.h_min);
+    const Quad c = load_quad(p.corners, k);
+    double vx, vy;
+    face_vector<N>(c, f, vx, vy);
+    const double sj = sqrt(vx * vx + vy * vy);
+    const double nx = vx / sj, ny = vy / sj;
+    const uint32_t l = p.links[4 * k + f];
+    const double* fc = P.facecot + gid * 3;
+    const Cons fsb = {fc[0], fc[1], fc[2]};
+    NodeBar qb = nodebar_zero();
+    if (l == LINK_BOUNDARY && p.tags[4 * k + f] == DG_BC_TIDE) {
+        const Cons qbd = characteristic(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min);
+        const Node nbn = make_node(qbd.h, qbd.hu, qbd.hv, q.b, p.h_min);
+        NodeBar nbb = nodebar_zero();
+        ec_flux_adj(nbn, nbn, nx, ny, p.g, fsb, nbb, nbb);
+        double bh = 0.0, bhu = 0.0, bhv = 0.0, bbb = 0.0;
+        make_node_adj(qbd.h, qbd.hu, qbd.hv, p.h_min, nbb, bh, bhu, bhv, bbb);
+        qb.b += bbb;
+        characteristic_adj(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min, Cons{bh, bhu, bhv}, qb.h, qb.hu, qb.hv,
+                           qb.b);
+    } else {
+        const bool wall = l == LINK_BOUNDARY;
+        const bool left = wall || (l & 4u);
+        double nhr = 0.0, nhur = 0.0, nhvr = 0.0, nbdr = 0.0;
+        if (!wall) {
+            const size_t m = (size_t)(l >> 4) * NN + face_node<N>((int)(l & 3u), N1 - 1 - fi);
+            nhr = p.ih[m]; nhur = p.ihu[m]; nhvr = p.ihv[m]; nbdr = p.bed[m];
+        }
+        const Cons gs = wall_ghost(q.h, q.hu, q.hv, nx, ny, p.h_min);
+        const Node qo = wall ? make_node(gs.h, gs.hu, gs.hv, q.b, p.h_min) : make_node(nhr, nhur, nhvr, nbdr, p.h_min);
+        double lx = nx, ly = ny;
+        if (!left) {
+            lx = -nx; ly = -ny;
+            if (!(l & 8u)) {
+                const Quad cn = load_quad(p.corners, (int64_t)(l >> 4));
+                double wx, wy;
+                face_vector<N>(cn, (int)(l & 3u), wx, wy);
+                double s2 = sqrt(wx * wx + wy * wy);
+                lx = wx / s2; ly = wy / s2;
+            }
+        }
+        const Side sa = left ? side(q) : side(qo), sbs = left ? side(qo) : side(q);
+        // per-field selects (a conditional between two struct lvalues would put them in memory)
+        const double fl = left ? 1.0 : 0.0, fr = left ? 0.0 : -1.0;
+        const Cons fab = {fl * fsb.h, fl * fsb.hu, fl * fsb.hv};
+        const Cons fbb = {fr * fsb.h, fr * fsb.hu, fr * fsb.hv};
+        SideBar sab = sbar_zero(), sbb = sbar_zero();
+        hydro_hll_adj(sa, sbs, lx, ly, p.g, p.h_min, fab, fbb, sab, sbb);
+        const SideBar mine = {left ? sab.h : sbb.h, left ? sab.u : sbb.u, left ? sab.v : sbb.v, left ? sab.b : sbb.b};
+        const SideBar os = {left ? sbb.h : sab.h, left ? sbb.u : sab.u, left ? sbb.v : sab.v, left ? sbb.b : sab.b};
+        nbar_add_side(qb, mine);
+        const NodeBar ob = {os.h, 0.0, 0.0, os.u, os.v, os.b};
+        double oh = 0.0, ohu = 0.0, ohv = 0.0, obb = 0.0;
+        if (wall) {
+            make_node_adj(gs.h, gs.hu, gs.hv, p.h_min, ob, oh, ohu, ohv, obb);
+            qb.b += obb;
+            wall_ghost_adj(q.h, q.hu, q.hv, nx, ny, p.h_min, Cons{oh, ohu, ohv}, qb.h, qb.hu, qb.hv);
+        } else {
+            make_node_adj(nhr, nhur, nhvr, p.h_min, ob, oh, ohu, ohv, obb);
+            double* o = P.facebuf + gid * 4;
+            o[0] = oh; o[1] = ohu; o[2] = ohv; o[3] = obb;
+        }
+    }
+    double gh = 0.0, ghu = 0.0, ghv = 0.0, gb = 0.0;
+    make_node_adj(h, hu, hv, p.h_min, qb, gh, ghu, ghv, gb);
+    double* o = P.ownbuf + gid * 4;
+    o[0] = gh; o[1] = ghu; o[2] = ghv; o[3] = gb;
+}
+
+// The face parts of the input's and the bed's cotangents: every node adds, face
+// by face in face order, what the face kernel wrote for it (its own side of
+// each face it lies on, and the neighbour side of the neighbour's face) and what
+// a neighbour that took the subcell volume term wrote for it as the inner node
+// behind that neighbour's face. One thread per node.
+template <int N>
+__global__ __launch_bounds__(256) void vjp_gather_kernel(const uint32_t* links, const double* ownbuf,
+                                                         const double* facebuf, const double* innerbuf,
+                                                         const uint8_t* kind, int64_t K, double* gh, double* ghu,
+                                                         double* ghv, double* gbed) {
     constexpr int N1 = N + 1, NN = N1 * N1, NF = 4 * N1;
     const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= (size_t)K * NN) return;
@@ -597,9 +649,6 @@ __global__ __launch_bounds__(256) void vjp_gather_kernel(const uint32_t* links, 
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
         const uint32_t l = links[4 * k + f];
-        if (l == LINK_BOUNDARY) continue;
-        const int64_t nb = (int64_t)(l >> 4);
-        const int nbf = (int)(l & 3u);
         // position of this node on face f, and as the inner node behind face f
         const int fi = f == 0 ? (j == 0 ? a : -1)
                      : f == 1 ? (a == N1 - 1 ? j : -1)
@@ -609,6 +658,13 @@ __global__ __launch_bounds__(256) void vjp_gather_kernel(const uint32_t* links, 
                      : f == 1 ? (a == N1 - 2 ? j : -1)
                      : f == 2 ? (j == N1 - 2 ? N1 - 1 - a : -1)
                               : (a == 1 ? N1 - 1 - j : -1);
+        if (fi >= 0) {
+            const double* o = ownbuf + ((size_t)k * NF + f * N1 + fi) * 4;
+            sh += o[0]; shu += o[1]; shv += o[2]; sb += o[3];
+        }
+        if (l == LINK_BOUNDARY) continue;
+        const int64_t nb = (int64_t)(l >> 4);
+        const int nbf = (int)(l & 3u);
         if (fi >= 0) {
             const double* o = facebuf + ((size_t)nb * NF + nbf * N1 + (N1 - 1 - fi)) * 4;
             sh += o[0]; shu += o[1]; shv += o[2]; sb += o[3];

@@ -13,21 +13,9 @@ void emu_init(void) {
     for (int n = 1; n <= 3; ++n) c_gll[n] = make_tables(n);
 }
 
-unsigned long long emu_stage(int order, int64_t K, const double* corners, const uint32_t* links, const uint8_t* tags,
-                             const uint8_t* affine, const double* bed, const double* manning, const double* base,
-                             const double* input, double* out, double a, double b, double c, double eta_e, double g,
-                             double h_min, double h_dry, double u_max, double tau, double f0, int rhs_only, int flags,
-                             double* fstar_out) {
-    size_t nn = (size_t)(order + 1) * (order + 1), nt = (size_t)K * nn;
-    unsigned long long clips = 0;
-    StageParams p;
-    p.bh = base; p.bhu = base + nt; p.bhv = base + 2 * nt;
-    p.ih = input; p.ihu = input + nt; p.ihv = input + 2 * nt;
-    p.oh = out; p.ohu = out + nt; p.ohv = out + 2 * nt;
-    p.bed = bed; p.corners = corners; p.manning = manning; p.links = links; p.tags = tags; p.affine = affine;
-    p.n_elements = K; p.a = a; p.b = b; p.c = c; p.eta_e = eta_e;
-    p.g = g; p.h_min = h_min; p.h_dry = h_dry; p.u_max = u_max; p.tau = tau; p.f0 = f0;
-    p.rhs_only = rhs_only; p.flags = flags; p.clips = &clips; p.fstar_out = fstar_out;
+// The forward stage's launch sequence (wet kernel, list compaction, subcell kernel)
+static void run_stage(int order, int64_t K, StageParams& p) {
+    size_t nn = (size_t)(order + 1) * (order + 1);
     int epb = 64 / (int)nn;
     unsigned nblk = (unsigned)((K + epb - 1) / epb);
     unsigned nchunk = (nblk + 1023) / 1024;
@@ -51,10 +39,32 @@ unsigned long long emu_stage(int order, int64_t K, const double* corners, const 
         case 2: emu_launch(sub_grid, 64, [&] { subcell_stage_kernel<2>(p, list.data(), &lcount); }); break;
         default: emu_launch(sub_grid, 64, [&] { subcell_stage_kernel<3>(p, list.data(), &lcount); }); break;
     }
+    p.queue = nullptr;
+    p.queue_count = nullptr;
+}
+
+unsigned long long emu_stage(int order, int64_t K, const double* corners, const uint32_t* links, const uint8_t* tags,
+                             const uint8_t* affine, const double* bed, const double* manning, const double* base,
+                             const double* input, double* out, double a, double b, double c, double eta_e, double g,
+                             double h_min, double h_dry, double u_max, double tau, double f0, int rhs_only, int flags,
+                             double* fstar_out) {
+    size_t nn = (size_t)(order + 1) * (order + 1), nt = (size_t)K * nn;
+    unsigned long long clips = 0;
+    StageParams p;
+    p.bh = base; p.bhu = base + nt; p.bhv = base + 2 * nt;
+    p.ih = input; p.ihu = input + nt; p.ihv = input + 2 * nt;
+    p.oh = out; p.ohu = out + nt; p.ohv = out + 2 * nt;
+    p.bed = bed; p.corners = corners; p.manning = manning; p.links = links; p.tags = tags; p.affine = affine;
+    p.n_elements = K; p.a = a; p.b = b; p.c = c; p.eta_e = eta_e;
+    p.g = g; p.h_min = h_min; p.h_dry = h_dry; p.u_max = u_max; p.tau = tau; p.f0 = f0;
+    p.rhs_only = rhs_only; p.flags = flags; p.clips = &clips; p.fstar_out = fstar_out;
+    p.rh = p.rhu = p.rhv = nullptr;
+    run_stage(order, K, p);
     return clips;
 }
 
 // The stage VJP: gbase (if not null) and ginput are added to; gbed and gman too.
+// The forward stage runs first and saves its RHS, as the library does.
 void emu_stage_vjp(int order, int64_t K, const double* corners, const uint32_t* links, const uint8_t* tags,
                    const uint8_t* affine, const double* bed, const double* manning, const double* base,
                    const double* input, const double* w, double* gbase, double* ginput, double* gbed, double* gman,
@@ -71,13 +81,22 @@ void emu_stage_vjp(int order, int64_t K, const double* corners, const uint32_t* 
     p.n_elements = K; p.a = a; p.b = b; p.c = c; p.eta_e = eta_e;
     p.g = g; p.h_min = h_min; p.h_dry = h_dry; p.u_max = u_max; p.tau = tau; p.f0 = f0;
     p.rhs_only = rhs_only; p.flags = flags; p.clips = &clips; p.fstar_out = nullptr;
+    std::vector<double> rhs(3 * nt, 0.0), fout(3 * nt, 0.0);
+    if (!rhs_only) {
+        p.oh = fout.data(); p.ohu = p.oh + nt; p.ohv = p.oh + 2 * nt;
+        p.rh = rhs.data(); p.rhu = p.rh + nt; p.rhv = p.rh + 2 * nt;
+        run_stage(order, K, p);
+    } else {
+        p.rh = p.rhu = p.rhv = nullptr;
+    }
     P.wh = w; P.whu = w + nt; P.whv = w + 2 * nt;
     P.gbh = gbase; P.gbhu = gbase ? gbase + nt : nullptr; P.gbhv = gbase ? gbase + 2 * nt : nullptr;
     P.gih = ginput; P.gihu = ginput + nt; P.gihv = ginput + 2 * nt;
     P.gbed = gbed; P.gman = gman;
-    std::vector<double> fb((size_t)K * nf * 4, 0.0), ib((size_t)K * nf * 4, 0.0);
+    std::vector<double> fb((size_t)K * nf * 4, 0.0), ib((size_t)K * nf * 4, 0.0), fc((size_t)K * nf * 3, 0.0),
+        ob((size_t)K * nf * 4, 0.0);
     std::vector<uint8_t> kind((size_t)K, 0);
-    P.facebuf = fb.data(); P.innerbuf = ib.data(); P.kind = kind.data();
+    P.facebuf = fb.data(); P.innerbuf = ib.data(); P.kind = kind.data(); P.facecot = fc.data(); P.ownbuf = ob.data();
     int epb = 64 / (int)nn;
     unsigned nblk = (unsigned)((K + epb - 1) / epb);
     unsigned nchunk = (nblk + 1023) / 1024;
@@ -101,11 +120,17 @@ void emu_stage_vjp(int order, int64_t K, const double* corners, const uint32_t* 
         case 2: emu_launch(sub_grid, 64, [&] { subcell_vjp_kernel<2>(P, list.data(), &lcount); }); break;
         default: emu_launch(sub_grid, 64, [&] { subcell_vjp_kernel<3>(P, list.data(), &lcount); }); break;
     }
+    unsigned fgrid = (unsigned)((K * nf + 255) / 256);
+    switch (order) {
+        case 1: emu_launch(fgrid, 256, [&] { face_vjp_kernel<1>(P); }); break;
+        case 2: emu_launch(fgrid, 256, [&] { face_vjp_kernel<2>(P); }); break;
+        default: emu_launch(fgrid, 256, [&] { face_vjp_kernel<3>(P); }); break;
+    }
     unsigned ggrid = (unsigned)((nt + 255) / 256);
     switch (order) {
-        case 1: emu_launch(ggrid, 256, [&] { vjp_gather_kernel<1>(links, fb.data(), ib.data(), kind.data(), K, ginput, ginput + nt, ginput + 2 * nt, gbed); }); break;
-        case 2: emu_launch(ggrid, 256, [&] { vjp_gather_kernel<2>(links, fb.data(), ib.data(), kind.data(), K, ginput, ginput + nt, ginput + 2 * nt, gbed); }); break;
-        default: emu_launch(ggrid, 256, [&] { vjp_gather_kernel<3>(links, fb.data(), ib.data(), kind.data(), K, ginput, ginput + nt, ginput + 2 * nt, gbed); }); break;
+        case 1: emu_launch(ggrid, 256, [&] { vjp_gather_kernel<1>(links, ob.data(), fb.data(), ib.data(), kind.data(), K, ginput, ginput + nt, ginput + 2 * nt, gbed); }); break;
+        case 2: emu_launch(ggrid, 256, [&] { vjp_gather_kernel<2>(links, ob.data(), fb.data(), ib.data(), kind.data(), K, ginput, ginput + nt, ginput + 2 * nt, gbed); }); break;
+        default: emu_launch(ggrid, 256, [&] { vjp_gather_kernel<3>(links, ob.data(), fb.data(), ib.data(), kind.data(), K, ginput, ginput + nt, ginput + 2 * nt, gbed); }); break;
     }
 }
 
