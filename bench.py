@@ -125,6 +125,64 @@ def adjoint_bench(n, n_steps, snapshots, every, n_stations):
             "per_step": per_step, "setup_s": setup}
 
 
+def ensemble_members(n_members, seed, k_lo, k_hi, n_steps):
+    """Seeded members: K log-uniform in [k_lo, k_hi] jittered quads of 100 m, N uniform
+    in {1, 2, 3}, M2 tide amplitude uniform in 0.2-1.5 m and phase in 0-2pi, GRF bed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for m in range(n_members):
+        K = float(np.exp(rng.uniform(np.log(k_lo), np.log(k_hi))))
+        n = max(4, int(round(np.sqrt(K))))
+        order = int(rng.integers(1, 4))
+        amp, phase = float(rng.uniform(0.2, 1.5)), float(rng.uniform(0.0, 2.0 * np.pi))
+        c = cases.large_unstructured(n=n, length=100.0 * n, order=order, seed=seed + 1000 + 17 * m, n_steps=n_steps)
+        c.tide = cases.Tide(amplitude=(amp,), omega=(2.0 * np.pi / cases.M2_PERIOD_S,), phase=(phase,), ramp=3600.0)
+        out.append(c)
+    return out
+
+
+def ensemble_bench(n_members, n_steps, k_lo, k_hi):
+    """An ensemble of uneven members advanced together (one launch per order per
+    kernel, graph-replayed stages, tuned launch shapes) against the same members
+    run one after another (one run per launch sequence)."""
+    t0 = time.time()
+    members = ensemble_members(n_members, 99, k_lo, k_hi, n_steps)
+    phys = [swe2d.SWEPhysics2D.from_case(c) for c in members]
+    setup = time.time() - t0
+    dofs = sum(3 * c.n_elements * c.nn * STAGES * n_steps for c in members)
+    # one run after another
+    seq = 0.0
+    for c, p in zip(members, phys):
+        s = p.state(c.h, c.hu, c.hv)
+        p.sync()
+        w0 = time.perf_counter()
+        p.run(s, 0.0, n_steps, c.cfl)
+        p.sync()
+        seq += time.perf_counter() - w0
+        s.free()
+    ens = swe2d.Ensemble(phys)
+    for m, c in enumerate(members):
+        ens.upload(m, c.h, c.hu, c.hv)
+    ens.run(1, members[0].cfl, graph=True, tune=True)  # tuning and graph capture, untimed
+    for m, c in enumerate(members):
+        ens.upload(m, c.h, c.hu, c.hv)
+    ens.t[:] = 0.0
+    ens.members[0].sync()
+    w0 = time.perf_counter()
+    ens.run(n_steps, members[0].cfl, graph=True)
+    ens.members[0].sync()
+    batched = time.perf_counter() - w0
+    orders = [c.order for c in members]
+    return {"workload": f"{n_members} members, K log-uniform {k_lo}-{k_hi} jittered quads, N uniform in {{1,2,3}}, "
+                        f"M2 tide 0.2-1.5 m with random phase, GRF beds, {n_steps} SSP-RK3 steps each",
+            "members": n_members, "elements_total": int(sum(c.n_elements for c in members)),
+            "members_per_order": {str(o): orders.count(o) for o in (1, 2, 3)},
+            "dof_updates": dofs, "batched_s": batched, "one_run_per_launch_s": seq,
+            "batched_dof_updates_per_s": dofs / batched, "one_run_per_launch_dof_updates_per_s": dofs / seq,
+            "speedup": seq / batched, "tuning": {str(o): ens.tuning(o) for o in (1, 2, 3) if o in orders},
+            "setup_s": setup}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +197,9 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--adjoint-snapshots", type=int, default=20)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--ensemble-members", type=int, default=64)
+    ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--no-ensemble", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the time step runs on a single device here "
@@ -209,6 +270,8 @@ def main():
     if not args.no_adjoint:
         phys.close()
         out["adjoint"] = adjoint_bench(args.adjoint_n, args.adjoint_steps, args.adjoint_snapshots, 10, 20)
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_bench(args.ensemble_members, args.ensemble_steps, 20_000, 400_000)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
     print(json.dumps(out), flush=True)

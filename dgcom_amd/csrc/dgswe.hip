@@ -22,6 +22,7 @@
 
 #include "../../include/dgswe.h"
 #include "swe_kernels.hpp"
+#include "swe_batch.hpp"
 
 using namespace dgswe;
 
@@ -593,6 +594,179 @@ static int64_t revolve_count_rec(const Revolve& rv, int l, int free, bool presto
     if (free == 0) return (int64_t)l * (l - 1) / 2;
     const int m = rv.split(l, free, prestored);
     return (prestored ? 0 : m) + revolve_count_rec(rv, l - m, free - 1, prestored) + revolve_count_rec(rv, m, free, false);
+}
+
+// ---------------------------------------------------------------------------
+// Ensembles (dg_batch): per order, the blocks of every member of that order in
+// one work list; per step one time-step launch per order, one reduction to the
+// members' steps, one copy of them to the host (the tide and the stage
+// coefficients of every member are evaluated there, as for a single run),
+// then the stages: per order, the wet kernel, the list compaction and the
+// subcell kernel, optionally captured once in a HIP graph per step parity.
+// ---------------------------------------------------------------------------
+struct BatchOrder {
+    unsigned int n_blocks = 0;
+    uint32_t* block_member = nullptr;
+    uint32_t* queue = nullptr;
+    uint8_t* queue_count = nullptr;
+    uint32_t *list = nullptr, *list_member = nullptr;
+    unsigned int *chunk_total = nullptr, *list_count = nullptr;
+    double* block_min = nullptr;
+    unsigned int xcd = 1;            // XCD-aware block mapping (tuned)
+    unsigned int sub_groups = SUB_GROUPS;  // subcell grid (tuned)
+};
+
+struct dg_batch {
+    dg_ctx* owner = nullptr;
+    std::vector<dg_ctx*> members;
+    int scheme = DG_SSP_RK3, ns = 3;
+    double cfl = -1.0;
+    std::vector<BatchMember> h_mem;
+    BatchMember* d_mem = nullptr;
+    BatchOrder ord[4];
+    BatchCoef* d_coef = nullptr;
+    BatchCoef* h_coef = nullptr;  // pinned
+    double* d_member_dt = nullptr;
+    double* h_member_dt = nullptr;  // pinned
+    int parity = 0;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    bool tuned = false;
+};
+
+static BatchArgs batch_args(dg_batch* b, int o, int q, int parity) {
+    BatchOrder& B = b->ord[o];
+    BatchArgs A;
+    A.mem = b->d_mem;
+    A.block_member = B.block_member;
+    A.coef = b->d_coef;
+    A.queue = B.queue;
+    A.queue_count = B.queue_count;
+    A.block_min = B.block_min;
+    A.n_blocks = B.n_blocks;
+    A.xcd_stride = B.xcd ? (B.n_blocks + 7) / 8 : 0;
+    A.q = q;
+    A.ns = b->ns;
+    A.parity = parity;
+    return A;
+}
+
+static int batch_stage_order(dg_batch* b, int o, int q, int parity) {
+    BatchOrder& B = b->ord[o];
+    if (B.n_blocks == 0) return 0;
+    hipStream_t st = b->owner->stream;
+    const BatchArgs A = batch_args(b, o, q, parity);
+    const int epb = 64 / ((o + 1) * (o + 1));
+    const unsigned int nchunk = (B.n_blocks + 1023) / 1024;
+    dim3 grid(A.xcd_stride ? 8 * A.xcd_stride : B.n_blocks), sub_grid(std::min(B.n_blocks, B.sub_groups));
+    switch (o) {
+        case 1: hipLaunchKernelGGL(batch_wet_kernel<1>, grid, dim3(64), 0, st, A); break;
+        case 2: hipLaunchKernelGGL(batch_wet_kernel<2>, grid, dim3(64), 0, st, A); break;
+        default: hipLaunchKernelGGL(batch_wet_kernel<3>, grid, dim3(64), 0, st, A); break;
+    }
+    hipLaunchKernelGGL(chunk_count_kernel, dim3(nchunk), dim3(256), 0, st, B.queue_count, B.n_blocks, B.chunk_total);
+    hipLaunchKernelGGL(batch_compact_kernel, dim3(nchunk), dim3(256), 0, st, B.queue_count, B.queue, B.block_member,
+                       B.n_blocks, B.chunk_total, epb, B.list, B.list_member, B.list_count);
+    switch (o) {
+        case 1: hipLaunchKernelGGL(batch_subcell_kernel<1>, sub_grid, dim3(64), 0, st, A, B.list, B.list_member, B.list_count); break;
+        case 2: hipLaunchKernelGGL(batch_subcell_kernel<2>, sub_grid, dim3(64), 0, st, A, B.list, B.list_member, B.list_count); break;
+        default: hipLaunchKernelGGL(batch_subcell_kernel<3>, sub_grid, dim3(64), 0, st, A, B.list, B.list_member, B.list_count); break;
+    }
+    DG_TRY(hipGetLastError());
+    return 0;
+}
+
+static int batch_stages(dg_batch* b, int parity) {
+    for (int q = 0; q < b->ns; ++q)
+        for (int o = 1; o <= 3; ++o)
+            if (int r = batch_stage_order(b, o, q, parity)) return r;
+    return 0;
+}
+
+static int batch_dt(dg_batch* b) {
+    hipStream_t st = b->owner->stream;
+    for (int o = 1; o <= 3; ++o) {
+        BatchOrder& B = b->ord[o];
+        if (B.n_blocks == 0) continue;
+        const BatchArgs A = batch_args(b, o, 0, b->parity);
+        dim3 grid(A.xcd_stride ? 8 * A.xcd_stride : B.n_blocks);
+        switch (o) {
+            case 1: hipLaunchKernelGGL(batch_dt_kernel<1>, grid, dim3(64), 0, st, A); break;
+            case 2: hipLaunchKernelGGL(batch_dt_kernel<2>, grid, dim3(64), 0, st, A); break;
+            default: hipLaunchKernelGGL(batch_dt_kernel<3>, grid, dim3(64), 0, st, A); break;
+        }
+    }
+    const int M = (int)b->members.size();
+    hipLaunchKernelGGL(batch_member_min_kernel, dim3(M), dim3(256), 0, st, b->d_mem, M, b->d_member_dt);
+    DG_TRY(hipGetLastError());
+    DG_TRY(hipMemcpyAsync(b->h_member_dt, b->d_member_dt, M * sizeof(double), hipMemcpyDeviceToHost, st));
+    DG_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+static int batch_drop_graphs(dg_batch* b) {
+    for (int q = 0; q < 2; ++q) {
+        if (b->exec[q]) DG_TRY(hipGraphExecDestroy(b->exec[q]));
+        if (b->graph[q]) DG_TRY(hipGraphDestroy(b->graph[q]));
+        b->exec[q] = nullptr;
+        b->graph[q] = nullptr;
+    }
+    return 0;
+}
+
+// The autotuner: per order, the wet kernel's block mapping (XCD-aware or
+// linear) and the subcell grid, timed on this ensemble's first stage
+static int batch_tune(dg_batch* b) {
+    hipStream_t st = b->owner->stream;
+    hipEvent_t e0 = b->owner->ev[0], e1 = b->owner->ev[1];
+    static const unsigned int subs[3] = {2048, SUB_GROUPS, 65536};
+    for (int o = 1; o <= 3; ++o) {
+        BatchOrder& B = b->ord[o];
+        if (B.n_blocks == 0) continue;
+        float best = 1e30f;
+        unsigned int bx = 1, bs = SUB_GROUPS;
+        for (unsigned int x = 0; x < 2; ++x)
+            for (unsigned int sg : subs) {
+                B.xcd = x;
+                B.sub_groups = sg;
+                if (int r = batch_stage_order(b, o, 0, b->parity)) return r;  // warm
+                DG_TRY(hipEventRecord(e0, st));
+                for (int rep = 0; rep < 3; ++rep)
+                    if (int r = batch_stage_order(b, o, 0, b->parity)) return r;
+                DG_TRY(hipEventRecord(e1, st));
+                DG_TRY(hipEventSynchronize(e1));
+                float ms = 0.f;
+                DG_TRY(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) { best = ms; bx = x; bs = sg; }
+            }
+        B.xcd = bx;
+        B.sub_groups = bs;
+    }
+    b->tuned = true;
+    return batch_drop_graphs(b);
+}
+
+static int batch_free(dg_batch* b) {
+    if (int r = batch_drop_graphs(b)) return r;
+    for (auto& M : b->h_mem) {
+        for (double* x : M.x)
+            if (x) DG_TRY(hipFree(x));
+        for (double* y : M.y)
+            if (y) DG_TRY(hipFree(y));
+    }
+    for (int o = 1; o <= 3; ++o) {
+        BatchOrder& B = b->ord[o];
+        void* ptrs[] = {B.block_member, B.queue, B.queue_count, B.list, B.list_member, B.chunk_total, B.list_count,
+                        B.block_min};
+        for (void* p : ptrs)
+            if (p) DG_TRY(hipFree(p));
+    }
+    if (b->d_mem) DG_TRY(hipFree(b->d_mem));
+    if (b->d_coef) DG_TRY(hipFree(b->d_coef));
+    if (b->d_member_dt) DG_TRY(hipFree(b->d_member_dt));
+    if (b->h_coef) DG_TRY(hipHostFree(b->h_coef));
+    if (b->h_member_dt) DG_TRY(hipHostFree(b->h_member_dt));
+    return 0;
 }
 
 extern "C" {
@@ -1207,6 +1381,183 @@ int dg_revolve_count(int n_steps, int snapshots, int64_t* forward_steps) {
     Revolve rv;
     rv.init(n_steps, snapshots - 1);
     *forward_steps = n_steps + revolve_count_rec(rv, n_steps, snapshots - 1, true);
+    return 0;
+}
+
+int dg_batch_create(int n, dg_ctx* const* members, int scheme, dg_batch** out) {
+    DG_REQUIRE(n >= 1 && members && out, "dg_batch_create: need at least one member");
+    DG_REQUIRE(scheme == DG_SSP_RK3 || scheme == DG_SSP_RK43, "dg_batch_create: unknown scheme");
+    for (int m = 0; m < n; ++m) {
+        DG_REQUIRE(members[m] && members[m]->K > 0, "dg_batch_create: every member needs an uploaded form");
+        DG_REQUIRE(members[m]->device == members[0]->device, "dg_batch_create: members on different devices");
+    }
+    DG_TRY(hipSetDevice(members[0]->device));
+    dg_batch* b = new dg_batch;
+    b->owner = members[0];
+    b->members.assign(members, members + n);
+    b->scheme = scheme;
+    b->ns = scheme == DG_SSP_RK3 ? 3 : 4;
+    b->h_mem.resize(n);
+    std::vector<uint32_t> bm[4];
+    for (int m = 0; m < n; ++m) {
+        dg_ctx* c = members[m];
+        BatchMember& M = b->h_mem[m];
+        std::memset(&M, 0, sizeof(M));
+        const size_t bytes = (size_t)3 * c->K * c->nn * sizeof(double);
+        for (int q = 0; q < 2; ++q) DG_TRY(hipMalloc(&M.x[q], bytes));
+        for (int q = 0; q < b->ns; ++q) DG_TRY(hipMalloc(&M.y[q], bytes));
+        StageParams& p = M.sp;
+        p.bed = c->bed; p.corners = c->corners; p.manning = c->manning;
+        p.links = c->links; p.tags = c->tags; p.affine = c->affine;
+        p.n_elements = c->K;
+        p.g = c->g; p.h_min = c->h_min; p.h_dry = c->h_dry; p.u_max = c->u_max; p.tau = c->tau; p.f0 = c->f0;
+        p.flags = c->flags;
+        p.clips = c->clips;
+        DtParams& d = M.dp;
+        d.corners = c->corners; d.links = c->links; d.affine = c->affine; d.n_elements = c->K;
+        d.g = c->g; d.h_min = c->h_min; d.h_dry = c->h_dry;
+        d.positivity = (c->flags & DG_WET_DRY) ? 1 : 0;
+        const int o = c->order;
+        const int epb = 64 / c->nn;
+        M.n_blocks = (uint32_t)((c->K + epb - 1) / epb);
+        M.block0 = (uint32_t)bm[o].size();
+        bm[o].insert(bm[o].end(), M.n_blocks, (uint32_t)m);
+    }
+    for (int o = 1; o <= 3; ++o) {
+        BatchOrder& B = b->ord[o];
+        B.n_blocks = (unsigned int)bm[o].size();
+        if (B.n_blocks == 0) continue;
+        const int epb = 64 / ((o + 1) * (o + 1));
+        const size_t nchunk = (B.n_blocks + 1023) / 1024;
+        DG_TRY(hipMalloc(&B.block_member, B.n_blocks * sizeof(uint32_t)));
+        DG_TRY(hipMalloc(&B.queue, (size_t)B.n_blocks * epb * sizeof(uint32_t)));
+        DG_TRY(hipMalloc(&B.queue_count, B.n_blocks));
+        DG_TRY(hipMalloc(&B.list, (size_t)B.n_blocks * epb * sizeof(uint32_t)));
+        DG_TRY(hipMalloc(&B.list_member, (size_t)B.n_blocks * epb * sizeof(uint32_t)));
+        DG_TRY(hipMalloc(&B.chunk_total, nchunk * sizeof(unsigned int)));
+        DG_TRY(hipMalloc(&B.list_count, sizeof(unsigned int)));
+        DG_TRY(hipMalloc(&B.block_min, B.n_blocks * sizeof(double)));
+        DG_TRY(hipMemcpy(B.block_member, bm[o].data(), B.n_blocks * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    for (int m = 0; m < n; ++m) {
+        b->h_mem[m].block_min = b->ord[members[m]->order].block_min;
+        b->h_mem[m].dp.block_min = nullptr;
+    }
+    DG_TRY(hipMalloc(&b->d_mem, n * sizeof(BatchMember)));
+    DG_TRY(hipMalloc(&b->d_coef, (size_t)n * 4 * sizeof(BatchCoef)));
+    DG_TRY(hipMalloc(&b->d_member_dt, n * sizeof(double)));
+    DG_TRY(hipHostMalloc((void**)&b->h_coef, (size_t)n * 4 * sizeof(BatchCoef)));
+    DG_TRY(hipHostMalloc((void**)&b->h_member_dt, n * sizeof(double)));
+    DG_TRY(hipMemcpy(b->d_mem, b->h_mem.data(), n * sizeof(BatchMember), hipMemcpyHostToDevice));
+    *out = b;
+    return 0;
+}
+
+int dg_batch_destroy(dg_batch* b) {
+    if (!b) return 0;
+    DG_TRY(hipSetDevice(b->owner->device));
+    DG_TRY(hipStreamSynchronize(b->owner->stream));
+    if (int r = batch_free(b)) return r;
+    delete b;
+    return 0;
+}
+
+int dg_batch_upload(dg_batch* b, int m, const double* h, const double* hu, const double* hv) {
+    DG_REQUIRE(b && m >= 0 && m < (int)b->members.size() && h && hu && hv, "dg_batch_upload: bad argument");
+    dg_ctx* c = b->members[m];
+    const size_t nt = (size_t)c->K * c->nn;
+    double* x = b->h_mem[m].x[b->parity];
+    hipStream_t st = b->owner->stream;
+    DG_TRY(hipMemcpyAsync(x, h, nt * 8, hipMemcpyHostToDevice, st));
+    DG_TRY(hipMemcpyAsync(x + nt, hu, nt * 8, hipMemcpyHostToDevice, st));
+    DG_TRY(hipMemcpyAsync(x + 2 * nt, hv, nt * 8, hipMemcpyHostToDevice, st));
+    DG_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+int dg_batch_download(dg_batch* b, int m, double* h, double* hu, double* hv) {
+    DG_REQUIRE(b && m >= 0 && m < (int)b->members.size() && h && hu && hv, "dg_batch_download: bad argument");
+    dg_ctx* c = b->members[m];
+    const size_t nt = (size_t)c->K * c->nn;
+    const double* x = b->h_mem[m].x[b->parity];
+    hipStream_t st = b->owner->stream;
+    DG_TRY(hipMemcpyAsync(h, x, nt * 8, hipMemcpyDeviceToHost, st));
+    DG_TRY(hipMemcpyAsync(hu, x + nt, nt * 8, hipMemcpyDeviceToHost, st));
+    DG_TRY(hipMemcpyAsync(hv, x + 2 * nt, nt * 8, hipMemcpyDeviceToHost, st));
+    DG_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+int dg_batch_run(dg_batch* b, int n_steps, double cfl, int options, double* t) {
+    DG_REQUIRE(b && t && n_steps >= 0, "dg_batch_run: bad argument");
+    DG_TRY(hipSetDevice(b->owner->device));
+    const int M = (int)b->members.size();
+    hipStream_t st = b->owner->stream;
+    if (cfl != b->cfl) {
+        // PositivityBound::new per member (swe_2d.rs:2188-2203), as launch_dt
+        static const double lin3[4] = {1.6, 1.04, 0.73, 0.56}, lin43[4] = {2.2, 1.51, 1.15, 0.93};
+        static const double sub3[4] = {0.94, 0.65, 0.47, 0.36}, sub43[4] = {1.5, 1.1, 0.97, 0.75};
+        for (int m = 0; m < M; ++m) {
+            DtParams& d = b->h_mem[m].dp;
+            const int order = b->members[m]->order;
+            const double nn = order;
+            const double ssp = b->scheme == DG_SSP_RK3 ? 1.0 : 2.0;
+            d.cfl = cfl;
+            d.pcfl = ssp * ((2.0 * nn + 1.0) / (2.0 * nn * (nn + 1.0)));
+            d.max_cfl = std::fmax(d.pcfl, 0.9 * (b->scheme == DG_SSP_RK3 ? lin3 : lin43)[order - 1]);
+            d.max_cfl_sub = std::fmax(d.pcfl, 0.9 * (b->scheme == DG_SSP_RK3 ? sub3 : sub43)[order - 1]);
+            d.xcd_stride = 0;
+        }
+        DG_TRY(hipMemcpyAsync(b->d_mem, b->h_mem.data(), M * sizeof(BatchMember), hipMemcpyHostToDevice, st));
+        b->cfl = cfl;
+    }
+    const double(*tab)[4] = b->scheme == DG_SSP_RK3 ? RK3 : RK43;
+    const bool use_graph = (options & DG_BATCH_GRAPH) != 0;
+    for (int s = 0; s < n_steps; ++s) {
+        if (int r = batch_dt(b)) return r;
+        for (int m = 0; m < M; ++m) {
+            const double dt = b->h_member_dt[m];
+            if (!(dt > 0.0) || !std::isfinite(dt)) {
+                g_err = "dg_batch_run: member " + std::to_string(m) + " has no finite positive time step at step " +
+                        std::to_string(s);
+                return 3;
+            }
+            for (int q = 0; q < b->ns; ++q) {
+                BatchCoef& cf = b->h_coef[(size_t)m * 4 + q];
+                cf.a = tab[q][0];
+                cf.b = tab[q][1];
+                cf.c = tab[q][2] * dt;
+                cf.eta = tide_elevation(b->members[m]->tide, t[m] + tab[q][3] * dt);
+            }
+        }
+        DG_TRY(hipMemcpyAsync(b->d_coef, b->h_coef, (size_t)M * 4 * sizeof(BatchCoef), hipMemcpyHostToDevice, st));
+        if ((options & DG_BATCH_TUNE) && !b->tuned)
+            if (int r = batch_tune(b)) return r;
+        const int par = b->parity;
+        if (use_graph) {
+            if (!b->exec[par]) {
+                DG_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                int r = batch_stages(b, par);
+                hipGraph_t g = nullptr;
+                hipError_t e = hipStreamEndCapture(st, &g);
+                if (r) return r;
+                DG_TRY(e);
+                b->graph[par] = g;
+                DG_TRY(hipGraphInstantiate(&b->exec[par], g, nullptr, nullptr, 0));
+            }
+            DG_TRY(hipGraphLaunch(b->exec[par], st));
+        } else if (int r = batch_stages(b, par)) return r;
+        b->parity ^= 1;
+        for (int m = 0; m < M; ++m) t[m] += b->h_member_dt[m];
+    }
+    DG_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+int dg_batch_tuning(const dg_batch* b, int order, int* xcd_mapping, int* subcell_groups) {
+    DG_REQUIRE(b && order >= 1 && order <= 3, "dg_batch_tuning: bad argument");
+    if (xcd_mapping) *xcd_mapping = (int)b->ord[order].xcd;
+    if (subcell_groups) *subcell_groups = (int)b->ord[order].sub_groups;
     return 0;
 }
 

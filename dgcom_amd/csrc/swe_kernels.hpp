@@ -807,7 +807,8 @@ __device__ __forceinline__ double dt_block(const DtParams& p, int64_t blk) {
             }
             double ratio = fmin_(water / water_b, weight / weight_b);
             double rho = fmax_(0.9 * ratio, 1.0);
-            double cap = shallow ? p.max_cfl_sub : p.max_cfl;
+            const double cap_sub = p.max_cfl_sub, cap_lin = p.max_cfl;
+            double cap = shallow ? cap_sub : cap_lin;  // values: selecting a field's address would put p in memory
             cfl = fmin_(cfl, fmin_(p.pcfl * rho, cap));
         }
         double dt = rmax < 1e-14 ? INFINITY : cfl / ((2.0 * N + 1.0) * rmax);

@@ -36,8 +36,10 @@ EXPORTS = (
     "dg_state_zero", "dg_state_wrap", "dg_set_stream", "dg_set_bed", "dg_set_manning", "dg_param_grads_zero",
     "dg_param_grads_read", "dg_rhs_vjp", "dg_stage_vjp", "dg_step_vjp", "dg_set_stations", "dg_sample_stations",
     "dg_tape_create", "dg_tape_destroy", "dg_misfit_forward", "dg_misfit_backward", "dg_tape_stats",
-    "dg_revolve_count",
+    "dg_revolve_count", "dg_batch_create", "dg_batch_destroy", "dg_batch_upload", "dg_batch_download", "dg_batch_run",
+    "dg_batch_tuning",
 )
+BATCH_GRAPH, BATCH_TUNE = 1, 2
 
 
 class Form(ctypes.Structure):
@@ -119,6 +121,12 @@ def load(fma: bool = False):
     L.dg_misfit_backward.argtypes = [vp, vp, ctypes.c_double, vp]
     L.dg_tape_stats.argtypes = [vp, i64p, i64p]
     L.dg_revolve_count.argtypes = [ctypes.c_int, ctypes.c_int, i64p]
+    L.dg_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
+    L.dg_batch_destroy.argtypes = [vp]
+    L.dg_batch_upload.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp]
+    L.dg_batch_download.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp]
+    L.dg_batch_run.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]
+    L.dg_batch_tuning.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     _libs[key] = L
     return L
 
@@ -478,3 +486,50 @@ class Simulation:
             if callback is not None:
                 callback(state, t)
         return t, n
+
+
+class Ensemble:
+    """Independent runs advanced together (dg_batch): members of any order and size,
+    each with its own adaptive time step and time; one launch per order per kernel
+    for the whole ensemble and one host synchronisation per step."""
+
+    def __init__(self, members, scheme: int = SSP_RK3):
+        self.members = list(members)
+        L = self.lib = self.members[0].lib
+        arr = (ctypes.c_void_p * len(self.members))(*[m.ctx.value for m in self.members])
+        self.handle = ctypes.c_void_p()
+        _check(L, L.dg_batch_create(len(self.members), arr, scheme, ctypes.byref(self.handle)), "dg_batch_create")
+        self.t = np.zeros(len(self.members))
+
+    def upload(self, m, h, hu, hv):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (h, hu, hv)]
+        if any(a.size != self.members[m].n_nodes_total for a in arrs):
+            raise ValueError("state arrays must have K*(N+1)^2 values each")
+        _check(self.lib, self.lib.dg_batch_upload(self.handle, m, *(_p(a) for a in arrs)), "dg_batch_upload")
+
+    def download(self, m):
+        out = [np.empty(self.members[m].n_nodes_total) for _ in range(3)]
+        _check(self.lib, self.lib.dg_batch_download(self.handle, m, *(_p(a) for a in out)), "dg_batch_download")
+        return out
+
+    def run(self, n_steps: int, cfl: float, graph: bool = True, tune: bool = False):
+        opts = (BATCH_GRAPH if graph else 0) | (BATCH_TUNE if tune else 0)
+        _check(self.lib, self.lib.dg_batch_run(self.handle, n_steps, cfl, opts, _p(self.t)), "dg_batch_run")
+        return self.t.copy()
+
+    def tuning(self, order: int):
+        x, g = ctypes.c_int(), ctypes.c_int()
+        _check(self.lib, self.lib.dg_batch_tuning(self.handle, order, ctypes.byref(x), ctypes.byref(g)),
+               "dg_batch_tuning")
+        return {"xcd_mapping": bool(x.value), "subcell_groups": g.value}
+
+    def close(self):
+        if self.handle:
+            self.lib.dg_batch_destroy(self.handle)
+        self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

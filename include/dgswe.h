@@ -199,6 +199,29 @@ int dg_misfit_backward(dg_ctx* ctx, dg_tape* tape, double scale, dg_state* cot_x
 int dg_tape_stats(const dg_tape* tape, int64_t* forward_steps, int64_t* step_vjps);
 int dg_revolve_count(int n_steps, int snapshots, int64_t* forward_steps);
 
+/* ------------------------------------------------------------------------
+ * Ensembles: independent runs of different meshes, orders and forcings,
+ * advanced together (the reference runs one case per process; no counterpart).
+ * Each member is a context with its form and tide uploaded; each keeps its own
+ * adaptive time step and time. A step of the ensemble is one launch per order
+ * per kernel over all members' element blocks and one host synchronisation for
+ * all the members' time steps; DG_BATCH_GRAPH replays the stages from a HIP
+ * graph captured on the first step of each parity, DG_BATCH_TUNE times the
+ * launch shapes per order on the first step and keeps the fastest. Every
+ * member's run equals, bit for bit, its dg_run.
+ * ------------------------------------------------------------------------ */
+typedef struct dg_batch dg_batch;
+enum { DG_BATCH_GRAPH = 1, DG_BATCH_TUNE = 2 };
+
+int dg_batch_create(int n_members, dg_ctx* const* members, int scheme, dg_batch** batch);
+int dg_batch_destroy(dg_batch* batch);
+int dg_batch_upload(dg_batch* batch, int member, const double* h, const double* hu, const double* hv);
+int dg_batch_download(dg_batch* batch, int member, double* h, double* hu, double* hv);
+/* n_steps adaptive SSP steps of every member; t[member] is each member's time (in/out). */
+int dg_batch_run(dg_batch* batch, int n_steps, double cfl, int options, double* t);
+/* The launch shape the tuner chose for an order: XCD-aware block mapping (0/1), subcell grid. */
+int dg_batch_tuning(const dg_batch* batch, int order, int* xcd_mapping, int* subcell_groups);
+
 #ifdef __cplusplus
 }
 #endif
