@@ -190,6 +190,7 @@ static int launch_stage(dg_ctx* c, const double* base, const double* input, doub
     p.a = a; p.b = b; p.c = cc;
     p.eta_e = tide_elevation(c->tide, time);
     p.g = c->g; p.h_min = c->h_min; p.h_dry = c->h_dry; p.u_max = c->u_max; p.tau = c->tau; p.f0 = c->f0;
+    p.sqrt_g = std::sqrt(c->g);
     p.rhs_only = rhs_only;
     p.flags = c->flags;
     p.clips = c->clips;
@@ -353,6 +354,7 @@ static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, 
     p.a = a; p.b = b; p.c = cc;
     p.eta_e = tide_elevation(c->tide, time);
     p.g = c->g; p.h_min = c->h_min; p.h_dry = c->h_dry; p.u_max = c->u_max; p.tau = c->tau; p.f0 = c->f0;
+    p.sqrt_g = std::sqrt(c->g);
     p.rhs_only = rhs_only;
     p.flags = c->flags;
     p.clips = c->clips;
@@ -1411,6 +1413,7 @@ int dg_batch_create(int n, dg_ctx* const* members, int scheme, dg_batch** out) {
         p.links = c->links; p.tags = c->tags; p.affine = c->affine;
         p.n_elements = c->K;
         p.g = c->g; p.h_min = c->h_min; p.h_dry = c->h_dry; p.u_max = c->u_max; p.tau = c->tau; p.f0 = c->f0;
+        p.sqrt_g = std::sqrt(c->g);
         p.flags = c->flags;
         p.clips = c->clips;
         DtParams& d = M.dp;

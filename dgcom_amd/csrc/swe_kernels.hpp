@@ -77,6 +77,7 @@ struct StageParams {
     double a, b, c;
     double eta_e;
     double g, h_min, h_dry, u_max, tau, f0;
+    double sqrt_g;       // sqrt(g), correctly rounded on the host
     int rhs_only;
     int flags;
     unsigned long long* clips;
@@ -419,7 +420,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
 #pragma unroll
                     for (int q = 1; q < N1; ++q)
                         if (q == pos + 1) { mx = imx[q]; my = imy[q]; }
-                    hydro_hll(rr[dir][t], rl[dir][t + ls], mx, my, p.g, p.h_min, ifa[dir][t], ifb[dir][t]);
+                    hydro_hll(rr[dir][t], rl[dir][t + ls], mx, my, p.g, p.sqrt_g, p.h_min, ifa[dir][t], ifb[dir][t]);
                 }
             }
         }
@@ -452,9 +453,14 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
                 // reflected ghost on the right
                 const bool wall = l == LINK_BOUNDARY;
                 const bool left = wall || (l & 4u);
-                Cons gs = wall_ghost(q.h, q.hu, q.hv, nx, ny, p.h_min);
-                Node qo = wall ? make_node(gs.h, gs.hu, gs.hv, q.b, p.h_min)
-                               : make_node(nh[r], nhu[r], nhv[r], nbd[r], p.h_min);
+                // the outside state: the neighbour's face node, or at a wall the
+                // reflected ghost (formed only where a lane has a wall face)
+                double oh = nh[r], ohu = nhu[r], ohv = nhv[r], ob = nbd[r];
+                if (wall) {
+                    const Cons gs = wall_ghost(q.h, q.hu, q.hv, nx, ny, p.h_min);
+                    oh = gs.h; ohu = gs.hu; ohv = gs.hv; ob = q.b;
+                }
+                Node qo = make_node(oh, ohu, ohv, ob, p.h_min);
                 double lx = nx, ly = ny;
                 if (!left) {
                     lx = -nx; ly = -ny;
@@ -469,7 +475,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
                 }
                 const Side sa = left ? side(q) : side(qo), sbb = left ? side(qo) : side(q);
                 Cons fa, fb;
-                hydro_hll(sa, sbb, lx, ly, p.g, p.h_min, fa, fb);
+                hydro_hll(sa, sbb, lx, ly, p.g, p.sqrt_g, p.h_min, fa, fb);
                 fstar = left ? fa : cneg(fb);
             }
             if (p.fstar_out) {

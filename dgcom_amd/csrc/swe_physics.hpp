@@ -54,11 +54,11 @@ DG_HD Cons ec_flux(const Node& l, const Node& r, double mx, double my, double g)
 // sides in face-aligned components (depth, normal / tangential momentum and velocity)
 struct Hll { double h, hun, hut, un, ut; };
 
-DG_HD Cons hll_aligned(const Hll& l, const Hll& r, double g, double h_min) {
+// sqg = sqrt(g), evaluated once by the caller
+DG_HD Cons hll_aligned(const Hll& l, const Hll& r, double g, double sqg, double h_min) {
     const double hl = l.h, hr = r.h;
     bool dry = (hl <= h_min) && (hr <= h_min);
     double sql = sqrt(fmax_(hl, 0.0)), sqr = sqrt(fmax_(hr, 0.0));
-    double sqg = sqrt(g);
     double cl = sqg * sql, cr = sqg * sqr;
     bool roe = (sql + sqr) > 1e-10;
     double h_roe = 0.5 * (hl + hr);
@@ -104,13 +104,14 @@ DG_HD Hll hll_side(const Side& q, double b_face, double nx, double ny, double h_
 
 // hydrostatic_hll (swe_2d_split_form.rs:336-402): Audusse states, HLL along m,
 // the hydrostatic correction per side; fa as seen from a, fb from b (both along +m)
-DG_HD void hydro_hll(const Side& qa, const Side& qb, double mx, double my, double g, double h_min, Cons& fa, Cons& fb) {
+DG_HD void hydro_hll(const Side& qa, const Side& qb, double mx, double my, double g, double sqg, double h_min, Cons& fa,
+                     Cons& fb) {
     double norm = sqrt(mx * mx + my * my);
     double inv_norm = 1.0 / norm;
     double nx = mx * inv_norm, ny = my * inv_norm;
     double b_face = fmax_(qa.b, qb.b);
     const Hll sa = hll_side(qa, b_face, nx, ny, h_min), sb = hll_side(qb, b_face, nx, ny, h_min);
-    Cons rot = hll_aligned(sa, sb, g, h_min);
+    Cons rot = hll_aligned(sa, sb, g, sqg, h_min);
     double px = rot.hu * nx - rot.hv * ny, py = rot.hu * ny + rot.hv * nx;
     double fh = norm * rot.h, fhu = norm * px, fhv = norm * py;
     bool hll_dry = (sa.h <= h_min) && (sb.h <= h_min);

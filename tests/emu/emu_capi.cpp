@@ -56,7 +56,7 @@ unsigned long long emu_stage(int order, int64_t K, const double* corners, const 
     p.oh = out; p.ohu = out + nt; p.ohv = out + 2 * nt;
     p.bed = bed; p.corners = corners; p.manning = manning; p.links = links; p.tags = tags; p.affine = affine;
     p.n_elements = K; p.a = a; p.b = b; p.c = c; p.eta_e = eta_e;
-    p.g = g; p.h_min = h_min; p.h_dry = h_dry; p.u_max = u_max; p.tau = tau; p.f0 = f0;
+    p.g = g; p.h_min = h_min; p.h_dry = h_dry; p.u_max = u_max; p.tau = tau; p.f0 = f0; p.sqrt_g = std::sqrt(g);
     p.rhs_only = rhs_only; p.flags = flags; p.clips = &clips; p.fstar_out = fstar_out;
     p.rh = p.rhu = p.rhv = nullptr;
     run_stage(order, K, p);
@@ -79,7 +79,7 @@ void emu_stage_vjp(int order, int64_t K, const double* corners, const uint32_t* 
     p.oh = p.ohu = p.ohv = nullptr;
     p.bed = bed; p.corners = corners; p.manning = manning; p.links = links; p.tags = tags; p.affine = affine;
     p.n_elements = K; p.a = a; p.b = b; p.c = c; p.eta_e = eta_e;
-    p.g = g; p.h_min = h_min; p.h_dry = h_dry; p.u_max = u_max; p.tau = tau; p.f0 = f0;
+    p.g = g; p.h_min = h_min; p.h_dry = h_dry; p.u_max = u_max; p.tau = tau; p.f0 = f0; p.sqrt_g = std::sqrt(g);
     p.rhs_only = rhs_only; p.flags = flags; p.clips = &clips; p.fstar_out = nullptr;
     std::vector<double> rhs(3 * nt, 0.0), fout(3 * nt, 0.0);
     if (!rhs_only) {
