@@ -34,6 +34,7 @@ import numpy as np  # noqa: E402
 from dgcom_amd import cases, swe2d  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 STAGES = 3
 
 
@@ -51,6 +52,18 @@ def read_traffic(workload: str):
         with open(path) as f:
             return json.load(f).get("bytes_per_stage_launch")
     return None
+
+
+def read_fp64(workload: str):
+    """FP64 FLOPs per stage launch (wet + subcell kernels) from rocprofv3's FP64 VALU
+    instruction counters, if that summary is in profiles/fp64_<workload>.json."""
+    path = os.path.join(ROOT, "profiles", f"fp64_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)["per_dispatch"]
+    flops = sum(v["fp64_flops"] for k, v in d.items() if "stage_kernel" in k)
+    return flops, d
 
 
 def cpu_baseline(case, seconds_target: float):
@@ -238,6 +251,7 @@ def main():
     nd, es = phys.masks(state)
     workload = f"large_unstructured_n{args.n}_N3"
     traffic = read_traffic(workload)
+    fp64 = read_fp64(workload)
 
     out = {
         "metric": "SWE-2D RK-stage DOF-updates/s + %HBM roofline; adjoint/forward time ratio",
@@ -267,6 +281,13 @@ def main():
                      "dt_kernel_avg_ms": dt_ms / max(dt_launches, 1)},
         "setup_s": gen_s,
     }
+    if fp64 is not None:
+        flops, per = fp64
+        tf = flops / (stage_ms / max(launches, 1) * 1e-3) / 1e12
+        out["fp64_valu"] = {"achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tf / FP64_PEAK_TFLOPS,
+                            "flops_per_stage_launch": flops,
+                            "wet_kernel_valu_insts_per_wave": per.get("wet_stage_kernel<3>", {}).get("valu_insts_per_wave"),
+                            "wet_kernel_fp64_insts_per_wave": per.get("wet_stage_kernel<3>", {}).get("fp64_insts_per_wave")}
     if not args.no_adjoint:
         phys.close()
         out["adjoint"] = adjoint_bench(args.adjoint_n, args.adjoint_steps, args.adjoint_snapshots, 10, 20)
