@@ -77,7 +77,7 @@ struct dg_ctx {
     // discrete adjoint
     double *gbed = nullptr, *gman = nullptr;          // parameter cotangents [K*NN], [K]
     double *facebuf = nullptr, *innerbuf = nullptr;  // [K][4(N+1)][4] neighbour cotangents
-    double *facecot = nullptr, *ownbuf = nullptr;    // [K][4(N+1)][3], [K][4(N+1)][4] F* cotangents, own sides
+    double* facecot = nullptr;                       // [K][4(N+1)][3] F* cotangents
     uint8_t* kind = nullptr;                         // [K] subcell elements of the last VJP
     dg_state* adj[2] = {nullptr, nullptr};           // stage cotangents of dg_step_vjp
     dg_state* rhsbuf[4] = {nullptr, nullptr, nullptr, nullptr};  // stage RHS saved by the forward for the VJP
@@ -137,10 +137,10 @@ static double tide_elevation(const dg_tide& t, double time) {
 static int free_statics(dg_ctx* c) {
     void* ptrs[] = {c->corners, c->bed, c->manning, c->element_dt, c->links, c->tags, c->affine, c->queue,
                     c->queue_count, c->block_min, c->sub_list, c->chunk_total, c->gbed, c->gman, c->facebuf,
-                    c->innerbuf, c->facecot, c->ownbuf, c->kind, c->st_elem, c->st_w, c->st_eta};
+                    c->innerbuf, c->facecot, c->kind, c->st_elem, c->st_w, c->st_eta};
     for (void* p : ptrs)
         if (p) DG_TRY(hipFree(p));
-    c->gbed = c->gman = c->facebuf = c->innerbuf = c->facecot = c->ownbuf = nullptr;
+    c->gbed = c->gman = c->facebuf = c->innerbuf = c->facecot = nullptr;
     c->kind = nullptr;
     c->st_elem = nullptr;
     c->st_w = c->st_eta = nullptr;
@@ -319,7 +319,6 @@ static int ensure_adjoint(dg_ctx* c) {
     DG_TRY(hipMalloc(&c->facebuf, (size_t)c->K * nf * 4 * sizeof(double)));
     DG_TRY(hipMalloc(&c->innerbuf, (size_t)c->K * nf * 4 * sizeof(double)));
     DG_TRY(hipMalloc(&c->facecot, (size_t)c->K * nf * 3 * sizeof(double)));
-    DG_TRY(hipMalloc(&c->ownbuf, (size_t)c->K * nf * 4 * sizeof(double)));
     DG_TRY(hipMalloc(&c->kind, (size_t)c->K));
     DG_TRY(hipMalloc(&c->gbed, nt * sizeof(double)));
     DG_TRY(hipMalloc(&c->gman, (size_t)c->K * sizeof(double)));
@@ -340,7 +339,8 @@ static int ensure_adjoint(dg_ctx* c) {
 // gathered after the element kernels. rhs: the stage's RHS as its forward
 // saved it (not read for rhs_only).
 static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, const double* rhs, const double* w,
-                            double* gbase, double* ginput, double a, double b, double cc, double time, int rhs_only) {
+                            double* gbase, double* ginput, double a, double b, double cc, double time, int rhs_only,
+                            int input_write = 0) {
     if (int r = ensure_adjoint(c)) return r;
     size_t nt = (size_t)c->K * c->nn;
     VjpParams P;
@@ -367,20 +367,20 @@ static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, 
     P.wh = w; P.whu = w + nt; P.whv = w + 2 * nt;
     P.gbh = gbase; P.gbhu = gbase ? gbase + nt : nullptr; P.gbhv = gbase ? gbase + 2 * nt : nullptr;
     P.gih = ginput; P.gihu = ginput + nt; P.gihv = ginput + 2 * nt;
+    P.input_write = input_write;
     P.gbed = c->gbed;
     P.gman = c->gman;
     P.facebuf = c->facebuf;
     P.innerbuf = c->innerbuf;
     P.facecot = c->facecot;
-    P.ownbuf = c->ownbuf;
     P.kind = c->kind;
     const int epb = 64 / c->nn;
     const unsigned int nblk = (unsigned int)((c->K + epb - 1) / epb);
     const unsigned int nchunk = (nblk + 1023) / 1024;
     p.xcd_stride = (nblk + 7) / 8;
-    const size_t nft = (size_t)c->K * 4 * (c->order + 1);
+    const int epf = 64 / (4 * (c->order + 1));
     dim3 grid(8 * p.xcd_stride), sub_grid(std::min(nblk, SUB_GROUPS)), ggrid((unsigned)((nt + 255) / 256)),
-        fgrid((unsigned)((nft + 255) / 256));
+        fgrid((unsigned)((c->K + epf - 1) / epf));
     switch (c->order) {
         case 1: hipLaunchKernelGGL(wet_vjp_kernel<1>, grid, dim3(64), 0, c->stream, P); break;
         case 2: hipLaunchKernelGGL(wet_vjp_kernel<2>, grid, dim3(64), 0, c->stream, P); break;
@@ -395,15 +395,15 @@ static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, 
         default: hipLaunchKernelGGL(subcell_vjp_kernel<3>, sub_grid, dim3(64), 0, c->stream, P, c->sub_list, c->sub_count); break;
     }
     switch (c->order) {
-        case 1: hipLaunchKernelGGL(face_vjp_kernel<1>, fgrid, dim3(256), 0, c->stream, P); break;
-        case 2: hipLaunchKernelGGL(face_vjp_kernel<2>, fgrid, dim3(256), 0, c->stream, P); break;
-        default: hipLaunchKernelGGL(face_vjp_kernel<3>, fgrid, dim3(256), 0, c->stream, P); break;
+        case 1: hipLaunchKernelGGL(face_vjp_kernel<1>, fgrid, dim3(64), 0, c->stream, P); break;
+        case 2: hipLaunchKernelGGL(face_vjp_kernel<2>, fgrid, dim3(64), 0, c->stream, P); break;
+        default: hipLaunchKernelGGL(face_vjp_kernel<3>, fgrid, dim3(64), 0, c->stream, P); break;
     }
     double *gh = ginput, *ghu = ginput + nt, *ghv = ginput + 2 * nt;
     switch (c->order) {
-        case 1: hipLaunchKernelGGL(vjp_gather_kernel<1>, ggrid, dim3(256), 0, c->stream, c->links, c->ownbuf, c->facebuf, c->innerbuf, c->kind, c->K, gh, ghu, ghv, c->gbed); break;
-        case 2: hipLaunchKernelGGL(vjp_gather_kernel<2>, ggrid, dim3(256), 0, c->stream, c->links, c->ownbuf, c->facebuf, c->innerbuf, c->kind, c->K, gh, ghu, ghv, c->gbed); break;
-        default: hipLaunchKernelGGL(vjp_gather_kernel<3>, ggrid, dim3(256), 0, c->stream, c->links, c->ownbuf, c->facebuf, c->innerbuf, c->kind, c->K, gh, ghu, ghv, c->gbed); break;
+        case 1: hipLaunchKernelGGL(vjp_gather_kernel<1>, ggrid, dim3(256), 0, c->stream, c->links, c->facebuf, c->innerbuf, c->kind, c->K, gh, ghu, ghv, c->gbed); break;
+        case 2: hipLaunchKernelGGL(vjp_gather_kernel<2>, ggrid, dim3(256), 0, c->stream, c->links, c->facebuf, c->innerbuf, c->kind, c->K, gh, ghu, ghv, c->gbed); break;
+        default: hipLaunchKernelGGL(vjp_gather_kernel<3>, ggrid, dim3(256), 0, c->stream, c->links, c->facebuf, c->innerbuf, c->kind, c->K, gh, ghu, ghv, c->gbed); break;
     }
     DG_TRY(hipGetLastError());
     return 0;
@@ -429,14 +429,11 @@ static int step_vjp_impl(dg_ctx* c, const double* X, double t, double dt, int sc
     const double* w = lam_next;
     int tb = 0;
     for (int q = ns - 1; q >= 0; --q) {
+        // the stage values' cotangents are fresh buffers, written (not added to)
         const double* inq = q == 0 ? X : c->work[q - 1]->d;
-        double* gin = lam;
-        if (q > 0) {
-            gin = c->adj[tb]->d;
-            if (int r = zero_async(c, gin, n3)) return r;
-        }
+        double* gin = q > 0 ? c->adj[tb]->d : lam;
         if (int r = launch_stage_vjp(c, X, inq, c->rhsbuf[q]->d, w, lam, gin, st[q][0], st[q][1], st[q][2] * dt,
-                                     t + st[q][3] * dt, 0))
+                                     t + st[q][3] * dt, 0, q > 0 ? 1 : 0))
             return r;
         w = gin;
         tb ^= 1;

@@ -19,17 +19,24 @@
 
 #include "swe_adjoint.hpp"
 
+// Three wavefronts per SIMD for the face kernel (it needs a few registers more)
+#ifdef DGSWE_EMULATE
+#define DG_WAVES_PER_EU(n)
+#else
+#define DG_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
+#endif
+
 namespace dgswe {
 
 struct VjpParams {
     StageParams s;                 // the forward stage's inputs (its outputs are not written)
     const double *wh, *whu, *whv;  // cotangent of the stage output (of the RHS when s.rhs_only)
     double *gbh, *gbhu, *gbhv;     // += a * (cotangent of the combination); null: not wanted
-    double *gih, *gihu, *gihv;     // += the element's own part of the input's cotangent
+    double *gih, *gihu, *gihv;     // += the element's own part of the input's cotangent (= with input_write)
+    int input_write;               // the input's cotangent is a fresh buffer: write it instead of adding
     double* gbed;                  // += the element's own part of the bed's cotangent
     double* gman;                  // += per element Manning n cotangent (null: not wanted)
     double* facecot;               // [K][4(N+1)][3]: cotangent of F* at each face node (element kernels)
-    double* ownbuf;                // [K][4(N+1)][4]: of the face node's own (h, hu, hv, B) through F*
     double* facebuf;               // [K][4(N+1)][4]: cotangent of the neighbour's face node (h, hu, hv, B)
     double* innerbuf;              // [K][4(N+1)][4]: of the neighbour's inner node (subcell elements)
     uint8_t* kind;                 // [K]: 1 where the element takes the subcell volume term
@@ -526,9 +533,15 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
         if (a == 0) nbar_add(nb, tcot[e * NF + 3 * N1 + (N1 - 1 - j)]);
         double gb = 0.0;
         make_node_adj(hin, huin, hvin, p.h_min, nb, gi_h, gi_hu, gi_hv, gb);
-        P.gih[n] += gi_h;
-        P.gihu[n] += gi_hu;
-        P.gihv[n] += gi_hv;
+        if (P.input_write) {
+            P.gih[n] = gi_h;
+            P.gihu[n] = gi_hu;
+            P.gihv[n] = gi_hv;
+        } else {
+            P.gih[n] += gi_h;
+            P.gihu[n] += gi_hu;
+            P.gihv[n] += gi_hv;
+        }
         P.gbed[n] += gb;
     }
     if (P.gman) {
@@ -544,99 +557,126 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
     (void)bin;
 }
 
-// The Riemann solves backwards: one thread per face node of every element
-// (the element's own side of the edge, or its boundary condition), from the
-// cotangent of F* that the element kernels wrote. The cotangent of the
-// element's own face node (h, hu, hv, B) goes to ownbuf, the neighbour's to
-// facebuf (both per face node, so that no two threads write one address).
+// The Riemann solves backwards, one lane per face node, 64 / 4(N+1) elements per
+// workgroup-wavefront, from the cotangent of F* that the element kernels wrote.
+// The cotangent of the neighbour's face node (h, hu, hv, B) goes to facebuf (per
+// face node: no two lanes write one address); those of the element's own face
+// nodes are summed in LDS over the faces a node lies on (two at a corner, in
+// face order) and added to the input's and the bed's cotangents by one lane
+// per node.
 template <int N>
-__global__ __launch_bounds__(256) void face_vjp_kernel(VjpParams P) {
-    constexpr int N1 = N + 1, NN = N1 * N1, NF = 4 * N1;
+__global__ __launch_bounds__(64) DG_WAVES_PER_EU(3) void face_vjp_kernel(VjpParams P) {
+    constexpr int N1 = N + 1, NN = N1 * N1, NF = 4 * N1, EPF = 64 / NF;
     const StageParams& p = P.s;
-    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= (size_t)p.n_elements * NF) return;
-    const int64_t k = (int64_t)(gid / NF);
-    const int task = (int)(gid - (size_t)k * NF);
-    const int f = task / N1, fi = task - f * N1;
-    const size_t n = (size_t)k * NN + face_node<N>(f, fi);
-    const double h = p.ih[n], hu = p.ihu[n], hv = p.ihv[n], bd = p.bed[n];
-    const Node q = make_node(h, hu, hv, bd, p.h_min);
-    const Quad c = load_quad(p.corners, k);
-    double vx, vy;
-    face_vector<N>(c, f, vx, vy);
-    const double sj = sqrt(vx * vx + vy * vy);
-    const double nx = vx / sj, ny = vy / sj;
-    const uint32_t l = p.links[4 * k + f];
-    const double* fc = P.facecot + gid * 3;
-    const Cons fsb = {fc[0], fc[1], fc[2]};
-    NodeBar qb = nodebar_zero();
-    if (l == LINK_BOUNDARY && p.tags[4 * k + f] == DG_BC_TIDE) {
-        const Cons qbd = characteristic(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min);
-        const Node nbn = make_node(qbd.h, qbd.hu, qbd.hv, q.b, p.h_min);
-        NodeBar nbb = nodebar_zero();
-        ec_flux_adj(nbn, nbn, nx, ny, p.g, fsb, nbb, nbb);
-        double bh = 0.0, bhu = 0.0, bhv = 0.0, bbb = 0.0;
-        make_node_adj(qbd.h, qbd.hu, qbd.hv, p.h_min, nbb, bh, bhu, bhv, bbb);
-        qb.b += bbb;
-        characteristic_adj(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min, Cons{bh, bhu, bhv}, qb.h, qb.hu, qb.hv,
-                           qb.b);
-    } else {
-        const bool wall = l == LINK_BOUNDARY;
-        const bool left = wall || (l & 4u);
-        double nhr = 0.0, nhur = 0.0, nhvr = 0.0, nbdr = 0.0;
-        if (!wall) {
-            const size_t m = (size_t)(l >> 4) * NN + face_node<N>((int)(l & 3u), N1 - 1 - fi);
-            nhr = p.ih[m]; nhur = p.ihu[m]; nhvr = p.ihv[m]; nbdr = p.bed[m];
-        }
-        const Cons gs = wall_ghost(q.h, q.hu, q.hv, nx, ny, p.h_min);
-        const Node qo = wall ? make_node(gs.h, gs.hu, gs.hv, q.b, p.h_min) : make_node(nhr, nhur, nhvr, nbdr, p.h_min);
-        double lx = nx, ly = ny;
-        if (!left) {
-            lx = -nx; ly = -ny;
-            if (!(l & 8u)) {
-                const Quad cn = load_quad(p.corners, (int64_t)(l >> 4));
-                double wx, wy;
-                face_vector<N>(cn, (int)(l & 3u), wx, wy);
-                double s2 = sqrt(wx * wx + wy * wy);
-                lx = wx / s2; ly = wy / s2;
+    __shared__ double own[EPF * NF][4];
+    const int t = threadIdx.x;
+    const int e = t / NF;
+    const int task = t - e * NF;
+    const int64_t k = (int64_t)blockIdx.x * EPF + e;
+    const bool active = e < EPF && k < p.n_elements;
+    if (active) {
+        const size_t gid = (size_t)k * NF + task;
+        const int f = task / N1, fi = task - f * N1;
+        const size_t n = (size_t)k * NN + face_node<N>(f, fi);
+        const double h = p.ih[n], hu = p.ihu[n], hv = p.ihv[n], bd = p.bed[n];
+        const Node q = make_node(h, hu, hv, bd, p.h_min);
+        const Quad c = load_quad(p.corners, k);
+        double vx, vy;
+        face_vector<N>(c, f, vx, vy);
+        const double sj = sqrt(vx * vx + vy * vy);
+        const double nx = vx / sj, ny = vy / sj;
+        const uint32_t l = p.links[4 * k + f];
+        const double* fc = P.facecot + gid * 3;
+        const Cons fsb = {fc[0], fc[1], fc[2]};
+        NodeBar qb = nodebar_zero();
+        if (l == LINK_BOUNDARY && p.tags[4 * k + f] == DG_BC_TIDE) {
+            const Cons qbd = characteristic(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min);
+            const Node nbn = make_node(qbd.h, qbd.hu, qbd.hv, q.b, p.h_min);
+            NodeBar nbb = nodebar_zero();
+            ec_flux_adj(nbn, nbn, nx, ny, p.g, fsb, nbb, nbb);
+            double bh = 0.0, bhu = 0.0, bhv = 0.0, bbb = 0.0;
+            make_node_adj(qbd.h, qbd.hu, qbd.hv, p.h_min, nbb, bh, bhu, bhv, bbb);
+            qb.b += bbb;
+            characteristic_adj(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min, Cons{bh, bhu, bhv}, qb.h, qb.hu,
+                               qb.hv, qb.b);
+        } else {
+            const bool wall = l == LINK_BOUNDARY;
+            const bool left = wall || (l & 4u);
+            double nhr = 0.0, nhur = 0.0, nhvr = 0.0, nbdr = 0.0;
+            if (!wall) {
+                const size_t m = (size_t)(l >> 4) * NN + face_node<N>((int)(l & 3u), N1 - 1 - fi);
+                nhr = p.ih[m]; nhur = p.ihu[m]; nhvr = p.ihv[m]; nbdr = p.bed[m];
+            }
+            const Cons gs = wall_ghost(q.h, q.hu, q.hv, nx, ny, p.h_min);
+            const Node qo = wall ? make_node(gs.h, gs.hu, gs.hv, q.b, p.h_min)
+                                 : make_node(nhr, nhur, nhvr, nbdr, p.h_min);
+            double lx = nx, ly = ny;
+            if (!left) {
+                lx = -nx; ly = -ny;
+                if (!(l & 8u)) {
+                    const Quad cn = load_quad(p.corners, (int64_t)(l >> 4));
+                    double wx, wy;
+                    face_vector<N>(cn, (int)(l & 3u), wx, wy);
+                    double s2 = sqrt(wx * wx + wy * wy);
+                    lx = wx / s2; ly = wy / s2;
+                }
+            }
+            const Side sa = left ? side(q) : side(qo), sbs = left ? side(qo) : side(q);
+            // per-field selects (a conditional between two struct lvalues would put them in memory)
+            const double fl = left ? 1.0 : 0.0, fr = left ? 0.0 : -1.0;
+            const Cons fab = {fl * fsb.h, fl * fsb.hu, fl * fsb.hv};
+            const Cons fbb = {fr * fsb.h, fr * fsb.hu, fr * fsb.hv};
+            SideBar sab = sbar_zero(), sbb = sbar_zero();
+            hydro_hll_adj(sa, sbs, lx, ly, p.g, p.h_min, fab, fbb, sab, sbb);
+            const SideBar mine = {left ? sab.h : sbb.h, left ? sab.u : sbb.u, left ? sab.v : sbb.v,
+                                  left ? sab.b : sbb.b};
+            const SideBar os = {left ? sbb.h : sab.h, left ? sbb.u : sab.u, left ? sbb.v : sab.v,
+                                left ? sbb.b : sab.b};
+            nbar_add_side(qb, mine);
+            const NodeBar ob = {os.h, 0.0, 0.0, os.u, os.v, os.b};
+            double oh = 0.0, ohu = 0.0, ohv = 0.0, obb = 0.0;
+            if (wall) {
+                make_node_adj(gs.h, gs.hu, gs.hv, p.h_min, ob, oh, ohu, ohv, obb);
+                qb.b += obb;
+                wall_ghost_adj(q.h, q.hu, q.hv, nx, ny, p.h_min, Cons{oh, ohu, ohv}, qb.h, qb.hu, qb.hv);
+            } else {
+                make_node_adj(nhr, nhur, nhvr, p.h_min, ob, oh, ohu, ohv, obb);
+                double* o = P.facebuf + gid * 4;
+                o[0] = oh; o[1] = ohu; o[2] = ohv; o[3] = obb;
             }
         }
-        const Side sa = left ? side(q) : side(qo), sbs = left ? side(qo) : side(q);
-        // per-field selects (a conditional between two struct lvalues would put them in memory)
-        const double fl = left ? 1.0 : 0.0, fr = left ? 0.0 : -1.0;
-        const Cons fab = {fl * fsb.h, fl * fsb.hu, fl * fsb.hv};
-        const Cons fbb = {fr * fsb.h, fr * fsb.hu, fr * fsb.hv};
-        SideBar sab = sbar_zero(), sbb = sbar_zero();
-        hydro_hll_adj(sa, sbs, lx, ly, p.g, p.h_min, fab, fbb, sab, sbb);
-        const SideBar mine = {left ? sab.h : sbb.h, left ? sab.u : sbb.u, left ? sab.v : sbb.v, left ? sab.b : sbb.b};
-        const SideBar os = {left ? sbb.h : sab.h, left ? sbb.u : sab.u, left ? sbb.v : sab.v, left ? sbb.b : sab.b};
-        nbar_add_side(qb, mine);
-        const NodeBar ob = {os.h, 0.0, 0.0, os.u, os.v, os.b};
-        double oh = 0.0, ohu = 0.0, ohv = 0.0, obb = 0.0;
-        if (wall) {
-            make_node_adj(gs.h, gs.hu, gs.hv, p.h_min, ob, oh, ohu, ohv, obb);
-            qb.b += obb;
-            wall_ghost_adj(q.h, q.hu, q.hv, nx, ny, p.h_min, Cons{oh, ohu, ohv}, qb.h, qb.hu, qb.hv);
-        } else {
-            make_node_adj(nhr, nhur, nhvr, p.h_min, ob, oh, ohu, ohv, obb);
-            double* o = P.facebuf + gid * 4;
-            o[0] = oh; o[1] = ohu; o[2] = ohv; o[3] = obb;
-        }
+        double gh = 0.0, ghu = 0.0, ghv = 0.0, gb = 0.0;
+        make_node_adj(h, hu, hv, p.h_min, qb, gh, ghu, ghv, gb);
+        own[t][0] = gh; own[t][1] = ghu; own[t][2] = ghv; own[t][3] = gb;
     }
-    double gh = 0.0, ghu = 0.0, ghv = 0.0, gb = 0.0;
-    make_node_adj(h, hu, hv, p.h_min, qb, gh, ghu, ghv, gb);
-    double* o = P.ownbuf + gid * 4;
-    o[0] = gh; o[1] = ghu; o[2] = ghv; o[3] = gb;
+    lds_sync();
+    // the element's 4N boundary nodes: node (f, fi < N) of face f; a corner
+    // (fi = 0, vertex f) also ends the face before it
+    if (active && task < 4 * N) {
+        const int f = task / N, fi = task - f * N;
+        const int tf = e * NF + f * N1 + fi;
+        double s0 = own[tf][0], s1 = own[tf][1], s2 = own[tf][2], s3 = own[tf][3];
+        if (fi == 0) {
+            const int tp = e * NF + ((f + 3) & 3) * N1 + N;  // face f-1 (face 3 for vertex 0), last node
+            if (f == 0) { s0 = s0 + own[tp][0]; s1 = s1 + own[tp][1]; s2 = s2 + own[tp][2]; s3 = s3 + own[tp][3]; }
+            else { s0 = own[tp][0] + s0; s1 = own[tp][1] + s1; s2 = own[tp][2] + s2; s3 = own[tp][3] + s3; }
+        }
+        const size_t n = (size_t)k * NN + face_node<N>(f, fi);
+        P.gih[n] += s0;
+        P.gihu[n] += s1;
+        P.gihv[n] += s2;
+        P.gbed[n] += s3;
+    }
 }
 
-// The face parts of the input's and the bed's cotangents: every node adds, face
-// by face in face order, what the face kernel wrote for it (its own side of
-// each face it lies on, and the neighbour side of the neighbour's face) and what
-// a neighbour that took the subcell volume term wrote for it as the inner node
-// behind that neighbour's face. One thread per node.
+// The neighbour parts of the input's and the bed's cotangents: every node adds,
+// face by face in face order, what the face kernel wrote for it from the
+// neighbour's side of the neighbour's face, and what a neighbour that took the
+// subcell volume term wrote for it as the inner node behind that neighbour's
+// face. One thread per node.
 template <int N>
-__global__ __launch_bounds__(256) void vjp_gather_kernel(const uint32_t* links, const double* ownbuf,
-                                                         const double* facebuf, const double* innerbuf,
+__global__ __launch_bounds__(256) void vjp_gather_kernel(const uint32_t* links, const double* facebuf,
+                                                         const double* innerbuf,
                                                          const uint8_t* kind, int64_t K, double* gh, double* ghu,
                                                          double* ghv, double* gbed) {
     constexpr int N1 = N + 1, NN = N1 * N1, NF = 4 * N1;
@@ -658,10 +698,6 @@ __global__ __launch_bounds__(256) void vjp_gather_kernel(const uint32_t* links, 
                      : f == 1 ? (a == N1 - 2 ? j : -1)
                      : f == 2 ? (j == N1 - 2 ? N1 - 1 - a : -1)
                               : (a == 1 ? N1 - 1 - j : -1);
-        if (fi >= 0) {
-            const double* o = ownbuf + ((size_t)k * NF + f * N1 + fi) * 4;
-            sh += o[0]; shu += o[1]; shv += o[2]; sb += o[3];
-        }
         if (l == LINK_BOUNDARY) continue;
         const int64_t nb = (int64_t)(l >> 4);
         const int nbf = (int)(l & 3u);

@@ -93,10 +93,10 @@ void emu_stage_vjp(int order, int64_t K, const double* corners, const uint32_t* 
     P.gbh = gbase; P.gbhu = gbase ? gbase + nt : nullptr; P.gbhv = gbase ? gbase + 2 * nt : nullptr;
     P.gih = ginput; P.gihu = ginput + nt; P.gihv = ginput + 2 * nt;
     P.gbed = gbed; P.gman = gman;
-    std::vector<double> fb((size_t)K * nf * 4, 0.0), ib((size_t)K * nf * 4, 0.0), fc((size_t)K * nf * 3, 0.0),
-        ob((size_t)K * nf * 4, 0.0);
+    std::vector<double> fb((size_t)K * nf * 4, 0.0), ib((size_t)K * nf * 4, 0.0), fc((size_t)K * nf * 3, 0.0);
     std::vector<uint8_t> kind((size_t)K, 0);
-    P.facebuf = fb.data(); P.innerbuf = ib.data(); P.kind = kind.data(); P.facecot = fc.data(); P.ownbuf = ob.data();
+    P.facebuf = fb.data(); P.innerbuf = ib.data(); P.kind = kind.data(); P.facecot = fc.data();
+    P.input_write = 0;
     int epb = 64 / (int)nn;
     unsigned nblk = (unsigned)((K + epb - 1) / epb);
     unsigned nchunk = (nblk + 1023) / 1024;
@@ -120,17 +120,18 @@ void emu_stage_vjp(int order, int64_t K, const double* corners, const uint32_t* 
         case 2: emu_launch(sub_grid, 64, [&] { subcell_vjp_kernel<2>(P, list.data(), &lcount); }); break;
         default: emu_launch(sub_grid, 64, [&] { subcell_vjp_kernel<3>(P, list.data(), &lcount); }); break;
     }
-    unsigned fgrid = (unsigned)((K * nf + 255) / 256);
+    const int epf = 64 / (int)nf;
+    unsigned fgrid = (unsigned)((K + epf - 1) / epf);
     switch (order) {
-        case 1: emu_launch(fgrid, 256, [&] { face_vjp_kernel<1>(P); }); break;
-        case 2: emu_launch(fgrid, 256, [&] { face_vjp_kernel<2>(P); }); break;
-        default: emu_launch(fgrid, 256, [&] { face_vjp_kernel<3>(P); }); break;
+        case 1: emu_launch(fgrid, 64, [&] { face_vjp_kernel<1>(P); }); break;
+        case 2: emu_launch(fgrid, 64, [&] { face_vjp_kernel<2>(P); }); break;
+        default: emu_launch(fgrid, 64, [&] { face_vjp_kernel<3>(P); }); break;
     }
     unsigned ggrid = (unsigned)((nt + 255) / 256);
     switch (order) {
-        case 1: emu_launch(ggrid, 256, [&] { vjp_gather_kernel<1>(links, ob.data(), fb.data(), ib.data(), kind.data(), K, ginput, ginput + nt, ginput + 2 * nt, gbed); }); break;
-        case 2: emu_launch(ggrid, 256, [&] { vjp_gather_kernel<2>(links, ob.data(), fb.data(), ib.data(), kind.data(), K, ginput, ginput + nt, ginput + 2 * nt, gbed); }); break;
-        default: emu_launch(ggrid, 256, [&] { vjp_gather_kernel<3>(links, ob.data(), fb.data(), ib.data(), kind.data(), K, ginput, ginput + nt, ginput + 2 * nt, gbed); }); break;
+        case 1: emu_launch(ggrid, 256, [&] { vjp_gather_kernel<1>(links, fb.data(), ib.data(), kind.data(), K, ginput, ginput + nt, ginput + 2 * nt, gbed); }); break;
+        case 2: emu_launch(ggrid, 256, [&] { vjp_gather_kernel<2>(links, fb.data(), ib.data(), kind.data(), K, ginput, ginput + nt, ginput + 2 * nt, gbed); }); break;
+        default: emu_launch(ggrid, 256, [&] { vjp_gather_kernel<3>(links, fb.data(), ib.data(), kind.data(), K, ginput, ginput + nt, ginput + 2 * nt, gbed); }); break;
     }
 }
 
