@@ -60,6 +60,7 @@ struct dg_ctx {
     double g = 9.81, h_min = 1e-6, h_dry = 1e-3, tau = 1.0, u_max = 20.0, f0 = 0.0;
     dg_tide tide{};
     double *corners = nullptr, *bed = nullptr, *manning = nullptr, *element_dt = nullptr;
+    double *geo = nullptr, *fgeo = nullptr;  // static geometry cache (geometry_kernel)
     uint32_t* links = nullptr;
     uint8_t *tags = nullptr, *affine = nullptr;
     unsigned long long* clips = nullptr;
@@ -137,10 +138,11 @@ static double tide_elevation(const dg_tide& t, double time) {
 static int free_statics(dg_ctx* c) {
     void* ptrs[] = {c->corners, c->bed, c->manning, c->element_dt, c->links, c->tags, c->affine, c->queue,
                     c->queue_count, c->block_min, c->sub_list, c->chunk_total, c->gbed, c->gman, c->facebuf,
-                    c->innerbuf, c->facecot, c->kind, c->st_elem, c->st_w, c->st_eta};
+                    c->innerbuf, c->facecot, c->kind, c->st_elem, c->st_w, c->st_eta, c->geo, c->fgeo};
     for (void* p : ptrs)
         if (p) DG_TRY(hipFree(p));
     c->gbed = c->gman = c->facebuf = c->innerbuf = c->facecot = nullptr;
+    c->geo = c->fgeo = nullptr;
     c->kind = nullptr;
     c->st_elem = nullptr;
     c->st_w = c->st_eta = nullptr;
@@ -177,6 +179,19 @@ static int free_statics(dg_ctx* c) {
     return 0;
 }
 
+// The static geometry cache of the stage kernels (after the form's upload and every bed change)
+static int launch_geometry(dg_ctx* c) {
+    const int epb = 64 / c->nn;
+    dim3 grid((unsigned)((c->K + epb - 1) / epb));
+    switch (c->order) {
+        case 1: hipLaunchKernelGGL(geometry_kernel<1>, grid, dim3(64), 0, c->stream, c->corners, c->bed, c->affine, c->K, c->geo, c->fgeo); break;
+        case 2: hipLaunchKernelGGL(geometry_kernel<2>, grid, dim3(64), 0, c->stream, c->corners, c->bed, c->affine, c->K, c->geo, c->fgeo); break;
+        default: hipLaunchKernelGGL(geometry_kernel<3>, grid, dim3(64), 0, c->stream, c->corners, c->bed, c->affine, c->K, c->geo, c->fgeo); break;
+    }
+    DG_TRY(hipGetLastError());
+    return 0;
+}
+
 static int launch_stage(dg_ctx* c, const double* base, const double* input, double* out, double a, double b, double cc,
                         double time, int rhs_only, double* fstar_out = nullptr, double* rhs_save = nullptr) {
     size_t nt = (size_t)c->K * c->nn;
@@ -195,6 +210,8 @@ static int launch_stage(dg_ctx* c, const double* base, const double* input, doub
     p.flags = c->flags;
     p.clips = c->clips;
     p.fstar_out = fstar_out;
+    p.geo = c->geo;
+    p.fgeo = c->fgeo;
     p.rh = rhs_save;
     p.rhu = rhs_save ? rhs_save + nt : nullptr;
     p.rhv = rhs_save ? rhs_save + 2 * nt : nullptr;
@@ -359,6 +376,8 @@ static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, 
     p.flags = c->flags;
     p.clips = c->clips;
     p.fstar_out = nullptr;
+    p.geo = c->geo;
+    p.fgeo = c->fgeo;
     p.queue = c->queue;
     p.queue_count = c->queue_count;
     p.rh = const_cast<double*>(rhs);
@@ -875,6 +894,10 @@ int dg_upload_form(dg_ctx* c, const dg_form* f) {
     DG_TRY(hipMemcpy(c->links, f->face_links, 4 * K * sizeof(uint32_t), hipMemcpyHostToDevice));
     DG_TRY(hipMemcpy(c->tags, f->face_tags, 4 * K, hipMemcpyHostToDevice));
     DG_TRY(hipMemcpy(c->affine, aff.data(), K, hipMemcpyHostToDevice));
+    DG_TRY(hipMalloc(&c->geo, (size_t)GEO_COUNT * K * nn * sizeof(double)));
+    DG_TRY(hipMalloc(&c->fgeo, (size_t)3 * 4 * K * sizeof(double)));
+    if (int r = launch_geometry(c)) return r;
+    DG_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }
 
@@ -1122,6 +1145,7 @@ int dg_set_bed(dg_ctx* c, const double* bed, int on_device) {
     DG_TRY(hipSetDevice(c->device));
     DG_TRY(hipMemcpyAsync(c->bed, bed, (size_t)c->K * c->nn * 8, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                           c->stream));
+    if (int r = launch_geometry(c)) return r;  // the volume bed-slope factors follow the bed
     DG_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -1411,6 +1435,8 @@ int dg_batch_create(int n, dg_ctx* const* members, int scheme, dg_batch** out) {
         p.n_elements = c->K;
         p.g = c->g; p.h_min = c->h_min; p.h_dry = c->h_dry; p.u_max = c->u_max; p.tau = c->tau; p.f0 = c->f0;
         p.sqrt_g = std::sqrt(c->g);
+        p.geo = c->geo;
+        p.fgeo = c->fgeo;
         p.flags = c->flags;
         p.clips = c->clips;
         DtParams& d = M.dp;

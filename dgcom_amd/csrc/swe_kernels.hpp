@@ -63,6 +63,8 @@ template <int N> __device__ __forceinline__ void face_vector(const Quad& c, int 
     vy = f == 0 ? v0y : f == 1 ? v1y : f == 2 ? v2y : v3y;
 }
 
+enum { GEO_RX, GEO_RY, GEO_SX, GEO_SY, GEO_DET, GEO_DINV, GEO_BXR, GEO_BYR, GEO_BXS, GEO_BYS, GEO_COUNT };
+
 struct StageParams {
     const double *bh, *bhu, *bhv;
     const double *ih, *ihu, *ihv;
@@ -83,6 +85,11 @@ struct StageParams {
     unsigned long long* clips;
     double* fstar_out;   // optional [K][4][N+1][3]: F* at every face node, outward (this element's side)
     double *rh, *rhu, *rhv;  // optional: the stage's RHS, written by the forward, read by the VJP
+    // Static geometry cached per node, SoA [GEO_COUNT][K*NN] (geometry_kernel): the contravariant
+    // metric J grad r, J grad s as the volume term uses it, J, 1/J, and the volume bed-slope
+    // factors of the r- and s-line; per face [3][K*4]: the unit normal and sJ
+    const double* geo;
+    const double* fgeo;
     uint32_t* queue;     // [blocks][EPB]: the block's elements that take the subcell volume term
     uint8_t* queue_count;  // [blocks]: how many
     unsigned int xcd_stride;  // workgroups per XCD share of the wet kernel's grid (0: no remap)
@@ -115,25 +122,25 @@ __device__ __forceinline__ unsigned int xcd_block(unsigned int b, unsigned int s
 // two-point flux and the metric average are bitwise symmetric in the pair.
 template <int N>
 __device__ __forceinline__ void volume_line(Cons& rhs, int p, int lb, int ls, const double* sh, const double* shu,
-                                            const double* shv, const double* su, const double* sv, const double* sb,
-                                            const double* smx, const double* smy, bool affine, double g) {
+                                            const double* shv, const double* su, const double* sv,
+                                            const double* smx, const double* smy, bool affine, double g, double bx,
+                                            double by) {
     constexpr int N1 = N + 1;
     const GllTables& T = c_gll[N];
     const int ip = lb + p * ls;
-    const Node me = {sh[ip], shu[ip], shv[ip], su[ip], sv[ip], sb[ip]};
+    const Node me = {sh[ip], shu[ip], shv[ip], su[ip], sv[ip], 0.0};
     const double mpx = smx[ip], mpy = smy[ip];
     double d[N1];
 #pragma unroll
     for (int q = 0; q < N1; ++q) d[q] = T.d1[p * N1 + q];
     Cons f[N1];
-    double bq[N1], mqx[N1], mqy[N1];
+    double mqx[N1], mqy[N1];
 #pragma unroll
     for (int q = 0; q < N1; ++q) {
         const int iq = lb + q * ls;
-        const Node o = {sh[iq], shu[iq], shv[iq], su[iq], sv[iq], sb[iq]};
+        const Node o = {sh[iq], shu[iq], shv[iq], su[iq], sv[iq], 0.0};
         mqx[q] = smx[iq];
         mqy[q] = smy[iq];
-        bq[q] = o.b;
         const bool own = q == p;
         const double mx = (own || affine) ? mpx : 0.5 * (mqx[q] + mpx);
         const double my = (own || affine) ? mpy : 0.5 * (mqy[q] + mpy);
@@ -149,20 +156,34 @@ __device__ __forceinline__ void volume_line(Cons& rhs, int p, int lb, int ls, co
 #pragma unroll
     for (int q = 0; q < N1; ++q)
         if (q > p) acc = cadd(acc, cscale(2.0 * d[q], f[q]));
-    double db = 0.0, dbx = 0.0, dby = 0.0;
-#pragma unroll
-    for (int q = 0; q < N1; ++q) {
-        double d_b = d[q] * bq[q];
-        db += d_b;
-        dbx += d_b * mqx[q];
-        dby += d_b * mqy[q];
-    }
-    double bx, by;
-    if (affine) { bx = db * mpx; by = db * mpy; }
-    else { bx = 0.5 * (db * mpx + dbx); by = 0.5 * (db * mpy + dby); }
+    // the bed slope's factors (bx, by) are static: geometry_kernel computed them
+    // with bed_line below
     const double force = g * me.h;
     acc = cadd(acc, Cons{0.0, force * bx, force * by});
     rhs = csub(rhs, acc);
+}
+
+// The volume bed-slope factors of the node at position p of a line
+// (swe_2d_split_form.rs:872-890): g h_p (bx, by) is the bed term, with
+// bx = 1/2 (Ja_p (D B)_p + (D (Ja B))_p) (Ja_p (D B)_p when the element is affine)
+template <int N>
+__device__ __forceinline__ void bed_line(int p, int lb, int ls, const double* sb, const double* smx, const double* smy,
+                                         bool affine, double& bx, double& by) {
+    constexpr int N1 = N + 1;
+    const GllTables& T = c_gll[N];
+    const int ip = lb + p * ls;
+    const double mpx = smx[ip], mpy = smy[ip];
+    double db = 0.0, dbx = 0.0, dby = 0.0;
+#pragma unroll
+    for (int q = 0; q < N1; ++q) {
+        const int iq = lb + q * ls;
+        double d_b = T.d1[p * N1 + q] * sb[iq];
+        db += d_b;
+        dbx += d_b * smx[iq];
+        dby += d_b * smy[iq];
+    }
+    if (affine) { bx = db * mpx; by = db * mpy; }
+    else { bx = 0.5 * (db * mpx + dbx); by = 0.5 * (db * mpy + dby); }
 }
 
 // Subcell interface directions of a line (subcells.rs:63-90): im[0..N1]
@@ -322,7 +343,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             }
         }
         hin = p.ih[n]; huin = p.ihu[n]; hvin = p.ihv[n]; bin = p.bed[n];
-        c = load_quad(p.corners, k);
+        if (SUB || VJP) c = load_quad(p.corners, k);
 #pragma unroll
         for (int r = 0; r < NT; ++r) {
             int task = i + r * NN;
@@ -358,19 +379,23 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
     }
 
     bool affine = false;
-    double det = 1.0, det_inv = 1.0;
+    double det = 1.0, det_inv = 1.0, bxr = 0.0, byr = 0.0, bxs = 0.0, bys = 0.0;
     if (active) {
         affine = p.affine[k] != 0;
         Node q = make_node(hin, huin, hvin, bin, p.h_min);
         sh[t] = q.h; shu[t] = q.hu; shv[t] = q.hv; su[t] = q.u; sv[t] = q.v; sb[t] = q.b;
-        Metric m = bilinear(c, T.x1[a], T.x1[j]);
-        det = m.det;
-        Metric mm = affine ? bilinear(c, T.x1[0], T.x1[0]) : m;
-        srx[t] = mm.det * (mm.y_s / mm.det);
-        sry[t] = mm.det * (-mm.x_s / mm.det);
-        ssx[t] = mm.det * (-mm.y_r / mm.det);
-        ssy[t] = mm.det * (mm.x_r / mm.det);
-        det_inv = 1.0 / mm.det;
+        // the static metric of this node, cached by geometry_kernel
+        const size_t nt = (size_t)p.n_elements * NN;
+        srx[t] = p.geo[GEO_RX * nt + n];
+        sry[t] = p.geo[GEO_RY * nt + n];
+        ssx[t] = p.geo[GEO_SX * nt + n];
+        ssy[t] = p.geo[GEO_SY * nt + n];
+        det = p.geo[GEO_DET * nt + n];
+        det_inv = p.geo[GEO_DINV * nt + n];
+        bxr = p.geo[GEO_BXR * nt + n];
+        byr = p.geo[GEO_BYR * nt + n];
+        bxs = p.geo[GEO_BXS * nt + n];
+        bys = p.geo[GEO_BYS * nt + n];
     }
     lds_sync();
 
@@ -436,10 +461,9 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             const int f = task / N1, fi = task - f * N1;
             const int li = eb + face_node<N>(f, fi);
             Node q = {sh[li], shu[li], shv[li], su[li], sv[li], sb[li]};
-            double vx, vy;
-            face_vector<N>(c, f, vx, vy);
-            double sj = sqrt(vx * vx + vy * vy);
-            double nx = vx / sj, ny = vy / sj;
+            // the face's unit normal and sJ, cached by geometry_kernel
+            const size_t fk = (size_t)k * 4 + f, nf4 = (size_t)p.n_elements * 4;
+            const double nx = p.fgeo[fk], ny = p.fgeo[nf4 + fk], sj = p.fgeo[2 * nf4 + fk];
             const uint32_t l = lk[r];
             Cons fstar;
             if (l == LINK_BOUNDARY && tg[r] == DG_BC_TIDE) {
@@ -466,11 +490,8 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
                     lx = -nx; ly = -ny;
                     if (!(l & 8u)) {
                         // a periodic pair: the left side's own normal
-                        const Quad cn = load_quad(p.corners, (int64_t)(l >> 4));
-                        double wx, wy;
-                        face_vector<N>(cn, (int)(l & 3u), wx, wy);
-                        double s2 = sqrt(wx * wx + wy * wy);
-                        lx = wx / s2; ly = wy / s2;
+                        const size_t fo = (size_t)(l >> 4) * 4 + (l & 3u);
+                        lx = p.fgeo[fo]; ly = p.fgeo[nf4 + fo];
                     }
                 }
                 const Side sa = left ? side(q) : side(qo), sbb = left ? side(qo) : side(q);
@@ -498,8 +519,8 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
                 if (rline) subcell_line<N>(rhs, a, rlb, 1, sh, shu, shv, su, sv, sb, srx, sry, affine, p.g, rl[0], rr[0], ifa[0], ifb[0]);
                 else subcell_line<N>(rhs, j, slb, N1, sh, shu, shv, su, sv, sb, ssx, ssy, affine, p.g, rl[1], rr[1], ifa[1], ifb[1]);
             } else {
-                if (rline) volume_line<N>(rhs, a, rlb, 1, sh, shu, shv, su, sv, sb, srx, sry, affine, p.g);
-                else volume_line<N>(rhs, j, slb, N1, sh, shu, shv, su, sv, sb, ssx, ssy, affine, p.g);
+                if (rline) volume_line<N>(rhs, a, rlb, 1, sh, shu, shv, su, sv, srx, sry, affine, p.g, bxr, byr);
+                else volume_line<N>(rhs, j, slb, N1, sh, shu, shv, su, sv, ssx, ssy, affine, p.g, bxs, bys);
             }
         }
         // Surface terms in face order: LIFT sJ (F(q).n - F*)
@@ -605,6 +626,59 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
         if (active) correct_node(vh, vhu, vhv, p.h_dry, p.u_max);
     }
     if (active) { p.oh[n] = vh; p.ohu[n] = vhu; p.ohv[n] = vhv; }
+}
+
+// The static geometry cache of the stage kernels, with the stage kernel's own
+// expressions (so the cached values are its bits): per node the contravariant
+// metric J grad r, J grad s (the affine element's node-0 metric), J at the node,
+// 1/J, and the volume bed-slope factors of its r- and s-line; per face the unit
+// normal and sJ. Run once per form and after every change of the bed.
+template <int N>
+__global__ __launch_bounds__(64) void geometry_kernel(const double* corners, const double* bed, const uint8_t* affine_flags,
+                                                      int64_t K, double* geo, double* fgeo) {
+    constexpr int N1 = N + 1, NN = N1 * N1, EPB = 64 / NN;
+    const GllTables& T = c_gll[N];
+    __shared__ double sb[64], srx[64], sry[64], ssx[64], ssy[64];
+    const int t = threadIdx.x, e = t / NN, i = t - e * NN, a = i % N1, j = i / N1, eb = e * NN;
+    const int64_t k = (int64_t)blockIdx.x * EPB + e;
+    const bool active = e < EPB && k < K;
+    const size_t n = (size_t)k * NN + i, nt = (size_t)K * NN;
+    bool affine = false;
+    if (active) {
+        affine = affine_flags[k] != 0;
+        const Quad c = load_quad(corners, k);
+        Metric m = bilinear(c, T.x1[a], T.x1[j]);
+        Metric mm = affine ? bilinear(c, T.x1[0], T.x1[0]) : m;
+        const double rx = mm.det * (mm.y_s / mm.det), ry = mm.det * (-mm.x_s / mm.det);
+        const double sx = mm.det * (-mm.y_r / mm.det), sy = mm.det * (mm.x_r / mm.det);
+        srx[t] = rx; sry[t] = ry; ssx[t] = sx; ssy[t] = sy;
+        sb[t] = bed[n];
+        geo[GEO_RX * nt + n] = rx;
+        geo[GEO_RY * nt + n] = ry;
+        geo[GEO_SX * nt + n] = sx;
+        geo[GEO_SY * nt + n] = sy;
+        geo[GEO_DET * nt + n] = m.det;
+        geo[GEO_DINV * nt + n] = 1.0 / mm.det;
+        if (i < 4) {
+            double vx, vy;
+            face_vector<N>(c, i, vx, vy);
+            const double sj = sqrt(vx * vx + vy * vy);
+            const size_t fk = (size_t)k * 4 + i, nf4 = (size_t)K * 4;
+            fgeo[fk] = vx / sj;
+            fgeo[nf4 + fk] = vy / sj;
+            fgeo[2 * nf4 + fk] = sj;
+        }
+    }
+    lds_sync();
+    if (active) {
+        double bx, by;
+        bed_line<N>(a, eb + j * N1, 1, sb, srx, sry, affine, bx, by);
+        geo[GEO_BXR * nt + n] = bx;
+        geo[GEO_BYR * nt + n] = by;
+        bed_line<N>(j, eb + a, N1, sb, ssx, ssy, affine, bx, by);
+        geo[GEO_BXS * nt + n] = bx;
+        geo[GEO_BYS * nt + n] = by;
+    }
 }
 
 // Elements with every node at or above h_dry, in mesh order (XCD-aware block

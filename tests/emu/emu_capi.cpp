@@ -13,6 +13,22 @@ void emu_init(void) {
     for (int n = 1; n <= 3; ++n) c_gll[n] = make_tables(n);
 }
 
+// The static geometry cache (geometry_kernel) of a form
+static void run_geometry(int order, int64_t K, StageParams& p, std::vector<double>& geo, std::vector<double>& fgeo) {
+    size_t nn = (size_t)(order + 1) * (order + 1);
+    geo.assign((size_t)GEO_COUNT * K * nn, 0.0);
+    fgeo.assign((size_t)3 * 4 * K, 0.0);
+    int epb = 64 / (int)nn;
+    unsigned grid = (unsigned)((K + epb - 1) / epb);
+    switch (order) {
+        case 1: emu_launch(grid, 64, [&] { geometry_kernel<1>(p.corners, p.bed, p.affine, K, geo.data(), fgeo.data()); }); break;
+        case 2: emu_launch(grid, 64, [&] { geometry_kernel<2>(p.corners, p.bed, p.affine, K, geo.data(), fgeo.data()); }); break;
+        default: emu_launch(grid, 64, [&] { geometry_kernel<3>(p.corners, p.bed, p.affine, K, geo.data(), fgeo.data()); }); break;
+    }
+    p.geo = geo.data();
+    p.fgeo = fgeo.data();
+}
+
 // The forward stage's launch sequence (wet kernel, list compaction, subcell kernel)
 static void run_stage(int order, int64_t K, StageParams& p) {
     size_t nn = (size_t)(order + 1) * (order + 1);
@@ -59,6 +75,8 @@ unsigned long long emu_stage(int order, int64_t K, const double* corners, const 
     p.g = g; p.h_min = h_min; p.h_dry = h_dry; p.u_max = u_max; p.tau = tau; p.f0 = f0; p.sqrt_g = std::sqrt(g);
     p.rhs_only = rhs_only; p.flags = flags; p.clips = &clips; p.fstar_out = fstar_out;
     p.rh = p.rhu = p.rhv = nullptr;
+    std::vector<double> geo, fgeo;
+    run_geometry(order, K, p, geo, fgeo);
     run_stage(order, K, p);
     return clips;
 }
@@ -81,7 +99,8 @@ void emu_stage_vjp(int order, int64_t K, const double* corners, const uint32_t* 
     p.n_elements = K; p.a = a; p.b = b; p.c = c; p.eta_e = eta_e;
     p.g = g; p.h_min = h_min; p.h_dry = h_dry; p.u_max = u_max; p.tau = tau; p.f0 = f0; p.sqrt_g = std::sqrt(g);
     p.rhs_only = rhs_only; p.flags = flags; p.clips = &clips; p.fstar_out = nullptr;
-    std::vector<double> rhs(3 * nt, 0.0), fout(3 * nt, 0.0);
+    std::vector<double> rhs(3 * nt, 0.0), fout(3 * nt, 0.0), geo, fgeo;
+    run_geometry(order, K, p, geo, fgeo);
     if (!rhs_only) {
         p.oh = fout.data(); p.ohu = p.oh + nt; p.ohv = p.oh + 2 * nt;
         p.rh = rhs.data(); p.rhu = p.rh + nt; p.rhv = p.rh + 2 * nt;
