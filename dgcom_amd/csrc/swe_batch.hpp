@@ -62,7 +62,7 @@ __device__ __forceinline__ void batch_stage_params(const BatchArgs& A, uint32_t 
 
 // The wet kernel of one order over the ensemble's blocks of that order
 template <int N>
-__global__ __launch_bounds__(64) void batch_wet_kernel(BatchArgs A) {
+__global__ __launch_bounds__(64) DG_WET_OCCUPANCY void batch_wet_kernel(BatchArgs A) {
     constexpr int NN = (N + 1) * (N + 1), EPB = 64 / NN;
     const unsigned int blk = xcd_block(blockIdx.x, A.xcd_stride);
     if (blk >= A.n_blocks) return;
