@@ -343,7 +343,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             }
         }
         hin = p.ih[n]; huin = p.ihu[n]; hvin = p.ihv[n]; bin = p.bed[n];
-        if (SUB || VJP) c = load_quad(p.corners, k);
+        if (SUB) c = load_quad(p.corners, k);
 #pragma unroll
         for (int r = 0; r < NT; ++r) {
             int task = i + r * NN;

@@ -513,10 +513,8 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
             const int f = task / N1, fi = task - f * N1;
             const int li = eb + face_node<N>(f, fi);
             const Node q = {sh[li], shu[li], shv[li], su[li], sv[li], sb[li]};
-            double vx, vy;
-            face_vector<N>(c, f, vx, vy);
-            const double sj = sqrt(vx * vx + vy * vy);
-            const double nx = vx / sj, ny = vy / sj;
+            const size_t fk = (size_t)k * 4 + f, nf4 = (size_t)p.n_elements * 4;
+            const double nx = p.fgeo[fk], ny = p.fgeo[nf4 + fk], sj = p.fgeo[2 * nf4 + fk];
             const Cons sfdb = cscale(sj * T.lift[f][fi], Cons{swh[li], swhu[li], swhv[li]});
             NodeBar qb = nodebar_zero();
             ec_flux_adj(q, q, nx, ny, p.g, sfdb, qb, qb);
@@ -580,11 +578,8 @@ __global__ __launch_bounds__(64) DG_WAVES_PER_EU(3) void face_vjp_kernel(VjpPara
         const size_t n = (size_t)k * NN + face_node<N>(f, fi);
         const double h = p.ih[n], hu = p.ihu[n], hv = p.ihv[n], bd = p.bed[n];
         const Node q = make_node(h, hu, hv, bd, p.h_min);
-        const Quad c = load_quad(p.corners, k);
-        double vx, vy;
-        face_vector<N>(c, f, vx, vy);
-        const double sj = sqrt(vx * vx + vy * vy);
-        const double nx = vx / sj, ny = vy / sj;
+        const size_t fk = (size_t)k * 4 + f, nf4 = (size_t)p.n_elements * 4;
+        const double nx = p.fgeo[fk], ny = p.fgeo[nf4 + fk];
         const uint32_t l = p.links[4 * k + f];
         const double* fc = P.facecot + gid * 3;
         const Cons fsb = {fc[0], fc[1], fc[2]};
@@ -614,11 +609,8 @@ __global__ __launch_bounds__(64) DG_WAVES_PER_EU(3) void face_vjp_kernel(VjpPara
             if (!left) {
                 lx = -nx; ly = -ny;
                 if (!(l & 8u)) {
-                    const Quad cn = load_quad(p.corners, (int64_t)(l >> 4));
-                    double wx, wy;
-                    face_vector<N>(cn, (int)(l & 3u), wx, wy);
-                    double s2 = sqrt(wx * wx + wy * wy);
-                    lx = wx / s2; ly = wy / s2;
+                    const size_t fo = (size_t)(l >> 4) * 4 + (l & 3u);
+                    lx = p.fgeo[fo]; ly = p.fgeo[nf4 + fo];
                 }
             }
             const Side sa = left ? side(q) : side(qo), sbs = left ? side(qo) : side(q);
