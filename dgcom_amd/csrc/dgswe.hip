@@ -40,8 +40,6 @@ struct DeviceArray {
 
 }  // namespace
 
-// Workgroups of the time-step kernel (a multiple of 8: XCDs)
-constexpr unsigned int DT_GROUPS = 8 * 4096;
 // Workgroups of the subcell kernel (a grid-stride loop over its list)
 constexpr unsigned int SUB_GROUPS = 16384;
 
@@ -66,7 +64,8 @@ struct dg_ctx {
     unsigned long long* clips = nullptr;
     uint32_t* queue = nullptr;
     uint8_t* queue_count = nullptr;
-    double *block_min = nullptr, *dt_value = nullptr;
+    double* dt_value = nullptr;
+    unsigned long long* dt_slots = nullptr;  // [DT_SLOTS]: the time step's running minima (dt_kernel)
     uint32_t* sub_list = nullptr;
     unsigned int *chunk_total = nullptr, *sub_count = nullptr;
     int* flag = nullptr;
@@ -137,7 +136,7 @@ static double tide_elevation(const dg_tide& t, double time) {
 
 static int free_statics(dg_ctx* c) {
     void* ptrs[] = {c->corners, c->bed, c->manning, c->element_dt, c->links, c->tags, c->affine, c->queue,
-                    c->queue_count, c->block_min, c->sub_list, c->chunk_total, c->gbed, c->gman, c->facebuf,
+                    c->queue_count, c->dt_slots, c->sub_list, c->chunk_total, c->gbed, c->gman, c->facebuf,
                     c->innerbuf, c->facecot, c->kind, c->st_elem, c->st_w, c->st_eta, c->geo, c->fgeo};
     for (void* p : ptrs)
         if (p) DG_TRY(hipFree(p));
@@ -166,7 +165,7 @@ static int free_statics(dg_ctx* c) {
     c->tags = c->affine = nullptr;
     c->queue = nullptr;
     c->queue_count = nullptr;
-    c->block_min = nullptr;
+    c->dt_slots = nullptr;
     c->sub_list = nullptr;
     c->chunk_total = nullptr;
     for (auto& w : c->work) {
@@ -252,7 +251,7 @@ static int launch_dt(dg_ctx* c, const double* state, double cfl, int scheme, boo
     size_t nt = (size_t)c->K * c->nn;
     DtParams p;
     p.h = state; p.hu = state + nt; p.hv = state + 2 * nt;
-    p.corners = c->corners; p.links = c->links; p.affine = c->affine;
+    p.geo = c->geo; p.links = c->links; p.affine = c->affine;
     p.n_elements = c->K;
     p.cfl = cfl; p.g = c->g; p.h_min = c->h_min; p.h_dry = c->h_dry;
     // PositivityBound::new (swe_2d.rs:2188-2203)
@@ -265,17 +264,17 @@ static int launch_dt(dg_ctx* c, const double* state, double cfl, int scheme, boo
     p.max_cfl_sub = std::fmax(p.pcfl, 0.9 * (scheme == DG_SSP_RK3 ? sub3 : sub43)[c->order - 1]);
     p.positivity = (c->flags & DG_WET_DRY) ? 1 : 0;
     p.element_dt = element_out ? c->element_dt : nullptr;
-    p.block_min = c->block_min;
-    const unsigned int groups = DT_GROUPS;
-    p.xcd_stride = 0;
-    dim3 grid(groups);
+    p.slots = c->dt_slots;
+    const int64_t epb = 64 / c->nn, nblk = (c->K + epb - 1) / epb;
+    p.xcd_stride = (unsigned int)((nblk + 7) / 8);
+    dim3 grid(8 * p.xcd_stride);
     if (c->timing) DG_TRY(hipEventRecord(c->ev[0], c->stream));
     switch (c->order) {
         case 1: hipLaunchKernelGGL(dt_kernel<1>, grid, dim3(64), 0, c->stream, p); break;
         case 2: hipLaunchKernelGGL(dt_kernel<2>, grid, dim3(64), 0, c->stream, p); break;
         default: hipLaunchKernelGGL(dt_kernel<3>, grid, dim3(64), 0, c->stream, p); break;
     }
-    hipLaunchKernelGGL(min_kernel, dim3(1), dim3(1024), 0, c->stream, c->block_min, (size_t)groups, c->dt_value);
+    hipLaunchKernelGGL(slots_min_kernel, dim3(1), dim3(DT_SLOTS), 0, c->stream, c->dt_slots, c->dt_value);
     DG_TRY(hipGetLastError());
     if (c->timing) DG_TRY(hipEventRecord(c->ev[1], c->stream));
     double value = 0.0;
@@ -884,7 +883,9 @@ int dg_upload_form(dg_ctx* c, const dg_form* f) {
         const int64_t epb = 64 / nn, nblk = (K + epb - 1) / epb;
         DG_TRY(hipMalloc(&c->queue, nblk * epb * sizeof(uint32_t)));
         DG_TRY(hipMalloc(&c->queue_count, nblk));
-        DG_TRY(hipMalloc(&c->block_min, DT_GROUPS * sizeof(double)));
+        DG_TRY(hipMalloc(&c->dt_slots, DT_SLOTS * sizeof(unsigned long long)));
+        const std::vector<unsigned long long> empty(DT_SLOTS, DT_SLOT_EMPTY);
+        DG_TRY(hipMemcpy(c->dt_slots, empty.data(), DT_SLOTS * sizeof(unsigned long long), hipMemcpyHostToDevice));
         DG_TRY(hipMalloc(&c->sub_list, K * sizeof(uint32_t)));
         DG_TRY(hipMalloc(&c->chunk_total, ((nblk + 1023) / 1024) * sizeof(unsigned int)));
     }
@@ -1440,7 +1441,7 @@ int dg_batch_create(int n, dg_ctx* const* members, int scheme, dg_batch** out) {
         p.flags = c->flags;
         p.clips = c->clips;
         DtParams& d = M.dp;
-        d.corners = c->corners; d.links = c->links; d.affine = c->affine; d.n_elements = c->K;
+        d.geo = c->geo; d.links = c->links; d.affine = c->affine; d.n_elements = c->K;
         d.g = c->g; d.h_min = c->h_min; d.h_dry = c->h_dry;
         d.positivity = (c->flags & DG_WET_DRY) ? 1 : 0;
         const int o = c->order;
@@ -1467,7 +1468,7 @@ int dg_batch_create(int n, dg_ctx* const* members, int scheme, dg_batch** out) {
     }
     for (int m = 0; m < n; ++m) {
         b->h_mem[m].block_min = b->ord[members[m]->order].block_min;
-        b->h_mem[m].dp.block_min = nullptr;
+        b->h_mem[m].dp.slots = nullptr;
     }
     DG_TRY(hipMalloc(&b->d_mem, n * sizeof(BatchMember)));
     DG_TRY(hipMalloc(&b->d_coef, (size_t)n * 4 * sizeof(BatchCoef)));

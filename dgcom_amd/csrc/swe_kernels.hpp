@@ -18,6 +18,10 @@ struct GllTables {
     double x1[4], w1[4], d1[16], w2[16];
     double lift[4][4];       // LIFT value at face node fi of face f (its node's entry)
     double xl[4], xr[4];     // subcell a spans [xl[a], xr[a]]
+    // 2 D split by position for the volume term of node p: vsub[p][q] = 2 D_pq
+    // where q < p (terms subtracted from the node), vacc[p][q] = 2 D_pq where
+    // q >= p (terms of the node's own sum); zero elsewhere
+    double vsub[16], vacc[16];
 };
 __constant__ GllTables c_gll[4];
 
@@ -63,7 +67,11 @@ template <int N> __device__ __forceinline__ void face_vector(const Quad& c, int 
     vy = f == 0 ? v0y : f == 1 ? v1y : f == 2 ? v2y : v3y;
 }
 
-enum { GEO_RX, GEO_RY, GEO_SX, GEO_SY, GEO_DET, GEO_DINV, GEO_BXR, GEO_BYR, GEO_BXS, GEO_BYS, GEO_COUNT };
+enum {
+    GEO_RX, GEO_RY, GEO_SX, GEO_SY, GEO_DET, GEO_DINV, GEO_BXR, GEO_BYR, GEO_BXS, GEO_BYS,
+    GEO_TRX, GEO_TRY, GEO_TSX, GEO_TSY,  // grad r, grad s of the node's own map (the time step's metric)
+    GEO_COUNT
+};
 
 struct StageParams {
     const double *bh, *bhu, *bhv;
@@ -123,39 +131,36 @@ __device__ __forceinline__ unsigned int xcd_block(unsigned int b, unsigned int s
 template <int N>
 __device__ __forceinline__ void volume_line(Cons& rhs, int p, int lb, int ls, const double* sh, const double* shu,
                                             const double* shv, const double* su, const double* sv,
-                                            const double* smx, const double* smy, bool affine, double g, double bx,
-                                            double by) {
+                                            const double* smx, const double* smy, double g, double bx, double by) {
     constexpr int N1 = N + 1;
     const GllTables& T = c_gll[N];
     const int ip = lb + p * ls;
     const Node me = {sh[ip], shu[ip], shv[ip], su[ip], sv[ip], 0.0};
     const double mpx = smx[ip], mpy = smy[ip];
-    double d[N1];
-#pragma unroll
-    for (int q = 0; q < N1; ++q) d[q] = T.d1[p * N1 + q];
+    // The metric average of the pair: at the node itself (q = p) and on an
+    // affine element (whose cached metric is the same at every node) the
+    // average of two equal values is that value exactly, so it is formed
+    // for every pair without a select.
     Cons f[N1];
-    double mqx[N1], mqy[N1];
 #pragma unroll
     for (int q = 0; q < N1; ++q) {
         const int iq = lb + q * ls;
         const Node o = {sh[iq], shu[iq], shv[iq], su[iq], sv[iq], 0.0};
-        mqx[q] = smx[iq];
-        mqy[q] = smy[iq];
-        const bool own = q == p;
-        const double mx = (own || affine) ? mpx : 0.5 * (mqx[q] + mpx);
-        const double my = (own || affine) ? mpy : 0.5 * (mqy[q] + mpy);
+        const double mx = 0.5 * (smx[iq] + mpx);
+        const double my = 0.5 * (smy[iq] + mpy);
         f[q] = ec_flux(o, me, mx, my, g);
     }
+    // The reference's order with the lane's position p as data instead of
+    // control (the coefficient tables hold 2 D_pq or zero by position):
+    // subtracting a zero term from the node (q >= p) or adding one to the
+    // node's own sum before its first term (q < p) leaves the value as it is
+    // (up to the sign of a zero result).
     Cons acc = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < N1; ++q) {
-        const Cons t = cscale(2.0 * d[q], f[q]);
-        if (q < p) rhs = csub(rhs, t);
-        if (q == p) acc = t;
+        rhs = csub(rhs, cscale(T.vsub[p * N1 + q], f[q]));
+        acc = cadd(acc, cscale(T.vacc[p * N1 + q], f[q]));
     }
-#pragma unroll
-    for (int q = 0; q < N1; ++q)
-        if (q > p) acc = cadd(acc, cscale(2.0 * d[q], f[q]));
     // the bed slope's factors (bx, by) are static: geometry_kernel computed them
     // with bed_line below
     const double force = g * me.h;
@@ -324,14 +329,22 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
     const size_t n = (size_t)k * NN + i;
     const int eb = e * NN;
 
-    // Loads first: the face tasks' links, this node, the corners, then the
-    // neighbours' face nodes, so that their latency overlaps the volume term.
+    // Every global load is issued here, before the first LDS exchange (whose
+    // fences the compiler does not move loads across): the face tasks' links,
+    // this node, the corners, the neighbours' face nodes, the cached geometry,
+    // the base state and Manning n; one wait for all of them.
     uint32_t lk[NT];
     uint8_t tg[NT];
     double nh[NT], nhu[NT], nhv[NT], nbd[NT];
+    double fnx[NT], fny[NT], fsj[NT];
     Quad c = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     double hin = 0.0, huin = 0.0, hvin = 0.0, bin = 0.0;
+    double g_rx = 0.0, g_ry = 0.0, g_sx = 0.0, g_sy = 0.0, det = 1.0, det_inv = 1.0;
+    double bxr = 0.0, byr = 0.0, bxs = 0.0, bys = 0.0;
+    double bh0 = 0.0, bhu0 = 0.0, bhv0 = 0.0, mann = 0.0;
+    bool affine = false;
     if (active) {
+        affine = p.affine[k] != 0;
 #pragma unroll
         for (int r = 0; r < NT; ++r) {
             int task = i + r * NN;
@@ -354,6 +367,35 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
                 size_t m = (size_t)(l >> 4) * NN + face_node<N>((int)(l & 3u), N1 - 1 - fi);
                 nh[r] = p.ih[m]; nhu[r] = p.ihu[m]; nhv[r] = p.ihv[m]; nbd[r] = p.bed[m];
             }
+        }
+        // the static metric of this node and the faces' unit normals and sJ,
+        // cached by geometry_kernel
+        const size_t nt = (size_t)p.n_elements * NN;
+        g_rx = p.geo[GEO_RX * nt + n];
+        g_ry = p.geo[GEO_RY * nt + n];
+        g_sx = p.geo[GEO_SX * nt + n];
+        g_sy = p.geo[GEO_SY * nt + n];
+        det = p.geo[GEO_DET * nt + n];
+        det_inv = p.geo[GEO_DINV * nt + n];
+        bxr = p.geo[GEO_BXR * nt + n];
+        byr = p.geo[GEO_BYR * nt + n];
+        bxs = p.geo[GEO_BXS * nt + n];
+        bys = p.geo[GEO_BYS * nt + n];
+        if (!VJP) {
+            const size_t nf4 = (size_t)p.n_elements * 4;
+            // (with two face tasks per lane, N < 3, they are loaded at use:
+            // held from here they would not fit the registers)
+#pragma unroll
+            for (int r = 0; r < (NT == 1 ? 1 : 0); ++r) {
+                const int task = i + r * NN;
+                fnx[r] = fny[r] = fsj[r] = 0.0;
+                if (task < NF) {
+                    const size_t fk = (size_t)k * 4 + task / N1;
+                    fnx[r] = p.fgeo[fk]; fny[r] = p.fgeo[nf4 + fk]; fsj[r] = p.fgeo[2 * nf4 + fk];
+                }
+            }
+            if (NT == 1 && !p.rhs_only && p.a != 0.0) { bh0 = p.bh[n]; bhu0 = p.bhu[n]; bhv0 = p.bhv[n]; }
+            if (NT == 1 && (p.flags & DG_FRICTION)) mann = p.manning[k];
         }
     }
     if (t < EPB) { sflag[t] = 0; wflag[t] = 0; }
@@ -378,24 +420,10 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
         if (active && sflag[e] != 0) active = false;
     }
 
-    bool affine = false;
-    double det = 1.0, det_inv = 1.0, bxr = 0.0, byr = 0.0, bxs = 0.0, bys = 0.0;
     if (active) {
-        affine = p.affine[k] != 0;
         Node q = make_node(hin, huin, hvin, bin, p.h_min);
         sh[t] = q.h; shu[t] = q.hu; shv[t] = q.hv; su[t] = q.u; sv[t] = q.v; sb[t] = q.b;
-        // the static metric of this node, cached by geometry_kernel
-        const size_t nt = (size_t)p.n_elements * NN;
-        srx[t] = p.geo[GEO_RX * nt + n];
-        sry[t] = p.geo[GEO_RY * nt + n];
-        ssx[t] = p.geo[GEO_SX * nt + n];
-        ssy[t] = p.geo[GEO_SY * nt + n];
-        det = p.geo[GEO_DET * nt + n];
-        det_inv = p.geo[GEO_DINV * nt + n];
-        bxr = p.geo[GEO_BXR * nt + n];
-        byr = p.geo[GEO_BYR * nt + n];
-        bxs = p.geo[GEO_BXS * nt + n];
-        bys = p.geo[GEO_BYS * nt + n];
+        srx[t] = g_rx; sry[t] = g_ry; ssx[t] = g_sx; ssy[t] = g_sy;
     }
     lds_sync();
 
@@ -461,9 +489,9 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             const int f = task / N1, fi = task - f * N1;
             const int li = eb + face_node<N>(f, fi);
             Node q = {sh[li], shu[li], shv[li], su[li], sv[li], sb[li]};
-            // the face's unit normal and sJ, cached by geometry_kernel
-            const size_t fk = (size_t)k * 4 + f, nf4 = (size_t)p.n_elements * 4;
-            const double nx = p.fgeo[fk], ny = p.fgeo[nf4 + fk], sj = p.fgeo[2 * nf4 + fk];
+            const size_t nf4 = (size_t)p.n_elements * 4, fk = (size_t)k * 4 + f;
+            const double nx = NT == 1 ? fnx[r] : p.fgeo[fk], ny = NT == 1 ? fny[r] : p.fgeo[nf4 + fk],
+                         sj = NT == 1 ? fsj[r] : p.fgeo[2 * nf4 + fk];
             const uint32_t l = lk[r];
             Cons fstar;
             if (l == LINK_BOUNDARY && tg[r] == DG_BC_TIDE) {
@@ -514,14 +542,18 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
         const bool r_first = j <= a;
 #pragma unroll
         for (int pass = 0; pass < 2; ++pass) {
-            bool rline = (pass == 0) == r_first;
-            if (SUB) {
-                if (rline) subcell_line<N>(rhs, a, rlb, 1, sh, shu, shv, su, sv, sb, srx, sry, affine, p.g, rl[0], rr[0], ifa[0], ifb[0]);
-                else subcell_line<N>(rhs, j, slb, N1, sh, shu, shv, su, sv, sb, ssx, ssy, affine, p.g, rl[1], rr[1], ifa[1], ifb[1]);
-            } else {
-                if (rline) volume_line<N>(rhs, a, rlb, 1, sh, shu, shv, su, sv, srx, sry, affine, p.g, bxr, byr);
-                else volume_line<N>(rhs, j, slb, N1, sh, shu, shv, su, sv, ssx, ssy, affine, p.g, bxs, bys);
-            }
+            // The line of this pass as data (a branch on it would diverge
+            // within the wavefront and run both lines' code on every lane)
+            const bool rline = (pass == 0) == r_first;
+            const int lp = rline ? a : j, lb = rline ? rlb : slb, ls = rline ? 1 : N1, dir = rline ? 0 : 1;
+            const double* smx = rline ? srx : ssx;
+            const double* smy = rline ? sry : ssy;
+            if (SUB)
+                subcell_line<N>(rhs, lp, lb, ls, sh, shu, shv, su, sv, sb, smx, smy, affine, p.g, rl[dir], rr[dir],
+                                ifa[dir], ifb[dir]);
+            else
+                volume_line<N>(rhs, lp, lb, ls, sh, shu, shv, su, sv, smx, smy, p.g, rline ? bxr : bxs,
+                               rline ? byr : bys);
         }
         // Surface terms in face order: LIFT sJ (F(q).n - F*)
         if (j == 0) rhs = cadd(rhs, cscale(ssj[e * 4 + 0] * T.lift[0][a], sfd[e * NF + 0 * N1 + a]));
@@ -550,7 +582,10 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
     double vh = 0.0, vhu = 0.0, vhv = 0.0;
     const bool friction = (p.flags & DG_FRICTION) != 0, wet_dry = (p.flags & DG_WET_DRY) != 0;
     if (active) {
-        if (p.a != 0.0) { vh = p.a * p.bh[n]; vhu = p.a * p.bhu[n]; vhv = p.a * p.bhv[n]; }
+        if (p.a != 0.0) {
+            if (NT != 1) { bh0 = p.bh[n]; bhu0 = p.bhu[n]; bhv0 = p.bhv[n]; }
+            vh = p.a * bh0; vhu = p.a * bhu0; vhv = p.a * bhv0;
+        }
         if (p.b != 0.0) { vh += p.b * hin; vhu += p.b * huin; vhv += p.b * hvin; }
         if (p.c != 0.0) { vh += p.c * rhs.h; vhu += p.c * rhs.hu; vhv += p.c * rhs.hv; }
         if (friction || wet_dry) {
@@ -563,7 +598,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
                     else { u = 0.0; v = 0.0; if (hin > p.h_min) { double inv = 1.0 / hin; u = huin * inv; v = hvin * inv; } }
                     double speed = sqrt(u * u + v * v);
                     if (speed > 0.0) {
-                        double mn = p.manning[k];
+                        double mn = NT == 1 ? mann : p.manning[k];
                         double gn2 = p.g * mn * mn;
                         double hh = fmax_(vh, p.h_min);
                         rate += gn2 * speed / (manning_cbrt(hh) * hh);
@@ -659,6 +694,10 @@ __global__ __launch_bounds__(64) void geometry_kernel(const double* corners, con
         geo[GEO_SY * nt + n] = sy;
         geo[GEO_DET * nt + n] = m.det;
         geo[GEO_DINV * nt + n] = 1.0 / mm.det;
+        geo[GEO_TRX * nt + n] = m.y_s / m.det;
+        geo[GEO_TRY * nt + n] = -m.x_s / m.det;
+        geo[GEO_TSX * nt + n] = -m.y_r / m.det;
+        geo[GEO_TSY * nt + n] = m.x_r / m.det;
         if (i < 4) {
             double vx, vy;
             face_vector<N>(c, i, vx, vy);
@@ -789,9 +828,12 @@ __global__ __launch_bounds__(64) void subcell_vjp_kernel(VjpParams P, const uint
 // Time step: per element bound and the global minimum (swe_2d.rs:2072-2258),
 // one lane per node
 // ---------------------------------------------------------------------------
+constexpr int DT_SLOTS = 256;           // partial minima of the time step (atomic, one per 256th of the grid)
+constexpr unsigned long long DT_SLOT_EMPTY = 0x7FF0000000000000ull;  // +infinity
+
 struct DtParams {
     const double *h, *hu, *hv;
-    const double* corners;
+    const double* geo;                  // the geometry cache (geometry_kernel): grad r, grad s and J per node
     const uint32_t* links;
     const uint8_t* affine;
     int64_t n_elements;
@@ -799,7 +841,7 @@ struct DtParams {
     double pcfl, max_cfl, max_cfl_sub;
     int positivity;
     double* element_dt;                 // optional
-    double* block_min;                  // [blocks]: the minimum of each block's elements
+    unsigned long long* slots;          // [DT_SLOTS]: running minima, as the bits of non-negative doubles
     unsigned int xcd_stride;
 };
 
@@ -821,14 +863,16 @@ __device__ __forceinline__ double dt_block(const DtParams& p, int64_t blk) {
     const GllTables& T = c_gll[N];
     __shared__ double mrx[64], mry[64], msx[64], msy[64], sr[64], smass[64], swater[64], shh[64];
     __shared__ double sdt[EPB];
-    const int t = threadIdx.x, e = t / NN, i = t - e * NN, a = i % N1, j = i / N1, eb = e * NN;
+    const int t = threadIdx.x, e = t / NN, i = t - e * NN, eb = e * NN;
     const int64_t k = blk * EPB + e;
     const bool active = e < EPB && k < p.n_elements;
-    const size_t n = (size_t)k * NN + i;
+    const size_t n = (size_t)k * NN + i, nt = (size_t)p.n_elements * NN;
+    // every load first: links, this node, its cached metric and J, the
+    // neighbours' face nodes
     uint32_t lk[NT];
     double nh[NT], nhu[NT], nhv[NT];
-    double h = 0.0, hu = 0.0, hv = 0.0;
-    Quad c = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double h = 0.0, hu = 0.0, hv = 0.0, rx = 0.0, ry = 0.0, sx = 0.0, sy = 0.0, det = 0.0;
+    bool affine = false;
     if (active) {
 #pragma unroll
         for (int r = 0; r < NT; ++r) {
@@ -836,7 +880,10 @@ __device__ __forceinline__ double dt_block(const DtParams& p, int64_t blk) {
             lk[r] = task < NF ? p.links[4 * k + task / N1] : LINK_BOUNDARY;
         }
         h = p.h[n]; hu = p.hu[n]; hv = p.hv[n];
-        c = load_quad(p.corners, k);
+        rx = p.geo[GEO_TRX * nt + n]; ry = p.geo[GEO_TRY * nt + n];
+        sx = p.geo[GEO_TSX * nt + n]; sy = p.geo[GEO_TSY * nt + n];
+        det = p.geo[GEO_DET * nt + n];
+        affine = p.affine[k] != 0;
 #pragma unroll
         for (int r = 0; r < NT; ++r) {
             int task = i + r * NN;
@@ -848,24 +895,20 @@ __device__ __forceinline__ double dt_block(const DtParams& p, int64_t blk) {
             }
         }
     }
-    double rate = 0.0;
     if (active) {
-        bool affine = p.affine[k] != 0;
-        Metric m = bilinear(c, T.x1[a], T.x1[j]);
-        double rx = m.y_s / m.det, ry = -m.x_s / m.det, sx = -m.y_r / m.det, sy = m.x_r / m.det;
         mrx[t] = rx; mry[t] = ry; msx[t] = sx; msy[t] = sy;
-        if (h > p.h_min) {
-            if (affine) {
-                Metric m0 = bilinear(c, T.x1[0], T.x1[0]);
-                rate = node_rate(h, hu, hv, m0.y_s / m0.det, -m0.x_s / m0.det, -m0.y_r / m0.det, m0.x_r / m0.det, p.g, p.h_min);
-            } else rate = node_rate(h, hu, hv, rx, ry, sx, sy, p.g, p.h_min);
-        }
-        smass[t] = T.w2[i] * m.det;
+        smass[t] = T.w2[i] * det;
         swater[t] = smass[t] * fmax_(h, 0.0);
         shh[t] = h;
     }
     lds_sync();
+    double rate = 0.0;
     if (active) {
+        // an affine element's own nodes take its node-0 metric (the map's constant one)
+        if (h > p.h_min) {
+            const int l0 = affine ? eb : t;
+            rate = node_rate(h, hu, hv, mrx[l0], mry[l0], msx[l0], msy[l0], p.g, p.h_min);
+        }
 #pragma unroll
         for (int r = 0; r < NT; ++r) {
             int task = i + r * NN;
@@ -912,18 +955,32 @@ __device__ __forceinline__ double dt_block(const DtParams& p, int64_t blk) {
     return m;
 }
 
-// Grid-stride over element blocks (each XCD a contiguous share); one minimum
-// per workgroup into block_min[blockIdx.x].
+// One element block per workgroup (XCD-aware mapping); the block's minimum is
+// folded into one of DT_SLOTS running minima (an atomic min on the bits, which
+// order like the values for non-negative doubles; the minimum does not depend
+// on the order of the folds).
 template <int N>
 __global__ __launch_bounds__(64) void dt_kernel(DtParams p) {
     constexpr int NN = (N + 1) * (N + 1), EPB = 64 / NN;
     const int64_t nblk = (p.n_elements + EPB - 1) / EPB;
-    const unsigned int per_xcd = gridDim.x / 8;
-    const int64_t share = (nblk + 7) / 8;
-    const int64_t lo = (int64_t)(blockIdx.x & 7u) * share, hi = lo + share < nblk ? lo + share : nblk;
-    double m = INFINITY;
-    for (int64_t blk = lo + (blockIdx.x >> 3); blk < hi; blk += per_xcd) m = fmin_(m, dt_block<N>(p, blk));
-    if (threadIdx.x == 0) p.block_min[blockIdx.x] = m;
+    const unsigned int blk = xcd_block(blockIdx.x, p.xcd_stride);
+    if (blk >= nblk) return;
+    const double m = dt_block<N>(p, blk);
+    if (threadIdx.x == 0) atomicMin(p.slots + (blockIdx.x & (DT_SLOTS - 1)), (unsigned long long)__double_as_longlong(m));
+}
+
+// The minimum of the slots into *out; the slots are left empty for the next
+// time step (so no clearing launch or memset is needed, in a graph either)
+__global__ __launch_bounds__(DT_SLOTS) void slots_min_kernel(unsigned long long* slots, double* out) {
+    __shared__ double red[DT_SLOTS];
+    red[threadIdx.x] = __longlong_as_double((long long)slots[threadIdx.x]);
+    slots[threadIdx.x] = DT_SLOT_EMPTY;
+    __syncthreads();
+    for (int w = DT_SLOTS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = fmin_(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
 }
 
 // The minimum of n values (one workgroup of 1024 lanes), into *out
@@ -1059,6 +1116,11 @@ inline GllTables make_tables(int n) {
     t.d1[n1 * n1 - 1] = (double)(n * (n + 1)) / 4.0;
     for (int j = 0; j < n1; ++j)
         for (int a = 0; a < n1; ++a) t.w2[j * n1 + a] = t.w1[a] * t.w1[j];
+    for (int r = 0; r < n1; ++r)
+        for (int c = 0; c < n1; ++c) {
+            t.vsub[r * n1 + c] = c < r ? 2.0 * t.d1[r * n1 + c] : 0.0;
+            t.vacc[r * n1 + c] = c < r ? 0.0 : 2.0 * t.d1[r * n1 + c];
+        }
     auto fnode = [&](int f, int fi) {
         return f == 0 ? fi : f == 1 ? fi * n1 + n1 - 1 : f == 2 ? (n1 - 1) * n1 + (n1 - 1 - fi) : (n1 - 1 - fi) * n1;
     };

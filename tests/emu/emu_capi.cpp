@@ -160,22 +160,29 @@ double emu_dt(int order, int64_t K, const double* state, const double* corners, 
     size_t nn = (size_t)(order + 1) * (order + 1), nt = (size_t)K * nn;
     int epb = 64 / (int)nn;
     unsigned nblk = (unsigned)((K + epb - 1) / epb);
-    const unsigned groups = 16;
-    std::vector<double> bmin(groups);
+    // the geometry cache of the form (the time step reads its metric and J)
+    StageParams sp;
+    std::memset(&sp, 0, sizeof(sp));
+    std::vector<double> bed(nt, 0.0), geo, fgeo;
+    sp.corners = corners; sp.bed = bed.data(); sp.affine = affine;
+    run_geometry(order, K, sp, geo, fgeo);
+    std::vector<unsigned long long> slots(DT_SLOTS, DT_SLOT_EMPTY);
     DtParams p;
     p.h = state; p.hu = state + nt; p.hv = state + 2 * nt;
-    p.corners = corners; p.links = links; p.affine = affine; p.n_elements = K;
+    p.geo = geo.data(); p.links = links; p.affine = affine; p.n_elements = K;
     p.cfl = cfl; p.g = g; p.h_min = h_min; p.h_dry = h_dry; p.pcfl = pcfl; p.max_cfl = max_cfl;
-    p.max_cfl_sub = max_cfl_sub; p.positivity = positivity; p.element_dt = element_dt; p.block_min = bmin.data();
-    p.xcd_stride = 0;
-    unsigned grid = groups;
+    p.max_cfl_sub = max_cfl_sub; p.positivity = positivity; p.element_dt = element_dt; p.slots = slots.data();
+    p.xcd_stride = (nblk + 7) / 8;
+    unsigned grid = 8 * p.xcd_stride;
     switch (order) {
         case 1: emu_launch(grid, 64, [&] { dt_kernel<1>(p); }); break;
         case 2: emu_launch(grid, 64, [&] { dt_kernel<2>(p); }); break;
         default: emu_launch(grid, 64, [&] { dt_kernel<3>(p); }); break;
     }
     double out = 0.0;
-    emu_launch(1, 1024, [&] { min_kernel(bmin.data(), groups, &out); });
+    emu_launch(1, DT_SLOTS, [&] { slots_min_kernel(slots.data(), &out); });
+    for (unsigned long long v : slots)
+        if (v != DT_SLOT_EMPTY) return -1.0;  // the reduction leaves the slots empty
     return out;
 }
 
