@@ -58,7 +58,9 @@ struct dg_ctx {
     double g = 9.81, h_min = 1e-6, h_dry = 1e-3, tau = 1.0, u_max = 20.0, f0 = 0.0;
     dg_tide tide{};
     double *corners = nullptr, *bed = nullptr, *manning = nullptr, *element_dt = nullptr;
-    double *geo = nullptr, *fgeo = nullptr;  // static geometry cache (geometry_kernel)
+    // static geometry cache (geometry_kernel): geo per node; fgeo per face [3][K*4], then per face node
+    // [K][4][N+1] the outer node's distance
+    double *geo = nullptr, *fgeo = nullptr;
     uint32_t* links = nullptr;
     uint8_t *tags = nullptr, *affine = nullptr;
     unsigned long long* clips = nullptr;
@@ -183,9 +185,9 @@ static int launch_geometry(dg_ctx* c) {
     const int epb = 64 / c->nn;
     dim3 grid((unsigned)((c->K + epb - 1) / epb));
     switch (c->order) {
-        case 1: hipLaunchKernelGGL(geometry_kernel<1>, grid, dim3(64), 0, c->stream, c->corners, c->bed, c->affine, c->K, c->geo, c->fgeo); break;
-        case 2: hipLaunchKernelGGL(geometry_kernel<2>, grid, dim3(64), 0, c->stream, c->corners, c->bed, c->affine, c->K, c->geo, c->fgeo); break;
-        default: hipLaunchKernelGGL(geometry_kernel<3>, grid, dim3(64), 0, c->stream, c->corners, c->bed, c->affine, c->K, c->geo, c->fgeo); break;
+        case 1: hipLaunchKernelGGL(geometry_kernel<1>, grid, dim3(64), 0, c->stream, c->corners, c->bed, c->affine, c->links, c->K, c->geo, c->fgeo); break;
+        case 2: hipLaunchKernelGGL(geometry_kernel<2>, grid, dim3(64), 0, c->stream, c->corners, c->bed, c->affine, c->links, c->K, c->geo, c->fgeo); break;
+        default: hipLaunchKernelGGL(geometry_kernel<3>, grid, dim3(64), 0, c->stream, c->corners, c->bed, c->affine, c->links, c->K, c->geo, c->fgeo); break;
     }
     DG_TRY(hipGetLastError());
     return 0;
@@ -896,7 +898,7 @@ int dg_upload_form(dg_ctx* c, const dg_form* f) {
     DG_TRY(hipMemcpy(c->tags, f->face_tags, 4 * K, hipMemcpyHostToDevice));
     DG_TRY(hipMemcpy(c->affine, aff.data(), K, hipMemcpyHostToDevice));
     DG_TRY(hipMalloc(&c->geo, (size_t)GEO_COUNT * K * nn * sizeof(double)));
-    DG_TRY(hipMalloc(&c->fgeo, (size_t)3 * 4 * K * sizeof(double)));
+    DG_TRY(hipMalloc(&c->fgeo, (size_t)(3 * 4 + 4 * (f->order + 1)) * K * sizeof(double)));
     if (int r = launch_geometry(c)) return r;
     DG_TRY(hipStreamSynchronize(c->stream));
     return 0;

@@ -191,19 +191,22 @@ __device__ __forceinline__ void bed_line(int p, int lb, int ls, const double* sb
     else { bx = 0.5 * (db * mpx + dbx); by = 0.5 * (db * mpy + dby); }
 }
 
-// Subcell interface directions of a line (subcells.rs:63-90): im[0..N1]
+// Subcell interface directions of a line (subcells.rs:63-90): im[0] and im[N1]
+// are the end nodes' metric, im[a+1] the running sum of the telescoped metric
+// over subcells 0..a. Only the pair around position p is kept, as the sums
+// run (ml = im[p], mr = im[p+1]), so the whole table is never held.
 template <int N>
-__device__ __forceinline__ void interfaces(int lb, int ls, const double* smx, const double* smy, bool affine,
-                                           double* imx, double* imy) {
+__device__ __forceinline__ void interface_pair(int p, int lb, int ls, const double* smx, const double* smy, bool affine,
+                                               double& mlx, double& mly, double& mrx, double& mry) {
     constexpr int N1 = N + 1;
     const GllTables& T = c_gll[N];
     if (affine) {
-#pragma unroll
-        for (int a = 0; a <= N1; ++a) { imx[a] = smx[lb]; imy[a] = smy[lb]; }
+        mlx = mrx = smx[lb];
+        mly = mry = smy[lb];
         return;
     }
-    imx[0] = smx[lb];
-    imy[0] = smy[lb];
+    mlx = smx[lb];
+    mly = smy[lb];
     double m0 = 0.0, m1 = 0.0;
 #pragma unroll
     for (int a = 0; a < N1 - 1; ++a) {
@@ -214,11 +217,10 @@ __device__ __forceinline__ void interfaces(int lb, int ls, const double* smx, co
             m0 += q2 * 0.5 * (smx[lb + a * ls] + smx[lb + c * ls]);
             m1 += q2 * 0.5 * (smy[lb + a * ls] + smy[lb + c * ls]);
         }
-        imx[a + 1] = m0;
-        imy[a + 1] = m1;
+        if (a + 1 == p) { mlx = m0; mly = m1; }
+        if (a == p) { mrx = m0; mry = m1; }
     }
-    imx[N1] = smx[lb + (N1 - 1) * ls];
-    imy[N1] = smy[lb + (N1 - 1) * ls];
+    if (p == N1 - 1) { mrx = smx[lb + (N1 - 1) * ls]; mry = smy[lb + (N1 - 1) * ls]; }
 }
 
 // The subcell update's contributions to the node at position p of a line
@@ -237,12 +239,8 @@ __device__ __forceinline__ void subcell_line(Cons& rhs, int p, int lb, int ls, c
     if (p == N1 - 1) rhs = csub(rhs, cscale(1.0 / T.w1[N1 - 1], ec_flux(me, me, smx[ip], smy[ip], g)));
     if (p > 0) rhs = cadd(rhs, cscale(1.0 / T.w1[p], ifb[ip - ls]));
     if (p < N1 - 1) rhs = csub(rhs, cscale(1.0 / T.w1[p], ifa[ip]));
-    double imx[N1 + 1], imy[N1 + 1];
-    interfaces<N>(lb, ls, smx, smy, affine, imx, imy);
-    double mlx = imx[0], mly = imy[0], mrx = imx[1], mry = imy[1];
-#pragma unroll
-    for (int q = 1; q < N1; ++q)
-        if (q == p) { mlx = imx[q]; mly = imy[q]; mrx = imx[q + 1]; mry = imy[q + 1]; }
+    double mlx, mly, mrx, mry;
+    interface_pair<N>(p, lb, ls, smx, smy, affine, mlx, mly, mrx, mry);
     const Side& left = rl[ip];
     const Side& right = rr[ip];
     double bed = -0.5 * g * (left.h + right.h) * (right.b - left.b) / T.w1[p];
@@ -270,25 +268,37 @@ __device__ __forceinline__ double face_height(const Quad& c, bool affine, int f,
 }
 
 // The outer node of face node fi of face f (swe_2d_split_form.rs:1038-1085): the
-// neighbour's first interior node on the GLL line and its reference distance
+// neighbour's first interior node on the GLL line, and its reference distance
+// (static: geometry_kernel caches it per face node with outer_distance below)
 template <int N>
-__device__ __forceinline__ bool outer_node(const StageParams& p, int64_t k, const Quad& c, bool affine, int f, int fi,
-                                           Side& out, double& dist) {
+__device__ __forceinline__ bool outer_node(const StageParams& p, int64_t k, int f, int fi, Side& out, double& dist) {
     constexpr int N1 = N + 1, NN = N1 * N1;
-    const GllTables& T = c_gll[N];
     uint32_t l = p.links[4 * k + f];
     if (l == LINK_BOUNDARY) return false;
     int64_t nb = (int64_t)(l >> 4);
     int nbf = (int)(l & 3u);
     int nb_node = face_node<N>(nbf, N1 - 1 - fi);
     int inner = nbf == 0 ? nb_node + N1 : nbf == 1 ? nb_node - 1 : nbf == 2 ? nb_node - N1 : nb_node + 1;
-    const Quad cn = load_quad(p.corners, nb);
-    bool nb_affine = p.affine[nb] != 0;
-    dist = (T.x1[1] - T.x1[0]) * face_height<N>(cn, nb_affine, nbf, N1 - 1 - fi) / face_height<N>(c, affine, f, fi);
+    dist = p.fgeo[(size_t)12 * p.n_elements + ((size_t)k * 4 + f) * N1 + fi];
     size_t n = (size_t)nb * NN + inner;
     Node q = make_node(p.ih[n], p.ihu[n], p.ihv[n], p.bed[n], p.h_min);
     out = side(q);
     return true;
+}
+
+// The reference distance of the outer node behind face node fi of face f:
+// the neighbour's first GLL spacing scaled by the ratio of the element
+// heights normal to the face (swe_2d_split_form.rs:1061-1080)
+template <int N>
+__device__ __forceinline__ double outer_distance(const double* corners, const uint8_t* affine_flags, const Quad& c,
+                                                 bool affine, uint32_t l, int f, int fi) {
+    constexpr int N1 = N + 1;
+    const GllTables& T = c_gll[N];
+    const int64_t nb = (int64_t)(l >> 4);
+    const int nbf = (int)(l & 3u);
+    const Quad cn = load_quad(corners, nb);
+    const bool nb_affine = affine_flags[nb] != 0;
+    return (T.x1[1] - T.x1[0]) * face_height<N>(cn, nb_affine, nbf, N1 - 1 - fi) / face_height<N>(c, affine, f, fi);
 }
 
 // One workgroup-wavefront of EPB elements through a fused stage: the element
@@ -356,7 +366,6 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             }
         }
         hin = p.ih[n]; huin = p.ihu[n]; hvin = p.ihv[n]; bin = p.bed[n];
-        if (SUB) c = load_quad(p.corners, k);
 #pragma unroll
         for (int r = 0; r < NT; ++r) {
             int task = i + r * NN;
@@ -430,55 +439,6 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
     // Lines through this node: the r-line (lanes eb + j*N1 + q) and the s-line (eb + q*N1 + a)
     const int rlb = eb + j * N1, slb = eb + a;
 
-    if (SUB) {
-        // Reconstructed face states of this node's subcell on its r- and s-line
-        if (active) {
-#pragma unroll
-            for (int dir = 0; dir < 2; ++dir) {
-                const int lb = dir == 0 ? rlb : slb, ls = dir == 0 ? 1 : N1;
-                int pos = dir == 0 ? a : j;
-                Side me = {sh[t], su[t], sv[t], sb[t]};
-                Side lft, rgt;
-                double xlf = 0.0, xrt = 0.0;
-                bool has_l = true, has_r = true;
-                if (pos > 0) { int q = t - ls; lft = {sh[q], su[q], sv[q], sb[q]}; xlf = T.x1[pos - 1]; }
-                else {
-                    double d;
-                    has_l = dir == 0 ? outer_node<N>(p, k, c, affine, 3, N1 - 1 - j, lft, d)
-                                     : outer_node<N>(p, k, c, affine, 0, a, lft, d);
-                    xlf = -1.0 - d;
-                }
-                if (pos < N1 - 1) { int q = t + ls; rgt = {sh[q], su[q], sv[q], sb[q]}; xrt = T.x1[pos + 1]; }
-                else {
-                    double d;
-                    has_r = dir == 0 ? outer_node<N>(p, k, c, affine, 1, j, rgt, d)
-                                     : outer_node<N>(p, k, c, affine, 2, N1 - 1 - a, rgt, d);
-                    xrt = 1.0 + d;
-                }
-                if (has_l && has_r)
-                    reconstruct(lft, me, rgt, xlf, T.x1[pos], xrt, T.xl[pos], T.xr[pos], p.h_dry, rl[dir][t], rr[dir][t]);
-                else { rl[dir][t] = me; rr[dir][t] = me; }
-            }
-        }
-        lds_sync();
-        if (active && !VJP) {
-#pragma unroll
-            for (int dir = 0; dir < 2; ++dir) {
-                const int lb = dir == 0 ? rlb : slb, ls = dir == 0 ? 1 : N1;
-                int pos = dir == 0 ? a : j;
-                if (pos < N1 - 1) {
-                    double imx[N1 + 1], imy[N1 + 1];
-                    interfaces<N>(lb, ls, dir == 0 ? srx : ssx, dir == 0 ? sry : ssy, affine, imx, imy);
-                    double mx = imx[1], my = imy[1];
-#pragma unroll
-                    for (int q = 1; q < N1; ++q)
-                        if (q == pos + 1) { mx = imx[q]; my = imy[q]; }
-                    hydro_hll(rr[dir][t], rl[dir][t + ls], mx, my, p.g, p.sqrt_g, p.h_min, ifa[dir][t], ifb[dir][t]);
-                }
-            }
-        }
-    }
-
     // Faces: F* at the element's face nodes, spread over its lanes (the VJP
     // reads the RHS that its forward saved instead)
     if (active && !VJP) {
@@ -535,13 +495,59 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             if (fi == 0) ssj[e * 4 + f] = sj;
         }
     }
+    // (the subcell path after the faces: the faces' registers are free by then)
+    if (SUB) {
+        // Reconstructed face states of this node's subcell on its r- and s-line
+        if (active) {
+#pragma unroll
+            for (int dir = 0; dir < 2; ++dir) {
+                const int lb = dir == 0 ? rlb : slb, ls = dir == 0 ? 1 : N1;
+                int pos = dir == 0 ? a : j;
+                Side me = {sh[t], su[t], sv[t], sb[t]};
+                Side lft = me, rgt = me;
+                double xlf = 0.0, xrt = 0.0;
+                bool has_l = true, has_r = true;
+                if (pos > 0) { int q = t - ls; lft = {sh[q], su[q], sv[q], sb[q]}; xlf = T.x1[pos - 1]; }
+                if (pos < N1 - 1) { int q = t + ls; rgt = {sh[q], su[q], sv[q], sb[q]}; xrt = T.x1[pos + 1]; }
+                // A line end's neighbour is the outer node behind the face there;
+                // a node is at one end of its line at most (N >= 1), so one read
+                // per direction with the face as data (not a branch per end)
+                const bool at_l = pos == 0;
+                if (at_l || pos == N1 - 1) {
+                    const int f = dir == 0 ? (at_l ? 3 : 1) : (at_l ? 0 : 2);
+                    const int fi = dir == 0 ? (at_l ? N1 - 1 - j : j) : (at_l ? a : N1 - 1 - a);
+                    Side o = me;
+                    double d = 0.0;
+                    const bool has = outer_node<N>(p, k, f, fi, o, d);
+                    if (at_l) { lft = o; xlf = -1.0 - d; has_l = has; }
+                    else { rgt = o; xrt = 1.0 + d; has_r = has; }
+                }
+                if (has_l && has_r)
+                    reconstruct(lft, me, rgt, xlf, T.x1[pos], xrt, T.xl[pos], T.xr[pos], p.h_dry, rl[dir][t], rr[dir][t]);
+                else { rl[dir][t] = me; rr[dir][t] = me; }
+            }
+        }
+        lds_sync();
+        if (active && !VJP) {
+#pragma unroll
+            for (int dir = 0; dir < 2; ++dir) {
+                const int lb = dir == 0 ? rlb : slb, ls = dir == 0 ? 1 : N1;
+                int pos = dir == 0 ? a : j;
+                if (pos < N1 - 1) {
+                    double mlx, mly, mx, my;  // the interface to the right of this subcell: im[pos + 1]
+                    interface_pair<N>(pos, lb, ls, dir == 0 ? srx : ssx, dir == 0 ? sry : ssy, affine, mlx, mly, mx, my);
+                    hydro_hll(rr[dir][t], rl[dir][t + ls], mx, my, p.g, p.sqrt_g, p.h_min, ifa[dir][t], ifb[dir][t]);
+                }
+            }
+        }
+    }
+
     lds_sync();
 
     Cons rhs = {0.0, 0.0, 0.0};
     if (active && !VJP) {
         const bool r_first = j <= a;
-#pragma unroll
-        for (int pass = 0; pass < 2; ++pass) {
+        auto line_pass = [&](int pass) {
             // The line of this pass as data (a branch on it would diverge
             // within the wavefront and run both lines' code on every lane)
             const bool rline = (pass == 0) == r_first;
@@ -554,6 +560,15 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             else
                 volume_line<N>(rhs, lp, lb, ls, sh, shu, shv, su, sv, smx, smy, p.g, rline ? bxr : bxs,
                                rline ? byr : bys);
+        };
+        if constexpr (SUB) {
+            // one copy of the subcell line for both passes (two copies
+            // scheduled together would not fit two wavefronts per SIMD)
+#pragma unroll 1
+            for (int pass = 0; pass < 2; ++pass) line_pass(pass);
+        } else {
+#pragma unroll
+            for (int pass = 0; pass < 2; ++pass) line_pass(pass);
         }
         // Surface terms in face order: LIFT sJ (F(q).n - F*)
         if (j == 0) rhs = cadd(rhs, cscale(ssj[e * 4 + 0] * T.lift[0][a], sfd[e * NF + 0 * N1 + a]));
@@ -670,7 +685,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
 // normal and sJ. Run once per form and after every change of the bed.
 template <int N>
 __global__ __launch_bounds__(64) void geometry_kernel(const double* corners, const double* bed, const uint8_t* affine_flags,
-                                                      int64_t K, double* geo, double* fgeo) {
+                                                      const uint32_t* links, int64_t K, double* geo, double* fgeo) {
     constexpr int N1 = N + 1, NN = N1 * N1, EPB = 64 / NN;
     const GllTables& T = c_gll[N];
     __shared__ double sb[64], srx[64], sry[64], ssx[64], ssy[64];
@@ -706,6 +721,14 @@ __global__ __launch_bounds__(64) void geometry_kernel(const double* corners, con
             fgeo[fk] = vx / sj;
             fgeo[nf4 + fk] = vy / sj;
             fgeo[2 * nf4 + fk] = sj;
+        }
+        // per face node, after the faces: the outer node's reference distance
+        constexpr int NF = 4 * N1;
+        for (int task = i; task < NF; task += NN) {
+            const int f = task / N1, fi = task - f * N1;
+            const uint32_t l = links[4 * k + f];
+            fgeo[(size_t)12 * K + (size_t)k * NF + task] =
+                l == LINK_BOUNDARY ? 0.0 : outer_distance<N>(corners, affine_flags, c, affine, l, f, fi);
         }
     }
     lds_sync();
@@ -787,9 +810,16 @@ __global__ __launch_bounds__(256) void compact_kernel(const uint8_t* counts, con
 }
 
 // The listed elements with a shallow node (subcell volume term), EPB per
-// workgroup pass, grid-stride over the compacted list.
+// workgroup pass, grid-stride over the compacted list. Two wavefronts per SIMD
+// (the register budget of 256 holds it without spills; its LDS allows 7
+// workgroups per CU).
+#ifdef DGSWE_EMULATE
+#define DG_SUB_OCCUPANCY
+#else
+#define DG_SUB_OCCUPANCY __attribute__((amdgpu_waves_per_eu(2)))
+#endif
 template <int N>
-__global__ __launch_bounds__(64) void subcell_stage_kernel(StageParams p, const uint32_t* list, const unsigned int* count_ptr) {
+__global__ __launch_bounds__(64) DG_SUB_OCCUPANCY void subcell_stage_kernel(StageParams p, const uint32_t* list, const unsigned int* count_ptr) {
     constexpr int NN = (N + 1) * (N + 1), EPB = 64 / NN;
     const unsigned int count = *count_ptr;
     const int e = threadIdx.x / NN;

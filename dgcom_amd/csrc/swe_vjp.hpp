@@ -399,12 +399,8 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
                 if (pp == N1 - 1) ec_flux_adj(me, me, smx[t], smy[t], p.g, cscale(-(1.0 / T.w1[N1 - 1]), Wp), nb, nb);
                 if (pp > 0) ifbb[dir][t - ls] = cscale(iw, Wp);
                 ifab[dir][t] = pp < N1 - 1 ? cscale(-iw, Wp) : Cons{0.0, 0.0, 0.0};
-                double imx[N1 + 1], imy[N1 + 1];
-                interfaces<N>(lb, ls, smx, smy, affine, imx, imy);
-                double mlx = imx[0], mly = imy[0], mrx = imx[1], mry = imy[1];
-#pragma unroll
-                for (int q = 1; q < N1; ++q)
-                    if (q == pp) { mlx = imx[q]; mly = imy[q]; mrx = imx[q + 1]; mry = imy[q + 1]; }
+                double mlx, mly, mrx, mry;
+                interface_pair<N>(pp, lb, ls, smx, smy, affine, mlx, mly, mrx, mry);
                 const Side L = rl[dir][t], R = rr[dir][t];
                 double bedb, metb;
                 if (affine) { bedb = Wp.hu * smx[lb] + Wp.hv * smy[lb]; metb = 0.0; }
@@ -435,12 +431,8 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
             for (int dir = 0; dir < 2; ++dir) {
                 const int lb = dir == 0 ? rlb0 : slb0, ls = dir == 0 ? 1 : N1, pp = dir == 0 ? a : j;
                 if (pp < N1 - 1) {
-                    double imx[N1 + 1], imy[N1 + 1];
-                    interfaces<N>(lb, ls, dir == 0 ? srx : ssx, dir == 0 ? sry : ssy, affine, imx, imy);
-                    double mx = imx[1], my = imy[1];
-#pragma unroll
-                    for (int q = 1; q < N1; ++q)
-                        if (q == pp + 1) { mx = imx[q]; my = imy[q]; }
+                    double mlx, mly, mx, my;  // im[pp + 1]
+                    interface_pair<N>(pp, lb, ls, dir == 0 ? srx : ssx, dir == 0 ? sry : ssy, affine, mlx, mly, mx, my);
                     SideBar sab = sbar_zero(), sbb = sbar_zero();
                     hydro_hll_adj(rr[dir][t], rl[dir][t + ls], mx, my, p.g, p.h_min, ifab[dir][t], ifbb[dir][t], sab, sbb);
                     SideBar& own = rrb[dir][t];
@@ -465,9 +457,16 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
                 const int fl = dir == 0 ? 3 : 0, fil = dir == 0 ? N1 - 1 - j : a;
                 const int fr = dir == 0 ? 1 : 2, fir = dir == 0 ? j : N1 - 1 - a;
                 if (pp > 0) { int q = t - ls; lft = {sh[q], su[q], sv[q], sb[q]}; xlf = T.x1[pp - 1]; }
-                else { double d; has_l = outer_node<N>(p, k, c, affine, fl, fil, lft, d); xlf = -1.0 - d; }
                 if (pp < N1 - 1) { int q = t + ls; rgt = {sh[q], su[q], sv[q], sb[q]}; xrt = T.x1[pp + 1]; }
-                else { double d; has_r = outer_node<N>(p, k, c, affine, fr, fir, rgt, d); xrt = 1.0 + d; }
+                // one outer node per direction at most (as the forward)
+                const bool at_l = pp == 0;
+                if (at_l || pp == N1 - 1) {
+                    Side o = me;
+                    double d = 0.0;
+                    const bool has = outer_node<N>(p, k, at_l ? fl : fr, at_l ? fil : fir, o, d);
+                    if (at_l) { lft = o; xlf = -1.0 - d; has_l = has; }
+                    else { rgt = o; xrt = 1.0 + d; has_r = has; }
+                }
                 SideBar lftb = sbar_zero(), rgtb = sbar_zero(), meb = sbar_zero();
                 if (has_l && has_r)
                     reconstruct_adj(lft, me, rgt, xlf, T.x1[pp], xrt, T.xl[pp], T.xr[pp], p.h_dry, Lb, Rb, lftb, meb, rgtb);

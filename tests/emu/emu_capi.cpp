@@ -17,13 +17,13 @@ void emu_init(void) {
 static void run_geometry(int order, int64_t K, StageParams& p, std::vector<double>& geo, std::vector<double>& fgeo) {
     size_t nn = (size_t)(order + 1) * (order + 1);
     geo.assign((size_t)GEO_COUNT * K * nn, 0.0);
-    fgeo.assign((size_t)3 * 4 * K, 0.0);
+    fgeo.assign((size_t)(3 * 4 + 4 * (order + 1)) * K, 0.0);
     int epb = 64 / (int)nn;
     unsigned grid = (unsigned)((K + epb - 1) / epb);
     switch (order) {
-        case 1: emu_launch(grid, 64, [&] { geometry_kernel<1>(p.corners, p.bed, p.affine, K, geo.data(), fgeo.data()); }); break;
-        case 2: emu_launch(grid, 64, [&] { geometry_kernel<2>(p.corners, p.bed, p.affine, K, geo.data(), fgeo.data()); }); break;
-        default: emu_launch(grid, 64, [&] { geometry_kernel<3>(p.corners, p.bed, p.affine, K, geo.data(), fgeo.data()); }); break;
+        case 1: emu_launch(grid, 64, [&] { geometry_kernel<1>(p.corners, p.bed, p.affine, p.links, K, geo.data(), fgeo.data()); }); break;
+        case 2: emu_launch(grid, 64, [&] { geometry_kernel<2>(p.corners, p.bed, p.affine, p.links, K, geo.data(), fgeo.data()); }); break;
+        default: emu_launch(grid, 64, [&] { geometry_kernel<3>(p.corners, p.bed, p.affine, p.links, K, geo.data(), fgeo.data()); }); break;
     }
     p.geo = geo.data();
     p.fgeo = fgeo.data();
@@ -164,7 +164,7 @@ double emu_dt(int order, int64_t K, const double* state, const double* corners, 
     StageParams sp;
     std::memset(&sp, 0, sizeof(sp));
     std::vector<double> bed(nt, 0.0), geo, fgeo;
-    sp.corners = corners; sp.bed = bed.data(); sp.affine = affine;
+    sp.corners = corners; sp.bed = bed.data(); sp.affine = affine; sp.links = links;
     run_geometry(order, K, sp, geo, fgeo);
     std::vector<unsigned long long> slots(DT_SLOTS, DT_SLOT_EMPTY);
     DtParams p;
