@@ -352,6 +352,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
     double g_rx = 0.0, g_ry = 0.0, g_sx = 0.0, g_sy = 0.0, det = 1.0, det_inv = 1.0;
     double bxr = 0.0, byr = 0.0, bxs = 0.0, bys = 0.0;
     double bh0 = 0.0, bhu0 = 0.0, bhv0 = 0.0, mann = 0.0;
+    VjpPre vpre = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     bool affine = false;
     if (active) {
         affine = p.affine[k] != 0;
@@ -384,7 +385,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
         g_ry = p.geo[GEO_RY * nt + n];
         g_sx = p.geo[GEO_SX * nt + n];
         g_sy = p.geo[GEO_SY * nt + n];
-        det = p.geo[GEO_DET * nt + n];
+        if (VJP) det = p.geo[GEO_DET * nt + n];  // the forward reads J only where the wet/dry correction needs it
         det_inv = p.geo[GEO_DINV * nt + n];
         bxr = p.geo[GEO_BXR * nt + n];
         byr = p.geo[GEO_BYR * nt + n];
@@ -405,6 +406,13 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             }
             if (NT == 1 && !p.rhs_only && p.a != 0.0) { bh0 = p.bh[n]; bhu0 = p.bhu[n]; bhv0 = p.bhv[n]; }
             if (NT == 1 && (p.flags & DG_FRICTION)) mann = p.manning[k];
+        } else {
+            // the reverse's own reads: the saved RHS, the base state, Manning n
+            // and the output cotangent
+            if (p.rh) { vpre.rh = p.rh[n]; vpre.rhu = p.rhu[n]; vpre.rhv = p.rhv[n]; }
+            if (!p.rhs_only && p.a != 0.0) { vpre.bh = p.bh[n]; vpre.bhu = p.bhu[n]; vpre.bhv = p.bhv[n]; }
+            if (p.flags & DG_FRICTION) vpre.mn = p.manning[k];
+            vpre.wh = vp->wh[n]; vpre.whu = vp->whu[n]; vpre.whv = vp->whv[n];
         }
     }
     if (t < EPB) { sflag[t] = 0; wflag[t] = 0; }
@@ -582,10 +590,10 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
         }
         if (p.rh) { p.rh[n] = rhs.h; p.rhu[n] = rhs.hu; p.rhv[n] = rhs.hv; }
     }
-    if (VJP && active && p.rh) rhs = {p.rh[n], p.rhu[n], p.rhv[n]};
+    if (VJP && active && p.rh) rhs = {vpre.rh, vpre.rhu, vpre.rhv};
     if constexpr (VJP) {
-        vjp_tail<N, SUB>(*vp, k, active, c, affine, det, det_inv, hin, huin, hvin, bin, rhs, sh, shu, shv, su, sv, sb,
-                         srx, sry, ssx, ssy, rl, rr);
+        vjp_tail<N, SUB>(*vp, k, active, c, affine, det, det_inv, hin, huin, hvin, bin, rhs, vpre, sh, shu, shv, su, sv,
+                         sb, srx, sry, ssx, ssy, rl, rr);
         return;
     }
     if (p.rhs_only) {
@@ -633,12 +641,16 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
     if (wet_dry) {
         lds_sync();  // the node states in LDS are reused here
         if (active) {
-            pw[t] = T.w2[i] * det;
             sh[t] = vh; shu[t] = vhu; shv[t] = vhv;
             if (vh < p.h_dry) wflag[e] = 1;
         }
         lds_sync();
         const bool needs = active && wflag[e] != 0;
+        if (needs) {
+            // the node masses of an element that is corrected (J read here only)
+            pw[t] = T.w2[i] * p.geo[GEO_DET * (size_t)p.n_elements * NN + n];
+        }
+        lds_sync();
         if (needs && i == 0) {
             double area = 0.0;
             for (int q = 0; q < NN; ++q) area += pw[eb + q];

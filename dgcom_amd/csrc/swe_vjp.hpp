@@ -28,6 +28,11 @@
 
 namespace dgswe {
 
+// The reverse stage's per-node reads, issued with the forward's loads at the
+// start of the stage kernel: the saved RHS, the base state, Manning n of the
+// element, the output cotangent
+struct VjpPre { double rh, rhu, rhv, bh, bhu, bhv, mn, wh, whu, whv; };
+
 struct VjpParams {
     StageParams s;                 // the forward stage's inputs (its outputs are not written)
     const double *wh, *whu, *whv;  // cotangent of the stage output (of the RHS when s.rhs_only)
@@ -76,7 +81,7 @@ __device__ __forceinline__ void outer_node_adj(const VjpParams& P, int64_t k, in
 template <int N, bool SUB>
 __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool active, const Quad& c, bool affine,
                                          double det, double det_inv, double hin, double huin, double hvin, double bin,
-                                         const Cons& rhs, const double* sh,
+                                         const Cons& rhs, const VjpPre& pre, const double* sh,
                                          const double* shu, const double* shv, const double* su, const double* sv,
                                          const double* sb, const double* srx, const double* sry, const double* ssx,
                                          const double* ssy, const Side (*rl)[SUB ? 64 : 1],
@@ -100,8 +105,15 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
     __shared__ int wfl[EPB];
     __shared__ double swh[64], swhu[64], swhv[64];
     __shared__ SideBar rlb[2][SUB ? 64 : 1], rrb[2][SUB ? 64 : 1], rlx[2][SUB ? 64 : 1];
-    __shared__ SideBar tol[2][SUB ? 64 : 1], tor[2][SUB ? 64 : 1];
-    __shared__ Cons ifab[2][SUB ? 64 : 1], ifbb[2][SUB ? 64 : 1];
+    // the interface fluxes' cotangents (ifab, ifbb: until the interface HLLs'
+    // reverse) and the reconstruction's cotangents for the line neighbours
+    // (tol, tor: from the reconstruction's reverse on) share one array
+    constexpr int SL = SUB ? 64 : 1;
+    __shared__ double line_ovl[2 * 2 * SL * (sizeof(SideBar) / sizeof(double))];
+    SideBar(*tol)[SL] = reinterpret_cast<SideBar(*)[SL]>(line_ovl);
+    SideBar(*tor)[SL] = reinterpret_cast<SideBar(*)[SL]>(line_ovl + 2 * SL * (sizeof(SideBar) / sizeof(double)));
+    Cons(*ifab)[SL] = reinterpret_cast<Cons(*)[SL]>(line_ovl);
+    Cons(*ifbb)[SL] = reinterpret_cast<Cons(*)[SL]>(line_ovl + 2 * SL * (sizeof(Cons) / sizeof(double)));
 
     const int t = threadIdx.x;
     const int e = t / NN;
@@ -118,7 +130,7 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
     bool zeroed = false, damped = false, fric = false, thin = false;
     double uf = 0.0, vf = 0.0, speed = 0.0, mn = 0.0, hh = 0.0, cbr = 0.0;
     if (active && post) {
-        if (p.a != 0.0) { vh = p.a * p.bh[n]; vhu = p.a * p.bhu[n]; vhv = p.a * p.bhv[n]; }
+        if (p.a != 0.0) { vh = p.a * pre.bh; vhu = p.a * pre.bhu; vhv = p.a * pre.bhv; }
         if (p.b != 0.0) { vh += p.b * hin; vhu += p.b * huin; vhv += p.b * hvin; }
         if (p.c != 0.0) { vh += p.c * rhs.h; vhu += p.c * rhs.hu; vhv += p.c * rhs.hv; }
         if (friction || wet_dry) {
@@ -130,7 +142,7 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
                     speed = sqrt(uf * uf + vf * vf);
                     if (speed > 0.0) {
                         fric = true;
-                        mn = p.manning[k];
+                        mn = pre.mn;
                         double gn2 = p.g * mn * mn;
                         hh = fmax_(vh, p.h_min);
                         cbr = manning_cbrt(hh);
@@ -208,7 +220,7 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
 
     // ---- reverse ----
     double gh = 0.0, ghu = 0.0, ghv = 0.0;
-    if (active) { gh = P.wh[n]; ghu = P.whu[n]; ghv = P.whv[n]; }
+    if (active) { gh = pre.wh; ghu = pre.whu; ghv = pre.whv; }
     double gi_h = 0.0, gi_hu = 0.0, gi_hv = 0.0;  // input cotangent, this node
     double gman = 0.0;
     Cons rb = {0.0, 0.0, 0.0};                     // cotangent of the RHS
