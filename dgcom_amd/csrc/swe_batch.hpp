@@ -84,24 +84,13 @@ __global__ __launch_bounds__(256) void batch_compact_kernel(const uint8_t* count
                                                             const uint32_t* block_member, unsigned int n_blocks,
                                                             const unsigned int* chunk_total, int epb, uint32_t* list,
                                                             uint32_t* list_member, unsigned int* list_count) {
-    __shared__ unsigned int scan[257];
-    __shared__ unsigned int base;
-    if (threadIdx.x == 0) {
-        unsigned int b = 0;
-        for (unsigned int c = 0; c < blockIdx.x; ++c) b += chunk_total[c];
-        base = b;
-    }
     const unsigned int b0 = blockIdx.x * 1024u + threadIdx.x * 4u;
     unsigned int cnt[4], s = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) { cnt[q] = b0 + q < n_blocks ? counts[b0 + q] : 0u; s += cnt[q]; }
-    scan[threadIdx.x + 1] = s;
-    if (threadIdx.x == 0) scan[0] = 0;
-    __syncthreads();
-    if (threadIdx.x == 0)
-        for (int q = 0; q < 256; ++q) scan[q + 1] += scan[q];
-    __syncthreads();
-    unsigned int off = base + scan[threadIdx.x];
+    unsigned int base, excl, total;
+    chunk_offsets(chunk_total, s, base, excl, total);
+    unsigned int off = base + excl;
 #pragma unroll
     for (int q = 0; q < 4; ++q)
         for (unsigned int r = 0; r < cnt[q]; ++r) {
@@ -109,7 +98,7 @@ __global__ __launch_bounds__(256) void batch_compact_kernel(const uint8_t* count
             list_member[off] = block_member[b0 + q];
             ++off;
         }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *list_count = base + scan[256];
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *list_count = base + total;
 }
 
 // The listed shallow elements of one order, EPB per workgroup pass: a pass takes

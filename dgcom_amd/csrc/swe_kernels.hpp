@@ -794,31 +794,44 @@ __global__ __launch_bounds__(256) void chunk_count_kernel(const uint8_t* counts,
     if (threadIdx.x == 0) chunk_total[blockIdx.x] = red[0];
 }
 
+// Exclusive prefix sum over the 256 threads of a workgroup (Hillis-Steele in
+// LDS) and the sum of chunk_total[0..blockIdx.x), each thread a share: the
+// list offsets of a compaction chunk (integer sums: any order is exact)
+__device__ __forceinline__ void chunk_offsets(const unsigned int* chunk_total, unsigned int s, unsigned int& base,
+                                              unsigned int& excl, unsigned int& chunk_sum) {
+    __shared__ unsigned int scan[256], part[256];
+    unsigned int b = 0;
+    for (unsigned int c = threadIdx.x; c < blockIdx.x; c += 256) b += chunk_total[c];
+    scan[threadIdx.x] = s;
+    part[threadIdx.x] = b;
+    __syncthreads();
+    for (unsigned int w = 1; w < 256; w <<= 1) {
+        const unsigned int v = threadIdx.x >= w ? scan[threadIdx.x - w] : 0u;
+        const unsigned int q = threadIdx.x + w < 256 ? part[threadIdx.x + w] : 0u;
+        __syncthreads();
+        scan[threadIdx.x] += v;
+        if ((threadIdx.x & (2 * w - 1)) == 0) part[threadIdx.x] += q;
+        __syncthreads();
+    }
+    base = part[0];
+    excl = scan[threadIdx.x] - s;
+    chunk_sum = scan[255];
+}
+
 __global__ __launch_bounds__(256) void compact_kernel(const uint8_t* counts, const uint32_t* queue, unsigned int n_blocks,
                                                       const unsigned int* chunk_total, int epb, uint32_t* list,
                                                       unsigned int* list_count) {
-    __shared__ unsigned int scan[257];
-    __shared__ unsigned int base;
-    if (threadIdx.x == 0) {
-        unsigned int b = 0;
-        for (unsigned int c = 0; c < blockIdx.x; ++c) b += chunk_total[c];
-        base = b;
-    }
     const unsigned int b0 = blockIdx.x * 1024u + threadIdx.x * 4u;
     unsigned int cnt[4], s = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) { cnt[q] = b0 + q < n_blocks ? counts[b0 + q] : 0u; s += cnt[q]; }
-    scan[threadIdx.x + 1] = s;
-    if (threadIdx.x == 0) scan[0] = 0;
-    __syncthreads();
-    if (threadIdx.x == 0)
-        for (int q = 0; q < 256; ++q) scan[q + 1] += scan[q];
-    __syncthreads();
-    unsigned int off = base + scan[threadIdx.x];
+    unsigned int base, excl, total;
+    chunk_offsets(chunk_total, s, base, excl, total);
+    unsigned int off = base + excl;
 #pragma unroll
     for (int q = 0; q < 4; ++q)
         for (unsigned int r = 0; r < cnt[q]; ++r) list[off++] = queue[(size_t)(b0 + q) * epb + r];
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *list_count = base + scan[256];
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *list_count = base + total;
 }
 
 // The listed elements with a shallow node (subcell volume term), EPB per
@@ -964,12 +977,14 @@ __device__ __forceinline__ double dt_block(const DtParams& p, int64_t blk) {
     lds_sync();
     if (active && i == 0) {
         double rmax = 0.0;
+#pragma unroll 4
         for (int q = 0; q < NN; ++q) rmax = fmax_(rmax, sr[eb + q]);
         double cfl = p.cfl;
         if (p.positivity) {
             double water = 0.0, water_b = 0.0, weight = 0.0, weight_b = 0.0;
             bool shallow = false;
             const double shallow_depth = 100.0 * p.h_dry;
+#pragma unroll 4
             for (int q = 0; q < NN; ++q) {
                 int qa = q % N1, qj = q / N1;
                 double mass = smass[eb + q], wi = swater[eb + q];
