@@ -858,7 +858,7 @@ __global__ __launch_bounds__(64) DG_SUB_OCCUPANCY void subcell_stage_kernel(Stag
 // The VJP of a stage: wet elements first (the others are listed as in the
 // forward), then the listed elements, grid-stride.
 template <int N>
-__global__ __launch_bounds__(64) void wet_vjp_kernel(VjpParams P) {
+__global__ __launch_bounds__(64) DG_WAVES_PER_EU(N == 3 ? 3 : 4) void wet_vjp_kernel(VjpParams P) {
     constexpr int NN = (N + 1) * (N + 1), EPB = 64 / NN;
     const int64_t blk = xcd_block(blockIdx.x, P.s.xcd_stride);
     if (blk * EPB >= P.s.n_elements) return;

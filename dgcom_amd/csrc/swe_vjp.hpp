@@ -344,24 +344,27 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
                 const double mpx = smx[t], mpy = smy[t];
                 const Cons Wp = {swh[t], swhu[t], swhv[t]};
                 double db = 0.0, dbx = 0.0, dby = 0.0;
-#pragma unroll
+#pragma unroll 1
                 for (int q = 0; q < N1; ++q) {
                     const int iq = lb + q * ls;
                     const Node o = {sh[iq], shu[iq], shv[iq], su[iq], sv[iq], sb[iq]};
                     const double dpq = T.d1[pp * N1 + q], dqp = T.d1[q * N1 + pp];
                     const double mqx = smx[iq], mqy = smy[iq];
-                    const bool own = q == pp;
-                    const double mx = (own || affine) ? mpx : 0.5 * (mqx + mpx);
-                    const double my = (own || affine) ? mpy : 0.5 * (mqy + mpy);
-                    if (own) {
-                        const Cons w = cscale(-2.0 * dpq, Wp);
-                        ec_flux_adj(me, me, mx, my, p.g, w, nb, nb);
-                    } else {
-                        const Cons Wq = {swh[iq], swhu[iq], swhv[iq]};
-                        const Cons w = cadd(cscale(-2.0 * dpq, Wp), cscale(-2.0 * dqp, Wq));
-                        NodeBar other = nodebar_zero();
-                        ec_flux_adj(o, me, mx, my, p.g, w, other, nb);
-                    }
+                    // The pair's flux is F#(q_q, q_p): its cotangent w collects both
+                    // ends' terms, and this node takes the right argument's part.
+                    // The diagonal pair (q = p) enters both arguments with the same
+                    // state; its two equal parts are one part of twice its w, which
+                    // is what the off-diagonal formula gives there (D_pp twice, W_p
+                    // twice). The metric average at q = p and on an affine element
+                    // (one metric at every node) is the node's metric exactly. So
+                    // every pair takes one code path: a branch on q = p would
+                    // diverge within the wavefront and run both.
+                    const double mx = 0.5 * (mqx + mpx);
+                    const double my = 0.5 * (mqy + mpy);
+                    const Cons Wq = {swh[iq], swhu[iq], swhv[iq]};
+                    const Cons w = cadd(cscale(-2.0 * dpq, Wp), cscale(-2.0 * dqp, Wq));
+                    NodeBar other = nodebar_zero();
+                    ec_flux_adj(o, me, mx, my, p.g, w, other, nb);
                     const double d_b = dpq * o.b;
                     db += d_b;
                     dbx += d_b * mqx;
@@ -613,9 +616,15 @@ __global__ __launch_bounds__(64) DG_WAVES_PER_EU(3) void face_vjp_kernel(VjpPara
                 const size_t m = (size_t)(l >> 4) * NN + face_node<N>((int)(l & 3u), N1 - 1 - fi);
                 nhr = p.ih[m]; nhur = p.ihu[m]; nhvr = p.ihv[m]; nbdr = p.bed[m];
             }
-            const Cons gs = wall_ghost(q.h, q.hu, q.hv, nx, ny, p.h_min);
-            const Node qo = wall ? make_node(gs.h, gs.hu, gs.hv, q.b, p.h_min)
-                                 : make_node(nhr, nhur, nhvr, nbdr, p.h_min);
+            // the outside state: the neighbour's face node, or a wall's reflected
+            // ghost (formed only where a lane has a wall face); one make_node
+            Cons gs = {0.0, 0.0, 0.0};
+            double xh = nhr, xhu = nhur, xhv = nhvr, xb = nbdr;
+            if (wall) {
+                gs = wall_ghost(q.h, q.hu, q.hv, nx, ny, p.h_min);
+                xh = gs.h; xhu = gs.hu; xhv = gs.hv; xb = q.b;
+            }
+            const Node qo = make_node(xh, xhu, xhv, xb, p.h_min);
             double lx = nx, ly = ny;
             if (!left) {
                 lx = -nx; ly = -ny;
@@ -638,12 +647,11 @@ __global__ __launch_bounds__(64) DG_WAVES_PER_EU(3) void face_vjp_kernel(VjpPara
             nbar_add_side(qb, mine);
             const NodeBar ob = {os.h, 0.0, 0.0, os.u, os.v, os.b};
             double oh = 0.0, ohu = 0.0, ohv = 0.0, obb = 0.0;
+            make_node_adj(xh, xhu, xhv, p.h_min, ob, oh, ohu, ohv, obb);
             if (wall) {
-                make_node_adj(gs.h, gs.hu, gs.hv, p.h_min, ob, oh, ohu, ohv, obb);
                 qb.b += obb;
                 wall_ghost_adj(q.h, q.hu, q.hv, nx, ny, p.h_min, Cons{oh, ohu, ohv}, qb.h, qb.hu, qb.hv);
             } else {
-                make_node_adj(nhr, nhur, nhvr, p.h_min, ob, oh, ohu, ohv, obb);
                 double* o = P.facebuf + gid * 4;
                 o[0] = oh; o[1] = ohu; o[2] = ohv; o[3] = obb;
             }
