@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void batch_compact_kernel(const uint8_t* count
 // the elements of one member at a time (the first unprocessed entry's), so that
 // the member's parameters stay uniform over the wavefront.
 template <int N>
-__global__ __launch_bounds__(64) void batch_subcell_kernel(BatchArgs A, const uint32_t* list,
+__global__ __launch_bounds__(64) DG_SUB_OCCUPANCY void batch_subcell_kernel(BatchArgs A, const uint32_t* list,
                                                            const uint32_t* list_member, const unsigned int* count_ptr) {
     constexpr int NN = (N + 1) * (N + 1), EPB = 64 / NN;
     const unsigned int count = *count_ptr;
