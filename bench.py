@@ -138,6 +138,50 @@ def adjoint_bench(n, n_steps, snapshots, every, n_stations):
             "per_step": per_step, "setup_s": setup}
 
 
+def adjoint_ensemble_bench(n_members, n_steps, snapshots):
+    """The inversion ensemble (inversion.ensemble_workload: uneven meshes, N in
+    {2, 3}, each member with its own truth/prior beds and 12 stations): every
+    member's misfit gradient from its own tape of `snapshots` states, the members'
+    sweeps in one sequence of launches, against the members' plain forward runs of
+    the same steps one after another."""
+    from dgcom_amd import inversion
+
+    t0 = time.time()
+    members = inversion.ensemble_workload(n_members=n_members, n_steps=n_steps)
+    physics = []
+    for inv in members:
+        phys = swe2d.SWEPhysics2D.from_case(inv.case)
+        inversion.observe(inv, phys)
+        physics.append(phys)
+    setup = time.time() - t0
+    fwd = 0.0
+    for inv, phys in zip(members, physics):
+        c = inv.case
+        x0 = phys.state(c.h, c.hu, c.hv)
+        plain = phys.tape(1)
+        phys.sync()
+        w0 = time.perf_counter()
+        plain.forward(x0, 0.0, inv.dts, inv.every, inv.obs)
+        phys.sync()
+        fwd += time.perf_counter() - w0
+        plain.close()
+        x0.free()
+    physics[0].sync()
+    w0 = time.perf_counter()
+    grads = inversion.ensemble_gradients(members, physics, snapshots)
+    for phys in physics:
+        phys.sync()
+    grad = time.perf_counter() - w0
+    orders = [inv.case.order for inv in members]
+    return {"workload": f"inversion ensemble: {n_members} members, K log-uniform 20000-250000 jittered quads, N in "
+                        f"{{2,3}}, own GRF truth/prior beds (land in each), 12 stations every 10 steps, {n_steps} "
+                        f"SSP-RK3 steps, {snapshots} checkpoints per member",
+            "members": n_members, "elements_total": int(sum(inv.case.n_elements for inv in members)),
+            "members_per_order": {str(o): orders.count(o) for o in (2, 3)},
+            "forward_runs_s": fwd, "gradients_s": grad, "ratio": grad / fwd,
+            "misfits": [float(J) for J, _, _ in grads], "setup_s": setup}
+
+
 def ensemble_members(n_members, seed, k_lo, k_hi, n_steps):
     """Seeded members: K log-uniform in [k_lo, k_hi] jittered quads of 100 m, N uniform
     in {1, 2, 3}, M2 tide amplitude uniform in 0.2-1.5 m and phase in 0-2pi, GRF bed."""
@@ -213,6 +257,9 @@ def main():
     ap.add_argument("--ensemble-members", type=int, default=64)
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--adjoint-ensemble-members", type=int, default=0,
+                    help="members of the inversion ensemble's gradient timing (0: not run)")
+    ap.add_argument("--adjoint-ensemble-steps", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the time step runs on a single device here "
@@ -293,6 +340,8 @@ def main():
         out["adjoint"] = adjoint_bench(args.adjoint_n, args.adjoint_steps, args.adjoint_snapshots, 10, 20)
     if not args.no_ensemble:
         out["ensemble"] = ensemble_bench(args.ensemble_members, args.ensemble_steps, 20_000, 400_000)
+    if args.adjoint_ensemble_members > 0:
+        out["adjoint_ensemble"] = adjoint_ensemble_bench(args.adjoint_ensemble_members, args.adjoint_ensemble_steps, 20)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
     print(json.dumps(out), flush=True)
