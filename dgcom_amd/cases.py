@@ -268,3 +268,67 @@ def drying_beach(nx=24, ny=6, order=2, slope=0.01, length=2000.0, width=500.0, e
     tide = Tide(amplitude=(eta_amp,), omega=(2 * math.pi / 600.0,), phase=(0.0,), ramp=60.0)
     return Case("drying_beach", order, mesh, bed, np.full(mesh.n_elements, 0.025), 1.0e-4, tide, h, hu, hv,
                 cfl=0.5, scheme=0, n_steps=n_steps)
+
+
+# ---------------------------------------------------------------------------
+# Thacker's planar surface in a paraboloid (tests/wet_dry_2d_test.rs:217-310)
+# ---------------------------------------------------------------------------
+THACKER_A, THACKER_H0, THACKER_ETA, THACKER_L = 1.0, 0.1, 0.5, 4.0   # SWASHES parameters
+
+
+def thacker_omega(g=9.81):
+    return math.sqrt(2.0 * g * THACKER_H0) / THACKER_A
+
+
+def thacker_bed(x, y):
+    r2 = (x - 0.5 * THACKER_L) ** 2 + (y - 0.5 * THACKER_L) ** 2
+    return -THACKER_H0 * (1.0 - r2 / (THACKER_A * THACKER_A))
+
+
+def thacker_exact(x, y, t, g=9.81):
+    """The exact (h, hu, hv) at points (x, y) and time t: a planar surface whose
+    wet disc slides round the bowl with period 2 pi / omega."""
+    w = thacker_omega(g)
+    xc, yc = x - 0.5 * THACKER_L, y - 0.5 * THACKER_L
+    surface = THACKER_ETA * THACKER_H0 / (THACKER_A * THACKER_A) * (
+        2.0 * xc * math.cos(w * t) + 2.0 * yc * math.sin(w * t) - THACKER_ETA)
+    h = surface - thacker_bed(x, y)
+    wet = h > 0.0
+    u, v = -THACKER_ETA * w * math.sin(w * t), THACKER_ETA * w * math.cos(w * t)
+    h = np.where(wet, h, 0.0)
+    return h, np.where(wet, h * u, 0.0), np.where(wet, h * v, 0.0)
+
+
+def thacker(n=40, order=2, h_dry=1e-5) -> Case:
+    """Thacker's case on an n x n mesh of [0, 4] m^2 (walls, no friction, no
+    rotation), started from the exact state at t = 0, SSP-RK3 at CFL 1 for one
+    period; dry threshold 1e-5 m (1e-4 of the 0.1 m bowl depth, as the
+    reference's test takes it)."""
+    mesh = structured_mesh(0.0, THACKER_L, 0.0, THACKER_L, n, n)
+    x, y = node_coordinates(mesh, order)
+    bed = thacker_bed(x, y).ravel()
+    h, hu, hv = thacker_exact(x.ravel(), y.ravel(), 0.0)
+    return Case("thacker", order, mesh, bed, np.zeros(mesh.n_elements), 0.0, None, h, hu, hv, cfl=1.0, scheme=0,
+                n_steps=0, h_dry=h_dry)
+
+
+def node_masses(case) -> np.ndarray:
+    """GLL quadrature weight times J at every node (K*NN,): the integral of a
+    nodal field f is sum(node_masses * f)."""
+    from numpy.polynomial import legendre as _leg
+    n = case.order
+    x1 = gll_nodes(n)
+    pn = _leg.legval(x1, [0] * n + [1])
+    w1 = 2.0 / (n * (n + 1) * pn * pn)
+    w2 = np.outer(w1, w1).ravel()   # node i = j (N+1) + a: w_a w_j
+    c = case.mesh.corners
+    out = np.empty((case.n_elements, (n + 1) ** 2))
+    for j in range(n + 1):
+        for a in range(n + 1):
+            r, s = x1[a], x1[j]
+            xr = 0.25 * ((1 - s) * (c[:, 1, 0] - c[:, 0, 0]) + (1 + s) * (c[:, 2, 0] - c[:, 3, 0]))
+            yr = 0.25 * ((1 - s) * (c[:, 1, 1] - c[:, 0, 1]) + (1 + s) * (c[:, 2, 1] - c[:, 3, 1]))
+            xs = 0.25 * ((1 - r) * (c[:, 3, 0] - c[:, 0, 0]) + (1 + r) * (c[:, 2, 0] - c[:, 1, 0]))
+            ys = 0.25 * ((1 - r) * (c[:, 3, 1] - c[:, 0, 1]) + (1 + r) * (c[:, 2, 1] - c[:, 1, 1]))
+            out[:, j * (n + 1) + a] = w2[j * (n + 1) + a] * (xr * ys - xs * yr)
+    return out.ravel()
