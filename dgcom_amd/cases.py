@@ -166,18 +166,25 @@ def gaussian_random_field(x, y, extent, sd, correlation, seed, spacing=None):
 
 @dataclasses.dataclass
 class Tide:
-    """Uniform harmonic tide `eta(t) = mean + R(t) sum A cos(w t + phase)`, with the
-    smooth ramp `R = 3 tau^2 - 2 tau^3`, `tau = t / ramp` (`harmonic_tide.rs:24-32,124-136`)."""
+    """Uniform harmonic tide `eta(t) = mean + R(t) sum f A cos(w t + phase + (V0 + u)
+    pi/180)`, with the smooth ramp `R = 3 tau^2 - 2 tau^3`, `tau = t / ramp`
+    (`harmonic_tide.rs:24-32,124-136`). `nodal`: per constituent (f0, f_rate, V0,
+    u0, u_rate) of its nodal drift (degrees, seconds; `tides.ModelClock.nodal_drift`),
+    empty for none (f = 1, V0 = u = 0)."""
     amplitude: tuple = ()
     omega: tuple = ()
     phase: tuple = ()
     mean: float = 0.0
     ramp: float = 3600.0
+    nodal: tuple = ()
 
     def elevation(self, t: float) -> float:
         tide = 0.0
-        for a, w, p in zip(self.amplitude, self.omega, self.phase):
-            tide += 1.0 * a * math.cos(w * t + p + 0.0)
+        for i, (a, w, p) in enumerate(zip(self.amplitude, self.omega, self.phase)):
+            f0, f_rate, v0, u0, u_rate = self.nodal[i] if self.nodal else (1.0, 0.0, 0.0, 0.0, 0.0)
+            f = f0 + f_rate * t
+            u = u0 + u_rate * t
+            tide += f * a * math.cos(w * t + p + (v0 + u) * (math.pi / 180.0))
         rf = 1.0
         if self.ramp > 0.0:
             tau = min(max(t / self.ramp, 0.0), 1.0)
@@ -332,3 +339,59 @@ def node_masses(case) -> np.ndarray:
             ys = 0.25 * ((1 - r) * (c[:, 3, 1] - c[:, 0, 1]) + (1 + r) * (c[:, 2, 1] - c[:, 1, 1]))
             out[:, j * (n + 1) + a] = w2[j * (n + 1) + a] * (xr * ys - xs * yr)
     return out.ravel()
+
+
+# ---------------------------------------------------------------------------
+# The reference's example case cases/tidal_channel/case.toml
+# ---------------------------------------------------------------------------
+EARTH_ROTATION_RATE = 7.292_115e-5   # source/swe_2d/coriolis.rs:25
+
+
+def coriolis_parameter(latitude_deg):
+    """f = 2 Omega sin(latitude) (coriolis.rs:29-31)."""
+    return 2.0 * EARTH_ROTATION_RATE * math.sin(math.radians(latitude_deg))
+
+
+def ramp_depth(x, y, from_m, to_m, depth_m):
+    """Depth along a ramp, constant across it (case/schema.rs:656-663)."""
+    x0, y0 = from_m
+    dx, dy = to_m[0] - x0, to_m[1] - y0
+    s = np.clip(((x - x0) * dx + (y - y0) * dy) / (dx * dx + dy * dy), 0.0, 1.0)
+    return depth_m[0] + s * (depth_m[1] - depth_m[0])
+
+
+def harmonic_tide(constituents, ramp_hours, start=None, mean=0.0):
+    """The case runner's tide (case/run.rs:686-704): each (name, amplitude, Greenwich
+    lag G in degrees) as A cos(w t - G) with w = 2 pi / the constituent's period
+    from its Doodson speed, ramped over `ramp_hours`, with the nodal corrections of
+    a run starting at `start` = (year, month, day[, hour, minute]) UTC if given."""
+    from . import tides
+    amp, omega, phase, nodal = [], [], [], []
+    clock = tides.ModelClock.at_datetime(*start) if start is not None else None
+    for name, a, g in constituents:
+        amp.append(a)
+        omega.append(2.0 * math.pi / tides.constituent_period(name))
+        phase.append(-math.radians(g))
+        if clock is not None:
+            d = clock.nodal_drift(name)
+            nodal.append((d.at_epoch.f, d.f_rate, d.at_epoch.v0_deg, d.at_epoch.u_deg, d.u_rate_deg))
+    return Tide(amplitude=tuple(amp), omega=tuple(omega), phase=tuple(phase), mean=mean, ramp=3600.0 * ramp_hours,
+                nodal=tuple(nodal))
+
+
+def tidal_channel(order=2, nx=40, ny=4) -> Case:
+    """cases/tidal_channel/case.toml: a channel 40 km x 4 km open to the sea at its
+    west end, its bed rising from 30 m below mean sea level at the mouth to 2 m
+    above at the head (tidal flats); M2 1.0 m / G 0 and S2 0.35 m / G 30 deg with
+    a 2 h ramp and the nodal corrections of a run from 2025-06-15 00:00 UTC through
+    the characteristic boundary at rest; Coriolis at 63.5 N, Manning 0.025, wet/dry
+    with the defaults; still water at mean sea level; SSP-RK(4,3) at its default
+    CFL, 0.9 of the linear limit (1.359 at N = 2)."""
+    mesh = structured_mesh(0.0, 40_000.0, 0.0, 4_000.0, nx, ny, sides=(WALL, WALL, WALL, TIDE))
+    x, y = node_coordinates(mesh, order)
+    bed = -ramp_depth(x, y, (0.0, 2000.0), (40_000.0, 2000.0), (30.0, -2.0)).ravel()
+    h, hu, hv = still_water(bed)
+    tide = harmonic_tide((("M2", 1.0, 0.0), ("S2", 0.35, 30.0)), 2.0, start=(2025, 6, 15))
+    lin43 = (2.2, 1.51, 1.15, 0.93)   # linear CFL limits of SSP-RK(4,3) (swe_2d.rs:1263-1302)
+    return Case("tidal_channel", order, mesh, bed, np.full(mesh.n_elements, 0.025), coriolis_parameter(63.5), tide,
+                h, hu, hv, cfl=0.9 * lin43[order - 1], scheme=1, n_steps=0)

@@ -57,6 +57,9 @@ struct dg_ctx {
     int flags = 0;
     double g = 9.81, h_min = 1e-6, h_dry = 1e-3, tau = 1.0, u_max = 20.0, f0 = 0.0;
     dg_tide tide{};
+    // nodal corrections of the tide's constituents, linear in the time since the
+    // epoch: f0, f_rate, V0 (deg), u0 (deg), u_rate (deg/s); identity by default
+    double tide_nodal[5][8] = {{1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0}, {}, {}, {}, {}};
     double *corners = nullptr, *bed = nullptr, *manning = nullptr, *element_dt = nullptr;
     // static geometry cache (geometry_kernel): geo per node; fgeo per face [3][K*4], then per face node
     // [K][4][N+1] the outer node's distance
@@ -125,9 +128,18 @@ static double host_bilinear_det(const double* c, double r, double s) {
     return x_r * y_s - x_s * y_r;
 }
 
-static double tide_elevation(const dg_tide& t, double time) {
+// HarmonicTide::elevation (harmonic_tide.rs:124-136) with NodalDrift::at
+// (astronomy.rs:252-259) and NodalCorrection::phase_offset_rad (astronomy.rs:213-215):
+// eta = mean + R(t) sum (f0 + f' t) A cos(w t + phase + (V0 + u0 + u' t) pi/180);
+// with the identity correction this is sum A cos(w t + phase) exactly
+static double tide_elevation(const dg_tide& t, const double (&nod)[5][8], double time) {
+    constexpr double DEG_TO_RAD = M_PI / 180.0;
     double tide = 0.0;
-    for (int i = 0; i < t.n; ++i) tide += 1.0 * t.amplitude[i] * std::cos(t.omega[i] * time + t.phase[i] + 0.0);
+    for (int i = 0; i < t.n; ++i) {
+        const double f = nod[0][i] + nod[1][i] * time;
+        const double u = nod[3][i] + nod[4][i] * time;
+        tide += f * t.amplitude[i] * std::cos(t.omega[i] * time + t.phase[i] + (nod[2][i] + u) * DEG_TO_RAD);
+    }
     double rf = 1.0;
     if (t.ramp > 0.0) {
         double tau = std::min(std::max(time / t.ramp, 0.0), 1.0);
@@ -204,7 +216,7 @@ static int launch_stage(dg_ctx* c, const double* base, const double* input, doub
     p.links = c->links; p.tags = c->tags; p.affine = c->affine;
     p.n_elements = c->K;
     p.a = a; p.b = b; p.c = cc;
-    p.eta_e = tide_elevation(c->tide, time);
+    p.eta_e = tide_elevation(c->tide, c->tide_nodal, time);
     p.g = c->g; p.h_min = c->h_min; p.h_dry = c->h_dry; p.u_max = c->u_max; p.tau = c->tau; p.f0 = c->f0;
     p.sqrt_g = std::sqrt(c->g);
     p.rhs_only = rhs_only;
@@ -370,7 +382,7 @@ static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, 
     p.links = c->links; p.tags = c->tags; p.affine = c->affine;
     p.n_elements = c->K;
     p.a = a; p.b = b; p.c = cc;
-    p.eta_e = tide_elevation(c->tide, time);
+    p.eta_e = tide_elevation(c->tide, c->tide_nodal, time);
     p.g = c->g; p.h_min = c->h_min; p.h_dry = c->h_dry; p.u_max = c->u_max; p.tau = c->tau; p.f0 = c->f0;
     p.sqrt_g = std::sqrt(c->g);
     p.rhs_only = rhs_only;
@@ -908,6 +920,23 @@ int dg_set_tide(dg_ctx* c, const dg_tide* t) {
     DG_REQUIRE(c && t, "dg_set_tide: null argument");
     DG_REQUIRE(t->n >= 0 && t->n <= 8, "dg_set_tide: at most 8 constituents");
     c->tide = *t;
+    for (int i = 0; i < 8; ++i) {
+        c->tide_nodal[0][i] = 1.0;
+        for (int q = 1; q < 5; ++q) c->tide_nodal[q][i] = 0.0;
+    }
+    return 0;
+}
+
+int dg_set_tide_nodal(dg_ctx* c, const double* f0, const double* f_rate, const double* v0_deg, const double* u0_deg,
+                      const double* u_rate_deg) {
+    DG_REQUIRE(c && f0 && f_rate && v0_deg && u0_deg && u_rate_deg, "dg_set_tide_nodal: null argument");
+    for (int i = 0; i < c->tide.n; ++i) {
+        c->tide_nodal[0][i] = f0[i];
+        c->tide_nodal[1][i] = f_rate[i];
+        c->tide_nodal[2][i] = v0_deg[i];
+        c->tide_nodal[3][i] = u0_deg[i];
+        c->tide_nodal[4][i] = u_rate_deg[i];
+    }
     return 0;
 }
 
@@ -1556,7 +1585,7 @@ int dg_batch_run(dg_batch* b, int n_steps, double cfl, int options, double* t) {
                 cf.a = tab[q][0];
                 cf.b = tab[q][1];
                 cf.c = tab[q][2] * dt;
-                cf.eta = tide_elevation(b->members[m]->tide, t[m] + tab[q][3] * dt);
+                cf.eta = tide_elevation(b->members[m]->tide, b->members[m]->tide_nodal, t[m] + tab[q][3] * dt);
             }
         }
         DG_TRY(hipMemcpyAsync(b->d_coef, b->h_coef, (size_t)M * 4 * sizeof(BatchCoef), hipMemcpyHostToDevice, st));

@@ -29,7 +29,7 @@ BC_WALL, BC_TIDE = 0, 1
 
 # Every entry point of include/dgswe.h
 EXPORTS = (
-    "dg_last_error", "dg_create", "dg_destroy", "dg_upload_form", "dg_set_tide", "dg_state_alloc",
+    "dg_last_error", "dg_create", "dg_destroy", "dg_upload_form", "dg_set_tide", "dg_set_tide_nodal", "dg_state_alloc",
     "dg_state_free", "dg_state_upload", "dg_state_download", "dg_state_copy", "dg_rhs", "dg_face_fluxes", "dg_stage",
     "dg_element_dt", "dg_dt_ssp", "dg_step", "dg_run", "dg_is_finite", "dg_clip_count", "dg_masks",
     "dg_timing_enable", "dg_timing_read", "dg_sync", "dg_copy_bench",
@@ -82,6 +82,7 @@ def load(fma: bool = False):
     L.dg_destroy.argtypes = [vp]
     L.dg_upload_form.argtypes = [vp, ctypes.POINTER(Form)]
     L.dg_set_tide.argtypes = [vp, ctypes.POINTER(TideDesc)]
+    L.dg_set_tide_nodal.argtypes = [vp, _dp, _dp, _dp, _dp, _dp]
     L.dg_state_alloc.argtypes = [vp, ctypes.POINTER(vp)]
     L.dg_state_free.argtypes = [vp, vp]
     L.dg_state_upload.argtypes = [vp, vp, _dp, _dp, _dp]
@@ -245,6 +246,10 @@ class SWEPhysics2D:
         for q in range(n):
             d.amplitude[q], d.omega[q], d.phase[q] = tide.amplitude[q], tide.omega[q], tide.phase[q]
         _check(self.lib, self.lib.dg_set_tide(self.ctx, ctypes.byref(d)), "dg_set_tide")
+        nodal = getattr(tide, "nodal", ())
+        if nodal:
+            cols = [np.array([c[q] for c in nodal], dtype=np.float64) for q in range(5)]
+            _check(self.lib, self.lib.dg_set_tide_nodal(self.ctx, *(_p(c) for c in cols)), "dg_set_tide_nodal")
 
     def close(self):
         if self.ctx:
@@ -469,22 +474,31 @@ class Simulation:
         self.scheme = scheme
         self.max_steps = max_steps
 
-    def run(self, state: DeviceState, t: float, t_end: float, callback=None):
-        n = 0
+    def run(self, state: DeviceState, t: float, t_end: float, callback=None, interval=None):
+        """Steps from t to t_end; with `interval`, every step that reaches the
+        next callback time t_start + k interval lands on it and `callback` is
+        called there (runner.rs:283-287, 405-455), else after every step."""
+        n, k, t_start = 0, 0, t
         while t < t_end:
             if self.max_steps is not None and n >= self.max_steps:
                 raise RuntimeError(f"Maximum step limit ({self.max_steps}) reached")
+            target = t_end if interval is None else min(t_start + (k + 1) * interval, t_end)
             dt = self.physics.compute_dt_ssp(state, self.cfl, self.scheme)
-            landed = t + dt * (1.0 + self.LANDING_SLACK) >= t_end
+            landed = t + dt * (1.0 + self.LANDING_SLACK) >= target
             if landed:
-                dt = t_end - t
+                dt = target - t
             self.physics.step(state, t, dt, self.scheme)
-            t = t_end if landed else t + dt
+            t = target if landed else t + dt
             n += 1
             if not state.is_finite():
                 raise RuntimeError(f"The state is not finite at t = {t:.1f} s: it has blown up")
-            if callback is not None:
-                callback(state, t)
+            if interval is None:
+                if callback is not None:
+                    callback(state, t)
+            elif t >= t_start + (k + 1) * interval:
+                k += 1
+                if callback is not None:
+                    callback(state, t)
         return t, n
 
 

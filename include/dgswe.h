@@ -85,6 +85,12 @@ int dg_destroy(dg_ctx* ctx);
 /* Upload the statics (GpuSWE2D::from_form, gpu/src/swe2d.rs:3360). */
 int dg_upload_form(dg_ctx* ctx, const dg_form* form);
 int dg_set_tide(dg_ctx* ctx, const dg_tide* tide);
+/* The tide's nodal corrections through a run (HarmonicTide::with_nodal_corrections,
+ * src/boundary/harmonic_tide.rs:80-87, from ModelClock::nodal_drift, src/time/clock.rs:108-117):
+ * per constituent of the last dg_set_tide, f = f0 + f_rate t and the phase offset
+ * (v0 + u0 + u_rate t) degrees, t in seconds since the epoch. dg_set_tide resets them. */
+int dg_set_tide_nodal(dg_ctx* ctx, const double* f0, const double* f_rate, const double* v0_deg,
+                      const double* u0_deg, const double* u_rate_deg);
 
 /* Device states (DeviceState::upload / download, gpu/src/swe2d.rs:2384-2396). */
 int dg_state_alloc(dg_ctx* ctx, dg_state** state);

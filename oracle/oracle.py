@@ -71,6 +71,7 @@ def _bind(L):
         L.oc_last_error.restype = ctypes.c_char_p
         L.oc_destroy.argtypes = [ctypes.c_void_p]
         L.oc_set_tide.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp]
+        L.oc_set_tide_nodal.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
         L.oc_tide_elevation.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.oc_tide_elevation.restype = ctypes.c_double
         L.oc_rhs.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, _dp, _dp, _dp]
@@ -164,6 +165,10 @@ class Oracle:
         w = np.array(tide.omega, dtype=np.float64)
         p = np.array(tide.phase, dtype=np.float64)
         self._L.oc_set_tide(self.h, tide.mean, tide.ramp, n, _p(a), _p(w), _p(p))
+        nodal = getattr(tide, "nodal", ())
+        if nodal:
+            cols = [np.array([c[q] for c in nodal], dtype=np.float64) for q in range(5)]
+            self._L.oc_set_tide_nodal(self.h, n, *(_p(c) for c in cols))
 
     def tide_elevation(self, t):
         return self._L.oc_tide_elevation(self.h, t)

@@ -68,6 +68,14 @@ void oc_set_tide(void* p, double mean, double ramp, int n, const double* amp, co
     auto& t = ((Handle*)p)->s->tide;
     t.mean = mean; t.ramp = ramp;
     t.amplitude.assign(amp, amp + n); t.omega.assign(omega, omega + n); t.phase.assign(phase, phase + n);
+    t.f0.clear(); t.f_rate.clear(); t.v0.clear(); t.u0.clear(); t.u_rate.clear();
+}
+
+void oc_set_tide_nodal(void* p, int n, const double* f0, const double* f_rate, const double* v0, const double* u0,
+                       const double* u_rate) {
+    auto& t = ((Handle*)p)->s->tide;
+    t.f0.assign(f0, f0 + n); t.f_rate.assign(f_rate, f_rate + n); t.v0.assign(v0, v0 + n);
+    t.u0.assign(u0, u0 + n); t.u_rate.assign(u_rate, u_rate + n);
 }
 
 double oc_tide_elevation(void* p, double t) { return ((Handle*)p)->s->tide.elevation(t); }

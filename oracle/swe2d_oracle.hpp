@@ -520,9 +520,20 @@ template <class T> inline T friction_cbrt(T x) {
 struct Tide {
     double mean = 0.0, ramp = 0.0;  // ramp <= 0: none
     std::vector<double> amplitude, omega, phase;
+    // NodalDrift per constituent (astronomy.rs:234-259): f0, f_rate, V0 (deg),
+    // u0 (deg), u_rate (deg/s); empty: the identity
+    std::vector<double> f0, f_rate, v0, u0, u_rate;
+    // HarmonicTide::elevation (harmonic_tide.rs:124-136)
     double elevation(double t) const {
+        constexpr double DEG_TO_RAD = M_PI / 180.0;
         double tide = 0.0;
-        for (size_t i = 0; i < amplitude.size(); ++i) tide += 1.0 * amplitude[i] * std::cos(omega[i] * t + phase[i] + 0.0);
+        for (size_t i = 0; i < amplitude.size(); ++i) {
+            const bool nodal = i < f0.size();
+            const double f = nodal ? f0[i] + f_rate[i] * t : 1.0 + 0.0 * t;
+            const double u = nodal ? u0[i] + u_rate[i] * t : 0.0 + 0.0 * t;
+            const double v = nodal ? v0[i] : 0.0;
+            tide += f * amplitude[i] * std::cos(omega[i] * t + phase[i] + (v + u) * DEG_TO_RAD);
+        }
         double rf = 1.0;
         if (ramp > 0.0) {
             double tau = std::clamp(t / ramp, 0.0, 1.0);
