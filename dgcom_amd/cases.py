@@ -216,6 +216,7 @@ class Case:
     g: float = 9.81
     h_min: float = 1e-6
     h_dry: float = 1e-3
+    incoming_wave: bool = False  # tide data as a wave entering from still water, not at rest
 
     @property
     def n_elements(self) -> int:
@@ -379,12 +380,13 @@ def harmonic_tide(constituents, ramp_hours, start=None, mean=0.0):
                 nodal=tuple(nodal))
 
 
-def tidal_channel(order=2, nx=40, ny=4) -> Case:
+def tidal_channel(order=2, nx=40, ny=4, velocity="at_rest") -> Case:
     """cases/tidal_channel/case.toml: a channel 40 km x 4 km open to the sea at its
     west end, its bed rising from 30 m below mean sea level at the mouth to 2 m
     above at the head (tidal flats); M2 1.0 m / G 0 and S2 0.35 m / G 30 deg with
     a 2 h ramp and the nodal corrections of a run from 2025-06-15 00:00 UTC through
-    the characteristic boundary at rest; Coriolis at 63.5 N, Manning 0.025, wet/dry
+    the characteristic boundary, its external velocity `velocity` ("at_rest", the
+    case's, or "incoming_wave"); Coriolis at 63.5 N, Manning 0.025, wet/dry
     with the defaults; still water at mean sea level; SSP-RK(4,3) at its default
     CFL, 0.9 of the linear limit (1.359 at N = 2)."""
     mesh = structured_mesh(0.0, 40_000.0, 0.0, 4_000.0, nx, ny, sides=(WALL, WALL, WALL, TIDE))
@@ -393,5 +395,7 @@ def tidal_channel(order=2, nx=40, ny=4) -> Case:
     h, hu, hv = still_water(bed)
     tide = harmonic_tide((("M2", 1.0, 0.0), ("S2", 0.35, 30.0)), 2.0, start=(2025, 6, 15))
     lin43 = (2.2, 1.51, 1.15, 0.93)   # linear CFL limits of SSP-RK(4,3) (swe_2d.rs:1263-1302)
+    if velocity not in ("at_rest", "incoming_wave"):
+        raise ValueError("velocity must be 'at_rest' or 'incoming_wave'")
     return Case("tidal_channel", order, mesh, bed, np.full(mesh.n_elements, 0.025), coriolis_parameter(63.5), tide,
-                h, hu, hv, cfl=0.9 * lin43[order - 1], scheme=1, n_steps=0)
+                h, hu, hv, cfl=0.9 * lin43[order - 1], scheme=1, n_steps=0, incoming_wave=velocity == "incoming_wave")

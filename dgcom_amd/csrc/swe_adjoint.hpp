@@ -269,12 +269,15 @@ DG_HD void wall_ghost_adj(double h, double hu, double hv, double nx, double ny, 
 // characteristic (elevation-only data at rest) reverse: cotangents of (h, hu, hv, bed)
 // given the boundary state's
 DG_HD void characteristic_adj(double h, double hu, double hv, double bed, double eta_e, double nx, double ny, double g,
-                              double h_min, const Cons& qbb, double& hb, double& hub, double& hvb, double& bedb) {
+                              double h_min, bool incoming, const Cons& qbb, double& hb, double& hub, double& hvb,
+                              double& bedb) {
     const double xe = eta_e - bed;
     const double h_e = fmax_(xe, 0.0);
     double ui = 0.0, vi = 0.0, inv1 = 0.0;
     if (h > h_min) { inv1 = 1.0 / h; ui = hu * inv1; vi = hv * inv1; }
-    const double ut_e = -ui * ny + vi * nx, un_e = 0.0;
+    const double c_0 = sqrt(g * fmax_(-bed, 0.0));
+    const double un_e = incoming ? 2.0 * (c_0 - sqrt(g * h_e)) : 0.0;
+    const double ut_e = -ui * ny + vi * nx;
     const double h_i = fmax_(h, 0.0);
     double un_i = 0.0, ut_i = 0.0, u2 = 0.0, v2 = 0.0;
     if (h_i > h_min) {
@@ -303,11 +306,11 @@ DG_HD void characteristic_adj(double h, double hu, double hv, double bed, double
     const double ut_bb = -ubb * ny + vbb * nx;
     double ut_ib = 0.0, ut_eb = 0.0;
     if (out_flow) ut_ib += ut_bb; else ut_eb += ut_bb;
-    double h_ib = 0.0, h_eb = 0.0, un_ib = 0.0, w_outb = 0.0, w_inb = 0.0;
+    double h_ib = 0.0, h_eb = 0.0, un_ib = 0.0, w_outb = 0.0, w_inb = 0.0, un_eb = 0.0;
     double ccb = 0.0;
     switch (branch) {
         case 1: h_ib += h_bb; un_ib += un_bb; break;
-        case 2: h_eb += h_bb; break;  // un_e is a constant (0)
+        case 2: h_eb += h_bb; un_eb += un_bb; break;
         case 3: ccb = h_bb * 2.0 * cc / g + un_bb;
             if (takes_first_max(w_out / 3.0, 0.0)) w_outb += ccb / 3.0;
             break;
@@ -321,7 +324,14 @@ DG_HD void characteristic_adj(double h, double hu, double hv, double bed, double
         default: break;
     }
     un_ib += w_outb;
-    const double c_ib = 2.0 * w_outb, c_eb = -2.0 * w_inb;
+    un_eb += w_inb;
+    // the incoming wave's u_n,e = 2 (c_0 - c_e): c_0 = sqrt(g max(-B, 0)) and c_e
+    double c_eb = -2.0 * w_inb;
+    if (incoming) {
+        c_eb += -2.0 * un_eb;
+        if (c_0 > 0.0) bedb += 2.0 * un_eb * (-g) / (2.0 * c_0);
+    }
+    const double c_ib = 2.0 * w_outb;
     if (c_i > 0.0) h_ib += c_ib * g / (2.0 * c_i);
     if (c_e > 0.0) h_eb += c_eb * g / (2.0 * c_e);
     if (h_i > h_min) {

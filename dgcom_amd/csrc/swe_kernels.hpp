@@ -463,7 +463,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             const uint32_t l = lk[r];
             Cons fstar;
             if (l == LINK_BOUNDARY && tg[r] == DG_BC_TIDE) {
-                Cons qb = characteristic(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min);
+                Cons qb = characteristic(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min, (p.flags & DG_INCOMING_WAVE) != 0);
                 Node nb = make_node(qb.h, qb.hu, qb.hv, q.b, p.h_min);
                 fstar = ec_flux(nb, nb, nx, ny, p.g);
             } else {

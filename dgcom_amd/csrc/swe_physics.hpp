@@ -176,11 +176,14 @@ DG_HD void reconstruct(const Side& m, const Side& c, const Side& p, double xm, d
 // characteristic_state (boundary/characteristic.rs:481-520) for elevation-only
 // data at rest, whose tangential velocity is the interior's (characteristic.rs:411-438)
 DG_HD Cons characteristic(double h, double hu, double hv, double bed, double eta_e, double nx, double ny, double g,
-                          double h_min) {
+                          double h_min, bool incoming) {
     double h_e = fmax_(eta_e - bed, 0.0);
     double ui = 0.0, vi = 0.0;
     if (h > h_min) { double inv = 1.0 / h; ui = hu * inv; vi = hv * inv; }
-    double ut_e = -ui * ny + vi * nx, un_e = 0.0;
+    // at rest, or a simple wave entering from still water at mean sea level
+    // (characteristic.rs:424-426): u_n,e = 2 (sqrt(g H) - sqrt(g h_e)), H = -B
+    double un_e = incoming ? 2.0 * (sqrt(g * fmax_(-bed, 0.0)) - sqrt(g * h_e)) : 0.0;
+    double ut_e = -ui * ny + vi * nx;
     double h_i = fmax_(h, 0.0);
     double un_i = 0.0, ut_i = 0.0;
     if (h_i > h_min) {

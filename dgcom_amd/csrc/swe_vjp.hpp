@@ -599,15 +599,16 @@ __global__ __launch_bounds__(64) DG_WAVES_PER_EU(3) void face_vjp_kernel(VjpPara
         const Cons fsb = {fc[0], fc[1], fc[2]};
         NodeBar qb = nodebar_zero();
         if (l == LINK_BOUNDARY && p.tags[4 * k + f] == DG_BC_TIDE) {
-            const Cons qbd = characteristic(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min);
+            const bool incoming = (p.flags & DG_INCOMING_WAVE) != 0;
+            const Cons qbd = characteristic(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min, incoming);
             const Node nbn = make_node(qbd.h, qbd.hu, qbd.hv, q.b, p.h_min);
             NodeBar nbb = nodebar_zero();
             ec_flux_adj(nbn, nbn, nx, ny, p.g, fsb, nbb, nbb);
             double bh = 0.0, bhu = 0.0, bhv = 0.0, bbb = 0.0;
             make_node_adj(qbd.h, qbd.hu, qbd.hv, p.h_min, nbb, bh, bhu, bhv, bbb);
             qb.b += bbb;
-            characteristic_adj(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min, Cons{bh, bhu, bhv}, qb.h, qb.hu,
-                               qb.hv, qb.b);
+            characteristic_adj(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min, incoming, Cons{bh, bhu, bhv}, qb.h,
+                               qb.hu, qb.hv, qb.b);
         } else {
             const bool wall = l == LINK_BOUNDARY;
             const bool left = wall || (l & 4u);

@@ -36,7 +36,10 @@ typedef struct dg_state dg_state;
 enum { DG_BC_WALL = 0, DG_BC_TIDE = 1 };
 
 /* Flags of dg_form.flags */
-enum { DG_FRICTION = 1, DG_WET_DRY = 2, DG_SUBCELLS = 4 };
+enum { DG_FRICTION = 1, DG_WET_DRY = 2, DG_SUBCELLS = 4,
+       /* elevation-only tide data as a wave entering from still water (ElevationOnly::IncomingWave,
+        * src/boundary/characteristic.rs:49-53,424-426); without it, at rest (ElevationOnly::AtRest) */
+       DG_INCOMING_WAVE = 8 };
 
 /* Schemes (src/time/integrator.rs:226-263) */
 enum { DG_SSP_RK3 = 0, DG_SSP_RK43 = 1 };
@@ -60,7 +63,7 @@ typedef struct dg_form {
     double g, h_min, h_dry;       /* 9.81, 1e-6, 1e-3 */
     double relaxation_time;       /* thin-layer tau, 1 s */
     double max_velocity;          /* 20 m/s */
-    int flags;                    /* DG_FRICTION | DG_WET_DRY | DG_SUBCELLS */
+    int flags;                    /* DG_FRICTION | DG_WET_DRY | DG_SUBCELLS | DG_INCOMING_WAVE */
 } dg_form;
 
 /* Uniform harmonic tide of the open boundary (src/boundary/harmonic_tide.rs:24-136). */

@@ -20,7 +20,7 @@ _u32p = ctypes.POINTER(ctypes.c_uint32)
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 _lib = None
 
-FLAG_SUBCELLS, FLAG_FRICTION, FLAG_WET_DRY, FLAG_MANUFACTURED = 1, 2, 4, 8
+FLAG_SUBCELLS, FLAG_FRICTION, FLAG_WET_DRY, FLAG_MANUFACTURED, FLAG_INCOMING = 1, 2, 4, 8, 16
 SURFACE_HLL, SURFACE_ES, SURFACE_EC = 0, 1, 2
 
 
@@ -101,6 +101,7 @@ def _bind(L):
         L.oc_tlm_create.argtypes = L.oc_create.argtypes
         L.oc_tlm_destroy.argtypes = [ctypes.c_void_p]
         L.oc_tlm_set_tide.argtypes = L.oc_set_tide.argtypes
+        L.oc_tlm_set_tide_nodal.argtypes = L.oc_set_tide_nodal.argtypes
         L.oc_tlm_set_param_tangents.argtypes = [ctypes.c_void_p, _dp, _dp]
         _pp = ctypes.POINTER(_dp)
         L.oc_tlm_stage.argtypes = [ctypes.c_void_p, _pp, _pp, _pp, _pp] + [ctypes.c_double] * 4 + [ctypes.c_int, _pp, _pp]
@@ -129,7 +130,7 @@ class Oracle:
     """The CPU restatement on one problem (a `dgcom_amd.cases.Case`-shaped input)."""
 
     def __init__(self, order, mesh, bed, manning=None, f0=0.0, g=9.81, h_min=1e-6, h_dry=1e-3, surface=SURFACE_HLL,
-                 subcells=True, friction=True, wet_dry=True, manufactured=False, tide=None):
+                 subcells=True, friction=True, wet_dry=True, manufactured=False, tide=None, incoming_wave=False):
         self._L = L = lib()
         self.order = order
         self.nn = (order + 1) ** 2
@@ -140,7 +141,8 @@ class Oracle:
         self._bed = np.ascontiguousarray(bed, dtype=np.float64)
         man = None if manning is None else np.ascontiguousarray(manning, dtype=np.float64)
         flags = (FLAG_SUBCELLS if subcells else 0) | (FLAG_FRICTION if friction else 0) | \
-                (FLAG_WET_DRY if wet_dry else 0) | (FLAG_MANUFACTURED if manufactured else 0)
+                (FLAG_WET_DRY if wet_dry else 0) | (FLAG_MANUFACTURED if manufactured else 0) | \
+                (FLAG_INCOMING if incoming_wave else 0)
         self.h = L.oc_create(order, self.K, _p(self._corners), self._links.ctypes.data_as(_u32p),
                              self._tags.ctypes.data_as(_u8p), _p(self._bed), None if man is None else _p(man),
                              g, h_min, h_dry, f0, surface, flags)
@@ -152,7 +154,7 @@ class Oracle:
     @classmethod
     def from_case(cls, case, **kw):
         return cls(case.order, case.mesh, case.bed, manning=case.manning, f0=case.f0, g=case.g, h_min=case.h_min,
-                   h_dry=case.h_dry, tide=case.tide, **kw)
+                   h_dry=case.h_dry, tide=case.tide, incoming_wave=getattr(case, "incoming_wave", False), **kw)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -245,7 +247,7 @@ class TangentLinear:
     prescribed, as a discrete adjoint freezes them)."""
 
     def __init__(self, order, mesh, bed, manning=None, f0=0.0, g=9.81, h_min=1e-6, h_dry=1e-3, tide=None,
-                 subcells=True, friction=True, wet_dry=True):
+                 subcells=True, friction=True, wet_dry=True, incoming_wave=False):
         self._L = L = lib()
         self.K = mesh.n_elements
         self.nn = (order + 1) ** 2
@@ -255,7 +257,8 @@ class TangentLinear:
                       np.ascontiguousarray(bed, dtype=np.float64),
                       None if manning is None else np.ascontiguousarray(manning, dtype=np.float64)]
         c, l, t, b, m = self._keep
-        flags = (FLAG_SUBCELLS if subcells else 0) | (FLAG_FRICTION if friction else 0) | (FLAG_WET_DRY if wet_dry else 0)
+        flags = (FLAG_SUBCELLS if subcells else 0) | (FLAG_FRICTION if friction else 0) | \
+                (FLAG_WET_DRY if wet_dry else 0) | (FLAG_INCOMING if incoming_wave else 0)
         self.h = L.oc_tlm_create(order, self.K, _p(c), l.ctypes.data_as(_u32p), t.ctypes.data_as(_u8p), _p(b),
                                  None if m is None else _p(m), g, h_min, h_dry, f0, SURFACE_HLL, flags)
         if not self.h:
@@ -265,11 +268,15 @@ class TangentLinear:
             w = np.array(tide.omega, dtype=np.float64)
             ph = np.array(tide.phase, dtype=np.float64)
             L.oc_tlm_set_tide(self.h, tide.mean, tide.ramp, len(a), _p(a), _p(w), _p(ph))
+            nodal = getattr(tide, "nodal", ())
+            if nodal:
+                cols = [np.array([c[q] for c in nodal], dtype=np.float64) for q in range(5)]
+                L.oc_tlm_set_tide_nodal(self.h, len(a), *(_p(c) for c in cols))
 
     @classmethod
     def from_case(cls, case, **kw):
         return cls(case.order, case.mesh, case.bed, manning=case.manning, f0=case.f0, g=case.g, h_min=case.h_min,
-                   h_dry=case.h_dry, tide=case.tide, **kw)
+                   h_dry=case.h_dry, tide=case.tide, incoming_wave=getattr(case, "incoming_wave", False), **kw)
 
     def __del__(self):
         if getattr(self, "h", None):

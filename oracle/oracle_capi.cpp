@@ -38,7 +38,7 @@ int oc_max_threads(void) {
 #endif
 }
 
-// flags: bit0 subcells, bit1 friction, bit2 wet_dry, bit3 manufactured source
+// flags: bit0 subcells, bit1 friction, bit2 wet_dry, bit3 manufactured source, bit4 incoming-wave tide
 // surface: 0 hydrostatic HLL (WetDry), 1 entropy stable, 2 entropy conservative
 void* oc_create(int order, int64_t K, const double* corners, const uint32_t* links, const uint8_t* tags,
                 const double* bed, const double* manning, double g, double h_min, double h_dry, double f0,
@@ -51,6 +51,7 @@ void* oc_create(int order, int64_t K, const double* corners, const uint32_t* lin
     auto& c = h->s->cfg;
     c.g = g; c.h_min = h_min; c.h_dry = h_dry; c.subcell_depth = h_dry; c.f0 = f0; c.surface = surface;
     c.subcells = flags & 1; c.friction = flags & 2; c.wet_dry = flags & 4; c.manufactured = flags & 8;
+    c.incoming_wave = flags & 16;
     size_t n = (size_t)K * h->s->ops.nn;
     std::copy(bed, bed + n, h->s->bed.begin());
     if (manning) std::copy(manning, manning + K, h->s->manning.begin());
@@ -164,6 +165,7 @@ void* oc_tlm_create(int order, int64_t K, const double* corners, const uint32_t*
     auto& c = h->s->cfg;
     c.g = g; c.h_min = h_min; c.h_dry = h_dry; c.subcell_depth = h_dry; c.f0 = f0; c.surface = surface;
     c.subcells = flags & 1; c.friction = flags & 2; c.wet_dry = flags & 4; c.manufactured = flags & 8;
+    c.incoming_wave = flags & 16;
     size_t n = (size_t)K * h->s->ops.nn;
     for (size_t i = 0; i < n; ++i) h->s->bed[i] = Dual(bed[i]);
     if (manning) for (int64_t k = 0; k < K; ++k) h->s->manning[k] = Dual(manning[k]);
@@ -181,6 +183,13 @@ void oc_tlm_set_tide(void* p, double mean, double ramp, int n, const double* amp
     auto& t = ((TlmHandle*)p)->s->tide;
     t.mean = mean; t.ramp = ramp;
     t.amplitude.assign(amp, amp + n); t.omega.assign(omega, omega + n); t.phase.assign(phase, phase + n);
+}
+
+void oc_tlm_set_tide_nodal(void* p, int n, const double* f0, const double* f_rate, const double* v0, const double* u0,
+                           const double* u_rate) {
+    auto& t = ((TlmHandle*)p)->s->tide;
+    t.f0.assign(f0, f0 + n); t.f_rate.assign(f_rate, f_rate + n); t.v0.assign(v0, v0 + n);
+    t.u0.assign(u0, u0 + n); t.u_rate.assign(u_rate, u_rate + n);
 }
 
 // Tangents of the bed and of Manning n (directions of the parameters)

@@ -23,6 +23,7 @@ struct Config {
     bool friction = true;                     // Manning, point-implicit
     bool wet_dry = true;                      // wet/dry correction, thin layer, desingularised friction velocity
     bool manufactured = false;                // the manufactured-solution source of tests/convergence_test.rs:961-1020
+    bool incoming_wave = false;               // tide data as a wave entering from still water (characteristic.rs:424-426)
 };
 
 // SSP schemes (time/integrator.rs:226-263)
@@ -132,12 +133,14 @@ struct Solver {
         double g = cfg.g, hm = cfg.h_min;
         T zero(0.0);
         if (tags[4 * k + f] == BC_TIDE) {
-            // CharacteristicOBC<HarmonicTide>, elevation-only data at rest (characteristic.rs:411-438)
+            // CharacteristicOBC<HarmonicTide>, elevation-only data at rest or as an incoming
+            // wave from still water, u_n,e = 2 (sqrt(g H) - sqrt(g h_e)), H = -B (characteristic.rs:411-438)
             T h_e = vmax(T(eta_e) - q.b, zero);
             T u = zero, v = zero;
             if (q.h > T(hm)) { T inv = T(1.0) / q.h; u = q.hu * inv; v = q.hv * inv; }
+            T un_e = cfg.incoming_wave ? T(2.0) * (sq_sqrt(T(g) * vmax(-q.b, zero)) - sq_sqrt(T(g) * h_e)) : zero;
             T ut_e = -u * T(ny) + v * T(nx);
-            State<T> qb = characteristic_state(State<T>{q.h, q.hu, q.hv}, h_e, zero, ut_e, T(nx), T(ny), g, hm);
+            State<T> qb = characteristic_state(State<T>{q.h, q.hu, q.hv}, h_e, un_e, ut_e, T(nx), T(ny), g, hm);
             NodeState<T> nb = node_state(qb.h, qb.hu, qb.hv, q.b, hm);
             return wintermeyer_flux(nb, nb, T(nx), T(ny), g);
         }

@@ -45,7 +45,7 @@ def parallelogram(c):
 class EmulatedKernels:
     """The product's stage and time-step kernels run on host threads."""
 
-    def __init__(self, case, flags=7):
+    def __init__(self, case, flags=None):
         self.case = case
         self.order = case.order
         self.K = case.n_elements
@@ -55,7 +55,8 @@ class EmulatedKernels:
         self.affine = parallelogram(self.corners)
         self.bed = np.ascontiguousarray(case.bed, dtype=np.float64)
         self.manning = np.ascontiguousarray(case.manning, dtype=np.float64)
-        self.flags = flags
+        # friction, wet/dry and subcells, and the incoming-wave tide where the case has it
+        self.flags = (7 | (8 if getattr(case, "incoming_wave", False) else 0)) if flags is None else flags
 
     def _p(self, a):
         return a.ctypes.data_as(_dp)

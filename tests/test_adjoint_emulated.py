@@ -76,6 +76,22 @@ def test_stage_vjp_dot_product(name, make, mode):
         assert abs(lhs - rhs) <= TOL * scale or (lhs == 0.0 and rhs == 0.0), (which, lhs, rhs)
 
 
+@pytest.mark.parametrize("velocity", ["at_rest", "incoming_wave"])
+def test_tide_boundary_vjp_dot_product(velocity):
+    # the tidal channel's characteristic boundary with the nodal tide at full
+    # amplitude (4 h, past the ramp), at rest or as an incoming wave
+    c = cases.tidal_channel(velocity=velocity)
+    orc = O.Oracle.from_case(c)
+    base = perturbed(c, seed=2)
+    dt = orc.dt(*base, c.cfl)
+    t = 4 * 3600.0
+    inp = orc.stage(base, base, 1.0, 0.0, dt, t, False)
+    res = dot_test(c, base, inp, 0.75, 0.25, 0.25 * dt, t + 0.5 * dt, False, 13)
+    for which, (lhs, rhs) in res.items():
+        scale = max(abs(lhs), 1e-300)
+        assert abs(lhs - rhs) <= TOL * scale or (lhs == 0.0 and rhs == 0.0), (which, lhs, rhs)
+
+
 def test_theta_limiter_adjoint():
     """A draining shore with a long stage: elements whose combined depth has a
     negative node and a positive mean take the theta limiter."""

@@ -3,7 +3,8 @@ answer (notes/gpu.md:146): 25 h of M2 and S2 with the ramp and the nodal
 corrections of a run from 2025-06-15 through the characteristic boundary at
 rest, Coriolis at 63.5 N, Manning 0.025, drying flats, SSP-RK(4,3) at the
 default CFL, outputs every 30 min (each step that reaches one lands on it):
-5,765 steps and no clip.
+5,765 steps and no clip; with `velocity = "incoming_wave"` (the tide as a wave
+entering from still water) 5,783 steps.
 
 The step count follows the whole run: every adaptive step's positivity-relaxed
 bound over the flooding and draining flats, the tide's amplitude and phase at
@@ -53,22 +54,29 @@ def test_the_case_as_the_reference_resolves_it(case):
     assert case.bed.max() == 2.0 and case.bed.min() == -30.0
 
 
-def test_known_answer_5765_steps(case):
-    x, n, k, clips = run_oracle(case)
+KNOWN_STEPS = {"at_rest": 5765, "incoming_wave": 5783}
+
+
+@pytest.mark.parametrize("velocity", ["at_rest", "incoming_wave"])
+def test_known_answer_steps(velocity):
+    c = cases.tidal_channel(velocity=velocity)
+    x, n, k, clips = run_oracle(c)
     assert k == 50
-    assert n == 5765
+    assert n == KNOWN_STEPS[velocity]
     assert clips == 0
     assert np.isfinite(x[0]).all() and x[0].min() >= 0.0
 
 
 @pytest.mark.gpu
-def test_gpu_run_equals_the_restatement(case):
+@pytest.mark.parametrize("velocity", ["at_rest", "incoming_wave"])
+def test_gpu_run_equals_the_restatement(velocity):
     from dgcom_amd import swe2d
+    case = cases.tidal_channel(velocity=velocity)
     x, n, _, _ = run_oracle(case)
     phys = swe2d.SWEPhysics2D.from_case(case)
     s = phys.state(case.h, case.hu, case.hv)
     t, steps = swe2d.Simulation(phys, case.cfl, scheme=case.scheme).run(s, 0.0, T_END, interval=INTERVAL)
-    assert steps == n == 5765 and t == T_END
+    assert steps == n == KNOWN_STEPS[velocity] and t == T_END
     got = s.download()
     for a, b in zip(got, x):
         assert np.array_equal(a, b)
