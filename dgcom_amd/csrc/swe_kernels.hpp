@@ -352,7 +352,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
     double g_rx = 0.0, g_ry = 0.0, g_sx = 0.0, g_sy = 0.0, det = 1.0, det_inv = 1.0;
     double bxr = 0.0, byr = 0.0, bxs = 0.0, bys = 0.0;
     double bh0 = 0.0, bhu0 = 0.0, bhv0 = 0.0, mann = 0.0;
-    VjpPre vpre = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    VjpPre vpre = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     bool affine = false;
     if (active) {
         affine = p.affine[k] != 0;
@@ -413,6 +413,8 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             if (!p.rhs_only && p.a != 0.0) { vpre.bh = p.bh[n]; vpre.bhu = p.bhu[n]; vpre.bhv = p.bhv[n]; }
             if (p.flags & DG_FRICTION) vpre.mn = p.manning[k];
             vpre.wh = vp->wh[n]; vpre.whu = vp->whu[n]; vpre.whv = vp->whv[n];
+            if (!p.rhs_only && p.a != 0.0 && vp->gbh) { vpre.gbh = vp->gbh[n]; vpre.gbhu = vp->gbhu[n]; vpre.gbhv = vp->gbhv[n]; }
+            vpre.gbed = vp->gbed[n];
         }
     }
     if (t < EPB) { sflag[t] = 0; wflag[t] = 0; }

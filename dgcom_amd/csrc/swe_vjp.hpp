@@ -30,8 +30,9 @@ namespace dgswe {
 
 // The reverse stage's per-node reads, issued with the forward's loads at the
 // start of the stage kernel: the saved RHS, the base state, Manning n of the
-// element, the output cotangent
-struct VjpPre { double rh, rhu, rhv, bh, bhu, bhv, mn, wh, whu, whv; };
+// element, the output cotangent, and the accumulators it adds to (the base's
+// and the bed's cotangents)
+struct VjpPre { double rh, rhu, rhv, bh, bhu, bhv, mn, wh, whu, whv, gbh, gbhu, gbhv, gbed; };
 
 struct VjpParams {
     StageParams s;                 // the forward stage's inputs (its outputs are not written)
@@ -315,7 +316,9 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
         }
         // combination
         if (active) {
-            if (p.a != 0.0 && P.gbh) { P.gbh[n] += p.a * gh; P.gbhu[n] += p.a * ghu; P.gbhv[n] += p.a * ghv; }
+            if (p.a != 0.0 && P.gbh) {
+                P.gbh[n] = pre.gbh + p.a * gh; P.gbhu[n] = pre.gbhu + p.a * ghu; P.gbhv[n] = pre.gbhv + p.a * ghv;
+            }
             if (p.b != 0.0) { gi_h += p.b * gh; gi_hu += p.b * ghu; gi_hv += p.b * ghv; }
             if (p.c != 0.0) rb = {p.c * gh, p.c * ghu, p.c * ghv};
         }
@@ -554,7 +557,7 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
             P.gihu[n] += gi_hu;
             P.gihv[n] += gi_hv;
         }
-        P.gbed[n] += gb;
+        P.gbed[n] = pre.gbed + gb;
     }
     if (P.gman) {
         lds_sync();  // sman overlays tcot
