@@ -399,3 +399,44 @@ def tidal_channel(order=2, nx=40, ny=4, velocity="at_rest") -> Case:
         raise ValueError("velocity must be 'at_rest' or 'incoming_wave'")
     return Case("tidal_channel", order, mesh, bed, np.full(mesh.n_elements, 0.025), coriolis_parameter(63.5), tide,
                 h, hu, hv, cfl=0.9 * lin43[order - 1], scheme=1, n_steps=0, incoming_wave=velocity == "incoming_wave")
+
+
+# ---------------------------------------------------------------------------
+# Local time stepping: the reference's graded channel (tests/local_time_stepping_test.rs:140-147,404-410)
+# ---------------------------------------------------------------------------
+def graded_mesh(lx, ly, nx, ny, ratio, sides=(WALL, WALL, WALL, WALL)) -> Mesh:
+    """nx columns growing geometrically by `ratio` over [0, lx] x [0, ly]
+    (graded_mesh, local_time_stepping_test.rs:140-147)."""
+    n = float(nx)
+
+    def grade(vx, vy):
+        return lx * (ratio ** (n * vx) - 1.0) / (ratio ** n - 1.0), vy
+
+    return structured_mesh(0.0, 1.0, 0.0, ly, nx, ny, sides=sides, distort=grade)
+
+
+def _gaussian(x, y, x0, y0, width):
+    return np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / (width * width))
+
+
+def graded_channel(order=2, nx=16, ny=3, ratio=1.25, hump=0.5, island=False, friction=False, scheme=0,
+                   cfl=0.5) -> Case:
+    """The reference's local-time-stepping channel (local_time_stepping_test.rs:
+    404-499): 10 km x 2 km, `nx` columns growing by `ratio` (16 by 1.25: 28x from
+    the first to the last), a bed from 20 to 110 m deep with a cross-channel
+    cosine, walls all round, no rotation. Without `island`: a Gaussian hump of
+    `hump` m at (2 km, 1 km) on still water (several_levels_conserve_mass). With
+    `island`: a rough bed with an island above the surface, the lake at rest,
+    Manning 0.025 (lake_at_rest_with_dry_land_across_levels)."""
+    mesh = graded_mesh(10_000.0, 2_000.0, nx, ny, ratio)
+    x, y = node_coordinates(mesh, order)
+    x, y = x.ravel(), y.ravel()
+    bed = -(20.0 + 90.0 * x / 10_000.0) - 5.0 * np.cos(2.0 * math.pi * y / 2_000.0)
+    if island:
+        bed = bed + 8.0 * np.sin(x / 700.0) * np.cos(y / 400.0) + 140.0 * _gaussian(x, y, 6_000.0, 1_000.0, 900.0)
+        eta = np.zeros_like(x)
+    else:
+        eta = hump * _gaussian(x, y, 2_000.0, 1_000.0, 800.0)
+    h = np.maximum(eta - bed, 0.0)
+    return Case("graded_channel", order, mesh, bed, np.full(mesh.n_elements, 0.025 if friction else 0.0), 0.0,
+                Tide(), h, np.zeros_like(h), np.zeros_like(h), cfl=cfl, scheme=scheme, n_steps=0)

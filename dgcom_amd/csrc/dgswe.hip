@@ -86,6 +86,16 @@ struct dg_ctx {
     uint8_t* kind = nullptr;                         // [K] subcell elements of the last VJP
     dg_state* adj[2] = {nullptr, nullptr};           // stage cotangents of dg_step_vjp
     dg_state* rhsbuf[4] = {nullptr, nullptr, nullptr, nullptr};  // stage RHS saved by the forward for the VJP
+    // local time stepping: the coarse step's levels and last-stage rates [K], the
+    // passes' element orders [n_stages + 1][K] (stage rates, then the level;
+    // largest key first, element order within a key) and, on the host, how
+    // many have a key >= d
+    uint8_t *lts_level = nullptr, *lts_lrate = nullptr;
+    uint32_t* lts_order = nullptr;
+    int lts_stages = 0, lts_finest = -1;
+    std::vector<uint8_t> lts_keys;  // the host copy of the last levels and rates
+    std::vector<int64_t> lts_count;  // [n_stages + 1][DG_MAX_LEVELS + 1]
+    uint8_t* depth_flags = nullptr;
     // stations
     int n_st = 0;
     int64_t* st_elem = nullptr;
@@ -151,11 +161,18 @@ static double tide_elevation(const dg_tide& t, const double (&nod)[5][8], double
 static int free_statics(dg_ctx* c) {
     void* ptrs[] = {c->corners, c->bed, c->manning, c->element_dt, c->links, c->tags, c->affine, c->queue,
                     c->queue_count, c->dt_slots, c->sub_list, c->chunk_total, c->gbed, c->gman, c->facebuf,
-                    c->innerbuf, c->facecot, c->kind, c->st_elem, c->st_w, c->st_eta, c->geo, c->fgeo};
+                    c->innerbuf, c->facecot, c->kind, c->st_elem, c->st_w, c->st_eta, c->geo, c->fgeo,
+                    c->lts_level, c->lts_lrate, c->lts_order, c->depth_flags};
     for (void* p : ptrs)
         if (p) DG_TRY(hipFree(p));
     c->gbed = c->gman = c->facebuf = c->innerbuf = c->facecot = nullptr;
     c->geo = c->fgeo = nullptr;
+    c->lts_level = c->lts_lrate = c->depth_flags = nullptr;
+    c->lts_order = nullptr;
+    c->lts_stages = 0;
+    c->lts_finest = -1;
+    c->lts_keys.clear();
+    c->lts_count.clear();
     c->kind = nullptr;
     c->st_elem = nullptr;
     c->st_w = c->st_eta = nullptr;
@@ -1013,6 +1030,178 @@ int dg_stage(dg_ctx* c, const dg_state* base, const dg_state* input, dg_state* o
     DG_REQUIRE(out != input && out != base, "dg_stage: out must differ from base and input");
     DG_TRY(hipSetDevice(c->device));
     if (int r = launch_stage(c, base->d, input->d, out->d, st->a, st->b, st->c, st->time, 0)) return r;
+    DG_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Local time stepping (LocalTimeStepping, src/time/multirate.rs:296-391, as
+// SWEPhysics2D implements it, src/physics/builder.rs:546-783)
+// ---------------------------------------------------------------------------
+int dg_begin_coarse_step(dg_ctx* c, const uint8_t* level, const uint8_t* rates, int n_stages) {
+    DG_REQUIRE(c && level && rates && c->K > 0, "dg_begin_coarse_step: null argument or no form");
+    DG_REQUIRE(n_stages == 3 || n_stages == 4, "dg_begin_coarse_step: 3 or 4 stages");
+    const int64_t K = c->K;
+    const int nkey = n_stages + 1;
+    // unchanged levels and rates keep their lists (gpu/src/swe2d.rs:3285)
+    std::vector<uint8_t> keys((size_t)nkey * K);
+    std::memcpy(keys.data(), rates, (size_t)n_stages * K);
+    std::memcpy(keys.data() + (size_t)n_stages * K, level, (size_t)K);
+    if (c->lts_order && c->lts_stages == n_stages && keys == c->lts_keys) return 0;
+    int finest = 0;
+    for (uint8_t v : keys) finest = std::max(finest, (int)v);
+    DG_REQUIRE(finest <= DG_MAX_LEVELS, "dg_begin_coarse_step: more than DG_MAX_LEVELS levels");
+    for (int64_t k = 0; k < K; ++k)
+        for (int s = 0; s < n_stages; ++s)
+            DG_REQUIRE(rates[(size_t)s * K + k] >= level[k], "dg_begin_coarse_step: a rate below its element's level");
+    DG_TRY(hipSetDevice(c->device));
+    if (!c->lts_order || c->lts_stages != n_stages) {
+        if (c->lts_order) DG_TRY(hipFree(c->lts_order));
+        if (!c->lts_level) DG_TRY(hipMalloc(&c->lts_level, (size_t)K));
+        if (!c->lts_lrate) DG_TRY(hipMalloc(&c->lts_lrate, (size_t)K));
+        DG_TRY(hipMalloc(&c->lts_order, (size_t)nkey * K * sizeof(uint32_t)));
+    }
+    // per key: elements by key, largest first, element order within a key
+    // (counting sort), and how many have a key of at least d
+    std::vector<uint32_t> order((size_t)nkey * K);
+    std::vector<int64_t> count((size_t)nkey * (DG_MAX_LEVELS + 1), 0);
+    for (int q = 0; q < nkey; ++q) {
+        const uint8_t* key = keys.data() + (size_t)q * K;
+        int64_t hist[DG_MAX_LEVELS + 2] = {0};
+        for (int64_t k = 0; k < K; ++k) ++hist[key[k]];
+        int64_t at_least[DG_MAX_LEVELS + 2] = {0};
+        for (int d = DG_MAX_LEVELS; d >= 0; --d) at_least[d] = at_least[d + 1] + hist[d];
+        int64_t next[DG_MAX_LEVELS + 1];
+        for (int v = 0; v <= DG_MAX_LEVELS; ++v) next[v] = at_least[v + 1];
+        uint32_t* o = order.data() + (size_t)q * K;
+        for (int64_t k = 0; k < K; ++k) o[next[key[k]]++] = (uint32_t)k;
+        for (int d = 0; d <= DG_MAX_LEVELS; ++d) count[(size_t)q * (DG_MAX_LEVELS + 1) + d] = at_least[d];
+    }
+    DG_TRY(hipMemcpyAsync(c->lts_order, order.data(), order.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    DG_TRY(hipMemcpyAsync(c->lts_level, level, (size_t)K, hipMemcpyHostToDevice, c->stream));
+    DG_TRY(hipMemcpyAsync(c->lts_lrate, rates + (size_t)(n_stages - 1) * K, (size_t)K, hipMemcpyHostToDevice, c->stream));
+    DG_TRY(hipStreamSynchronize(c->stream));
+    c->lts_keys.swap(keys);
+    c->lts_count.swap(count);
+    c->lts_stages = n_stages;
+    c->lts_finest = finest;
+    return 0;
+}
+
+int dg_stage_where(dg_ctx* c, const dg_state* base, const dg_state* input, const dg_stage_plan* plan, dg_state* out,
+                   dg_state* acc) {
+    DG_REQUIRE(c && base && input && plan && out && c->K > 0, "dg_stage_where: null argument or no form");
+    DG_REQUIRE(c->lts_order, "dg_stage_where: no coarse step begun (dg_begin_coarse_step)");
+    DG_REQUIRE(plan->stage >= 0 && plan->stage < c->lts_stages, "dg_stage_where: stage out of range");
+    DG_REQUIRE(plan->finest >= c->lts_finest && plan->finest <= DG_MAX_LEVELS,
+               "dg_stage_where: the plan's finest level is not the coarse step's");
+    DG_REQUIRE(plan->depth >= 0 && plan->depth <= plan->finest, "dg_stage_where: depth out of range");
+    DG_REQUIRE(plan->substep >= 0 && plan->substep < (1 << plan->finest), "dg_stage_where: substep out of range");
+    DG_REQUIRE(!plan->last || acc, "dg_stage_where: an accumulating stage needs the sum buffer");
+    DG_REQUIRE(out != input && out != base && acc != input && acc != base && (!acc || acc != out),
+               "dg_stage_where: out and acc must differ from base and input");
+    const int64_t n_list = c->lts_count[(size_t)plan->stage * (DG_MAX_LEVELS + 1) + plan->depth];
+    if (n_list == 0) return 0;
+    DG_TRY(hipSetDevice(c->device));
+    size_t nt = (size_t)c->K * c->nn;
+    StageParams p;
+    std::memset(&p, 0, sizeof(p));
+    p.bh = base->d; p.bhu = base->d + nt; p.bhv = base->d + 2 * nt;
+    p.ih = input->d; p.ihu = input->d + nt; p.ihv = input->d + 2 * nt;
+    p.oh = out->d; p.ohu = out->d + nt; p.ohv = out->d + 2 * nt;
+    p.bed = c->bed; p.corners = c->corners; p.manning = c->manning;
+    p.links = c->links; p.tags = c->tags; p.affine = c->affine;
+    p.n_elements = c->K;
+    p.a = plan->start; p.b = plan->input; p.c = 0.0; p.eta_e = 0.0;
+    p.g = c->g; p.h_min = c->h_min; p.h_dry = c->h_dry; p.u_max = c->u_max; p.tau = c->tau; p.f0 = c->f0;
+    p.sqrt_g = std::sqrt(c->g);
+    p.flags = c->flags;
+    p.clips = c->clips;
+    p.geo = c->geo;
+    p.fgeo = c->fgeo;
+    p.queue = c->queue;
+    p.queue_count = c->queue_count;
+    p.elist = c->lts_order + (size_t)plan->stage * c->K;
+    p.n_list = n_list;
+    p.level = c->lts_level;
+    p.lrate = c->lts_lrate;
+    p.depth = plan->depth;
+    if (plan->last) { p.ah = acc->d; p.ahu = acc->d + nt; p.ahv = acc->d + 2 * nt; }
+    // element_stage (multirate.rs:169-200) of every level: its substep's start,
+    // its stage time and c = beta dt_l; the tide at that time
+    for (int l = 0; l <= DG_MAX_LEVELS; ++l) {
+        const int lc = std::min(l, plan->finest);
+        const double dt_l = plan->dt / (double)(1ull << lc);
+        const uint64_t q = (uint64_t)plan->substep >> (plan->finest - lc);
+        const double t_k = plan->t + (double)q * dt_l;
+        const double time = t_k + plan->c * dt_l;
+        p.lvl_c[l] = plan->beta * dt_l;
+        p.lvl_eta[l] = tide_elevation(c->tide, c->tide_nodal, time);
+    }
+    const int epb = 64 / c->nn;
+    const unsigned int nblk = (unsigned int)((n_list + epb - 1) / epb);
+    const unsigned int nchunk = (nblk + 1023) / 1024;
+    p.xcd_stride = (nblk + 7) / 8;
+    dim3 grid(8 * p.xcd_stride), sub_grid(std::min(nblk, SUB_GROUPS));
+    switch (c->order) {
+        case 1: hipLaunchKernelGGL(wet_lts_kernel<1>, grid, dim3(64), 0, c->stream, p); break;
+        case 2: hipLaunchKernelGGL(wet_lts_kernel<2>, grid, dim3(64), 0, c->stream, p); break;
+        default: hipLaunchKernelGGL(wet_lts_kernel<3>, grid, dim3(64), 0, c->stream, p); break;
+    }
+    hipLaunchKernelGGL(chunk_count_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, nblk, c->chunk_total);
+    hipLaunchKernelGGL(compact_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, c->queue, nblk,
+                       c->chunk_total, epb, c->sub_list, c->sub_count);
+    switch (c->order) {
+        case 1: hipLaunchKernelGGL((subcell_stage_kernel<1, true>), sub_grid, dim3(64), 0, c->stream, p, c->sub_list, c->sub_count); break;
+        case 2: hipLaunchKernelGGL((subcell_stage_kernel<2, true>), sub_grid, dim3(64), 0, c->stream, p, c->sub_list, c->sub_count); break;
+        default: hipLaunchKernelGGL((subcell_stage_kernel<3, true>), sub_grid, dim3(64), 0, c->stream, p, c->sub_list, c->sub_count); break;
+    }
+    DG_TRY(hipGetLastError());
+    return 0;
+}
+
+int dg_finish_where(dg_ctx* c, dg_state* state, const dg_state* acc, int depth) {
+    DG_REQUIRE(c && state && acc && c->K > 0, "dg_finish_where: null argument or no form");
+    DG_REQUIRE(c->lts_order, "dg_finish_where: no coarse step begun (dg_begin_coarse_step)");
+    DG_REQUIRE(depth >= 0 && depth <= DG_MAX_LEVELS, "dg_finish_where: depth out of range");
+    DG_REQUIRE(state != acc, "dg_finish_where: state and acc must differ");
+    const int64_t n_list = c->lts_count[(size_t)c->lts_stages * (DG_MAX_LEVELS + 1) + depth];
+    if (n_list == 0) return 0;
+    DG_TRY(hipSetDevice(c->device));
+    size_t nt = (size_t)c->K * c->nn;
+    FinishParams p;
+    p.elist = c->lts_order + (size_t)c->lts_stages * c->K;
+    p.n_list = n_list;
+    p.ah = acc->d; p.ahu = acc->d + nt; p.ahv = acc->d + 2 * nt;
+    p.h = state->d; p.hu = state->d + nt; p.hv = state->d + 2 * nt;
+    p.geo = c->geo;
+    p.n_elements = c->K;
+    p.h_dry = c->h_dry; p.u_max = c->u_max;
+    p.wet_dry = (c->flags & DG_WET_DRY) ? 1 : 0;
+    p.clips = c->clips;
+    const int epb = 64 / c->nn;
+    dim3 grid((unsigned int)((n_list + epb - 1) / epb));
+    switch (c->order) {
+        case 1: hipLaunchKernelGGL(finish_kernel<1>, grid, dim3(64), 0, c->stream, p); break;
+        case 2: hipLaunchKernelGGL(finish_kernel<2>, grid, dim3(64), 0, c->stream, p); break;
+        default: hipLaunchKernelGGL(finish_kernel<3>, grid, dim3(64), 0, c->stream, p); break;
+    }
+    DG_TRY(hipGetLastError());
+    return 0;
+}
+
+int dg_depth_flags(dg_ctx* c, const dg_state* s, double d0, double d1, double d2, uint8_t* out) {
+    DG_REQUIRE(c && s && out && c->K > 0, "dg_depth_flags: null argument or no form");
+    DG_TRY(hipSetDevice(c->device));
+    if (!c->depth_flags) DG_TRY(hipMalloc(&c->depth_flags, (size_t)c->K));
+    dim3 grid((unsigned)((c->K + 255) / 256));
+    switch (c->order) {
+        case 1: hipLaunchKernelGGL(depth_flags_kernel<1>, grid, dim3(256), 0, c->stream, s->d, c->K, d0, d1, d2, c->depth_flags); break;
+        case 2: hipLaunchKernelGGL(depth_flags_kernel<2>, grid, dim3(256), 0, c->stream, s->d, c->K, d0, d1, d2, c->depth_flags); break;
+        default: hipLaunchKernelGGL(depth_flags_kernel<3>, grid, dim3(256), 0, c->stream, s->d, c->K, d0, d1, d2, c->depth_flags); break;
+    }
+    DG_TRY(hipGetLastError());
+    DG_TRY(hipMemcpyAsync(out, c->depth_flags, (size_t)c->K, hipMemcpyDeviceToHost, c->stream));
     DG_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }

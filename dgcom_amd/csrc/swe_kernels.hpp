@@ -101,6 +101,19 @@ struct StageParams {
     uint32_t* queue;     // [blocks][EPB]: the block's elements that take the subcell volume term
     uint8_t* queue_count;  // [blocks]: how many
     unsigned int xcd_stride;  // workgroups per XCD share of the wet kernel's grid (0: no remap)
+    // Local time stepping (the kernels' LTS instantiation; multirate.rs:169-200
+    // element_stage): the pass's elements elist[n_list]; an element on level l
+    // takes c = lvl_c[l] and the tide elevation lvl_eta[l] of its own stage time;
+    // with acc (the scheme's last stage) its value is summed, acc <- w out + keep
+    // acc with w = 2^(level - rate) and keep = (level < depth), instead of
+    // post-processed and stored
+    const uint32_t* elist;
+    int64_t n_list;
+    const uint8_t* level;
+    const uint8_t* lrate;  // the last stage's rates
+    int depth;
+    double *ah, *ahu, *ahv;
+    double lvl_c[DG_MAX_LEVELS + 1], lvl_eta[DG_MAX_LEVELS + 1];
 };
 
 // LDS exchange among the lanes of a workgroup that is one wavefront: the
@@ -301,6 +314,68 @@ __device__ __forceinline__ double outer_distance(const double* corners, const ui
     return (T.x1[1] - T.x1[0]) * face_height<N>(cn, nb_affine, nbf, N1 - 1 - fi) / face_height<N>(c, affine, f, fi);
 }
 
+// The wet/dry correction of one element held one node per lane: if any node is
+// below h_dry, Zhang-Shu scaling towards the mass-weighted mean until h >= 0,
+// momentum zeroed if the mean is below h_dry, a negative mean emptied and
+// counted; then the desingularised velocity and the speed cap at every node
+// (wetting_drying.rs:194-213; limiters/swe_2d.rs:104-197). wflag[] must be
+// zero on entry; sh, shu, shv, pw and savg are overwritten.
+template <int N>
+__device__ __forceinline__ void wet_dry_correct(double h_dry, double u_max, const double* geo, int64_t K,
+                                                unsigned long long* clips, bool active, size_t n, double& vh,
+                                                double& vhu, double& vhv, double* sh, double* shu, double* shv,
+                                                double* pw, int* wflag, double* savg) {
+    constexpr int N1 = N + 1, NN = N1 * N1;
+    const GllTables& T = c_gll[N];
+    const int t = threadIdx.x, e = t / NN, i = t - e * NN, eb = e * NN;
+    if (active) {
+        sh[t] = vh; shu[t] = vhu; shv[t] = vhv;
+        if (vh < h_dry) wflag[e] = 1;
+    }
+    lds_sync();
+    const bool needs = active && wflag[e] != 0;
+    if (needs) {
+        // the node masses of an element that is corrected (J read here only)
+        pw[t] = T.w2[i] * geo[GEO_DET * (size_t)K * NN + n];
+    }
+    lds_sync();
+    if (needs && i == 0) {
+        double area = 0.0;
+        for (int q = 0; q < NN; ++q) area += pw[eb + q];
+        double inv_total = 1.0 / area;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, hmin = INFINITY;
+        for (int q = 0; q < NN; ++q) {
+            double w = pw[eb + q];
+            s0 += w * sh[eb + q];
+            s1 += w * shu[eb + q];
+            s2 += w * shv[eb + q];
+        }
+        for (int q = 0; q < NN; ++q) hmin = fmin_(hmin, sh[eb + q]);
+        savg[e * 4 + 0] = s0 * inv_total;
+        savg[e * 4 + 1] = s1 * inv_total;
+        savg[e * 4 + 2] = s2 * inv_total;
+        savg[e * 4 + 3] = hmin;
+    }
+    lds_sync();
+    if (needs) {
+        double ah = savg[e * 4 + 0], ahu = savg[e * 4 + 1], ahv = savg[e * 4 + 2], hmin = savg[e * 4 + 3];
+        if (ah <= 0.0) {
+            vh = 0.0; vhu = 0.0; vhv = 0.0;
+            if (i == 0 && ah < 0.0) atomicAdd(clips, 1ull);
+        } else {
+            if (hmin < 0.0) {
+                double theta = fmax_(0.0, fmin_(1.0, ah / (ah - hmin)));
+                vh = theta * (vh - ah) + ah;
+                vhu = theta * (vhu - ahu) + ahu;
+                vhv = theta * (vhv - ahv) + ahv;
+                vh = fmax_(vh, 0.0);
+            }
+            if (ah < h_dry) { vhu = 0.0; vhv = 0.0; }
+        }
+    }
+    if (active) correct_node(vh, vhu, vhv, h_dry, u_max);
+}
+
 // One workgroup-wavefront of EPB elements through a fused stage: the element
 // RHS (volume term, Riemann fluxes at the 4(N+1) face nodes from the left side
 // of each edge, LIFT, 1/J, Coriolis), then out = a*base + b*input + c*RHS, the
@@ -316,9 +391,10 @@ namespace dgswe {
 
 // VJP: the same forward up to the element RHS, then the reverse (vjp_tail)
 // instead of the post-processing and the stores.
-template <int N, bool SUB, bool VJP = false>
+// LTS: k is an element of the pass's list p.elist, lts_blk the list block.
+template <int N, bool SUB, bool VJP = false, bool LTS = false>
 __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, bool active,
-                                            const VjpParams* vp = nullptr) {
+                                            const VjpParams* vp = nullptr, int64_t lts_blk = 0) {
     constexpr int N1 = N + 1, NN = N1 * N1, EPB = 64 / NN, NF = 4 * N1;
     constexpr int NT = (NF + NN - 1) / NN;  // face tasks per lane
     const GllTables& T = c_gll[N];
@@ -331,6 +407,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
     __shared__ double savg[EPB * 4];
     __shared__ Side rl[2][SUB ? 64 : 1], rr[2][SUB ? 64 : 1];
     __shared__ Cons ifa[2][SUB ? 64 : 1], ifb[2][SUB ? 64 : 1];
+    __shared__ int64_t skid[LTS && !SUB ? EPB : 1];
 
     const int t = threadIdx.x;
     const int e = t / NN;
@@ -354,8 +431,13 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
     double bh0 = 0.0, bhu0 = 0.0, bhv0 = 0.0, mann = 0.0;
     VjpPre vpre = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     bool affine = false;
+    int lv = 0, lr = 0;  // LTS: the element's level and last-stage rate
     if (active) {
         affine = p.affine[k] != 0;
+        if constexpr (LTS) {
+            lv = p.level[k];
+            if (p.ah) lr = p.lrate[k];
+        }
 #pragma unroll
         for (int r = 0; r < NT; ++r) {
             int task = i + r * NN;
@@ -418,6 +500,18 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
         }
     }
     if (t < EPB) { sflag[t] = 0; wflag[t] = 0; }
+    if constexpr (LTS && !SUB) {
+        if (i == 0 && e < EPB) skid[e] = active ? k : -1;
+    }
+    // the element's own stage weight and tide elevation (its level's)
+    double sc = p.c, seta = p.eta_e;
+    if constexpr (LTS) {
+#pragma unroll
+        for (int l = 0; l <= DG_MAX_LEVELS; ++l) {
+            sc = lv == l ? p.lvl_c[l] : sc;
+            seta = lv == l ? p.lvl_eta[l] : seta;
+        }
+    }
     lds_sync();
     const bool subcells_on = (p.flags & DG_SUBCELLS) != 0;
     if (!SUB && active && subcells_on && hin < p.h_dry) sflag[e] = 1;
@@ -427,11 +521,12 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
         // block (in element order) for the subcell kernel
         if (t == 0) {
             int cnt = 0;
+            const int64_t qb = LTS ? lts_blk : k / EPB;
             for (int q = 0; q < EPB; ++q) {
-                const int64_t kq = k - e + q;
-                if (kq < p.n_elements && sflag[q] != 0) p.queue[(size_t)(k / EPB) * EPB + cnt++] = (uint32_t)kq;
+                const int64_t kq = LTS ? skid[LTS ? q : 0] : k - e + q;
+                if ((LTS ? kq >= 0 : kq < p.n_elements) && sflag[q] != 0) p.queue[(size_t)qb * EPB + cnt++] = (uint32_t)kq;
             }
-            p.queue_count[k / EPB] = (uint8_t)cnt;
+            p.queue_count[qb] = (uint8_t)cnt;
         }
         if constexpr (VJP) {
             if (active && i == 0) vp->kind[k] = (uint8_t)(sflag[e] != 0);
@@ -465,7 +560,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             const uint32_t l = lk[r];
             Cons fstar;
             if (l == LINK_BOUNDARY && tg[r] == DG_BC_TIDE) {
-                Cons qb = characteristic(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min, (p.flags & DG_INCOMING_WAVE) != 0);
+                Cons qb = characteristic(q.h, q.hu, q.hv, q.b, seta, nx, ny, p.g, p.h_min, (p.flags & DG_INCOMING_WAVE) != 0);
                 Node nb = make_node(qb.h, qb.hu, qb.hv, q.b, p.h_min);
                 fstar = ec_flux(nb, nb, nx, ny, p.g);
             } else {
@@ -612,7 +707,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             vh = p.a * bh0; vhu = p.a * bhu0; vhv = p.a * bhv0;
         }
         if (p.b != 0.0) { vh += p.b * hin; vhu += p.b * huin; vhv += p.b * hvin; }
-        if (p.c != 0.0) { vh += p.c * rhs.h; vhu += p.c * rhs.hu; vhv += p.c * rhs.hv; }
+        if (sc != 0.0) { vh += sc * rhs.h; vhu += sc * rhs.hu; vhv += sc * rhs.hv; }
         if (friction || wet_dry) {
             if (vh <= 0.0) { vhu = 0.0; vhv = 0.0; }
             else {
@@ -633,61 +728,29 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
                     double ex = p.h_dry / vh - 1.0;
                     rate += ex * ex / p.tau;
                 }
-                double factor = 1.0 / (1.0 + p.c * rate);
+                double factor = 1.0 / (1.0 + sc * rate);
                 vhu *= factor;
                 vhv *= factor;
             }
         }
     }
+    if constexpr (LTS) {
+        if (p.ah) {
+            // The scheme's last stage: summed into the substep's result, which
+            // finish_kernel post-processes (builder.rs:716-728)
+            if (active) {
+                const double w = ldexp(1.0, lv - lr);
+                if (lv >= p.depth) { p.ah[n] = w * vh; p.ahu[n] = w * vhu; p.ahv[n] = w * vhv; }
+                else { p.ah[n] = w * vh + p.ah[n]; p.ahu[n] = w * vhu + p.ahu[n]; p.ahv[n] = w * vhv + p.ahv[n]; }
+            }
+            return;
+        }
+    }
     // Wet/dry correction (wetting_drying.rs:194-213; limiters/swe_2d.rs:104-197)
     if (wet_dry) {
         lds_sync();  // the node states in LDS are reused here
-        if (active) {
-            sh[t] = vh; shu[t] = vhu; shv[t] = vhv;
-            if (vh < p.h_dry) wflag[e] = 1;
-        }
-        lds_sync();
-        const bool needs = active && wflag[e] != 0;
-        if (needs) {
-            // the node masses of an element that is corrected (J read here only)
-            pw[t] = T.w2[i] * p.geo[GEO_DET * (size_t)p.n_elements * NN + n];
-        }
-        lds_sync();
-        if (needs && i == 0) {
-            double area = 0.0;
-            for (int q = 0; q < NN; ++q) area += pw[eb + q];
-            double inv_total = 1.0 / area;
-            double s0 = 0.0, s1 = 0.0, s2 = 0.0, hmin = INFINITY;
-            for (int q = 0; q < NN; ++q) {
-                double w = pw[eb + q];
-                s0 += w * sh[eb + q];
-                s1 += w * shu[eb + q];
-                s2 += w * shv[eb + q];
-            }
-            for (int q = 0; q < NN; ++q) hmin = fmin_(hmin, sh[eb + q]);
-            savg[e * 4 + 0] = s0 * inv_total;
-            savg[e * 4 + 1] = s1 * inv_total;
-            savg[e * 4 + 2] = s2 * inv_total;
-            savg[e * 4 + 3] = hmin;
-        }
-        lds_sync();
-        if (needs) {
-            double ah = savg[e * 4 + 0], ahu = savg[e * 4 + 1], ahv = savg[e * 4 + 2], hmin = savg[e * 4 + 3];
-            if (ah <= 0.0) {
-                vh = 0.0; vhu = 0.0; vhv = 0.0;
-                if (i == 0 && ah < 0.0) atomicAdd(p.clips, 1ull);
-            } else {
-                if (hmin < 0.0) {
-                    double theta = fmax_(0.0, fmin_(1.0, ah / (ah - hmin)));
-                    vh = theta * (vh - ah) + ah;
-                    vhu = theta * (vhu - ahu) + ahu;
-                    vhv = theta * (vhv - ahv) + ahv;
-                    vh = fmax_(vh, 0.0);
-                }
-                if (ah < p.h_dry) { vhu = 0.0; vhv = 0.0; }
-            }
-        }
-        if (active) correct_node(vh, vhu, vhv, p.h_dry, p.u_max);
+        wet_dry_correct<N>(p.h_dry, p.u_max, p.geo, p.n_elements, p.clips, active, n, vh, vhu, vhv, sh, shu, shv, pw,
+                           wflag, savg);
     }
     if (active) { p.oh[n] = vh; p.ohu[n] = vhu; p.ohv[n] = vhv; }
 }
@@ -777,6 +840,21 @@ __global__ __launch_bounds__(64) DG_WET_OCCUPANCY void wet_stage_kernel(StagePar
     stage_block<N, false>(p, k, e < EPB && blk * EPB + e < p.n_elements);
 }
 
+// A pass of local time stepping: the listed elements (p.elist[0..n_list)),
+// EPB per workgroup in list order (LocalTimeStepping::stage_where,
+// builder.rs:652-737); those with a shallow node are queued per block for
+// subcell_stage_kernel<N, true>.
+template <int N>
+__global__ __launch_bounds__(64) DG_WET_OCCUPANCY void wet_lts_kernel(StageParams p) {
+    constexpr int NN = (N + 1) * (N + 1), EPB = 64 / NN;
+    const int64_t blk = xcd_block(blockIdx.x, p.xcd_stride);
+    if (blk * EPB >= p.n_list) return;
+    const int e = threadIdx.x / NN;
+    const bool active = e < EPB && blk * EPB + e < p.n_list;
+    const int64_t k = p.elist[active ? blk * EPB + e : blk * EPB];
+    stage_block<N, false, false, true>(p, k, active, nullptr, blk);
+}
+
 // Compaction of the per-block subcell lists into one list in element order:
 // chunk_count_kernel sums the counts of each chunk of 1024 blocks, then
 // compact_kernel scans them and scatters the element indices.
@@ -845,16 +923,51 @@ __global__ __launch_bounds__(256) void compact_kernel(const uint8_t* counts, con
 #else
 #define DG_SUB_OCCUPANCY __attribute__((amdgpu_waves_per_eu(2)))
 #endif
-template <int N>
+template <int N, bool LTS = false>
 __global__ __launch_bounds__(64) DG_SUB_OCCUPANCY void subcell_stage_kernel(StageParams p, const uint32_t* list, const unsigned int* count_ptr) {
     constexpr int NN = (N + 1) * (N + 1), EPB = 64 / NN;
     const unsigned int count = *count_ptr;
     const int e = threadIdx.x / NN;
     for (size_t first = (size_t)blockIdx.x * EPB; first < count; first += (size_t)gridDim.x * EPB) {
         const bool active = e < EPB && first + e < count;
-        stage_block<N, true>(p, active ? (int64_t)list[first + e] : 0, active);
+        stage_block<N, true, false, LTS>(p, active ? (int64_t)list[first + e] : 0, active);
         lds_sync();
     }
+}
+
+// LocalTimeStepping::finish_where (builder.rs:739-782): the listed elements'
+// state from the substep's sum, then the wet/dry correction; one lane per node.
+struct FinishParams {
+    const uint32_t* elist;
+    int64_t n_list;
+    const double *ah, *ahu, *ahv;
+    double *h, *hu, *hv;
+    const double* geo;
+    int64_t n_elements;
+    double h_dry, u_max;
+    int wet_dry;
+    unsigned long long* clips;
+};
+
+template <int N>
+__global__ __launch_bounds__(64) void finish_kernel(FinishParams p) {
+    constexpr int NN = (N + 1) * (N + 1), EPB = 64 / NN;
+    __shared__ double sh[64], shu[64], shv[64], pw[64], savg[EPB * 4];
+    __shared__ int wflag[EPB];
+    const int64_t blk = blockIdx.x;
+    const int t = threadIdx.x, e = t / NN, i = t - e * NN;
+    const bool active = e < EPB && blk * EPB + e < p.n_list;
+    const int64_t k = active ? (int64_t)p.elist[blk * EPB + e] : 0;
+    const size_t n = (size_t)k * NN + i;
+    double vh = 0.0, vhu = 0.0, vhv = 0.0;
+    if (active) { vh = p.ah[n]; vhu = p.ahu[n]; vhv = p.ahv[n]; }
+    if (t < EPB) wflag[t] = 0;
+    if (p.wet_dry) {
+        lds_sync();
+        wet_dry_correct<N>(p.h_dry, p.u_max, p.geo, p.n_elements, p.clips, active, n, vh, vhu, vhv, sh, shu, shv, pw,
+                           wflag, savg);
+    }
+    if (active) { p.h[n] = vh; p.hu[n] = vhu; p.hv[n] = vhv; }
 }
 
 // The VJP of a stage: wet elements first (the others are listed as in the
@@ -1082,6 +1195,21 @@ __global__ void masks_kernel(const double* h, int64_t K, double h_dry, int subce
         any |= d;
     }
     elem_sub[k] = subcells ? any : 0;
+}
+
+// Bit j of element k: every node of k deeper than d[j] (the line-local elements
+// and the coarse-step bound of local time stepping, builder.rs:569-637)
+template <int N>
+__global__ void depth_flags_kernel(const double* h, int64_t K, double d0, double d1, double d2, uint8_t* out) {
+    constexpr int NN = (N + 1) * (N + 1);
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    bool a0 = true, a1 = true, a2 = true;
+    for (int i = 0; i < NN; ++i) {
+        const double x = h[(size_t)k * NN + i];
+        a0 = a0 && x > d0; a1 = a1 && x > d1; a2 = a2 && x > d2;
+    }
+    out[k] = (uint8_t)((a0 ? 1 : 0) | (a1 ? 2 : 0) | (a2 ? 4 : 0));
 }
 
 __global__ void copy_kernel(const double4* __restrict__ src, double4* __restrict__ dst, size_t n) {

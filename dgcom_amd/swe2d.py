@@ -37,7 +37,7 @@ EXPORTS = (
     "dg_param_grads_read", "dg_rhs_vjp", "dg_stage_vjp", "dg_step_vjp", "dg_set_stations", "dg_sample_stations",
     "dg_tape_create", "dg_tape_destroy", "dg_misfit_forward", "dg_misfit_backward", "dg_tape_stats",
     "dg_revolve_count", "dg_batch_create", "dg_batch_destroy", "dg_batch_upload", "dg_batch_download", "dg_batch_run",
-    "dg_batch_tuning",
+    "dg_batch_tuning", "dg_begin_coarse_step", "dg_stage_where", "dg_finish_where", "dg_depth_flags",
 )
 BATCH_GRAPH, BATCH_TUNE = 1, 2
 
@@ -57,6 +57,12 @@ class TideDesc(ctypes.Structure):
 
 class StageDesc(ctypes.Structure):
     _fields_ = [("a", ctypes.c_double), ("b", ctypes.c_double), ("c", ctypes.c_double), ("time", ctypes.c_double)]
+
+
+class StagePlanDesc(ctypes.Structure):
+    _fields_ = [("t", ctypes.c_double), ("dt", ctypes.c_double), ("finest", ctypes.c_int), ("substep", ctypes.c_int),
+                ("depth", ctypes.c_int), ("stage", ctypes.c_int), ("start", ctypes.c_double),
+                ("input", ctypes.c_double), ("beta", ctypes.c_double), ("c", ctypes.c_double), ("last", ctypes.c_int)]
 
 
 _libs: dict = {}
@@ -128,6 +134,10 @@ def load(fma: bool = False):
     L.dg_batch_download.argtypes = [vp, ctypes.c_int, _dp, _dp, _dp]
     L.dg_batch_run.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]
     L.dg_batch_tuning.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.dg_begin_coarse_step.argtypes = [vp, _u8p, _u8p, ctypes.c_int]
+    L.dg_stage_where.argtypes = [vp, vp, vp, ctypes.POINTER(StagePlanDesc), vp, vp]
+    L.dg_finish_where.argtypes = [vp, vp, vp, ctypes.c_int]
+    L.dg_depth_flags.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u8p]
     _libs[key] = L
     return L
 
@@ -230,6 +240,8 @@ class SWEPhysics2D:
                     tags.ctypes.data_as(_u8p), _p(bed), None if man is None else _p(man), manning_uniform, f0, g,
                     h_min, h_dry, relaxation_time, max_velocity, flags)
         _check(L, L.dg_upload_form(self.ctx, ctypes.byref(form)), "dg_upload_form")
+        self.flags, self.h_dry = flags, h_dry
+        self.links = links
         if tide is not None:
             self.set_tide(tide)
 
@@ -295,6 +307,11 @@ class SWEPhysics2D:
         d = StageDesc(a, b, c, time)
         _check(self.lib, self.lib.dg_stage(self.ctx, base.handle, input.handle, out.handle, ctypes.byref(d)),
                "dg_stage")
+
+    def local_time_stepping(self) -> "LocalTimeStepping":
+        """The element-subset stages of multirate stepping on this device
+        (PhysicsModule::local_time_stepping, traits.rs:61-163)."""
+        return LocalTimeStepping(self)
 
     def step(self, state: DeviceState, t: float, dt: float, scheme: int = SSP_RK3):
         _check(self.lib, self.lib.dg_step(self.ctx, state.handle, t, dt, scheme), "dg_step")
@@ -462,12 +479,70 @@ class Tape:
             pass
 
 
+class LocalTimeStepping:
+    """`LocalTimeStepping` (src/time/multirate.rs:296-391) of a device physics,
+    as `dgcom_amd.multirate.MultirateStepper` drives it: element time steps,
+    the depth flags of `coarse_step_bound`, and the coarse step's passes through
+    `dg_begin_coarse_step` / `dg_stage_where` / `dg_finish_where` (their element
+    lists are the device's: those with a rate, or a level, of at least the
+    pass's depth)."""
+
+    def __init__(self, physics: SWEPhysics2D):
+        self.physics = physics
+        self.lib = physics.lib
+        self.positivity_h_dry = physics.h_dry if physics.flags & WET_DRY else None
+        self.n_stages = 0
+
+    def clone_state(self, x: DeviceState) -> DeviceState:
+        s = DeviceState(self.physics)
+        s.copy_from(x)
+        return s
+
+    def copy_state(self, dst: DeviceState, src: DeviceState):
+        dst.copy_from(src)
+
+    def free_state(self, x: DeviceState):
+        x.free()
+
+    def element_dt_state(self, state: DeviceState, cfl: float, scheme: int = SSP_RK3) -> np.ndarray:
+        return self.physics.element_dt(state, cfl, scheme)
+
+    def depth_flags(self, state: DeviceState, d0: float, d1: float, d2: float) -> np.ndarray:
+        out = np.empty(self.physics.K, dtype=np.uint8)
+        _check(self.lib, self.lib.dg_depth_flags(self.physics.ctx, state.handle, d0, d1, d2,
+                                                 out.ctypes.data_as(_u8p)), "dg_depth_flags")
+        return out
+
+    def begin_coarse_step(self, level, rates):
+        lv = np.ascontiguousarray(level, dtype=np.uint8)
+        rt = np.ascontiguousarray(np.stack([np.asarray(r, dtype=np.uint8) for r in rates]))
+        self.n_stages = rt.shape[0]
+        _check(self.lib, self.lib.dg_begin_coarse_step(self.physics.ctx, lv.ctypes.data_as(_u8p),
+                                                       rt.ctypes.data_as(_u8p), self.n_stages),
+               "dg_begin_coarse_step")
+
+    def stage_where(self, base: DeviceState, input: DeviceState, elements, plan, out: DeviceState, acc):
+        if len(elements) == 0:
+            return
+        w = plan.weights
+        d = StagePlanDesc(plan.t, plan.dt, plan.finest, plan.substep, plan.depth, plan.stage, w.start, w.input,
+                          w.beta, w.c, int(plan.last))
+        _check(self.lib, self.lib.dg_stage_where(self.physics.ctx, base.handle, input.handle, ctypes.byref(d),
+                                                 out.handle, None if acc is None else acc.handle), "dg_stage_where")
+
+    def finish_where(self, state: DeviceState, acc: DeviceState, elements, depth: int):
+        if len(elements) == 0:
+            return
+        _check(self.lib, self.lib.dg_finish_where(self.physics.ctx, state.handle, acc.handle, depth),
+               "dg_finish_where")
+
+
 class Simulation:
     """The dt loop of `Simulation::run_with_callback` over a device physics
     (`src/simulation/runner.rs:264-483`): SSP-capped CFL, landing on `t_end`
     within LANDING_SLACK, a finiteness check per step."""
 
-    LANDING_SLACK = 1e-3
+    LANDING_SLACK = 1e-9   # multirate.rs:588, the runner's (runner.rs:133)
 
     def __init__(self, physics: SWEPhysics2D, cfl: float, scheme: int = SSP_RK3, max_steps=None):
         self.physics = physics

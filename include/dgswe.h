@@ -35,6 +35,10 @@ typedef struct dg_state dg_state;
 /* Boundary kinds of a boundary face (face_tags). */
 enum { DG_BC_WALL = 0, DG_BC_TIDE = 1 };
 
+/* Levels of local time stepping below the coarsest that the device takes
+ * (gpu/src/swe2d.rs:264 MAX_DEVICE_LEVELS). */
+#define DG_MAX_LEVELS 8
+
 /* Flags of dg_form.flags */
 enum { DG_FRICTION = 1, DG_WET_DRY = 2, DG_SUBCELLS = 4,
        /* elevation-only tide data as a wave entering from still water (ElevationOnly::IncomingWave,
@@ -114,6 +118,52 @@ int dg_face_fluxes(dg_ctx* ctx, const dg_state* in, double t, double* out);
 /* LocalTimeStepping::stage_where over all elements (src/physics/builder.rs:652-737),
  * with the post-process of finish_where fused into every stage (one level). */
 int dg_stage(dg_ctx* ctx, const dg_state* base, const dg_state* input, dg_state* out, const dg_stage_desc* stage);
+
+/* ------------------------------------------------------------------------
+ * Local time stepping: the element-subset operations of LocalTimeStepping
+ * (src/time/multirate.rs:296-391) as SWEPhysics2D implements them
+ * (src/physics/builder.rs:546-783) and the reference GPU takes them
+ * (gpu/src/swe2d.rs:4503-4660). The stepper that drives them (levels, stage
+ * rates, the passes of a coarse step: MultirateStepper, multirate.rs:713-1330)
+ * is the caller's; at most DG_MAX_LEVELS levels below the coarsest.
+ * ------------------------------------------------------------------------ */
+
+/* One stage pass of a coarse step (StagePlan, multirate.rs:203-243): the coarse
+ * step (t, dt), its finest level, the finest substep and its depth, the stage
+ * (0-based) with its Shu-Osher weights (ShuOsherStage: start, input, beta, c)
+ * and whether it is the scheme's last stage. Element k of the pass takes
+ * element_stage (multirate.rs:169-200) of its level and, for the last stage,
+ * its rate. */
+typedef struct dg_stage_plan {
+    double t, dt;
+    int finest, substep, depth, stage;
+    double start, input, beta, c;
+    int last;
+} dg_stage_plan;
+
+/* LocalTimeStepping::begin_coarse_step (multirate.rs:356-362): every element's
+ * level level[K] and stage rates rates[n_stages][K] (row-major, rates >= level)
+ * of the coarse step about to be taken; the passes' element lists are made here
+ * (and kept while the levels and rates are the last ones). */
+int dg_begin_coarse_step(dg_ctx* ctx, const uint8_t* level, const uint8_t* rates, int n_stages);
+
+/* LocalTimeStepping::stage_where (multirate.rs:364-382; builder.rs:652-737): for
+ * every element with rates[plan->stage][k] >= plan->depth, its RHS from `input` at
+ * its own stage time, out = start*base + input*input + c*F, the damping over c,
+ * then either acc <- w*out + keep*acc (plan->last; `out` is then not written) or
+ * the wet/dry correction into out. Asynchronous on the context's stream. */
+int dg_stage_where(dg_ctx* ctx, const dg_state* base, const dg_state* input, const dg_stage_plan* plan,
+                   dg_state* out, dg_state* acc);
+
+/* LocalTimeStepping::finish_where (multirate.rs:384-387; builder.rs:739-782): for
+ * every element with level >= depth, state <- acc and the wet/dry correction.
+ * Asynchronous on the context's stream. */
+int dg_finish_where(dg_ctx* ctx, dg_state* state, const dg_state* acc, int depth);
+
+/* Bit j of out[k] (host, K bytes): every node of element k deeper than d_j, for
+ * LocalTimeStepping::line_local and coarse_step_bound (builder.rs:569-637; the
+ * reference GPU's depth_flags, gpu/src/swe2d.rs:2030). */
+int dg_depth_flags(dg_ctx* ctx, const dg_state* state, double d0, double d1, double d2, uint8_t* out);
 
 /* LocalTimeStepping::element_dt per element (src/solver/rhs/swe_2d.rs:2018-2115) into
  * the host array out[K], and PhysicsModule::compute_dt_ssp (builder.rs:464-485). */

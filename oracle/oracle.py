@@ -110,6 +110,10 @@ def _bind(L):
         L.oc_tlm_run.restype = ctypes.c_double
         L.oc_run_dts.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, ctypes.c_int]
         L.oc_run_dts.restype = ctypes.c_double
+        L.oc_stage_where.argtypes = ([ctypes.c_void_p] + [_dp] * 6 + [_u32p, ctypes.c_int64, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+                                     + [ctypes.c_double] * 4 + [ctypes.c_int, _u8p, _u8p] + [_dp] * 6)
+        L.oc_finish_where.argtypes = [ctypes.c_void_p] + [_dp] * 6 + [_u32p, ctypes.c_int64]
     return L
 
 
@@ -148,6 +152,7 @@ class Oracle:
                              g, h_min, h_dry, f0, surface, flags)
         if not self.h:
             raise RuntimeError(L.oc_last_error().decode())
+        self.positivity_h_dry = h_dry if wet_dry else None
         if tide is not None:
             self.set_tide(tide)
 
@@ -191,6 +196,48 @@ class Oracle:
         self._L.oc_stage(self.h, *(_p(x) for x in base), *(_p(x) for x in inp), *(_p(o) for o in out), a, b, c, time,
                        int(last))
         return out
+
+    # --- LocalTimeStepping (multirate.rs:296-391) over host states [h, hu, hv] ---
+    def clone_state(self, x):
+        return [np.array(a, dtype=np.float64, copy=True) for a in x]
+
+    def copy_state(self, dst, src):
+        for d, s_ in zip(dst, src):
+            d[...] = s_
+
+    def begin_coarse_step(self, level, rates):
+        self._level = np.ascontiguousarray(level, dtype=np.uint8)
+        self._rates = [np.ascontiguousarray(r, dtype=np.uint8) for r in rates]
+
+    def stage_where(self, base, inp, elements, plan, out, acc):
+        """builder.rs:652-737 on the listed elements; element k takes plan's
+        element_stage of its level and (last stage) rate."""
+        el = np.ascontiguousarray(elements, dtype=np.uint32)
+        if el.size == 0:
+            return
+        w = plan.weights
+        a3 = acc if acc is not None else [None, None, None]
+        self._L.oc_stage_where(self.h, *(_p(x) for x in base), *(_p(x) for x in inp), el.ctypes.data_as(_u32p),
+                               el.size, plan.t, plan.dt, plan.finest, plan.substep, plan.depth, w.start, w.input,
+                               w.beta, w.c, int(plan.last), self._level.ctypes.data_as(_u8p),
+                               self._rates[plan.stage].ctypes.data_as(_u8p), *(_p(o) for o in out),
+                               *((_p(x) if x is not None else None) for x in a3))
+
+    def finish_where(self, state, acc, elements, depth):
+        """builder.rs:739-782 on the listed elements."""
+        el = np.ascontiguousarray(elements, dtype=np.uint32)
+        if el.size == 0:
+            return
+        self._L.oc_finish_where(self.h, *(_p(x) for x in state), *(_p(x) for x in acc), el.ctypes.data_as(_u32p),
+                                el.size)
+
+    def element_dt_state(self, x, cfl, scheme=0):
+        return self.element_dt(x[0], x[1], x[2], cfl, scheme)
+
+    def depth_flags(self, x, d0, d1, d2):
+        h = np.asarray(x[0]).reshape(self.K, -1)
+        return ((h > d0).all(1).astype(np.uint8) | ((h > d1).all(1).astype(np.uint8) << 1)
+                | ((h > d2).all(1).astype(np.uint8) << 2))
 
     def step(self, h, hu, hv, t, dt, scheme=0):
         """One SSP step in place."""

@@ -109,6 +109,22 @@ void oc_stage(void* p, const double* bh, const double* bhu, const double* bhv, c
     ((Handle*)p)->s->stage(bh, bhu, bhv, ih, ihu, ihv, oh, ohu, ohv, a, b, c, time, last != 0);
 }
 
+// LocalTimeStepping::stage_where / finish_where over listed elements (builder.rs:652-782)
+void oc_stage_where(void* p, const double* bh, const double* bhu, const double* bhv, const double* ih,
+                    const double* ihu, const double* ihv, const uint32_t* elements, int64_t n_list, double t,
+                    double dt, int finest, int substep, int depth, double start, double input, double beta, double c,
+                    int last, const uint8_t* level, const uint8_t* rate, double* oh, double* ohu, double* ohv,
+                    double* ah, double* ahu, double* ahv) {
+    oracle::ShuOsher st{start, input, beta, c};
+    ((Handle*)p)->s->stage_where(bh, bhu, bhv, ih, ihu, ihv, elements, n_list, t, dt, finest, substep, depth, st,
+                                 last != 0, level, rate, oh, ohu, ohv, ah, ahu, ahv);
+}
+
+void oc_finish_where(void* p, double* h, double* hu, double* hv, const double* ah, const double* ahu,
+                     const double* ahv, const uint32_t* elements, int64_t n_list) {
+    ((Handle*)p)->s->finish_where(h, hu, hv, ah, ahu, ahv, elements, n_list);
+}
+
 void oc_step(void* p, double* h, double* hu, double* hv, double t, double dt, int scheme) {
     Handle* hd = (Handle*)p;
     hd->s->step(h, hu, hv, t, dt, scheme, hd->work);

@@ -531,6 +531,90 @@ struct Solver {
         clips += local_clips;
     }
 
+    // multirate.rs:169-200 element_stage: the stage of an element on level `level` whose stage RHS
+    // follows level `rate`, in finest substep `substep` (of 2^finest) of depth `depth` of the coarse
+    // step of dt from t
+    struct ElementStage { double time, a, b, c; bool accumulate; double w, keep; };
+    static ElementStage element_stage(double t, double dt, int finest, int substep, int depth, const ShuOsher& st,
+                                      bool last, int level, int rate) {
+        double dt_l = dt / (double)(1ull << level);
+        uint64_t q = (uint64_t)substep >> (finest - level);
+        double t_k = t + (double)q * dt_l;
+        ElementStage e;
+        e.time = t_k + st.c * dt_l;
+        e.a = st.start;
+        e.b = st.input;
+        e.c = st.beta * dt_l;
+        e.accumulate = last;
+        e.w = last ? 1.0 / (double)(1ull << (rate - level)) : 0.0;
+        e.keep = last ? (level >= depth ? 0.0 : 1.0) : 0.0;
+        return e;
+    }
+
+    // builder.rs:652-737 stage_where: the listed elements' RHS from `input` at their own stage times,
+    // out = a*base + b*input + c*F, the damping over c, then acc <- w*out + keep*acc (the last stage)
+    // or the wet/dry post-process
+    void stage_where(const T* bh, const T* bhu, const T* bhv, const T* ih, const T* ihu, const T* ihv,
+                     const uint32_t* elements, int64_t n_list, double t, double dt, int finest, int substep,
+                     int depth, const ShuOsher& st, bool last, const uint8_t* level, const uint8_t* rate,
+                     T* oh, T* ohu, T* ohv, T* ah, T* ahu, T* ahv) {
+        face_pass(ih, ihu, ihv);
+        const int nn = ops.nn;
+        uint64_t local_clips = 0;
+#pragma omp parallel reduction(+ : local_clips)
+        {
+            std::vector<NodeState<T>> nodes(nn);
+            std::vector<State<T>> racc(nn);
+#pragma omp for schedule(static)
+            for (int64_t li = 0; li < n_list; ++li) {
+                const int64_t k = elements[li];
+                ElementStage es = element_stage(t, dt, finest, substep, depth, st, last, level[k], rate[k]);
+                element_rhs(ih, ihu, ihv, k, es.time, oh, ohu, ohv, nodes, racc);
+                size_t n0 = (size_t)k * nn;
+                const T* base[3] = {bh + n0, bhu + n0, bhv + n0};
+                const T* inp[3] = {ih + n0, ihu + n0, ihv + n0};
+                T* out[3] = {oh + n0, ohu + n0, ohv + n0};
+                for (int var = 0; var < 3; ++var)
+                    for (int i = 0; i < nn; ++i) {
+                        T v = (es.a != 0.0) ? T(es.a) * base[var][i] : T(0.0);
+                        if (es.b != 0.0) v += T(es.b) * inp[var][i];
+                        if (es.c != 0.0) v += T(es.c) * out[var][i];
+                        out[var][i] = v;
+                    }
+                if (cfg.friction || cfg.wet_dry)
+                    for (int i = 0; i < nn; ++i)
+                        damp_node(k, out[0][i], out[1][i], out[2][i], inp[0][i], inp[1][i], inp[2][i], es.c);
+                if (es.accumulate) {
+                    T* sum[3] = {ah + n0, ahu + n0, ahv + n0};
+                    for (int var = 0; var < 3; ++var)
+                        for (int i = 0; i < nn; ++i)
+                            sum[var][i] = es.keep != 0.0 ? T(es.w) * out[var][i] + T(es.keep) * sum[var][i]
+                                                         : T(es.w) * out[var][i];
+                } else if (cfg.wet_dry) {
+                    local_clips += wet_dry_element(k, out[0], out[1], out[2]) ? 1 : 0;
+                }
+            }
+        }
+        clips += local_clips;
+    }
+
+    // builder.rs:739-782 finish_where: the listed elements' state from acc, then the post-process
+    void finish_where(T* h, T* hu, T* hv, const T* ah, const T* ahu, const T* ahv, const uint32_t* elements,
+                      int64_t n_list) {
+        const int nn = ops.nn;
+        uint64_t local_clips = 0;
+#pragma omp parallel for schedule(static) reduction(+ : local_clips)
+        for (int64_t li = 0; li < n_list; ++li) {
+            const int64_t k = elements[li];
+            size_t n0 = (size_t)k * nn;
+            std::copy(ah + n0, ah + n0 + nn, h + n0);
+            std::copy(ahu + n0, ahu + n0 + nn, hu + n0);
+            std::copy(ahv + n0, ahv + n0 + nn, hv + n0);
+            if (cfg.wet_dry) local_clips += wet_dry_element(k, h + n0, hu + n0, hv + n0) ? 1 : 0;
+        }
+        clips += local_clips;
+    }
+
     // multirate.rs:1285-1311 with one level: stage values in their own buffers, the last into acc
     void step(T* h, T* hu, T* hv, double t, double dt, int scheme, std::vector<T>& work) {
         ShuOsher st[4];

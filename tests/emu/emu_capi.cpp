@@ -81,6 +81,84 @@ unsigned long long emu_stage(int order, int64_t K, const double* corners, const 
     return clips;
 }
 
+// A pass of local time stepping over the listed elements (wet_lts_kernel, the
+// list compaction, the subcell kernel's LTS instantiation), each element with its
+// level's c and tide elevation; acc (if not null) the last stage's sum.
+unsigned long long emu_stage_where(int order, int64_t K, const double* corners, const uint32_t* links,
+                                   const uint8_t* tags, const uint8_t* affine, const double* bed,
+                                   const double* manning, const double* base, const double* input, double* out,
+                                   double* acc, const uint32_t* elist, int64_t n_list, const uint8_t* level,
+                                   const uint8_t* lrate, int depth, double a, double b, const double* lvl_c,
+                                   const double* lvl_eta, double g, double h_min, double h_dry, double u_max,
+                                   double tau, double f0, int flags) {
+    size_t nn = (size_t)(order + 1) * (order + 1), nt = (size_t)K * nn;
+    unsigned long long clips = 0;
+    StageParams p;
+    std::memset(&p, 0, sizeof(p));
+    p.bh = base; p.bhu = base + nt; p.bhv = base + 2 * nt;
+    p.ih = input; p.ihu = input + nt; p.ihv = input + 2 * nt;
+    p.oh = out; p.ohu = out + nt; p.ohv = out + 2 * nt;
+    p.bed = bed; p.corners = corners; p.manning = manning; p.links = links; p.tags = tags; p.affine = affine;
+    p.n_elements = K; p.a = a; p.b = b;
+    p.g = g; p.h_min = h_min; p.h_dry = h_dry; p.u_max = u_max; p.tau = tau; p.f0 = f0; p.sqrt_g = std::sqrt(g);
+    p.flags = flags; p.clips = &clips;
+    p.elist = elist; p.n_list = n_list; p.level = level; p.lrate = lrate; p.depth = depth;
+    if (acc) { p.ah = acc; p.ahu = acc + nt; p.ahv = acc + 2 * nt; }
+    for (int l = 0; l <= DG_MAX_LEVELS; ++l) { p.lvl_c[l] = lvl_c[l]; p.lvl_eta[l] = lvl_eta[l]; }
+    std::vector<double> geo, fgeo;
+    run_geometry(order, K, p, geo, fgeo);
+    int epb = 64 / (int)nn;
+    unsigned nblk = (unsigned)((n_list + epb - 1) / epb);
+    unsigned nchunk = (nblk + 1023) / 1024;
+    std::vector<uint32_t> queue((size_t)nblk * epb), list((size_t)K);
+    std::vector<uint8_t> qcount(nblk);
+    std::vector<unsigned int> ctot(nchunk);
+    unsigned int lcount = 0;
+    p.queue = queue.data();
+    p.queue_count = qcount.data();
+    p.xcd_stride = (nblk + 7) / 8;
+    unsigned grid = 8 * p.xcd_stride, sub_grid = std::min(nblk, 5u);
+    switch (order) {
+        case 1: emu_launch(grid, 64, [&] { wet_lts_kernel<1>(p); }); break;
+        case 2: emu_launch(grid, 64, [&] { wet_lts_kernel<2>(p); }); break;
+        default: emu_launch(grid, 64, [&] { wet_lts_kernel<3>(p); }); break;
+    }
+    emu_launch(nchunk, 256, [&] { chunk_count_kernel(qcount.data(), nblk, ctot.data()); });
+    emu_launch(nchunk, 256, [&] { compact_kernel(qcount.data(), queue.data(), nblk, ctot.data(), epb, list.data(), &lcount); });
+    switch (order) {
+        case 1: emu_launch(sub_grid, 64, [&] { subcell_stage_kernel<1, true>(p, list.data(), &lcount); }); break;
+        case 2: emu_launch(sub_grid, 64, [&] { subcell_stage_kernel<2, true>(p, list.data(), &lcount); }); break;
+        default: emu_launch(sub_grid, 64, [&] { subcell_stage_kernel<3, true>(p, list.data(), &lcount); }); break;
+    }
+    return clips;
+}
+
+unsigned long long emu_finish_where(int order, int64_t K, const double* corners, const uint8_t* affine,
+                                    double* state, const double* acc, const uint32_t* elist, int64_t n_list,
+                                    double h_dry, double u_max, int wet_dry) {
+    size_t nn = (size_t)(order + 1) * (order + 1), nt = (size_t)K * nn;
+    unsigned long long clips = 0;
+    StageParams sp;
+    std::memset(&sp, 0, sizeof(sp));
+    std::vector<double> bed(nt, 0.0), geo, fgeo;
+    std::vector<uint32_t> links((size_t)4 * K, 0xFFFFFFFFu);
+    sp.corners = corners; sp.bed = bed.data(); sp.affine = affine; sp.links = links.data();
+    run_geometry(order, K, sp, geo, fgeo);
+    FinishParams p;
+    p.elist = elist; p.n_list = n_list;
+    p.ah = acc; p.ahu = acc + nt; p.ahv = acc + 2 * nt;
+    p.h = state; p.hu = state + nt; p.hv = state + 2 * nt;
+    p.geo = geo.data(); p.n_elements = K; p.h_dry = h_dry; p.u_max = u_max; p.wet_dry = wet_dry; p.clips = &clips;
+    int epb = 64 / (int)nn;
+    unsigned grid = (unsigned)((n_list + epb - 1) / epb);
+    switch (order) {
+        case 1: emu_launch(grid, 64, [&] { finish_kernel<1>(p); }); break;
+        case 2: emu_launch(grid, 64, [&] { finish_kernel<2>(p); }); break;
+        default: emu_launch(grid, 64, [&] { finish_kernel<3>(p); }); break;
+    }
+    return clips;
+}
+
 // The stage VJP: gbase (if not null) and ginput are added to; gbed and gman too.
 // The forward stage runs first and saves its RHS, as the library does.
 void emu_stage_vjp(int order, int64_t K, const double* corners, const uint32_t* links, const uint8_t* tags,

@@ -9,6 +9,7 @@ import numpy as np
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "emu")
 _dp = ctypes.POINTER(ctypes.c_double)
 _lib = None
+M_LEVELS = 8   # DG_MAX_LEVELS of include/dgswe.h
 
 
 def lib():
@@ -26,6 +27,14 @@ def lib():
         L.emu_stage_vjp.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, ctypes.c_void_p, ctypes.c_void_p,
                                     ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp] + \
             [ctypes.c_double] * 10 + [ctypes.c_int, ctypes.c_int]
+        vp = ctypes.c_void_p
+        L.emu_stage_where.restype = ctypes.c_ulonglong
+        L.emu_stage_where.argtypes = ([ctypes.c_int, ctypes.c_int64, _dp, vp, vp, vp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                       vp, ctypes.c_int64, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                       _dp, _dp] + [ctypes.c_double] * 6 + [ctypes.c_int])
+        L.emu_finish_where.restype = ctypes.c_ulonglong
+        L.emu_finish_where.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, vp, _dp, _dp, vp, ctypes.c_int64,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.emu_init()
         _lib = L
     return _lib
@@ -93,6 +102,51 @@ class EmulatedKernels:
                           self.affine.ctypes.data, cfl, self.case.g, self.case.h_min, self.case.h_dry, pcfl,
                           max(pcfl, 0.9 * lin), max(pcfl, 0.9 * sub), int(bool(self.flags & 2)), self._p(out))
         return out, dt
+
+    def stage_where(self, base, inp, out, acc, elements, plan):
+        """A pass of local time stepping (wet_lts_kernel and the subcell kernel's
+        LTS instantiation) with each level's c and tide elevation from
+        element_stage; out and acc (lists of 3 arrays) are updated in place."""
+        from dgcom_amd import multirate as M
+        c_ = self.case
+        B = np.concatenate(base)
+        I = np.concatenate(inp)
+        O = np.concatenate(out)
+        A = np.concatenate(acc) if acc is not None else None
+        lc = np.zeros(M_LEVELS + 1)
+        le = np.zeros(M_LEVELS + 1)
+        for l in range(M_LEVELS + 1):
+            lv = min(l, plan.finest)
+            es = M.element_stage((plan.t, plan.dt), plan.finest, plan.substep, plan.depth, plan.weights, False, lv, lv)
+            lc[l] = es.c
+            le[l] = c_.tide.elevation(es.time)
+        el = np.ascontiguousarray(elements, dtype=np.uint32)
+        level = np.ascontiguousarray(plan.level, dtype=np.uint8)
+        lrate = np.ascontiguousarray(plan.rate, dtype=np.uint8)
+        w = plan.weights
+        clips = lib().emu_stage_where(self.order, self.K, self._p(self.corners), self.links.ctypes.data,
+                                      self.tags.ctypes.data, self.affine.ctypes.data, self._p(self.bed),
+                                      self._p(self.manning), self._p(B), self._p(I), self._p(O),
+                                      None if A is None else self._p(A), el.ctypes.data, el.size, level.ctypes.data,
+                                      lrate.ctypes.data, plan.depth, w.start, w.input, self._p(lc), self._p(le), c_.g,
+                                      c_.h_min, c_.h_dry, 20.0, 1.0, c_.f0, self.flags)
+        for dst, src in zip(out, np.split(O, 3)):
+            dst[...] = src
+        if acc is not None:
+            for dst, src in zip(acc, np.split(A, 3)):
+                dst[...] = src
+        return clips
+
+    def finish_where(self, state, acc, elements):
+        S = np.concatenate(state)
+        A = np.concatenate(acc)
+        el = np.ascontiguousarray(elements, dtype=np.uint32)
+        clips = lib().emu_finish_where(self.order, self.K, self._p(self.corners), self.affine.ctypes.data, self._p(S),
+                                       self._p(A), el.ctypes.data, el.size, self.case.h_dry, 20.0,
+                                       int(bool(self.flags & 2)))
+        for dst, src in zip(state, np.split(S, 3)):
+            dst[...] = src
+        return clips
 
     def stage_vjp(self, base, inp, w, a, b, c, time, rhs_only=False):
         """Cotangents of (base, input, bed, Manning n) of a stage for the output cotangent w."""

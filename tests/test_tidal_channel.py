@@ -18,7 +18,7 @@ import pytest
 from dgcom_amd import cases
 import oracle as O
 
-T_END, INTERVAL, LANDING_SLACK = 25 * 3600.0, 1800.0, 1e-3
+T_END, INTERVAL, LANDING_SLACK = 25 * 3600.0, 1800.0, 1e-9   # multirate.rs:588
 
 
 def run_oracle(c):
