@@ -240,6 +240,56 @@ def ensemble_bench(n_members, n_steps, k_lo, k_hi):
             "setup_s": setup}
 
 
+def lts_bench(nx, ny, t_end, max_levels):
+    """Local time stepping against global steps over the same model time: the
+    reference's tidal channel refined to nx x ny elements (N = 2, SSP-RK(4,3) at
+    its default CFL, wet/dry flats, M2 + S2 with nodal corrections), from still
+    water for t_end seconds at up to `max_levels` levels. Global steps: the
+    runner's loop (one dt and one fused step per step, landing on t_end). Local:
+    the library's multirate stepper (levels and stage rates from the device's
+    element time steps each coarse step, then its passes). Both from the same
+    state, warm."""
+    c = cases.tidal_channel(nx=nx, ny=ny)
+    phys = swe2d.SWEPhysics2D.from_case(c)
+    s = phys.state(c.h, c.hu, c.hv)
+    sim = swe2d.Simulation(phys, c.cfl, c.scheme)
+    sim.run(s, 0.0, min(60.0, t_end))                      # warm-up (library, caches)
+    s.upload(c.h, c.hu, c.hv)
+    phys.sync()
+    w0 = time.perf_counter()
+    _, n_glob = sim.run(s, 0.0, t_end)
+    phys.sync()
+    glob = time.perf_counter() - w0
+    h_glob = s.download()[0]
+    s.upload(c.h, c.hu, c.hv)
+    warm = swe2d.Multirate(phys, max_levels, c.scheme)
+    warm.run(s, 0.0, min(60.0, t_end), c.cfl)                        # warm-up
+    warm.close()
+    mr = swe2d.Multirate(phys, max_levels, c.scheme)
+    s.upload(c.h, c.hu, c.hv)
+    phys.sync()
+    w0 = time.perf_counter()
+    _, n_lts = mr.run(s, 0.0, t_end, c.cfl)
+    phys.sync()
+    lts = time.perf_counter() - w0
+    h_lts = s.download()[0]
+    stats = mr.stats()
+    mr.close()
+    s.free()
+    m = cases.node_masses(c)
+    glob_evals = 4 * c.n_elements * n_glob
+    phys.close()
+    return {"workload": f"tidal channel refined to {c.n_elements} elements ({nx}x{ny}, N=2), {t_end:.0f} s from "
+                        f"still water, SSP-RK(4,3) CFL {c.cfl:.3f}, up to {max_levels} levels",
+            "global_s": glob, "global_steps": n_glob, "lts_s": lts, "coarse_steps": n_lts,
+            "finest_level": stats["finest_level"], "speedup": glob / lts,
+            "global_element_evaluations": glob_evals, "lts_element_evaluations": stats["element_evaluations"],
+            "evaluations_ratio": glob_evals / stats["element_evaluations"],
+            "lts_element_evaluations_per_s": stats["element_evaluations"] / lts,
+            "global_element_evaluations_per_s": glob_evals / glob,
+            "volume_difference_rel": abs(float(m @ h_lts) - float(m @ h_glob)) / float(m @ h_glob)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -260,6 +310,10 @@ def main():
     ap.add_argument("--adjoint-ensemble-members", type=int, default=0,
                     help="members of the inversion ensemble's gradient timing (0: not run)")
     ap.add_argument("--adjoint-ensemble-steps", type=int, default=1000)
+    ap.add_argument("--lts", default="800x80,2000x200", help="refined tidal channels for the local-time-stepping "
+                    "leg, comma-separated nx x ny ('' for none)")
+    ap.add_argument("--lts-seconds", type=float, default=900.0)
+    ap.add_argument("--lts-levels", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the time step runs on a single device here "
@@ -342,6 +396,9 @@ def main():
         out["ensemble"] = ensemble_bench(args.ensemble_members, args.ensemble_steps, 20_000, 400_000)
     if args.adjoint_ensemble_members > 0:
         out["adjoint_ensemble"] = adjoint_ensemble_bench(args.adjoint_ensemble_members, args.adjoint_ensemble_steps, 20)
+    if args.lts:
+        out["lts"] = [lts_bench(*map(int, spec.split("x")), args.lts_seconds, args.lts_levels)
+                      for spec in args.lts.split(",")]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -23,6 +23,7 @@
 #include "../../include/dgswe.h"
 #include "swe_kernels.hpp"
 #include "swe_batch.hpp"
+#include "multirate.hpp"
 
 using namespace dgswe;
 
@@ -1121,7 +1122,9 @@ int dg_stage_where(dg_ctx* c, const dg_state* base, const dg_state* input, const
     p.fgeo = c->fgeo;
     p.queue = c->queue;
     p.queue_count = c->queue_count;
-    p.elist = c->lts_order + (size_t)plan->stage * c->K;
+    // a pass over every element goes in mesh order (no list: contiguous blocks,
+    // each XCD's share of the mesh in its own L2)
+    p.elist = n_list == c->K ? nullptr : c->lts_order + (size_t)plan->stage * c->K;
     p.n_list = n_list;
     p.level = c->lts_level;
     p.lrate = c->lts_lrate;
@@ -1170,7 +1173,7 @@ int dg_finish_where(dg_ctx* c, dg_state* state, const dg_state* acc, int depth) 
     DG_TRY(hipSetDevice(c->device));
     size_t nt = (size_t)c->K * c->nn;
     FinishParams p;
-    p.elist = c->lts_order + (size_t)c->lts_stages * c->K;
+    p.elist = n_list == c->K ? nullptr : c->lts_order + (size_t)c->lts_stages * c->K;
     p.n_list = n_list;
     p.ah = acc->d; p.ahu = acc->d + nt; p.ahv = acc->d + 2 * nt;
     p.h = state->d; p.hu = state->d + nt; p.hv = state->d + 2 * nt;
@@ -1203,6 +1206,152 @@ int dg_depth_flags(dg_ctx* c, const dg_state* s, double d0, double d1, double d2
     DG_TRY(hipGetLastError());
     DG_TRY(hipMemcpyAsync(out, c->depth_flags, (size_t)c->K, hipMemcpyDeviceToHost, c->stream));
     DG_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// The multirate stepper (MultirateStepper, multirate.rs:713-1330) over the
+// passes above, on the host side of the library
+// ---------------------------------------------------------------------------
+struct dg_multirate {
+    dg_ctx* c = nullptr;
+    int scheme = 0;
+    dgswe::multirate::Stage st[4];
+    dgswe::multirate::Assigner as;
+    std::vector<double> edt;
+    std::vector<uint8_t> flags;
+    dg_state* values[3] = {nullptr, nullptr, nullptr};
+    dg_state *last = nullptr, *acc = nullptr;
+    dg_multirate_stats stats{};
+};
+
+static int mr_free_states(dg_multirate* m) {
+    dg_state* all[5] = {m->values[0], m->values[1], m->values[2], m->last, m->acc};
+    for (dg_state* x : all)
+        if (x) { if (int r = dg_state_free(m->c, x)) return r; }
+    m->values[0] = m->values[1] = m->values[2] = m->last = m->acc = nullptr;
+    return 0;
+}
+
+int dg_multirate_create(dg_ctx* c, int max_levels, int scheme, int line_rates, dg_multirate** out) {
+    DG_REQUIRE(c && out && c->K > 0, "dg_multirate_create: null argument or no form");
+    DG_REQUIRE(max_levels >= 0 && max_levels <= DG_MAX_LEVELS, "dg_multirate_create: 0 to DG_MAX_LEVELS levels");
+    DG_REQUIRE(scheme == DG_SSP_RK3 || scheme == DG_SSP_RK43, "dg_multirate_create: unknown scheme");
+    DG_TRY(hipSetDevice(c->device));
+    const size_t K = (size_t)c->K;
+    std::vector<uint32_t> links(4 * K);
+    DG_TRY(hipMemcpy(links.data(), c->links, 4 * K * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    dg_multirate* m = new dg_multirate;
+    m->c = c;
+    m->scheme = scheme;
+    m->as.max_levels = max_levels;
+    m->as.line_rates = line_rates != 0;
+    m->as.n1 = c->order + 1;
+    m->as.n_stages = dgswe::multirate::stages(scheme == DG_SSP_RK3 ? 0 : 1, m->st);
+    m->as.init(links.data(), K);
+    m->edt.resize(K);
+    m->flags.resize(K);
+    *out = m;
+    return 0;
+}
+
+int dg_multirate_destroy(dg_multirate* m) {
+    if (!m) return 0;
+    if (int r = mr_free_states(m)) return r;
+    delete m;
+    return 0;
+}
+
+int dg_multirate_assign_levels(dg_multirate* m, const dg_state* s, double cfl, double dt_cap, double landing,
+                               double* dt_out, double* stable_out) {
+    namespace mr = dgswe::multirate;
+    DG_REQUIRE(m && s && dt_out, "dg_multirate_assign_levels: null argument");
+    dg_ctx* c = m->c;
+    const size_t K = (size_t)c->K;
+    DG_TRY(hipSetDevice(c->device));
+    // LocalTimeStepping::element_dt on the device, and the depth flags of the
+    // coarse-step bound and the line-local elements
+    double dtmin = 0.0;
+    if (int r = launch_dt(c, s->d, cfl, m->scheme, true, &dtmin)) return r;
+    DG_TRY(hipMemcpy(m->edt.data(), c->element_dt, K * sizeof(double), hipMemcpyDeviceToHost));
+    if (int r = dg_depth_flags(c, s, mr::SUBCELL_CAP_DEPTH_FACTOR * c->h_dry, c->h_dry, c->h_dry + mr::LINE_LOCAL_HEADROOM,
+                               m->flags.data())) return r;
+    *dt_out = m->as.assign(m->edt, m->flags.data(), (c->flags & DG_WET_DRY) != 0, dt_cap, landing);
+    if (stable_out) *stable_out = m->as.stable;
+    return 0;
+}
+
+int dg_multirate_step(dg_multirate* m, dg_state* s, double t, double dt) {
+    DG_REQUIRE(m && s, "dg_multirate_step: null argument");
+    DG_REQUIRE(!m->as.level.empty(), "dg_multirate_step: no levels assigned (dg_multirate_assign_levels)");
+    dg_ctx* c = m->c;
+    const int ns = m->as.n_stages;
+    DG_TRY(hipSetDevice(c->device));
+    if (!m->acc) {
+        for (int q = 0; q < ns - 1; ++q)
+            if (int r = dg_state_alloc(c, &m->values[q])) return r;
+        if (int r = dg_state_alloc(c, &m->last)) return r;
+        if (int r = dg_state_alloc(c, &m->acc)) return r;
+    }
+    if (int r = dg_begin_coarse_step(c, m->as.level.data(), m->as.rates.data(), ns)) return r;
+    const int finest = m->as.finest;
+    for (int p = 0; p < (1 << finest); ++p) {
+        const int d = dgswe::multirate::depth(p, finest);
+        for (int q = 0; q < ns; ++q) {
+            const bool last = q + 1 == ns;
+            dg_stage_plan plan;
+            plan.t = t; plan.dt = dt; plan.finest = finest; plan.substep = p; plan.depth = d; plan.stage = q;
+            plan.start = m->st[q].start; plan.input = m->st[q].input; plan.beta = m->st[q].beta; plan.c = m->st[q].c;
+            plan.last = last ? 1 : 0;
+            const dg_state* input = q == 0 ? s : m->values[q - 1];
+            if (int r = dg_stage_where(c, s, input, &plan, last ? m->last : m->values[q], last ? m->acc : nullptr)) return r;
+        }
+        if (int r = dg_finish_where(c, s, m->acc, dgswe::multirate::depth(p + 1, finest))) return r;
+    }
+    m->stats.steps += 1;
+    m->stats.element_evaluations += m->as.evals_per_step;
+    m->stats.global_evaluations += (int64_t)ns * c->K * (int64_t(1) << finest);
+    m->stats.finest_level = std::max(m->stats.finest_level, finest);
+    m->stats.model_seconds += dt;
+    return 0;
+}
+
+int dg_multirate_levels(dg_multirate* m, uint8_t* level, uint8_t* rates, int* finest) {
+    DG_REQUIRE(m, "dg_multirate_levels: null argument");
+    if (level && !m->as.level.empty()) std::memcpy(level, m->as.level.data(), m->as.level.size());
+    if (rates && !m->as.rates.empty()) std::memcpy(rates, m->as.rates.data(), m->as.rates.size());
+    if (finest) *finest = m->as.finest;
+    return 0;
+}
+
+int dg_multirate_stats_read(dg_multirate* m, dg_multirate_stats* out) {
+    DG_REQUIRE(m && out, "dg_multirate_stats_read: null argument");
+    *out = m->stats;
+    return 0;
+}
+
+int dg_multirate_run(dg_multirate* m, dg_state* s, double t, double t_end, double cfl, double dt_max, double interval,
+                     double* t_out, int64_t* coarse_steps) {
+    namespace mr = dgswe::multirate;
+    DG_REQUIRE(m && s, "dg_multirate_run: null argument");
+    int64_t n = 0, k = 0;
+    const double t_start = t;
+    while (t < t_end) {
+        const double target = interval > 0.0 ? std::fmin(t_start + (double)(k + 1) * interval, t_end) : t_end;
+        double fitted = 0.0, dt = 0.0;
+        if (int r = dg_multirate_assign_levels(m, s, cfl, dt_max, target - t, &fitted, &dt)) return r;
+        if (dt_max > 0.0) dt = std::fmin(dt, dt_max);
+        dt = std::fmin(dt, fitted);
+        const bool landed = t + dt * (1.0 + mr::LANDING_SLACK) >= target;
+        if (landed) dt = target - t;
+        if (int r = dg_multirate_step(m, s, t, dt)) return r;
+        t = landed ? target : t + dt;
+        ++n;
+        if (interval > 0.0 && t >= t_start + (double)(k + 1) * interval) ++k;
+    }
+    DG_TRY(hipStreamSynchronize(m->c->stream));
+    if (t_out) *t_out = t;
+    if (coarse_steps) *coarse_steps = n;
     return 0;
 }
 

@@ -503,15 +503,6 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
     if constexpr (LTS && !SUB) {
         if (i == 0 && e < EPB) skid[e] = active ? k : -1;
     }
-    // the element's own stage weight and tide elevation (its level's)
-    double sc = p.c, seta = p.eta_e;
-    if constexpr (LTS) {
-#pragma unroll
-        for (int l = 0; l <= DG_MAX_LEVELS; ++l) {
-            sc = lv == l ? p.lvl_c[l] : sc;
-            seta = lv == l ? p.lvl_eta[l] : seta;
-        }
-    }
     lds_sync();
     const bool subcells_on = (p.flags & DG_SUBCELLS) != 0;
     if (!SUB && active && subcells_on && hin < p.h_dry) sflag[e] = 1;
@@ -560,6 +551,12 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             const uint32_t l = lk[r];
             Cons fstar;
             if (l == LINK_BOUNDARY && tg[r] == DG_BC_TIDE) {
+                // the tide at the element's own stage time (its level's)
+                double seta = p.eta_e;
+                if constexpr (LTS) {
+#pragma unroll
+                    for (int l = 0; l <= DG_MAX_LEVELS; ++l) seta = lv == l ? p.lvl_eta[l] : seta;
+                }
                 Cons qb = characteristic(q.h, q.hu, q.hv, q.b, seta, nx, ny, p.g, p.h_min, (p.flags & DG_INCOMING_WAVE) != 0);
                 Node nb = make_node(qb.h, qb.hu, qb.hv, q.b, p.h_min);
                 fstar = ec_flux(nb, nb, nx, ny, p.g);
@@ -698,7 +695,13 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
         return;
     }
 
-    // Shu-Osher combination and point-implicit damping (builder.rs:668-690; wetting_drying.rs:324-363)
+    // Shu-Osher combination and point-implicit damping (builder.rs:668-690; wetting_drying.rs:324-363),
+    // c the element's own (its level's)
+    double sc = p.c;
+    if constexpr (LTS) {
+#pragma unroll
+        for (int l = 0; l <= DG_MAX_LEVELS; ++l) sc = lv == l ? p.lvl_c[l] : sc;
+    }
     double vh = 0.0, vhu = 0.0, vhv = 0.0;
     const bool friction = (p.flags & DG_FRICTION) != 0, wet_dry = (p.flags & DG_WET_DRY) != 0;
     if (active) {
@@ -851,7 +854,8 @@ __global__ __launch_bounds__(64) DG_WET_OCCUPANCY void wet_lts_kernel(StageParam
     if (blk * EPB >= p.n_list) return;
     const int e = threadIdx.x / NN;
     const bool active = e < EPB && blk * EPB + e < p.n_list;
-    const int64_t k = p.elist[active ? blk * EPB + e : blk * EPB];
+    const int64_t i = active ? blk * EPB + e : blk * EPB;
+    const int64_t k = p.elist ? (int64_t)p.elist[i] : i;   // no list: every element, in mesh order
     stage_block<N, false, false, true>(p, k, active, nullptr, blk);
 }
 
@@ -957,7 +961,7 @@ __global__ __launch_bounds__(64) void finish_kernel(FinishParams p) {
     const int64_t blk = blockIdx.x;
     const int t = threadIdx.x, e = t / NN, i = t - e * NN;
     const bool active = e < EPB && blk * EPB + e < p.n_list;
-    const int64_t k = active ? (int64_t)p.elist[blk * EPB + e] : 0;
+    const int64_t k = !active ? 0 : p.elist ? (int64_t)p.elist[blk * EPB + e] : blk * EPB + e;
     const size_t n = (size_t)k * NN + i;
     double vh = 0.0, vhu = 0.0, vhv = 0.0;
     if (active) { vh = p.ah[n]; vhu = p.ahu[n]; vhv = p.ahv[n]; }

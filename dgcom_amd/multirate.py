@@ -157,6 +157,58 @@ def spread_to_neighbours(nb: np.ndarray, frm: np.ndarray, drop: int):
     return to, bool((to != frm).any())
 
 
+def face_nodes(n1: int) -> np.ndarray:
+    """Node indices [4][n1] of the four faces, counter-clockwise (node j*n1 + i
+    on r-line j and s-line i; DGOperators2D::face_nodes): node fi of a face
+    coincides with node n1 - 1 - fi of the neighbour's."""
+    fi = np.arange(n1)
+    return np.stack([fi, fi * n1 + n1 - 1, (n1 - 1) * n1 + (n1 - 1 - fi), (n1 - 1 - fi) * n1])
+
+
+def inner_offset(face: int, n1: int) -> int:
+    """The node one in from a face node, normal to the face (multirate.rs:274-283)."""
+    return (n1, -1, -n1, 1)[face]
+
+
+def spread_over_lines(nb: np.ndarray, links: np.ndarray, n1: int, line_local: np.ndarray, frm: np.ndarray):
+    """One stage's node-by-node spreading (multirate.rs:705-767): `frm` the node
+    rates [K][n1^2] of the stage's input; returns the node rates of its value and
+    each element's RHS rate. A line-local element's node reads its two GLL lines
+    and, on a face, the neighbour's coincident node; any other element reads all
+    its own nodes and each neighbour's face nodes and the nodes one in from them."""
+    K, nn = frm.shape
+    fn = face_nodes(n1)
+    links = np.asarray(links, dtype=np.uint32)
+    nbf = (links & 3).astype(np.int64)
+    # whole elements
+    finest = frm.max(axis=1)
+    for f in range(4):
+        has = nb[:, f] >= 0
+        j = np.where(has, nb[:, f], 0)
+        g = nbf[:, f]
+        # the neighbour's nodes of its face g and the nodes one in from them
+        nodes = fn[g]                                            # [K][n1]
+        inner = nodes + np.array([inner_offset(q, n1) for q in range(4)])[g][:, None]
+        vals = np.maximum(frm[j[:, None], nodes], frm[j[:, None], inner]).max(axis=1)
+        finest = np.where(has, np.maximum(finest, vals), finest)
+    to = np.repeat(finest[:, None], nn, axis=1).astype(np.uint8)
+    # line-local elements
+    own = frm.reshape(K, n1, n1)                                 # [k][j][i]
+    line = np.maximum(own.max(axis=2)[:, :, None], own.max(axis=1)[:, None, :]).reshape(K, nn)
+    for f in range(4):
+        has = nb[:, f] >= 0
+        j = np.where(has, nb[:, f], 0)
+        g = nbf[:, f]
+        other = fn[g][:, ::-1]                                   # the neighbour's node coinciding with fi
+        vals = frm[j[:, None], other]
+        cur = line[:, fn[f]]
+        line[:, fn[f]] = np.where(has[:, None], np.maximum(cur, vals), cur)
+    ll = np.asarray(line_local, dtype=bool)
+    to[ll] = line[ll]
+    rate = to.max(axis=1).astype(np.uint8)
+    return to, rate
+
+
 def limit_level_jumps(nb: np.ndarray, level: np.ndarray) -> np.ndarray:
     """Raise levels until face neighbours differ by at most one (multirate.rs:815-821)."""
     level = np.asarray(level, dtype=np.uint8)
@@ -198,8 +250,10 @@ class MultirateStepper:
     """Levels, stage rates, pass lists and stage storage of a multirate run
     (MultirateStepper, multirate.rs:713-1330; the face stencil)."""
 
-    def __init__(self, max_levels: int, scheme: int = SSP_RK3):
+    def __init__(self, max_levels: int, scheme: int = SSP_RK3, line_rates: bool = True):
         assert max_levels <= MAX_LEVELS
+        self.line_rates = line_rates
+        self.memo = None
         self.scheme = scheme
         self.stages = STAGES[scheme]
         self.max_levels = max_levels
@@ -221,19 +275,32 @@ class MultirateStepper:
         the coarse step at most the module's bound and `dt_cap`, fitted to
         `landing` (the time left to a time the run must reach)."""
         edt = local.element_dt_state(state, cfl, self.scheme)
-        bound = self.coarse_step_bound(local, state, edt)
+        flags = self.depth_flags(local, state)
+        bound = self.coarse_step_bound(local, edt, flags)
         cap = bound if dt_cap is None else min(dt_cap, bound)
-        return self.assign(edt, nb, cap if math.isfinite(cap) else None, landing)
+        lines = None
+        if self.line_rates and flags is not None and getattr(local, "n1", 0) >= 2:
+            lines = (local.n1, local.links, (flags & 4) != 0)
+        return self.assign(edt, nb, cap if math.isfinite(cap) else None, landing, lines)
 
     @staticmethod
-    def coarse_step_bound(local, state, element_dt) -> float:
+    def depth_flags(local, state):
+        """Bit 0: every node deeper than SUBCELL_CAP_DEPTH_FACTOR h_dry; bit 1:
+        than h_dry (the coarse-step bound); bit 2: than the line-local depth,
+        h_dry + LINE_LOCAL_HEADROOM (SWEPhysics2D::line_local, builder.rs:563-594,
+        the depth where the WetDry kernel changes)."""
+        h_dry = getattr(local, "line_local_h_dry", None)
+        if h_dry is None:
+            return None
+        return local.depth_flags(state, SUBCELL_CAP_DEPTH_FACTOR * h_dry, h_dry, h_dry + LINE_LOCAL_HEADROOM)
+
+    @staticmethod
+    def coarse_step_bound(local, element_dt, flags) -> float:
         """SWEPhysics2D::coarse_step_bound (builder.rs:614-637): the largest step
         of an element whose every node is deeper than SUBCELL_CAP_DEPTH_FACTOR
         h_dry, else than h_dry; none without a positivity bound."""
-        h_dry = getattr(local, "positivity_h_dry", None)
-        if h_dry is None:
+        if getattr(local, "positivity_h_dry", None) is None or flags is None:
             return math.inf
-        flags = local.depth_flags(state, SUBCELL_CAP_DEPTH_FACTOR * h_dry, h_dry, h_dry)
         edt = np.asarray(element_dt)
         fin = np.isfinite(edt)
         for bit in (1, 2):
@@ -243,8 +310,9 @@ class MultirateStepper:
                 return longest
         return math.inf
 
-    def assign(self, element_dt, nb: np.ndarray, dt_cap=None, landing=None) -> float:
-        """multirate.rs:1072-1168 with the face stencil."""
+    def assign(self, element_dt, nb: np.ndarray, dt_cap=None, landing=None, lines=None) -> float:
+        """multirate.rs:1072-1168 with the face stencil; with `lines` = (n1,
+        face links, line-local flags) the rates are spread node by node."""
         dt, finest, level = assign_levels(element_dt, self.max_levels, dt_cap)
         self.stable_step = dt
         if landing is not None and finest > 0 and landing > 0.0 and math.isfinite(dt):
@@ -254,12 +322,25 @@ class MultirateStepper:
                 dt = fitted
                 level, finest = levels_for_step(element_dt, dt, self.max_levels)
         self.finest = finest
+        # the limit, the rates and the lists stand while their inputs do
+        # (multirate.rs:1114-1121)
+        key = (finest, level.tobytes(), None if lines is None else np.asarray(lines[2], dtype=bool).tobytes())
+        if self.memo is not None and self.memo == key:
+            return dt
         level = limit_level_jumps(nb, level)
         rates = []
-        frm = level
-        for _ in self.stages:
-            frm, _ = spread_to_neighbours(nb, frm, 0)
-            rates.append(frm)
+        if lines is not None:
+            n1, links, ll = lines
+            frm = np.repeat(level[:, None], n1 * n1, axis=1)
+            for _ in self.stages:
+                frm, rate = spread_over_lines(nb, links, n1, ll, frm)
+                rates.append(rate)
+        else:
+            frm = level
+            for _ in self.stages:
+                frm, _ = spread_to_neighbours(nb, frm, 0)
+                rates.append(frm)
+        self.memo = key
         self.level, self.rate = level, rates
         self.evaluations_per_step = int(sum((np.left_shift(1, r.astype(np.int64))).sum() for r in rates))
         self.active = [at_or_above(r, finest) for r in rates]
@@ -269,6 +350,7 @@ class MultirateStepper:
     def assign_one_level(self, n_elements: int):
         """Every element on one level: global steps through the element stages
         (multirate.rs:1205-1218)."""
+        self.memo = None
         self.finest = 0
         self.level = np.zeros(n_elements, dtype=np.uint8)
         self.rate = [self.level.copy() for _ in self.stages]
@@ -331,24 +413,31 @@ class MultirateStepper:
 
 
 def run(local, nb, state, t: float, t_end: float, cfl: float, max_levels: int = 8, scheme: int = SSP_RK3,
-        dt_max=None, stepper: MultirateStepper | None = None, max_steps=None):
+        dt_max=None, stepper: MultirateStepper | None = None, max_steps=None, interval=None, callback=None):
     """Simulation::run_with_callback with a multirate integrator
-    (runner.rs:298-455): levels and the coarse step fitted to land on t_end,
-    then one coarse step, until t_end. Returns (t, coarse steps, stepper)."""
+    (runner.rs:298-455): levels and the coarse step fitted to land on the next
+    target (t_end, or with `interval` the next output time t_start + k interval,
+    where `callback(state, t)` is called), then one coarse step, until t_end.
+    Returns (t, coarse steps, stepper)."""
     stepper = stepper or MultirateStepper(max_levels, scheme)
-    n = 0
+    n, k, t_start = 0, 0, t
     while t < t_end:
         if max_steps is not None and n >= max_steps:
             raise RuntimeError(f"Maximum step limit ({max_steps}) reached")
-        fitted = stepper.assign_levels(local, nb, state, cfl, dt_max, t_end - t)
+        target = t_end if interval is None else min(t_start + (k + 1) * interval, t_end)
+        fitted = stepper.assign_levels(local, nb, state, cfl, dt_max, target - t)
         dt = stepper.stable_step
         if dt_max is not None:
             dt = min(dt, dt_max)
         dt = min(dt, fitted)
-        landed = t + dt * (1.0 + LANDING_SLACK) >= t_end
+        landed = t + dt * (1.0 + LANDING_SLACK) >= target
         if landed:
-            dt = t_end - t
+            dt = target - t
         stepper.step(local, state, t, dt)
-        t = t_end if landed else t + dt
+        t = target if landed else t + dt
         n += 1
+        if interval is not None and t >= t_start + (k + 1) * interval:
+            k += 1
+            if callback is not None:
+                callback(state, t)
     return t, n, stepper

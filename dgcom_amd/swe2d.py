@@ -38,6 +38,8 @@ EXPORTS = (
     "dg_tape_create", "dg_tape_destroy", "dg_misfit_forward", "dg_misfit_backward", "dg_tape_stats",
     "dg_revolve_count", "dg_batch_create", "dg_batch_destroy", "dg_batch_upload", "dg_batch_download", "dg_batch_run",
     "dg_batch_tuning", "dg_begin_coarse_step", "dg_stage_where", "dg_finish_where", "dg_depth_flags",
+    "dg_multirate_create", "dg_multirate_destroy", "dg_multirate_assign_levels", "dg_multirate_step",
+    "dg_multirate_levels", "dg_multirate_stats_read", "dg_multirate_run",
 )
 BATCH_GRAPH, BATCH_TUNE = 1, 2
 
@@ -63,6 +65,12 @@ class StagePlanDesc(ctypes.Structure):
     _fields_ = [("t", ctypes.c_double), ("dt", ctypes.c_double), ("finest", ctypes.c_int), ("substep", ctypes.c_int),
                 ("depth", ctypes.c_int), ("stage", ctypes.c_int), ("start", ctypes.c_double),
                 ("input", ctypes.c_double), ("beta", ctypes.c_double), ("c", ctypes.c_double), ("last", ctypes.c_int)]
+
+
+class MultirateStatsDesc(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int64), ("element_evaluations", ctypes.c_int64),
+                ("global_evaluations", ctypes.c_int64), ("finest_level", ctypes.c_int),
+                ("model_seconds", ctypes.c_double)]
 
 
 _libs: dict = {}
@@ -138,6 +146,14 @@ def load(fma: bool = False):
     L.dg_stage_where.argtypes = [vp, vp, vp, ctypes.POINTER(StagePlanDesc), vp, vp]
     L.dg_finish_where.argtypes = [vp, vp, vp, ctypes.c_int]
     L.dg_depth_flags.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u8p]
+    L.dg_multirate_create.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+    L.dg_multirate_destroy.argtypes = [vp]
+    L.dg_multirate_assign_levels.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]
+    L.dg_multirate_step.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double]
+    L.dg_multirate_levels.argtypes = [vp, _u8p, _u8p, ctypes.POINTER(ctypes.c_int)]
+    L.dg_multirate_stats_read.argtypes = [vp, ctypes.POINTER(MultirateStatsDesc)]
+    L.dg_multirate_run.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double, _dp, i64p]
     _libs[key] = L
     return L
 
@@ -491,6 +507,11 @@ class LocalTimeStepping:
         self.physics = physics
         self.lib = physics.lib
         self.positivity_h_dry = physics.h_dry if physics.flags & WET_DRY else None
+        # the line-local depth's h_dry (builder.rs:296-322: the WetDry kernel
+        # changes at h_dry) and the node layout for the node-by-node rates
+        self.line_local_h_dry = physics.h_dry
+        self.n1 = physics.order + 1
+        self.links = physics.links
         self.n_stages = 0
 
     def clone_state(self, x: DeviceState) -> DeviceState:
@@ -535,6 +556,92 @@ class LocalTimeStepping:
             return
         _check(self.lib, self.lib.dg_finish_where(self.physics.ctx, state.handle, acc.handle, depth),
                "dg_finish_where")
+
+
+class Multirate:
+    """Local time stepping with the library's own stepper (`dg_multirate_*`:
+    MultirateStepper, src/time/multirate.rs:713-1330, in C++ beside the
+    kernels): levels and stage rates from the device's element time steps, then
+    the coarse step's passes, with no host round trip per pass."""
+
+    LANDING_SLACK = 1e-9   # multirate.rs:588
+
+    def __init__(self, physics: SWEPhysics2D, max_levels: int = 8, scheme: int = SSP_RK3, line_rates: bool = True):
+        self.physics = physics
+        self.lib = physics.lib
+        self.scheme = scheme
+        self.n_stages = 3 if scheme == SSP_RK3 else 4
+        self.h = ctypes.c_void_p()
+        _check(self.lib, self.lib.dg_multirate_create(physics.ctx, max_levels, scheme, int(line_rates),
+                                                      ctypes.byref(self.h)), "dg_multirate_create")
+
+    def assign_levels(self, state: DeviceState, cfl: float, dt_cap=None, landing=None):
+        """(the coarse step fitted to `landing`, the step the elements allow)."""
+        dt, stable = ctypes.c_double(), ctypes.c_double()
+        _check(self.lib, self.lib.dg_multirate_assign_levels(self.h, state.handle, cfl, dt_cap or 0.0, landing or 0.0,
+                                                             ctypes.byref(dt), ctypes.byref(stable)),
+               "dg_multirate_assign_levels")
+        return dt.value, stable.value
+
+    def step(self, state: DeviceState, t: float, dt: float):
+        _check(self.lib, self.lib.dg_multirate_step(self.h, state.handle, t, dt), "dg_multirate_step")
+
+    def levels(self):
+        K = self.physics.K
+        level = np.zeros(K, dtype=np.uint8)
+        rates = np.zeros((self.n_stages, K), dtype=np.uint8)
+        finest = ctypes.c_int()
+        _check(self.lib, self.lib.dg_multirate_levels(self.h, level.ctypes.data_as(_u8p), rates.ctypes.data_as(_u8p),
+                                                      ctypes.byref(finest)), "dg_multirate_levels")
+        return level, rates, finest.value
+
+    def stats(self) -> dict:
+        d = MultirateStatsDesc()
+        _check(self.lib, self.lib.dg_multirate_stats_read(self.h, ctypes.byref(d)), "dg_multirate_stats_read")
+        return {"steps": d.steps, "element_evaluations": d.element_evaluations,
+                "global_evaluations": d.global_evaluations, "finest_level": d.finest_level,
+                "model_seconds": d.model_seconds, "speedup": d.global_evaluations / max(d.element_evaluations, 1)}
+
+    def run(self, state: DeviceState, t: float, t_end: float, cfl: float, dt_max=None, interval=None, callback=None):
+        """Simulation::run_with_callback with a multirate integrator
+        (runner.rs:298-455): coarse steps to t_end, landing on every output
+        time t + k interval (callback(state, t) there). Returns (t, coarse steps)."""
+        if callback is None:
+            t_out, n = ctypes.c_double(), ctypes.c_int64()
+            _check(self.lib, self.lib.dg_multirate_run(self.h, state.handle, t, t_end, cfl, dt_max or 0.0,
+                                                       interval or 0.0, ctypes.byref(t_out), ctypes.byref(n)),
+                   "dg_multirate_run")
+            return t_out.value, n.value
+        n, k, t_start = 0, 0, t
+        while t < t_end:
+            target = t_end if interval is None else min(t_start + (k + 1) * interval, t_end)
+            fitted, dt = self.assign_levels(state, cfl, dt_max, target - t)
+            if dt_max is not None:
+                dt = min(dt, dt_max)
+            dt = min(dt, fitted)
+            landed = t + dt * (1.0 + self.LANDING_SLACK) >= target
+            if landed:
+                dt = target - t
+            self.step(state, t, dt)
+            t = target if landed else t + dt
+            n += 1
+            if interval is None:
+                callback(state, t)
+            elif t >= t_start + (k + 1) * interval:
+                k += 1
+                callback(state, t)
+        return t, n
+
+    def close(self):
+        if self.h:
+            self.lib.dg_multirate_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Simulation:

@@ -165,6 +165,39 @@ int dg_finish_where(dg_ctx* ctx, dg_state* state, const dg_state* acc, int depth
  * reference GPU's depth_flags, gpu/src/swe2d.rs:2030). */
 int dg_depth_flags(dg_ctx* ctx, const dg_state* state, double d0, double d1, double d2, uint8_t* out);
 
+/* The multirate stepper in the library (MultirateStepper, multirate.rs:713-1330,
+ * with the face stencil, node-by-node rates over SWEPhysics2D's line-local
+ * elements and its coarse-step bound, builder.rs:563-637): the caller's loop is
+ * assign_levels then step per coarse step, as Simulation::run_with_callback's
+ * (runner.rs:298-455); dg_multirate_run is that loop without callbacks. */
+typedef struct dg_multirate dg_multirate;
+typedef struct dg_multirate_stats {
+    int64_t steps, element_evaluations, global_evaluations;
+    int finest_level;
+    double model_seconds;
+} dg_multirate_stats;   /* MultirateStats (multirate.rs:856-889) */
+
+/* At most max_levels (<= DG_MAX_LEVELS) levels below the coarsest; line_rates:
+ * spread the rates of line-local elements node by node (the reference's default). */
+int dg_multirate_create(dg_ctx* ctx, int max_levels, int scheme, int line_rates, dg_multirate** out);
+int dg_multirate_destroy(dg_multirate* mr);
+/* MultirateStepper::assign_levels (multirate.rs:1014-1043): levels from the
+ * state's element time steps at `cfl`; the coarse step at most dt_cap (<= 0: none)
+ * and the module's bound, fitted to `landing` (<= 0: none) into *dt; *stable the
+ * step the elements allow (stable_step). */
+int dg_multirate_assign_levels(dg_multirate* mr, const dg_state* state, double cfl, double dt_cap, double landing,
+                               double* dt, double* stable);
+/* MultirateStepper::step (multirate.rs:1298-1330): one coarse step of dt from t. */
+int dg_multirate_step(dg_multirate* mr, dg_state* state, double t, double dt);
+/* The levels [K], the stage rates [n_stages][K] and the finest level of the last
+ * assignment (any pointer may be NULL). */
+int dg_multirate_levels(dg_multirate* mr, uint8_t* level, uint8_t* rates, int* finest);
+int dg_multirate_stats_read(dg_multirate* mr, dg_multirate_stats* out);
+/* Coarse steps from t to t_end, each landing on the next multiple of `interval`
+ * after t (<= 0: t_end only), dt_max (<= 0: none); *t_out, *coarse_steps. */
+int dg_multirate_run(dg_multirate* mr, dg_state* state, double t, double t_end, double cfl, double dt_max,
+                     double interval, double* t_out, int64_t* coarse_steps);
+
 /* LocalTimeStepping::element_dt per element (src/solver/rhs/swe_2d.rs:2018-2115) into
  * the host array out[K], and PhysicsModule::compute_dt_ssp (builder.rs:464-485). */
 int dg_element_dt(dg_ctx* ctx, const dg_state* state, double cfl, int scheme, double* out);

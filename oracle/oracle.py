@@ -153,6 +153,9 @@ class Oracle:
         if not self.h:
             raise RuntimeError(L.oc_last_error().decode())
         self.positivity_h_dry = h_dry if wet_dry else None
+        self.line_local_h_dry = h_dry
+        self.n1 = order + 1
+        self.links = self._links
         if tide is not None:
             self.set_tide(tide)
 

@@ -4,6 +4,7 @@
 #include <algorithm>
 
 #include "../../dgcom_amd/csrc/swe_kernels.hpp"
+#include "../../dgcom_amd/csrc/multirate.hpp"
 
 using namespace dgswe;
 
@@ -157,6 +158,29 @@ unsigned long long emu_finish_where(int order, int64_t K, const double* corners,
         default: emu_launch(grid, 64, [&] { finish_kernel<3>(p); }); break;
     }
     return clips;
+}
+
+// The library's multirate level assignment (multirate::Assigner, host code) over
+// a sequence of element time steps and depth flags, one assigner throughout
+// (its kept rates included)
+void emu_multirate_assign(int64_t K, int n1, const uint32_t* links, int max_levels, int n_stages, int line_rates,
+                          int calls, const double* edt, const uint8_t* flags, int wet_dry, const double* dt_cap,
+                          const double* landing, double* dt_out, double* stable_out, int* finest_out,
+                          uint8_t* level_out, uint8_t* rates_out) {
+    dgswe::multirate::Assigner as;
+    as.max_levels = max_levels;
+    as.n_stages = n_stages;
+    as.n1 = n1;
+    as.line_rates = line_rates != 0;
+    as.init(links, (size_t)K);
+    for (int q = 0; q < calls; ++q) {
+        std::vector<double> e(edt + (size_t)q * K, edt + (size_t)(q + 1) * K);
+        dt_out[q] = as.assign(e, flags + (size_t)q * K, wet_dry != 0, dt_cap[q], landing[q]);
+        stable_out[q] = as.stable;
+        finest_out[q] = as.finest;
+        std::memcpy(level_out + (size_t)q * K, as.level.data(), (size_t)K);
+        std::memcpy(rates_out + (size_t)q * n_stages * K, as.rates.data(), (size_t)n_stages * K);
+    }
 }
 
 // The stage VJP: gbase (if not null) and ginput are added to; gbed and gman too.

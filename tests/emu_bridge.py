@@ -35,6 +35,9 @@ def lib():
         L.emu_finish_where.restype = ctypes.c_ulonglong
         L.emu_finish_where.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, vp, _dp, _dp, vp, ctypes.c_int64,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.emu_multirate_assign.restype = None
+        L.emu_multirate_assign.argtypes = [ctypes.c_int64, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, _dp, vp, ctypes.c_int, _dp, _dp, _dp, _dp, vp, vp, vp]
         L.emu_init()
         _lib = L
     return _lib
@@ -164,3 +167,25 @@ class EmulatedKernels:
                             self._p(W), self._p(gb), self._p(gi), self._p(gbed), self._p(gman), a, b, c, eta, c_.g,
                             c_.h_min, c_.h_dry, 20.0, 1.0, c_.f0, int(rhs_only), self.flags)
         return np.split(gb, 3), np.split(gi, 3), gbed, gman
+
+
+def multirate_assign(links, n1, max_levels, n_stages, line_rates, edts, flags, wet_dry, dt_caps, landings):
+    """The library's level assignment (multirate::Assigner) over a sequence of
+    calls: per call (dt, stable, finest, level[K], rates[n_stages][K])."""
+    links = np.ascontiguousarray(links, dtype=np.uint32).ravel()
+    K = links.size // 4
+    calls = len(edts)
+    E = np.ascontiguousarray(np.concatenate(edts), dtype=np.float64)
+    F = np.ascontiguousarray(np.concatenate(flags), dtype=np.uint8)
+    cap = np.array([c if c is not None else 0.0 for c in dt_caps], dtype=np.float64)
+    land = np.array([x if x is not None else 0.0 for x in landings], dtype=np.float64)
+    dt, stable = np.zeros(calls), np.zeros(calls)
+    finest = np.zeros(calls, dtype=np.int32)
+    level = np.zeros(calls * K, dtype=np.uint8)
+    rates = np.zeros(calls * n_stages * K, dtype=np.uint8)
+    P = lambda a: a.ctypes.data_as(_dp)
+    lib().emu_multirate_assign(K, n1, links.ctypes.data, max_levels, n_stages, int(line_rates), calls, P(E),
+                               F.ctypes.data, int(wet_dry), P(cap), P(land), P(dt), P(stable), finest.ctypes.data,
+                               level.ctypes.data, rates.ctypes.data)
+    return [(dt[q], stable[q], int(finest[q]), level[q * K:(q + 1) * K],
+             rates[q * n_stages * K:(q + 1) * n_stages * K].reshape(n_stages, K)) for q in range(calls)]

@@ -60,3 +60,75 @@ def test_tidal_drying_beach_equals_the_oracle():
     nd_o, es_o = orc.masks(x[0])
     assert np.array_equal(nd_g, nd_o) and np.array_equal(es_g, es_o) and es_g.any()
     assert phys.negative_depth_clips() == orc.clips()
+
+
+@pytest.mark.parametrize("velocity", ["at_rest", "incoming_wave"])
+def test_tidal_channel_lts_equals_the_oracle(velocity):
+    """The reference's 3-level tidal channel gate: 25 h, 1,470 coarse steps
+    (1,529 with the incoming wave), the state bit for bit."""
+    c = cases.tidal_channel(velocity=velocity)
+    nb = M.neighbours(c.mesh.links)
+    orc = O.Oracle.from_case(c)
+    x = [c.h.copy(), c.hu.copy(), c.hv.copy()]
+    _, n_o, _ = M.run(orc, nb, x, 0.0, 25 * 3600.0, c.cfl, 3, c.scheme, interval=1800.0)
+    phys = swe2d.SWEPhysics2D.from_case(c)
+    local = phys.local_time_stepping()
+    s = phys.state(c.h, c.hu, c.hv)
+    _, n_g, st = M.run(local, nb, s, 0.0, 25 * 3600.0, c.cfl, 3, c.scheme, interval=1800.0)
+    assert n_g == n_o == {"at_rest": 1470, "incoming_wave": 1529}[velocity]
+    for a, b in zip(s.download(), x):
+        assert np.array_equal(a, b)
+    assert phys.negative_depth_clips() == 0
+
+
+# ---------------------------------------------------------------------------
+# The library's own stepper (dg_multirate_*, C++) against the restatement
+# driven by dgcom_amd.multirate
+# ---------------------------------------------------------------------------
+def _library_run(c, t_end, cfl, friction, wet_dry, max_levels=8, interval=None):
+    phys = swe2d.SWEPhysics2D.from_case(c, friction=friction, wet_dry=wet_dry)
+    mr = swe2d.Multirate(phys, max_levels, c.scheme)
+    s = phys.state(c.h, c.hu, c.hv)
+    t, n = mr.run(s, 0.0, t_end, cfl, interval=interval)
+    got = s.download()
+    stats = mr.stats()
+    mr.close()
+    return t, n, got, stats, phys
+
+
+def _oracle_run(c, t_end, cfl, friction, wet_dry, max_levels=8, interval=None):
+    orc = O.Oracle.from_case(c, friction=friction, wet_dry=wet_dry)
+    x = [c.h.copy(), c.hu.copy(), c.hv.copy()]
+    t, n, st = M.run(orc, M.neighbours(c.mesh.links), x, 0.0, t_end, cfl, max_levels, c.scheme, interval=interval)
+    return t, n, x, st, orc
+
+
+@pytest.mark.parametrize("which", ["graded_rk3", "graded_rk43", "island_rk3", "island_rk43", "beach"])
+def test_library_stepper_equals_the_oracle(which):
+    if which.startswith("graded"):
+        c = cases.graded_channel(scheme=0 if which.endswith("rk3") else 1)
+        args = (400.0, 0.5, False, False)
+    elif which.startswith("island"):
+        c = cases.graded_channel(island=True, friction=True, scheme=0 if which.endswith("rk3") else 1, cfl=1.0)
+        args = (600.0, 1.0, True, True)
+    else:
+        c = cases.drying_beach(nx=24, ny=6)
+        args = (120.0, c.cfl, True, True)
+    t_o, n_o, x, st, orc = _oracle_run(c, *args)
+    t_g, n_g, got, stats, phys = _library_run(c, *args)
+    assert (t_g, n_g) == (t_o, n_o)
+    assert stats["element_evaluations"] == st.stats.element_evaluations
+    assert stats["finest_level"] == st.stats.finest_level
+    for a, b in zip(got, x):
+        assert np.array_equal(a, b)
+    assert phys.negative_depth_clips() == orc.clips()
+
+
+@pytest.mark.parametrize("velocity", ["at_rest", "incoming_wave"])
+def test_library_stepper_tidal_channel(velocity):
+    c = cases.tidal_channel(velocity=velocity)
+    t_o, n_o, x, st, orc = _oracle_run(c, 25 * 3600.0, c.cfl, True, True, 3, 1800.0)
+    t_g, n_g, got, stats, phys = _library_run(c, 25 * 3600.0, c.cfl, True, True, 3, 1800.0)
+    assert n_g == n_o == {"at_rest": 1470, "incoming_wave": 1529}[velocity]
+    for a, b in zip(got, x):
+        assert np.array_equal(a, b)

@@ -243,3 +243,149 @@ def test_emulated_lts_passes_equal_the_oracle(scheme):
 
     st.schedule(t0, dt, visit)
     assert n_sub > 0, "no subcell element: the subcell LTS kernel went untested"
+
+
+def test_line_local_rates_cross_an_element_in_two_stages():
+    # multirate.rs:1511-1523: a row of line-local elements, the last on level 1:
+    # the finer rate reaches one element further every two stages
+    n = 6
+    mesh = cases.structured_mesh(0.0, float(n), 0.0, 1.0, n, 1)
+    nb = M.neighbours(mesh.links)
+    n1 = 3
+    level = np.array([0, 0, 0, 0, 0, 1], dtype=np.uint8)
+    frm = np.repeat(level[:, None], n1 * n1, axis=1)
+    ll = np.ones(n, dtype=bool)
+    rates = []
+    for _ in range(4):
+        frm, rate = M.spread_over_lines(nb, mesh.links, n1, ll, frm)
+        rates.append(list(rate))
+    assert rates == [[0, 0, 0, 0, 1, 1], [0, 0, 0, 0, 1, 1], [0, 0, 0, 1, 1, 1], [0, 0, 0, 1, 1, 1]]
+    # whole elements: one face per stage
+    frm = np.repeat(level[:, None], n1 * n1, axis=1)
+    rates = []
+    for _ in range(2):
+        frm, rate = M.spread_over_lines(nb, mesh.links, n1, np.zeros(n, dtype=bool), frm)
+        rates.append(list(rate))
+    assert rates == [[0, 0, 0, 0, 1, 1], [0, 0, 0, 1, 1, 1]]
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_line_local_rates_are_the_whole_element_rates_bit_for_bit(scheme):
+    """local_time_stepping_test.rs:1122-1200: a run on node-by-node rates is the
+    run on whole-element rates bit for bit, with fewer evaluations."""
+    c = cases.graded_channel(scheme=scheme)
+    nb = M.neighbours(c.mesh.links)
+    out = []
+    for line_rates in (True, False):
+        orc = oracle_for(c, wet_dry=False)
+        x = [c.h.copy(), c.hu.copy(), c.hv.copy()]
+        st = M.MultirateStepper(8, scheme, line_rates=line_rates)
+        t, n, st = M.run(orc, nb, x, 0.0, 200.0, 0.5, 8, scheme, stepper=st)
+        out.append((x, n, st.stats))
+    (xa, na, sa), (xb, nb_, sb) = out
+    assert na == nb_
+    for a, b in zip(xa, xb):
+        assert np.array_equal(a, b)
+    assert sa.element_evaluations < sb.element_evaluations
+
+
+# ---------------------------------------------------------------------------
+# The reference's published local-time-stepping known answers (notes/gpu.md:184)
+# ---------------------------------------------------------------------------
+LTS_KNOWN_STEPS = {"at_rest": 1470, "incoming_wave": 1529}
+
+
+@pytest.mark.parametrize("velocity", ["at_rest", "incoming_wave"])
+@pytest.mark.parametrize("max_levels", [3, 8])
+def test_tidal_channel_coarse_steps(velocity, max_levels):
+    """cases/tidal_channel over 25 h with local time stepping, landing on the
+    30-minute outputs: 1,470 coarse steps (1,529 with the incoming wave), at 3
+    levels and at 8 (the channel uses 3), no clip; the mass conserved."""
+    c = cases.tidal_channel(velocity=velocity)
+    orc = O.Oracle.from_case(c)
+    x = [c.h.copy(), c.hu.copy(), c.hv.copy()]
+    nb = M.neighbours(c.mesh.links)
+    t, n, st = M.run(orc, nb, x, 0.0, 25 * 3600.0, c.cfl, max_levels, c.scheme, interval=1800.0)
+    assert n == LTS_KNOWN_STEPS[velocity]
+    assert st.stats.finest_level == 2 and orc.clips() == 0
+
+
+def test_refined_tidal_channel_levels():
+    """The channel refined to 64,000 elements, 15 min at 8 levels: 143 coarse
+    steps of 6.29 s with 2^3 finest substeps, 1.23x fewer element evaluations
+    than global steps (notes/gpu.md:184,186)."""
+    c = cases.tidal_channel(nx=800, ny=80)
+    orc = O.Oracle.from_case(c)
+    x = [c.h.copy(), c.hu.copy(), c.hv.copy()]
+    nb = M.neighbours(c.mesh.links)
+    st = M.MultirateStepper(8, c.scheme)
+    dt = st.assign_levels(orc, nb, x, c.cfl, None, 900.0)
+    assert dt == 900.0 / 143 and st.finest == 3
+    assert abs(st.planned_speedup() - 1.23) < 0.005
+
+
+# ---------------------------------------------------------------------------
+# The library's own stepper (C++, multirate::Assigner) against this one
+# ---------------------------------------------------------------------------
+class _Bound:
+    def __init__(self, wet_dry):
+        self.positivity_h_dry = 1e-3 if wet_dry else None
+
+
+def _py_assign(st, edt, flags, nb, links, n1, wet_dry, cap, landing, line_rates):
+    bound = M.MultirateStepper.coarse_step_bound(_Bound(wet_dry), edt, flags)
+    c = bound if cap is None else min(cap, bound)
+    lines = (n1, links, (flags & 4) != 0) if line_rates else None
+    dt = st.assign(edt, nb, c if math.isfinite(c) else None, landing, lines)
+    return dt, st.stable_step, st.finest, st.level.copy(), np.stack(st.rate)
+
+
+@pytest.mark.parametrize("line_rates", [True, False])
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_library_assigner_equals_the_stepper(line_rates, scheme):
+    """Levels, coarse steps, the limit and the stage rates of the library's C++
+    stepper equal this module's over a sequence of calls on a jittered mesh:
+    random element steps (some infinite), depth flags, caps and landings, calls
+    repeated with the same inputs (the kept rates) and with changed ones."""
+    from emu_bridge import multirate_assign
+    mesh = cases.structured_mesh(0.0, 1.0, 0.0, 1.0, 13, 9, jitter=0.2, seed=5)
+    K = mesh.n_elements
+    nb = M.neighbours(mesh.links)
+    rng = np.random.default_rng(11 + scheme)
+    n1 = 3
+    calls = []
+    for q in range(10):
+        edt = 2.0 ** rng.uniform(-4.0, 0.0, size=K)
+        edt[rng.random(K) < 0.05] = math.inf
+        flags = rng.integers(0, 8, size=K).astype(np.uint8)
+        cap = [None, 0.7, 0.2][q % 3]
+        landing = [None, 3.3, 0.9, 1e-3][q % 4]
+        calls.append((edt, flags, cap, landing))
+        if q % 3 == 0:
+            calls.append((edt, flags, cap, landing))     # the same again: the kept rates
+    for wet_dry in (True, False):
+        ns = len(M.STAGES[scheme])
+        got = multirate_assign(mesh.links, n1, 8, ns, line_rates, [c[0] for c in calls], [c[1] for c in calls],
+                               wet_dry, [c[2] for c in calls], [c[3] for c in calls])
+        st = M.MultirateStepper(8, scheme, line_rates=line_rates)
+        for (edt, flags, cap, landing), g in zip(calls, got):
+            want = _py_assign(st, edt, flags, nb, mesh.links, n1, wet_dry, cap, landing, line_rates)
+            assert g[0] == want[0] and g[1] == want[1] and g[2] == want[2]
+            assert np.array_equal(g[3], want[3])
+            assert np.array_equal(g[4], want[4])
+
+
+def test_library_assigner_on_the_refined_tidal_channel():
+    from emu_bridge import multirate_assign
+    c = cases.tidal_channel(nx=800, ny=80)
+    orc = O.Oracle.from_case(c)
+    x = [c.h.copy(), c.hu.copy(), c.hv.copy()]
+    edt = orc.element_dt_state(x, c.cfl, c.scheme)
+    flags = orc.depth_flags(x, 0.1, 1e-3, 1e-3 + 1.0)
+    (dt, stable, finest, level, rates), = multirate_assign(c.mesh.links, 3, 8, 4, True, [edt], [flags], True, [None],
+                                                           [900.0])
+    assert dt == 900.0 / 143 and finest == 3
+    nb = M.neighbours(c.mesh.links)
+    st = M.MultirateStepper(8, c.scheme)
+    assert st.assign_levels(orc, nb, x, c.cfl, None, 900.0) == dt
+    assert np.array_equal(st.level, level) and np.array_equal(np.stack(st.rate), rates)
