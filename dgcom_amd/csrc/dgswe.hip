@@ -94,6 +94,7 @@ struct dg_ctx {
     uint8_t *lts_level = nullptr, *lts_lrate = nullptr;
     uint32_t* lts_order = nullptr;
     int lts_stages = 0, lts_finest = -1;
+    uint64_t lts_generation = 0;    // bumped whenever the lists are rebuilt
     std::vector<uint8_t> lts_keys;  // the host copy of the last levels and rates
     std::vector<int64_t> lts_count;  // [n_stages + 1][DG_MAX_LEVELS + 1]
     uint8_t* depth_flags = nullptr;
@@ -173,6 +174,7 @@ static int free_statics(dg_ctx* c) {
     c->lts_stages = 0;
     c->lts_finest = -1;
     c->lts_keys.clear();
+    ++c->lts_generation;
     c->lts_count.clear();
     c->kind = nullptr;
     c->st_elem = nullptr;
@@ -1084,6 +1086,7 @@ int dg_begin_coarse_step(dg_ctx* c, const uint8_t* level, const uint8_t* rates, 
     DG_TRY(hipStreamSynchronize(c->stream));
     c->lts_keys.swap(keys);
     c->lts_count.swap(count);
+    ++c->lts_generation;
     c->lts_stages = n_stages;
     c->lts_finest = finest;
     return 0;
@@ -1218,10 +1221,11 @@ struct dg_multirate {
     int scheme = 0;
     dgswe::multirate::Stage st[4];
     dgswe::multirate::Assigner as;
-    std::vector<double> edt;
-    std::vector<uint8_t> flags;
+    double* edt = nullptr;     // pinned host copies of the element time steps and depth flags
+    uint8_t* flags = nullptr;
     dg_state* values[3] = {nullptr, nullptr, nullptr};
     dg_state *last = nullptr, *acc = nullptr;
+    uint64_t generation = ~0ull;   // the context's list generation this stepper made
     dg_multirate_stats stats{};
 };
 
@@ -1249,15 +1253,23 @@ int dg_multirate_create(dg_ctx* c, int max_levels, int scheme, int line_rates, d
     m->as.n1 = c->order + 1;
     m->as.n_stages = dgswe::multirate::stages(scheme == DG_SSP_RK3 ? 0 : 1, m->st);
     m->as.init(links.data(), K);
-    m->edt.resize(K);
-    m->flags.resize(K);
+    if (hipHostMalloc((void**)&m->edt, K * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&m->flags, K, hipHostMallocDefault) != hipSuccess) {
+        if (m->edt) (void)hipHostFree(m->edt);
+        delete m;
+        g_err = "dg_multirate_create: pinned host memory";
+        return 1;
+    }
     *out = m;
     return 0;
 }
 
 int dg_multirate_destroy(dg_multirate* m) {
     if (!m) return 0;
+    DG_TRY(hipSetDevice(m->c->device));
     if (int r = mr_free_states(m)) return r;
+    DG_TRY(hipHostFree(m->edt));
+    DG_TRY(hipHostFree(m->flags));
     delete m;
     return 0;
 }
@@ -1273,10 +1285,10 @@ int dg_multirate_assign_levels(dg_multirate* m, const dg_state* s, double cfl, d
     // coarse-step bound and the line-local elements
     double dtmin = 0.0;
     if (int r = launch_dt(c, s->d, cfl, m->scheme, true, &dtmin)) return r;
-    DG_TRY(hipMemcpy(m->edt.data(), c->element_dt, K * sizeof(double), hipMemcpyDeviceToHost));
+    DG_TRY(hipMemcpyAsync(m->edt, c->element_dt, K * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     if (int r = dg_depth_flags(c, s, mr::SUBCELL_CAP_DEPTH_FACTOR * c->h_dry, c->h_dry, c->h_dry + mr::LINE_LOCAL_HEADROOM,
-                               m->flags.data())) return r;
-    *dt_out = m->as.assign(m->edt, m->flags.data(), (c->flags & DG_WET_DRY) != 0, dt_cap, landing);
+                               m->flags)) return r;   // (synchronises the stream)
+    *dt_out = m->as.assign(m->edt, K, m->flags, (c->flags & DG_WET_DRY) != 0, dt_cap, landing);
     if (stable_out) *stable_out = m->as.stable;
     return 0;
 }
@@ -1293,7 +1305,13 @@ int dg_multirate_step(dg_multirate* m, dg_state* s, double t, double dt) {
         if (int r = dg_state_alloc(c, &m->last)) return r;
         if (int r = dg_state_alloc(c, &m->acc)) return r;
     }
-    if (int r = dg_begin_coarse_step(c, m->as.level.data(), m->as.rates.data(), ns)) return r;
+    // the passes' lists are the last ones while the levels and rates are (and
+    // nothing else has begun a coarse step on the context since)
+    if (m->as.changed || m->generation != c->lts_generation) {
+        if (int r = dg_begin_coarse_step(c, m->as.level.data(), m->as.rates.data(), ns)) return r;
+        m->as.changed = false;
+        m->generation = c->lts_generation;
+    }
     const int finest = m->as.finest;
     for (int p = 0; p < (1 << finest); ++p) {
         const int d = dgswe::multirate::depth(p, finest);

@@ -168,6 +168,7 @@ struct Assigner {
     double stable = 0.0;
     std::vector<uint8_t> level, rates;  // [K], [n_stages][K]
     int64_t evals_per_step = 0;
+    bool changed = true;  // the level limit or the rates differ from the last assignment's
     // kept while their inputs are (multirate.rs:1114-1121)
     bool memo = false;
     int memo_finest = -1;
@@ -182,7 +183,14 @@ struct Assigner {
 
     // Returns the coarse step fitted to `landing` (<= 0: none); `stable` the one
     // the elements allow. dt_cap <= 0: none.
-    double assign(const std::vector<double>& edt, const uint8_t* flags, bool wet_dry, double dt_cap, double landing) {
+    double assign(const double* edt_p, size_t K, const uint8_t* flags, bool wet_dry, double dt_cap, double landing) {
+        edt.assign(edt_p, edt_p + K);
+        return assign_kept(flags, wet_dry, dt_cap, landing);
+    }
+    std::vector<double> edt;
+
+  private:
+    double assign_kept(const uint8_t* flags, bool wet_dry, double dt_cap, double landing) {
         const size_t K = edt.size();
         double bound = std::numeric_limits<double>::infinity();
         if (wet_dry) {
@@ -214,6 +222,7 @@ struct Assigner {
             level = memo_limited;
             return dt;
         }
+        changed = true;
         memo_given = level;
         while (spread_to_neighbours(nb, level, scratch, 1)) level.swap(scratch);
         rates.resize((size_t)n_stages * K);

@@ -174,8 +174,7 @@ void emu_multirate_assign(int64_t K, int n1, const uint32_t* links, int max_leve
     as.line_rates = line_rates != 0;
     as.init(links, (size_t)K);
     for (int q = 0; q < calls; ++q) {
-        std::vector<double> e(edt + (size_t)q * K, edt + (size_t)(q + 1) * K);
-        dt_out[q] = as.assign(e, flags + (size_t)q * K, wet_dry != 0, dt_cap[q], landing[q]);
+        dt_out[q] = as.assign(edt + (size_t)q * K, (size_t)K, flags + (size_t)q * K, wet_dry != 0, dt_cap[q], landing[q]);
         stable_out[q] = as.stable;
         finest_out[q] = as.finest;
         std::memcpy(level_out + (size_t)q * K, as.level.data(), (size_t)K);
