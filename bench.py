@@ -311,7 +311,7 @@ def main():
                     help="members of the inversion ensemble's gradient timing (0: not run)")
     ap.add_argument("--adjoint-ensemble-steps", type=int, default=1000)
     ap.add_argument("--lts", default="800x80,2000x200", help="refined tidal channels for the local-time-stepping "
-                    "leg, comma-separated nx x ny ('' for none)")
+                    "leg, comma-separated nx x ny ('none' for none)")
     ap.add_argument("--lts-seconds", type=float, default=900.0)
     ap.add_argument("--lts-levels", type=int, default=8)
     args = ap.parse_args()
@@ -396,7 +396,7 @@ def main():
         out["ensemble"] = ensemble_bench(args.ensemble_members, args.ensemble_steps, 20_000, 400_000)
     if args.adjoint_ensemble_members > 0:
         out["adjoint_ensemble"] = adjoint_ensemble_bench(args.adjoint_ensemble_members, args.adjoint_ensemble_steps, 20)
-    if args.lts:
+    if args.lts and args.lts != "none":
         out["lts"] = [lts_bench(*map(int, spec.split("x")), args.lts_seconds, args.lts_levels)
                       for spec in args.lts.split(",")]
     if not args.no_cpu:
