@@ -50,6 +50,8 @@ class Mesh:
     corners: np.ndarray  # (K, 4, 2) float64, counter-clockwise from (r, s) = (-1, -1)
     links: np.ndarray    # (K, 4) uint32
     tags: np.ndarray     # (K, 4) uint8, boundary kind of boundary faces
+    vertices: np.ndarray = None  # (V, 2) the shared vertices, where the builder keeps them
+    quads: np.ndarray = None     # (K, 4) each element's vertex indices
 
     @property
     def n_elements(self) -> int:
@@ -104,7 +106,11 @@ def structured_mesh(x0, x1, y0, y1, nx, ny, sides=(WALL, WALL, WALL, WALL), peri
     for face, (mask) in enumerate((j == 0, i == nx - 1, j == ny - 1, i == 0)):
         b = mask & (links[:, face] == BOUNDARY)
         tags[b, face] = sides[face]
-    return Mesh(corners=corners, links=links.astype(np.uint32), tags=tags)
+    # the grid's vertices, row by row, as the reference's structured_quads numbers them (mesh2d.rs:216-243)
+    v0 = j * (nx + 1) + i
+    quads = np.stack([v0, v0 + 1, v0 + nx + 2, v0 + nx + 1], axis=1).astype(np.uint64)
+    vertices = np.stack([vx.ravel(), vy.ravel()], axis=1)
+    return Mesh(corners=corners, links=links.astype(np.uint32), tags=tags, vertices=vertices, quads=quads)
 
 
 def node_coordinates(mesh: Mesh, order: int):

@@ -1373,6 +1373,11 @@ int dg_multirate_run(dg_multirate* m, dg_state* s, double t, double t_end, doubl
     return 0;
 }
 
+uint64_t dg_fnv1a(const uint8_t* data, size_t n, uint64_t h) {
+    for (size_t i = 0; i < n; ++i) h = (h ^ (uint64_t)data[i]) * 0x100000001b3ull;
+    return h;
+}
+
 int dg_element_dt(dg_ctx* c, const dg_state* s, double cfl, int scheme, double* out) {
     DG_REQUIRE(c && s && out && c->K > 0, "dg_element_dt: null argument or no form");
     DG_TRY(hipSetDevice(c->device));

@@ -39,7 +39,7 @@ EXPORTS = (
     "dg_revolve_count", "dg_batch_create", "dg_batch_destroy", "dg_batch_upload", "dg_batch_download", "dg_batch_run",
     "dg_batch_tuning", "dg_begin_coarse_step", "dg_stage_where", "dg_finish_where", "dg_depth_flags",
     "dg_multirate_create", "dg_multirate_destroy", "dg_multirate_assign_levels", "dg_multirate_step",
-    "dg_multirate_levels", "dg_multirate_stats_read", "dg_multirate_run",
+    "dg_multirate_levels", "dg_multirate_stats_read", "dg_multirate_run", "dg_fnv1a",
 )
 BATCH_GRAPH, BATCH_TUNE = 1, 2
 
@@ -146,6 +146,8 @@ def load(fma: bool = False):
     L.dg_stage_where.argtypes = [vp, vp, vp, ctypes.POINTER(StagePlanDesc), vp, vp]
     L.dg_finish_where.argtypes = [vp, vp, vp, ctypes.c_int]
     L.dg_depth_flags.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u8p]
+    L.dg_fnv1a.argtypes = [_u8p, ctypes.c_size_t, ctypes.c_uint64]
+    L.dg_fnv1a.restype = ctypes.c_uint64
     L.dg_multirate_create.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
     L.dg_multirate_destroy.argtypes = [vp]
     L.dg_multirate_assign_levels.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]

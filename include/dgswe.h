@@ -198,6 +198,11 @@ int dg_multirate_stats_read(dg_multirate* mr, dg_multirate_stats* out);
 int dg_multirate_run(dg_multirate* mr, dg_state* state, double t, double t_end, double cfl, double dt_max,
                      double interval, double* t_out, int64_t* coarse_steps);
 
+/* FNV-1a (64 bit) of n bytes continuing from h (0xcbf29ce484222325 to start):
+ * the checksum and domain fingerprint of the reference's restart files
+ * (src/io/restart.rs:90-91,660-697), for the restart writer above the ABI. */
+uint64_t dg_fnv1a(const uint8_t* data, size_t n, uint64_t h);
+
 /* LocalTimeStepping::element_dt per element (src/solver/rhs/swe_2d.rs:2018-2115) into
  * the host array out[K], and PhysicsModule::compute_dt_ssp (builder.rs:464-485). */
 int dg_element_dt(dg_ctx* ctx, const dg_state* state, double cfl, int scheme, double* out);
