@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared():
     text = open(os.path.join(ROOT, "include", "dgswe.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(dg_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|uint64_t|const char\*)\s+(dg_\w+)\s*\(", text, re.M)))
 
 
 @pytest.mark.parametrize("fma", [False, True], ids=["exact", "fma"])
