@@ -391,6 +391,7 @@ static int ensure_adjoint(dg_ctx* c) {
 static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, const double* rhs, const double* w,
                             double* gbase, double* ginput, double a, double b, double cc, double time, int rhs_only,
                             int input_write = 0) {
+    DG_REQUIRE(!(c->flags & DG_ENTROPY_STABLE), "the EntropyStable faces have no VJP here (the adjoint is of the WetDry faces)");
     if (int r = ensure_adjoint(c)) return r;
     size_t nt = (size_t)c->K * c->nn;
     VjpParams P;
@@ -872,6 +873,8 @@ int dg_upload_form(dg_ctx* c, const dg_form* f) {
     DG_REQUIRE(f->order >= 1 && f->order <= 3, "dg_upload_form: order must be 1, 2 or 3");
     DG_REQUIRE(f->n_elements > 0 && f->n_elements < (int64_t(1) << 28) - 1, "dg_upload_form: element count out of range");
     DG_REQUIRE(f->corners && f->face_links && f->face_tags && f->bed, "dg_upload_form: missing mesh, tags or bed");
+    DG_REQUIRE(!((f->flags & DG_ENTROPY_STABLE) && (f->flags & DG_SUBCELLS)),
+               "dg_upload_form: the EntropyStable faces take no subcells (DG_SUBCELLS off)");
     DG_TRY(hipSetDevice(c->device));
     if (int r = free_statics(c)) return r;
     c->order = f->order;

@@ -584,10 +584,19 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
                         lx = p.fgeo[fo]; ly = p.fgeo[nf4 + fo];
                     }
                 }
-                const Side sa = left ? side(q) : side(qo), sbb = left ? side(qo) : side(q);
-                Cons fa, fb;
-                hydro_hll(sa, sbb, lx, ly, p.g, p.sqrt_g, p.h_min, fa, fb);
-                fstar = left ? fa : cneg(fb);
+                if (p.flags & DG_ENTROPY_STABLE) {
+                    // EntropyStable faces (swe_2d_split_form.rs:611-651): the EC flux and
+                    // its dissipation from the left side, plus each side's bed term
+                    const Node na = left ? q : qo, nb = left ? qo : q;
+                    const Cons shared = cadd(ec_flux(na, nb, lx, ly, p.g), es_dissipation(na, nb, lx, ly, p.g));
+                    fstar = left ? cadd(shared, bed_interface(q.h, q.b, qo.b, lx, ly, p.g))
+                                 : cadd(cneg(shared), bed_interface(q.h, q.b, qo.b, -lx, -ly, p.g));
+                } else {
+                    const Side sa = left ? side(q) : side(qo), sbb = left ? side(qo) : side(q);
+                    Cons fa, fb;
+                    hydro_hll(sa, sbb, lx, ly, p.g, p.sqrt_g, p.h_min, fa, fb);
+                    fstar = left ? fa : cneg(fb);
+                }
             }
             if (p.fstar_out) {
                 double* o = p.fstar_out + ((size_t)k * NF + task) * 3;

@@ -50,6 +50,25 @@ DG_HD Cons ec_flux(const Node& l, const Node& r, double mx, double my, double g)
     return {mass, mass * ua + pressure * mx, mass * va + pressure * my};
 }
 
+// The EntropyStable faces' dissipation (swe_2d_entropy.rs:166-203): -1/2 the
+// larger of the two sides' |u.n| + sqrt(g h) times the jump of the surface and
+// velocity, weighted by the average state; odd in the pair, even in n
+DG_HD Cons es_dissipation(const Node& i, const Node& e, double nx, double ny, double g) {
+    const double si = fabs(i.u * nx + i.v * ny) + sqrt(g * fmax_(i.h, 0.0));
+    const double se = fabs(e.u * nx + e.v * ny) + sqrt(g * fmax_(e.h, 0.0));
+    const double sc = -0.5 * fmax_(si, se);
+    const double jump = (e.h + e.b) - (i.h + i.b);
+    const double ha = 0.5 * (i.h + e.h), ua = 0.5 * (i.u + e.u), va = 0.5 * (i.v + e.v);
+    return {sc * jump, sc * (ua * jump + ha * (e.u - i.u)), sc * (va * jump + ha * (e.v - i.v))};
+}
+
+// The bed's interface term of one side of an EntropyStable face (swe_2d_split_form.rs:626-651):
+// 1/2 g h (B_outside - B_inside) n
+DG_HD Cons bed_interface(double h, double b, double b_out, double nx, double ny, double g) {
+    const double force = (0.5 * g) * h * (b_out - b);
+    return {0.0, force * nx, force * ny};
+}
+
 // hll_flux_face_aligned + einfeldt_speeds_2d (flux/swe_2d.rs:277-374), with the
 // sides in face-aligned components (depth, normal / tangential momentum and velocity)
 struct Hll { double h, hun, hut, un, ut; };

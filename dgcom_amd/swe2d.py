@@ -24,7 +24,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 
 SSP_RK3, SSP_RK43 = 0, 1
-FRICTION, WET_DRY, SUBCELLS, INCOMING_WAVE = 1, 2, 4, 8
+FRICTION, WET_DRY, SUBCELLS, INCOMING_WAVE, ENTROPY_STABLE = 1, 2, 4, 8, 16
 BC_WALL, BC_TIDE = 0, 1
 
 # Every entry point of include/dgswe.h
@@ -237,7 +237,7 @@ class SWEPhysics2D:
 
     def __init__(self, order, mesh, bed, manning=None, manning_uniform=0.025, f0=0.0, tide=None, g=9.81,
                  h_min=1e-6, h_dry=1e-3, relaxation_time=1.0, max_velocity=20.0, friction=True, wet_dry=True,
-                 subcells=True, device=0, fma=False, incoming_wave=False):
+                 subcells=True, device=0, fma=False, incoming_wave=False, entropy_stable=False):
         self.lib = L = load(fma)
         self.order = order
         self.nn = (order + 1) ** 2
@@ -252,8 +252,10 @@ class SWEPhysics2D:
         if bed.size != self.n_nodes_total:
             raise ValueError("bed must have K*(N+1)^2 values")
         man = None if manning is None else np.ascontiguousarray(manning, dtype=np.float64)
-        flags = (FRICTION if friction else 0) | (WET_DRY if wet_dry else 0) | (SUBCELLS if subcells else 0) | \
-                (INCOMING_WAVE if incoming_wave else 0)
+        # the EntropyStable faces take no subcells (swe_2d_split_form.rs:711-713 with its formulation)
+        flags = (FRICTION if friction else 0) | (WET_DRY if wet_dry else 0) | \
+                (SUBCELLS if subcells and not entropy_stable else 0) | (INCOMING_WAVE if incoming_wave else 0) | \
+                (ENTROPY_STABLE if entropy_stable else 0)
         form = Form(order, self.K, _p(corners), links.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
                     tags.ctypes.data_as(_u8p), _p(bed), None if man is None else _p(man), manning_uniform, f0, g,
                     h_min, h_dry, relaxation_time, max_velocity, flags)

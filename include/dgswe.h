@@ -43,7 +43,11 @@ enum { DG_BC_WALL = 0, DG_BC_TIDE = 1 };
 enum { DG_FRICTION = 1, DG_WET_DRY = 2, DG_SUBCELLS = 4,
        /* elevation-only tide data as a wave entering from still water (ElevationOnly::IncomingWave,
         * src/boundary/characteristic.rs:49-53,424-426); without it, at rest (ElevationOnly::AtRest) */
-       DG_INCOMING_WAVE = 8 };
+       DG_INCOMING_WAVE = 8,
+       /* SWEFormulation2D::EntropyStable faces: the EC flux with entropy-stable dissipation and
+        * the bed's interface term (swe_2d_split_form.rs:611-651, swe_2d_entropy.rs:166-203) instead
+        * of the hydrostatic HLL; no subcells (DG_SUBCELLS must be off); no VJP */
+       DG_ENTROPY_STABLE = 16 };
 
 /* Schemes (src/time/integrator.rs:226-263) */
 enum { DG_SSP_RK3 = 0, DG_SSP_RK43 = 1 };
