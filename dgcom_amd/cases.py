@@ -446,3 +446,83 @@ def graded_channel(order=2, nx=16, ny=3, ratio=1.25, hump=0.5, island=False, fri
     h = np.maximum(eta - bed, 0.0)
     return Case("graded_channel", order, mesh, bed, np.full(mesh.n_elements, 0.025 if friction else 0.0), 0.0,
                 Tide(), h, np.zeros_like(h), np.zeros_like(h), cfl=cfl, scheme=scheme, n_steps=0)
+
+
+# ---------------------------------------------------------------------------
+# Tides per boundary node (BoundaryTides, src/boundary/tidal_atlas.rs:858-1140)
+# ---------------------------------------------------------------------------
+@dataclasses.dataclass
+class BoundaryTides:
+    """The open boundary's tides per boundary face node: constituents' angular
+    speeds `omega` (rad/s, the nodal phase drift included) and nodal factor
+    drifts `f_rate` (1/s); `slot` [K][4][N+1] (0xFFFFFFFF: the uniform tide);
+    per slot `coef` [n_c][6] (C, S of eta, u, v: f(t)/f0 (C cos w't + S sin w't)),
+    `mean` (added unramped) and `source_depth`; the ramp (s), whether the tides
+    carry a velocity and the transport ratio limit (0: unscaled)."""
+    omega: np.ndarray
+    f_rate: np.ndarray
+    slot: np.ndarray
+    coef: np.ndarray
+    mean: np.ndarray
+    source_depth: np.ndarray
+    ramp: float = 0.0
+    velocity: bool = False
+    max_ratio: float = 0.0
+
+
+def open_face_nodes(case):
+    """The open-boundary face nodes (k, f, fi) of a case, in mesh order, with
+    their coordinates."""
+    n1 = case.order + 1
+    x, y = node_coordinates(case.mesh, case.order)
+    x, y = x.reshape(case.n_elements, -1), y.reshape(case.n_elements, -1)
+    fnodes = [[fi if f == 0 else fi * n1 + n1 - 1 if f == 1 else (n1 - 1) * n1 + (n1 - 1 - fi) if f == 2
+               else (n1 - 1 - fi) * n1 for fi in range(n1)] for f in range(4)]
+    out = []
+    for k in range(case.n_elements):
+        for f in range(4):
+            if case.mesh.links[k, f] == BOUNDARY and case.mesh.tags[k, f] == TIDE:
+                for fi in range(n1):
+                    nd = fnodes[f][fi]
+                    out.append((k, f, fi, float(x[k, nd]), float(y[k, nd])))
+    return out
+
+
+def boundary_tides(case, omega, f_rate, constants, ramp=0.0, velocity=False, max_ratio=0.0, mean=None,
+                   source_depth=None) -> BoundaryTides:
+    """BoundaryTides with one slot per open-boundary face node: `constants(x, y)`
+    gives a node's [n_c][6] C, S of eta, u, v; `mean(x, y)`, `source_depth(x, y)`
+    its mean level and source depth (zeros by default)."""
+    nodes = open_face_nodes(case)
+    n1 = case.order + 1
+    slot = np.full((case.n_elements, 4, n1), 0xFFFFFFFF, dtype=np.uint32)
+    coef = np.zeros((len(nodes), len(omega), 6))
+    me = np.zeros(len(nodes))
+    sd = np.zeros(len(nodes))
+    for i, (k, f, fi, x, y) in enumerate(nodes):
+        slot[k, f, fi] = i
+        coef[i] = constants(x, y)
+        if mean is not None:
+            me[i] = mean(x, y)
+        if source_depth is not None:
+            sd[i] = source_depth(x, y)
+    return BoundaryTides(np.asarray(omega, dtype=np.float64), np.asarray(f_rate, dtype=np.float64), slot, coef, me,
+                         sd, ramp, velocity, max_ratio)
+
+
+def boundary_tides_of(case, tide: Tide) -> BoundaryTides:
+    """A uniform harmonic tide written as boundary tides (elevation only): per
+    constituent C = f0 A cos(phi0), S = -f0 A sin(phi0) with phi0 = phase +
+    (V0 + u0) degrees, omega' = omega + u_rate, f_rate / f0."""
+    n = len(tide.amplitude)
+    nod = tide.nodal if tide.nodal else [(1.0, 0.0, 0.0, 0.0, 0.0)] * n
+    omega = [tide.omega[j] + math.radians(nod[j][4]) for j in range(n)]
+    f_rate = [nod[j][1] / nod[j][0] for j in range(n)]
+    c = np.zeros((n, 6))
+    for j in range(n):
+        f0, _, v0, u0, _ = nod[j]
+        phi0 = tide.phase[j] + math.radians(v0 + u0)
+        c[j, 0] = f0 * tide.amplitude[j] * math.cos(phi0)
+        c[j, 1] = -f0 * tide.amplitude[j] * math.sin(phi0)
+    return boundary_tides(case, omega, f_rate, lambda x, y: c, ramp=tide.ramp,
+                          mean=(lambda x, y: tide.mean) if tide.mean else None)

@@ -98,6 +98,11 @@ struct dg_ctx {
     std::vector<uint8_t> lts_keys;  // the host copy of the last levels and rates
     std::vector<int64_t> lts_count;  // [n_stages + 1][DG_MAX_LEVELS + 1]
     uint8_t* depth_flags = nullptr;
+    // boundary tides per face node (dg_set_boundary_tides): device tables and the host's constituents
+    uint32_t* tide_slot = nullptr;
+    double *tide_coef = nullptr, *tide_mean = nullptr, *tide_src = nullptr;
+    int atlas_nc = 0, atlas_velocity = 0;
+    double atlas_omega[8] = {}, atlas_f_rate[8] = {}, atlas_ramp = 0.0, atlas_max_ratio = 0.0;
     // stations
     int n_st = 0;
     int64_t* st_elem = nullptr;
@@ -164,12 +169,16 @@ static int free_statics(dg_ctx* c) {
     void* ptrs[] = {c->corners, c->bed, c->manning, c->element_dt, c->links, c->tags, c->affine, c->queue,
                     c->queue_count, c->dt_slots, c->sub_list, c->chunk_total, c->gbed, c->gman, c->facebuf,
                     c->innerbuf, c->facecot, c->kind, c->st_elem, c->st_w, c->st_eta, c->geo, c->fgeo,
-                    c->lts_level, c->lts_lrate, c->lts_order, c->depth_flags};
+                    c->lts_level, c->lts_lrate, c->lts_order, c->depth_flags, c->tide_slot, c->tide_coef,
+                    c->tide_mean, c->tide_src};
     for (void* p : ptrs)
         if (p) DG_TRY(hipFree(p));
     c->gbed = c->gman = c->facebuf = c->innerbuf = c->facecot = nullptr;
     c->geo = c->fgeo = nullptr;
     c->lts_level = c->lts_lrate = c->depth_flags = nullptr;
+    c->tide_slot = nullptr;
+    c->tide_coef = c->tide_mean = c->tide_src = nullptr;
+    c->atlas_nc = 0;
     c->lts_order = nullptr;
     c->lts_stages = 0;
     c->lts_finest = -1;
@@ -225,10 +234,36 @@ static int launch_geometry(dg_ctx* c) {
     return 0;
 }
 
+// The boundary tides' tables and their factors at the stage time (BoundaryTides::moment,
+// tidal_atlas.rs:995-1008): the ramp, then per constituent f(t)/f0 = 1 + f_rate t and
+// cos, sin of omega' t, evaluated here on the host (the device sums the nodes' terms)
+static void set_atlas(const dg_ctx* c, StageParams& p, double time) {
+    if (!c->tide_slot) return;
+    p.tide_slot = c->tide_slot;
+    p.tide_coef = c->tide_coef;
+    p.tide_mean = c->tide_mean;
+    p.tide_src = c->tide_src;
+    p.atlas_nc = c->atlas_nc;
+    p.atlas_velocity = c->atlas_velocity;
+    p.atlas_max_ratio = c->atlas_max_ratio;
+    double rf = 1.0;
+    if (c->atlas_ramp > 0.0) {
+        double tau = std::min(std::max(time / c->atlas_ramp, 0.0), 1.0);
+        rf = tau * tau * (3.0 - 2.0 * tau);
+    }
+    p.atlas_moment[0] = rf;
+    for (int j = 0; j < c->atlas_nc; ++j) {
+        const double arg = c->atlas_omega[j] * time;
+        p.atlas_moment[1 + 3 * j] = 1.0 + c->atlas_f_rate[j] * time;
+        p.atlas_moment[2 + 3 * j] = std::cos(arg);
+        p.atlas_moment[3 + 3 * j] = std::sin(arg);
+    }
+}
+
 static int launch_stage(dg_ctx* c, const double* base, const double* input, double* out, double a, double b, double cc,
                         double time, int rhs_only, double* fstar_out = nullptr, double* rhs_save = nullptr) {
     size_t nt = (size_t)c->K * c->nn;
-    StageParams p;
+    StageParams p{};
     p.bh = base; p.bhu = base + nt; p.bhv = base + 2 * nt;
     p.ih = input; p.ihu = input + nt; p.ihv = input + 2 * nt;
     p.oh = out; p.ohu = out + nt; p.ohv = out + 2 * nt;
@@ -250,6 +285,7 @@ static int launch_stage(dg_ctx* c, const double* base, const double* input, doub
     p.rhv = rhs_save ? rhs_save + 2 * nt : nullptr;
     p.queue = c->queue;
     p.queue_count = c->queue_count;
+    set_atlas(c, p, time);
     const int epb = 64 / c->nn;
     const unsigned int nblk = (unsigned int)((c->K + epb - 1) / epb);
     const unsigned int nchunk = (nblk + 1023) / 1024;
@@ -392,9 +428,10 @@ static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, 
                             double* gbase, double* ginput, double a, double b, double cc, double time, int rhs_only,
                             int input_write = 0) {
     DG_REQUIRE(!(c->flags & DG_ENTROPY_STABLE), "the EntropyStable faces have no VJP here (the adjoint is of the WetDry faces)");
+    DG_REQUIRE(!c->tide_slot, "the boundary tides per node have no VJP here (the adjoint is of the uniform tide)");
     if (int r = ensure_adjoint(c)) return r;
     size_t nt = (size_t)c->K * c->nn;
-    VjpParams P;
+    VjpParams P{};
     StageParams& p = P.s;
     p.bh = base; p.bhu = base + nt; p.bhv = base + 2 * nt;
     p.ih = input; p.ihu = input + nt; p.ihv = input + 2 * nt;
@@ -1099,6 +1136,7 @@ int dg_stage_where(dg_ctx* c, const dg_state* base, const dg_state* input, const
                    dg_state* acc) {
     DG_REQUIRE(c && base && input && plan && out && c->K > 0, "dg_stage_where: null argument or no form");
     DG_REQUIRE(c->lts_order, "dg_stage_where: no coarse step begun (dg_begin_coarse_step)");
+    DG_REQUIRE(!c->tide_slot, "dg_stage_where: the boundary tides per node run with global steps only");
     DG_REQUIRE(plan->stage >= 0 && plan->stage < c->lts_stages, "dg_stage_where: stage out of range");
     DG_REQUIRE(plan->finest >= c->lts_finest && plan->finest <= DG_MAX_LEVELS,
                "dg_stage_where: the plan's finest level is not the coarse step's");
@@ -1111,7 +1149,7 @@ int dg_stage_where(dg_ctx* c, const dg_state* base, const dg_state* input, const
     if (n_list == 0) return 0;
     DG_TRY(hipSetDevice(c->device));
     size_t nt = (size_t)c->K * c->nn;
-    StageParams p;
+    StageParams p{};
     std::memset(&p, 0, sizeof(p));
     p.bh = base->d; p.bhu = base->d + nt; p.bhv = base->d + 2 * nt;
     p.ih = input->d; p.ihu = input->d + nt; p.ihv = input->d + 2 * nt;
@@ -1373,6 +1411,50 @@ int dg_multirate_run(dg_multirate* m, dg_state* s, double t, double t_end, doubl
     DG_TRY(hipStreamSynchronize(m->c->stream));
     if (t_out) *t_out = t;
     if (coarse_steps) *coarse_steps = n;
+    return 0;
+}
+
+int dg_set_boundary_tides(dg_ctx* c, int n_c, const double* omega, const double* f_rate, double ramp, int velocity,
+                          double max_ratio, int64_t n_slots, const uint32_t* slot_of_face_node, const double* coef,
+                          const double* mean, const double* source_depth) {
+    DG_REQUIRE(c && c->K > 0, "dg_set_boundary_tides: null context or no form");
+    DG_REQUIRE(n_c >= 0 && n_c <= 8, "dg_set_boundary_tides: 0 to 8 constituents");
+    DG_TRY(hipSetDevice(c->device));
+    DG_TRY(hipStreamSynchronize(c->stream));
+    for (void* ptr : {(void*)c->tide_slot, (void*)c->tide_coef, (void*)c->tide_mean, (void*)c->tide_src})
+        if (ptr) DG_TRY(hipFree(ptr));
+    c->tide_slot = nullptr;
+    c->tide_coef = c->tide_mean = c->tide_src = nullptr;
+    c->atlas_nc = 0;
+    if (n_c == 0) return 0;
+    DG_REQUIRE(omega && f_rate && slot_of_face_node && coef && mean && source_depth && n_slots > 0,
+               "dg_set_boundary_tides: missing tables");
+    const size_t nf = (size_t)c->K * 4 * (c->order + 1);
+    std::vector<uint8_t> tags((size_t)c->K * 4);
+    DG_TRY(hipMemcpy(tags.data(), c->tags, tags.size(), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> links((size_t)c->K * 4);
+    DG_TRY(hipMemcpy(links.data(), c->links, links.size() * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < nf; ++i) {
+        const uint32_t sl = slot_of_face_node[i];
+        if (sl == 0xFFFFFFFFu) continue;
+        const size_t kf = i / (size_t)(c->order + 1);
+        DG_REQUIRE((int64_t)sl < n_slots, "dg_set_boundary_tides: a slot out of range");
+        DG_REQUIRE(links[kf] == LINK_BOUNDARY && tags[kf] == DG_BC_TIDE,
+                   "dg_set_boundary_tides: a slot on a face that is not an open boundary");
+    }
+    DG_TRY(hipMalloc(&c->tide_slot, nf * sizeof(uint32_t)));
+    DG_TRY(hipMalloc(&c->tide_coef, (size_t)n_slots * n_c * 6 * sizeof(double)));
+    DG_TRY(hipMalloc(&c->tide_mean, (size_t)n_slots * sizeof(double)));
+    DG_TRY(hipMalloc(&c->tide_src, (size_t)n_slots * sizeof(double)));
+    DG_TRY(hipMemcpy(c->tide_slot, slot_of_face_node, nf * sizeof(uint32_t), hipMemcpyHostToDevice));
+    DG_TRY(hipMemcpy(c->tide_coef, coef, (size_t)n_slots * n_c * 6 * sizeof(double), hipMemcpyHostToDevice));
+    DG_TRY(hipMemcpy(c->tide_mean, mean, (size_t)n_slots * sizeof(double), hipMemcpyHostToDevice));
+    DG_TRY(hipMemcpy(c->tide_src, source_depth, (size_t)n_slots * sizeof(double), hipMemcpyHostToDevice));
+    c->atlas_nc = n_c;
+    for (int j = 0; j < n_c; ++j) { c->atlas_omega[j] = omega[j]; c->atlas_f_rate[j] = f_rate[j]; }
+    c->atlas_ramp = ramp;
+    c->atlas_velocity = velocity ? 1 : 0;
+    c->atlas_max_ratio = max_ratio;
     return 0;
 }
 
@@ -1809,6 +1891,7 @@ int dg_batch_create(int n, dg_ctx* const* members, int scheme, dg_batch** out) {
     for (int m = 0; m < n; ++m) {
         DG_REQUIRE(members[m] && members[m]->K > 0, "dg_batch_create: every member needs an uploaded form");
         DG_REQUIRE(members[m]->device == members[0]->device, "dg_batch_create: members on different devices");
+        DG_REQUIRE(!members[m]->tide_slot, "dg_batch_create: a member with boundary tides per node (not batched)");
     }
     DG_TRY(hipSetDevice(members[0]->device));
     dg_batch* b = new dg_batch;

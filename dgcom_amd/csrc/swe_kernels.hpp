@@ -114,6 +114,15 @@ struct StageParams {
     int depth;
     double *ah, *ahu, *ahv;
     double lvl_c[DG_MAX_LEVELS + 1], lvl_eta[DG_MAX_LEVELS + 1];
+    // Boundary tides per face node (BoundaryTides, tidal_atlas.rs:858-1140): the
+    // slot of face node (k, f, fi) (UINT32_MAX: the uniform tide), per slot the
+    // constants [n_c][6] (C, S of eta, u, v), the mean level (0 unless used) and
+    // the source model's depth; the stage time's ramp and (f, cos, sin) per constituent
+    const uint32_t* tide_slot;
+    const double *tide_coef, *tide_mean, *tide_src;
+    int atlas_nc, atlas_velocity;
+    double atlas_max_ratio;
+    double atlas_moment[1 + 3 * ATLAS_MAX_CONSTITUENTS];
 };
 
 // LDS exchange among the lanes of a workgroup that is one wavefront: the
@@ -551,13 +560,25 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             const uint32_t l = lk[r];
             Cons fstar;
             if (l == LINK_BOUNDARY && tg[r] == DG_BC_TIDE) {
-                // the tide at the element's own stage time (its level's)
-                double seta = p.eta_e;
-                if constexpr (LTS) {
+                const uint32_t slot = p.tide_slot ? p.tide_slot[((size_t)k * 4 + f) * N1 + fi] : 0xFFFFFFFFu;
+                Cons qb;
+                if (slot != 0xFFFFFFFFu) {
+                    // the node's own atlas tide (global steps only)
+                    double eta, ue, ve;
+                    atlas_state(p.tide_coef + (size_t)slot * 6 * p.atlas_nc, p.atlas_nc, p.atlas_moment, p.tide_mean[slot],
+                                p.atlas_velocity != 0, p.atlas_max_ratio, p.tide_src[slot], q.b, eta, ue, ve);
+                    qb = p.atlas_velocity ? characteristic_velocity(q.h, q.hu, q.hv, q.b, eta, ue, ve, nx, ny, p.g, p.h_min)
+                                          : characteristic(q.h, q.hu, q.hv, q.b, eta, nx, ny, p.g, p.h_min,
+                                                           (p.flags & DG_INCOMING_WAVE) != 0);
+                } else {
+                    // the tide at the element's own stage time (its level's)
+                    double seta = p.eta_e;
+                    if constexpr (LTS) {
 #pragma unroll
-                    for (int l = 0; l <= DG_MAX_LEVELS; ++l) seta = lv == l ? p.lvl_eta[l] : seta;
+                        for (int l = 0; l <= DG_MAX_LEVELS; ++l) seta = lv == l ? p.lvl_eta[l] : seta;
+                    }
+                    qb = characteristic(q.h, q.hu, q.hv, q.b, seta, nx, ny, p.g, p.h_min, (p.flags & DG_INCOMING_WAVE) != 0);
                 }
-                Cons qb = characteristic(q.h, q.hu, q.hv, q.b, seta, nx, ny, p.g, p.h_min, (p.flags & DG_INCOMING_WAVE) != 0);
                 Node nb = make_node(qb.h, qb.hu, qb.hv, q.b, p.h_min);
                 fstar = ec_flux(nb, nb, nx, ny, p.g);
             } else {

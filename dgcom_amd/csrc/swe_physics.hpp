@@ -194,15 +194,10 @@ DG_HD void reconstruct(const Side& m, const Side& c, const Side& p, double xm, d
 
 // characteristic_state (boundary/characteristic.rs:481-520) for elevation-only
 // data at rest, whose tangential velocity is the interior's (characteristic.rs:411-438)
-DG_HD Cons characteristic(double h, double hu, double hv, double bed, double eta_e, double nx, double ny, double g,
-                          double h_min, bool incoming) {
-    double h_e = fmax_(eta_e - bed, 0.0);
-    double ui = 0.0, vi = 0.0;
-    if (h > h_min) { double inv = 1.0 / h; ui = hu * inv; vi = hv * inv; }
-    // at rest, or a simple wave entering from still water at mean sea level
-    // (characteristic.rs:424-426): u_n,e = 2 (sqrt(g H) - sqrt(g h_e)), H = -B
-    double un_e = incoming ? 2.0 * (sqrt(g * fmax_(-bed, 0.0)) - sqrt(g * h_e)) : 0.0;
-    double ut_e = -ui * ny + vi * nx;
+// characteristic_state from the interior trace and the external depth and
+// normal / tangential velocity (characteristic.rs:481-520)
+DG_HD Cons characteristic_state(double h, double hu, double hv, double h_e, double un_e, double ut_e, double nx,
+                                double ny, double g, double h_min) {
     double h_i = fmax_(h, 0.0);
     double un_i = 0.0, ut_i = 0.0;
     if (h_i > h_min) {
@@ -223,6 +218,60 @@ DG_HD Cons characteristic(double h, double hu, double hv, double bed, double eta
     double ut_b = (un_b > 0.0) ? ut_i : ut_e;
     double ub = un_b * nx - ut_b * ny, vb = un_b * ny + ut_b * nx;
     return {h_b, h_b * ub, h_b * vb};
+}
+
+DG_HD Cons characteristic(double h, double hu, double hv, double bed, double eta_e, double nx, double ny, double g,
+                          double h_min, bool incoming) {
+    double h_e = fmax_(eta_e - bed, 0.0);
+    double ui = 0.0, vi = 0.0;
+    if (h > h_min) { double inv = 1.0 / h; ui = hu * inv; vi = hv * inv; }
+    // at rest, or a simple wave entering from still water at mean sea level
+    // (characteristic.rs:424-426): u_n,e = 2 (sqrt(g H) - sqrt(g h_e)), H = -B
+    double un_e = incoming ? 2.0 * (sqrt(g * fmax_(-bed, 0.0)) - sqrt(g * h_e)) : 0.0;
+    double ut_e = -ui * ny + vi * nx;
+    return characteristic_state(h, hu, hv, h_e, un_e, ut_e, nx, ny, g, h_min);
+}
+
+// The external state with a velocity (characteristic.rs:419-420): u_n,e = u.n,
+// u_t,e = -u n_y + v n_x
+DG_HD Cons characteristic_velocity(double h, double hu, double hv, double bed, double eta_e, double ue, double ve,
+                                   double nx, double ny, double g, double h_min) {
+    double h_e = fmax_(eta_e - bed, 0.0);
+    return characteristic_state(h, hu, hv, h_e, ue * nx + ve * ny, -ue * ny + ve * nx, nx, ny, g, h_min);
+}
+
+// A boundary node's atlas tide (BoundaryTides, tidal_atlas.rs:1002-1140): the
+// sum over constituents of f (C cos + S sin) of eta, u, v with the stage time's
+// factors `moment` = [ramp, (f, cos, sin) per constituent], then the ramp, the
+// node's mean level and, with a transport ratio limit, the velocity scaled to
+// carry the source model's transport over the local bed
+constexpr int ATLAS_MAX_CONSTITUENTS = 8;
+DG_HD void atlas_state(const double* c, int n_c, const double (&moment)[1 + 3 * ATLAS_MAX_CONSTITUENTS], double mean,
+                       bool velocity, double max_ratio, double source_depth, double bed, double& eta, double& u,
+                       double& v) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    // (a fixed trip count, so the moment is read at constant offsets)
+#pragma unroll
+    for (int j = 0; j < ATLAS_MAX_CONSTITUENTS; ++j) {
+        if (j >= n_c) break;
+        const double f = moment[1 + 3 * j], cs = moment[2 + 3 * j], sn = moment[3 + 3 * j];
+        const double* cj = c + 6 * j;
+        s0 += f * (cj[0] * cs + cj[1] * sn);
+        s1 += f * (cj[2] * cs + cj[3] * sn);
+        s2 += f * (cj[4] * cs + cj[5] * sn);
+    }
+    const double ramp = moment[0];
+    eta = mean + ramp * s0;
+    u = ramp * s1;
+    v = ramp * s2;
+    if (velocity && max_ratio > 0.0) {
+        const double d_source = source_depth + eta, d_child = eta - bed;
+        if (source_depth > 0.0 && d_source > 0.0 && d_child > 0.0) {
+            const double ratio = fmin_(fmax_(d_source / d_child, 1.0 / max_ratio), max_ratio);
+            u *= ratio;
+            v *= ratio;
+        }
+    }
 }
 
 // Reflective2D ghost state (boundary/boundary_2d.rs:271-285)

@@ -39,7 +39,7 @@ EXPORTS = (
     "dg_revolve_count", "dg_batch_create", "dg_batch_destroy", "dg_batch_upload", "dg_batch_download", "dg_batch_run",
     "dg_batch_tuning", "dg_begin_coarse_step", "dg_stage_where", "dg_finish_where", "dg_depth_flags",
     "dg_multirate_create", "dg_multirate_destroy", "dg_multirate_assign_levels", "dg_multirate_step",
-    "dg_multirate_levels", "dg_multirate_stats_read", "dg_multirate_run", "dg_fnv1a",
+    "dg_multirate_levels", "dg_multirate_stats_read", "dg_multirate_run", "dg_fnv1a", "dg_set_boundary_tides",
 )
 BATCH_GRAPH, BATCH_TUNE = 1, 2
 
@@ -146,6 +146,8 @@ def load(fma: bool = False):
     L.dg_stage_where.argtypes = [vp, vp, vp, ctypes.POINTER(StagePlanDesc), vp, vp]
     L.dg_finish_where.argtypes = [vp, vp, vp, ctypes.c_int]
     L.dg_depth_flags.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u8p]
+    L.dg_set_boundary_tides.argtypes = [vp, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_int64, ctypes.POINTER(ctypes.c_uint32), _dp, _dp, _dp]
     L.dg_fnv1a.argtypes = [_u8p, ctypes.c_size_t, ctypes.c_uint64]
     L.dg_fnv1a.restype = ctypes.c_uint64
     L.dg_multirate_create.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
@@ -283,6 +285,20 @@ class SWEPhysics2D:
         if nodal:
             cols = [np.array([c[q] for c in nodal], dtype=np.float64) for q in range(5)]
             _check(self.lib, self.lib.dg_set_tide_nodal(self.ctx, *(_p(c) for c in cols)), "dg_set_tide_nodal")
+
+    def set_boundary_tides(self, bt):
+        """The open boundary's tides per face node (a `cases.BoundaryTides`), or None to clear them."""
+        if bt is None:
+            _check(self.lib, self.lib.dg_set_boundary_tides(self.ctx, 0, None, None, 0.0, 0, 0.0, 0, None, None,
+                                                            None, None), "dg_set_boundary_tides")
+            return
+        om, fr, co, me, sd = (np.ascontiguousarray(a, dtype=np.float64) for a in (bt.omega, bt.f_rate, bt.coef,
+                                                                                  bt.mean, bt.source_depth))
+        sl = np.ascontiguousarray(bt.slot, dtype=np.uint32)
+        _check(self.lib, self.lib.dg_set_boundary_tides(self.ctx, om.size, _p(om), _p(fr), bt.ramp, int(bt.velocity),
+                                                        bt.max_ratio, me.size,
+                                                        sl.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _p(co),
+                                                        _p(me), _p(sd)), "dg_set_boundary_tides")
 
     def close(self):
         if self.ctx:

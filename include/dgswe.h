@@ -103,6 +103,19 @@ int dg_set_tide(dg_ctx* ctx, const dg_tide* tide);
 int dg_set_tide_nodal(dg_ctx* ctx, const double* f0, const double* f_rate, const double* v0_deg,
                       const double* u0_deg, const double* u_rate_deg);
 
+/* The open boundary's tides per boundary face node (BoundaryTides, src/boundary/tidal_atlas.rs:858-1140; the
+ * reference device's moment-and-sums form, notes/gpu.md:191): n_c (<= 8) constituents with the angular speed
+ * omega' (rad/s, the nodal phase drift included) and the nodal factor's drift f_rate (1/s); per boundary face
+ * node [K][4][N+1] its slot (UINT32_MAX: the uniform tide of dg_set_tide); per slot coef[n_slots][n_c][6]
+ * (C, S of eta, u, v), mean[n_slots] (added unramped; zeros for none) and source_depth[n_slots]; the ramp (s,
+ * <= 0: none); velocity: the tides carry u, v through the characteristic boundary (else elevation only, as
+ * the form's flags say); max_ratio > 0: the velocity scaled to carry the source's transport, limited to that
+ * factor either way (with_transport_scaling). n_c = 0 clears them. Global steps only: local time stepping,
+ * the ensembles and the VJP refuse them. */
+int dg_set_boundary_tides(dg_ctx* ctx, int n_c, const double* omega, const double* f_rate, double ramp, int velocity,
+                          double max_ratio, int64_t n_slots, const uint32_t* slot_of_face_node, const double* coef,
+                          const double* mean, const double* source_depth);
+
 /* Device states (DeviceState::upload / download, gpu/src/swe2d.rs:2384-2396). */
 int dg_state_alloc(dg_ctx* ctx, dg_state** state);
 int dg_state_free(dg_ctx* ctx, dg_state* state);

@@ -114,6 +114,8 @@ def _bind(L):
                                      ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
                                      + [ctypes.c_double] * 4 + [ctypes.c_int, _u8p, _u8p] + [_dp] * 6)
         L.oc_finish_where.argtypes = [ctypes.c_void_p] + [_dp] * 6 + [_u32p, ctypes.c_int64]
+        L.oc_set_boundary_tides.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int,
+                                            ctypes.c_double, ctypes.c_int64, _u32p, _dp, _dp, _dp]
     return L
 
 
@@ -179,6 +181,15 @@ class Oracle:
         if nodal:
             cols = [np.array([c[q] for c in nodal], dtype=np.float64) for q in range(5)]
             self._L.oc_set_tide_nodal(self.h, n, *(_p(c) for c in cols))
+
+    def set_boundary_tides(self, bt):
+        """BoundaryTides per boundary face node (a `cases.BoundaryTides`)."""
+        self._bt = [np.ascontiguousarray(a, dtype=np.float64) for a in (bt.omega, bt.f_rate, bt.coef, bt.mean,
+                                                                        bt.source_depth)]
+        self._bt_slot = np.ascontiguousarray(bt.slot, dtype=np.uint32)
+        om, fr, co, me, sd = self._bt
+        self._L.oc_set_boundary_tides(self.h, len(om), _p(om), _p(fr), bt.ramp, int(bt.velocity), bt.max_ratio,
+                                      me.size, self._bt_slot.ctypes.data_as(_u32p), _p(co), _p(me), _p(sd))
 
     def tide_elevation(self, t):
         return self._L.oc_tide_elevation(self.h, t)

@@ -99,7 +99,7 @@ void oc_face_mass(void* p, const double* h, const double* hu, const double* hv, 
                 if (oracle::link_is_boundary(l)) {
                     double nx, ny, sj;
                     s.normal(k, f, fi, nx, ny, sj);
-                    out[idx] = s.boundary_flux(k, f, s.node(h, hu, hv, k, s.ops.face_nodes[f][fi]), nx, ny, eta).h;
+                    out[idx] = s.boundary_flux(k, f, s.node(h, hu, hv, k, s.ops.face_nodes[f][fi]), nx, ny, eta, fi, t).h;
                 } else out[idx] = s.faces[idx].h;
             }
 }
@@ -123,6 +123,24 @@ void oc_stage_where(void* p, const double* bh, const double* bhu, const double* 
 void oc_finish_where(void* p, double* h, double* hu, double* hv, const double* ah, const double* ahu,
                      const double* ahv, const uint32_t* elements, int64_t n_list) {
     ((Handle*)p)->s->finish_where(h, hu, hv, ah, ahu, ahv, elements, n_list);
+}
+
+// BoundaryTides per boundary face node (tidal_atlas.rs:858-1140)
+void oc_set_boundary_tides(void* p, int n_c, const double* omega, const double* f_rate, double ramp, int velocity,
+                           double max_ratio, int64_t n_slots, const uint32_t* slot, const double* coef,
+                           const double* mean, const double* source_depth) {
+    Solver<double>& s = *((Handle*)p)->s;
+    auto& a = s.atlas;
+    a.n_c = n_c;
+    a.omega.assign(omega, omega + n_c);
+    a.f_rate.assign(f_rate, f_rate + n_c);
+    a.ramp = ramp;
+    a.velocity = velocity != 0;
+    a.max_ratio = max_ratio;
+    a.slot.assign(slot, slot + (size_t)s.K * 4 * s.ops.n1);
+    a.coef.assign(coef, coef + (size_t)n_slots * n_c * 6);
+    a.mean.assign(mean, mean + n_slots);
+    a.source_depth.assign(source_depth, source_depth + n_slots);
 }
 
 void oc_step(void* p, double* h, double* hu, double* hv, double t, double dt, int scheme) {

@@ -54,6 +54,16 @@ struct Solver {
     Geometry geom;
     Config cfg;
     Tide tide;
+    // BoundaryTides (boundary/tidal_atlas.rs:858-1140): per face node [K][4][N+1]
+    // its slot (UINT32_MAX: the uniform tide), per slot [n_c][6] C, S of eta, u, v,
+    // the mean level and the source model's depth
+    struct Atlas {
+        int n_c = 0;
+        std::vector<double> omega, f_rate, coef, mean, source_depth;
+        std::vector<uint32_t> slot;
+        double ramp = 0.0, max_ratio = 0.0;
+        bool velocity = false;
+    } atlas;
     int64_t K = 0;
     std::vector<double> corners;
     std::vector<uint32_t> links;
@@ -129,9 +139,62 @@ struct Solver {
     }
 
     // rhs/swe_2d.rs:304-327 boundary_state + swe_2d_split_form.rs:1090-1123 boundary_flux
-    State<T> boundary_flux(int64_t k, int f, const NodeState<T>& q, double nx, double ny, double eta_e) const {
+    // BoundaryTides::external_state (tidal_atlas.rs:1045-1060 with tide_term, tide_level, tide_state,
+    // :1078-1125): the node's sum over constituents at time t, ramped, its mean, the transport scaling
+    void atlas_state(uint32_t slot, double time, double bed, double& eta, double& u, double& v) const {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int j = 0; j < atlas.n_c; ++j) {
+            double arg = atlas.omega[j] * time;
+            double f = 1.0 + atlas.f_rate[j] * time, cs = std::cos(arg), sn = std::sin(arg);
+            const double* c = atlas.coef.data() + ((size_t)slot * atlas.n_c + j) * 6;
+            s0 += f * (c[0] * cs + c[1] * sn);
+            s1 += f * (c[2] * cs + c[3] * sn);
+            s2 += f * (c[4] * cs + c[5] * sn);
+        }
+        double ramp = 1.0;
+        if (atlas.ramp > 0.0) {
+            double tau = std::min(std::max(time / atlas.ramp, 0.0), 1.0);
+            ramp = tau * tau * (3.0 - 2.0 * tau);
+        }
+        eta = atlas.mean[slot] + ramp * s0;
+        u = ramp * s1;
+        v = ramp * s2;
+        if (atlas.velocity && atlas.max_ratio > 0.0) {
+            double hs = atlas.source_depth[slot], ds = hs + eta, dc = eta - bed;
+            if (hs > 0.0 && ds > 0.0 && dc > 0.0) {
+                double r = ds / dc;
+                r = r < 1.0 / atlas.max_ratio ? 1.0 / atlas.max_ratio : (r > atlas.max_ratio ? atlas.max_ratio : r);
+                u *= r;
+                v *= r;
+            }
+        }
+    }
+
+    State<T> boundary_flux(int64_t k, int f, const NodeState<T>& q, double nx, double ny, double eta_e,
+                           int fi = -1, double time = 0.0) const {
         double g = cfg.g, hm = cfg.h_min;
         T zero(0.0);
+        const uint32_t slot = (atlas.n_c > 0 && fi >= 0) ? atlas.slot[((size_t)k * 4 + f) * ops.n1 + fi] : 0xFFFFFFFFu;
+        if (tags[4 * k + f] == BC_TIDE && slot != 0xFFFFFFFFu) {
+            // CharacteristicOBC<BoundaryTides>: the node's own tide; with velocity, u_n,e = u.n,
+            // u_t,e = -u n_y + v n_x (characteristic.rs:411-438)
+            double eta, ue, ve;
+            atlas_state(slot, time, val(q.b), eta, ue, ve);
+            T h_e = vmax(T(eta) - q.b, zero);
+            T un_e, ut_e;
+            if (atlas.velocity) {
+                un_e = T(ue * nx + ve * ny);
+                ut_e = T(-ue * ny + ve * nx);
+            } else {
+                T u = zero, v = zero;
+                if (q.h > T(hm)) { T inv = T(1.0) / q.h; u = q.hu * inv; v = q.hv * inv; }
+                un_e = cfg.incoming_wave ? T(2.0) * (sq_sqrt(T(g) * vmax(-q.b, zero)) - sq_sqrt(T(g) * h_e)) : zero;
+                ut_e = -u * T(ny) + v * T(nx);
+            }
+            State<T> qb = characteristic_state(State<T>{q.h, q.hu, q.hv}, h_e, un_e, ut_e, T(nx), T(ny), g, hm);
+            NodeState<T> nb = node_state(qb.h, qb.hu, qb.hv, q.b, hm);
+            return wintermeyer_flux(nb, nb, T(nx), T(ny), g);
+        }
         if (tags[4 * k + f] == BC_TIDE) {
             // CharacteristicOBC<HarmonicTide>, elevation-only data at rest or as an incoming
             // wave from still water, u_n,e = 2 (sqrt(g H) - sqrt(g h_e)), H = -B (characteristic.rs:411-438)
@@ -351,7 +414,7 @@ struct Solver {
                 double nx, ny, sj;
                 normal(k, f, fi, nx, ny, sj);
                 const NodeState<T>& q = nodes[nd];
-                State<T> fstar = link_is_boundary(l) ? boundary_flux(k, f, q, nx, ny, eta_e)
+                State<T> fstar = link_is_boundary(l) ? boundary_flux(k, f, q, nx, ny, eta_e, fi, time)
                                                      : faces[((size_t)k * 4 + f) * n1 + fi];
                 State<T> diff = wintermeyer_flux(q, q, T(nx), T(ny), g) - fstar;
 #ifdef ORACLE_MUTANT_LIFT

@@ -67,7 +67,7 @@ unsigned long long emu_stage(int order, int64_t K, const double* corners, const 
                              double* fstar_out) {
     size_t nn = (size_t)(order + 1) * (order + 1), nt = (size_t)K * nn;
     unsigned long long clips = 0;
-    StageParams p;
+    StageParams p{};
     p.bh = base; p.bhu = base + nt; p.bhv = base + 2 * nt;
     p.ih = input; p.ihu = input + nt; p.ihv = input + 2 * nt;
     p.oh = out; p.ohu = out + nt; p.ohv = out + 2 * nt;
@@ -94,7 +94,7 @@ unsigned long long emu_stage_where(int order, int64_t K, const double* corners, 
                                    double tau, double f0, int flags) {
     size_t nn = (size_t)(order + 1) * (order + 1), nt = (size_t)K * nn;
     unsigned long long clips = 0;
-    StageParams p;
+    StageParams p{};
     std::memset(&p, 0, sizeof(p));
     p.bh = base; p.bhu = base + nt; p.bhv = base + 2 * nt;
     p.ih = input; p.ihu = input + nt; p.ihv = input + 2 * nt;
@@ -139,7 +139,7 @@ unsigned long long emu_finish_where(int order, int64_t K, const double* corners,
                                     double h_dry, double u_max, int wet_dry) {
     size_t nn = (size_t)(order + 1) * (order + 1), nt = (size_t)K * nn;
     unsigned long long clips = 0;
-    StageParams sp;
+    StageParams sp{};
     std::memset(&sp, 0, sizeof(sp));
     std::vector<double> bed(nt, 0.0), geo, fgeo;
     std::vector<uint32_t> links((size_t)4 * K, 0xFFFFFFFFu);
@@ -191,7 +191,7 @@ void emu_stage_vjp(int order, int64_t K, const double* corners, const uint32_t* 
                    double tau, double f0, int rhs_only, int flags) {
     size_t nn = (size_t)(order + 1) * (order + 1), nt = (size_t)K * nn, nf = 4 * (size_t)(order + 1);
     unsigned long long clips = 0;
-    VjpParams P;
+    VjpParams P{};
     StageParams& p = P.s;
     p.bh = base; p.bhu = base + nt; p.bhv = base + 2 * nt;
     p.ih = input; p.ihu = input + nt; p.ihv = input + 2 * nt;
@@ -262,7 +262,7 @@ double emu_dt(int order, int64_t K, const double* state, const double* corners, 
     int epb = 64 / (int)nn;
     unsigned nblk = (unsigned)((K + epb - 1) / epb);
     // the geometry cache of the form (the time step reads its metric and J)
-    StageParams sp;
+    StageParams sp{};
     std::memset(&sp, 0, sizeof(sp));
     std::vector<double> bed(nt, 0.0), geo, fgeo;
     sp.corners = corners; sp.bed = bed.data(); sp.affine = affine; sp.links = links;
