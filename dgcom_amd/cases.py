@@ -470,14 +470,19 @@ class BoundaryTides:
     max_ratio: float = 0.0
 
 
+def face_node(n1, f, fi):
+    """The element node of face f's node fi (faces counter-clockwise from (-1, -1))."""
+    return (fi if f == 0 else fi * n1 + n1 - 1 if f == 1 else (n1 - 1) * n1 + (n1 - 1 - fi) if f == 2
+            else (n1 - 1 - fi) * n1)
+
+
 def open_face_nodes(case):
     """The open-boundary face nodes (k, f, fi) of a case, in mesh order, with
     their coordinates."""
     n1 = case.order + 1
     x, y = node_coordinates(case.mesh, case.order)
     x, y = x.reshape(case.n_elements, -1), y.reshape(case.n_elements, -1)
-    fnodes = [[fi if f == 0 else fi * n1 + n1 - 1 if f == 1 else (n1 - 1) * n1 + (n1 - 1 - fi) if f == 2
-               else (n1 - 1 - fi) * n1 for fi in range(n1)] for f in range(4)]
+    fnodes = [[face_node(n1, f, fi) for fi in range(n1)] for f in range(4)]
     out = []
     for k in range(case.n_elements):
         for f in range(4):
@@ -508,6 +513,77 @@ def boundary_tides(case, omega, f_rate, constants, ramp=0.0, velocity=False, max
             sd[i] = source_depth(x, y)
     return BoundaryTides(np.asarray(omega, dtype=np.float64), np.asarray(f_rate, dtype=np.float64), slot, coef, me,
                          sd, ramp, velocity, max_ratio)
+
+
+# ---------------------------------------------------------------------------
+# The nesting band (NestingRelaxation2D, src/boundary/ocean_nesting.rs:851-1060)
+# ---------------------------------------------------------------------------
+@dataclasses.dataclass
+class Nesting:
+    """A relaxation band towards a parent model: per node `slot` [K*NN]
+    (0xFFFFFFFF: not relaxed); the parent's snapshot `times` (Unix s) with
+    `epoch` the Unix time of model time 0, `cubic` time interpolation; per slot
+    `series` [n_times][3] (zeta, u, v; f32), band `weight` (1 on the open
+    boundary) and parent `depth` (NaN: unknown); the timescale (s), the wet
+    depth, momentum only, whether the parent has a velocity, the transport ratio
+    limit (0: unscaled), the datum shift and the ramp (s)."""
+    slot: np.ndarray
+    times: np.ndarray
+    series: np.ndarray
+    weight: np.ndarray
+    depth: np.ndarray
+    timescale: float = 3600.0
+    epoch: float = 0.0
+    cubic: bool = False
+    wet_depth: float = 0.05
+    momentum_only: bool = False
+    velocity: bool = False
+    max_ratio: float = 0.0
+    reference_level: float = 0.0
+    ramp: float = 0.0
+
+
+def cosine_profile(xi):
+    """SpongeProfile::Cosine (source/boundary/sponge.rs:67-81)."""
+    xi = min(max(xi, 0.0), 1.0)
+    return 0.5 * (1.0 - math.cos(math.pi * xi))
+
+
+def nesting_band(case, width, times, parent, parent_depth=None, profile=cosine_profile, **options) -> Nesting:
+    """A nesting band over the open (TIDE) boundary: its face nodes (weight 1)
+    and every node closer than `width` to one of its faces (weight
+    profile(1 - d / width), as OceanModelState::new, ocean_nesting.rs:333-391,
+    with d the distance to the nearest open face, straight between its corners);
+    `parent(x, y, t)` gives the parent's (zeta, u, v) at a node and snapshot
+    time, `parent_depth(x, y)` its depth."""
+    x, y = node_coordinates(case.mesh, case.order)
+    x, y = x.ravel(), y.ravel()
+    nn = case.nn
+    on = np.zeros(x.size, dtype=bool)
+    for k, f, fi, _, _ in open_face_nodes(case):
+        on[k * nn + face_node(case.order + 1, f, fi)] = True
+    segs = []
+    for k in range(case.n_elements):
+        for f in range(4):
+            if case.mesh.links[k, f] == BOUNDARY and case.mesh.tags[k, f] == TIDE:
+                segs.append((case.mesh.corners[k, f], case.mesh.corners[k, (f + 1) % 4]))
+    d = np.full(x.size, np.inf)
+    for a, b in segs:
+        ab = b - a
+        t = np.clip(((x - a[0]) * ab[0] + (y - a[1]) * ab[1]) / (ab @ ab), 0.0, 1.0)
+        d = np.minimum(d, np.hypot(x - (a[0] + t * ab[0]), y - (a[1] + t * ab[1])))
+    times = np.asarray(times, dtype=np.float64)
+    slot = np.full(x.size, 0xFFFFFFFF, dtype=np.uint32)
+    series, weight, depth = [], [], []
+    for n in range(x.size):
+        if not (on[n] or d[n] < width):
+            continue
+        slot[n] = len(weight)
+        weight.append(1.0 if on[n] else profile(1.0 - d[n] / width))
+        depth.append(float("nan") if parent_depth is None else parent_depth(x[n], y[n]))
+        series.append([parent(x[n], y[n], t) for t in times])
+    return Nesting(slot, times, np.asarray(series, dtype=np.float32).reshape(len(weight), times.size, 3),
+                   np.asarray(weight), np.asarray(depth), **options)
 
 
 def boundary_tides_of(case, tide: Tide) -> BoundaryTides:

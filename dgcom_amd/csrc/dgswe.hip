@@ -103,6 +103,14 @@ struct dg_ctx {
     double *tide_coef = nullptr, *tide_mean = nullptr, *tide_src = nullptr;
     int atlas_nc = 0, atlas_velocity = 0;
     double atlas_omega[8] = {}, atlas_f_rate[8] = {}, atlas_ramp = 0.0, atlas_max_ratio = 0.0;
+    // the nesting band (dg_set_nesting): device tables and the snapshot times on the host
+    uint32_t* nest_slot = nullptr;
+    float* nest_series = nullptr;
+    double *nest_weight = nullptr, *nest_depth = nullptr;
+    std::vector<double> nest_times;
+    int nest_velocity = 0, nest_momentum_only = 0, nest_cubic = 0;
+    double nest_epoch = 0.0;
+    double nest_rate = 0.0, nest_wet_depth = 0.05, nest_max_ratio = 0.0, nest_reference = 0.0, nest_ramp = 0.0;
     // stations
     int n_st = 0;
     int64_t* st_elem = nullptr;
@@ -170,7 +178,7 @@ static int free_statics(dg_ctx* c) {
                     c->queue_count, c->dt_slots, c->sub_list, c->chunk_total, c->gbed, c->gman, c->facebuf,
                     c->innerbuf, c->facecot, c->kind, c->st_elem, c->st_w, c->st_eta, c->geo, c->fgeo,
                     c->lts_level, c->lts_lrate, c->lts_order, c->depth_flags, c->tide_slot, c->tide_coef,
-                    c->tide_mean, c->tide_src};
+                    c->tide_mean, c->tide_src, c->nest_slot, c->nest_series, c->nest_weight, c->nest_depth};
     for (void* p : ptrs)
         if (p) DG_TRY(hipFree(p));
     c->gbed = c->gman = c->facebuf = c->innerbuf = c->facecot = nullptr;
@@ -179,6 +187,10 @@ static int free_statics(dg_ctx* c) {
     c->tide_slot = nullptr;
     c->tide_coef = c->tide_mean = c->tide_src = nullptr;
     c->atlas_nc = 0;
+    c->nest_slot = nullptr;
+    c->nest_series = nullptr;
+    c->nest_weight = c->nest_depth = nullptr;
+    c->nest_times.clear();
     c->lts_order = nullptr;
     c->lts_stages = 0;
     c->lts_finest = -1;
@@ -260,6 +272,61 @@ static void set_atlas(const dg_ctx* c, StageParams& p, double time) {
     }
 }
 
+// The nesting band's tables and the snapshot stencil at the stage time (OceanModelState::moment,
+// ocean_nesting.rs:673-682, with TimeStencil::new, io/field_series.rs:41-81, at the parent's clock
+// epoch + time): a single snapshot, or one hit exactly, alone; else the Lagrange weights over the
+// two bracketing snapshots (cubic: up to one more either side), and the ramp
+static int set_nesting(const dg_ctx* c, StageParams& p, double time) {
+    if (!c->nest_slot) return 0;
+    const std::vector<double>& ts = c->nest_times;
+    const size_t n = ts.size();
+    const double t = c->nest_epoch + time;
+    p.nest_len = 1;
+    p.nest_idx[0] = 0;
+    p.nest_w[0] = 1.0;
+    if (n > 1) {
+        DG_REQUIRE(t >= ts[0] && t <= ts[n - 1],
+                   "the nesting parent's snapshots do not cover the stage time (its forcing would freeze)");
+        size_t i1 = (size_t)(std::lower_bound(ts.begin(), ts.end(), t) - ts.begin());
+        i1 = std::min(std::max(i1, (size_t)1), n - 1);
+        const size_t i0 = i1 - 1;
+        if (t == ts[i0]) {
+            p.nest_idx[0] = (int)i0;
+        } else if (t == ts[i1]) {
+            p.nest_idx[0] = (int)i1;
+        } else {
+            const size_t lo = c->nest_cubic ? (i0 > 0 ? i0 - 1 : 0) : i0;
+            const size_t hi = c->nest_cubic ? std::min(i1 + 1, n - 1) : i1;
+            p.nest_len = (int)(hi - lo + 1);
+            for (size_t a = lo; a <= hi; ++a) {
+                double w = 1.0;
+                for (size_t b = lo; b <= hi; ++b)
+                    if (b != a) w *= (t - ts[b]) / (ts[a] - ts[b]);
+                p.nest_idx[a - lo] = (int)a;
+                p.nest_w[a - lo] = w;
+            }
+        }
+    }
+    p.nest_slot = c->nest_slot;
+    p.nest_series = c->nest_series;
+    p.nest_weight = c->nest_weight;
+    p.nest_depth = c->nest_depth;
+    p.nest_times = (int)n;
+    p.nest_velocity = c->nest_velocity;
+    p.nest_momentum_only = c->nest_momentum_only;
+    p.nest_rate = c->nest_rate;
+    p.nest_wet_depth = c->nest_wet_depth;
+    p.nest_max_ratio = c->nest_max_ratio;
+    p.nest_reference = c->nest_reference;
+    double rf = 1.0;
+    if (c->nest_ramp > 0.0) {
+        const double tau = std::min(std::max(time / c->nest_ramp, 0.0), 1.0);
+        rf = tau * tau * (3.0 - 2.0 * tau);
+    }
+    p.nest_ramp = rf;
+    return 0;
+}
+
 static int launch_stage(dg_ctx* c, const double* base, const double* input, double* out, double a, double b, double cc,
                         double time, int rhs_only, double* fstar_out = nullptr, double* rhs_save = nullptr) {
     size_t nt = (size_t)c->K * c->nn;
@@ -286,6 +353,7 @@ static int launch_stage(dg_ctx* c, const double* base, const double* input, doub
     p.queue = c->queue;
     p.queue_count = c->queue_count;
     set_atlas(c, p, time);
+    if (int r = set_nesting(c, p, time)) return r;
     const int epb = 64 / c->nn;
     const unsigned int nblk = (unsigned int)((c->K + epb - 1) / epb);
     const unsigned int nchunk = (nblk + 1023) / 1024;
@@ -429,6 +497,7 @@ static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, 
                             int input_write = 0) {
     DG_REQUIRE(!(c->flags & DG_ENTROPY_STABLE), "the EntropyStable faces have no VJP here (the adjoint is of the WetDry faces)");
     DG_REQUIRE(!c->tide_slot, "the boundary tides per node have no VJP here (the adjoint is of the uniform tide)");
+    DG_REQUIRE(!c->nest_slot, "the nesting band has no VJP here");
     if (int r = ensure_adjoint(c)) return r;
     size_t nt = (size_t)c->K * c->nn;
     VjpParams P{};
@@ -1137,6 +1206,7 @@ int dg_stage_where(dg_ctx* c, const dg_state* base, const dg_state* input, const
     DG_REQUIRE(c && base && input && plan && out && c->K > 0, "dg_stage_where: null argument or no form");
     DG_REQUIRE(c->lts_order, "dg_stage_where: no coarse step begun (dg_begin_coarse_step)");
     DG_REQUIRE(!c->tide_slot, "dg_stage_where: the boundary tides per node run with global steps only");
+    DG_REQUIRE(!c->nest_slot, "dg_stage_where: the nesting band runs with global steps only");
     DG_REQUIRE(plan->stage >= 0 && plan->stage < c->lts_stages, "dg_stage_where: stage out of range");
     DG_REQUIRE(plan->finest >= c->lts_finest && plan->finest <= DG_MAX_LEVELS,
                "dg_stage_where: the plan's finest level is not the coarse step's");
@@ -1455,6 +1525,48 @@ int dg_set_boundary_tides(dg_ctx* c, int n_c, const double* omega, const double*
     c->atlas_ramp = ramp;
     c->atlas_velocity = velocity ? 1 : 0;
     c->atlas_max_ratio = max_ratio;
+    return 0;
+}
+
+int dg_set_nesting(dg_ctx* c, int64_t n_slots, const uint32_t* slot_of_node, int n_times, const double* times,
+                   double epoch, int cubic, const float* series, const double* weight, const double* parent_depth, double timescale,
+                   double wet_depth, int momentum_only, int velocity, double max_ratio, double reference_level,
+                   double ramp) {
+    DG_REQUIRE(c && c->K > 0, "dg_set_nesting: null context or no form");
+    DG_TRY(hipSetDevice(c->device));
+    DG_TRY(hipStreamSynchronize(c->stream));
+    for (void* ptr : {(void*)c->nest_slot, (void*)c->nest_series, (void*)c->nest_weight, (void*)c->nest_depth})
+        if (ptr) DG_TRY(hipFree(ptr));
+    c->nest_slot = nullptr;
+    c->nest_series = nullptr;
+    c->nest_weight = c->nest_depth = nullptr;
+    c->nest_times.clear();
+    if (n_slots == 0) return 0;
+    DG_REQUIRE(slot_of_node && times && series && weight && parent_depth && n_slots > 0 && n_times >= 1,
+               "dg_set_nesting: missing tables");
+    DG_REQUIRE(timescale > 0.0, "dg_set_nesting: the relaxation timescale must be positive");
+    for (int i = 1; i < n_times; ++i) DG_REQUIRE(times[i] > times[i - 1], "dg_set_nesting: snapshot times must increase");
+    const size_t nt = (size_t)c->K * c->nn;
+    for (size_t i = 0; i < nt; ++i)
+        DG_REQUIRE(slot_of_node[i] == 0xFFFFFFFFu || (int64_t)slot_of_node[i] < n_slots, "dg_set_nesting: a slot out of range");
+    DG_TRY(hipMalloc(&c->nest_slot, nt * sizeof(uint32_t)));
+    DG_TRY(hipMalloc(&c->nest_series, (size_t)n_slots * n_times * 3 * sizeof(float)));
+    DG_TRY(hipMalloc(&c->nest_weight, (size_t)n_slots * sizeof(double)));
+    DG_TRY(hipMalloc(&c->nest_depth, (size_t)n_slots * sizeof(double)));
+    DG_TRY(hipMemcpy(c->nest_slot, slot_of_node, nt * sizeof(uint32_t), hipMemcpyHostToDevice));
+    DG_TRY(hipMemcpy(c->nest_series, series, (size_t)n_slots * n_times * 3 * sizeof(float), hipMemcpyHostToDevice));
+    DG_TRY(hipMemcpy(c->nest_weight, weight, (size_t)n_slots * sizeof(double), hipMemcpyHostToDevice));
+    DG_TRY(hipMemcpy(c->nest_depth, parent_depth, (size_t)n_slots * sizeof(double), hipMemcpyHostToDevice));
+    c->nest_times.assign(times, times + n_times);
+    c->nest_epoch = epoch;
+    c->nest_cubic = cubic ? 1 : 0;
+    c->nest_rate = 1.0 / timescale;
+    c->nest_wet_depth = wet_depth;
+    c->nest_momentum_only = momentum_only ? 1 : 0;
+    c->nest_velocity = velocity ? 1 : 0;
+    c->nest_max_ratio = max_ratio;
+    c->nest_reference = reference_level;
+    c->nest_ramp = ramp;
     return 0;
 }
 
@@ -1892,6 +2004,7 @@ int dg_batch_create(int n, dg_ctx* const* members, int scheme, dg_batch** out) {
         DG_REQUIRE(members[m] && members[m]->K > 0, "dg_batch_create: every member needs an uploaded form");
         DG_REQUIRE(members[m]->device == members[0]->device, "dg_batch_create: members on different devices");
         DG_REQUIRE(!members[m]->tide_slot, "dg_batch_create: a member with boundary tides per node (not batched)");
+        DG_REQUIRE(!members[m]->nest_slot, "dg_batch_create: a member with a nesting band (not batched)");
     }
     DG_TRY(hipSetDevice(members[0]->device));
     dg_batch* b = new dg_batch;

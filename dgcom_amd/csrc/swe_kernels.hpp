@@ -123,6 +123,16 @@ struct StageParams {
     int atlas_nc, atlas_velocity;
     double atlas_max_ratio;
     double atlas_moment[1 + 3 * ATLAS_MAX_CONSTITUENTS];
+    // The nesting band (NestingRelaxation2D, ocean_nesting.rs:890-1060): every
+    // node's slot (UINT32_MAX: none), per slot its snapshots' (zeta, u, v) [n_times][3]
+    // (f32), band weight and parent depth; the stage time's snapshot stencil and ramp
+    const uint32_t* nest_slot;
+    const float* nest_series;
+    const double *nest_weight, *nest_depth;
+    int nest_times, nest_len, nest_velocity, nest_momentum_only;
+    int nest_idx[4];
+    double nest_w[4];
+    double nest_rate, nest_wet_depth, nest_max_ratio, nest_reference, nest_ramp;
 };
 
 // LDS exchange among the lanes of a workgroup that is one wavefront: the
@@ -711,6 +721,27 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
         if (p.f0 != 0.0) {
             rhs.hu += p.f0 * hvin;
             rhs.hv += (-p.f0) * huin;
+        }
+        if (p.nest_slot) {
+            // the nesting band's relaxation, after Coriolis (swe_2d.rs:929-973)
+            const uint32_t slot = p.nest_slot[n];
+            if (slot != 0xFFFFFFFFu) {
+                double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (q >= p.nest_len) break;
+                    const float* v = p.nest_series + ((size_t)slot * p.nest_times + p.nest_idx[q]) * 3;
+                    s0 += p.nest_w[q] * (double)v[0];
+                    s1 += p.nest_w[q] * (double)v[1];
+                    s2 += p.nest_w[q] * (double)v[2];
+                }
+                double eta, pu, pv;
+                bool hv_;
+                parent_state(s0, s1, s2, p.nest_ramp, p.nest_reference, p.nest_velocity != 0, p.nest_max_ratio,
+                             p.nest_depth[slot], bin, eta, hv_, pu, pv);
+                rhs = cadd(rhs, relaxation_source(p.nest_weight[slot], p.nest_rate, p.nest_wet_depth,
+                                                  p.nest_momentum_only != 0, hin, huin, hvin, bin, eta, hv_, pu, pv));
+            }
         }
         if (p.rh) { p.rh[n] = rhs.h; p.rhu[n] = rhs.hu; p.rhv[n] = rhs.hv; }
     }

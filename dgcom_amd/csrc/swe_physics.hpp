@@ -69,6 +69,43 @@ DG_HD Cons bed_interface(double h, double b, double b_out, double nx, double ny,
     return {0.0, force * nx, force * ny};
 }
 
+// The nesting band's parent at a node (parent_state, ocean_nesting.rs:851-880):
+// the ramped snapshot sum (zeta, u, v), the datum shift ramped with it, and with
+// a parent velocity, that velocity scaled by D_parent / D_child (limited either
+// way by max_ratio > 0, where the parent depth is known)
+DG_HD void parent_state(double s0, double s1, double s2, double ramp, double reference_level, bool velocity,
+                        double max_ratio, double parent_depth, double bed, double& eta, bool& has_velocity,
+                        double& um, double& vm) {
+    const double zeta = ramp * s0, u = ramp * s1, v = ramp * s2;
+    eta = zeta + ramp * reference_level;
+    has_velocity = velocity;
+    um = u;
+    vm = v;
+    if (velocity && max_ratio > 0.0 && isfinite(parent_depth)) {
+        const double d_parent = zeta + parent_depth, d_child = eta - bed;
+        if (d_parent > 0.0 && d_child > 0.0) {
+            const double ratio = fmin_(fmax_(d_parent / d_child, 1.0 / max_ratio), max_ratio);
+            um *= ratio;
+            vm *= ratio;
+        }
+    }
+}
+
+// The flow-relaxation source towards the parent (relaxation_source,
+// ocean_nesting.rs:962-988): gamma = weight / tau times the parent's depth and
+// transport less the node's, nothing where either is shallower than wet_depth
+DG_HD Cons relaxation_source(double weight, double rate, double wet_depth, bool momentum_only, double qh, double qhu,
+                             double qhv, double bed, double eta, bool has_velocity, double pu, double pv) {
+    if (weight <= 0.0 || qh < wet_depth) return {0.0, 0.0, 0.0};
+    const double h = eta - bed;
+    if (h < wet_depth) return {0.0, 0.0, 0.0};
+    const double gamma = weight * rate;
+    const double u = has_velocity ? pu : qhu / qh, v = has_velocity ? pv : qhv / qh;
+    const double dh = momentum_only ? 0.0 : h - qh;
+    const double depth = momentum_only ? qh : h;
+    return {gamma * dh, gamma * (depth * u - qhu), gamma * (depth * v - qhv)};
+}
+
 // hll_flux_face_aligned + einfeldt_speeds_2d (flux/swe_2d.rs:277-374), with the
 // sides in face-aligned components (depth, normal / tangential momentum and velocity)
 struct Hll { double h, hun, hut, un, ut; };

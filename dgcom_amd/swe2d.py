@@ -40,6 +40,7 @@ EXPORTS = (
     "dg_batch_tuning", "dg_begin_coarse_step", "dg_stage_where", "dg_finish_where", "dg_depth_flags",
     "dg_multirate_create", "dg_multirate_destroy", "dg_multirate_assign_levels", "dg_multirate_step",
     "dg_multirate_levels", "dg_multirate_stats_read", "dg_multirate_run", "dg_fnv1a", "dg_set_boundary_tides",
+    "dg_set_nesting",
 )
 BATCH_GRAPH, BATCH_TUNE = 1, 2
 
@@ -148,6 +149,9 @@ def load(fma: bool = False):
     L.dg_depth_flags.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u8p]
     L.dg_set_boundary_tides.argtypes = [vp, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                         ctypes.c_int64, ctypes.POINTER(ctypes.c_uint32), _dp, _dp, _dp]
+    L.dg_set_nesting.argtypes = ([vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int, _dp,
+                                  ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_float), _dp, _dp]
+                                 + [ctypes.c_double] * 2 + [ctypes.c_int] * 2 + [ctypes.c_double] * 3)
     L.dg_fnv1a.argtypes = [_u8p, ctypes.c_size_t, ctypes.c_uint64]
     L.dg_fnv1a.restype = ctypes.c_uint64
     L.dg_multirate_create.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
@@ -299,6 +303,24 @@ class SWEPhysics2D:
                                                         bt.max_ratio, me.size,
                                                         sl.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _p(co),
                                                         _p(me), _p(sd)), "dg_set_boundary_tides")
+
+    def set_nesting(self, ns):
+        """A relaxation band towards a parent model (a `cases.Nesting`), or None to clear it."""
+        if ns is None:
+            _check(self.lib, self.lib.dg_set_nesting(self.ctx, 0, None, 0, None, 0.0, 0, None, None, None, 1.0, 0.05,
+                                                     0, 0, 0.0, 0.0, 0.0), "dg_set_nesting")
+            return
+        sl = np.ascontiguousarray(ns.slot, dtype=np.uint32)
+        ti = np.ascontiguousarray(ns.times, dtype=np.float64)
+        se = np.ascontiguousarray(ns.series, dtype=np.float32)
+        we = np.ascontiguousarray(ns.weight, dtype=np.float64)
+        de = np.ascontiguousarray(ns.depth, dtype=np.float64)
+        _check(self.lib, self.lib.dg_set_nesting(self.ctx, we.size, sl.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                                 ti.size, _p(ti), ns.epoch, int(ns.cubic),
+                                                 se.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), _p(we), _p(de),
+                                                 ns.timescale, ns.wet_depth, int(ns.momentum_only),
+                                                 int(ns.velocity), ns.max_ratio, ns.reference_level, ns.ramp),
+               "dg_set_nesting")
 
     def close(self):
         if self.ctx:

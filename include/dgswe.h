@@ -116,6 +116,22 @@ int dg_set_boundary_tides(dg_ctx* ctx, int n_c, const double* omega, const doubl
                           double max_ratio, int64_t n_slots, const uint32_t* slot_of_face_node, const double* coef,
                           const double* mean, const double* source_depth);
 
+/* The nesting band (NestingRelaxation2D towards an OceanModelState parent, src/boundary/ocean_nesting.rs:
+ * 851-1060; the reference device's moment-and-sums form, notes/gpu.md:199-201): every node's slot
+ * slot_of_node[K*NN] (UINT32_MAX: not relaxed); the parent's snapshots at times[n_times] (Unix seconds,
+ * increasing; model time t is epoch + t, ModelClock::unix, time/clock.rs:67), interpolated linearly or
+ * (cubic) by Lagrange over the four nearest (TimeStencil::new, io/field_series.rs:41); per slot series[n_slots][n_times][3] = (zeta, u, v) (f32), the band weight (1 on the boundary,
+ * 0 outside the band) and the parent's still-water depth (NaN: unknown); relaxation 1/timescale, nothing
+ * where either depth is below wet_depth (0.05 m); momentum_only; velocity: the parent has a depth-mean
+ * velocity (else the node's own); max_ratio > 0: that velocity scaled to the parent's transport; the datum
+ * shift reference_level; the ramp (s, <= 0: none). A stage time outside the snapshots is an error (a
+ * single snapshot covers every time). n_slots = 0 clears it. Global steps only: local time stepping, the
+ * ensembles and the VJP refuse it. */
+int dg_set_nesting(dg_ctx* ctx, int64_t n_slots, const uint32_t* slot_of_node, int n_times, const double* times,
+                   double epoch, int cubic, const float* series, const double* weight, const double* parent_depth, double timescale,
+                   double wet_depth, int momentum_only, int velocity, double max_ratio, double reference_level,
+                   double ramp);
+
 /* Device states (DeviceState::upload / download, gpu/src/swe2d.rs:2384-2396). */
 int dg_state_alloc(dg_ctx* ctx, dg_state** state);
 int dg_state_free(dg_ctx* ctx, dg_state* state);

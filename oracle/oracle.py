@@ -116,6 +116,9 @@ def _bind(L):
         L.oc_finish_where.argtypes = [ctypes.c_void_p] + [_dp] * 6 + [_u32p, ctypes.c_int64]
         L.oc_set_boundary_tides.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int,
                                             ctypes.c_double, ctypes.c_int64, _u32p, _dp, _dp, _dp]
+    L.oc_set_nesting.argtypes = ([ctypes.c_void_p, ctypes.c_int64, _u32p, ctypes.c_int, _dp, ctypes.c_double,
+                                  ctypes.c_int, ctypes.POINTER(ctypes.c_float), _dp, _dp]
+                                 + [ctypes.c_double] * 2 + [ctypes.c_int] * 2 + [ctypes.c_double] * 3)
     return L
 
 
@@ -190,6 +193,17 @@ class Oracle:
         om, fr, co, me, sd = self._bt
         self._L.oc_set_boundary_tides(self.h, len(om), _p(om), _p(fr), bt.ramp, int(bt.velocity), bt.max_ratio,
                                       me.size, self._bt_slot.ctypes.data_as(_u32p), _p(co), _p(me), _p(sd))
+
+    def set_nesting(self, ns):
+        """The relaxation band towards a parent (a `cases.Nesting`)."""
+        self._ns = (np.ascontiguousarray(ns.slot, dtype=np.uint32), np.ascontiguousarray(ns.times, dtype=np.float64),
+                    np.ascontiguousarray(ns.series, dtype=np.float32), np.ascontiguousarray(ns.weight, dtype=np.float64),
+                    np.ascontiguousarray(ns.depth, dtype=np.float64))
+        sl, ti, se, we, de = self._ns
+        self._L.oc_set_nesting(self.h, we.size, sl.ctypes.data_as(_u32p), ti.size, _p(ti), ns.epoch, int(ns.cubic),
+                               se.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), _p(we), _p(de), ns.timescale,
+                               ns.wet_depth, int(ns.momentum_only), int(ns.velocity), ns.max_ratio,
+                               ns.reference_level, ns.ramp)
 
     def tide_elevation(self, t):
         return self._L.oc_tide_elevation(self.h, t)

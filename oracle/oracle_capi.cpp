@@ -143,6 +143,30 @@ void oc_set_boundary_tides(void* p, int n_c, const double* omega, const double* 
     a.source_depth.assign(source_depth, source_depth + n_slots);
 }
 
+void oc_set_nesting(void* p, int64_t n_slots, const uint32_t* slot, int n_times, const double* times, double epoch,
+                    int cubic, const float* series, const double* weight, const double* depth, double timescale,
+                    double wet_depth, int momentum_only, int velocity, double max_ratio, double reference_level,
+                    double ramp) {
+    Solver<double>& s = *((Handle*)p)->s;
+    auto& ns = s.nesting;
+    ns.on = n_slots > 0;
+    if (!ns.on) return;
+    ns.slot.assign(slot, slot + (size_t)s.K * s.ops.nn);
+    ns.times.assign(times, times + n_times);
+    ns.series.assign(series, series + (size_t)n_slots * n_times * 3);
+    ns.weight.assign(weight, weight + n_slots);
+    ns.depth.assign(depth, depth + n_slots);
+    ns.epoch = epoch;
+    ns.cubic = cubic != 0;
+    ns.rate = 1.0 / timescale;
+    ns.wet_depth = wet_depth;
+    ns.momentum_only = momentum_only != 0;
+    ns.velocity = velocity != 0;
+    ns.max_ratio = max_ratio;
+    ns.reference_level = reference_level;
+    ns.ramp = ramp;
+}
+
 void oc_step(void* p, double* h, double* hu, double* hv, double t, double dt, int scheme) {
     Handle* hd = (Handle*)p;
     hd->s->step(h, hu, hv, t, dt, scheme, hd->work);

@@ -4,6 +4,8 @@
 // the CPU with OpenMP over elements.
 #pragma once
 
+#include <cstdio>
+#include <cstdlib>
 #include "swe2d_oracle.hpp"
 
 #ifdef _OPENMP
@@ -64,6 +66,17 @@ struct Solver {
         double ramp = 0.0, max_ratio = 0.0;
         bool velocity = false;
     } atlas;
+    // NestingRelaxation2D (boundary/ocean_nesting.rs:891-1032): per node [K][nn] its slot (UINT32_MAX:
+    // outside the band), the parent's snapshots (Unix s) and per slot [n_times][3] (zeta, u, v) as f32,
+    // the band weight and the parent's depth
+    struct Nesting {
+        bool on = false;
+        std::vector<uint32_t> slot;
+        std::vector<double> times, weight, depth;
+        std::vector<float> series;
+        double epoch = 0.0, rate = 0.0, wet_depth = 0.05, max_ratio = 0.0, reference_level = 0.0, ramp = 0.0;
+        bool cubic = false, momentum_only = false, velocity = false;
+    } nesting;
     int64_t K = 0;
     std::vector<double> corners;
     std::vector<uint32_t> links;
@@ -87,6 +100,74 @@ struct Solver {
     NodeState<T> node(const T* h, const T* hu, const T* hv, int64_t k, int i) const {
         size_t n = (size_t)k * ops.nn + i;
         return node_state(h[n], hu[n], hv[n], bed[n], cfg.h_min);
+    }
+
+    // TimeStencil::new (io/field_series.rs:41-81) at Unix time t: the snapshots and their weights
+    int nesting_stencil(double t, size_t idx[4], double w[4]) const {
+        const std::vector<double>& ts = nesting.times;
+        const size_t n = ts.size();
+        idx[0] = 0;
+        w[0] = 1.0;
+        if (n == 1) return 1;
+        if (!(t >= ts[0] && t <= ts[n - 1])) {
+            // the reference panics here (OceanModelState::moment_at)
+            std::fprintf(stderr, "nesting: time %.17g outside the parent's snapshots\n", t);
+            std::abort();
+        }
+        size_t i1 = 0;
+        while (i1 < n && ts[i1] < t) ++i1;
+        i1 = std::min(std::max(i1, (size_t)1), n - 1);
+        size_t i0 = i1 - 1;
+        if (t == ts[i0]) { idx[0] = i0; return 1; }
+        if (t == ts[i1]) { idx[0] = i1; return 1; }
+        size_t lo = nesting.cubic ? (i0 == 0 ? 0 : i0 - 1) : i0;
+        size_t hi = nesting.cubic ? std::min(i1 + 1, n - 1) : i1;
+        for (size_t a = lo; a <= hi; ++a) {
+            double p = 1.0;
+            for (size_t b = lo; b <= hi; ++b)
+                if (b != a) p = p * ((t - ts[b]) / (ts[a] - ts[b]));
+            idx[a - lo] = a;
+            w[a - lo] = p;
+        }
+        return (int)(hi - lo + 1);
+    }
+
+    // relaxation_source with evaluate / parent_term / parent_state (ocean_nesting.rs:583-600, 837-880,
+    // 962-988) at a node of slot `slot`, state (qh, qhu, qhv) over bed `b`
+    State<double> nesting_source(uint32_t slot, int len, const size_t* idx, const double* w, double ramp,
+                                 double qh, double qhu, double qhv, double b) const {
+        const Nesting& ns = nesting;
+        const double weight = ns.weight[slot];
+        if (weight <= 0.0 || qh < ns.wet_depth) return {0.0, 0.0, 0.0};
+        double sum[3] = {0.0, 0.0, 0.0};
+        for (int q = 0; q < len; ++q) {
+            const float* v = ns.series.data() + ((size_t)slot * ns.times.size() + idx[q]) * 3;
+            for (int c = 0; c < 3; ++c) sum[c] += w[q] * (double)v[c];
+        }
+        const double zeta = ramp * sum[0];
+        const double eta = zeta + ramp * ns.reference_level;
+        double u = qhu / qh, v = qhv / qh;
+        if (ns.velocity) {
+            u = ramp * sum[1];
+            v = ramp * sum[2];
+            const double pd = ns.depth[slot];
+            if (ns.max_ratio > 0.0 && std::isfinite(pd)) {
+                const double dp = zeta + pd, dc = eta - b;
+                if (dp > 0.0 && dc > 0.0) {
+                    double r = dp / dc;
+                    if (r < 1.0 / ns.max_ratio) r = 1.0 / ns.max_ratio;
+                    if (r > ns.max_ratio) r = ns.max_ratio;
+                    u *= r;
+                    v *= r;
+                }
+            }
+        }
+        const double h = eta - b;
+        if (h < ns.wet_depth) return {0.0, 0.0, 0.0};
+        const double gamma = weight * ns.rate;
+        const double dh = ns.momentum_only ? 0.0 : h - qh;
+        const double depth = ns.momentum_only ? qh : h;
+        return {gamma * dh, gamma * (depth * u - qhu), gamma * (depth * v - qhv)};
     }
 
     // The face normal at face node fi (the per-node field; a parallelogram's is its node-0 value)
@@ -436,6 +517,25 @@ struct Solver {
                 size_t n = (size_t)k * nn + i;
                 ohu[n] += T(cfg.f0) * hv[n];
                 ohv[n] += T(-cfg.f0) * hu[n];
+            }
+        }
+        if (nesting.on) {
+            // NestingRelaxation2D::add_element (ocean_nesting.rs:1003-1029), at the parent's moment
+            size_t idx[4];
+            double w[4];
+            const int len = nesting_stencil(nesting.epoch + time, idx, w);
+            double ramp = 1.0;
+            if (nesting.ramp > 0.0) {
+                double tau = std::min(std::max(time / nesting.ramp, 0.0), 1.0);
+                ramp = tau * tau * (3.0 - 2.0 * tau);
+            }
+            for (int i = 0; i < nn; ++i) {
+                size_t n = (size_t)k * nn + i;
+                const uint32_t slot = nesting.slot[n];
+                if (slot == 0xFFFFFFFFu) continue;
+                State<double> src = nesting_source(slot, len, idx, w, ramp, val(h[n]), val(hu[n]), val(hv[n]),
+                                                   val(bed[n]));
+                oh[n] += T(src.h); ohu[n] += T(src.hu); ohv[n] += T(src.hv);
             }
         }
         if (cfg.manufactured) {

@@ -1,0 +1,74 @@
+"""The nesting band on the GPU against the restatement: whole adaptive runs bit
+for bit with a band over the open boundary relaxing towards a parent that
+varies in space and time (elevation only with linear snapshots; a velocity with
+its transport scaling, the datum shift, the ramp and cubic snapshots; momentum
+only), and a stage time the snapshots do not cover, local time stepping, the
+ensembles and the VJP refused by name."""
+import numpy as np
+import pytest
+
+import oracle as O
+from dgcom_amd import cases, multirate as M, swe2d
+
+pytestmark = pytest.mark.gpu
+
+T0 = 1_706_594_400.0
+
+
+def _band(case, **options):
+    times = T0 + np.arange(-600.0, 4 * 3600.0, 600.0)
+
+    def parent(x, y, t):
+        ph = 2.0 * np.pi * (t - T0) / 3600.0 + y / 300.0
+        return 0.05 * np.sin(ph), 0.1 * np.cos(ph), 0.02 + 0.01 * np.sin(x / 50.0)
+
+    return cases.nesting_band(case, 200.0, times, parent, parent_depth=lambda x, y: 1.5 + 0.002 * y, epoch=T0,
+                              **options)
+
+
+VARIANTS = [
+    ("elevation", dict(timescale=300.0)),
+    ("velocity", dict(timescale=120.0, velocity=True, max_ratio=2.0, reference_level=0.03, ramp=50.0, cubic=True)),
+    ("momentum", dict(timescale=200.0, velocity=True, momentum_only=True, wet_depth=0.1)),
+]
+
+
+@pytest.mark.parametrize("name,options", VARIANTS, ids=[v[0] for v in VARIANTS])
+def test_runs_equal_the_oracle(name, options):
+    c = cases.drying_beach(nx=16, ny=6, jitter=0.15)
+    ns = _band(c, **options)
+    assert (ns.slot != 0xFFFFFFFF).sum() > c.nn * 6          # more than the column of elements at the boundary
+    orc = O.Oracle.from_case(c)
+    orc.set_nesting(ns)
+    phys = swe2d.SWEPhysics2D.from_case(c)
+    phys.set_nesting(ns)
+    s = phys.state(c.h, c.hu, c.hv)
+    x = [c.h.copy(), c.hu.copy(), c.hv.copy()]
+    t_g = phys.run(s, 0.0, 60, c.cfl)
+    t_o = orc.run(*x, 0.0, 60, c.cfl)
+    assert t_g == t_o
+    for g, r in zip(s.download(), x):
+        assert np.array_equal(g, r)
+    assert phys.negative_depth_clips() == orc.clips()
+    out = phys.state()
+    phys.compute_rhs_into(s, t_g, out)
+    for a, b in zip(out.download(), orc.rhs(*x, t_g)):
+        assert np.array_equal(a, b)
+
+
+def test_refusals():
+    c = cases.drying_beach(nx=8, ny=3)
+    phys = swe2d.SWEPhysics2D.from_case(c)
+    phys.set_nesting(_band(c, timescale=300.0))
+    s = phys.state(c.h, c.hu, c.hv)
+    with pytest.raises(RuntimeError, match="do not cover"):
+        phys.step(s, 5.0 * 3600.0, 0.1)
+    with pytest.raises(RuntimeError, match="global steps only"):
+        M.run(phys.local_time_stepping(), M.neighbours(c.mesh.links), s, 0.0, 5.0, c.cfl, 2)
+    w, gb, gi = phys.state(c.h, c.hu, c.hv), phys.state().zero(), phys.state().zero()
+    with pytest.raises(RuntimeError, match="no VJP"):
+        phys.stage_vjp(s, s, 1.0, 0.0, 1.0, 0.0, w, gb, gi)
+    with pytest.raises(RuntimeError, match="nesting"):
+        swe2d.Ensemble([phys])
+    phys.set_nesting(None)
+    phys.step(s, 5.0 * 3600.0, 0.1)
