@@ -586,6 +586,48 @@ def nesting_band(case, width, times, parent, parent_depth=None, profile=cosine_p
                    np.asarray(weight), np.asarray(depth), **options)
 
 
+# ---------------------------------------------------------------------------
+# Gridded weather (GriddedAtmosphere2D, src/source/swe_2d/gridded_atmosphere.rs)
+# ---------------------------------------------------------------------------
+DRAG_CONSTANT, DRAG_LARGE_POND, DRAG_WU, DRAG_SMITH, DRAG_YELLAND_TAYLOR = 0, 1, 2, 3, 4
+
+
+@dataclasses.dataclass
+class Weather:
+    """Weather regridded to the nodes: snapshot `times` (Unix s) with `epoch`
+    the Unix time of model time 0, `cubic` time interpolation; `series`
+    [n_times][5][K*NN] f32 of u, v (m/s, mesh axes), dp/dx, dp/dy (Pa/m) and
+    p - p_ref (Pa); the wind and pressure switches, the drag coefficient (a
+    DRAG_* kind, `cd` for DRAG_CONSTANT), the densities, the depth below which a
+    node is not forced and the ramp (s)."""
+    times: np.ndarray
+    series: np.ndarray
+    epoch: float = 0.0
+    cubic: bool = False
+    wind: bool = True
+    pressure: bool = True
+    drag: int = DRAG_LARGE_POND
+    cd: float = 0.0
+    rho_air: float = 1.225
+    rho_water: float = 1025.0
+    h_min: float = 0.01
+    ramp: float = 0.0
+
+
+def weather_at_nodes(case, times, fields, **options) -> Weather:
+    """Weather from `fields(x, y, t)` -> (u, v, dp/dx, dp/dy, p - p_ref), each an
+    array over the nodes, at every snapshot time (as GriddedAtmosphere2D::
+    regridded holds a weather model's fields at the nodes)."""
+    x, y = node_coordinates(case.mesh, case.order)
+    x, y = x.ravel(), y.ravel()
+    times = np.asarray(times, dtype=np.float64)
+    series = np.empty((times.size, 5, x.size), dtype=np.float32)
+    for j, t in enumerate(times):
+        for f, a in enumerate(fields(x, y, t)):
+            series[j, f] = a
+    return Weather(times, series, **options)
+
+
 def boundary_tides_of(case, tide: Tide) -> BoundaryTides:
     """A uniform harmonic tide written as boundary tides (elevation only): per
     constituent C = f0 A cos(phi0), S = -f0 A sin(phi0) with phi0 = phase +

@@ -110,6 +110,11 @@ struct dg_ctx {
     std::vector<double> nest_times;
     int nest_velocity = 0, nest_momentum_only = 0, nest_cubic = 0;
     double nest_epoch = 0.0;
+    // the gridded weather (dg_set_weather): the snapshots on the device, their times on the host
+    float* wx_series = nullptr;
+    std::vector<double> wx_times;
+    double wx_epoch = 0.0, wx_ramp = 0.0, wx_cd = 0.0, wx_rho_air = 1.225, wx_rho_water = 1025.0, wx_h_min = 0.01;
+    int wx_cubic = 0, wx_wind = 0, wx_pressure = 0, wx_drag = 1;
     double nest_rate = 0.0, nest_wet_depth = 0.05, nest_max_ratio = 0.0, nest_reference = 0.0, nest_ramp = 0.0;
     // stations
     int n_st = 0;
@@ -178,7 +183,8 @@ static int free_statics(dg_ctx* c) {
                     c->queue_count, c->dt_slots, c->sub_list, c->chunk_total, c->gbed, c->gman, c->facebuf,
                     c->innerbuf, c->facecot, c->kind, c->st_elem, c->st_w, c->st_eta, c->geo, c->fgeo,
                     c->lts_level, c->lts_lrate, c->lts_order, c->depth_flags, c->tide_slot, c->tide_coef,
-                    c->tide_mean, c->tide_src, c->nest_slot, c->nest_series, c->nest_weight, c->nest_depth};
+                    c->tide_mean, c->tide_src, c->nest_slot, c->nest_series, c->nest_weight, c->nest_depth,
+                    c->wx_series};
     for (void* p : ptrs)
         if (p) DG_TRY(hipFree(p));
     c->gbed = c->gman = c->facebuf = c->innerbuf = c->facecot = nullptr;
@@ -191,6 +197,8 @@ static int free_statics(dg_ctx* c) {
     c->nest_series = nullptr;
     c->nest_weight = c->nest_depth = nullptr;
     c->nest_times.clear();
+    c->wx_series = nullptr;
+    c->wx_times.clear();
     c->lts_order = nullptr;
     c->lts_stages = 0;
     c->lts_finest = -1;
@@ -272,41 +280,48 @@ static void set_atlas(const dg_ctx* c, StageParams& p, double time) {
     }
 }
 
-// The nesting band's tables and the snapshot stencil at the stage time (OceanModelState::moment,
-// ocean_nesting.rs:673-682, with TimeStencil::new, io/field_series.rs:41-81, at the parent's clock
-// epoch + time): a single snapshot, or one hit exactly, alone; else the Lagrange weights over the
-// two bracketing snapshots (cubic: up to one more either side), and the ramp
+// TimeStencil::new (io/field_series.rs:41-81) at time t over increasing snapshot times: a single snapshot, or
+// one hit exactly, alone; else the Lagrange weights of the two bracketing snapshots (cubic: up to one more
+// either side), each product taken in the reference's order. False outside the snapshots.
+static bool time_stencil(const std::vector<double>& ts, double t, bool cubic, int& len, int* idx, double* w) {
+    const size_t n = ts.size();
+    len = 1;
+    idx[0] = 0;
+    w[0] = 1.0;
+    if (n == 1) return true;
+    if (!(t >= ts[0] && t <= ts[n - 1])) return false;
+    size_t i1 = (size_t)(std::lower_bound(ts.begin(), ts.end(), t) - ts.begin());
+    i1 = std::min(std::max(i1, (size_t)1), n - 1);
+    const size_t i0 = i1 - 1;
+    if (t == ts[i0]) { idx[0] = (int)i0; return true; }
+    if (t == ts[i1]) { idx[0] = (int)i1; return true; }
+    const size_t lo = cubic ? (i0 > 0 ? i0 - 1 : 0) : i0;
+    const size_t hi = cubic ? std::min(i1 + 1, n - 1) : i1;
+    len = (int)(hi - lo + 1);
+    for (size_t a = lo; a <= hi; ++a) {
+        double wa = 1.0;
+        for (size_t b = lo; b <= hi; ++b)
+            if (b != a) wa *= (t - ts[b]) / (ts[a] - ts[b]);
+        idx[a - lo] = (int)a;
+        w[a - lo] = wa;
+    }
+    return true;
+}
+
+// tidal_ramp (harmonic_tide.rs:24-32): a smoothstep over `duration` seconds, none for duration <= 0
+static double ramp_factor(double time, double duration) {
+    if (!(duration > 0.0)) return 1.0;
+    const double tau = std::min(std::max(time / duration, 0.0), 1.0);
+    return tau * tau * (3.0 - 2.0 * tau);
+}
+
+// The nesting band's tables and the parent's moment at the stage time (OceanModelState::moment,
+// ocean_nesting.rs:673-682): the snapshot stencil at the parent's clock (epoch + time) and the ramp
 static int set_nesting(const dg_ctx* c, StageParams& p, double time) {
     if (!c->nest_slot) return 0;
-    const std::vector<double>& ts = c->nest_times;
-    const size_t n = ts.size();
-    const double t = c->nest_epoch + time;
-    p.nest_len = 1;
-    p.nest_idx[0] = 0;
-    p.nest_w[0] = 1.0;
-    if (n > 1) {
-        DG_REQUIRE(t >= ts[0] && t <= ts[n - 1],
-                   "the nesting parent's snapshots do not cover the stage time (its forcing would freeze)");
-        size_t i1 = (size_t)(std::lower_bound(ts.begin(), ts.end(), t) - ts.begin());
-        i1 = std::min(std::max(i1, (size_t)1), n - 1);
-        const size_t i0 = i1 - 1;
-        if (t == ts[i0]) {
-            p.nest_idx[0] = (int)i0;
-        } else if (t == ts[i1]) {
-            p.nest_idx[0] = (int)i1;
-        } else {
-            const size_t lo = c->nest_cubic ? (i0 > 0 ? i0 - 1 : 0) : i0;
-            const size_t hi = c->nest_cubic ? std::min(i1 + 1, n - 1) : i1;
-            p.nest_len = (int)(hi - lo + 1);
-            for (size_t a = lo; a <= hi; ++a) {
-                double w = 1.0;
-                for (size_t b = lo; b <= hi; ++b)
-                    if (b != a) w *= (t - ts[b]) / (ts[a] - ts[b]);
-                p.nest_idx[a - lo] = (int)a;
-                p.nest_w[a - lo] = w;
-            }
-        }
-    }
+    DG_REQUIRE(time_stencil(c->nest_times, c->nest_epoch + time, c->nest_cubic != 0, p.nest_len, p.nest_idx, p.nest_w),
+               "the nesting parent's snapshots do not cover the stage time (its forcing would freeze)");
+    const size_t n = c->nest_times.size();
     p.nest_slot = c->nest_slot;
     p.nest_series = c->nest_series;
     p.nest_weight = c->nest_weight;
@@ -318,12 +333,26 @@ static int set_nesting(const dg_ctx* c, StageParams& p, double time) {
     p.nest_wet_depth = c->nest_wet_depth;
     p.nest_max_ratio = c->nest_max_ratio;
     p.nest_reference = c->nest_reference;
-    double rf = 1.0;
-    if (c->nest_ramp > 0.0) {
-        const double tau = std::min(std::max(time / c->nest_ramp, 0.0), 1.0);
-        rf = tau * tau * (3.0 - 2.0 * tau);
-    }
-    p.nest_ramp = rf;
+    p.nest_ramp = ramp_factor(time, c->nest_ramp);
+    return 0;
+}
+
+// The weather's snapshots and its moment at the stage time (GriddedAtmosphere2D::moment,
+// gridded_atmosphere.rs:564-566): the stencil at the clock (epoch + time), the switches times the ramp
+static int set_weather(const dg_ctx* c, StageParams& p, double time) {
+    if (!c->wx_series) return 0;
+    DG_REQUIRE(time_stencil(c->wx_times, c->wx_epoch + time, c->wx_cubic != 0, p.wx_len, p.wx_idx, p.wx_w),
+               "the weather's snapshots do not cover the stage time (its forcing would freeze)");
+    const double ramp = ramp_factor(time, c->wx_ramp);
+    p.wx_series = c->wx_series;
+    p.wx_nodes = (int64_t)c->K * c->nn;
+    p.wx_drag = c->wx_drag;
+    p.wx_wind = c->wx_wind ? ramp : 0.0;
+    p.wx_pres = c->wx_pressure ? ramp : 0.0;
+    p.wx_cd = c->wx_cd;
+    p.wx_rho_air = c->wx_rho_air;
+    p.wx_rho_water = c->wx_rho_water;
+    p.wx_h_min = c->wx_h_min;
     return 0;
 }
 
@@ -354,6 +383,7 @@ static int launch_stage(dg_ctx* c, const double* base, const double* input, doub
     p.queue_count = c->queue_count;
     set_atlas(c, p, time);
     if (int r = set_nesting(c, p, time)) return r;
+    if (int r = set_weather(c, p, time)) return r;
     const int epb = 64 / c->nn;
     const unsigned int nblk = (unsigned int)((c->K + epb - 1) / epb);
     const unsigned int nchunk = (nblk + 1023) / 1024;
@@ -498,6 +528,7 @@ static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, 
     DG_REQUIRE(!(c->flags & DG_ENTROPY_STABLE), "the EntropyStable faces have no VJP here (the adjoint is of the WetDry faces)");
     DG_REQUIRE(!c->tide_slot, "the boundary tides per node have no VJP here (the adjoint is of the uniform tide)");
     DG_REQUIRE(!c->nest_slot, "the nesting band has no VJP here");
+    DG_REQUIRE(!c->wx_series, "the gridded weather has no VJP here");
     if (int r = ensure_adjoint(c)) return r;
     size_t nt = (size_t)c->K * c->nn;
     VjpParams P{};
@@ -1207,6 +1238,7 @@ int dg_stage_where(dg_ctx* c, const dg_state* base, const dg_state* input, const
     DG_REQUIRE(c->lts_order, "dg_stage_where: no coarse step begun (dg_begin_coarse_step)");
     DG_REQUIRE(!c->tide_slot, "dg_stage_where: the boundary tides per node run with global steps only");
     DG_REQUIRE(!c->nest_slot, "dg_stage_where: the nesting band runs with global steps only");
+    DG_REQUIRE(!c->wx_series, "dg_stage_where: the gridded weather runs with global steps only");
     DG_REQUIRE(plan->stage >= 0 && plan->stage < c->lts_stages, "dg_stage_where: stage out of range");
     DG_REQUIRE(plan->finest >= c->lts_finest && plan->finest <= DG_MAX_LEVELS,
                "dg_stage_where: the plan's finest level is not the coarse step's");
@@ -1567,6 +1599,43 @@ int dg_set_nesting(dg_ctx* c, int64_t n_slots, const uint32_t* slot_of_node, int
     c->nest_max_ratio = max_ratio;
     c->nest_reference = reference_level;
     c->nest_ramp = ramp;
+    return 0;
+}
+
+int dg_set_weather(dg_ctx* c, int n_times, const double* times, double epoch, int cubic, const float* series,
+                   int wind, int pressure, int drag, double cd, double rho_air, double rho_water, double h_min,
+                   double ramp) {
+    DG_REQUIRE(c && c->K > 0, "dg_set_weather: null context or no form");
+    DG_TRY(hipSetDevice(c->device));
+    DG_TRY(hipStreamSynchronize(c->stream));
+    if (c->wx_series) DG_TRY(hipFree(c->wx_series));
+    c->wx_series = nullptr;
+    c->wx_times.clear();
+    if (n_times == 0) return 0;
+    DG_REQUIRE(times && series && n_times > 0, "dg_set_weather: missing snapshots");
+    DG_REQUIRE(drag >= 0 && drag <= 4, "dg_set_weather: drag is 0 (constant) to 4 (Yelland & Taylor)");
+    DG_REQUIRE(rho_water > 0.0, "dg_set_weather: the water density must be positive");
+    for (int i = 1; i < n_times; ++i) DG_REQUIRE(times[i] > times[i - 1], "dg_set_weather: snapshot times must increase");
+    const size_t bytes = (size_t)n_times * 5 * (size_t)c->K * c->nn * sizeof(float);
+    DG_TRY(hipMalloc(&c->wx_series, bytes));
+    DG_TRY(hipMemcpy(c->wx_series, series, bytes, hipMemcpyHostToDevice));
+    c->wx_times.assign(times, times + n_times);
+    c->wx_epoch = epoch;
+    c->wx_cubic = cubic ? 1 : 0;
+    c->wx_wind = wind ? 1 : 0;
+    c->wx_pressure = pressure ? 1 : 0;
+    c->wx_drag = drag;
+    c->wx_cd = cd;
+    c->wx_rho_air = rho_air;
+    c->wx_rho_water = rho_water;
+    c->wx_h_min = h_min;
+    c->wx_ramp = ramp;
+    if (!wind && !pressure) {
+        // GriddedAtmosphere2D::add_element adds nothing with both switches off
+        DG_TRY(hipFree(c->wx_series));
+        c->wx_series = nullptr;
+        c->wx_times.clear();
+    }
     return 0;
 }
 
@@ -2005,6 +2074,7 @@ int dg_batch_create(int n, dg_ctx* const* members, int scheme, dg_batch** out) {
         DG_REQUIRE(members[m]->device == members[0]->device, "dg_batch_create: members on different devices");
         DG_REQUIRE(!members[m]->tide_slot, "dg_batch_create: a member with boundary tides per node (not batched)");
         DG_REQUIRE(!members[m]->nest_slot, "dg_batch_create: a member with a nesting band (not batched)");
+        DG_REQUIRE(!members[m]->wx_series, "dg_batch_create: a member with gridded weather (not batched)");
     }
     DG_TRY(hipSetDevice(members[0]->device));
     dg_batch* b = new dg_batch;

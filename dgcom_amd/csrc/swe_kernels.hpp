@@ -133,6 +133,14 @@ struct StageParams {
     int nest_idx[4];
     double nest_w[4];
     double nest_rate, nest_wet_depth, nest_max_ratio, nest_reference, nest_ramp;
+    // the gridded weather (dg_set_weather): per snapshot, field-major f32 [n_times][5][K*NN] of u, v,
+    // dp/dx, dp/dy, p - p_ref; the stage's snapshot stencil, the ramped switches and the constants
+    const float* wx_series;
+    int64_t wx_nodes;
+    int wx_len, wx_drag;
+    int wx_idx[4];
+    double wx_w[4];
+    double wx_wind, wx_pres, wx_cd, wx_rho_air, wx_rho_water, wx_h_min;
 };
 
 // LDS exchange among the lanes of a workgroup that is one wavefront: the
@@ -742,6 +750,25 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
                 rhs = cadd(rhs, relaxation_source(p.nest_weight[slot], p.nest_rate, p.nest_wet_depth,
                                                   p.nest_momentum_only != 0, hin, huin, hvin, bin, eta, hv_, pu, pv));
             }
+        }
+        if (p.wx_series && hin >= p.wx_h_min) {
+            // the weather's wind stress and pressure gradient, after the band (GriddedAtmosphere2D::add_element,
+            // gridded_atmosphere.rs:703-731): the snapshots summed in the stencil's order (add_snapshot)
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (q >= p.wx_len) break;
+                const float* v = p.wx_series + (size_t)p.wx_idx[q] * 5 * p.wx_nodes + n;
+                s0 += p.wx_w[q] * (double)v[0];
+                s1 += p.wx_w[q] * (double)v[p.wx_nodes];
+                s2 += p.wx_w[q] * (double)v[2 * p.wx_nodes];
+                s3 += p.wx_w[q] * (double)v[3 * p.wx_nodes];
+            }
+            double su, sv;
+            atmosphere_source(hin, p.wx_wind * s0, p.wx_wind * s1, p.wx_pres * s2, p.wx_pres * s3, p.wx_drag, p.wx_cd,
+                              p.wx_rho_air, p.wx_rho_water, p.wx_h_min, su, sv);
+            rhs.hu += su;
+            rhs.hv += sv;
         }
         if (p.rh) { p.rh[n] = rhs.h; p.rhu[n] = rhs.hu; p.rhv[n] = rhs.hv; }
     }

@@ -106,6 +106,65 @@ DG_HD Cons relaxation_source(double weight, double rate, double wet_depth, bool 
     return {gamma * dh, gamma * (depth * u - qhu), gamma * (depth * v - qhv)};
 }
 
+// hypot as the reference computes it (math.rs:205-249, musl's hypot): each square
+// split exactly into hi + lo (Dekker, 2^27 + 1), scaled by 2^+-700 away from
+// overflow and underflow, the square root of the four parts' sum; the same bits on
+// every machine with a correctly rounded square root
+DG_HD void exact_square(double x, double& hi, double& lo) {
+    const double xc = x * 134217729.0;
+    const double xh = x - xc + xc;
+    const double xl = x - xh;
+    hi = x * x;
+    lo = xh * xh - hi + 2.0 * xh * xl + xl * xl;
+}
+
+DG_HD double hypot_exact(double x, double y) {
+    uint64_t ux = __builtin_bit_cast(uint64_t, x) & 0x7FFFFFFFFFFFFFFFull;
+    uint64_t uy = __builtin_bit_cast(uint64_t, y) & 0x7FFFFFFFFFFFFFFFull;
+    if (ux < uy) { const uint64_t t = ux; ux = uy; uy = t; }
+    const int64_t ex = (int64_t)(ux >> 52), ey = (int64_t)(uy >> 52);
+    double a = __builtin_bit_cast(double, ux), b = __builtin_bit_cast(double, uy);
+    if (ey == 0x7ff) return b;
+    if (ex == 0x7ff || uy == 0) return a;
+    if (ex - ey > 64) return a + b;
+    double z = 1.0;
+    if (ex > 0x3ff + 510) {
+        z = 0x1p700; a *= 0x1p-700; b *= 0x1p-700;
+    } else if (ey < 0x3ff - 450) {
+        z = 0x1p-700; a *= 0x1p700; b *= 0x1p700;
+    }
+    double hx, lx, hy, ly;
+    exact_square(a, hx, lx);
+    exact_square(b, hy, ly);
+    return z * sqrt(ly + lx + hy + hx);
+}
+
+// DragCoefficient::compute (source/swe_2d/wind.rs:134-158): 0 a constant cd,
+// 1 Large & Pond (wind capped at 25 m/s), 2 Wu, 3 Smith, 4 Yelland & Taylor
+DG_HD double drag_coefficient(int kind, double cd, double speed) {
+    switch (kind) {
+        case 1: return speed <= 11.0 ? 1.2e-3 : (0.49 + 0.065 * fmin_(speed, 25.0)) * 1e-3;
+        case 2: return (0.8 + 0.065 * speed) * 1e-3;
+        case 3: return (0.61 + 0.063 * speed) * 1e-3;
+        case 4: return speed <= 6.0 ? 1.1e-3 : (0.50 + 0.071 * speed) * 1e-3;
+        default: return cd;
+    }
+}
+
+// atmosphere_source (gridded_atmosphere.rs:169-196): (tau - h grad p) / rho_water
+// with tau = rho_air C_d(|U|) |U| U, the wind and the gradient already ramped;
+// nothing below h_min
+DG_HD void atmosphere_source(double h, double u, double v, double px, double py, int drag, double cd,
+                             double rho_air, double rho_water, double h_min, double& su, double& sv) {
+    su = 0.0;
+    sv = 0.0;
+    if (h < h_min) return;
+    const double speed = hypot_exact(u, v);
+    const double stress = rho_air * drag_coefficient(drag, cd, speed) * speed;
+    su = (stress * u - h * px) / rho_water;
+    sv = (stress * v - h * py) / rho_water;
+}
+
 // hll_flux_face_aligned + einfeldt_speeds_2d (flux/swe_2d.rs:277-374), with the
 // sides in face-aligned components (depth, normal / tangential momentum and velocity)
 struct Hll { double h, hun, hut, un, ut; };

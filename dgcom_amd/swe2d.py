@@ -40,7 +40,7 @@ EXPORTS = (
     "dg_batch_tuning", "dg_begin_coarse_step", "dg_stage_where", "dg_finish_where", "dg_depth_flags",
     "dg_multirate_create", "dg_multirate_destroy", "dg_multirate_assign_levels", "dg_multirate_step",
     "dg_multirate_levels", "dg_multirate_stats_read", "dg_multirate_run", "dg_fnv1a", "dg_set_boundary_tides",
-    "dg_set_nesting",
+    "dg_set_nesting", "dg_set_weather",
 )
 BATCH_GRAPH, BATCH_TUNE = 1, 2
 
@@ -152,6 +152,8 @@ def load(fma: bool = False):
     L.dg_set_nesting.argtypes = ([vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int, _dp,
                                   ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_float), _dp, _dp]
                                  + [ctypes.c_double] * 2 + [ctypes.c_int] * 2 + [ctypes.c_double] * 3)
+    L.dg_set_weather.argtypes = ([vp, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+                                 + [ctypes.c_int] * 3 + [ctypes.c_double] * 5)
     L.dg_fnv1a.argtypes = [_u8p, ctypes.c_size_t, ctypes.c_uint64]
     L.dg_fnv1a.restype = ctypes.c_uint64
     L.dg_multirate_create.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
@@ -321,6 +323,21 @@ class SWEPhysics2D:
                                                  ns.timescale, ns.wet_depth, int(ns.momentum_only),
                                                  int(ns.velocity), ns.max_ratio, ns.reference_level, ns.ramp),
                "dg_set_nesting")
+
+    def set_weather(self, wx):
+        """Gridded weather at the nodes (a `cases.Weather`), or None to clear it."""
+        if wx is None:
+            _check(self.lib, self.lib.dg_set_weather(self.ctx, 0, None, 0.0, 0, None, 0, 0, 1, 0.0, 1.225, 1025.0,
+                                                     0.01, 0.0), "dg_set_weather")
+            return
+        ti = np.ascontiguousarray(wx.times, dtype=np.float64)
+        se = np.ascontiguousarray(wx.series, dtype=np.float32)
+        if se.shape != (ti.size, 5, self.n_nodes_total):
+            raise ValueError(f"weather series of shape {se.shape}, expected {(ti.size, 5, self.n_nodes_total)}")
+        _check(self.lib, self.lib.dg_set_weather(self.ctx, ti.size, _p(ti), wx.epoch, int(wx.cubic),
+                                                 se.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), int(wx.wind),
+                                                 int(wx.pressure), wx.drag, wx.cd, wx.rho_air, wx.rho_water, wx.h_min,
+                                                 wx.ramp), "dg_set_weather")
 
     def close(self):
         if self.ctx:

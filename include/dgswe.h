@@ -132,6 +132,21 @@ int dg_set_nesting(dg_ctx* ctx, int64_t n_slots, const uint32_t* slot_of_node, i
                    double wet_depth, int momentum_only, int velocity, double max_ratio, double reference_level,
                    double ramp);
 
+/* Gridded weather (GriddedAtmosphere2D's wind stress and pressure gradient, src/source/swe_2d/
+ * gridded_atmosphere.rs:137-196, 564-566, 689-731; the reference device's regridded-snapshot form,
+ * DeviceSource2D::Atmosphere): per snapshot at times[n_times] (Unix seconds, increasing; model time t is
+ * epoch + t; cubic: Lagrange over the four nearest) the fields at every node, f32, field-major
+ * series[n_times][5][K*NN] = u, v (m/s, mesh axes), dp/dx, dp/dy (Pa/m), p - p_ref (Pa, unused here);
+ * the switches wind and pressure, ramped together (ramp, s; <= 0: none); the drag coefficient
+ * (DragCoefficient, wind.rs:134-158: 0 constant cd, 1 Large & Pond, 2 Wu, 3 Smith, 4 Yelland & Taylor),
+ * rho_air, rho_water and the depth h_min (1 cm) below which a node is not forced. The momentum source
+ * (tau - h grad p) / rho_water, tau = rho_air C_d(|U|) |U| U, is added after Coriolis and the nesting
+ * band. A stage time outside the snapshots is an error. n_times = 0 clears it. Global steps only: local
+ * time stepping, the ensembles and the VJP refuse it. */
+int dg_set_weather(dg_ctx* ctx, int n_times, const double* times, double epoch, int cubic, const float* series,
+                   int wind, int pressure, int drag, double cd, double rho_air, double rho_water, double h_min,
+                   double ramp);
+
 /* Device states (DeviceState::upload / download, gpu/src/swe2d.rs:2384-2396). */
 int dg_state_alloc(dg_ctx* ctx, dg_state** state);
 int dg_state_free(dg_ctx* ctx, dg_state* state);

@@ -116,14 +116,23 @@ def _bind(L):
         L.oc_finish_where.argtypes = [ctypes.c_void_p] + [_dp] * 6 + [_u32p, ctypes.c_int64]
         L.oc_set_boundary_tides.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int,
                                             ctypes.c_double, ctypes.c_int64, _u32p, _dp, _dp, _dp]
-    L.oc_set_nesting.argtypes = ([ctypes.c_void_p, ctypes.c_int64, _u32p, ctypes.c_int, _dp, ctypes.c_double,
-                                  ctypes.c_int, ctypes.POINTER(ctypes.c_float), _dp, _dp]
-                                 + [ctypes.c_double] * 2 + [ctypes.c_int] * 2 + [ctypes.c_double] * 3)
+        L.oc_set_nesting.argtypes = ([ctypes.c_void_p, ctypes.c_int64, _u32p, ctypes.c_int, _dp, ctypes.c_double,
+                                      ctypes.c_int, ctypes.POINTER(ctypes.c_float), _dp, _dp]
+                                     + [ctypes.c_double] * 2 + [ctypes.c_int] * 2 + [ctypes.c_double] * 3)
+        L.oc_set_weather.argtypes = ([ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_float)] + [ctypes.c_int] * 3 + [ctypes.c_double] * 5)
+        L.oc_hypot.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.oc_hypot.restype = ctypes.c_double
     return L
 
 
 def _p(a):
     return a.ctypes.data_as(_dp)
+
+
+def hypot(x, y):
+    """The reference's hypot (math.rs:205-249) as the restatement computes it."""
+    return lib().oc_hypot(x, y)
 
 
 def operators(order):
@@ -204,6 +213,14 @@ class Oracle:
                                se.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), _p(we), _p(de), ns.timescale,
                                ns.wet_depth, int(ns.momentum_only), int(ns.velocity), ns.max_ratio,
                                ns.reference_level, ns.ramp)
+
+    def set_weather(self, wx):
+        """Gridded weather at the nodes (a `cases.Weather`)."""
+        self._wx = (np.ascontiguousarray(wx.times, dtype=np.float64), np.ascontiguousarray(wx.series, dtype=np.float32))
+        ti, se = self._wx
+        self._L.oc_set_weather(self.h, ti.size, _p(ti), wx.epoch, int(wx.cubic),
+                               se.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), int(wx.wind), int(wx.pressure),
+                               wx.drag, wx.cd, wx.rho_air, wx.rho_water, wx.h_min, wx.ramp)
 
     def tide_elevation(self, t):
         return self._L.oc_tide_elevation(self.h, t)

@@ -167,6 +167,28 @@ void oc_set_nesting(void* p, int64_t n_slots, const uint32_t* slot, int n_times,
     ns.ramp = ramp;
 }
 
+void oc_set_weather(void* p, int n_times, const double* times, double epoch, int cubic, const float* series, int wind,
+                    int pressure, int drag, double cd, double rho_air, double rho_water, double h_min, double ramp) {
+    Solver<double>& s = *((Handle*)p)->s;
+    auto& w = s.weather;
+    w.on = n_times > 0;
+    if (!w.on) return;
+    w.times.assign(times, times + n_times);
+    w.series.assign(series, series + (size_t)n_times * 5 * (size_t)s.K * s.ops.nn);
+    w.epoch = epoch;
+    w.cubic = cubic != 0;
+    w.wind = wind != 0;
+    w.pressure = pressure != 0;
+    w.drag = drag;
+    w.cd = cd;
+    w.rho_air = rho_air;
+    w.rho_water = rho_water;
+    w.h_min = h_min;
+    w.ramp = ramp;
+}
+
+double oc_hypot(double x, double y) { return Solver<double>::ref_hypot(x, y); }
+
 void oc_step(void* p, double* h, double* hu, double* hv, double t, double dt, int scheme) {
     Handle* hd = (Handle*)p;
     hd->s->step(h, hu, hv, t, dt, scheme, hd->work);

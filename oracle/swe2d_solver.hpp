@@ -5,6 +5,9 @@
 #pragma once
 
 #include <cstdio>
+#include <cstring>
+#include <cmath>
+#include <algorithm>
 #include <cstdlib>
 #include "swe2d_oracle.hpp"
 
@@ -77,6 +80,15 @@ struct Solver {
         double epoch = 0.0, rate = 0.0, wet_depth = 0.05, max_ratio = 0.0, reference_level = 0.0, ramp = 0.0;
         bool cubic = false, momentum_only = false, velocity = false;
     } nesting;
+    // GriddedAtmosphere2D (source/swe_2d/gridded_atmosphere.rs:137-196, 689-731): per snapshot (Unix s) the
+    // regridded fields, field-major [n_times][5][K*nn] f32 (u, v, dp/dx, dp/dy, p - p_ref)
+    struct Weather {
+        bool on = false, cubic = false, wind = false, pressure = false;
+        std::vector<double> times;
+        std::vector<float> series;
+        double epoch = 0.0, ramp = 0.0, cd = 0.0, rho_air = 1.225, rho_water = 1025.0, h_min = 0.01;
+        int drag = 1;
+    } weather;
     int64_t K = 0;
     std::vector<double> corners;
     std::vector<uint32_t> links;
@@ -104,14 +116,16 @@ struct Solver {
 
     // TimeStencil::new (io/field_series.rs:41-81) at Unix time t: the snapshots and their weights
     int nesting_stencil(double t, size_t idx[4], double w[4]) const {
-        const std::vector<double>& ts = nesting.times;
+        return snapshot_stencil(nesting.times, nesting.cubic, t, idx, w);
+    }
+    static int snapshot_stencil(const std::vector<double>& ts, bool cubic, double t, size_t idx[4], double w[4]) {
         const size_t n = ts.size();
         idx[0] = 0;
         w[0] = 1.0;
         if (n == 1) return 1;
         if (!(t >= ts[0] && t <= ts[n - 1])) {
             // the reference panics here (OceanModelState::moment_at)
-            std::fprintf(stderr, "nesting: time %.17g outside the parent's snapshots\n", t);
+            std::fprintf(stderr, "time %.17g outside the snapshots\n", t);
             std::abort();
         }
         size_t i1 = 0;
@@ -120,8 +134,8 @@ struct Solver {
         size_t i0 = i1 - 1;
         if (t == ts[i0]) { idx[0] = i0; return 1; }
         if (t == ts[i1]) { idx[0] = i1; return 1; }
-        size_t lo = nesting.cubic ? (i0 == 0 ? 0 : i0 - 1) : i0;
-        size_t hi = nesting.cubic ? std::min(i1 + 1, n - 1) : i1;
+        size_t lo = cubic ? (i0 == 0 ? 0 : i0 - 1) : i0;
+        size_t hi = cubic ? std::min(i1 + 1, n - 1) : i1;
         for (size_t a = lo; a <= hi; ++a) {
             double p = 1.0;
             for (size_t b = lo; b <= hi; ++b)
@@ -168,6 +182,47 @@ struct Solver {
         const double dh = ns.momentum_only ? 0.0 : h - qh;
         const double depth = ns.momentum_only ? qh : h;
         return {gamma * dh, gamma * (depth * u - qhu), gamma * (depth * v - qhv)};
+    }
+
+    // crate::math::hypot (math.rs:205-249): musl's hypot, the squares split exactly (Dekker)
+    static double ref_hypot(double x, double y) {
+        auto square = [](double v, double& hi, double& lo) {
+            const double split = 134217728.0 + 1.0;
+            double vc = v * split, vh = v - vc + vc, vl = v - vh;
+            hi = v * v;
+            lo = vh * vh - hi + 2.0 * vh * vl + vl * vl;
+        };
+        uint64_t ux, uy;
+        std::memcpy(&ux, &x, 8);
+        std::memcpy(&uy, &y, 8);
+        ux &= ~(1ull << 63);
+        uy &= ~(1ull << 63);
+        if (ux < uy) std::swap(ux, uy);
+        const int64_t ex = (int64_t)(ux >> 52), ey = (int64_t)(uy >> 52);
+        std::memcpy(&x, &ux, 8);
+        std::memcpy(&y, &uy, 8);
+        if (ey == 0x7ff) return y;
+        if (ex == 0x7ff || uy == 0) return x;
+        if (ex - ey > 64) return x + y;
+        double z = 1.0;
+        if (ex > 0x3ff + 510) {
+            z = std::ldexp(1.0, 700); x *= std::ldexp(1.0, -700); y *= std::ldexp(1.0, -700);
+        } else if (ey < 0x3ff - 450) {
+            z = std::ldexp(1.0, -700); x *= std::ldexp(1.0, 700); y *= std::ldexp(1.0, 700);
+        }
+        double hx, lx, hy, ly;
+        square(x, hx, lx);
+        square(y, hy, ly);
+        return z * std::sqrt(ly + lx + hy + hx);
+    }
+
+    // DragCoefficient::compute (source/swe_2d/wind.rs:134-158)
+    static double drag_cd(int kind, double cd, double s) {
+        if (kind == 1) return s <= 11.0 ? 1.2e-3 : (0.49 + 0.065 * std::min(s, 25.0)) * 1e-3;
+        if (kind == 2) return (0.8 + 0.065 * s) * 1e-3;
+        if (kind == 3) return (0.61 + 0.063 * s) * 1e-3;
+        if (kind == 4) return s <= 6.0 ? 1.1e-3 : (0.50 + 0.071 * s) * 1e-3;
+        return cd;
     }
 
     // The face normal at face node fi (the per-node field; a parallelogram's is its node-0 value)
@@ -536,6 +591,33 @@ struct Solver {
                 State<double> src = nesting_source(slot, len, idx, w, ramp, val(h[n]), val(hu[n]), val(hv[n]),
                                                    val(bed[n]));
                 oh[n] += T(src.h); ohu[n] += T(src.hu); ohv[n] += T(src.hv);
+            }
+        }
+        if (weather.on && (weather.wind || weather.pressure)) {
+            // GriddedAtmosphere2D::add_element (gridded_atmosphere.rs:703-731) with atmosphere_source (:169-196)
+            size_t idx[4];
+            double w[4];
+            const int len = snapshot_stencil(weather.times, weather.cubic, weather.epoch + time, idx, w);
+            double ramp = 1.0;
+            if (weather.ramp > 0.0) {
+                double tau = std::min(std::max(time / weather.ramp, 0.0), 1.0);
+                ramp = tau * tau * (3.0 - 2.0 * tau);
+            }
+            const double fw = weather.wind ? ramp : 0.0, fp = weather.pressure ? ramp : 0.0;
+            const size_t total = (size_t)K * nn;
+            for (int i = 0; i < nn; ++i) {
+                size_t n = (size_t)k * nn + i;
+                const double hh = val(h[n]);
+                if (hh < weather.h_min) continue;
+                double sum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+                for (int q = 0; q < len; ++q)
+                    for (int f = 0; f < 5; ++f)
+                        sum[f] += w[q] * (double)weather.series[(idx[q] * 5 + f) * total + n];
+                const double u = fw * sum[0], v = fw * sum[1], px = fp * sum[2], py = fp * sum[3];
+                const double speed = ref_hypot(u, v);
+                const double stress = weather.rho_air * drag_cd(weather.drag, weather.cd, speed) * speed;
+                ohu[n] += T((stress * u - hh * px) / weather.rho_water);
+                ohv[n] += T((stress * v - hh * py) / weather.rho_water);
             }
         }
         if (cfg.manufactured) {

@@ -1,0 +1,116 @@
+"""Gridded weather (GriddedAtmosphere2D, src/source/swe_2d/gridded_atmosphere.rs)
+in the restatement, pinned by what the mathematics fixes: the reference's hypot
+(musl's, math.rs:205-249) within one ulp of the correctly rounded hypot and
+exact on Pythagorean triples, its special cases; in a closed basin at rest under
+uniform weather the RHS is the stress rho_air C_d(|U|) |U| U / rho_water and
+the pressure force -h grad p / rho_water alone, for every drag law's branches
+(Large & Pond's cap at 25 m/s); the ramp scales the wind before the stress; the
+switches; no force on nodes shallower than h_min. The GPU build is held to the
+restatement in tests/test_gpu_weather.py."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from dgcom_amd import cases
+
+T0 = 1_706_594_400.0
+RHO_AIR, RHO_WATER = 1.225, 1025.0
+
+
+def test_hypot():
+    rng = np.random.default_rng(3)
+    for scale in (1.0, 1e-300, 1e300, 1e-160, 1e160):
+        for _ in range(2000):
+            x, y = rng.standard_normal(2) * scale * 10.0 ** rng.integers(-3, 3, 2)
+            got, want = O.hypot(x, y), math.hypot(x, y)
+            assert abs(got - want) <= math.ulp(want), (x, y, got, want)
+    for a, b, c in ((3, 4, 5), (5, 12, 13), (8, 15, 17), (20, 21, 29), (119, 120, 169),
+                    (3 * 2.0 ** 600, 4 * 2.0 ** 600, 5 * 2.0 ** 600),
+                    (3 * 2.0 ** -600, 4 * 2.0 ** -600, 5 * 2.0 ** -600), (-3.0, 4.0, 5.0)):
+        assert O.hypot(a, b) == c and O.hypot(b, a) == c
+    assert O.hypot(math.inf, math.nan) == math.inf and O.hypot(math.nan, -math.inf) == math.inf
+    assert math.isnan(O.hypot(math.nan, 1.0))
+    assert O.hypot(0.0, -2.5) == 2.5 and O.hypot(1e300, 1e-300) == 1e300
+
+
+def _basin(depth=5.0):
+    mesh = cases.structured_mesh(0.0, 400.0, 0.0, 400.0, 4, 4)
+    x, _ = cases.node_coordinates(mesh, 2)
+    bed = np.full(x.size, -depth)
+    h, hu, hv = cases.still_water(bed)
+    return cases.Case("basin", 2, mesh, bed, np.zeros(mesh.n_elements), 0.0, cases.Tide(), h, hu, hv)
+
+
+def _uniform(case, u, v, px=0.0, py=0.0, times=(T0,), **kw):
+    n = case.h.size
+    fields = lambda x, y, t: (np.full(n, u), np.full(n, v), np.full(n, px), np.full(n, py), np.zeros(n))  # noqa: E731
+    return cases.weather_at_nodes(case, times, fields, epoch=T0, **kw)
+
+
+def _rhs(case, wx, t=0.0, h=None):
+    orc = O.Oracle.from_case(case, friction=False)
+    orc.set_weather(wx)
+    return orc.rhs(case.h if h is None else h, case.hu, case.hv, t)
+
+
+@pytest.mark.parametrize("drag,cd_of", [
+    (cases.DRAG_LARGE_POND, lambda s: 1.2e-3 if s <= 11.0 else (0.49 + 0.065 * min(s, 25.0)) * 1e-3),
+    (cases.DRAG_WU, lambda s: (0.8 + 0.065 * s) * 1e-3),
+    (cases.DRAG_SMITH, lambda s: (0.61 + 0.063 * s) * 1e-3),
+    (cases.DRAG_YELLAND_TAYLOR, lambda s: 1.1e-3 if s <= 6.0 else (0.50 + 0.071 * s) * 1e-3),
+    (cases.DRAG_CONSTANT, lambda s: 2.5e-3),
+])
+def test_wind_stress_of_each_drag_law(drag, cd_of):
+    c = _basin()
+    for u, v in ((3.0, 4.0), (-12.0, 16.0), (18.0, -24.0), (0.0, 6.0)):
+        s = math.hypot(u, v)
+        h, hu, hv = _rhs(c, _uniform(c, u, v, drag=drag, cd=2.5e-3, pressure=False))
+        tau = RHO_AIR * cd_of(s) * s
+        assert np.abs(h).max() < 1e-14
+        assert np.allclose(hu, tau * u / RHO_WATER, rtol=1e-13, atol=1e-13), (drag, u, v)
+        assert np.allclose(hv, tau * v / RHO_WATER, rtol=1e-13, atol=1e-13), (drag, u, v)
+
+
+def test_pressure_gradient_and_switches():
+    # values exact in f32, so that the closed forms hold to rounding
+    c = _basin()
+    _, hu, hv = _rhs(c, _uniform(c, 3.0, 4.0, px=0.015625, py=-0.0625, wind=False))
+    assert np.allclose(hu, -5.0 * 0.015625 / RHO_WATER, rtol=1e-12, atol=1e-13)
+    assert np.allclose(hv, 5.0 * 0.0625 / RHO_WATER, rtol=1e-12, atol=1e-13)
+    _, hu, _ = _rhs(c, _uniform(c, 3.0, 4.0, px=0.015625, py=-0.0625))
+    assert np.allclose(hu, (RHO_AIR * 1.2e-3 * 5.0 * 3.0 - 5.0 * 0.015625) / RHO_WATER, rtol=1e-12, atol=1e-13)
+    base = O.Oracle.from_case(c, friction=False).rhs(c.h, c.hu, c.hv, 0.0)
+    for a, b in zip(_rhs(c, _uniform(c, 3.0, 4.0, px=0.015625, wind=False, pressure=False)), base):
+        assert np.array_equal(a, b)
+
+
+def test_the_ramp_scales_the_wind_before_the_stress():
+    # 20 m/s ramped by 1/2 is 10 m/s: Large & Pond's low branch, a quarter of the stress at 1.2e-3
+    c = _basin()
+    wx = _uniform(c, 12.0, 16.0, px=0.0078125, ramp=1000.0)
+    _, hu, _ = _rhs(c, wx, t=500.0)
+    tau = RHO_AIR * 1.2e-3 * 10.0
+    assert np.allclose(hu, (tau * 6.0 - 5.0 * 0.5 * 0.0078125) / RHO_WATER, rtol=1e-12, atol=1e-13)
+    _, hu, _ = _rhs(c, wx, t=0.0)
+    assert np.abs(hu).max() < 1e-14
+
+
+def test_snapshots_in_time_and_shallow_nodes():
+    # a wind linear in time between hourly snapshots; nodes under 1 cm are not forced
+    c = _basin()
+    n = c.h.size
+    times = T0 + np.array([0.0, 3600.0, 7200.0])
+    wx = cases.weather_at_nodes(c, times, lambda x, y, t: (np.full(n, 2.0 + (t - T0) / 3600.0), np.zeros(n),
+                                                           np.zeros(n), np.zeros(n), np.zeros(n)),
+                                epoch=T0, pressure=False)
+    _, hu, _ = _rhs(c, wx, t=5400.0)
+    assert np.allclose(hu, RHO_AIR * 1.2e-3 * 3.5 * 3.5 / RHO_WATER, rtol=1e-13)
+    h = c.h.copy()
+    h[::4] = 0.005
+    base = O.Oracle.from_case(c, friction=False).rhs(h, c.hu, c.hv, 5400.0)
+    got = _rhs(c, wx, t=5400.0, h=h)
+    for a, b in zip(got, base):
+        assert np.array_equal(a[::4], b[::4])
+    assert np.abs(got[1][1::4] - base[1][1::4]).min() > 0.0
