@@ -389,19 +389,21 @@ static int launch_stage(dg_ctx* c, const double* base, const double* input, doub
     const unsigned int nchunk = (nblk + 1023) / 1024;
     p.xcd_stride = (nblk + 7) / 8;
     dim3 grid(8 * p.xcd_stride), sub_grid(std::min(nblk, SUB_GROUPS));
+    // the forcing given per node compiled in only where some is set
+    const bool force = c->tide_slot || c->nest_slot || c->wx_series;
     if (c->timing) DG_TRY(hipEventRecord(c->ev[0], c->stream));
     switch (c->order) {
-        case 1: hipLaunchKernelGGL(wet_stage_kernel<1>, grid, dim3(64), 0, c->stream, p); break;
-        case 2: hipLaunchKernelGGL(wet_stage_kernel<2>, grid, dim3(64), 0, c->stream, p); break;
-        default: hipLaunchKernelGGL(wet_stage_kernel<3>, grid, dim3(64), 0, c->stream, p); break;
+        case 1: hipLaunchKernelGGL((force ? wet_stage_kernel<1, true> : wet_stage_kernel<1>), grid, dim3(64), 0, c->stream, p); break;
+        case 2: hipLaunchKernelGGL((force ? wet_stage_kernel<2, true> : wet_stage_kernel<2>), grid, dim3(64), 0, c->stream, p); break;
+        default: hipLaunchKernelGGL((force ? wet_stage_kernel<3, true> : wet_stage_kernel<3>), grid, dim3(64), 0, c->stream, p); break;
     }
     hipLaunchKernelGGL(chunk_count_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, nblk, c->chunk_total);
     hipLaunchKernelGGL(compact_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, c->queue, nblk,
                        c->chunk_total, epb, c->sub_list, c->sub_count);
     switch (c->order) {
-        case 1: hipLaunchKernelGGL(subcell_stage_kernel<1>, sub_grid, dim3(64), 0, c->stream, p, c->sub_list, c->sub_count); break;
-        case 2: hipLaunchKernelGGL(subcell_stage_kernel<2>, sub_grid, dim3(64), 0, c->stream, p, c->sub_list, c->sub_count); break;
-        default: hipLaunchKernelGGL(subcell_stage_kernel<3>, sub_grid, dim3(64), 0, c->stream, p, c->sub_list, c->sub_count); break;
+        case 1: hipLaunchKernelGGL((force ? subcell_stage_kernel<1, false, true> : subcell_stage_kernel<1>), sub_grid, dim3(64), 0, c->stream, p, c->sub_list, c->sub_count); break;
+        case 2: hipLaunchKernelGGL((force ? subcell_stage_kernel<2, false, true> : subcell_stage_kernel<2>), sub_grid, dim3(64), 0, c->stream, p, c->sub_list, c->sub_count); break;
+        default: hipLaunchKernelGGL((force ? subcell_stage_kernel<3, false, true> : subcell_stage_kernel<3>), sub_grid, dim3(64), 0, c->stream, p, c->sub_list, c->sub_count); break;
     }
     DG_TRY(hipGetLastError());
     if (c->timing) {

@@ -419,7 +419,7 @@ namespace dgswe {
 // VJP: the same forward up to the element RHS, then the reverse (vjp_tail)
 // instead of the post-processing and the stores.
 // LTS: k is an element of the pass's list p.elist, lts_blk the list block.
-template <int N, bool SUB, bool VJP = false, bool LTS = false>
+template <int N, bool SUB, bool VJP = false, bool LTS = false, bool FORCE = false>
 __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, bool active,
                                             const VjpParams* vp = nullptr, int64_t lts_blk = 0) {
     constexpr int N1 = N + 1, NN = N1 * N1, EPB = 64 / NN, NF = 4 * N1;
@@ -578,9 +578,9 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             const uint32_t l = lk[r];
             Cons fstar;
             if (l == LINK_BOUNDARY && tg[r] == DG_BC_TIDE) {
-                const uint32_t slot = p.tide_slot ? p.tide_slot[((size_t)k * 4 + f) * N1 + fi] : 0xFFFFFFFFu;
+                const uint32_t slot = (FORCE && p.tide_slot) ? p.tide_slot[((size_t)k * 4 + f) * N1 + fi] : 0xFFFFFFFFu;
                 Cons qb;
-                if (slot != 0xFFFFFFFFu) {
+                if (FORCE && slot != 0xFFFFFFFFu) {
                     // the node's own atlas tide (global steps only)
                     double eta, ue, ve;
                     atlas_state(p.tide_coef + (size_t)slot * 6 * p.atlas_nc, p.atlas_nc, p.atlas_moment, p.tide_mean[slot],
@@ -730,7 +730,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             rhs.hu += p.f0 * hvin;
             rhs.hv += (-p.f0) * huin;
         }
-        if (p.nest_slot) {
+        if (FORCE && p.nest_slot) {
             // the nesting band's relaxation, after Coriolis (swe_2d.rs:929-973)
             const uint32_t slot = p.nest_slot[n];
             if (slot != 0xFFFFFFFFu) {
@@ -751,7 +751,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
                                                   p.nest_momentum_only != 0, hin, huin, hvin, bin, eta, hv_, pu, pv));
             }
         }
-        if (p.wx_series && hin >= p.wx_h_min) {
+        if (FORCE && p.wx_series && hin >= p.wx_h_min) {
             // the weather's wind stress and pressure gradient, after the band (GriddedAtmosphere2D::add_element,
             // gridded_atmosphere.rs:703-731): the snapshots summed in the stencil's order (add_snapshot)
             double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
@@ -921,14 +921,16 @@ __global__ __launch_bounds__(64) void geometry_kernel(const double* corners, con
 #define DG_WET_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4)))
 #endif
 
-template <int N>
+// FORCE: the forcing given per node (tides per boundary node, the nesting band, the weather) compiled in;
+// runs without it do not carry its registers
+template <int N, bool FORCE = false>
 __global__ __launch_bounds__(64) DG_WET_OCCUPANCY void wet_stage_kernel(StageParams p) {
     constexpr int NN = (N + 1) * (N + 1), EPB = 64 / NN;
     const int64_t blk = xcd_block(blockIdx.x, p.xcd_stride);
     if (blk * EPB >= p.n_elements) return;
     const int e = threadIdx.x / NN;
     const int64_t k = blk * EPB + (e < EPB ? e : 0);
-    stage_block<N, false>(p, k, e < EPB && blk * EPB + e < p.n_elements);
+    stage_block<N, false, false, false, FORCE>(p, k, e < EPB && blk * EPB + e < p.n_elements);
 }
 
 // A pass of local time stepping: the listed elements (p.elist[0..n_list)),
@@ -1015,14 +1017,14 @@ __global__ __launch_bounds__(256) void compact_kernel(const uint8_t* counts, con
 #else
 #define DG_SUB_OCCUPANCY __attribute__((amdgpu_waves_per_eu(2)))
 #endif
-template <int N, bool LTS = false>
+template <int N, bool LTS = false, bool FORCE = false>
 __global__ __launch_bounds__(64) DG_SUB_OCCUPANCY void subcell_stage_kernel(StageParams p, const uint32_t* list, const unsigned int* count_ptr) {
     constexpr int NN = (N + 1) * (N + 1), EPB = 64 / NN;
     const unsigned int count = *count_ptr;
     const int e = threadIdx.x / NN;
     for (size_t first = (size_t)blockIdx.x * EPB; first < count; first += (size_t)gridDim.x * EPB) {
         const bool active = e < EPB && first + e < count;
-        stage_block<N, true, false, LTS>(p, active ? (int64_t)list[first + e] : 0, active);
+        stage_block<N, true, false, LTS, FORCE>(p, active ? (int64_t)list[first + e] : 0, active);
         lds_sync();
     }
 }
