@@ -115,6 +115,7 @@ struct dg_ctx {
     std::vector<double> wx_times;
     double wx_epoch = 0.0, wx_ramp = 0.0, wx_cd = 0.0, wx_rho_air = 1.225, wx_rho_water = 1025.0, wx_h_min = 0.01;
     int wx_cubic = 0, wx_wind = 0, wx_pressure = 0, wx_drag = 1;
+    double wx_raise = 0.0;   // dg_set_inverse_barometer
     double nest_rate = 0.0, nest_wet_depth = 0.05, nest_max_ratio = 0.0, nest_reference = 0.0, nest_ramp = 0.0;
     // stations
     int n_st = 0;
@@ -353,6 +354,8 @@ static int set_weather(const dg_ctx* c, StageParams& p, double time) {
     p.wx_rho_air = c->wx_rho_air;
     p.wx_rho_water = c->wx_rho_water;
     p.wx_h_min = c->wx_h_min;
+    // GriddedAtmosphere2D's level is 0 without its pressure (gridded_atmosphere.rs:879-881)
+    p.wx_raise = c->wx_pressure ? c->wx_raise : 0.0;
     return 0;
 }
 
@@ -1638,6 +1641,13 @@ int dg_set_weather(dg_ctx* c, int n_times, const double* times, double epoch, in
         c->wx_series = nullptr;
         c->wx_times.clear();
     }
+    return 0;
+}
+
+int dg_set_inverse_barometer(dg_ctx* c, double factor) {
+    DG_REQUIRE(c, "dg_set_inverse_barometer: null context");
+    DG_REQUIRE(std::isfinite(factor), "dg_set_inverse_barometer: the factor must be finite");
+    c->wx_raise = factor;
     return 0;
 }
 

@@ -141,6 +141,7 @@ struct StageParams {
     int wx_idx[4];
     double wx_w[4];
     double wx_wind, wx_pres, wx_cd, wx_rho_air, wx_rho_water, wx_h_min;
+    double wx_raise;   // the open boundary's level raised by this times the inverse barometer (0: not)
 };
 
 // LDS exchange among the lanes of a workgroup that is one wavefront: the
@@ -579,12 +580,27 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             Cons fstar;
             if (l == LINK_BOUNDARY && tg[r] == DG_BC_TIDE) {
                 const uint32_t slot = (FORCE && p.tide_slot) ? p.tide_slot[((size_t)k * 4 + f) * N1 + fi] : 0xFFFFFFFFu;
+                double raise = 0.0;
+                if (FORCE && p.wx_raise != 0.0) {
+                    // InverseBarometer over the weather's level at the face node (characteristic.rs:324-328,
+                    // inverse_barometer_level, gridded_atmosphere.rs:198-200): -ramp p / (rho g), p summed
+                    // over the stage's snapshots at the node
+                    const size_t nf = (size_t)k * NN + face_node<N>(f, fi);
+                    double sp = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        if (q >= p.wx_len) break;
+                        sp += p.wx_w[q] * (double)p.wx_series[((size_t)p.wx_idx[q] * 5 + 4) * p.wx_nodes + nf];
+                    }
+                    raise = p.wx_raise * (-p.wx_pres * sp / (p.wx_rho_water * 9.81));
+                }
                 Cons qb;
                 if (FORCE && slot != 0xFFFFFFFFu) {
                     // the node's own atlas tide (global steps only)
                     double eta, ue, ve;
                     atlas_state(p.tide_coef + (size_t)slot * 6 * p.atlas_nc, p.atlas_nc, p.atlas_moment, p.tide_mean[slot],
                                 p.atlas_velocity != 0, p.atlas_max_ratio, p.tide_src[slot], q.b, eta, ue, ve);
+                    if (p.wx_raise != 0.0) eta += raise;
                     qb = p.atlas_velocity ? characteristic_velocity(q.h, q.hu, q.hv, q.b, eta, ue, ve, nx, ny, p.g, p.h_min)
                                           : characteristic(q.h, q.hu, q.hv, q.b, eta, nx, ny, p.g, p.h_min,
                                                            (p.flags & DG_INCOMING_WAVE) != 0);
@@ -595,6 +611,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
 #pragma unroll
                         for (int l = 0; l <= DG_MAX_LEVELS; ++l) seta = lv == l ? p.lvl_eta[l] : seta;
                     }
+                    if (FORCE && p.wx_raise != 0.0) seta += raise;
                     qb = characteristic(q.h, q.hu, q.hv, q.b, seta, nx, ny, p.g, p.h_min, (p.flags & DG_INCOMING_WAVE) != 0);
                 }
                 Node nb = make_node(qb.h, qb.hu, qb.hv, q.b, p.h_min);

@@ -147,6 +147,13 @@ int dg_set_weather(dg_ctx* ctx, int n_times, const double* times, double epoch, 
                    int wind, int pressure, int drag, double cd, double rho_air, double rho_water, double h_min,
                    double ramp);
 
+/* The open boundary's external level raised by factor times the weather's inverse-barometer level at each
+ * face node, -ramp (p - p_ref) / (rho_water 9.81) (InverseBarometer::with_factor over GriddedAtmosphere2D's
+ * BoundaryLevel, src/boundary/characteristic.rs:277-345, gridded_atmosphere.rs:198-200, 873-890; the device's
+ * `raise`, gpu/src/swe2d.rs:900-909): on the uniform tide and on the tides per node alike. 0 (the default):
+ * not raised. Acts while dg_set_weather holds weather with its pressure switched on. */
+int dg_set_inverse_barometer(dg_ctx* ctx, double factor);
+
 /* Device states (DeviceState::upload / download, gpu/src/swe2d.rs:2384-2396). */
 int dg_state_alloc(dg_ctx* ctx, dg_state** state);
 int dg_state_free(dg_ctx* ctx, dg_state* state);

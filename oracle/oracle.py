@@ -121,6 +121,7 @@ def _bind(L):
                                      + [ctypes.c_double] * 2 + [ctypes.c_int] * 2 + [ctypes.c_double] * 3)
         L.oc_set_weather.argtypes = ([ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int,
                                       ctypes.POINTER(ctypes.c_float)] + [ctypes.c_int] * 3 + [ctypes.c_double] * 5)
+        L.oc_set_inverse_barometer.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.oc_hypot.argtypes = [ctypes.c_double, ctypes.c_double]
         L.oc_hypot.restype = ctypes.c_double
     return L
@@ -221,6 +222,10 @@ class Oracle:
         self._L.oc_set_weather(self.h, ti.size, _p(ti), wx.epoch, int(wx.cubic),
                                se.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), int(wx.wind), int(wx.pressure),
                                wx.drag, wx.cd, wx.rho_air, wx.rho_water, wx.h_min, wx.ramp)
+
+    def set_inverse_barometer(self, factor):
+        """The open boundary's level raised by factor times the weather's inverse barometer."""
+        self._L.oc_set_inverse_barometer(self.h, factor)
 
     def tide_elevation(self, t):
         return self._L.oc_tide_elevation(self.h, t)

@@ -187,6 +187,8 @@ void oc_set_weather(void* p, int n_times, const double* times, double epoch, int
     w.ramp = ramp;
 }
 
+void oc_set_inverse_barometer(void* p, double factor) { ((Handle*)p)->s->weather.raise = factor; }
+
 double oc_hypot(double x, double y) { return Solver<double>::ref_hypot(x, y); }
 
 void oc_step(void* p, double* h, double* hu, double* hv, double t, double dt, int scheme) {

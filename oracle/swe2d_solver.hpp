@@ -88,7 +88,27 @@ struct Solver {
         std::vector<float> series;
         double epoch = 0.0, ramp = 0.0, cd = 0.0, rho_air = 1.225, rho_water = 1025.0, h_min = 0.01;
         int drag = 1;
+        double raise = 0.0;   // InverseBarometer's factor over the weather's level (0: none)
     } weather;
+
+    // InverseBarometer::external_state's raise (characteristic.rs:324-328) with GriddedAtmosphere2D::level
+    // (gridded_atmosphere.rs:873-890): factor * -ramp p / (rho_w G) at the face node's flat index, G = 9.81
+    bool barometer_raise(int64_t k, int f, int fi, double time, double& raise) const {
+        if (!(weather.on && weather.pressure && weather.raise != 0.0 && fi >= 0)) return false;
+        size_t idx[4];
+        double w[4];
+        const int len = snapshot_stencil(weather.times, weather.cubic, weather.epoch + time, idx, w);
+        const size_t total = (size_t)K * ops.nn, n = (size_t)k * ops.nn + ops.face_nodes[f][fi];
+        double p = 0.0;
+        for (int q = 0; q < len; ++q) p += w[q] * (double)weather.series[(idx[q] * 5 + 4) * total + n];
+        double ramp = 1.0;
+        if (weather.ramp > 0.0) {
+            double tau = std::min(std::max(time / weather.ramp, 0.0), 1.0);
+            ramp = tau * tau * (3.0 - 2.0 * tau);
+        }
+        raise = weather.raise * (-ramp * p / (weather.rho_water * 9.81));
+        return true;
+    }
     int64_t K = 0;
     std::vector<double> corners;
     std::vector<uint32_t> links;
@@ -314,8 +334,9 @@ struct Solver {
         if (tags[4 * k + f] == BC_TIDE && slot != 0xFFFFFFFFu) {
             // CharacteristicOBC<BoundaryTides>: the node's own tide; with velocity, u_n,e = u.n,
             // u_t,e = -u n_y + v n_x (characteristic.rs:411-438)
-            double eta, ue, ve;
+            double eta, ue, ve, raise;
             atlas_state(slot, time, val(q.b), eta, ue, ve);
+            if (barometer_raise(k, f, fi, time, raise)) eta += raise;
             T h_e = vmax(T(eta) - q.b, zero);
             T un_e, ut_e;
             if (atlas.velocity) {
@@ -334,6 +355,8 @@ struct Solver {
         if (tags[4 * k + f] == BC_TIDE) {
             // CharacteristicOBC<HarmonicTide>, elevation-only data at rest or as an incoming
             // wave from still water, u_n,e = 2 (sqrt(g H) - sqrt(g h_e)), H = -B (characteristic.rs:411-438)
+            double raise;
+            if (barometer_raise(k, f, fi, time, raise)) eta_e += raise;
             T h_e = vmax(T(eta_e) - q.b, zero);
             T u = zero, v = zero;
             if (q.h > T(hm)) { T inv = T(1.0) / q.h; u = q.hu * inv; v = q.hv * inv; }

@@ -1,7 +1,9 @@
 """Gridded weather on the GPU against the restatement: whole adaptive runs bit
 for bit under wind and pressure fields varying in space and time across the
 drag laws' branches (Large & Pond with both switches and linear snapshots; Wu,
-the wind alone, cubic snapshots and a ramp; the pressure alone), and a stage
+the wind alone, cubic snapshots and a ramp; the pressure alone), the open
+boundary raised by the inverse barometer (on the uniform tide and on tides per
+node), and a stage
 time the snapshots do not cover, local time stepping, the ensembles and the VJP
 refused by name."""
 import numpy as np
@@ -54,6 +56,41 @@ def test_runs_equal_the_oracle(name, options):
     phys.compute_rhs_into(s, t_g, out)
     for a, b in zip(out.download(), orc.rhs(*x, t_g)):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("factor,atlas", [(1.0, False), (0.25, True)], ids=["uniform_tide", "tides_per_node"])
+def test_the_inverse_barometer_equals_the_oracle(factor, atlas):
+    # the open boundary raised by the weather's inverse barometer over a pressure field varying along it
+    c = cases.drying_beach(nx=16, ny=6, jitter=0.15)
+    n = c.h.size
+    times = T0 + np.arange(0.0, 4 * 3600.0, 1800.0)
+    wx = cases.weather_at_nodes(c, times, lambda x, y, t: (np.full(n, 5.0), np.zeros(n), 0.002 * np.ones(n),
+                                                           np.zeros(n), 800.0 * np.sin(y / 150.0 + (t - T0) / 900.0)),
+                                epoch=T0, ramp=30.0)
+    orc = O.Oracle.from_case(c)
+    phys = swe2d.SWEPhysics2D.from_case(c)
+    for m in (orc, phys):
+        m.set_weather(wx)
+        m.set_inverse_barometer(factor)
+    if atlas:
+        bt = cases.boundary_tides(c, [2 * np.pi / 600.0], [0.0], lambda x, y: np.array([[0.3, 0.1, 0.2, 0.0, 0.05, 0.0]]),
+                                  velocity=True, mean=lambda x, y: 0.02)
+        orc.set_boundary_tides(bt)
+        phys.set_boundary_tides(bt)
+    s = phys.state(c.h, c.hu, c.hv)
+    x = [c.h.copy(), c.hu.copy(), c.hv.copy()]
+    t_g = phys.run(s, 0.0, 60, c.cfl)
+    t_o = orc.run(*x, 0.0, 60, c.cfl)
+    assert t_g == t_o
+    for g, r in zip(s.download(), x):
+        assert np.array_equal(g, r)
+    ref = O.Oracle.from_case(c)
+    ref.set_weather(wx)
+    if atlas:
+        ref.set_boundary_tides(bt)
+    y = [c.h.copy(), c.hu.copy(), c.hv.copy()]
+    ref.run(*y, 0.0, 60, c.cfl)
+    assert np.abs(y[0] - x[0]).max() > 1e-6          # the raise acts
 
 
 def test_refusals():

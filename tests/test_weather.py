@@ -114,3 +114,36 @@ def test_snapshots_in_time_and_shallow_nodes():
     for a, b in zip(got, base):
         assert np.array_equal(a[::4], b[::4])
     assert np.abs(got[1][1::4] - base[1][1::4]).min() > 0.0
+
+
+def _channel(depth=5.0):
+    mesh = cases.structured_mesh(0.0, 400.0, 0.0, 100.0, 4, 1, sides=(cases.WALL, cases.WALL, cases.WALL, cases.TIDE))
+    x, _ = cases.node_coordinates(mesh, 2)
+    bed = np.full(x.size, -depth)
+    h, hu, hv = cases.still_water(bed)
+    return cases.Case("channel", 2, mesh, bed, np.zeros(mesh.n_elements), 0.0, cases.Tide(), h, hu, hv)
+
+
+@pytest.mark.parametrize("factor,pressure,mean", [(1.0, True, 0.0), (0.25, True, 0.0), (1.0, False, 0.0),
+                                                   (1.0, True, 0.125)])
+def test_the_inverse_barometer_at_the_open_boundary(factor, pressure, mean):
+    # the west boundary's level raised by factor * -p / (rho g): the mass flux of the characteristic state
+    # between the interior at rest and the raised level, c_b = (c_i + c_e) / 2, u_n,b = c_i - c_e outward
+    c = _channel()
+    n = c.h.size
+    wx = cases.weather_at_nodes(c, [T0], lambda x, y, t: (np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n),
+                                                          np.full(n, 1024.0)), epoch=T0, wind=False, pressure=pressure)
+    orc = O.Oracle.from_case(c, friction=False)
+    orc.set_weather(wx)
+    orc.set_inverse_barometer(factor)
+    if mean:
+        orc.set_boundary_tides(cases.boundary_tides(c, [1e-4], [0.0], lambda x, y: np.zeros((1, 6)),
+                                                    mean=lambda x, y: mean))
+    fm = orc.face_mass(c.h, c.hu, c.hv, 0.0).ravel()
+    west = [(k * 4 + 3) * 3 + fi for k in range(c.n_elements) for fi in range(3) if c.mesh.links[k, 3] == cases.BOUNDARY]
+    level = factor * -1024.0 / (RHO_WATER * 9.81) if pressure else 0.0
+    ci, ce = math.sqrt(9.81 * 5.0), math.sqrt(9.81 * (5.0 + mean + level))
+    expect = ((ci + ce) / 2.0) ** 2 / 9.81 * (ci - ce)
+    assert np.allclose(fm[west], expect, rtol=1e-12, atol=1e-15), (fm[west], expect)
+    if not pressure and not mean:
+        assert np.abs(fm[west]).max() < 1e-15
