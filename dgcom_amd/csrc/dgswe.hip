@@ -116,6 +116,14 @@ struct dg_ctx {
     double wx_epoch = 0.0, wx_ramp = 0.0, wx_cd = 0.0, wx_rho_air = 1.225, wx_rho_water = 1025.0, wx_h_min = 0.01;
     int wx_cubic = 0, wx_wind = 0, wx_pressure = 0, wx_drag = 1;
     double wx_raise = 0.0;   // dg_set_inverse_barometer
+    // local time stepping with forcing: a ring of per-level moment tables, pinned on the host and on the
+    // device; a slot's host copy is written again only after its event (its upload) has completed
+    static constexpr int MOMENT_RING = 64;
+    double* lvl_moment_dev = nullptr;
+    double* lvl_moment_host = nullptr;
+    hipEvent_t lvl_moment_ev[MOMENT_RING] = {};
+    bool lvl_moment_used[MOMENT_RING] = {};
+    int lvl_moment_next = 0;
     double nest_rate = 0.0, nest_wet_depth = 0.05, nest_max_ratio = 0.0, nest_reference = 0.0, nest_ramp = 0.0;
     // stations
     int n_st = 0;
@@ -1005,6 +1013,11 @@ int dg_destroy(dg_ctx* c) {
     DG_TRY(hipFree(c->flag));
     DG_TRY(hipEventDestroy(c->ev[0]));
     DG_TRY(hipEventDestroy(c->ev[1]));
+    if (c->lvl_moment_dev) {
+        DG_TRY(hipFree(c->lvl_moment_dev));
+        DG_TRY(hipHostFree(c->lvl_moment_host));
+        for (int i = 0; i < dg_ctx::MOMENT_RING; ++i) DG_TRY(hipEventDestroy(c->lvl_moment_ev[i]));
+    }
     DG_TRY(hipStreamDestroy(c->own_stream));
     delete c;
     return 0;
@@ -1241,9 +1254,6 @@ int dg_stage_where(dg_ctx* c, const dg_state* base, const dg_state* input, const
                    dg_state* acc) {
     DG_REQUIRE(c && base && input && plan && out && c->K > 0, "dg_stage_where: null argument or no form");
     DG_REQUIRE(c->lts_order, "dg_stage_where: no coarse step begun (dg_begin_coarse_step)");
-    DG_REQUIRE(!c->tide_slot, "dg_stage_where: the boundary tides per node run with global steps only");
-    DG_REQUIRE(!c->nest_slot, "dg_stage_where: the nesting band runs with global steps only");
-    DG_REQUIRE(!c->wx_series, "dg_stage_where: the gridded weather runs with global steps only");
     DG_REQUIRE(plan->stage >= 0 && plan->stage < c->lts_stages, "dg_stage_where: stage out of range");
     DG_REQUIRE(plan->finest >= c->lts_finest && plan->finest <= DG_MAX_LEVELS,
                "dg_stage_where: the plan's finest level is not the coarse step's");
@@ -1292,23 +1302,73 @@ int dg_stage_where(dg_ctx* c, const dg_state* base, const dg_state* input, const
         p.lvl_c[l] = plan->beta * dt_l;
         p.lvl_eta[l] = tide_elevation(c->tide, c->tide_nodal, time);
     }
+    // the forcing given per node: its tables, and per level the moments at the level's stage time
+    const bool force = c->tide_slot || c->nest_slot || c->wx_series;
+    if (force) {
+        constexpr int ROW = LVL_MOMENT, ROWS = DG_MAX_LEVELS + 1;
+        if (!c->lvl_moment_dev) {
+            DG_TRY(hipMalloc(&c->lvl_moment_dev, sizeof(double) * dg_ctx::MOMENT_RING * ROWS * ROW));
+            DG_TRY(hipHostMalloc(&c->lvl_moment_host, sizeof(double) * dg_ctx::MOMENT_RING * ROWS * ROW));
+            for (int i = 0; i < dg_ctx::MOMENT_RING; ++i)
+                DG_TRY(hipEventCreateWithFlags(&c->lvl_moment_ev[i], hipEventDisableTiming));
+        }
+        const int slot = c->lvl_moment_next;
+        c->lvl_moment_next = (slot + 1) % dg_ctx::MOMENT_RING;
+        if (c->lvl_moment_used[slot]) DG_TRY(hipEventSynchronize(c->lvl_moment_ev[slot]));
+        double* rows = c->lvl_moment_host + (size_t)slot * ROWS * ROW;
+        for (int l = 0; l <= DG_MAX_LEVELS; ++l) {
+            const int lc = std::min(l, plan->finest);
+            const double dt_l = plan->dt / (double)(1ull << lc);
+            const uint64_t q = (uint64_t)plan->substep >> (plan->finest - lc);
+            const double time = plan->t + (double)q * dt_l + plan->c * dt_l;
+            StageParams m{};
+            set_atlas(c, m, time);
+            if (int r = set_nesting(c, m, time)) return r;
+            if (int r = set_weather(c, m, time)) return r;
+            double* row = rows + (size_t)l * ROW;
+            std::fill(row, row + ROW, 0.0);
+            for (int j = 0; j < 1 + 3 * ATLAS_MAX_CONSTITUENTS; ++j) row[j] = m.atlas_moment[j];
+            row[LM_NEST] = m.nest_ramp;
+            row[LM_NEST + 2] = m.nest_len;
+            row[LM_WX] = m.wx_wind;
+            row[LM_WX + 1] = m.wx_pres;
+            row[LM_WX + 2] = m.wx_len;
+            for (int j = 0; j < 4; ++j) {
+                row[LM_NEST + 3 + j] = m.nest_idx[j];
+                row[LM_NEST + 7 + j] = m.nest_w[j];
+                row[LM_WX + 3 + j] = m.wx_idx[j];
+                row[LM_WX + 7 + j] = m.wx_w[j];
+            }
+            if (l == 0) {
+                // the tables and constants, the same at every level
+                set_atlas(c, p, time);
+                if (int r = set_nesting(c, p, time)) return r;
+                if (int r = set_weather(c, p, time)) return r;
+            }
+        }
+        double* dev = c->lvl_moment_dev + (size_t)slot * ROWS * ROW;
+        DG_TRY(hipMemcpyAsync(dev, rows, sizeof(double) * ROWS * ROW, hipMemcpyHostToDevice, c->stream));
+        DG_TRY(hipEventRecord(c->lvl_moment_ev[slot], c->stream));
+        c->lvl_moment_used[slot] = true;
+        p.lvl_moment = dev;
+    }
     const int epb = 64 / c->nn;
     const unsigned int nblk = (unsigned int)((n_list + epb - 1) / epb);
     const unsigned int nchunk = (nblk + 1023) / 1024;
     p.xcd_stride = (nblk + 7) / 8;
     dim3 grid(8 * p.xcd_stride), sub_grid(std::min(nblk, SUB_GROUPS));
     switch (c->order) {
-        case 1: hipLaunchKernelGGL(wet_lts_kernel<1>, grid, dim3(64), 0, c->stream, p); break;
-        case 2: hipLaunchKernelGGL(wet_lts_kernel<2>, grid, dim3(64), 0, c->stream, p); break;
-        default: hipLaunchKernelGGL(wet_lts_kernel<3>, grid, dim3(64), 0, c->stream, p); break;
+        case 1: hipLaunchKernelGGL((force ? wet_lts_kernel<1, true> : wet_lts_kernel<1>), grid, dim3(64), 0, c->stream, p); break;
+        case 2: hipLaunchKernelGGL((force ? wet_lts_kernel<2, true> : wet_lts_kernel<2>), grid, dim3(64), 0, c->stream, p); break;
+        default: hipLaunchKernelGGL((force ? wet_lts_kernel<3, true> : wet_lts_kernel<3>), grid, dim3(64), 0, c->stream, p); break;
     }
     hipLaunchKernelGGL(chunk_count_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, nblk, c->chunk_total);
     hipLaunchKernelGGL(compact_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, c->queue, nblk,
                        c->chunk_total, epb, c->sub_list, c->sub_count);
     switch (c->order) {
-        case 1: hipLaunchKernelGGL((subcell_stage_kernel<1, true>), sub_grid, dim3(64), 0, c->stream, p, c->sub_list, c->sub_count); break;
-        case 2: hipLaunchKernelGGL((subcell_stage_kernel<2, true>), sub_grid, dim3(64), 0, c->stream, p, c->sub_list, c->sub_count); break;
-        default: hipLaunchKernelGGL((subcell_stage_kernel<3, true>), sub_grid, dim3(64), 0, c->stream, p, c->sub_list, c->sub_count); break;
+        case 1: hipLaunchKernelGGL((force ? subcell_stage_kernel<1, true, true> : subcell_stage_kernel<1, true>), sub_grid, dim3(64), 0, c->stream, p, c->sub_list, c->sub_count); break;
+        case 2: hipLaunchKernelGGL((force ? subcell_stage_kernel<2, true, true> : subcell_stage_kernel<2, true>), sub_grid, dim3(64), 0, c->stream, p, c->sub_list, c->sub_count); break;
+        default: hipLaunchKernelGGL((force ? subcell_stage_kernel<3, true, true> : subcell_stage_kernel<3, true>), sub_grid, dim3(64), 0, c->stream, p, c->sub_list, c->sub_count); break;
     }
     DG_TRY(hipGetLastError());
     return 0;

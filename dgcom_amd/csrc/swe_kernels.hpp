@@ -142,7 +142,37 @@ struct StageParams {
     double wx_w[4];
     double wx_wind, wx_pres, wx_cd, wx_rho_air, wx_rho_water, wx_h_min;
     double wx_raise;   // the open boundary's level raised by this times the inverse barometer (0: not)
+    // local time stepping with forcing: per level a row of LVL_MOMENT values, the moments at the level's stage time
+    const double* lvl_moment;
 };
+
+// A level's row of the forcing moments: the atlas moment (25 values), then the nesting band's and the weather's
+// snapshot moments at LM_NEST and LM_WX, each [ramp or wind, 0 or pressure, len, idx x 4, w x 4]
+constexpr int LVL_MOMENT = 48, LM_NEST = 25, LM_WX = 36;
+
+struct SnapMoment { int len; int idx[4]; double w[4]; double a, b; };
+
+// The nesting band's (WX false: a = ramp) or the weather's (WX true: a, b = the ramped switches) snapshot
+// moment at the element's stage time: the kernel's arguments, or with local time stepping the level's row
+template <bool LTS, bool WX>
+__device__ __forceinline__ SnapMoment snap_moment(const StageParams& p, int lv) {
+    SnapMoment m;
+    if constexpr (LTS) {
+        const double* r = p.lvl_moment + (size_t)lv * LVL_MOMENT + (WX ? LM_WX : LM_NEST);
+        m.a = r[0]; m.b = r[1]; m.len = (int)r[2];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { m.idx[q] = (int)r[3 + q]; m.w[q] = r[7 + q]; }
+    } else if constexpr (WX) {
+        m.a = p.wx_wind; m.b = p.wx_pres; m.len = p.wx_len;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { m.idx[q] = p.wx_idx[q]; m.w[q] = p.wx_w[q]; }
+    } else {
+        m.a = p.nest_ramp; m.b = 0.0; m.len = p.nest_len;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { m.idx[q] = p.nest_idx[q]; m.w[q] = p.nest_w[q]; }
+    }
+    return m;
+}
 
 // LDS exchange among the lanes of a workgroup that is one wavefront: the
 // wavefront's LDS operations complete in issue order, so only the compiler's
@@ -586,20 +616,27 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
                     // inverse_barometer_level, gridded_atmosphere.rs:198-200): -ramp p / (rho g), p summed
                     // over the stage's snapshots at the node
                     const size_t nf = (size_t)k * NN + face_node<N>(f, fi);
+                    const SnapMoment m = snap_moment<LTS, true>(p, lv);
                     double sp = 0.0;
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
-                        if (q >= p.wx_len) break;
-                        sp += p.wx_w[q] * (double)p.wx_series[((size_t)p.wx_idx[q] * 5 + 4) * p.wx_nodes + nf];
+                        if (q >= m.len) break;
+                        sp += m.w[q] * (double)p.wx_series[((size_t)m.idx[q] * 5 + 4) * p.wx_nodes + nf];
                     }
-                    raise = p.wx_raise * (-p.wx_pres * sp / (p.wx_rho_water * 9.81));
+                    raise = p.wx_raise * (-m.b * sp / (p.wx_rho_water * 9.81));
                 }
                 Cons qb;
                 if (FORCE && slot != 0xFFFFFFFFu) {
-                    // the node's own atlas tide (global steps only)
+                    // the node's own atlas tide at the element's stage time
                     double eta, ue, ve;
-                    atlas_state(p.tide_coef + (size_t)slot * 6 * p.atlas_nc, p.atlas_nc, p.atlas_moment, p.tide_mean[slot],
-                                p.atlas_velocity != 0, p.atlas_max_ratio, p.tide_src[slot], q.b, eta, ue, ve);
+                    if constexpr (LTS)
+                        atlas_state(p.tide_coef + (size_t)slot * 6 * p.atlas_nc, p.atlas_nc,
+                                    p.lvl_moment + (size_t)lv * LVL_MOMENT, p.tide_mean[slot], p.atlas_velocity != 0,
+                                    p.atlas_max_ratio, p.tide_src[slot], q.b, eta, ue, ve);
+                    else
+                        atlas_state(p.tide_coef + (size_t)slot * 6 * p.atlas_nc, p.atlas_nc, p.atlas_moment,
+                                    p.tide_mean[slot], p.atlas_velocity != 0, p.atlas_max_ratio, p.tide_src[slot], q.b,
+                                    eta, ue, ve);
                     if (p.wx_raise != 0.0) eta += raise;
                     qb = p.atlas_velocity ? characteristic_velocity(q.h, q.hu, q.hv, q.b, eta, ue, ve, nx, ny, p.g, p.h_min)
                                           : characteristic(q.h, q.hu, q.hv, q.b, eta, nx, ny, p.g, p.h_min,
@@ -751,18 +788,19 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             // the nesting band's relaxation, after Coriolis (swe_2d.rs:929-973)
             const uint32_t slot = p.nest_slot[n];
             if (slot != 0xFFFFFFFFu) {
+                const SnapMoment m = snap_moment<LTS, false>(p, lv);
                 double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    if (q >= p.nest_len) break;
-                    const float* v = p.nest_series + ((size_t)slot * p.nest_times + p.nest_idx[q]) * 3;
-                    s0 += p.nest_w[q] * (double)v[0];
-                    s1 += p.nest_w[q] * (double)v[1];
-                    s2 += p.nest_w[q] * (double)v[2];
+                    if (q >= m.len) break;
+                    const float* v = p.nest_series + ((size_t)slot * p.nest_times + m.idx[q]) * 3;
+                    s0 += m.w[q] * (double)v[0];
+                    s1 += m.w[q] * (double)v[1];
+                    s2 += m.w[q] * (double)v[2];
                 }
                 double eta, pu, pv;
                 bool hv_;
-                parent_state(s0, s1, s2, p.nest_ramp, p.nest_reference, p.nest_velocity != 0, p.nest_max_ratio,
+                parent_state(s0, s1, s2, m.a, p.nest_reference, p.nest_velocity != 0, p.nest_max_ratio,
                              p.nest_depth[slot], bin, eta, hv_, pu, pv);
                 rhs = cadd(rhs, relaxation_source(p.nest_weight[slot], p.nest_rate, p.nest_wet_depth,
                                                   p.nest_momentum_only != 0, hin, huin, hvin, bin, eta, hv_, pu, pv));
@@ -771,18 +809,19 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
         if (FORCE && p.wx_series && hin >= p.wx_h_min) {
             // the weather's wind stress and pressure gradient, after the band (GriddedAtmosphere2D::add_element,
             // gridded_atmosphere.rs:703-731): the snapshots summed in the stencil's order (add_snapshot)
+            const SnapMoment m = snap_moment<LTS, true>(p, lv);
             double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                if (q >= p.wx_len) break;
-                const float* v = p.wx_series + (size_t)p.wx_idx[q] * 5 * p.wx_nodes + n;
-                s0 += p.wx_w[q] * (double)v[0];
-                s1 += p.wx_w[q] * (double)v[p.wx_nodes];
-                s2 += p.wx_w[q] * (double)v[2 * p.wx_nodes];
-                s3 += p.wx_w[q] * (double)v[3 * p.wx_nodes];
+                if (q >= m.len) break;
+                const float* v = p.wx_series + (size_t)m.idx[q] * 5 * p.wx_nodes + n;
+                s0 += m.w[q] * (double)v[0];
+                s1 += m.w[q] * (double)v[p.wx_nodes];
+                s2 += m.w[q] * (double)v[2 * p.wx_nodes];
+                s3 += m.w[q] * (double)v[3 * p.wx_nodes];
             }
             double su, sv;
-            atmosphere_source(hin, p.wx_wind * s0, p.wx_wind * s1, p.wx_pres * s2, p.wx_pres * s3, p.wx_drag, p.wx_cd,
+            atmosphere_source(hin, m.a * s0, m.a * s1, m.b * s2, m.b * s3, p.wx_drag, p.wx_cd,
                               p.wx_rho_air, p.wx_rho_water, p.wx_h_min, su, sv);
             rhs.hu += su;
             rhs.hv += sv;
@@ -954,7 +993,7 @@ __global__ __launch_bounds__(64) DG_WET_OCCUPANCY void wet_stage_kernel(StagePar
 // EPB per workgroup in list order (LocalTimeStepping::stage_where,
 // builder.rs:652-737); those with a shallow node are queued per block for
 // subcell_stage_kernel<N, true>.
-template <int N>
+template <int N, bool FORCE = false>
 __global__ __launch_bounds__(64) DG_WET_OCCUPANCY void wet_lts_kernel(StageParams p) {
     constexpr int NN = (N + 1) * (N + 1), EPB = 64 / NN;
     const int64_t blk = xcd_block(blockIdx.x, p.xcd_stride);
@@ -963,7 +1002,7 @@ __global__ __launch_bounds__(64) DG_WET_OCCUPANCY void wet_lts_kernel(StageParam
     const bool active = e < EPB && blk * EPB + e < p.n_list;
     const int64_t i = active ? blk * EPB + e : blk * EPB;
     const int64_t k = p.elist ? (int64_t)p.elist[i] : i;   // no list: every element, in mesh order
-    stage_block<N, false, false, true>(p, k, active, nullptr, blk);
+    stage_block<N, false, false, true, FORCE>(p, k, active, nullptr, blk);
 }
 
 // Compaction of the per-block subcell lists into one list in element order:

@@ -342,7 +342,9 @@ DG_HD Cons characteristic_velocity(double h, double hu, double hv, double bed, d
 // node's mean level and, with a transport ratio limit, the velocity scaled to
 // carry the source model's transport over the local bed
 constexpr int ATLAS_MAX_CONSTITUENTS = 8;
-DG_HD void atlas_state(const double* c, int n_c, const double (&moment)[1 + 3 * ATLAS_MAX_CONSTITUENTS], double mean,
+// (the moment: an array in the kernel's arguments, or a row of a table in device memory)
+template <class Moment>
+DG_HD void atlas_state(const double* c, int n_c, const Moment& moment, double mean,
                        bool velocity, double max_ratio, double source_depth, double bed, double& eta, double& u,
                        double& v) {
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;

@@ -2,13 +2,13 @@
 runs bit for bit with per-node constants varying along the open boundary (with
 a velocity, its transport scaling and a mean level; and elevation only as an
 incoming wave), the tidal channel with its tide written as node constants
-within rounding of the uniform tide's run; local time stepping and the VJP
-refuse them by name."""
+within rounding of the uniform tide's run; the VJP refuses them by name (local
+time stepping: tests/test_gpu_forcing_lts.py)."""
 import numpy as np
 import pytest
 
 import oracle as O
-from dgcom_amd import cases, multirate as M, swe2d
+from dgcom_amd import cases, swe2d
 
 pytestmark = pytest.mark.gpu
 
@@ -68,8 +68,6 @@ def test_refusals():
     phys = swe2d.SWEPhysics2D.from_case(c)
     phys.set_boundary_tides(_varying(c, True, 0.0))
     s = phys.state(c.h, c.hu, c.hv)
-    with pytest.raises(RuntimeError, match="global steps only"):
-        M.run(phys.local_time_stepping(), M.neighbours(c.mesh.links), s, 0.0, 5.0, c.cfl, 2)
     w, gb, gi = phys.state(c.h, c.hu, c.hv), phys.state().zero(), phys.state().zero()
     with pytest.raises(RuntimeError, match="no VJP"):
         phys.stage_vjp(s, s, 1.0, 0.0, 1.0, 0.0, w, gb, gi)

@@ -1,0 +1,82 @@
+"""The forcing given per node under local time stepping on the GPU against the
+restatement driven by the same stepper: tides per boundary node with a
+velocity, the nesting band, the gridded weather and the inverse-barometer
+raise, each element forced at its own level's stage time (the library's
+per-level moment table), whole multirate runs bit for bit through the library's
+stepper (dg_multirate_run) and the host's (dg_stage_where)."""
+import numpy as np
+import pytest
+
+from dgcom_amd import cases, multirate as M, swe2d
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+T0 = 1_706_594_400.0
+
+
+def _forcing(c, which):
+    n = c.h.size
+    out = {}
+    if "tides" in which:
+        out["tides"] = cases.boundary_tides(c, [2 * np.pi / 600.0, 2 * np.pi / 400.0], [1e-7, 0.0],
+                                            lambda x, y: np.array([[0.3, 0.1, 0.2, 0.0, 0.05, 0.0],
+                                                                   [0.1, -0.05, 0.0, 0.1, 0.0, 0.02]]),
+                                            ramp=60.0, velocity=True, max_ratio=2.0, mean=lambda x, y: 0.02,
+                                            source_depth=lambda x, y: 6.0)
+    if "nesting" in which:
+        times = T0 + np.arange(-600.0, 3 * 3600.0, 300.0)
+        out["nesting"] = cases.nesting_band(
+            c, 200.0, times, lambda x, y, t: (0.05 * np.sin((t - T0) / 200.0 + y / 300.0), 0.1, -0.02),
+            parent_depth=lambda x, y: 2.0, epoch=T0, timescale=120.0, velocity=True, max_ratio=2.0, cubic=True)
+    if "weather" in which:
+        times = T0 + np.arange(0.0, 3 * 3600.0, 600.0)
+        out["weather"] = cases.weather_at_nodes(
+            c, times, lambda x, y, t: (8.0 + 10.0 * np.sin(x / 300.0 + (t - T0) / 500.0), np.full(n, 4.0),
+                                       0.002 * np.cos(y / 200.0), np.zeros(n), 600.0 * np.sin(y / 150.0 + (t - T0) / 700.0)),
+            epoch=T0, ramp=30.0)
+    return out
+
+
+def _apply(m, f, raise_factor):
+    if "tides" in f:
+        m.set_boundary_tides(f["tides"])
+    if "nesting" in f:
+        m.set_nesting(f["nesting"])
+    if "weather" in f:
+        m.set_weather(f["weather"])
+        m.set_inverse_barometer(raise_factor)
+
+
+CASES = [("tides", "tides"), ("nesting", "nesting"), ("weather", "weather"), ("all", "tides nesting weather")]
+
+
+@pytest.mark.parametrize("name,which", CASES, ids=[c[0] for c in CASES])
+def test_forced_multirate_runs_equal_the_oracle(name, which):
+    c = cases.drying_beach(nx=24, ny=6)
+    f = _forcing(c, which)
+    nb = M.neighbours(c.mesh.links)
+    orc = O.Oracle.from_case(c)
+    _apply(orc, f, 0.5)
+    x = [c.h.copy(), c.hu.copy(), c.hv.copy()]
+    t_o, n_o, st = M.run(orc, nb, x, 0.0, 120.0, c.cfl, 8, c.scheme)
+    assert st.stats.finest_level >= 1
+    # the library's stepper
+    phys = swe2d.SWEPhysics2D.from_case(c)
+    _apply(phys, f, 0.5)
+    mr = swe2d.Multirate(phys, 8, c.scheme)
+    s = phys.state(c.h, c.hu, c.hv)
+    t_g, n_g = mr.run(s, 0.0, 120.0, c.cfl)
+    assert (t_g, n_g) == (t_o, n_o)
+    assert mr.stats()["element_evaluations"] == st.stats.element_evaluations
+    mr.close()
+    for a, b in zip(s.download(), x):
+        assert np.array_equal(a, b)
+    # the host's stepper over the element-subset entry points
+    local = phys.local_time_stepping()
+    s2 = phys.state(c.h, c.hu, c.hv)
+    t_h, n_h, st_h = M.run(local, nb, s2, 0.0, 120.0, c.cfl, 8, c.scheme)
+    st_h.release(local)
+    assert (t_h, n_h) == (t_o, n_o)
+    for a, b in zip(s2.download(), x):
+        assert np.array_equal(a, b)

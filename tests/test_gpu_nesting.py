@@ -2,13 +2,13 @@
 for bit with a band over the open boundary relaxing towards a parent that
 varies in space and time (elevation only with linear snapshots; a velocity with
 its transport scaling, the datum shift, the ramp and cubic snapshots; momentum
-only), and a stage time the snapshots do not cover, local time stepping, the
-ensembles and the VJP refused by name."""
+only), and a stage time the snapshots do not cover, the ensembles and the VJP
+refused by name (local time stepping: tests/test_gpu_forcing_lts.py)."""
 import numpy as np
 import pytest
 
 import oracle as O
-from dgcom_amd import cases, multirate as M, swe2d
+from dgcom_amd import cases, swe2d
 
 pytestmark = pytest.mark.gpu
 
@@ -63,8 +63,6 @@ def test_refusals():
     s = phys.state(c.h, c.hu, c.hv)
     with pytest.raises(RuntimeError, match="do not cover"):
         phys.step(s, 5.0 * 3600.0, 0.1)
-    with pytest.raises(RuntimeError, match="global steps only"):
-        M.run(phys.local_time_stepping(), M.neighbours(c.mesh.links), s, 0.0, 5.0, c.cfl, 2)
     w, gb, gi = phys.state(c.h, c.hu, c.hv), phys.state().zero(), phys.state().zero()
     with pytest.raises(RuntimeError, match="no VJP"):
         phys.stage_vjp(s, s, 1.0, 0.0, 1.0, 0.0, w, gb, gi)
