@@ -110,8 +110,8 @@ int dg_set_tide_nodal(dg_ctx* ctx, const double* f0, const double* f_rate, const
  * (C, S of eta, u, v), mean[n_slots] (added unramped; zeros for none) and source_depth[n_slots]; the ramp (s,
  * <= 0: none); velocity: the tides carry u, v through the characteristic boundary (else elevation only, as
  * the form's flags say); max_ratio > 0: the velocity scaled to carry the source's transport, limited to that
- * factor either way (with_transport_scaling). n_c = 0 clears them. Global steps only: local time stepping,
- * the ensembles and the VJP refuse them. */
+ * factor either way (with_transport_scaling). n_c = 0 clears them. The ensembles and the VJP refuse them; under local
+ * time stepping each element takes them at its level's stage time. */
 int dg_set_boundary_tides(dg_ctx* ctx, int n_c, const double* omega, const double* f_rate, double ramp, int velocity,
                           double max_ratio, int64_t n_slots, const uint32_t* slot_of_face_node, const double* coef,
                           const double* mean, const double* source_depth);
@@ -125,8 +125,8 @@ int dg_set_boundary_tides(dg_ctx* ctx, int n_c, const double* omega, const doubl
  * where either depth is below wet_depth (0.05 m); momentum_only; velocity: the parent has a depth-mean
  * velocity (else the node's own); max_ratio > 0: that velocity scaled to the parent's transport; the datum
  * shift reference_level; the ramp (s, <= 0: none). A stage time outside the snapshots is an error (a
- * single snapshot covers every time). n_slots = 0 clears it. Global steps only: local time stepping, the
- * ensembles and the VJP refuse it. */
+ * single snapshot covers every time). n_slots = 0 clears it. The ensembles and the VJP refuse it; under
+ * local time stepping each element takes it at its level's stage time. */
 int dg_set_nesting(dg_ctx* ctx, int64_t n_slots, const uint32_t* slot_of_node, int n_times, const double* times,
                    double epoch, int cubic, const float* series, const double* weight, const double* parent_depth, double timescale,
                    double wet_depth, int momentum_only, int velocity, double max_ratio, double reference_level,
@@ -141,8 +141,8 @@ int dg_set_nesting(dg_ctx* ctx, int64_t n_slots, const uint32_t* slot_of_node, i
  * (DragCoefficient, wind.rs:134-158: 0 constant cd, 1 Large & Pond, 2 Wu, 3 Smith, 4 Yelland & Taylor),
  * rho_air, rho_water and the depth h_min (1 cm) below which a node is not forced. The momentum source
  * (tau - h grad p) / rho_water, tau = rho_air C_d(|U|) |U| U, is added after Coriolis and the nesting
- * band. A stage time outside the snapshots is an error. n_times = 0 clears it. Global steps only: local
- * time stepping, the ensembles and the VJP refuse it. */
+ * band. A stage time outside the snapshots is an error. n_times = 0 clears it. The ensembles and the VJP
+ * refuse it; under local time stepping each element takes it at its level's stage time. */
 int dg_set_weather(dg_ctx* ctx, int n_times, const double* times, double epoch, int cubic, const float* series,
                    int wind, int pressure, int drag, double cd, double rho_air, double rho_water, double h_min,
                    double ramp);
