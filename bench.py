@@ -290,6 +290,33 @@ def lts_bench(nx, ny, t_end, max_levels):
             "volume_difference_rel": abs(float(m @ h_lts) - float(m @ h_glob)) / float(m @ h_glob)}
 
 
+def weather_bench(case, phys, state, t, n_steps, ms_plain):
+    """The main workload's step under gridded weather (GriddedAtmosphere2D's source: wind stress and pressure
+    gradient from two snapshots an hour apart, linear in time, at every node): its cost over the plain step."""
+    x, y = cases.node_coordinates(case.mesh, case.order)
+    x, y = x.ravel(), y.ravel()
+
+    def fields(xx, yy, tt):
+        u = 8.0 + 6.0 * np.sin(xx / 7000.0 + tt / 3000.0)
+        return u, 4.0 * np.cos(yy / 9000.0), 1e-4 * np.sin(xx / 20000.0), 1e-4 * np.cos(yy / 15000.0), 0.0 * xx
+
+    t0 = time.time()
+    wx = cases.weather_at_nodes(case, [t - 1.0, t + 3600.0], fields)
+    phys.set_weather(wx)
+    setup = time.time() - t0
+    t = phys.run(state, t, 2, case.cfl)
+    phys.sync()
+    w0 = time.perf_counter()
+    t = phys.run(state, t, n_steps, case.cfl)
+    phys.sync()
+    ms = (time.perf_counter() - w0) / n_steps * 1e3
+    phys.set_weather(None)
+    return {"workload": f"the main workload under gridded weather: {x.size} nodes x 2 snapshots x 5 fields f32 "
+                        f"({wx.series.nbytes / 1e6:.0f} MB resident), Large & Pond drag, linear in time",
+            "steps": n_steps, "ms_per_step": ms, "ms_per_step_plain": ms_plain, "ratio": ms / ms_plain,
+            "setup_s": setup}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -314,6 +341,8 @@ def main():
                     "leg, comma-separated nx x ny ('none' for none)")
     ap.add_argument("--lts-seconds", type=float, default=900.0)
     ap.add_argument("--lts-levels", type=int, default=8)
+    ap.add_argument("--weather-steps", type=int, default=0,
+                    help="steps of the main workload under gridded weather (wind, pressure, 2 snapshots; 0: not run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the time step runs on a single device here "
@@ -389,6 +418,8 @@ def main():
                             "flops_per_stage_launch": flops,
                             "wet_kernel_valu_insts_per_wave": per.get("wet_stage_kernel<3>", {}).get("valu_insts_per_wave"),
                             "wet_kernel_fp64_insts_per_wave": per.get("wet_stage_kernel<3>", {}).get("fp64_insts_per_wave")}
+    if args.weather_steps > 0:
+        out["weather"] = weather_bench(case, phys, state, t, args.weather_steps, out["ms_per_step"])
     if not args.no_adjoint:
         phys.close()
         out["adjoint"] = adjoint_bench(args.adjoint_n, args.adjoint_steps, args.adjoint_snapshots, 10, 20)
