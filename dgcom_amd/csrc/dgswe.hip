@@ -108,7 +108,7 @@ struct dg_ctx {
     float* nest_series = nullptr;
     double *nest_weight = nullptr, *nest_depth = nullptr;
     std::vector<double> nest_times;
-    int nest_velocity = 0, nest_momentum_only = 0, nest_cubic = 0;
+    int nest_velocity = 0, nest_momentum_only = 0, nest_cubic = 0, nest_boundary = 0;
     double nest_epoch = 0.0;
     // the gridded weather (dg_set_weather): the snapshots on the device, their times on the host
     float* wx_series = nullptr;
@@ -343,6 +343,7 @@ static int set_nesting(const dg_ctx* c, StageParams& p, double time) {
     p.nest_max_ratio = c->nest_max_ratio;
     p.nest_reference = c->nest_reference;
     p.nest_ramp = ramp_factor(time, c->nest_ramp);
+    p.nest_boundary = c->nest_boundary;
     return 0;
 }
 
@@ -1708,6 +1709,12 @@ int dg_set_inverse_barometer(dg_ctx* c, double factor) {
     DG_REQUIRE(c, "dg_set_inverse_barometer: null context");
     DG_REQUIRE(std::isfinite(factor), "dg_set_inverse_barometer: the factor must be finite");
     c->wx_raise = factor;
+    return 0;
+}
+
+int dg_set_nesting_boundary(dg_ctx* c, int on) {
+    DG_REQUIRE(c, "dg_set_nesting_boundary: null context");
+    c->nest_boundary = on ? 1 : 0;
     return 0;
 }
 

@@ -133,6 +133,7 @@ struct StageParams {
     int nest_idx[4];
     double nest_w[4];
     double nest_rate, nest_wet_depth, nest_max_ratio, nest_reference, nest_ramp;
+    int nest_boundary;   // the open boundary's external state is the parent's at its face nodes' slots
     // the gridded weather (dg_set_weather): per snapshot, field-major f32 [n_times][5][K*NN] of u, v,
     // dp/dx, dp/dy, p - p_ref; the stage's snapshot stencil, the ramped switches and the constants
     const float* wx_series;
@@ -625,8 +626,31 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
                     }
                     raise = p.wx_raise * (-m.b * sp / (p.wx_rho_water * 9.81));
                 }
+                const uint32_t pslot = (FORCE && p.nest_boundary) ? p.nest_slot[(size_t)k * NN + face_node<N>(f, fi)]
+                                                                  : 0xFFFFFFFFu;
                 Cons qb;
-                if (FORCE && slot != 0xFFFFFFFFu) {
+                if (FORCE && pslot != 0xFFFFFFFFu) {
+                    // the nesting parent's state at the face node's slot (OceanModelState::external_state,
+                    // ocean_nesting.rs:823-826), raised by the inverse barometer
+                    const SnapMoment m = snap_moment<LTS, false>(p, lv);
+                    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+                    for (int qq = 0; qq < 4; ++qq) {
+                        if (qq >= m.len) break;
+                        const float* v = p.nest_series + ((size_t)pslot * p.nest_times + m.idx[qq]) * 3;
+                        s0 += m.w[qq] * (double)v[0];
+                        s1 += m.w[qq] * (double)v[1];
+                        s2 += m.w[qq] * (double)v[2];
+                    }
+                    double eta, pu, pv;
+                    bool hv_;
+                    parent_state(s0, s1, s2, m.a, p.nest_reference, p.nest_velocity != 0, p.nest_max_ratio,
+                                 p.nest_depth[pslot], q.b, eta, hv_, pu, pv);
+                    if (p.wx_raise != 0.0) eta += raise;
+                    qb = hv_ ? characteristic_velocity(q.h, q.hu, q.hv, q.b, eta, pu, pv, nx, ny, p.g, p.h_min)
+                             : characteristic(q.h, q.hu, q.hv, q.b, eta, nx, ny, p.g, p.h_min,
+                                              (p.flags & DG_INCOMING_WAVE) != 0);
+                } else if (FORCE && slot != 0xFFFFFFFFu) {
                     // the node's own atlas tide at the element's stage time
                     double eta, ue, ve;
                     if constexpr (LTS)

@@ -40,7 +40,7 @@ EXPORTS = (
     "dg_batch_tuning", "dg_begin_coarse_step", "dg_stage_where", "dg_finish_where", "dg_depth_flags",
     "dg_multirate_create", "dg_multirate_destroy", "dg_multirate_assign_levels", "dg_multirate_step",
     "dg_multirate_levels", "dg_multirate_stats_read", "dg_multirate_run", "dg_fnv1a", "dg_set_boundary_tides",
-    "dg_set_nesting", "dg_set_weather", "dg_set_inverse_barometer",
+    "dg_set_nesting", "dg_set_weather", "dg_set_inverse_barometer", "dg_set_nesting_boundary",
 )
 BATCH_GRAPH, BATCH_TUNE = 1, 2
 
@@ -155,6 +155,7 @@ def load(fma: bool = False):
     L.dg_set_weather.argtypes = ([vp, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
                                  + [ctypes.c_int] * 3 + [ctypes.c_double] * 5)
     L.dg_set_inverse_barometer.argtypes = [vp, ctypes.c_double]
+    L.dg_set_nesting_boundary.argtypes = [vp, ctypes.c_int]
     L.dg_fnv1a.argtypes = [_u8p, ctypes.c_size_t, ctypes.c_uint64]
     L.dg_fnv1a.restype = ctypes.c_uint64
     L.dg_multirate_create.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
@@ -339,6 +340,10 @@ class SWEPhysics2D:
                                                  se.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), int(wx.wind),
                                                  int(wx.pressure), wx.drag, wx.cd, wx.rho_air, wx.rho_water, wx.h_min,
                                                  wx.ramp), "dg_set_weather")
+
+    def set_nesting_boundary(self, on: bool = True):
+        """The open boundary's external state taken from the nesting parent at its face nodes' slots."""
+        _check(self.lib, self.lib.dg_set_nesting_boundary(self.ctx, int(on)), "dg_set_nesting_boundary")
 
     def set_inverse_barometer(self, factor: float):
         """Raise the open boundary's level by factor times the weather's inverse barometer (0: not)."""

@@ -132,6 +132,12 @@ int dg_set_nesting(dg_ctx* ctx, int64_t n_slots, const uint32_t* slot_of_node, i
                    double wet_depth, int momentum_only, int velocity, double max_ratio, double reference_level,
                    double ramp);
 
+/* on != 0: the open boundary's external state is the nesting parent's at each face node's slot (the
+ * characteristic boundary over OceanModelState, DeviceExternal2D::Parent, src/boundary/ocean_nesting.rs:823-831,
+ * characteristic.rs:135-151): its level and, if it has one, its transport-scaled velocity, raised by the inverse
+ * barometer if that is set; face nodes without a slot keep the tides. Acts while dg_set_nesting holds a parent. */
+int dg_set_nesting_boundary(dg_ctx* ctx, int on);
+
 /* Gridded weather (GriddedAtmosphere2D's wind stress and pressure gradient, src/source/swe_2d/
  * gridded_atmosphere.rs:137-196, 564-566, 689-731; the reference device's regridded-snapshot form,
  * DeviceSource2D::Atmosphere): per snapshot at times[n_times] (Unix seconds, increasing; model time t is

@@ -122,6 +122,7 @@ def _bind(L):
         L.oc_set_weather.argtypes = ([ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int,
                                       ctypes.POINTER(ctypes.c_float)] + [ctypes.c_int] * 3 + [ctypes.c_double] * 5)
         L.oc_set_inverse_barometer.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.oc_set_nesting_boundary.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.oc_hypot.argtypes = [ctypes.c_double, ctypes.c_double]
         L.oc_hypot.restype = ctypes.c_double
     return L
@@ -226,6 +227,10 @@ class Oracle:
     def set_inverse_barometer(self, factor):
         """The open boundary's level raised by factor times the weather's inverse barometer."""
         self._L.oc_set_inverse_barometer(self.h, factor)
+
+    def set_nesting_boundary(self, on=True):
+        """The open boundary's external state taken from the nesting parent at its face nodes' slots."""
+        self._L.oc_set_nesting_boundary(self.h, int(on))
 
     def tide_elevation(self, t):
         return self._L.oc_tide_elevation(self.h, t)

@@ -189,6 +189,8 @@ void oc_set_weather(void* p, int n_times, const double* times, double epoch, int
 
 void oc_set_inverse_barometer(void* p, double factor) { ((Handle*)p)->s->weather.raise = factor; }
 
+void oc_set_nesting_boundary(void* p, int on) { ((Handle*)p)->s->nesting.boundary = on != 0; }
+
 double oc_hypot(double x, double y) { return Solver<double>::ref_hypot(x, y); }
 
 void oc_step(void* p, double* h, double* hu, double* hv, double t, double dt, int scheme) {

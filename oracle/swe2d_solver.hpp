@@ -79,6 +79,7 @@ struct Solver {
         std::vector<float> series;
         double epoch = 0.0, rate = 0.0, wet_depth = 0.05, max_ratio = 0.0, reference_level = 0.0, ramp = 0.0;
         bool cubic = false, momentum_only = false, velocity = false;
+        bool boundary = false;   // the open boundary's external state is the parent's (DeviceExternal2D::Parent)
     } nesting;
     // GriddedAtmosphere2D (source/swe_2d/gridded_atmosphere.rs:137-196, 689-731): per snapshot (Unix s) the
     // regridded fields, field-major [n_times][5][K*nn] f32 (u, v, dp/dx, dp/dy, p - p_ref)
@@ -173,35 +174,45 @@ struct Solver {
         const Nesting& ns = nesting;
         const double weight = ns.weight[slot];
         if (weight <= 0.0 || qh < ns.wet_depth) return {0.0, 0.0, 0.0};
-        double sum[3] = {0.0, 0.0, 0.0};
-        for (int q = 0; q < len; ++q) {
-            const float* v = ns.series.data() + ((size_t)slot * ns.times.size() + idx[q]) * 3;
-            for (int c = 0; c < 3; ++c) sum[c] += w[q] * (double)v[c];
-        }
-        const double zeta = ramp * sum[0];
-        const double eta = zeta + ramp * ns.reference_level;
-        double u = qhu / qh, v = qhv / qh;
-        if (ns.velocity) {
-            u = ramp * sum[1];
-            v = ramp * sum[2];
-            const double pd = ns.depth[slot];
-            if (ns.max_ratio > 0.0 && std::isfinite(pd)) {
-                const double dp = zeta + pd, dc = eta - b;
-                if (dp > 0.0 && dc > 0.0) {
-                    double r = dp / dc;
-                    if (r < 1.0 / ns.max_ratio) r = 1.0 / ns.max_ratio;
-                    if (r > ns.max_ratio) r = ns.max_ratio;
-                    u *= r;
-                    v *= r;
-                }
-            }
-        }
+        double eta, u, v;
+        const bool has_velocity = parent_eval(slot, len, idx, w, ramp, b, eta, u, v);
+        if (!has_velocity) { u = qhu / qh; v = qhv / qh; }
         const double h = eta - b;
         if (h < ns.wet_depth) return {0.0, 0.0, 0.0};
         const double gamma = weight * ns.rate;
         const double dh = ns.momentum_only ? 0.0 : h - qh;
         const double depth = ns.momentum_only ? qh : h;
         return {gamma * dh, gamma * (depth * u - qhu), gamma * (depth * v - qhv)};
+    }
+
+    // OceanModelState::evaluate with parent_term and parent_state (ocean_nesting.rs:583-600, 837-880): the
+    // parent's level and, if it has one, its (transport-scaled) velocity at a slot over a child bed b
+    bool parent_eval(uint32_t slot, int len, const size_t* idx, const double* w, double ramp, double b, double& eta,
+                     double& u, double& v) const {
+        const Nesting& ns = nesting;
+        double sum[3] = {0.0, 0.0, 0.0};
+        for (int q = 0; q < len; ++q) {
+            const float* v = ns.series.data() + ((size_t)slot * ns.times.size() + idx[q]) * 3;
+            for (int c = 0; c < 3; ++c) sum[c] += w[q] * (double)v[c];
+        }
+        const double zeta = ramp * sum[0];
+        eta = zeta + ramp * ns.reference_level;
+        u = v = 0.0;
+        if (!ns.velocity) return false;
+        u = ramp * sum[1];
+        v = ramp * sum[2];
+        const double pd = ns.depth[slot];
+        if (ns.max_ratio > 0.0 && std::isfinite(pd)) {
+            const double dp = zeta + pd, dc = eta - b;
+            if (dp > 0.0 && dc > 0.0) {
+                double r = dp / dc;
+                if (r < 1.0 / ns.max_ratio) r = 1.0 / ns.max_ratio;
+                if (r > ns.max_ratio) r = ns.max_ratio;
+                u *= r;
+                v *= r;
+            }
+        }
+        return true;
     }
 
     // crate::math::hypot (math.rs:205-249): musl's hypot, the squares split exactly (Dekker)
@@ -330,6 +341,38 @@ struct Solver {
                            int fi = -1, double time = 0.0) const {
         double g = cfg.g, hm = cfg.h_min;
         T zero(0.0);
+        const uint32_t pslot = (nesting.on && nesting.boundary && fi >= 0)
+                                   ? nesting.slot[(size_t)k * ops.nn + ops.face_nodes[f][fi]] : 0xFFFFFFFFu;
+        if (tags[4 * k + f] == BC_TIDE && pslot != 0xFFFFFFFFu) {
+            // CharacteristicOBC<OceanModelState>: the parent's state at the face node's slot (external_state,
+            // ocean_nesting.rs:823-826), raised by the inverse barometer; with a parent velocity, u_n,e = u.n,
+            // u_t,e = -u n_y + v n_x (characteristic.rs:411-438)
+            size_t idx[4];
+            double w[4];
+            const int len = nesting_stencil(nesting.epoch + time, idx, w);
+            double ramp = 1.0;
+            if (nesting.ramp > 0.0) {
+                double tau = std::min(std::max(time / nesting.ramp, 0.0), 1.0);
+                ramp = tau * tau * (3.0 - 2.0 * tau);
+            }
+            double eta, ue, ve, raise;
+            const bool has_velocity = parent_eval(pslot, len, idx, w, ramp, val(q.b), eta, ue, ve);
+            if (barometer_raise(k, f, fi, time, raise)) eta += raise;
+            T h_e = vmax(T(eta) - q.b, zero);
+            T un_e, ut_e;
+            if (has_velocity) {
+                un_e = T(ue * nx + ve * ny);
+                ut_e = T(-ue * ny + ve * nx);
+            } else {
+                T u = zero, v = zero;
+                if (q.h > T(hm)) { T inv = T(1.0) / q.h; u = q.hu * inv; v = q.hv * inv; }
+                un_e = cfg.incoming_wave ? T(2.0) * (sq_sqrt(T(g) * vmax(-q.b, zero)) - sq_sqrt(T(g) * h_e)) : zero;
+                ut_e = -u * T(ny) + v * T(nx);
+            }
+            State<T> qb = characteristic_state(State<T>{q.h, q.hu, q.hv}, h_e, un_e, ut_e, T(nx), T(ny), g, hm);
+            NodeState<T> nb = node_state(qb.h, qb.hu, qb.hv, q.b, hm);
+            return wintermeyer_flux(nb, nb, T(nx), T(ny), g);
+        }
         const uint32_t slot = (atlas.n_c > 0 && fi >= 0) ? atlas.slot[((size_t)k * 4 + f) * ops.n1 + fi] : 0xFFFFFFFFu;
         if (tags[4 * k + f] == BC_TIDE && slot != 0xFFFFFFFFu) {
             // CharacteristicOBC<BoundaryTides>: the node's own tide; with velocity, u_n,e = u.n,

@@ -2,7 +2,7 @@
 for bit with a band over the open boundary relaxing towards a parent that
 varies in space and time (elevation only with linear snapshots; a velocity with
 its transport scaling, the datum shift, the ramp and cubic snapshots; momentum
-only), and a stage time the snapshots do not cover, the ensembles and the VJP
+only; the parent as the open boundary's external state too), and a stage time the snapshots do not cover, the ensembles and the VJP
 refused by name (local time stepping: tests/test_gpu_forcing_lts.py)."""
 import numpy as np
 import pytest
@@ -30,6 +30,7 @@ VARIANTS = [
     ("elevation", dict(timescale=300.0)),
     ("velocity", dict(timescale=120.0, velocity=True, max_ratio=2.0, reference_level=0.03, ramp=50.0, cubic=True)),
     ("momentum", dict(timescale=200.0, velocity=True, momentum_only=True, wet_depth=0.1)),
+    ("boundary", dict(timescale=150.0, velocity=True, max_ratio=2.0, cubic=True)),
 ]
 
 
@@ -42,6 +43,10 @@ def test_runs_equal_the_oracle(name, options):
     orc.set_nesting(ns)
     phys = swe2d.SWEPhysics2D.from_case(c)
     phys.set_nesting(ns)
+    if name == "boundary":
+        # the open boundary's external state from the parent too
+        orc.set_nesting_boundary(True)
+        phys.set_nesting_boundary(True)
     s = phys.state(c.h, c.hu, c.hv)
     x = [c.h.copy(), c.hu.copy(), c.hv.copy()]
     t_g = phys.run(s, 0.0, 60, c.cfl)

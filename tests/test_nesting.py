@@ -142,3 +142,26 @@ def test_the_band():
     h = orc.rhs(c.h, c.hu, c.hv, 0.0)[0]
     assert np.abs(h[~inside]).max() < 1e-14
     assert np.allclose(h[inside], w * 0.125 / 3600.0, rtol=1e-10, atol=1e-15)
+
+
+@pytest.mark.parametrize("u_p", [0.0, 0.5])
+def test_the_parent_as_the_open_boundary(u_p):
+    # the west face's external state is the parent's (level 0.125, velocity u_p along +x): with the interior at
+    # rest, w_out = 2 c_i and w_in = -u_p - 2 c_e, so u_n,b = c_i - c_e - u_p / 2 and c_b = (c_i + c_e) / 2 + u_p / 4
+    mesh = cases.structured_mesh(0.0, 400.0, 0.0, 100.0, 4, 1, sides=(cases.WALL, cases.WALL, cases.WALL, cases.TIDE))
+    x, _ = cases.node_coordinates(mesh, 2)
+    bed = np.full(x.size, -5.0)
+    h, hu, hv = cases.still_water(bed)
+    c = cases.Case("channel", 2, mesh, bed, np.zeros(mesh.n_elements), 0.0, cases.Tide(), h, hu, hv)
+    ns = cases.nesting_band(c, 1.0, [T0], lambda x, y, t: (0.125, u_p, 0.0), epoch=T0, velocity=u_p != 0.0)
+    ns.weight[:] = 0.0                      # the boundary alone, no relaxation
+    orc = O.Oracle.from_case(c, friction=False)
+    orc.set_nesting(ns)
+    orc.set_nesting_boundary(True)
+    fm = orc.face_mass(c.h, c.hu, c.hv, 0.0).ravel()
+    west = [(k * 4 + 3) * 3 + fi for k in range(c.n_elements) for fi in range(3) if c.mesh.links[k, 3] == cases.BOUNDARY]
+    ci, ce = math.sqrt(G * 5.0), math.sqrt(G * 5.125)
+    cb = (ci + ce) / 2.0 + u_p / 4.0
+    assert np.allclose(fm[west], cb * cb / G * (ci - ce - u_p / 2.0), rtol=1e-12, atol=1e-15)
+    orc.set_nesting_boundary(False)
+    assert np.abs(orc.face_mass(c.h, c.hu, c.hv, 0.0).ravel()[west]).max() < 1e-15
