@@ -123,6 +123,9 @@ def _bind(L):
                                       ctypes.POINTER(ctypes.c_float)] + [ctypes.c_int] * 3 + [ctypes.c_double] * 5)
         L.oc_set_inverse_barometer.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.oc_set_nesting_boundary.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.oc_set_relaxation_time.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.oc_set_beta.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L.oc_set_manning_nodal.argtypes = [ctypes.c_void_p, _dp]
         L.oc_hypot.argtypes = [ctypes.c_double, ctypes.c_double]
         L.oc_hypot.restype = ctypes.c_double
     return L
@@ -231,6 +234,19 @@ class Oracle:
     def set_nesting_boundary(self, on=True):
         """The open boundary's external state taken from the nesting parent at its face nodes' slots."""
         self._L.oc_set_nesting_boundary(self.h, int(on))
+
+    def set_relaxation_time(self, tau):
+        """The thin-layer relaxation time (wetting_drying.rs:82-89)."""
+        self._L.oc_set_relaxation_time(self.h, tau)
+
+    def set_beta(self, beta, y0=0.0):
+        """A beta-plane f = f0 + beta (y - y0) (coriolis.rs:101-121)."""
+        self._L.oc_set_beta(self.h, beta, y0)
+
+    def set_manning_nodal(self, n):
+        """Manning n per node [K*NN] (SpatiallyVaryingManning2D); None: per element."""
+        self._mn_node = None if n is None else np.ascontiguousarray(n, dtype=np.float64)
+        self._L.oc_set_manning_nodal(self.h, None if n is None else _p(self._mn_node))
 
     def tide_elevation(self, t):
         return self._L.oc_tide_elevation(self.h, t)

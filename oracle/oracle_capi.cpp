@@ -193,6 +193,20 @@ void oc_set_nesting_boundary(void* p, int on) { ((Handle*)p)->s->nesting.boundar
 
 double oc_hypot(double x, double y) { return Solver<double>::ref_hypot(x, y); }
 
+// The thin-layer relaxation time tau (WetDryConfig, wetting_drying.rs:82-89; 1 s by default)
+void oc_set_relaxation_time(void* p, double tau) { ((Handle*)p)->s->cfg.relaxation_time = tau; }
+// A beta-plane f = f0 + beta (y - y0) (CoriolisSource2D::beta_plane, coriolis.rs:101-121)
+void oc_set_beta(void* p, double beta, double y0) {
+    auto& c = ((Handle*)p)->s->cfg;
+    c.beta = beta; c.beta_y0 = y0;
+}
+// Manning n per node [K*nn] (SpatiallyVaryingManning2D::from_nodal, friction.rs:512-537); NULL: per element
+void oc_set_manning_nodal(void* p, const double* n) {
+    Solver<double>& s = *((Handle*)p)->s;
+    if (!n) { s.manning_node.clear(); return; }
+    s.manning_node.assign(n, n + (size_t)s.K * s.ops.nn);
+}
+
 void oc_step(void* p, double* h, double* hu, double* hv, double t, double dt, int scheme) {
     Handle* hd = (Handle*)p;
     hd->s->step(h, hu, hv, t, dt, scheme, hd->work);

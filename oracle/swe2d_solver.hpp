@@ -23,6 +23,7 @@ struct Config {
     double relaxation_time = 1.0, max_velocity = 20.0;  // wetting_drying.rs:82-89
     double friction_h_min = 1e-6;            // source/swe_2d/friction.rs:203-209
     double f0 = 0.0;                          // f-plane Coriolis (source/swe_2d/coriolis.rs:150-171)
+    double beta = 0.0, beta_y0 = 0.0;         // beta-plane f = f0 + beta (y - y0) (coriolis.rs:118-121,157-169)
     int surface = 0;                          // 0 hydrostatic HLL (WetDry), 1 entropy stable, 2 entropy conservative
     bool subcells = true;                     // WetDry subcell finite volumes
     bool friction = true;                     // Manning, point-implicit
@@ -116,6 +117,7 @@ struct Solver {
     std::vector<uint8_t> tags;
     std::vector<T> bed;           // K*nn
     std::vector<T> manning;       // K (per element Manning n)
+    std::vector<T> manning_node;  // K*nn per node Manning n (SpatiallyVaryingManning2D, friction.rs:463-553), or empty
     std::vector<State<T>> faces;  // K*4*n1: F* of interior faces, each element's own side
     uint64_t clips = 0;
 
@@ -632,12 +634,25 @@ struct Solver {
             ohu[n] = T(j_inv) * rhs[i].hu;
             ohv[n] = T(j_inv) * rhs[i].hv;
         }
-        if (cfg.f0 != 0.0) {
-            // coriolis.rs:150-171
+        if (cfg.f0 != 0.0 || cfg.beta != 0.0) {
+            // coriolis.rs:150-171: f0 on an f-plane; on a beta-plane f0 + beta (y - y0) at the node's
+            // position by the bilinear map (mesh2d.rs:578-595)
+            const double* v = corners.data() + 8 * k;
             for (int i = 0; i < nn; ++i) {
                 size_t n = (size_t)k * nn + i;
-                ohu[n] += T(cfg.f0) * hv[n];
-                ohv[n] += T(-cfg.f0) * hu[n];
+                double f = cfg.f0;
+                if (cfg.beta != 0.0) {
+                    const double r = ops.r[i], s = ops.s[i];
+                    const double n0 = (1.0 - r) * (1.0 - s) / 4.0, n1 = (1.0 + r) * (1.0 - s) / 4.0;
+                    const double n2 = (1.0 + r) * (1.0 + s) / 4.0, n3 = (1.0 - r) * (1.0 + s) / 4.0;
+                    const double y = n0 * v[1] + n1 * v[3] + n2 * v[5] + n3 * v[7];
+                    f = cfg.f0 + cfg.beta * (y - cfg.beta_y0);
+                }
+#ifdef ORACLE_MUTANT_CORIOLIS_SIGN
+                f = -f;  // planted mistake: the rotation turned the wrong way
+#endif
+                ohu[n] += T(f) * hv[n];
+                ohv[n] += T(-f) * hu[n];
             }
         }
         if (nesting.on) {
@@ -743,6 +758,9 @@ struct Solver {
         if (h >= T(cfg.h_dry)) return T(0.0);
         if (h <= T(0.0)) return T(std::numeric_limits<double>::infinity());
         T excess = T(cfg.h_dry) / h - T(1.0);
+#ifdef ORACLE_MUTANT_THIN_LAYER
+        return excess / T(cfg.relaxation_time);  // planted mistake: the excess not squared
+#endif
         return excess * excess / T(cfg.relaxation_time);
     }
     void correct_node(T h, T& hu, T& hv) const {
@@ -756,7 +774,7 @@ struct Solver {
         }
     }
     // wetting_drying.rs:324-363 damp_node (Manning through friction.rs:71-99,318-324)
-    void damp_node(int64_t k, T h, T& hu, T& hv, T fh, T fhu, T fhv, double dt) const {
+    void damp_node(int64_t k, int i, T h, T& hu, T& hv, T fh, T fhu, T fhv, double dt) const {
         if (h <= T(0.0)) { hu = T(0.0); hv = T(0.0); return; }
         T rate(0.0);
         if (cfg.friction) {
@@ -765,10 +783,14 @@ struct Solver {
             else { u = T(0.0); v = T(0.0); if (fh > T(cfg.h_min)) { T inv = T(1.0) / fh; u = fhu * inv; v = fhv * inv; } }
             T speed = sq_sqrt(u * u + v * v);
             if (speed > T(0.0)) {
-                T n = manning[k];
+                T n = manning_node.empty() ? manning[k] : manning_node[(size_t)k * ops.nn + i];
                 T g_n2 = T(cfg.g) * n * n;
                 T hh = vmax(h, T(cfg.friction_h_min));
+#ifdef ORACLE_MUTANT_MANNING_EXPONENT
+                rate += g_n2 * speed / hh;  // planted mistake: h^-1 instead of h^-4/3
+#else
                 rate += g_n2 * speed / (friction_cbrt(hh) * hh);
+#endif
             }
         }
         if (cfg.wet_dry) rate += thin_layer_rate(h);
@@ -832,7 +854,7 @@ struct Solver {
                     out[var][i] = v;
                 }
             if (cfg.friction || cfg.wet_dry)
-                for (int i = 0; i < nn; ++i) damp_node(k, out[0][i], out[1][i], out[2][i], inp[0][i], inp[1][i], inp[2][i], c);
+                for (int i = 0; i < nn; ++i) damp_node(k, i, out[0][i], out[1][i], out[2][i], inp[0][i], inp[1][i], inp[2][i], c);
             if (last) {
                 for (int var = 0; var < 3; ++var)
                     for (int i = 0; i < nn; ++i) out[var][i] = T(1.0) * out[var][i];
@@ -894,7 +916,7 @@ struct Solver {
                     }
                 if (cfg.friction || cfg.wet_dry)
                     for (int i = 0; i < nn; ++i)
-                        damp_node(k, out[0][i], out[1][i], out[2][i], inp[0][i], inp[1][i], inp[2][i], es.c);
+                        damp_node(k, i, out[0][i], out[1][i], out[2][i], inp[0][i], inp[1][i], inp[2][i], es.c);
                 if (es.accumulate) {
                     T* sum[3] = {ah + n0, ahu + n0, ahv + n0};
                     for (int var = 0; var < 3; ++var)
