@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -43,6 +44,32 @@ struct DeviceArray {
 
 // Workgroups of the subcell kernel (a grid-stride loop over its list)
 constexpr unsigned int SUB_GROUPS = 16384;
+
+// The fixed staging area of dg_masks_noalloc, one copy per GPU for the life of
+// the process. Host code never passes `scratch` itself to a kernel or a copy:
+// it is a host-side stand-in; its device address on each GPU is looked up once
+// (scratch_address) and kept in scratch_ptr.
+constexpr size_t SCRATCH_BYTES = (size_t)DG_NOALLOC_MAX_BYTES;
+static __device__ __align__(256) uint8_t scratch[SCRATCH_BYTES];
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mutex;
+void* scratch_ptr[MAX_DEVICES] = {};
+
+// The device address of `scratch` on `device`, which must be current.
+hipError_t scratch_address(int device, void** out) {
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[device]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        scratch_ptr[device] = ptr;
+    }
+    *out = scratch_ptr[device];
+    return hipSuccess;
+}
+}  // namespace
 
 struct dg_state {
     double* d = nullptr;  // 3*K*NN: h, hu, hv
@@ -78,6 +105,7 @@ struct dg_ctx {
     dg_state* work[4] = {nullptr, nullptr, nullptr, nullptr};
     bool timing = false;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    uint8_t* scratch = nullptr;    // this GPU's copy of the static staging area (scratch_address)
     double stage_ms = 0.0, dt_ms = 0.0;
     int64_t stage_n = 0, dt_n = 0;
     // discrete adjoint
@@ -999,6 +1027,7 @@ int dg_create(int device, dg_ctx** out) {
     DG_TRY(hipMemset(c->clips, 0, sizeof(unsigned long long)));
     DG_TRY(hipEventCreate(&c->ev[0]));
     DG_TRY(hipEventCreate(&c->ev[1]));
+    DG_TRY(scratch_address(device, (void**)&c->scratch));
     *out = c;
     return 0;
 }
@@ -1807,6 +1836,31 @@ int dg_masks(dg_ctx* c, const dg_state* s, uint8_t* node_dry, uint8_t* elem_subc
     DG_TRY(hipStreamSynchronize(c->stream));
     DG_TRY(hipFree(dn));
     DG_TRY(hipFree(de));
+    return 0;
+}
+
+int dg_masks_noalloc(dg_ctx* c, const dg_state* s, uint8_t* node_dry, uint8_t* elem_subcell) {
+    DG_REQUIRE(c && s && node_dry && elem_subcell, "dg_masks_noalloc: null argument");
+    const size_t nodes = (size_t)c->K * c->nn, need = nodes + (size_t)c->K;
+    if (need > SCRATCH_BYTES) {
+        g_err = "dg_masks_noalloc: the masks of " + std::to_string(c->K) + " elements take " + std::to_string(need) +
+                " bytes, more than the " + std::to_string(SCRATCH_BYTES) + "-byte static staging area";
+        return 2;
+    }
+    DG_REQUIRE(c->scratch, "dg_masks_noalloc: the context has no staging area");
+    // every byte of [0, need) is written by the kernel: nothing to clear
+    uint8_t *dn = c->scratch, *de = c->scratch + nodes;
+    dim3 grid((unsigned)((c->K + 255) / 256));
+    int sub = (c->flags & DG_SUBCELLS) ? 1 : 0;
+    switch (c->order) {
+        case 1: hipLaunchKernelGGL(masks_kernel<1>, grid, dim3(256), 0, c->stream, s->d, c->K, c->h_dry, sub, dn, de); break;
+        case 2: hipLaunchKernelGGL(masks_kernel<2>, grid, dim3(256), 0, c->stream, s->d, c->K, c->h_dry, sub, dn, de); break;
+        default: hipLaunchKernelGGL(masks_kernel<3>, grid, dim3(256), 0, c->stream, s->d, c->K, c->h_dry, sub, dn, de); break;
+    }
+    DG_TRY(hipGetLastError());
+    DG_TRY(hipMemcpyAsync(node_dry, dn, nodes, hipMemcpyDeviceToHost, c->stream));
+    DG_TRY(hipMemcpyAsync(elem_subcell, de, (size_t)c->K, hipMemcpyDeviceToHost, c->stream));
+    DG_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }
 

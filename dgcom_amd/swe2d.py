@@ -40,7 +40,7 @@ EXPORTS = (
     "dg_batch_tuning", "dg_begin_coarse_step", "dg_stage_where", "dg_finish_where", "dg_depth_flags",
     "dg_multirate_create", "dg_multirate_destroy", "dg_multirate_assign_levels", "dg_multirate_step",
     "dg_multirate_levels", "dg_multirate_stats_read", "dg_multirate_run", "dg_fnv1a", "dg_set_boundary_tides",
-    "dg_set_nesting", "dg_set_weather", "dg_set_inverse_barometer", "dg_set_nesting_boundary",
+    "dg_set_nesting", "dg_set_weather", "dg_set_inverse_barometer", "dg_set_nesting_boundary", "dg_masks_noalloc",
 )
 BATCH_GRAPH, BATCH_TUNE = 1, 2
 
@@ -113,6 +113,7 @@ def load(fma: bool = False):
     L.dg_is_finite.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int)]
     L.dg_clip_count.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
     L.dg_masks.argtypes = [vp, vp, _u8p, _u8p]
+    L.dg_masks_noalloc.argtypes = [vp, vp, _u8p, _u8p]
     L.dg_timing_enable.argtypes = [vp, ctypes.c_int]
     L.dg_timing_read.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_int64), _dp, ctypes.POINTER(ctypes.c_int64)]
     L.dg_sync.argtypes = [vp]
@@ -419,6 +420,14 @@ class SWEPhysics2D:
         es = np.empty(self.K, dtype=np.uint8)
         _check(self.lib, self.lib.dg_masks(self.ctx, state.handle, nd.ctypes.data_as(_u8p), es.ctypes.data_as(_u8p)),
                "dg_masks")
+        return nd, es
+
+    def masks_noalloc(self, state: DeviceState):
+        """`masks` staged through the library's static device area (no allocation on the device)."""
+        nd = np.empty(self.n_nodes_total, dtype=np.uint8)
+        es = np.empty(self.K, dtype=np.uint8)
+        _check(self.lib, self.lib.dg_masks_noalloc(self.ctx, state.handle, nd.ctypes.data_as(_u8p),
+                                                   es.ctypes.data_as(_u8p)), "dg_masks_noalloc")
         return nd, es
 
     # --- measurement ---

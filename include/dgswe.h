@@ -286,6 +286,14 @@ int dg_clip_count(dg_ctx* ctx, uint64_t* clips);
  * takes the subcell volume term (swe_2d_split_form.rs:711-713). */
 int dg_masks(dg_ctx* ctx, const dg_state* state, uint8_t* node_dry, uint8_t* elem_subcell);
 
+/* dg_masks without allocating, for callers that cannot call hipMalloc (tight serving loops, harnesses
+ * that forbid allocation): the same masks, staged through a fixed area of DG_NOALLOC_MAX_BYTES in the
+ * library's static device memory, one per GPU, held until the process ends. No counterpart in the
+ * reference. K*(NN+1) bytes at most (the call is refused beyond); calls that overlap on one GPU (from
+ * several threads or contexts) share that area and race. */
+#define DG_NOALLOC_MAX_BYTES (32u << 20)
+int dg_masks_noalloc(dg_ctx* ctx, const dg_state* state, uint8_t* node_dry, uint8_t* elem_subcell);
+
 /* Measurement: per-launch HIP-event timing of the stage kernel on the context's
  * stream. dg_timing_read returns the summed kernel milliseconds and launches since
  * the last reset. dg_copy_bench times a plain device copy of `bytes` (reps times). */
