@@ -283,9 +283,9 @@ static int launch_geometry(dg_ctx* c) {
     const int epb = 64 / c->nn;
     dim3 grid((unsigned)((c->K + epb - 1) / epb));
     switch (c->order) {
-        case 1: hipLaunchKernelGGL(geometry_kernel<1>, grid, dim3(64), 0, c->stream, c->corners, c->bed, c->affine, c->links, c->K, c->geo, c->fgeo); break;
-        case 2: hipLaunchKernelGGL(geometry_kernel<2>, grid, dim3(64), 0, c->stream, c->corners, c->bed, c->affine, c->links, c->K, c->geo, c->fgeo); break;
-        default: hipLaunchKernelGGL(geometry_kernel<3>, grid, dim3(64), 0, c->stream, c->corners, c->bed, c->affine, c->links, c->K, c->geo, c->fgeo); break;
+        case 1: hipLaunchKernelGGL(geometry_kernel<1>, grid, dim3(64), 0, c->stream, c->corners, c->affine, c->links, c->K, c->geo, c->fgeo); break;
+        case 2: hipLaunchKernelGGL(geometry_kernel<2>, grid, dim3(64), 0, c->stream, c->corners, c->affine, c->links, c->K, c->geo, c->fgeo); break;
+        default: hipLaunchKernelGGL(geometry_kernel<3>, grid, dim3(64), 0, c->stream, c->corners, c->affine, c->links, c->K, c->geo, c->fgeo); break;
     }
     DG_TRY(hipGetLastError());
     return 0;
@@ -1058,6 +1058,11 @@ int dg_upload_form(dg_ctx* c, const dg_form* f) {
     DG_REQUIRE(f->order >= 1 && f->order <= 3, "dg_upload_form: order must be 1, 2 or 3");
     DG_REQUIRE(f->n_elements > 0 && f->n_elements < (int64_t(1) << 28) - 1, "dg_upload_form: element count out of range");
     DG_REQUIRE(f->corners && f->face_links && f->face_tags && f->bed, "dg_upload_form: missing mesh, tags or bed");
+    // the kernels address per-node arrays (8 (N+1)^2 K bytes) and the corners (64 K bytes) by 32-bit byte offsets
+    DG_REQUIRE((uint64_t)f->n_elements * 8u * (uint64_t)((f->order + 1) * (f->order + 1)) < (uint64_t(1) << 32) &&
+                   (uint64_t)f->n_elements * 64u < (uint64_t(1) << 32),
+               "dg_upload_form: a per-node array of this mesh would exceed 4 GiB (at most 536,870,911 nodes and "
+               "67,108,863 elements per context)");
     DG_REQUIRE(!((f->flags & DG_ENTROPY_STABLE) && (f->flags & DG_SUBCELLS)),
                "dg_upload_form: the EntropyStable faces take no subcells (DG_SUBCELLS off)");
     DG_TRY(hipSetDevice(c->device));
@@ -1937,8 +1942,7 @@ int dg_set_bed(dg_ctx* c, const double* bed, int on_device) {
     DG_TRY(hipSetDevice(c->device));
     DG_TRY(hipMemcpyAsync(c->bed, bed, (size_t)c->K * c->nn * 8, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                           c->stream));
-    if (int r = launch_geometry(c)) return r;  // the volume bed-slope factors follow the bed
-    DG_TRY(hipStreamSynchronize(c->stream));
+    DG_TRY(hipStreamSynchronize(c->stream));  // (the kernels form the bed-slope factors from the bed itself)
     return 0;
 }
 

@@ -75,7 +75,7 @@ __global__ __launch_bounds__(64) DG_WET_OCCUPANCY void batch_wet_kernel(BatchArg
     const int64_t lb = (int64_t)(blk - block0);
     const int e = threadIdx.x / NN;
     const int64_t k = lb * EPB + (e < EPB ? e : 0);
-    stage_block<N, false>(p, k, e < EPB && lb * EPB + e < p.n_elements);
+    stage_block<N, false>(p, k < p.n_elements ? k : p.n_elements - 1, e < EPB && lb * EPB + e < p.n_elements);
 }
 
 // Compaction of the per-block subcell lists of one order, with the member of

@@ -27,9 +27,14 @@ __constant__ GllTables c_gll[4];
 
 constexpr uint32_t LINK_BOUNDARY = 0xFFFFFFFFu;
 
+// Node of face node fi of face f (faces 2 and 3 list their nodes in reverse):
+// a start node and a step of 1 (faces 0, 2) or N+1 (faces 1, 3), backwards on
+// faces 2 and 3; integer selects, no branch on a neighbour's face
 template <int N> __device__ __forceinline__ int face_node(int f, int fi) {
-    constexpr int N1 = N + 1;
-    return f == 0 ? fi : f == 1 ? fi * N1 + N1 - 1 : f == 2 ? (N1 - 1) * N1 + (N1 - 1 - fi) : (N1 - 1 - fi) * N1;
+    constexpr int N1 = N + 1, NN = N1 * N1;
+    const int step = (f & 1) ? fi * N1 : fi;
+    const int start = f == 0 ? 0 : f == 1 ? N : f == 2 ? NN - 1 : N * N1;
+    return (f & 2) ? start - step : start + step;
 }
 
 // The four corners of a quadrilateral, counter-clockwise from (r, s) = (-1, -1),
@@ -38,6 +43,32 @@ struct Quad { double x0, y0, x1, y1, x2, y2, x3, y3; };
 __device__ __forceinline__ Quad load_quad(const double* corners, int64_t k) {
     const double* c = corners + 8 * k;
     return {c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7]};
+}
+
+// Element data at a 32-bit byte offset from a kernel-argument base: one SGPR base
+// and one VGPR offset per access, no 64-bit address arithmetic per lane. Every
+// per-node array of a context is below 4 GiB (dg_upload_form refuses more).
+template <class V> __device__ __forceinline__ V ld32(const V* base, uint32_t byte_off) {
+    return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+template <class V> __device__ __forceinline__ void st32(V* base, uint32_t byte_off, V v) {
+    *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
+__device__ __forceinline__ Quad load_quad32(const double* corners, uint32_t k) {
+    const uint32_t o = k * 64u;
+    return {ld32(corners, o), ld32(corners, o + 8u), ld32(corners, o + 16u), ld32(corners, o + 24u),
+            ld32(corners, o + 32u), ld32(corners, o + 40u), ld32(corners, o + 48u), ld32(corners, o + 56u)};
+}
+
+// a / b from y, the correctly rounded 1/b (Markstein's theorem: q0 = a y, the
+// remainder r = a - b q0 is exact, and q0 + r y rounded once is the correctly
+// rounded quotient unless it underflows): the IEEE quotient bit for bit for the
+// metric's normal-range values, with one division shared by four. A zero
+// remainder keeps q0, so a zero numerator keeps its sign.
+__device__ __forceinline__ double div_by(double a, double b, double y) {
+    const double q0 = a * y;
+    const double r = fma(-b, q0, a);
+    return r == 0.0 ? q0 : fma(r, y, q0);
 }
 
 // Bilinear map derivatives at (r, s) (operators/geometric.rs:438-460)
@@ -67,9 +98,12 @@ template <int N> __device__ __forceinline__ void face_vector(const Quad& c, int 
     vy = f == 0 ? v0y : f == 1 ? v1y : f == 2 ? v2y : v3y;
 }
 
+// The per-node geometry cache of geometry_kernel: J and grad r, grad s of the
+// node's own map, for the time step and the wet/dry correction's masses. The
+// stage kernels evaluate their metric and bed-slope factors from the element's
+// corners (64 B per element) with the same expressions, so the same bits.
 enum {
-    GEO_RX, GEO_RY, GEO_SX, GEO_SY, GEO_DET, GEO_DINV, GEO_BXR, GEO_BYR, GEO_BXS, GEO_BYS,
-    GEO_TRX, GEO_TRY, GEO_TSX, GEO_TSY,  // grad r, grad s of the node's own map (the time step's metric)
+    GEO_DET, GEO_TRX, GEO_TRY, GEO_TSX, GEO_TSY,
     GEO_COUNT
 };
 
@@ -93,9 +127,8 @@ struct StageParams {
     unsigned long long* clips;
     double* fstar_out;   // optional [K][4][N+1][3]: F* at every face node, outward (this element's side)
     double *rh, *rhu, *rhv;  // optional: the stage's RHS, written by the forward, read by the VJP
-    // Static geometry cached per node, SoA [GEO_COUNT][K*NN] (geometry_kernel): the contravariant
-    // metric J grad r, J grad s as the volume term uses it, J, 1/J, and the volume bed-slope
-    // factors of the r- and s-line; per face [3][K*4]: the unit normal and sJ
+    // Static geometry cached per node, SoA [GEO_COUNT][K*NN] (geometry_kernel): J and the time
+    // step's grad r, grad s; per face [3][K*4]: the unit normal and sJ
     const double* geo;
     const double* fgeo;
     uint32_t* queue;     // [blocks][EPB]: the block's elements that take the subcell volume term
@@ -226,15 +259,16 @@ __device__ __forceinline__ void volume_line(Cons& rhs, int p, int lb, int ls, co
     // control (the coefficient tables hold 2 D_pq or zero by position):
     // subtracting a zero term from the node (q >= p) or adding one to the
     // node's own sum before its first term (q < p) leaves the value as it is
-    // (up to the sign of a zero result).
-    Cons acc = {0.0, 0.0, 0.0};
+    // (up to the sign of a zero result). The node's own sum starts from its first
+    // term, not from a zero it is added to.
+    Cons acc = cscale(T.vacc[p * N1], f[0]);
+    rhs = csub(rhs, cscale(T.vsub[p * N1], f[0]));
 #pragma unroll
-    for (int q = 0; q < N1; ++q) {
+    for (int q = 1; q < N1; ++q) {
         rhs = csub(rhs, cscale(T.vsub[p * N1 + q], f[q]));
         acc = cadd(acc, cscale(T.vacc[p * N1 + q], f[q]));
     }
-    // the bed slope's factors (bx, by) are static: geometry_kernel computed them
-    // with bed_line below
+    // the bed slope's factors (bx, by): bed_line below, before the passes
     const double force = g * me.h;
     acc = cadd(acc, Cons{0.0, force * bx, force * by});
     rhs = csub(rhs, acc);
@@ -451,6 +485,7 @@ namespace dgswe {
 // VJP: the same forward up to the element RHS, then the reverse (vjp_tail)
 // instead of the post-processing and the stores.
 // LTS: k is an element of the pass's list p.elist, lts_blk the list block.
+// k is a valid element on every lane (an inactive lane's is not used).
 template <int N, bool SUB, bool VJP = false, bool LTS = false, bool FORCE = false>
 __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, bool active,
                                             const VjpParams* vp = nullptr, int64_t lts_blk = 0) {
@@ -473,6 +508,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
     const int i = t - e * NN;
     const int a = i % N1, j = i / N1;
     const size_t n = (size_t)k * NN + i;
+    const uint32_t k32 = (uint32_t)k, nb = ((uint32_t)k * NN + i) * 8u;  // the node's byte offset in a state array
     const int eb = e * NN;
 
     // Every global load is issued here, before the first LDS exchange (whose
@@ -492,7 +528,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
     bool affine = false;
     int lv = 0, lr = 0;  // LTS: the element's level and last-stage rate
     if (active) {
-        affine = p.affine[k] != 0;
+        affine = ld32(p.affine, k32) != 0;
         if constexpr (LTS) {
             lv = p.level[k];
             if (p.ah) lr = p.lrate[k];
@@ -503,35 +539,29 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             lk[r] = LINK_BOUNDARY;
             tg[r] = 0;
             if (task < NF) {
-                lk[r] = p.links[4 * k + task / N1];
-                tg[r] = p.tags[4 * k + task / N1];
+                lk[r] = ld32(p.links, (4u * k32 + (uint32_t)(task / N1)) * 4u);
+                tg[r] = ld32(p.tags, 4u * k32 + (uint32_t)(task / N1));
             }
         }
-        hin = p.ih[n]; huin = p.ihu[n]; hvin = p.ihv[n]; bin = p.bed[n];
+        hin = ld32(p.ih, nb); huin = ld32(p.ihu, nb); hvin = ld32(p.ihv, nb); bin = ld32(p.bed, nb);
 #pragma unroll
         for (int r = 0; r < NT; ++r) {
             int task = i + r * NN;
             nh[r] = nhu[r] = nhv[r] = nbd[r] = 0.0;
-            if (task < NF && lk[r] != LINK_BOUNDARY) {
+            if (task < NF) {
+                // the neighbour's face node (at a boundary this node, unused)
                 int fi = task - (task / N1) * N1;
                 uint32_t l = lk[r];
-                size_t m = (size_t)(l >> 4) * NN + face_node<N>((int)(l & 3u), N1 - 1 - fi);
-                nh[r] = p.ih[m]; nhu[r] = p.ihu[m]; nhv[r] = p.ihv[m]; nbd[r] = p.bed[m];
+                const uint32_t mb = l == LINK_BOUNDARY ? nb
+                                    : ((l >> 4) * NN + (uint32_t)face_node<N>((int)(l & 3u), N1 - 1 - fi)) * 8u;
+                nh[r] = ld32(p.ih, mb); nhu[r] = ld32(p.ihu, mb); nhv[r] = ld32(p.ihv, mb); nbd[r] = ld32(p.bed, mb);
             }
         }
-        // the static metric of this node and the faces' unit normals and sJ,
-        // cached by geometry_kernel
+        // the static metric of this node from the element's corners (evaluated
+        // below); the faces' unit normals and sJ from the cache
         const size_t nt = (size_t)p.n_elements * NN;
-        g_rx = p.geo[GEO_RX * nt + n];
-        g_ry = p.geo[GEO_RY * nt + n];
-        g_sx = p.geo[GEO_SX * nt + n];
-        g_sy = p.geo[GEO_SY * nt + n];
+        c = load_quad32(p.corners, k32);
         if (VJP) det = p.geo[GEO_DET * nt + n];  // the forward reads J only where the wet/dry correction needs it
-        det_inv = p.geo[GEO_DINV * nt + n];
-        bxr = p.geo[GEO_BXR * nt + n];
-        byr = p.geo[GEO_BYR * nt + n];
-        bxs = p.geo[GEO_BXS * nt + n];
-        bys = p.geo[GEO_BYS * nt + n];
         if (!VJP) {
             const size_t nf4 = (size_t)p.n_elements * 4;
             // (with two face tasks per lane, N < 3, they are loaded at use:
@@ -541,12 +571,12 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
                 const int task = i + r * NN;
                 fnx[r] = fny[r] = fsj[r] = 0.0;
                 if (task < NF) {
-                    const size_t fk = (size_t)k * 4 + task / N1;
-                    fnx[r] = p.fgeo[fk]; fny[r] = p.fgeo[nf4 + fk]; fsj[r] = p.fgeo[2 * nf4 + fk];
+                    const uint32_t fb = (4u * k32 + (uint32_t)(task / N1)) * 8u;
+                    fnx[r] = ld32(p.fgeo, fb); fny[r] = ld32(p.fgeo + nf4, fb); fsj[r] = ld32(p.fgeo + 2 * nf4, fb);
                 }
             }
-            if (NT == 1 && !p.rhs_only && p.a != 0.0) { bh0 = p.bh[n]; bhu0 = p.bhu[n]; bhv0 = p.bhv[n]; }
-            if (NT == 1 && (p.flags & DG_FRICTION)) mann = p.manning[k];
+            if (NT == 1 && !p.rhs_only && p.a != 0.0) { bh0 = ld32(p.bh, nb); bhu0 = ld32(p.bhu, nb); bhv0 = ld32(p.bhv, nb); }
+            if (NT == 1 && (p.flags & DG_FRICTION)) mann = ld32(p.manning, k32 * 8u);
         } else {
             // the reverse's own reads: the saved RHS, the base state, Manning n
             // and the output cotangent
@@ -557,6 +587,17 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             if (!p.rhs_only && p.a != 0.0 && vp->gbh) { vpre.gbh = vp->gbh[n]; vpre.gbhu = vp->gbhu[n]; vpre.gbhv = vp->gbhv[n]; }
             vpre.gbed = vp->gbed[n];
         }
+    }
+    if (active) {
+        // GeometricFactors2D::compute's expressions (GeometricFactors2D::compute, geometric.rs:150-175): the
+        // contravariant metric J grad r, J grad s and 1/J of the node's map (a parallelogram's at node 0)
+        const double r = affine ? T.x1[0] : T.x1[a], s = affine ? T.x1[0] : T.x1[j];
+        const Metric mm = bilinear(c, r, s);
+        det_inv = 1.0 / mm.det;
+        g_rx = mm.det * div_by(mm.y_s, mm.det, det_inv);
+        g_ry = mm.det * div_by(-mm.x_s, mm.det, det_inv);
+        g_sx = mm.det * div_by(-mm.y_r, mm.det, det_inv);
+        g_sy = mm.det * div_by(mm.x_r, mm.det, det_inv);
     }
     if (t < EPB) { sflag[t] = 0; wflag[t] = 0; }
     if constexpr (LTS && !SUB) {
@@ -593,6 +634,11 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
 
     // Lines through this node: the r-line (lanes eb + j*N1 + q) and the s-line (eb + q*N1 + a)
     const int rlb = eb + j * N1, slb = eb + a;
+    if (!SUB && !VJP && active) {
+        // the volume bed-slope factors of the node's r- and s-line
+        bed_line<N>(a, rlb, 1, sb, srx, sry, affine, bxr, byr);
+        bed_line<N>(j, slb, N1, sb, ssx, ssy, affine, bxs, bys);
+    }
 
     // Faces: F* at the element's face nodes, spread over its lanes (the VJP
     // reads the RHS that its forward saved instead)
@@ -874,7 +920,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
     const bool friction = (p.flags & DG_FRICTION) != 0, wet_dry = (p.flags & DG_WET_DRY) != 0;
     if (active) {
         if (p.a != 0.0) {
-            if (NT != 1) { bh0 = p.bh[n]; bhu0 = p.bhu[n]; bhv0 = p.bhv[n]; }
+            if (NT != 1) { bh0 = ld32(p.bh, nb); bhu0 = ld32(p.bhu, nb); bhv0 = ld32(p.bhv, nb); }
             vh = p.a * bh0; vhu = p.a * bhu0; vhv = p.a * bhv0;
         }
         if (p.b != 0.0) { vh += p.b * hin; vhu += p.b * huin; vhv += p.b * hvin; }
@@ -885,11 +931,15 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
                 double rate = 0.0;
                 if (friction) {
                     double u, v;
-                    if (wet_dry) desing_velocity(hin, huin, hvin, p.h_dry, u, v);
+                    if (wet_dry && !SUB && !LTS && hin >= p.h_dry) {
+                        // desingularized_velocity of a node at or above h_dry (> h_min) is make_node's
+                        // hu (1/h), already in LDS
+                        u = su[t]; v = sv[t];
+                    } else if (wet_dry) desing_velocity(hin, huin, hvin, p.h_dry, u, v);
                     else { u = 0.0; v = 0.0; if (hin > p.h_min) { double inv = 1.0 / hin; u = huin * inv; v = hvin * inv; } }
                     double speed = sqrt(u * u + v * v);
                     if (speed > 0.0) {
-                        double mn = NT == 1 ? mann : p.manning[k];
+                        double mn = NT == 1 ? mann : ld32(p.manning, k32 * 8u);
                         double gn2 = p.g * mn * mn;
                         double hh = fmax_(vh, p.h_min);
                         rate += gn2 * speed / (manning_cbrt(hh) * hh);
@@ -923,40 +973,27 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
         wet_dry_correct<N>(p.h_dry, p.u_max, p.geo, p.n_elements, p.clips, active, n, vh, vhu, vhv, sh, shu, shv, pw,
                            wflag, savg);
     }
-    if (active) { p.oh[n] = vh; p.ohu[n] = vhu; p.ohv[n] = vhv; }
+    if (active) { st32(p.oh, nb, vh); st32(p.ohu, nb, vhu); st32(p.ohv, nb, vhv); }
 }
 
-// The static geometry cache of the stage kernels, with the stage kernel's own
-// expressions (so the cached values are its bits): per node the contravariant
-// metric J grad r, J grad s (the affine element's node-0 metric), J at the node,
-// 1/J, and the volume bed-slope factors of its r- and s-line; per face the unit
-// normal and sJ. Run once per form and after every change of the bed.
+// The static geometry cache, with GeometricFactors2D::compute's expressions (so the
+// cached values are its bits): per node J and grad r, grad s of the node's own
+// map; per face the unit normal and sJ; per face node the subcell reconstruction's
+// outer-node distance. Run once per form.
 template <int N>
-__global__ __launch_bounds__(64) void geometry_kernel(const double* corners, const double* bed, const uint8_t* affine_flags,
+__global__ __launch_bounds__(64) void geometry_kernel(const double* corners, const uint8_t* affine_flags,
                                                       const uint32_t* links, int64_t K, double* geo, double* fgeo) {
     constexpr int N1 = N + 1, NN = N1 * N1, EPB = 64 / NN;
     const GllTables& T = c_gll[N];
-    __shared__ double sb[64], srx[64], sry[64], ssx[64], ssy[64];
-    const int t = threadIdx.x, e = t / NN, i = t - e * NN, a = i % N1, j = i / N1, eb = e * NN;
+    const int t = threadIdx.x, e = t / NN, i = t - e * NN, a = i % N1, j = i / N1;
     const int64_t k = (int64_t)blockIdx.x * EPB + e;
     const bool active = e < EPB && k < K;
     const size_t n = (size_t)k * NN + i, nt = (size_t)K * NN;
-    bool affine = false;
     if (active) {
-        affine = affine_flags[k] != 0;
+        const bool affine = affine_flags[k] != 0;
         const Quad c = load_quad(corners, k);
-        Metric m = bilinear(c, T.x1[a], T.x1[j]);
-        Metric mm = affine ? bilinear(c, T.x1[0], T.x1[0]) : m;
-        const double rx = mm.det * (mm.y_s / mm.det), ry = mm.det * (-mm.x_s / mm.det);
-        const double sx = mm.det * (-mm.y_r / mm.det), sy = mm.det * (mm.x_r / mm.det);
-        srx[t] = rx; sry[t] = ry; ssx[t] = sx; ssy[t] = sy;
-        sb[t] = bed[n];
-        geo[GEO_RX * nt + n] = rx;
-        geo[GEO_RY * nt + n] = ry;
-        geo[GEO_SX * nt + n] = sx;
-        geo[GEO_SY * nt + n] = sy;
+        const Metric m = bilinear(c, T.x1[a], T.x1[j]);
         geo[GEO_DET * nt + n] = m.det;
-        geo[GEO_DINV * nt + n] = 1.0 / mm.det;
         geo[GEO_TRX * nt + n] = m.y_s / m.det;
         geo[GEO_TRY * nt + n] = -m.x_s / m.det;
         geo[GEO_TSX * nt + n] = -m.y_r / m.det;
@@ -979,16 +1016,6 @@ __global__ __launch_bounds__(64) void geometry_kernel(const double* corners, con
                 l == LINK_BOUNDARY ? 0.0 : outer_distance<N>(corners, affine_flags, c, affine, l, f, fi);
         }
     }
-    lds_sync();
-    if (active) {
-        double bx, by;
-        bed_line<N>(a, eb + j * N1, 1, sb, srx, sry, affine, bx, by);
-        geo[GEO_BXR * nt + n] = bx;
-        geo[GEO_BYR * nt + n] = by;
-        bed_line<N>(j, eb + a, N1, sb, ssx, ssy, affine, bx, by);
-        geo[GEO_BXS * nt + n] = bx;
-        geo[GEO_BYS * nt + n] = by;
-    }
 }
 
 // Elements with every node at or above h_dry, in mesh order (XCD-aware block
@@ -1010,7 +1037,9 @@ __global__ __launch_bounds__(64) DG_WET_OCCUPANCY void wet_stage_kernel(StagePar
     if (blk * EPB >= p.n_elements) return;
     const int e = threadIdx.x / NN;
     const int64_t k = blk * EPB + (e < EPB ? e : 0);
-    stage_block<N, false, false, false, FORCE>(p, k, e < EPB && blk * EPB + e < p.n_elements);
+    // (a lane past the mesh's end works on the last element and writes nothing)
+    stage_block<N, false, false, false, FORCE>(p, k < p.n_elements ? k : p.n_elements - 1,
+                                               e < EPB && blk * EPB + e < p.n_elements);
 }
 
 // A pass of local time stepping: the listed elements (p.elist[0..n_list)),
@@ -1153,7 +1182,8 @@ __global__ __launch_bounds__(64) DG_WAVES_PER_EU(N == 3 ? 3 : 4) void wet_vjp_ke
     if (blk * EPB >= P.s.n_elements) return;
     const int e = threadIdx.x / NN;
     const int64_t k = blk * EPB + (e < EPB ? e : 0);
-    stage_block<N, false, true>(P.s, k, e < EPB && blk * EPB + e < P.s.n_elements, &P);
+    stage_block<N, false, true>(P.s, k < P.s.n_elements ? k : P.s.n_elements - 1,
+                                e < EPB && blk * EPB + e < P.s.n_elements, &P);
 }
 
 template <int N>

@@ -22,9 +22,9 @@ static void run_geometry(int order, int64_t K, StageParams& p, std::vector<doubl
     int epb = 64 / (int)nn;
     unsigned grid = (unsigned)((K + epb - 1) / epb);
     switch (order) {
-        case 1: emu_launch(grid, 64, [&] { geometry_kernel<1>(p.corners, p.bed, p.affine, p.links, K, geo.data(), fgeo.data()); }); break;
-        case 2: emu_launch(grid, 64, [&] { geometry_kernel<2>(p.corners, p.bed, p.affine, p.links, K, geo.data(), fgeo.data()); }); break;
-        default: emu_launch(grid, 64, [&] { geometry_kernel<3>(p.corners, p.bed, p.affine, p.links, K, geo.data(), fgeo.data()); }); break;
+        case 1: emu_launch(grid, 64, [&] { geometry_kernel<1>(p.corners, p.affine, p.links, K, geo.data(), fgeo.data()); }); break;
+        case 2: emu_launch(grid, 64, [&] { geometry_kernel<2>(p.corners, p.affine, p.links, K, geo.data(), fgeo.data()); }); break;
+        default: emu_launch(grid, 64, [&] { geometry_kernel<3>(p.corners, p.affine, p.links, K, geo.data(), fgeo.data()); }); break;
     }
     p.geo = geo.data();
     p.fgeo = fgeo.data();
