@@ -182,28 +182,12 @@ def adjoint_ensemble_bench(n_members, n_steps, snapshots):
             "misfits": [float(J) for J, _, _ in grads], "setup_s": setup}
 
 
-def ensemble_members(n_members, seed, k_lo, k_hi, n_steps):
-    """Seeded members: K log-uniform in [k_lo, k_hi] jittered quads of 100 m, N uniform
-    in {1, 2, 3}, M2 tide amplitude uniform in 0.2-1.5 m and phase in 0-2pi, GRF bed."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for m in range(n_members):
-        K = float(np.exp(rng.uniform(np.log(k_lo), np.log(k_hi))))
-        n = max(4, int(round(np.sqrt(K))))
-        order = int(rng.integers(1, 4))
-        amp, phase = float(rng.uniform(0.2, 1.5)), float(rng.uniform(0.0, 2.0 * np.pi))
-        c = cases.large_unstructured(n=n, length=100.0 * n, order=order, seed=seed + 1000 + 17 * m, n_steps=n_steps)
-        c.tide = cases.Tide(amplitude=(amp,), omega=(2.0 * np.pi / cases.M2_PERIOD_S,), phase=(phase,), ramp=3600.0)
-        out.append(c)
-    return out
-
-
 def ensemble_bench(n_members, n_steps, k_lo, k_hi):
     """An ensemble of uneven members advanced together (one launch per order per
     kernel, graph-replayed stages, tuned launch shapes) against the same members
     run one after another (one run per launch sequence)."""
     t0 = time.time()
-    members = ensemble_members(n_members, 99, k_lo, k_hi, n_steps)
+    members = cases.ensemble_members(n_members, 99, k_lo, k_hi, n_steps)
     phys = [swe2d.SWEPhysics2D.from_case(c) for c in members]
     setup = time.time() - t0
     dofs = sum(3 * c.n_elements * c.nn * STAGES * n_steps for c in members)

@@ -284,6 +284,23 @@ def drying_beach(nx=24, ny=6, order=2, slope=0.01, length=2000.0, width=500.0, e
                 cfl=0.5, scheme=0, n_steps=n_steps)
 
 
+
+def ensemble_members(n_members=64, seed=99, k_lo=20_000, k_hi=400_000, n_steps=500) -> list:
+    """The batched forward ensemble: seeded members with K log-uniform in [k_lo, k_hi]
+    jittered quads of 100 m, N uniform in {1, 2, 3}, an M2 tide of amplitude uniform
+    in 0.2-1.5 m and phase in 0-2pi, and their own Gaussian-random-field beds."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for m in range(n_members):
+        K = float(np.exp(rng.uniform(np.log(k_lo), np.log(k_hi))))
+        n = max(4, int(round(np.sqrt(K))))
+        order = int(rng.integers(1, 4))
+        amp, phase = float(rng.uniform(0.2, 1.5)), float(rng.uniform(0.0, 2.0 * np.pi))
+        c = large_unstructured(n=n, length=100.0 * n, order=order, seed=seed + 1000 + 17 * m, n_steps=n_steps)
+        c.tide = Tide(amplitude=(amp,), omega=(2.0 * np.pi / M2_PERIOD_S,), phase=(phase,), ramp=3600.0)
+        out.append(c)
+    return out
+
 # ---------------------------------------------------------------------------
 # Thacker's planar surface in a paraboloid (tests/wet_dry_2d_test.rs:217-310)
 # ---------------------------------------------------------------------------
