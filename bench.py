@@ -224,6 +224,38 @@ def ensemble_bench(n_members, n_steps, k_lo, k_hi):
             "setup_s": setup}
 
 
+def c1_bench():
+    """The idealised tidal estuary (100 x 20 affine quads, N = 2, 0.5 m M2 tide, 25 h from rest): the steps that
+    reach 25 h, timed on the GPU and in the CPU restatement on all host threads, and whether the two end states
+    are equal bit for bit."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as checker
+
+    c = cases.tidal_estuary()
+    phys = swe2d.SWEPhysics2D.from_case(c)
+    s = phys.state(c.h, c.hu, c.hv)
+    _, dts = phys.run(s, 0.0, 100_000, c.cfl, record_dt=True)
+    n = int(np.searchsorted(np.cumsum(dts), 25 * 3600.0)) + 1
+    s.upload(c.h, c.hu, c.hv)
+    phys.sync()
+    w0 = time.perf_counter()
+    t_gpu = phys.run(s, 0.0, n, c.cfl)
+    phys.sync()
+    gpu_s = time.perf_counter() - w0
+    got = s.download()
+    orc = checker.Oracle.from_case(c)
+    h, hu, hv = c.h.copy(), c.hu.copy(), c.hv.copy()
+    t_cpu, cpu_s = orc.run_timed(h, hu, hv, 0.0, n, c.cfl)
+    equal = t_gpu == t_cpu and all(np.array_equal(a, b) for a, b in zip(got, (h, hu, hv)))
+    dofs = 3 * c.n_elements * c.nn * STAGES * n
+    phys.close()
+    return {"workload": "idealised tidal estuary: 100x20 affine quads over 10 km x 2 km, N=2, depth 10 -> 2 m, "
+                        "0.5 m M2 tide (1 h ramp), Manning 0.025, f=1.2e-4, SSP-RK3 CFL 0.5, 25 h from rest",
+            "steps": n, "model_time_s": t_gpu, "gpu_s": gpu_s, "gpu_ms_per_step": gpu_s / n * 1e3,
+            "gpu_dof_updates_per_s": dofs / gpu_s, "cpu_s": cpu_s, "cpu_dof_updates_per_s": dofs / cpu_s,
+            "cpu_cores": checker.max_threads(), "cpu_kind": "port", "bitwise_equal": bool(equal)}
+
+
 def lts_bench(nx, ny, t_end, max_levels):
     """Local time stepping against global steps over the same model time: the
     reference's tidal channel refined to nx x ny elements (N = 2, SSP-RK(4,3) at
@@ -318,7 +350,8 @@ def main():
     ap.add_argument("--ensemble-members", type=int, default=64)
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--no-ensemble", action="store_true")
-    ap.add_argument("--adjoint-ensemble-members", type=int, default=0,
+    ap.add_argument("--no-c1", action="store_true", help="skip the 25 h tidal estuary (GPU and CPU)")
+    ap.add_argument("--adjoint-ensemble-members", type=int, default=32,
                     help="members of the inversion ensemble's gradient timing (0: not run)")
     ap.add_argument("--adjoint-ensemble-steps", type=int, default=1000)
     ap.add_argument("--lts", default="800x80,2000x200", help="refined tidal channels for the local-time-stepping "
@@ -414,6 +447,8 @@ def main():
     if args.lts and args.lts != "none":
         out["lts"] = [lts_bench(*map(int, spec.split("x")), args.lts_seconds, args.lts_levels)
                       for spec in args.lts.split(",")]
+    if not args.no_c1:
+        out["c1"] = c1_bench()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -153,6 +153,12 @@ struct dg_ctx {
     bool lvl_moment_used[MOMENT_RING] = {};
     int lvl_moment_next = 0;
     double nest_rate = 0.0, nest_wet_depth = 0.05, nest_max_ratio = 0.0, nest_reference = 0.0, nest_ramp = 0.0;
+    // Manning n per node and a beta-plane's f per node (dg_set_manning_nodal, dg_set_coriolis_beta), or NULL
+    double *manning_node = nullptr, *coriolis_node = nullptr;
+    double beta = 0.0, beta_y0 = 0.0;
+    // every entry point on the context holds this (the reference guards its device module's scratch with a
+    // Mutex, gpu/src/swe2d.rs:3689): calls from several host threads are serialised
+    std::recursive_mutex mu;
     // stations
     int n_st = 0;
     int64_t* st_elem = nullptr;
@@ -167,6 +173,11 @@ struct dg_ctx {
             return 1;                                                                             \
         }                                                                                         \
     } while (0)
+
+// Every entry point on a context holds its mutex for the call (recursive: an entry point may call another)
+#define DG_LOCK(ctx)                                                   \
+    DG_REQUIRE((ctx) != nullptr, "null context");                      \
+    std::lock_guard<std::recursive_mutex> dg_lock_((ctx)->mu)
 
 #define DG_REQUIRE(cond, msg)       \
     do {                            \
@@ -221,7 +232,7 @@ static int free_statics(dg_ctx* c) {
                     c->innerbuf, c->facecot, c->kind, c->st_elem, c->st_w, c->st_eta, c->geo, c->fgeo,
                     c->lts_level, c->lts_lrate, c->lts_order, c->depth_flags, c->tide_slot, c->tide_coef,
                     c->tide_mean, c->tide_src, c->nest_slot, c->nest_series, c->nest_weight, c->nest_depth,
-                    c->wx_series};
+                    c->wx_series, c->manning_node, c->coriolis_node};
     for (void* p : ptrs)
         if (p) DG_TRY(hipFree(p));
     c->gbed = c->gman = c->facebuf = c->innerbuf = c->facecot = nullptr;
@@ -229,6 +240,8 @@ static int free_statics(dg_ctx* c) {
     c->lts_level = c->lts_lrate = c->depth_flags = nullptr;
     c->tide_slot = nullptr;
     c->tide_coef = c->tide_mean = c->tide_src = nullptr;
+    c->manning_node = c->coriolis_node = nullptr;
+    c->beta = c->beta_y0 = 0.0;
     c->atlas_nc = 0;
     c->nest_slot = nullptr;
     c->nest_series = nullptr;
@@ -430,7 +443,9 @@ static int launch_stage(dg_ctx* c, const double* base, const double* input, doub
     p.xcd_stride = (nblk + 7) / 8;
     dim3 grid(8 * p.xcd_stride), sub_grid(std::min(nblk, SUB_GROUPS));
     // the forcing given per node compiled in only where some is set
-    const bool force = c->tide_slot || c->nest_slot || c->wx_series;
+    const bool force = c->tide_slot || c->nest_slot || c->wx_series || c->manning_node || c->coriolis_node;
+    p.manning_node = c->manning_node;
+    p.coriolis_node = c->coriolis_node;
     if (c->timing) DG_TRY(hipEventRecord(c->ev[0], c->stream));
     switch (c->order) {
         case 1: hipLaunchKernelGGL((force ? wet_stage_kernel<1, true> : wet_stage_kernel<1>), grid, dim3(64), 0, c->stream, p); break;
@@ -571,6 +586,8 @@ static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, 
     DG_REQUIRE(!c->tide_slot, "the boundary tides per node have no VJP here (the adjoint is of the uniform tide)");
     DG_REQUIRE(!c->nest_slot, "the nesting band has no VJP here");
     DG_REQUIRE(!c->wx_series, "the gridded weather has no VJP here");
+    DG_REQUIRE(!c->manning_node, "Manning n per node has no VJP here (the gradient is per element n)");
+    DG_REQUIRE(!c->coriolis_node, "the beta-plane has no VJP here (the adjoint is of the f-plane)");
     if (int r = ensure_adjoint(c)) return r;
     size_t nt = (size_t)c->K * c->nn;
     VjpParams P{};
@@ -1054,6 +1071,7 @@ int dg_destroy(dg_ctx* c) {
 }
 
 int dg_upload_form(dg_ctx* c, const dg_form* f) {
+    DG_LOCK(c);
     DG_REQUIRE(c && f, "dg_upload_form: null argument");
     DG_REQUIRE(f->order >= 1 && f->order <= 3, "dg_upload_form: order must be 1, 2 or 3");
     DG_REQUIRE(f->n_elements > 0 && f->n_elements < (int64_t(1) << 28) - 1, "dg_upload_form: element count out of range");
@@ -1130,6 +1148,7 @@ int dg_upload_form(dg_ctx* c, const dg_form* f) {
 }
 
 int dg_set_tide(dg_ctx* c, const dg_tide* t) {
+    DG_LOCK(c);
     DG_REQUIRE(c && t, "dg_set_tide: null argument");
     DG_REQUIRE(t->n >= 0 && t->n <= 8, "dg_set_tide: at most 8 constituents");
     c->tide = *t;
@@ -1142,6 +1161,7 @@ int dg_set_tide(dg_ctx* c, const dg_tide* t) {
 
 int dg_set_tide_nodal(dg_ctx* c, const double* f0, const double* f_rate, const double* v0_deg, const double* u0_deg,
                       const double* u_rate_deg) {
+    DG_LOCK(c);
     DG_REQUIRE(c && f0 && f_rate && v0_deg && u0_deg && u_rate_deg, "dg_set_tide_nodal: null argument");
     for (int i = 0; i < c->tide.n; ++i) {
         c->tide_nodal[0][i] = f0[i];
@@ -1154,6 +1174,7 @@ int dg_set_tide_nodal(dg_ctx* c, const double* f0, const double* f_rate, const d
 }
 
 int dg_state_alloc(dg_ctx* c, dg_state** s) {
+    DG_LOCK(c);
     DG_REQUIRE(c && s && c->K > 0, "dg_state_alloc: upload a form first");
     DG_TRY(hipSetDevice(c->device));
     dg_state* st = new dg_state;
@@ -1164,6 +1185,7 @@ int dg_state_alloc(dg_ctx* c, dg_state** s) {
 }
 
 int dg_state_free(dg_ctx* c, dg_state* s) {
+    DG_LOCK(c);
     if (!s) return 0;
     DG_TRY(hipSetDevice(c->device));
     DG_TRY(hipStreamSynchronize(c->stream));
@@ -1173,6 +1195,7 @@ int dg_state_free(dg_ctx* c, dg_state* s) {
 }
 
 int dg_state_upload(dg_ctx* c, dg_state* s, const double* h, const double* hu, const double* hv) {
+    DG_LOCK(c);
     DG_REQUIRE(c && s && h && hu && hv, "dg_state_upload: null argument");
     size_t nt = (size_t)c->K * c->nn;
     DG_TRY(hipMemcpyAsync(s->d, h, nt * 8, hipMemcpyHostToDevice, c->stream));
@@ -1183,6 +1206,7 @@ int dg_state_upload(dg_ctx* c, dg_state* s, const double* h, const double* hu, c
 }
 
 int dg_state_download(dg_ctx* c, const dg_state* s, double* h, double* hu, double* hv) {
+    DG_LOCK(c);
     DG_REQUIRE(c && s && h && hu && hv, "dg_state_download: null argument");
     size_t nt = (size_t)c->K * c->nn;
     DG_TRY(hipMemcpyAsync(h, s->d, nt * 8, hipMemcpyDeviceToHost, c->stream));
@@ -1193,12 +1217,14 @@ int dg_state_download(dg_ctx* c, const dg_state* s, double* h, double* hu, doubl
 }
 
 int dg_state_copy(dg_ctx* c, dg_state* dst, const dg_state* src) {
+    DG_LOCK(c);
     DG_REQUIRE(c && dst && src, "dg_state_copy: null argument");
     DG_TRY(hipMemcpyAsync(dst->d, src->d, (size_t)3 * c->K * c->nn * 8, hipMemcpyDeviceToDevice, c->stream));
     return 0;
 }
 
 int dg_rhs(dg_ctx* c, const dg_state* in, double t, dg_state* out) {
+    DG_LOCK(c);
     DG_REQUIRE(c && in && out && c->K > 0, "dg_rhs: null argument or no form");
     DG_REQUIRE(in != out, "dg_rhs: in and out must differ");
     DG_TRY(hipSetDevice(c->device));
@@ -1208,6 +1234,7 @@ int dg_rhs(dg_ctx* c, const dg_state* in, double t, dg_state* out) {
 }
 
 int dg_face_fluxes(dg_ctx* c, const dg_state* in, double t, double* out) {
+    DG_LOCK(c);
     DG_REQUIRE(c && in && out && c->K > 0, "dg_face_fluxes: null argument or no form");
     DG_TRY(hipSetDevice(c->device));
     size_t n = (size_t)c->K * 4 * (c->order + 1) * 3;
@@ -1222,6 +1249,7 @@ int dg_face_fluxes(dg_ctx* c, const dg_state* in, double t, double* out) {
 }
 
 int dg_stage(dg_ctx* c, const dg_state* base, const dg_state* input, dg_state* out, const dg_stage_desc* st) {
+    DG_LOCK(c);
     DG_REQUIRE(c && base && input && out && st && c->K > 0, "dg_stage: null argument or no form");
     DG_REQUIRE(out != input && out != base, "dg_stage: out must differ from base and input");
     DG_TRY(hipSetDevice(c->device));
@@ -1235,6 +1263,7 @@ int dg_stage(dg_ctx* c, const dg_state* base, const dg_state* input, dg_state* o
 // SWEPhysics2D implements it, src/physics/builder.rs:546-783)
 // ---------------------------------------------------------------------------
 int dg_begin_coarse_step(dg_ctx* c, const uint8_t* level, const uint8_t* rates, int n_stages) {
+    DG_LOCK(c);
     DG_REQUIRE(c && level && rates && c->K > 0, "dg_begin_coarse_step: null argument or no form");
     DG_REQUIRE(n_stages == 3 || n_stages == 4, "dg_begin_coarse_step: 3 or 4 stages");
     const int64_t K = c->K;
@@ -1287,6 +1316,7 @@ int dg_begin_coarse_step(dg_ctx* c, const uint8_t* level, const uint8_t* rates, 
 
 int dg_stage_where(dg_ctx* c, const dg_state* base, const dg_state* input, const dg_stage_plan* plan, dg_state* out,
                    dg_state* acc) {
+    DG_LOCK(c);
     DG_REQUIRE(c && base && input && plan && out && c->K > 0, "dg_stage_where: null argument or no form");
     DG_REQUIRE(c->lts_order, "dg_stage_where: no coarse step begun (dg_begin_coarse_step)");
     DG_REQUIRE(plan->stage >= 0 && plan->stage < c->lts_stages, "dg_stage_where: stage out of range");
@@ -1338,8 +1368,10 @@ int dg_stage_where(dg_ctx* c, const dg_state* base, const dg_state* input, const
         p.lvl_eta[l] = tide_elevation(c->tide, c->tide_nodal, time);
     }
     // the forcing given per node: its tables, and per level the moments at the level's stage time
-    const bool force = c->tide_slot || c->nest_slot || c->wx_series;
-    if (force) {
+    const bool force = c->tide_slot || c->nest_slot || c->wx_series || c->manning_node || c->coriolis_node;
+    p.manning_node = c->manning_node;
+    p.coriolis_node = c->coriolis_node;
+    if (c->tide_slot || c->nest_slot || c->wx_series) {
         constexpr int ROW = LVL_MOMENT, ROWS = DG_MAX_LEVELS + 1;
         if (!c->lvl_moment_dev) {
             DG_TRY(hipMalloc(&c->lvl_moment_dev, sizeof(double) * dg_ctx::MOMENT_RING * ROWS * ROW));
@@ -1410,6 +1442,7 @@ int dg_stage_where(dg_ctx* c, const dg_state* base, const dg_state* input, const
 }
 
 int dg_finish_where(dg_ctx* c, dg_state* state, const dg_state* acc, int depth) {
+    DG_LOCK(c);
     DG_REQUIRE(c && state && acc && c->K > 0, "dg_finish_where: null argument or no form");
     DG_REQUIRE(c->lts_order, "dg_finish_where: no coarse step begun (dg_begin_coarse_step)");
     DG_REQUIRE(depth >= 0 && depth <= DG_MAX_LEVELS, "dg_finish_where: depth out of range");
@@ -1440,6 +1473,7 @@ int dg_finish_where(dg_ctx* c, dg_state* state, const dg_state* acc, int depth) 
 }
 
 int dg_depth_flags(dg_ctx* c, const dg_state* s, double d0, double d1, double d2, uint8_t* out) {
+    DG_LOCK(c);
     DG_REQUIRE(c && s && out && c->K > 0, "dg_depth_flags: null argument or no form");
     DG_TRY(hipSetDevice(c->device));
     if (!c->depth_flags) DG_TRY(hipMalloc(&c->depth_flags, (size_t)c->K));
@@ -1481,6 +1515,7 @@ static int mr_free_states(dg_multirate* m) {
 }
 
 int dg_multirate_create(dg_ctx* c, int max_levels, int scheme, int line_rates, dg_multirate** out) {
+    DG_LOCK(c);
     DG_REQUIRE(c && out && c->K > 0, "dg_multirate_create: null argument or no form");
     DG_REQUIRE(max_levels >= 0 && max_levels <= DG_MAX_LEVELS, "dg_multirate_create: 0 to DG_MAX_LEVELS levels");
     DG_REQUIRE(scheme == DG_SSP_RK3 || scheme == DG_SSP_RK43, "dg_multirate_create: unknown scheme");
@@ -1509,6 +1544,7 @@ int dg_multirate_create(dg_ctx* c, int max_levels, int scheme, int line_rates, d
 
 int dg_multirate_destroy(dg_multirate* m) {
     if (!m) return 0;
+    DG_LOCK(m->c);
     DG_TRY(hipSetDevice(m->c->device));
     if (int r = mr_free_states(m)) return r;
     DG_TRY(hipHostFree(m->edt));
@@ -1519,6 +1555,8 @@ int dg_multirate_destroy(dg_multirate* m) {
 
 int dg_multirate_assign_levels(dg_multirate* m, const dg_state* s, double cfl, double dt_cap, double landing,
                                double* dt_out, double* stable_out) {
+    DG_REQUIRE(m != nullptr, "dg_multirate_assign_levels: null stepper");
+    DG_LOCK(m->c);
     namespace mr = dgswe::multirate;
     DG_REQUIRE(m && s && dt_out, "dg_multirate_assign_levels: null argument");
     dg_ctx* c = m->c;
@@ -1537,6 +1575,8 @@ int dg_multirate_assign_levels(dg_multirate* m, const dg_state* s, double cfl, d
 }
 
 int dg_multirate_step(dg_multirate* m, dg_state* s, double t, double dt) {
+    DG_REQUIRE(m != nullptr, "dg_multirate_step: null stepper");
+    DG_LOCK(m->c);
     DG_REQUIRE(m && s, "dg_multirate_step: null argument");
     DG_REQUIRE(!m->as.level.empty(), "dg_multirate_step: no levels assigned (dg_multirate_assign_levels)");
     dg_ctx* c = m->c;
@@ -1578,6 +1618,8 @@ int dg_multirate_step(dg_multirate* m, dg_state* s, double t, double dt) {
 }
 
 int dg_multirate_levels(dg_multirate* m, uint8_t* level, uint8_t* rates, int* finest) {
+    DG_REQUIRE(m != nullptr, "dg_multirate_levels: null stepper");
+    DG_LOCK(m->c);
     DG_REQUIRE(m, "dg_multirate_levels: null argument");
     if (level && !m->as.level.empty()) std::memcpy(level, m->as.level.data(), m->as.level.size());
     if (rates && !m->as.rates.empty()) std::memcpy(rates, m->as.rates.data(), m->as.rates.size());
@@ -1586,6 +1628,8 @@ int dg_multirate_levels(dg_multirate* m, uint8_t* level, uint8_t* rates, int* fi
 }
 
 int dg_multirate_stats_read(dg_multirate* m, dg_multirate_stats* out) {
+    DG_REQUIRE(m != nullptr, "dg_multirate_stats_read: null stepper");
+    DG_LOCK(m->c);
     DG_REQUIRE(m && out, "dg_multirate_stats_read: null argument");
     *out = m->stats;
     return 0;
@@ -1593,6 +1637,8 @@ int dg_multirate_stats_read(dg_multirate* m, dg_multirate_stats* out) {
 
 int dg_multirate_run(dg_multirate* m, dg_state* s, double t, double t_end, double cfl, double dt_max, double interval,
                      double* t_out, int64_t* coarse_steps) {
+    DG_REQUIRE(m != nullptr, "dg_multirate_run: null stepper");
+    DG_LOCK(m->c);
     namespace mr = dgswe::multirate;
     DG_REQUIRE(m && s, "dg_multirate_run: null argument");
     int64_t n = 0, k = 0;
@@ -1619,6 +1665,7 @@ int dg_multirate_run(dg_multirate* m, dg_state* s, double t, double t_end, doubl
 int dg_set_boundary_tides(dg_ctx* c, int n_c, const double* omega, const double* f_rate, double ramp, int velocity,
                           double max_ratio, int64_t n_slots, const uint32_t* slot_of_face_node, const double* coef,
                           const double* mean, const double* source_depth) {
+    DG_LOCK(c);
     DG_REQUIRE(c && c->K > 0, "dg_set_boundary_tides: null context or no form");
     DG_REQUIRE(n_c >= 0 && n_c <= 8, "dg_set_boundary_tides: 0 to 8 constituents");
     DG_TRY(hipSetDevice(c->device));
@@ -1664,6 +1711,7 @@ int dg_set_nesting(dg_ctx* c, int64_t n_slots, const uint32_t* slot_of_node, int
                    double epoch, int cubic, const float* series, const double* weight, const double* parent_depth, double timescale,
                    double wet_depth, int momentum_only, int velocity, double max_ratio, double reference_level,
                    double ramp) {
+    DG_LOCK(c);
     DG_REQUIRE(c && c->K > 0, "dg_set_nesting: null context or no form");
     DG_TRY(hipSetDevice(c->device));
     DG_TRY(hipStreamSynchronize(c->stream));
@@ -1705,6 +1753,7 @@ int dg_set_nesting(dg_ctx* c, int64_t n_slots, const uint32_t* slot_of_node, int
 int dg_set_weather(dg_ctx* c, int n_times, const double* times, double epoch, int cubic, const float* series,
                    int wind, int pressure, int drag, double cd, double rho_air, double rho_water, double h_min,
                    double ramp) {
+    DG_LOCK(c);
     DG_REQUIRE(c && c->K > 0, "dg_set_weather: null context or no form");
     DG_TRY(hipSetDevice(c->device));
     DG_TRY(hipStreamSynchronize(c->stream));
@@ -1740,6 +1789,7 @@ int dg_set_weather(dg_ctx* c, int n_times, const double* times, double epoch, in
 }
 
 int dg_set_inverse_barometer(dg_ctx* c, double factor) {
+    DG_LOCK(c);
     DG_REQUIRE(c, "dg_set_inverse_barometer: null context");
     DG_REQUIRE(std::isfinite(factor), "dg_set_inverse_barometer: the factor must be finite");
     c->wx_raise = factor;
@@ -1747,6 +1797,7 @@ int dg_set_inverse_barometer(dg_ctx* c, double factor) {
 }
 
 int dg_set_nesting_boundary(dg_ctx* c, int on) {
+    DG_LOCK(c);
     DG_REQUIRE(c, "dg_set_nesting_boundary: null context");
     c->nest_boundary = on ? 1 : 0;
     return 0;
@@ -1758,6 +1809,7 @@ uint64_t dg_fnv1a(const uint8_t* data, size_t n, uint64_t h) {
 }
 
 int dg_element_dt(dg_ctx* c, const dg_state* s, double cfl, int scheme, double* out) {
+    DG_LOCK(c);
     DG_REQUIRE(c && s && out && c->K > 0, "dg_element_dt: null argument or no form");
     DG_TRY(hipSetDevice(c->device));
     double dt;
@@ -1767,12 +1819,14 @@ int dg_element_dt(dg_ctx* c, const dg_state* s, double cfl, int scheme, double* 
 }
 
 int dg_dt_ssp(dg_ctx* c, const dg_state* s, double cfl, int scheme, double* dt) {
+    DG_LOCK(c);
     DG_REQUIRE(c && s && dt && c->K > 0, "dg_dt_ssp: null argument or no form");
     DG_TRY(hipSetDevice(c->device));
     return launch_dt(c, s->d, cfl, scheme, false, dt);
 }
 
 int dg_step(dg_ctx* c, dg_state* s, double t, double dt, int scheme) {
+    DG_LOCK(c);
     DG_REQUIRE(c && s && c->K > 0, "dg_step: null argument or no form");
     DG_REQUIRE(scheme == DG_SSP_RK3 || scheme == DG_SSP_RK43, "dg_step: unknown scheme");
     DG_TRY(hipSetDevice(c->device));
@@ -1782,6 +1836,7 @@ int dg_step(dg_ctx* c, dg_state* s, double t, double dt, int scheme) {
 }
 
 int dg_run(dg_ctx* c, dg_state* s, double t, int n_steps, double cfl, int scheme, double* t_end, double* dts) {
+    DG_LOCK(c);
     DG_REQUIRE(c && s && c->K > 0, "dg_run: null argument or no form");
     DG_REQUIRE(scheme == DG_SSP_RK3 || scheme == DG_SSP_RK43, "dg_run: unknown scheme");
     DG_TRY(hipSetDevice(c->device));
@@ -1802,6 +1857,7 @@ int dg_run(dg_ctx* c, dg_state* s, double t, int n_steps, double cfl, int scheme
 }
 
 int dg_is_finite(dg_ctx* c, const dg_state* s, int* finite) {
+    DG_LOCK(c);
     DG_REQUIRE(c && s && finite, "dg_is_finite: null argument");
     DG_TRY(hipMemsetAsync(c->flag, 0, sizeof(int), c->stream));
     size_t n = (size_t)3 * c->K * c->nn;
@@ -1815,6 +1871,7 @@ int dg_is_finite(dg_ctx* c, const dg_state* s, int* finite) {
 }
 
 int dg_clip_count(dg_ctx* c, uint64_t* clips) {
+    DG_LOCK(c);
     DG_REQUIRE(c && clips, "dg_clip_count: null argument");
     unsigned long long v = 0;
     DG_TRY(hipMemcpyAsync(&v, c->clips, 8, hipMemcpyDeviceToHost, c->stream));
@@ -1824,6 +1881,7 @@ int dg_clip_count(dg_ctx* c, uint64_t* clips) {
 }
 
 int dg_masks(dg_ctx* c, const dg_state* s, uint8_t* node_dry, uint8_t* elem_subcell) {
+    DG_LOCK(c);
     DG_REQUIRE(c && s && node_dry && elem_subcell, "dg_masks: null argument");
     uint8_t *dn = nullptr, *de = nullptr;
     DG_TRY(hipMalloc(&dn, (size_t)c->K * c->nn));
@@ -1845,6 +1903,7 @@ int dg_masks(dg_ctx* c, const dg_state* s, uint8_t* node_dry, uint8_t* elem_subc
 }
 
 int dg_masks_noalloc(dg_ctx* c, const dg_state* s, uint8_t* node_dry, uint8_t* elem_subcell) {
+    DG_LOCK(c);
     DG_REQUIRE(c && s && node_dry && elem_subcell, "dg_masks_noalloc: null argument");
     const size_t nodes = (size_t)c->K * c->nn, need = nodes + (size_t)c->K;
     if (need > SCRATCH_BYTES) {
@@ -1870,6 +1929,7 @@ int dg_masks_noalloc(dg_ctx* c, const dg_state* s, uint8_t* node_dry, uint8_t* e
 }
 
 int dg_timing_enable(dg_ctx* c, int on) {
+    DG_LOCK(c);
     DG_REQUIRE(c, "dg_timing_enable: null context");
     c->timing = on != 0;
     c->stage_ms = c->dt_ms = 0.0;
@@ -1878,6 +1938,7 @@ int dg_timing_enable(dg_ctx* c, int on) {
 }
 
 int dg_timing_read(dg_ctx* c, double* stage_ms, int64_t* stage_n, double* dt_ms, int64_t* dt_n) {
+    DG_LOCK(c);
     DG_REQUIRE(c, "dg_timing_read: null context");
     if (stage_ms) *stage_ms = c->stage_ms;
     if (stage_n) *stage_n = c->stage_n;
@@ -1887,12 +1948,14 @@ int dg_timing_read(dg_ctx* c, double* stage_ms, int64_t* stage_n, double* dt_ms,
 }
 
 int dg_sync(dg_ctx* c) {
+    DG_LOCK(c);
     DG_REQUIRE(c, "dg_sync: null context");
     DG_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }
 
 int dg_copy_bench(dg_ctx* c, size_t bytes, int reps, double* ms_per_copy) {
+    DG_LOCK(c);
     DG_REQUIRE(c && ms_per_copy && reps > 0 && bytes >= 32, "dg_copy_bench: bad argument");
     DG_TRY(hipSetDevice(c->device));
     size_t n4 = bytes / 32;
@@ -1916,12 +1979,14 @@ int dg_copy_bench(dg_ctx* c, size_t bytes, int reps, double* ms_per_copy) {
 }
 
 int dg_state_zero(dg_ctx* c, dg_state* s) {
+    DG_LOCK(c);
     DG_REQUIRE(c && s && c->K > 0, "dg_state_zero: null argument or no form");
     DG_TRY(hipSetDevice(c->device));
     return zero_async(c, s->d, (size_t)3 * c->K * c->nn);
 }
 
 int dg_state_wrap(dg_ctx* c, double* device_ptr, dg_state** s) {
+    DG_LOCK(c);
     DG_REQUIRE(c && device_ptr && s && c->K > 0, "dg_state_wrap: null argument or no form");
     dg_state* st = new dg_state;
     st->d = device_ptr;
@@ -1931,6 +1996,7 @@ int dg_state_wrap(dg_ctx* c, double* device_ptr, dg_state** s) {
 }
 
 int dg_set_stream(dg_ctx* c, void* stream) {
+    DG_LOCK(c);
     DG_REQUIRE(c, "dg_set_stream: null context");
     DG_TRY(hipStreamSynchronize(c->stream));
     c->stream = stream ? (hipStream_t)stream : c->own_stream;
@@ -1938,6 +2004,7 @@ int dg_set_stream(dg_ctx* c, void* stream) {
 }
 
 int dg_set_bed(dg_ctx* c, const double* bed, int on_device) {
+    DG_LOCK(c);
     DG_REQUIRE(c && bed && c->K > 0, "dg_set_bed: null argument or no form");
     DG_TRY(hipSetDevice(c->device));
     DG_TRY(hipMemcpyAsync(c->bed, bed, (size_t)c->K * c->nn * 8, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
@@ -1947,6 +2014,7 @@ int dg_set_bed(dg_ctx* c, const double* bed, int on_device) {
 }
 
 int dg_set_manning(dg_ctx* c, const double* manning, int on_device) {
+    DG_LOCK(c);
     DG_REQUIRE(c && manning && c->K > 0, "dg_set_manning: null argument or no form");
     DG_TRY(hipSetDevice(c->device));
     DG_TRY(hipMemcpyAsync(c->manning, manning, (size_t)c->K * 8,
@@ -1955,7 +2023,56 @@ int dg_set_manning(dg_ctx* c, const double* manning, int on_device) {
     return 0;
 }
 
+int dg_set_manning_nodal(dg_ctx* c, const double* manning, int on_device) {
+    DG_LOCK(c);
+    DG_REQUIRE(c && c->K > 0, "dg_set_manning_nodal: null context or no form");
+    DG_TRY(hipSetDevice(c->device));
+    DG_TRY(hipStreamSynchronize(c->stream));
+    if (!manning) {
+        if (c->manning_node) DG_TRY(hipFree(c->manning_node));
+        c->manning_node = nullptr;
+        return 0;
+    }
+    const size_t nt = (size_t)c->K * c->nn;
+    if (!on_device)
+        for (size_t i = 0; i < nt; ++i)
+            DG_REQUIRE(std::isfinite(manning[i]) && manning[i] >= 0.0,
+                       "dg_set_manning_nodal: a Manning coefficient is negative or not finite (friction.rs:520-526)");
+    if (!c->manning_node) DG_TRY(hipMalloc(&c->manning_node, nt * sizeof(double)));
+    DG_TRY(hipMemcpyAsync(c->manning_node, manning, nt * sizeof(double),
+                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+    DG_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dg_set_coriolis_beta(dg_ctx* c, double beta, double y0) {
+    DG_LOCK(c);
+    DG_REQUIRE(c && c->K > 0, "dg_set_coriolis_beta: null context or no form");
+    DG_REQUIRE(std::isfinite(beta) && std::isfinite(y0), "dg_set_coriolis_beta: beta and y0 must be finite");
+    DG_TRY(hipSetDevice(c->device));
+    DG_TRY(hipStreamSynchronize(c->stream));
+    c->beta = beta;
+    c->beta_y0 = y0;
+    if (beta == 0.0) {   // the f-plane (CoriolisSource2D::add_element takes f0 when beta is 0)
+        if (c->coriolis_node) DG_TRY(hipFree(c->coriolis_node));
+        c->coriolis_node = nullptr;
+        return 0;
+    }
+    const int64_t nt = c->K * c->nn;
+    if (!c->coriolis_node) DG_TRY(hipMalloc(&c->coriolis_node, (size_t)nt * sizeof(double)));
+    dim3 grid((unsigned)((nt + 255) / 256));
+    switch (c->order) {
+        case 1: hipLaunchKernelGGL(coriolis_node_kernel<1>, grid, dim3(256), 0, c->stream, c->corners, c->K, c->f0, beta, y0, c->coriolis_node); break;
+        case 2: hipLaunchKernelGGL(coriolis_node_kernel<2>, grid, dim3(256), 0, c->stream, c->corners, c->K, c->f0, beta, y0, c->coriolis_node); break;
+        default: hipLaunchKernelGGL(coriolis_node_kernel<3>, grid, dim3(256), 0, c->stream, c->corners, c->K, c->f0, beta, y0, c->coriolis_node); break;
+    }
+    DG_TRY(hipGetLastError());
+    DG_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 int dg_param_grads_zero(dg_ctx* c) {
+    DG_LOCK(c);
     DG_REQUIRE(c && c->K > 0, "dg_param_grads_zero: no form");
     DG_TRY(hipSetDevice(c->device));
     if (int r = ensure_adjoint(c)) return r;
@@ -1964,6 +2081,7 @@ int dg_param_grads_zero(dg_ctx* c) {
 }
 
 int dg_param_grads_read(dg_ctx* c, double* gbed, double* gman, int on_device) {
+    DG_LOCK(c);
     DG_REQUIRE(c && c->K > 0, "dg_param_grads_read: no form");
     DG_TRY(hipSetDevice(c->device));
     if (int r = ensure_adjoint(c)) return r;
@@ -1975,6 +2093,7 @@ int dg_param_grads_read(dg_ctx* c, double* gbed, double* gman, int on_device) {
 }
 
 int dg_rhs_vjp(dg_ctx* c, const dg_state* in, double t, const dg_state* cot_out, dg_state* cot_in) {
+    DG_LOCK(c);
     DG_REQUIRE(c && in && cot_out && cot_in && c->K > 0, "dg_rhs_vjp: null argument or no form");
     DG_REQUIRE(cot_in != cot_out, "dg_rhs_vjp: cot_in must differ from cot_out");
     DG_TRY(hipSetDevice(c->device));
@@ -1985,6 +2104,7 @@ int dg_rhs_vjp(dg_ctx* c, const dg_state* in, double t, const dg_state* cot_out,
 
 int dg_stage_vjp(dg_ctx* c, const dg_state* base, const dg_state* input, const dg_stage_desc* st,
                  const dg_state* cot_out, dg_state* cot_base, dg_state* cot_input) {
+    DG_LOCK(c);
     DG_REQUIRE(c && base && input && st && cot_out && cot_input && c->K > 0, "dg_stage_vjp: null argument or no form");
     DG_REQUIRE(cot_input != cot_out && cot_base != cot_out, "dg_stage_vjp: the cotangent outputs must differ from cot_out");
     DG_TRY(hipSetDevice(c->device));
@@ -2002,6 +2122,7 @@ int dg_stage_vjp(dg_ctx* c, const dg_state* base, const dg_state* input, const d
 
 int dg_step_vjp(dg_ctx* c, const dg_state* s, double t, double dt, int scheme, const dg_state* cot_next,
                 dg_state* cot_state) {
+    DG_LOCK(c);
     DG_REQUIRE(c && s && cot_next && cot_state && c->K > 0, "dg_step_vjp: null argument or no form");
     DG_REQUIRE(scheme == DG_SSP_RK3 || scheme == DG_SSP_RK43, "dg_step_vjp: unknown scheme");
     DG_REQUIRE(cot_next != cot_state && cot_next->d != cot_state->d, "dg_step_vjp: cot_next must differ from cot_state");
@@ -2012,6 +2133,7 @@ int dg_step_vjp(dg_ctx* c, const dg_state* s, double t, double dt, int scheme, c
 }
 
 int dg_set_stations(dg_ctx* c, int n, const int64_t* element, const double* weights) {
+    DG_LOCK(c);
     DG_REQUIRE(c && c->K > 0 && n >= 0 && (n == 0 || (element && weights)), "dg_set_stations: bad argument or no form");
     for (int s = 0; s < n; ++s)
         DG_REQUIRE(element[s] >= 0 && element[s] < c->K, "dg_set_stations: station element out of range");
@@ -2033,6 +2155,7 @@ int dg_set_stations(dg_ctx* c, int n, const int64_t* element, const double* weig
 }
 
 int dg_sample_stations(dg_ctx* c, const dg_state* s, double* eta) {
+    DG_LOCK(c);
     DG_REQUIRE(c && s && eta && c->n_st > 0, "dg_sample_stations: null argument or no stations");
     DG_TRY(hipSetDevice(c->device));
     hipLaunchKernelGGL(station_sample_kernel, dim3((c->n_st + 63) / 64), dim3(64), 0, c->stream, s->d, c->bed,
@@ -2044,6 +2167,7 @@ int dg_sample_stations(dg_ctx* c, const dg_state* s, double* eta) {
 }
 
 int dg_tape_create(dg_ctx* c, int snapshots, dg_tape** out) {
+    DG_LOCK(c);
     DG_REQUIRE(c && out && c->K > 0, "dg_tape_create: null argument or no form");
     DG_REQUIRE(snapshots >= 1, "dg_tape_create: at least one snapshot (the initial state)");
     DG_TRY(hipSetDevice(c->device));
@@ -2070,6 +2194,7 @@ int dg_tape_create(dg_ctx* c, int snapshots, dg_tape** out) {
 
 int dg_tape_destroy(dg_ctx* c, dg_tape* tp) {
     if (!tp) return 0;
+    DG_LOCK(c);
     DG_REQUIRE(c, "dg_tape_destroy: null context");
     DG_TRY(hipSetDevice(c->device));
     DG_TRY(hipStreamSynchronize(c->stream));
@@ -2093,6 +2218,7 @@ int dg_tape_destroy(dg_ctx* c, dg_tape* tp) {
 
 int dg_misfit_forward(dg_ctx* c, dg_tape* tp, const dg_state* x0, double t0, int n_steps, const double* dts, int scheme,
                       int every, const double* obs, double* misfit) {
+    DG_LOCK(c);
     DG_REQUIRE(c && tp && x0 && dts && obs && misfit && c->K > 0, "dg_misfit_forward: null argument or no form");
     DG_REQUIRE(tp->K == c->K && tp->nn == c->nn, "dg_misfit_forward: the tape belongs to another form");
     DG_REQUIRE(n_steps >= 1 && every >= 1 && every <= n_steps, "dg_misfit_forward: need 1 <= every <= n_steps");
@@ -2169,6 +2295,7 @@ int dg_misfit_forward(dg_ctx* c, dg_tape* tp, const dg_state* x0, double t0, int
 }
 
 int dg_misfit_backward(dg_ctx* c, dg_tape* tp, double scale, dg_state* cot_x0) {
+    DG_LOCK(c);
     DG_REQUIRE(c && tp && c->K > 0, "dg_misfit_backward: null argument or no form");
     DG_REQUIRE(tp->ready, "dg_misfit_backward: run dg_misfit_forward first (once per backward)");
     DG_REQUIRE(c->n_st == tp->n_st, "dg_misfit_backward: the stations changed since the forward");
@@ -2212,6 +2339,8 @@ int dg_batch_create(int n, dg_ctx* const* members, int scheme, dg_batch** out) {
         DG_REQUIRE(!members[m]->tide_slot, "dg_batch_create: a member with boundary tides per node (not batched)");
         DG_REQUIRE(!members[m]->nest_slot, "dg_batch_create: a member with a nesting band (not batched)");
         DG_REQUIRE(!members[m]->wx_series, "dg_batch_create: a member with gridded weather (not batched)");
+        DG_REQUIRE(!members[m]->manning_node && !members[m]->coriolis_node,
+                   "dg_batch_create: a member with Manning n per node or a beta-plane (not batched)");
     }
     DG_TRY(hipSetDevice(members[0]->device));
     dg_batch* b = new dg_batch;
@@ -2280,6 +2409,7 @@ int dg_batch_create(int n, dg_ctx* const* members, int scheme, dg_batch** out) {
 
 int dg_batch_destroy(dg_batch* b) {
     if (!b) return 0;
+    DG_LOCK(b->owner);
     DG_TRY(hipSetDevice(b->owner->device));
     DG_TRY(hipStreamSynchronize(b->owner->stream));
     if (int r = batch_free(b)) return r;
@@ -2288,6 +2418,8 @@ int dg_batch_destroy(dg_batch* b) {
 }
 
 int dg_batch_upload(dg_batch* b, int m, const double* h, const double* hu, const double* hv) {
+    DG_REQUIRE(b != nullptr, "dg_batch_upload: null ensemble");
+    DG_LOCK(b->owner);
     DG_REQUIRE(b && m >= 0 && m < (int)b->members.size() && h && hu && hv, "dg_batch_upload: bad argument");
     dg_ctx* c = b->members[m];
     const size_t nt = (size_t)c->K * c->nn;
@@ -2301,6 +2433,8 @@ int dg_batch_upload(dg_batch* b, int m, const double* h, const double* hu, const
 }
 
 int dg_batch_download(dg_batch* b, int m, double* h, double* hu, double* hv) {
+    DG_REQUIRE(b != nullptr, "dg_batch_download: null ensemble");
+    DG_LOCK(b->owner);
     DG_REQUIRE(b && m >= 0 && m < (int)b->members.size() && h && hu && hv, "dg_batch_download: bad argument");
     dg_ctx* c = b->members[m];
     const size_t nt = (size_t)c->K * c->nn;
@@ -2314,6 +2448,8 @@ int dg_batch_download(dg_batch* b, int m, double* h, double* hu, double* hv) {
 }
 
 int dg_batch_run(dg_batch* b, int n_steps, double cfl, int options, double* t) {
+    DG_REQUIRE(b != nullptr, "dg_batch_run: null ensemble");
+    DG_LOCK(b->owner);
     DG_REQUIRE(b && t && n_steps >= 0, "dg_batch_run: bad argument");
     DG_TRY(hipSetDevice(b->owner->device));
     const int M = (int)b->members.size();
@@ -2380,6 +2516,8 @@ int dg_batch_run(dg_batch* b, int n_steps, double cfl, int options, double* t) {
 }
 
 int dg_batch_tuning(const dg_batch* b, int order, int* xcd_mapping, int* subcell_groups) {
+    DG_REQUIRE(b != nullptr, "dg_batch_tuning: null ensemble");
+    DG_LOCK(b->owner);
     DG_REQUIRE(b && order >= 1 && order <= 3, "dg_batch_tuning: bad argument");
     if (xcd_mapping) *xcd_mapping = (int)b->ord[order].xcd;
     if (subcell_groups) *subcell_groups = (int)b->ord[order].sub_groups;

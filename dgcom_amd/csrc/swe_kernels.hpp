@@ -178,6 +178,9 @@ struct StageParams {
     double wx_raise;   // the open boundary's level raised by this times the inverse barometer (0: not)
     // local time stepping with forcing: per level a row of LVL_MOMENT values, the moments at the level's stage time
     const double* lvl_moment;
+    // Manning n per node [K*NN] (SpatiallyVaryingManning2D, friction.rs:463-553) and a beta-plane's f per node
+    // [K*NN] (CoriolisSource2D::beta_plane, coriolis.rs:101-121), or NULL: per element n, the f-plane f0
+    const double *manning_node, *coriolis_node;
 };
 
 // A level's row of the forcing moments: the atlas moment (25 values), then the nesting band's and the weather's
@@ -850,7 +853,12 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
         if (j == N1 - 1) rhs = cadd(rhs, cscale(ssj[e * 4 + 2] * T.lift[2][N1 - 1 - a], sfd[e * NF + 2 * N1 + (N1 - 1 - a)]));
         if (a == 0) rhs = cadd(rhs, cscale(ssj[e * 4 + 3] * T.lift[3][N1 - 1 - j], sfd[e * NF + 3 * N1 + (N1 - 1 - j)]));
         rhs = {det_inv * rhs.h, det_inv * rhs.hu, det_inv * rhs.hv};
-        if (p.f0 != 0.0) {
+        if (FORCE && p.coriolis_node) {
+            // a beta-plane: f0 + beta (y - y0) at the node (CoriolisSource2D::add_element, coriolis.rs:157-169)
+            const double f = p.coriolis_node[n];
+            rhs.hu += f * hvin;
+            rhs.hv += (-f) * huin;
+        } else if (p.f0 != 0.0) {
             rhs.hu += p.f0 * hvin;
             rhs.hv += (-p.f0) * huin;
         }
@@ -940,6 +948,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
                     double speed = sqrt(u * u + v * v);
                     if (speed > 0.0) {
                         double mn = NT == 1 ? mann : ld32(p.manning, k32 * 8u);
+                        if (FORCE && p.manning_node) mn = ld32(p.manning_node, nb);  // the node's own n
                         double gn2 = p.g * mn * mn;
                         double hh = fmax_(vh, p.h_min);
                         rate += gn2 * speed / (manning_cbrt(hh) * hh);
@@ -1016,6 +1025,24 @@ __global__ __launch_bounds__(64) void geometry_kernel(const double* corners, con
                 l == LINK_BOUNDARY ? 0.0 : outer_distance<N>(corners, affine_flags, c, affine, l, f, fi);
         }
     }
+}
+
+// f = f0 + beta (y - y0) at every node, y by the element's bilinear map (Mesh2D::bilinear, mesh2d.rs:578-595)
+template <int N>
+__global__ __launch_bounds__(256) void coriolis_node_kernel(const double* corners, int64_t K, double f0, double beta,
+                                                            double y0, double* f) {
+    constexpr int N1 = N + 1, NN = N1 * N1;
+    const GllTables& T = c_gll[N];
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= K * NN) return;
+    const int64_t k = n / NN;
+    const int i = (int)(n - k * NN);
+    const double r = T.x1[i % N1], s = T.x1[i / N1];
+    const double* v = corners + 8 * k;
+    const double n0 = (1.0 - r) * (1.0 - s) / 4.0, n1 = (1.0 + r) * (1.0 - s) / 4.0;
+    const double n2 = (1.0 + r) * (1.0 + s) / 4.0, n3 = (1.0 - r) * (1.0 + s) / 4.0;
+    const double y = n0 * v[1] + n1 * v[3] + n2 * v[5] + n3 * v[7];
+    f[n] = f0 + beta * (y - y0);
 }
 
 // Elements with every node at or above h_dry, in mesh order (XCD-aware block

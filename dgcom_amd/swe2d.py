@@ -41,6 +41,7 @@ EXPORTS = (
     "dg_multirate_create", "dg_multirate_destroy", "dg_multirate_assign_levels", "dg_multirate_step",
     "dg_multirate_levels", "dg_multirate_stats_read", "dg_multirate_run", "dg_fnv1a", "dg_set_boundary_tides",
     "dg_set_nesting", "dg_set_weather", "dg_set_inverse_barometer", "dg_set_nesting_boundary", "dg_masks_noalloc",
+    "dg_set_manning_nodal", "dg_set_coriolis_beta",
 )
 BATCH_GRAPH, BATCH_TUNE = 1, 2
 
@@ -124,6 +125,8 @@ def load(fma: bool = False):
     L.dg_set_stream.argtypes = [vp, vp]
     L.dg_set_bed.argtypes = [vp, vp, ctypes.c_int]
     L.dg_set_manning.argtypes = [vp, vp, ctypes.c_int]
+    L.dg_set_manning_nodal.argtypes = [vp, vp, ctypes.c_int]
+    L.dg_set_coriolis_beta.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     L.dg_param_grads_zero.argtypes = [vp]
     L.dg_param_grads_read.argtypes = [vp, vp, vp, ctypes.c_int]
     L.dg_rhs_vjp.argtypes = [vp, vp, ctypes.c_double, vp, vp]
@@ -466,6 +469,20 @@ class SWEPhysics2D:
         if b.size != self.n_nodes_total:
             raise ValueError("bed must have K*(N+1)^2 values")
         _check(self.lib, self.lib.dg_set_bed(self.ctx, b.ctypes.data, 0), "dg_set_bed")
+
+    def set_manning_nodal(self, manning):
+        """Manning n per node [K*NN] (SpatiallyVaryingManning2D); None: per element again."""
+        if manning is None:
+            _check(self.lib, self.lib.dg_set_manning_nodal(self.ctx, None, 0), "dg_set_manning_nodal")
+            return
+        a = np.ascontiguousarray(manning, dtype=np.float64)
+        assert a.size == self.n_nodes_total
+        _check(self.lib, self.lib.dg_set_manning_nodal(self.ctx, a.ctypes.data_as(ctypes.c_void_p), 0),
+               "dg_set_manning_nodal")
+
+    def set_coriolis_beta(self, beta, y0=0.0):
+        """A beta-plane f = f0 + beta (y - y0) (CoriolisSource2D::beta_plane); beta = 0: the f-plane."""
+        _check(self.lib, self.lib.dg_set_coriolis_beta(self.ctx, beta, y0), "dg_set_coriolis_beta")
 
     def set_manning(self, manning=None, device_ptr=None):
         if device_ptr is not None:

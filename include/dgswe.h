@@ -325,6 +325,14 @@ int dg_set_stream(dg_ctx* ctx, void* hip_stream);
 /* Replace the bed [K*NN] or the per-element Manning n [K] (host or device memory). */
 int dg_set_bed(dg_ctx* ctx, const double* bed, int on_device);
 int dg_set_manning(dg_ctx* ctx, const double* manning, int on_device);
+/* Manning n per node [K*NN] (SpatiallyVaryingManning2D::from_nodal, src/source/swe_2d/friction.rs:463-553: the
+ * point-implicit damping with the node's g n^2) instead of the per-element n; NULL: back to per element. The
+ * ensembles and the VJP refuse it. */
+int dg_set_manning_nodal(dg_ctx* ctx, const double* manning, int on_device);
+/* A beta-plane f = f0 + beta (y - y0), y of each node by the element's bilinear map (CoriolisSource2D::beta_plane
+ * and add_element, src/source/swe_2d/coriolis.rs:101-121,157-169), f0 the form's coriolis_f0; beta = 0: the
+ * f-plane again. The ensembles and the VJP refuse it. */
+int dg_set_coriolis_beta(dg_ctx* ctx, double beta, double y0);
 /* The accumulated cotangents of the bed [K*NN] and of Manning n [K]. */
 int dg_param_grads_zero(dg_ctx* ctx);
 int dg_param_grads_read(dg_ctx* ctx, double* bed_grad, double* manning_grad, int on_device);

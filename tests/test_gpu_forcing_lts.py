@@ -1,9 +1,11 @@
 """The forcing given per node under local time stepping on the GPU against the
 restatement driven by the same stepper: tides per boundary node with a
-velocity, the nesting band, the gridded weather and the inverse-barometer
-raise, each element forced at its own level's stage time (the library's
-per-level moment table), whole multirate runs bit for bit through the library's
-stepper (dg_multirate_run) and the host's (dg_stage_where)."""
+velocity, the nesting band (also as the open boundary's external state), the
+gridded weather and the inverse-barometer raise, each element forced at its own
+level's stage time (the library's per-level moment table), and the sources
+given per node (Manning n per node, a beta-plane): whole multirate runs bit for
+bit through the library's stepper (dg_multirate_run) and the host's
+(dg_stage_where)."""
 import numpy as np
 import pytest
 
@@ -35,6 +37,13 @@ def _forcing(c, which):
             c, times, lambda x, y, t: (8.0 + 10.0 * np.sin(x / 300.0 + (t - T0) / 500.0), np.full(n, 4.0),
                                        0.002 * np.cos(y / 200.0), np.zeros(n), 600.0 * np.sin(y / 150.0 + (t - T0) / 700.0)),
             epoch=T0, ramp=30.0)
+    if "manning" in which:
+        rng = np.random.default_rng(17)
+        out["manning"] = np.repeat(c.manning, c.nn) * rng.uniform(0.5, 1.5, n)
+    if "beta" in which:
+        out["beta"] = (1e-8, 250.0)
+    if "boundary" in which:
+        out["boundary"] = True
     return out
 
 
@@ -43,12 +52,20 @@ def _apply(m, f, raise_factor):
         m.set_boundary_tides(f["tides"])
     if "nesting" in f:
         m.set_nesting(f["nesting"])
+        if "boundary" in f:
+            m.set_nesting_boundary(True)
     if "weather" in f:
         m.set_weather(f["weather"])
         m.set_inverse_barometer(raise_factor)
+    if "manning" in f:
+        m.set_manning_nodal(f["manning"])
+    if "beta" in f:
+        (m.set_beta if isinstance(m, O.Oracle) else m.set_coriolis_beta)(*f["beta"])
 
 
-CASES = [("tides", "tides"), ("nesting", "nesting"), ("weather", "weather"), ("all", "tides nesting weather")]
+CASES = [("tides", "tides"), ("nesting", "nesting"), ("weather", "weather"), ("all", "tides nesting weather"),
+         ("boundary", "nesting boundary"), ("manning_per_node", "manning"), ("beta_plane", "beta"),
+         ("sources_and_forcing", "manning beta tides weather")]
 
 
 @pytest.mark.parametrize("name,which", CASES, ids=[c[0] for c in CASES])
