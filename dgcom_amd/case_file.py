@@ -112,6 +112,7 @@ class CaseRun:
     dt_max: float | None
     lts_levels: int
     start: tuple | None
+    doc: dict = dataclasses.field(default_factory=dict)   # the case as read, overrides applied
 
 
 def load(path_or_text: str, overrides=()) -> CaseRun:
@@ -208,7 +209,7 @@ def load(path_or_text: str, overrides=()) -> CaseRun:
                                 float(s.get("max_offset_m", DEFAULT_MAX_OFFSET))))
     return CaseRun(run.get("name", "case"), case, 3600.0 * float(run.get("duration_hours", 1.0)),
                    60.0 * float(out.get("interval_minutes", 60.0)), stations, num.get("dt_max_s"),
-                   int(num.get("local_time_stepping_levels", 0)), start)
+                   int(num.get("local_time_stepping_levels", 0)), start, doc)
 
 
 # ---------------------------------------------------------------------------
@@ -398,6 +399,108 @@ def rust_display(x: float) -> str:
 
 
 # ---------------------------------------------------------------------------
+# The runner's restart data (src/case/run/restart.rs:1-20,191-275)
+# ---------------------------------------------------------------------------
+BUILD = "dgcom_amd (MI355X HIP path)"
+STATION_FIELDS = ("time_s", "eta_m", "u_m_s", "v_m_s", "depth_m", "hs_m", "tp_s", "wave_from_deg")
+
+
+def _toml_value(v) -> str:
+    if isinstance(v, bool):
+        return "true" if v else "false"
+    if isinstance(v, (int, float)):
+        return repr(v) if not (isinstance(v, float) and v == int(v) and abs(v) < 1e16) else f"{v:.1f}"
+    if isinstance(v, str):
+        return '"' + v.replace("\\", "\\\\").replace('"', '\\"') + '"'
+    if isinstance(v, (_dt.datetime, _dt.date)):
+        return v.isoformat()
+    if isinstance(v, list):
+        return "[" + ", ".join(_toml_value(x) for x in v) + "]"
+    raise CaseError(f"a case value of type {type(v).__name__} cannot be written")
+
+
+def to_toml(doc: dict) -> str:
+    """The case as TOML text: a table's plain keys, then its sub-tables and
+    arrays of tables (`[[boundary]]`), depth first."""
+    out = []
+
+    def walk(prefix, table):
+        plain = {k: v for k, v in table.items()
+                 if not isinstance(v, dict) and not (isinstance(v, list) and v and all(isinstance(x, dict) for x in v))}
+        for k, v in plain.items():
+            out.append(f"{k} = {_toml_value(v)}")
+        for k, v in table.items():
+            name = f"{prefix}.{k}" if prefix else k
+            if isinstance(v, dict):
+                out.append(f"\n[{name}]")
+                walk(name, v)
+            elif isinstance(v, list) and v and all(isinstance(x, dict) for x in v):
+                for item in v:
+                    out.append(f"\n[[{name}]]")
+                    walk(name, item)
+
+    walk("", doc)
+    return "\n".join(out).lstrip("\n") + "\n"
+
+
+def driver_data(run: CaseRun, records, volume_start: float, steps: int):
+    """A case run's restart data in the reference runner's layout
+    (restart.rs:191-213): metadata `build` and each station's name
+    `station.<j>`; the text `case` (the case with its build line); the arrays
+    `output.counters` (VTK frames written: none here; the volume at the start)
+    and `station.<j>.<field>` (its records so far; the sea-state fields empty
+    without waves). `steps` is this runner's own addition (metadata the
+    reference does not read)."""
+    metadata = [("build", BUILD)] + [(f"station.{j}", s.name) for j, s in enumerate(run.stations)]
+    metadata.append(("steps", str(steps)))
+    text = [("case", f"# {BUILD}\n{to_toml(run.doc)}")]
+    extra = [("output.counters", np.array([0.0, volume_start]))]
+    names = [s.name for s in run.stations]
+    for j, name in enumerate(names):
+        rows = np.array([r[1:] for r in records if r[0] == name], dtype=np.float64).reshape(-1, 5)
+        for c, field in enumerate(STATION_FIELDS[:5]):
+            extra.append((f"station.{j}.{field}", rows[:, c].copy()))
+        for field in STATION_FIELDS[5:]:
+            extra.append((f"station.{j}.{field}", np.zeros(0)))
+    return metadata, text, extra
+
+
+def restore_driver_data(run: CaseRun, rst):
+    """The records, the volume at the start and the steps so far from a
+    restart's driver data (restart.rs:216-275): refused by name without
+    `output.counters`, with other stations, a missing record array or more
+    stations than the run; the records interleaved by time as recorded."""
+    extra = dict(rst.extra)
+    meta = dict(rst.metadata)
+    counters = extra.get("output.counters")
+    if counters is None or counters.size != 2:
+        raise CaseError("the restart has no output.counters")
+    series = []
+    for j, s in enumerate(run.stations):
+        theirs = meta.get(f"station.{j}")
+        if theirs != s.name:
+            raise CaseError(f"the restart's stations: station {j} is {s.name}, the restart's {theirs or '(none)'}")
+        cols = []
+        for field in STATION_FIELDS:
+            v = extra.get(f"station.{j}.{field}")
+            if v is None:
+                raise CaseError(f"the restart's stations: no station.{j}.{field}")
+            cols.append(v)
+        if any(c.size != cols[0].size for c in cols[1:5]):
+            raise CaseError(f"the restart's stations: station {j}'s records differ in length")
+        series.append(cols)
+    if f"station.{len(run.stations)}" in meta:
+        raise CaseError("the restart's stations: it has more stations than this run")
+    records = []
+    n = max((c[0].size for c in series), default=0)
+    for i in range(n):
+        for s, cols in zip(run.stations, series):
+            if i < cols[0].size:
+                records.append((s.name, *(float(c[i]) for c in cols[:5])))
+    return records, float(counters[1]), int(meta.get("steps", "0"))
+
+
+# ---------------------------------------------------------------------------
 # The runner (case::run, src/case/run.rs; Simulation::run_with_callback)
 # ---------------------------------------------------------------------------
 @dataclasses.dataclass
@@ -407,6 +510,7 @@ class RunReport:
     records: list       # [(station, t, eta, u, v, depth)]
     clips: int
     state: tuple = ()   # (h, hu, hv) at the end
+    volume: tuple = ()  # the volume at the run's start (time 0) and at its end, m^3 (run.rs:626)
 
 
 def run_case(run: CaseRun, out_dir: str | None = None, device: int = 0, restart_from: str | None = None,
@@ -414,11 +518,12 @@ def run_case(run: CaseRun, out_dir: str | None = None, device: int = 0, restart_
     """Run a case on the GPU from still water (or from a restart) to its end
     (or `stop_at`), landing on every output time, recording the stations there
     (and at the start of a fresh run) into `out_dir/stations.csv`, and writing a
-    restart at the end if asked (its driver data carry the records so far, as
-    the reference's do)."""
+    restart at the end if asked (its driver data carry the records so far and
+    the counters, in the reference runner's layout)."""
     from . import swe2d
     c = run.case
     phys = physics or swe2d.SWEPhysics2D.from_case(c, device=device)
+    masses = cases.node_masses(c)
     t = 0.0
     h0, hu0, hv0 = c.h, c.hu, c.hv
     records = []
@@ -427,12 +532,9 @@ def run_case(run: CaseRun, out_dir: str | None = None, device: int = 0, restart_
         rst = restart.Restart2D.read(restart_from)
         rst.check_domain(c.mesh, c.bed, c.order)
         t, (h0, hu0, hv0) = rst.time, (rst.h, rst.hu, rst.hv)
-        prev = dict(rst.extra)
-        if "station_records" in prev:
-            rec = prev["station_records"].reshape(-1, 5)
-            names = [s.name for s in run.stations]
-            records = [(names[i % len(names)], *map(float, row)) for i, row in enumerate(rec)]
-        steps = int(dict(rst.metadata).get("steps", "0"))
+        records, volume_start, steps = restore_driver_data(run, rst)
+    else:
+        volume_start = float(np.sum(masses * h0))
     state = phys.state(h0, hu0, hv0)
     probes = station_probes(run)
     t_end = run.duration_s if stop_at is None else min(stop_at, run.duration_s)
@@ -483,13 +585,12 @@ def run_case(run: CaseRun, out_dir: str | None = None, device: int = 0, restart_
         os.makedirs(out_dir, exist_ok=True)
         write_stations(os.path.join(out_dir, "stations.csv"), records)
     if restart_to is not None:
-        rec = np.array([r[1:] for r in records], dtype=np.float64).ravel()
+        metadata, text, extra = driver_data(run, records, volume_start, steps)
         restart.Restart2D(t, restart.domain_fingerprint(c.mesh, c.bed), x[0], x[1], x[2], c.n_elements, c.nn,
-                          metadata=[("steps", str(steps)), ("case", run.name)],
-                          extra=[("station_records", rec)]).write(restart_to)
+                          metadata=metadata, text=text, extra=extra).write(restart_to)
     clips = phys.negative_depth_clips()
     state.free()
-    return RunReport(t, steps, records, clips, x)
+    return RunReport(t, steps, records, clips, x, (volume_start, float(np.sum(masses * x[0]))))
 
 
 def write_stations(path: str, records):

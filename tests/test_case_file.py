@@ -161,6 +161,70 @@ def test_restart_round_trip_and_checks(tmp_path):
         rst.check_domain(c.mesh, bed2, c.order)
 
 
+def _reference_layout_restart(run, path, rep):
+    """A restart of `rep`'s end with the driver data laid out as the reference
+    runner writes them (src/case/run/restart.rs:191-213), built here key by key."""
+    c = run.case
+    meta = [("build", "dgcom 0.1.0 (reference)")] + [(f"station.{j}", s.name) for j, s in enumerate(run.stations)]
+    extra = [("output.counters", np.array([3.0, 1.25e9]))]
+    for j, s in enumerate(run.stations):
+        rows = np.array([r[1:] for r in rep.records if r[0] == s.name])
+        for col, field in enumerate(("time_s", "eta_m", "u_m_s", "v_m_s", "depth_m")):
+            extra.append((f"station.{j}.{field}", rows[:, col].copy()))
+        for field in ("hs_m", "tp_s", "wave_from_deg"):
+            extra.append((f"station.{j}.{field}", np.zeros(0)))
+    R.Restart2D(rep.t, R.domain_fingerprint(c.mesh, c.bed), *rep.state, c.n_elements, c.nn, metadata=meta,
+                text=[("case", "# dgcom 0.1.0 (reference)\n[run]\nname = \"tidal_channel\"\n")],
+                extra=extra).write(path)
+
+
+def test_a_reference_layout_restart_resumes(tmp_path):
+    """A restart in the reference runner's layout (no `steps`, the records in
+    `station.<j>.<field>`, `output.counters`) resumes on the restatement: the
+    earlier records reach stations.csv and the run equals a straight one bit for
+    bit; the runner's own restart has the same keys; without the counters, with
+    other stations or more of them it is refused by name."""
+    run = CF.load(CASE, ["run.duration_hours=2.0"])
+    straight = CF.run_case(run, out_dir=str(tmp_path / "straight"), physics=OraclePhysics(run.case))
+    first = CF.run_case(run, stop_at=3600.0, physics=OraclePhysics(run.case))
+    path = str(tmp_path / "ref.restart")
+    _reference_layout_restart(run, path, first)
+    rest = CF.run_case(run, out_dir=str(tmp_path / "rest"), restart_from=path, physics=OraclePhysics(run.case))
+    assert rest.records == straight.records and len(rest.records) == 5 * 3
+    assert rest.volume[0] == 1.25e9
+    for a, b in zip(rest.state, straight.state):
+        assert np.array_equal(a, b)
+    assert open(tmp_path / "rest" / "stations.csv").read() == open(tmp_path / "straight" / "stations.csv").read()
+    # the runner's own restart: the reference's keys (and its step count)
+    own = str(tmp_path / "own.restart")
+    CF.run_case(run, stop_at=3600.0, restart_to=own, physics=OraclePhysics(run.case))
+    back = R.Restart2D.read(own)
+    ref = R.Restart2D.read(path)
+    assert [k for k, _ in back.extra] == [k for k, _ in ref.extra]
+    assert [k for k, _ in back.metadata if k != "steps"] == [k for k, _ in ref.metadata]
+    assert [k for k, _ in back.text] == ["case"]
+    assert dict(back.extra)["output.counters"][1] == straight.volume[0]
+    assert CF.load(dict(back.text)["case"].split("\n", 1)[1]).doc == run.doc
+    resumed_own = CF.run_case(run, restart_from=own, physics=OraclePhysics(run.case))
+    assert resumed_own.records == straight.records and resumed_own.steps == straight.steps
+    # refusals, by name
+    for drop, match in (("output.counters", "no output.counters"), ("station.1.eta_m", "no station.1.eta_m")):
+        bad = R.Restart2D.read(path)
+        bad.extra = [(k, v) for k, v in bad.extra if k != drop]
+        bad.write(path + ".bad")
+        with pytest.raises(CF.CaseError, match=match):
+            CF.run_case(run, restart_from=path + ".bad", physics=OraclePhysics(run.case))
+    bad = R.Restart2D.read(path)
+    bad.metadata = [(k, "elsewhere" if k == "station.2" else v) for k, v in bad.metadata]
+    bad.write(path + ".bad")
+    with pytest.raises(CF.CaseError, match="station 2 is head, the restart's elsewhere"):
+        CF.run_case(run, restart_from=path + ".bad", physics=OraclePhysics(run.case))
+    bad.metadata = [(k, v) for k, v in ref.metadata] + [("station.3", "extra")]
+    bad.write(path + ".bad")
+    with pytest.raises(CF.CaseError, match="more stations"):
+        CF.run_case(run, restart_from=path + ".bad", physics=OraclePhysics(run.case))
+
+
 def test_the_runner_on_the_restatement():
     """The runner over the restatement: 5,765 steps to 25 h, 51 records per
     station (the start and every 30 min), no clip."""
