@@ -6,10 +6,12 @@ state kept in, and resumed from, the reference's `DGRST2D` restart files
 (`dgcom_amd.restart`).
 
 The schema's parts that this path runs: `[run]` (name, start, duration_hours),
-`[mesh]` (order, drop_land) with `[mesh.rectangle]`, `[bathymetry.ramp]` or
+`[mesh]` (order, drop_land, origin) with `[mesh.rectangle]`, `[bathymetry.ramp]` or
 `[bathymetry.flat]`, `[physics]` (coriolis by latitude or f0, friction by
 Manning n), `[[boundary]]` with an elevation-only harmonic tide
-(`velocity` "at_rest" or "incoming_wave"), `[numerics]` (integrator, cfl,
+(`velocity` "at_rest" or "incoming_wave") or a tidal atlas (`[boundary.atlas]`:
+its file, coverage, correction "none", "minors" or "theory", transport scaling
+and ramp; `dgcom_amd.atlas`), `[numerics]` (integrator, cfl,
 dt_max_s, local_time_stepping_levels), `[output]` (interval_minutes and
 `[[output.station]]`). Anything else is refused by name, as the reference's
 device path refuses physics it does not run (device.rs:79-143).
@@ -24,7 +26,7 @@ import os
 import numpy as np
 import tomli
 
-from . import cases, restart
+from . import atlas as A, cases, restart, tides as T
 
 H_VELOCITY = 1e-3            # run.rs:51: WetDryConfig::DEFAULT_H_DRY
 DEFAULT_OPEN_WATER_DEPTH = 3.0   # schema.rs:1900
@@ -35,14 +37,16 @@ LANDING_SLACK = 1e-9         # multirate.rs:588
 
 _KNOWN = {
     "run": {"name", "start", "duration_hours"},
-    "mesh": {"order", "drop_land", "rectangle"},
+    "mesh": {"order", "drop_land", "rectangle", "origin"},
+    "mesh.origin": {"lat", "lon"},
     "mesh.rectangle": {"x_m", "y_m", "nx", "ny", "south", "east", "north", "west"},
     "bathymetry": {"ramp", "flat"},
     "bathymetry.ramp": {"from_m", "to_m", "depth_m"},
     "bathymetry.flat": {"depth_m"},
     "physics": {"coriolis", "friction"},
-    "boundary": {"tag", "velocity", "tide"},
+    "boundary": {"tag", "velocity", "tide", "atlas"},
     "boundary.tide": {"ramp_hours", "constituents", "mean_m"},
+    "boundary.atlas": {"file", "coverage_m", "max_closed", "correction", "transport_scaling", "ramp_hours"},
     "numerics": {"integrator", "cfl", "dt_max_s", "local_time_stepping_levels"},
     "output": {"interval_minutes", "station"},
     "output.station": {"name", "x_m", "y_m", "open_water_depth_m", "max_offset_m"},
@@ -113,6 +117,8 @@ class CaseRun:
     lts_levels: int
     start: tuple | None
     doc: dict = dataclasses.field(default_factory=dict)   # the case as read, overrides applied
+    boundary_tides: object = None   # the open boundary's tides per node from an atlas (cases.BoundaryTides)
+    notes: list = dataclasses.field(default_factory=list)  # what the loader did (faces closed, minors inferred)
 
 
 def load(path_or_text: str, overrides=()) -> CaseRun:
@@ -120,8 +126,10 @@ def load(path_or_text: str, overrides=()) -> CaseRun:
     if os.path.exists(path_or_text):
         with open(path_or_text, "rb") as f:
             doc = tomli.load(f)
+        base = os.path.dirname(os.path.abspath(path_or_text))
     else:
         doc = tomli.loads(path_or_text)
+        base = os.getcwd()
     doc = apply_overrides(doc, overrides)
     for section in doc:
         if section not in ("run", "mesh", "bathymetry", "physics", "boundary", "numerics", "output"):
@@ -137,6 +145,10 @@ def load(path_or_text: str, overrides=()) -> CaseRun:
     order = int(mesh_t.get("order", 2))
     if mesh_t.get("drop_land", False):
         raise CaseError("[mesh] drop_land = true is not run by this device path")
+    origin = mesh_t.get("origin")
+    if origin is not None:
+        _check_keys("mesh.origin", origin)
+        origin = (float(origin["lat"]), float(origin["lon"]))
     sides = []
     for side in ("south", "east", "north", "west"):
         kind = rect.get(side, "wall")
@@ -175,6 +187,7 @@ def load(path_or_text: str, overrides=()) -> CaseRun:
         start = (st.year, st.month, st.day, st.hour, st.minute, st.second)
     tide = cases.Tide()
     incoming = False
+    atlas_spec = None
     for b in doc.get("boundary", []):
         _check_keys("boundary", b)
         if b.get("tag", "open") != "open":
@@ -184,11 +197,16 @@ def load(path_or_text: str, overrides=()) -> CaseRun:
             raise CaseError(f"[[boundary]] velocity = {vel!r}: 'at_rest' or 'incoming_wave'")
         incoming = vel == "incoming_wave"
         t = b.get("tide")
+        if t is not None and b.get("atlas") is not None:
+            raise CaseError("[[boundary]]: a tide or an atlas, not both")
         if t is not None:
             _check_keys("boundary.tide", t)
             cons = [(c["name"], float(c["amplitude_m"]), float(c.get("phase_deg", 0.0))) for c in t["constituents"]]
             tide = cases.harmonic_tide(cons, float(t.get("ramp_hours", 1.0)), start=start,
                                        mean=float(t.get("mean_m", 0.0)))
+        if b.get("atlas") is not None:
+            atlas_spec = b["atlas"]
+            _check_keys("boundary.atlas", atlas_spec)
     num = doc.get("numerics", {})
     _check_keys("numerics", num)
     integ = num.get("integrator", "ssp_rk43")
@@ -199,6 +217,10 @@ def load(path_or_text: str, overrides=()) -> CaseRun:
     h, hu, hv = cases.still_water(bed)
     case = cases.Case(run.get("name", "case"), order, mesh, bed, np.full(mesh.n_elements, manning), f0, tide,
                       h, hu, hv, cfl=cfl, scheme=scheme, n_steps=0, incoming_wave=incoming)
+    notes = []
+    boundary_tides = None
+    if atlas_spec is not None:
+        boundary_tides = _atlas_tides(case, atlas_spec, base, origin, start, notes)
     out = doc.get("output", {})
     _check_keys("output", out)
     stations = []
@@ -209,7 +231,50 @@ def load(path_or_text: str, overrides=()) -> CaseRun:
                                 float(s.get("max_offset_m", DEFAULT_MAX_OFFSET))))
     return CaseRun(run.get("name", "case"), case, 3600.0 * float(run.get("duration_hours", 1.0)),
                    60.0 * float(out.get("interval_minutes", 60.0)), stations, num.get("dt_max_s"),
-                   int(num.get("local_time_stepping_levels", 0)), start, doc)
+                   int(num.get("local_time_stepping_levels", 0)), start, doc, boundary_tides, notes)
+
+
+def _atlas_tides(case, spec: dict, base: str, origin, start, notes: list):
+    """`[boundary.atlas]` as the reference's domain and runner build it (domain.rs:193-243,
+    run.rs:706-730): read, corrected, its coverage checked (uncovered open faces become
+    walls, within `max_closed`), projected onto the open boundary's nodes with the nodal
+    terms of `run.start`, ramped and transport-scaled."""
+    if origin is None:
+        raise CaseError("[boundary.atlas] needs a placed mesh ([mesh] origin)")
+    if start is None:
+        raise CaseError("[boundary.atlas] needs run.start (the nodal corrections' epoch)")
+    coverage = float(spec.get("coverage_m", 5000.0))
+    max_closed = float(spec.get("max_closed", 0.05))
+    ramp_hours = float(spec.get("ramp_hours", 1.0))
+    ratio = spec.get("transport_scaling")
+    if not coverage > 0.0:
+        raise CaseError("boundary.atlas.coverage_m must be positive")
+    if not 0.0 <= max_closed <= 1.0:
+        raise CaseError("boundary.atlas.max_closed must be in [0, 1]")
+    if ratio is not None and not float(ratio) > 0.0:
+        raise CaseError("boundary.atlas.transport_scaling must be positive")
+    if ramp_hours < 0.0:
+        raise CaseError("boundary.atlas.ramp_hours must not be negative")
+    path = spec["file"] if os.path.isabs(spec["file"]) else os.path.join(base, spec["file"])
+    try:
+        atlas = A.TidalAtlas.read(path)
+        correction = spec.get("correction", "none")
+        if correction == "minors":
+            added = atlas.infer_missing_minors()
+            notes.append(f"tidal atlas {path}: {len(added)} minor constituents inferred: {' '.join(added)}")
+        elif correction == "theory":
+            atlas.infer_from_theory()
+        elif correction != "none":
+            raise CaseError(f"[boundary.atlas] correction = {correction!r}: 'none', 'minors' or 'theory'")
+        projection = A.LocalProjection(*origin)
+        faces, length, missed = A.uncovered_faces(case, projection, atlas, coverage)
+        closed = A.close_uncovered(case, faces, length, missed, max_closed)
+        notes.append(f"tidal atlas {path}: {closed} open faces farther than {coverage / 1000.0:.1f} km made walls")
+        clock = T.ModelClock.at_datetime(*start)
+        return A.boundary_tides(case, atlas, projection, clock, coverage, ramp=3600.0 * ramp_hours,
+                                max_ratio=float(ratio) if ratio is not None else 0.0)
+    except (A.TidalAtlasError, OSError) as e:
+        raise CaseError(f"{path}: {e}") from e
 
 
 # ---------------------------------------------------------------------------
@@ -523,6 +588,8 @@ def run_case(run: CaseRun, out_dir: str | None = None, device: int = 0, restart_
     from . import swe2d
     c = run.case
     phys = physics or swe2d.SWEPhysics2D.from_case(c, device=device)
+    if run.boundary_tides is not None:
+        phys.set_boundary_tides(run.boundary_tides)
     masses = cases.node_masses(c)
     t = 0.0
     h0, hu0, hv0 = c.h, c.hu, c.hv
