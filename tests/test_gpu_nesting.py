@@ -75,3 +75,80 @@ def test_refusals():
         swe2d.Ensemble([phys])
     phys.set_nesting(None)
     phys.step(s, 5.0 * 3600.0, 0.1)
+    # the parent as the open boundary's state needs a slot at every open face node
+    with pytest.raises(RuntimeError, match="no parent"):
+        phys.set_nesting_boundary(True)
+    ns = _band(c, timescale=300.0)
+    n1 = c.order + 1
+    k, f, fi, _, _ = cases.open_face_nodes(c)[0]
+    ns.slot[k * c.nn + cases.face_node(n1, f, fi)] = 0xFFFFFFFF
+    phys.set_nesting(ns)
+    with pytest.raises(RuntimeError, match=f"open face node {fi} of face {f} of element {k} has no parent slot"):
+        phys.set_nesting_boundary(True)
+
+
+# ---------------------------------------------------------------------------
+# The reference's nesting gate (gpu/src/nest.rs, notes/gpu.md:199): the GPU equals the
+# restatement bit for bit over its 8 h, with global steps and with 3 levels
+# ---------------------------------------------------------------------------
+GATE = {"cubic": (10845, 2285), "linear": (10862, 2293)}
+
+
+def _landed_steps(step_dt, t_end):
+    t, n = 0.0, 0
+    while t < t_end:
+        dt = step_dt(t, None)
+        landed = t + dt * (1.0 + 1e-9) >= t_end
+        if landed:
+            dt = t_end - t
+        step_dt(t, dt)
+        t = t_end if landed else t + dt
+        n += 1
+    return n
+
+
+@pytest.mark.parametrize("variant", ["cubic", "linear"])
+def test_the_nesting_gate_equals_the_restatement(variant):
+    from dgcom_amd import multirate as M, nesting as NS
+    c, pm = NS.gate_case(variant)
+    t_end = NS.GATE_HOURS * 3600.0
+    orc = O.Oracle.from_case(c)
+    orc.set_nesting(pm.nesting)
+    orc.set_nesting_boundary(True)
+    x = [c.h.copy(), c.hu.copy(), c.hv.copy()]
+
+    def orc_step(t, dt):
+        if dt is None:
+            return orc.dt(*x, c.cfl, c.scheme)
+        orc.run_dts(*x, t, [dt], c.scheme)
+
+    n_o = _landed_steps(orc_step, t_end)
+    phys = swe2d.SWEPhysics2D.from_case(c)
+    phys.set_nesting(pm.nesting)
+    phys.set_nesting_boundary(True)
+    s = phys.state(c.h, c.hu, c.hv)
+
+    def gpu_step(t, dt):
+        if dt is None:
+            return phys.compute_dt_ssp(s, c.cfl, c.scheme)
+        phys.step(s, t, dt, c.scheme)
+
+    n_g = _landed_steps(gpu_step, t_end)
+    assert n_g == n_o == GATE[variant][0]
+    for a, b in zip(s.download(), x):
+        assert np.array_equal(a, b)
+    assert phys.negative_depth_clips() == orc.clips() == 0
+    # three levels of local time stepping
+    x = [c.h.copy(), c.hu.copy(), c.hv.copy()]
+    orc = O.Oracle.from_case(c)
+    orc.set_nesting(pm.nesting)
+    orc.set_nesting_boundary(True)
+    nb = M.neighbours(c.mesh.links)
+    _, n_o, _ = M.run(orc, nb, x, 0.0, t_end, c.cfl, 3, c.scheme)
+    s2 = phys.state(c.h, c.hu, c.hv)
+    local = phys.local_time_stepping()
+    _, n_g, st = M.run(local, nb, s2, 0.0, t_end, c.cfl, 3, c.scheme)
+    st.release(local)
+    assert n_g == n_o == GATE[variant][1]
+    for a, b in zip(s2.download(), x):
+        assert np.array_equal(a, b)
