@@ -435,23 +435,37 @@ def main():
                             "flops_per_stage_launch": flops,
                             "wet_kernel_valu_insts_per_wave": per.get("wet_stage_kernel<3>", {}).get("valu_insts_per_wave"),
                             "wet_kernel_fp64_insts_per_wave": per.get("wet_stage_kernel<3>", {}).get("fp64_insts_per_wave")}
+    progress("the large unstructured workload", t0)
     if args.weather_steps > 0:
         out["weather"] = weather_bench(case, phys, state, t, args.weather_steps, out["ms_per_step"])
+        progress("the weather leg", t0)
     if not args.no_adjoint:
         phys.close()
         out["adjoint"] = adjoint_bench(args.adjoint_n, args.adjoint_steps, args.adjoint_snapshots, 10, 20)
+        progress("the adjoint", t0)
     if not args.no_ensemble:
         out["ensemble"] = ensemble_bench(args.ensemble_members, args.ensemble_steps, 20_000, 400_000)
+        progress("the ensemble", t0)
     if args.adjoint_ensemble_members > 0:
         out["adjoint_ensemble"] = adjoint_ensemble_bench(args.adjoint_ensemble_members, args.adjoint_ensemble_steps, 20)
+        progress("the inversion ensemble", t0)
     if args.lts and args.lts != "none":
-        out["lts"] = [lts_bench(*map(int, spec.split("x")), args.lts_seconds, args.lts_levels)
-                      for spec in args.lts.split(",")]
+        out["lts"] = []
+        for spec in args.lts.split(","):
+            out["lts"].append(lts_bench(*map(int, spec.split("x")), args.lts_seconds, args.lts_levels))
+            progress(f"local time stepping on the {spec} channel", t0)
     if not args.no_c1:
         out["c1"] = c1_bench()
+        progress("the 25 h tidal estuary", t0)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
+        progress("the CPU baseline", t0)
     print(json.dumps(out), flush=True)
+
+
+def progress(what: str, t0: float):
+    """One line on stderr per finished leg (the JSON result is the only stdout line)."""
+    print(f"bench: {what} done ({time.time() - t0:.0f} s)", file=sys.stderr, flush=True)
 
 
 if __name__ == "__main__":

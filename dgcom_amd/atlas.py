@@ -41,9 +41,8 @@ class LocalProjection:
 
     def __init__(self, ref_lat: float, ref_lon: float):
         lat_rad = ref_lat * math.pi / 180.0
-        cos_lat = math.cos(lat_rad)
+        sin_lat, cos_lat = T.sincos(lat_rad)
         e2 = 2.0 * self.F - self.F * self.F
-        sin_lat = math.sin(lat_rad)
         sin2 = sin_lat * sin_lat
         rho = self.A * (1.0 - e2) / math.pow(1.0 - e2 * sin2, 1.5)
         nu = self.A / math.sqrt(1.0 - e2 * sin2)
@@ -222,9 +221,9 @@ class TidalAtlas:
         """The nearest point and its distance (m, spherical earth) (tidal_atlas.rs:518-530)."""
         best = None
         for p in self.points:
-            la1, la2 = T.to_radians(lat), T.to_radians(p.lat)
+            (s1, c1), (s2, c2) = T.sincos(T.to_radians(lat)), T.sincos(T.to_radians(p.lat))
             dlon = T.to_radians(p.lon - lon)
-            c = math.sin(la1) * math.sin(la2) + math.cos(la1) * math.cos(la2) * math.cos(dlon)
+            c = s1 * s2 + c1 * c2 * math.cos(dlon)
             d = EARTH_RADIUS * math.acos(min(max(c, -1.0), 1.0))
             if best is None or d < best[1]:
                 best = (p, d)
@@ -264,7 +263,7 @@ class TidalAtlas:
             lat, lon = projection.xy_to_geo(x, y)
             nx, ny = projection.geo_to_xy(lat + 1e-4, lon)
             theta = math.atan2(nx - x, ny - y)
-            sin_t, cos_t = math.sin(theta), math.cos(theta)
+            sin_t, cos_t = T.sincos(theta)
             acc = 0.0
             for w, i in weights:
                 acc += w * (self.points[i].mean if self.points[i].mean is not None else 0.0)
@@ -280,8 +279,8 @@ class TidalAtlas:
                     re, im = 0.0, 0.0
                     for w, i in weights:
                         amp, lag = get(self.points[i].constituents[j])
-                        g = T.to_radians(lag)
-                        re, im = re + w * amp * math.cos(g), im - w * amp * math.sin(g)
+                        sg, cg = T.sincos(T.to_radians(lag))
+                        re, im = re + w * amp * cg, im - w * amp * sg
                     return re, im
 
                 eta = interp(lambda c: c.eta)
@@ -292,8 +291,7 @@ class TidalAtlas:
                     v = (-east[0] * sin_t + north[0] * cos_t, -east[1] * sin_t + north[1] * cos_t)
                 else:
                     u = v = (0.0, 0.0)
-                ph = n.phase_offset_rad()
-                sin_p, cos_p = math.sin(ph), math.cos(ph)
+                sin_p, cos_p = T.sincos(n.phase_offset_rad())
                 for q, (re, im) in enumerate((eta, u, v)):
                     re, im = re * cos_p - im * sin_p, re * sin_p + im * cos_p
                     coefficients[s, j, 2 * q] = n.f * re

@@ -92,6 +92,7 @@ struct dg_ctx {
     // static geometry cache (geometry_kernel): geo per node; fgeo per face [3][K*4], then per face node
     // [K][4][N+1] the outer node's distance
     double *geo = nullptr, *fgeo = nullptr;
+    double* emass = nullptr;  // [K][2]: the positivity bound's static mass sums (geometry_kernel)
     uint32_t* links = nullptr;
     uint8_t *tags = nullptr, *affine = nullptr;
     unsigned long long* clips = nullptr;
@@ -130,13 +131,15 @@ struct dg_ctx {
     uint32_t* tide_slot = nullptr;
     double *tide_coef = nullptr, *tide_mean = nullptr, *tide_src = nullptr;
     int atlas_nc = 0, atlas_velocity = 0;
-    double atlas_omega[8] = {}, atlas_f_rate[8] = {}, atlas_ramp = 0.0, atlas_max_ratio = 0.0;
+    double atlas_omega[ATLAS_MAX_CONSTITUENTS] = {}, atlas_f_rate[ATLAS_MAX_CONSTITUENTS] = {};
+    double atlas_ramp = 0.0, atlas_max_ratio = 0.0;
     // the nesting band (dg_set_nesting): device tables and the snapshot times on the host
     uint32_t* nest_slot = nullptr;
     float* nest_series = nullptr;
     double *nest_weight = nullptr, *nest_depth = nullptr;
     std::vector<double> nest_times;
     int nest_velocity = 0, nest_momentum_only = 0, nest_cubic = 0, nest_boundary = 0;
+    std::vector<uint32_t> h_nest_slot;  // the host copy of the slot of every node (the boundary's coverage check)
     double nest_epoch = 0.0;
     // the gridded weather (dg_set_weather): the snapshots on the device, their times on the host
     float* wx_series = nullptr;
@@ -232,11 +235,11 @@ static int free_statics(dg_ctx* c) {
                     c->innerbuf, c->facecot, c->kind, c->st_elem, c->st_w, c->st_eta, c->geo, c->fgeo,
                     c->lts_level, c->lts_lrate, c->lts_order, c->depth_flags, c->tide_slot, c->tide_coef,
                     c->tide_mean, c->tide_src, c->nest_slot, c->nest_series, c->nest_weight, c->nest_depth,
-                    c->wx_series, c->manning_node, c->coriolis_node};
+                    c->wx_series, c->manning_node, c->coriolis_node, c->emass};
     for (void* p : ptrs)
         if (p) DG_TRY(hipFree(p));
     c->gbed = c->gman = c->facebuf = c->innerbuf = c->facecot = nullptr;
-    c->geo = c->fgeo = nullptr;
+    c->geo = c->fgeo = c->emass = nullptr;
     c->lts_level = c->lts_lrate = c->depth_flags = nullptr;
     c->tide_slot = nullptr;
     c->tide_coef = c->tide_mean = c->tide_src = nullptr;
@@ -296,9 +299,9 @@ static int launch_geometry(dg_ctx* c) {
     const int epb = 64 / c->nn;
     dim3 grid((unsigned)((c->K + epb - 1) / epb));
     switch (c->order) {
-        case 1: hipLaunchKernelGGL(geometry_kernel<1>, grid, dim3(64), 0, c->stream, c->corners, c->affine, c->links, c->K, c->geo, c->fgeo); break;
-        case 2: hipLaunchKernelGGL(geometry_kernel<2>, grid, dim3(64), 0, c->stream, c->corners, c->affine, c->links, c->K, c->geo, c->fgeo); break;
-        default: hipLaunchKernelGGL(geometry_kernel<3>, grid, dim3(64), 0, c->stream, c->corners, c->affine, c->links, c->K, c->geo, c->fgeo); break;
+        case 1: hipLaunchKernelGGL(geometry_kernel<1>, grid, dim3(64), 0, c->stream, c->corners, c->affine, c->links, c->K, c->geo, c->fgeo, c->emass); break;
+        case 2: hipLaunchKernelGGL(geometry_kernel<2>, grid, dim3(64), 0, c->stream, c->corners, c->affine, c->links, c->K, c->geo, c->fgeo, c->emass); break;
+        default: hipLaunchKernelGGL(geometry_kernel<3>, grid, dim3(64), 0, c->stream, c->corners, c->affine, c->links, c->K, c->geo, c->fgeo, c->emass); break;
     }
     DG_TRY(hipGetLastError());
     return 0;
@@ -323,10 +326,14 @@ static void set_atlas(const dg_ctx* c, StageParams& p, double time) {
     }
     p.atlas_moment[0] = rf;
     for (int j = 0; j < c->atlas_nc; ++j) {
-        const double arg = c->atlas_omega[j] * time;
+        // the reference's `(omega t).sin_cos()` (tidal_atlas.rs:966-970) is one call of the C
+        // library's sincos (LLVM merges the pair), which differs from its separate sin and cos
+        // by an ulp for some arguments
+        double sn, cs;
+        sincos(c->atlas_omega[j] * time, &sn, &cs);
         p.atlas_moment[1 + 3 * j] = 1.0 + c->atlas_f_rate[j] * time;
-        p.atlas_moment[2 + 3 * j] = std::cos(arg);
-        p.atlas_moment[3 + 3 * j] = std::sin(arg);
+        p.atlas_moment[2 + 3 * j] = cs;
+        p.atlas_moment[3 + 3 * j] = sn;
     }
 }
 
@@ -476,7 +483,7 @@ static int launch_dt(dg_ctx* c, const double* state, double cfl, int scheme, boo
     size_t nt = (size_t)c->K * c->nn;
     DtParams p;
     p.h = state; p.hu = state + nt; p.hv = state + 2 * nt;
-    p.geo = c->geo; p.links = c->links; p.affine = c->affine;
+    p.corners = c->corners; p.emass = c->emass; p.links = c->links; p.affine = c->affine;
     p.n_elements = c->K;
     p.cfl = cfl; p.g = c->g; p.h_min = c->h_min; p.h_dry = c->h_dry;
     // PositivityBound::new (swe_2d.rs:2188-2203)
@@ -1142,6 +1149,7 @@ int dg_upload_form(dg_ctx* c, const dg_form* f) {
     DG_TRY(hipMemcpy(c->affine, aff.data(), K, hipMemcpyHostToDevice));
     DG_TRY(hipMalloc(&c->geo, (size_t)GEO_COUNT * K * nn * sizeof(double)));
     DG_TRY(hipMalloc(&c->fgeo, (size_t)(3 * 4 + 4 * (f->order + 1)) * K * sizeof(double)));
+    DG_TRY(hipMalloc(&c->emass, (size_t)2 * K * sizeof(double)));
     if (int r = launch_geometry(c)) return r;
     DG_TRY(hipStreamSynchronize(c->stream));
     return 0;
@@ -1372,6 +1380,14 @@ int dg_stage_where(dg_ctx* c, const dg_state* base, const dg_state* input, const
     p.manning_node = c->manning_node;
     p.coriolis_node = c->coriolis_node;
     if (c->tide_slot || c->nest_slot || c->wx_series) {
+        // The per-level moment rows go up through a ring of pinned host slots, reused after
+        // their upload's event: neither the event wait nor a copy from a slot that is written
+        // again before a replay belongs in a captured graph
+        hipStreamCaptureStatus capturing = hipStreamCaptureStatusNone;
+        DG_TRY(hipStreamIsCapturing(c->stream, &capturing));
+        DG_REQUIRE(capturing == hipStreamCaptureStatusNone,
+                   "dg_stage_where: a stage forced per node (boundary tides, nesting or weather) cannot be "
+                   "captured into a graph");
         constexpr int ROW = LVL_MOMENT, ROWS = DG_MAX_LEVELS + 1;
         if (!c->lvl_moment_dev) {
             DG_TRY(hipMalloc(&c->lvl_moment_dev, sizeof(double) * dg_ctx::MOMENT_RING * ROWS * ROW));
@@ -1667,7 +1683,7 @@ int dg_set_boundary_tides(dg_ctx* c, int n_c, const double* omega, const double*
                           const double* mean, const double* source_depth) {
     DG_LOCK(c);
     DG_REQUIRE(c && c->K > 0, "dg_set_boundary_tides: null context or no form");
-    DG_REQUIRE(n_c >= 0 && n_c <= 8, "dg_set_boundary_tides: 0 to 8 constituents");
+    DG_REQUIRE(n_c >= 0 && n_c <= ATLAS_MAX_CONSTITUENTS, "dg_set_boundary_tides: 0 to 21 constituents");
     DG_TRY(hipSetDevice(c->device));
     DG_TRY(hipStreamSynchronize(c->stream));
     for (void* ptr : {(void*)c->tide_slot, (void*)c->tide_coef, (void*)c->tide_mean, (void*)c->tide_src})
@@ -1707,6 +1723,8 @@ int dg_set_boundary_tides(dg_ctx* c, int n_c, const double* omega, const double*
     return 0;
 }
 
+static int check_parent_covers_boundary(dg_ctx* c);
+
 int dg_set_nesting(dg_ctx* c, int64_t n_slots, const uint32_t* slot_of_node, int n_times, const double* times,
                    double epoch, int cubic, const float* series, const double* weight, const double* parent_depth, double timescale,
                    double wet_depth, int momentum_only, int velocity, double max_ratio, double reference_level,
@@ -1721,7 +1739,11 @@ int dg_set_nesting(dg_ctx* c, int64_t n_slots, const uint32_t* slot_of_node, int
     c->nest_series = nullptr;
     c->nest_weight = c->nest_depth = nullptr;
     c->nest_times.clear();
-    if (n_slots == 0) return 0;
+    c->h_nest_slot.clear();
+    if (n_slots == 0) {
+        c->nest_boundary = 0;
+        return 0;
+    }
     DG_REQUIRE(slot_of_node && times && series && weight && parent_depth && n_slots > 0 && n_times >= 1,
                "dg_set_nesting: missing tables");
     DG_REQUIRE(timescale > 0.0, "dg_set_nesting: the relaxation timescale must be positive");
@@ -1738,6 +1760,12 @@ int dg_set_nesting(dg_ctx* c, int64_t n_slots, const uint32_t* slot_of_node, int
     DG_TRY(hipMemcpy(c->nest_weight, weight, (size_t)n_slots * sizeof(double), hipMemcpyHostToDevice));
     DG_TRY(hipMemcpy(c->nest_depth, parent_depth, (size_t)n_slots * sizeof(double), hipMemcpyHostToDevice));
     c->nest_times.assign(times, times + n_times);
+    c->h_nest_slot.assign(slot_of_node, slot_of_node + nt);
+    if (c->nest_boundary)
+        if (int r = check_parent_covers_boundary(c)) {
+            c->nest_boundary = 0;
+            return r;
+        }
     c->nest_epoch = epoch;
     c->nest_cubic = cubic ? 1 : 0;
     c->nest_rate = 1.0 / timescale;
@@ -1796,9 +1824,43 @@ int dg_set_inverse_barometer(dg_ctx* c, double factor) {
     return 0;
 }
 
+// Every open-boundary face node must have a parent slot when the parent is the
+// open boundary's external state (DeviceExternal2D::Parent maps each open face node
+// to one, ocean_nesting.rs:543-562): an uncovered node is refused by name here
+// instead of falling back to the tide.
+static int check_parent_covers_boundary(dg_ctx* c) {
+    const int n1 = c->order + 1, nn = c->nn;
+    std::vector<uint8_t> tags((size_t)c->K * 4);
+    DG_TRY(hipMemcpy(tags.data(), c->tags, tags.size(), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> links((size_t)c->K * 4);
+    DG_TRY(hipMemcpy(links.data(), c->links, links.size() * 4, hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < c->K; ++k)
+        for (int f = 0; f < 4; ++f) {
+            if (links[4 * k + f] != LINK_BOUNDARY || tags[4 * k + f] != DG_BC_TIDE) continue;
+            for (int fi = 0; fi < n1; ++fi) {
+                const int step = (f & 1) ? fi * n1 : fi;
+                const int start = f == 0 ? 0 : f == 1 ? n1 - 1 : f == 2 ? nn - 1 : (n1 - 1) * n1;
+                const int node = (f & 2) ? start - step : start + step;
+                if (c->h_nest_slot[(size_t)k * nn + node] == 0xFFFFFFFFu) {
+                    char msg[200];
+                    std::snprintf(msg, sizeof msg,
+                                  "dg_set_nesting_boundary: open face node %d of face %d of element %lld has no parent "
+                                  "slot (the slot map must cover the open boundary)", fi, f, (long long)k);
+                    g_err = msg;
+                    return 2;
+                }
+            }
+        }
+    return 0;
+}
+
 int dg_set_nesting_boundary(dg_ctx* c, int on) {
     DG_LOCK(c);
     DG_REQUIRE(c, "dg_set_nesting_boundary: null context");
+    if (on) {
+        DG_REQUIRE(c->nest_slot, "dg_set_nesting_boundary: no parent (dg_set_nesting first)");
+        if (int r = check_parent_covers_boundary(c)) return r;
+    }
     c->nest_boundary = on ? 1 : 0;
     return 0;
 }
@@ -2368,7 +2430,7 @@ int dg_batch_create(int n, dg_ctx* const* members, int scheme, dg_batch** out) {
         p.flags = c->flags;
         p.clips = c->clips;
         DtParams& d = M.dp;
-        d.geo = c->geo; d.links = c->links; d.affine = c->affine; d.n_elements = c->K;
+        d.corners = c->corners; d.emass = c->emass; d.links = c->links; d.affine = c->affine; d.n_elements = c->K;
         d.g = c->g; d.h_min = c->h_min; d.h_dry = c->h_dry;
         d.positivity = (c->flags & DG_WET_DRY) ? 1 : 0;
         const int o = c->order;

@@ -55,7 +55,7 @@ DG_HD void make_node_adj(double h, double hu, double hv, double h_min, const Nod
 
 // sqrt(max(x, 0)) reverse
 DG_HD double sqrt_pos_adj(double x, double yb) {
-    const double y = sqrt(fmax_(x, 0.0));
+    const double y = sqrt_rn(fmax_(x, 0.0));
     return (x >= 0.0 && y > 0.0) ? yb / (2.0 * y) : 0.0;
 }
 
@@ -64,14 +64,14 @@ DG_HD void hll_aligned_adj(const Hll& l, const Hll& r, double g, double h_min, c
     const double hl = l.h, hr = r.h;
     const bool dry = (hl <= h_min) && (hr <= h_min);
     if (dry) return;
-    const double sql = sqrt(fmax_(hl, 0.0)), sqr = sqrt(fmax_(hr, 0.0));
-    const double sqg = sqrt(g);
+    const double sql = sqrt_rn(fmax_(hl, 0.0)), sqr = sqrt_rn(fmax_(hr, 0.0));
+    const double sqg = sqrt_rn(g);
     const double cl = sqg * sql, cr = sqg * sqr;
     const bool roe = (sql + sqr) > 1e-10;
     const double h_roe = 0.5 * (hl + hr);
     const double den = sql + sqr;
     const double un_roe = roe ? (sql * l.un + sqr * r.un) / den : 0.0;
-    const double c_roe = roe ? sqrt(g * h_roe) : 0.0;
+    const double c_roe = roe ? sqrt_rn(g * h_roe) : 0.0;
     const double a1 = l.un - cl, a2 = un_roe - c_roe;
     const double b1 = r.un + cr, b2 = un_roe + c_roe;
     const bool lwet = hl > h_min, rwet = hr > h_min;
@@ -157,7 +157,7 @@ DG_HD void ram_pressure_adj(double h, double un, double sill, double g, double o
     const double head = up * up * (0.5 / g);
     if (!(sill > head)) return;
     const double A = 1.0 - head / sill;
-    const double sq = sqrt(g * h);
+    const double sq = sqrt_rn(g * h);
     const double C = 2.0 * up + sq;
     // ram = ((A h) up) C
     const double Ab = outb * C * up * h;
@@ -198,7 +198,7 @@ DG_HD void hll_side_adj(const Side& q, double b_face, double nx, double ny, doub
 // hydrostatic_hll reverse: the cotangents of the two sides, given those of (fa, fb)
 DG_HD void hydro_hll_adj(const Side& qa, const Side& qb, double mx, double my, double g, double h_min, const Cons& fab,
                          const Cons& fbb, SideBar& qab, SideBar& qbb) {
-    const double norm = sqrt(mx * mx + my * my);
+    const double norm = sqrt_rn(mx * mx + my * my);
     const double inv_norm = 1.0 / norm;
     const double nx = mx * inv_norm, ny = my * inv_norm;
     const double b_face = fmax_(qa.b, qb.b);
@@ -275,8 +275,8 @@ DG_HD void characteristic_adj(double h, double hu, double hv, double bed, double
     const double h_e = fmax_(xe, 0.0);
     double ui = 0.0, vi = 0.0, inv1 = 0.0;
     if (h > h_min) { inv1 = 1.0 / h; ui = hu * inv1; vi = hv * inv1; }
-    const double c_0 = sqrt(g * fmax_(-bed, 0.0));
-    const double un_e = incoming ? 2.0 * (c_0 - sqrt(g * h_e)) : 0.0;
+    const double c_0 = sqrt_rn(g * fmax_(-bed, 0.0));
+    const double un_e = incoming ? 2.0 * (c_0 - sqrt_rn(g * h_e)) : 0.0;
     const double ut_e = -ui * ny + vi * nx;
     const double h_i = fmax_(h, 0.0);
     double un_i = 0.0, ut_i = 0.0, u2 = 0.0, v2 = 0.0;
@@ -285,7 +285,7 @@ DG_HD void characteristic_adj(double h, double hu, double hv, double bed, double
         un_i = u2 * nx + v2 * ny;
         ut_i = -u2 * ny + v2 * nx;
     }
-    const double c_i = sqrt(g * h_i), c_e = sqrt(g * h_e);
+    const double c_i = sqrt_rn(g * h_i), c_e = sqrt_rn(g * h_e);
     const double w_out = un_i + 2.0 * c_i, w_in = un_e - 2.0 * c_e;
     const bool wi = h_i > h_min, we = h_e > h_min;
     int branch;
@@ -394,7 +394,7 @@ DG_HD void desing_velocity_adj(double h, double hu, double hv, double eps, doubl
     }
     const double r = h / eps;
     const double r4 = r * r * r * r;
-    const double s = sqrt(1.0 + r4);
+    const double s = sqrt_rn(1.0 + r4);
     const double num = 1.4142135623730951 * r, den = eps * s;
     const double factor = num / den;
     const double factorb = ub * hu + vb * hv;
@@ -419,7 +419,7 @@ DG_HD void correct_node_adj(double h, double hu, double hv, double h_dry, double
         hu1 = h * u;
         hv1 = h * v;
     }
-    const double m = sqrt(hu1 * hu1 + hv1 * hv1);
+    const double m = sqrt_rn(hu1 * hu1 + hv1 * hv1);
     double hu1b = hub, hv1b = hvb;
     if (m > u_max * h) {
         const double sc = u_max * h / m;

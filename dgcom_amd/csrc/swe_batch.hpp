@@ -136,7 +136,7 @@ __global__ __launch_bounds__(64) void batch_dt_kernel(BatchArgs A) {
     const uint32_t m = A.block_member[blk];
     const BatchMember& M = A.mem[m];
     DtParams p;  // field by field (an aggregate copy of the record would go through scratch)
-    p.geo = M.dp.geo; p.links = M.dp.links; p.affine = M.dp.affine; p.n_elements = M.dp.n_elements;
+    p.corners = M.dp.corners; p.emass = M.dp.emass; p.links = M.dp.links; p.affine = M.dp.affine; p.n_elements = M.dp.n_elements;
     p.cfl = M.dp.cfl; p.g = M.dp.g; p.h_min = M.dp.h_min; p.h_dry = M.dp.h_dry;
     p.pcfl = M.dp.pcfl; p.max_cfl = M.dp.max_cfl; p.max_cfl_sub = M.dp.max_cfl_sub; p.positivity = M.dp.positivity;
     p.element_dt = nullptr; p.slots = nullptr; p.xcd_stride = 0;

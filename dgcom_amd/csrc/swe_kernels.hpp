@@ -98,12 +98,13 @@ template <int N> __device__ __forceinline__ void face_vector(const Quad& c, int 
     vy = f == 0 ? v0y : f == 1 ? v1y : f == 2 ? v2y : v3y;
 }
 
-// The per-node geometry cache of geometry_kernel: J and grad r, grad s of the
-// node's own map, for the time step and the wet/dry correction's masses. The
-// stage kernels evaluate their metric and bed-slope factors from the element's
-// corners (64 B per element) with the same expressions, so the same bits.
+// The per-node geometry cache of geometry_kernel: J of the node's own map, for
+// the wet/dry correction's masses (read only for the elements it corrects). The
+// stage and time-step kernels evaluate their metric, bed-slope factors and
+// grad r, grad s from the element's corners (64 B per element) with
+// GeometricFactors2D::compute's expressions, so the same bits.
 enum {
-    GEO_DET, GEO_TRX, GEO_TRY, GEO_TSX, GEO_TSY,
+    GEO_DET,
     GEO_COUNT
 };
 
@@ -127,8 +128,8 @@ struct StageParams {
     unsigned long long* clips;
     double* fstar_out;   // optional [K][4][N+1][3]: F* at every face node, outward (this element's side)
     double *rh, *rhu, *rhv;  // optional: the stage's RHS, written by the forward, read by the VJP
-    // Static geometry cached per node, SoA [GEO_COUNT][K*NN] (geometry_kernel): J and the time
-    // step's grad r, grad s; per face [3][K*4]: the unit normal and sJ
+    // Static geometry cached per node, SoA [GEO_COUNT][K*NN] (geometry_kernel): J; per face
+    // [3][K*4]: the unit normal and sJ
     const double* geo;
     const double* fgeo;
     uint32_t* queue;     // [blocks][EPB]: the block's elements that take the subcell volume term
@@ -185,7 +186,7 @@ struct StageParams {
 
 // A level's row of the forcing moments: the atlas moment (25 values), then the nesting band's and the weather's
 // snapshot moments at LM_NEST and LM_WX, each [ramp or wind, 0 or pressure, len, idx x 4, w x 4]
-constexpr int LVL_MOMENT = 48, LM_NEST = 25, LM_WX = 36;
+constexpr int LM_NEST = 1 + 3 * ATLAS_MAX_CONSTITUENTS, LM_WX = LM_NEST + 11, LVL_MOMENT = LM_WX + 11;
 
 struct SnapMoment { int len; int idx[4]; double w[4]; double a, b; };
 
@@ -370,7 +371,7 @@ __device__ __forceinline__ double face_height(const Quad& c, bool affine, int f,
     constexpr int N1 = N + 1;
     double vx, vy;
     face_vector<N>(c, f, vx, vy);
-    double sj = sqrt(vx * vx + vy * vy);
+    double sj = sqrt_rn(vx * vx + vy * vy);
     int node = affine ? 0 : face_node<N>(f, fi);
     Metric m = bilinear(c, T.x1[node % N1], T.x1[node / N1]);
     return m.det / sj;
@@ -945,7 +946,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
                         u = su[t]; v = sv[t];
                     } else if (wet_dry) desing_velocity(hin, huin, hvin, p.h_dry, u, v);
                     else { u = 0.0; v = 0.0; if (hin > p.h_min) { double inv = 1.0 / hin; u = huin * inv; v = hvin * inv; } }
-                    double speed = sqrt(u * u + v * v);
+                    double speed = sqrt_rn(u * u + v * v);
                     if (speed > 0.0) {
                         double mn = NT == 1 ? mann : ld32(p.manning, k32 * 8u);
                         if (FORCE && p.manning_node) mn = ld32(p.manning_node, nb);  // the node's own n
@@ -991,26 +992,38 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
 // outer-node distance. Run once per form.
 template <int N>
 __global__ __launch_bounds__(64) void geometry_kernel(const double* corners, const uint8_t* affine_flags,
-                                                      const uint32_t* links, int64_t K, double* geo, double* fgeo) {
+                                                      const uint32_t* links, int64_t K, double* geo, double* fgeo,
+                                                      double* emass) {
     constexpr int N1 = N + 1, NN = N1 * N1, EPB = 64 / NN;
     const GllTables& T = c_gll[N];
+    __shared__ double smass[64];
     const int t = threadIdx.x, e = t / NN, i = t - e * NN, a = i % N1, j = i / N1;
     const int64_t k = (int64_t)blockIdx.x * EPB + e;
     const bool active = e < EPB && k < K;
     const size_t n = (size_t)k * NN + i, nt = (size_t)K * NN;
+    if (active) smass[t] = T.w2[i] * bilinear(load_quad(corners, k), T.x1[a], T.x1[j]).det;
+    lds_sync();
+    if (active && i == 0) {
+        // the positivity bound's static sums (swe_2d.rs:2165-2203): the node masses w_i J_i over
+        // the element and over its boundary nodes, in node order
+        double weight = 0.0, weight_b = 0.0;
+        for (int q = 0; q < NN; ++q) {
+            const int qa = q % N1, qj = q / N1;
+            weight += smass[t + q];
+            if (qa == 0 || qa == N1 - 1 || qj == 0 || qj == N1 - 1) weight_b += smass[t + q];
+        }
+        emass[2 * k] = weight;
+        emass[2 * k + 1] = weight_b;
+    }
     if (active) {
         const bool affine = affine_flags[k] != 0;
         const Quad c = load_quad(corners, k);
         const Metric m = bilinear(c, T.x1[a], T.x1[j]);
         geo[GEO_DET * nt + n] = m.det;
-        geo[GEO_TRX * nt + n] = m.y_s / m.det;
-        geo[GEO_TRY * nt + n] = -m.x_s / m.det;
-        geo[GEO_TSX * nt + n] = -m.y_r / m.det;
-        geo[GEO_TSY * nt + n] = m.x_r / m.det;
         if (i < 4) {
             double vx, vy;
             face_vector<N>(c, i, vx, vy);
-            const double sj = sqrt(vx * vx + vy * vy);
+            const double sj = sqrt_rn(vx * vx + vy * vy);
             const size_t fk = (size_t)k * 4 + i, nf4 = (size_t)K * 4;
             fgeo[fk] = vx / sj;
             fgeo[nf4 + fk] = vy / sj;
@@ -1234,7 +1247,8 @@ constexpr unsigned long long DT_SLOT_EMPTY = 0x7FF0000000000000ull;  // +infinit
 
 struct DtParams {
     const double *h, *hu, *hv;
-    const double* geo;                  // the geometry cache (geometry_kernel): grad r, grad s and J per node
+    const double* corners;              // [K][4][2]: grad r, grad s and J of each node from its element's map
+    const double* emass;                // [K][2]: the element's node masses summed, all and on its boundary
     const uint32_t* links;
     const uint8_t* affine;
     int64_t n_elements;
@@ -1246,12 +1260,13 @@ struct DtParams {
     unsigned int xcd_stride;
 };
 
+// node_reference_rate (swe_2d.rs:1346-1367) with the metric's norms |grad r|, |grad s| given
+// (nr, ns: sqrt(rx^2 + ry^2), sqrt(sx^2 + sy^2) of the same metric, formed once per node)
 __device__ __forceinline__ double node_rate(double h, double hu, double hv, double rx, double ry, double sx, double sy,
-                                            double g, double h_min) {
+                                            double nr, double ns, double g, double h_min) {
     double u = 0.0, v = 0.0;
     if (h > h_min) { double inv = 1.0 / h; u = hu * inv; v = hv * inv; }
-    double nr = sqrt(rx * rx + ry * ry), ns = sqrt(sx * sx + sy * sy);
-    double cc = sqrt(g * fmax_(h, 0.0));
+    double cc = sqrt_rn(g * fmax_(h, 0.0));
     double lr = fabs(u * rx + v * ry) + cc * nr;
     double ls = fabs(u * sx + v * sy) + cc * ns;
     return 0.25 * (lr + ls);
@@ -1262,53 +1277,63 @@ __device__ __forceinline__ double dt_block(const DtParams& p, int64_t blk) {
     constexpr int N1 = N + 1, NN = N1 * N1, EPB = 64 / NN, NF = 4 * N1;
     constexpr int NT = (NF + NN - 1) / NN;
     const GllTables& T = c_gll[N];
-    __shared__ double mrx[64], mry[64], msx[64], msy[64], sr[64], smass[64], swater[64], shh[64];
+    __shared__ double mrx[64], mry[64], msx[64], msy[64], mnr[64], mns[64], sr[64], swater[64];
     __shared__ double sdt[EPB];
+    __shared__ int sshallow[EPB];
     const int t = threadIdx.x, e = t / NN, i = t - e * NN, eb = e * NN;
     const int64_t k = blk * EPB + e;
     const bool active = e < EPB && k < p.n_elements;
-    const size_t n = (size_t)k * NN + i, nt = (size_t)p.n_elements * NN;
-    // every load first: links, this node, its cached metric and J, the
+    const int a = i % N1, j = i / N1;
+    const uint32_t k32 = (uint32_t)k, nb = ((uint32_t)k * NN + i) * 8u;  // the node's byte offset in a state array
+    // every load first: links, this node, the element's corners, the
     // neighbours' face nodes
     uint32_t lk[NT];
     double nh[NT], nhu[NT], nhv[NT];
     double h = 0.0, hu = 0.0, hv = 0.0, rx = 0.0, ry = 0.0, sx = 0.0, sy = 0.0, det = 0.0;
+    Quad c = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     bool affine = false;
     if (active) {
 #pragma unroll
         for (int r = 0; r < NT; ++r) {
             int task = i + r * NN;
-            lk[r] = task < NF ? p.links[4 * k + task / N1] : LINK_BOUNDARY;
+            lk[r] = task < NF ? ld32(p.links, (4u * k32 + (uint32_t)(task / N1)) * 4u) : LINK_BOUNDARY;
         }
-        h = p.h[n]; hu = p.hu[n]; hv = p.hv[n];
-        rx = p.geo[GEO_TRX * nt + n]; ry = p.geo[GEO_TRY * nt + n];
-        sx = p.geo[GEO_TSX * nt + n]; sy = p.geo[GEO_TSY * nt + n];
-        det = p.geo[GEO_DET * nt + n];
-        affine = p.affine[k] != 0;
+        h = ld32(p.h, nb); hu = ld32(p.hu, nb); hv = ld32(p.hv, nb);
+        c = load_quad32(p.corners, k32);
+        affine = ld32(p.affine, k32) != 0;
 #pragma unroll
         for (int r = 0; r < NT; ++r) {
             int task = i + r * NN;
             nh[r] = nhu[r] = nhv[r] = 0.0;
             if (task < NF && lk[r] != LINK_BOUNDARY) {
                 int fi = task - (task / N1) * N1;
-                size_t m = (size_t)(lk[r] >> 4) * NN + face_node<N>((int)(lk[r] & 3u), N1 - 1 - fi);
-                nh[r] = p.h[m]; nhu[r] = p.hu[m]; nhv[r] = p.hv[m];
+                const uint32_t mb = ((lk[r] >> 4) * NN + (uint32_t)face_node<N>((int)(lk[r] & 3u), N1 - 1 - fi)) * 8u;
+                nh[r] = ld32(p.h, mb); nhu[r] = ld32(p.hu, mb); nhv[r] = ld32(p.hv, mb);
             }
         }
     }
     if (active) {
+        // grad r, grad s and J of the node's own map, GeometricFactors2D::compute's quotients
+        // (geometric.rs:150-175): y_s / J, -x_s / J, -y_r / J, x_r / J, each the IEEE quotient
+        const Metric m = bilinear(c, T.x1[a], T.x1[j]);
+        const double y = 1.0 / m.det;
+        det = m.det;
+        rx = div_by(m.y_s, m.det, y); ry = div_by(-m.x_s, m.det, y);
+        sx = div_by(-m.y_r, m.det, y); sy = div_by(m.x_r, m.det, y);
         mrx[t] = rx; mry[t] = ry; msx[t] = sx; msy[t] = sy;
-        smass[t] = T.w2[i] * det;
-        swater[t] = smass[t] * fmax_(h, 0.0);
-        shh[t] = h;
+        // the metric's norms, once per node (the node's own rate and a neighbour's face node read them)
+        mnr[t] = sqrt_rn(rx * rx + ry * ry);
+        mns[t] = sqrt_rn(sx * sx + sy * sy);
+        swater[t] = (T.w2[i] * det) * fmax_(h, 0.0);
     }
+    if (i == 0 && e < EPB) sshallow[e] = 0;
     lds_sync();
     double rate = 0.0;
     if (active) {
         // an affine element's own nodes take its node-0 metric (the map's constant one)
         if (h > p.h_min) {
             const int l0 = affine ? eb : t;
-            rate = node_rate(h, hu, hv, mrx[l0], mry[l0], msx[l0], msy[l0], p.g, p.h_min);
+            rate = node_rate(h, hu, hv, mrx[l0], mry[l0], msx[l0], msy[l0], mnr[l0], mns[l0], p.g, p.h_min);
         }
 #pragma unroll
         for (int r = 0; r < NT; ++r) {
@@ -1316,29 +1341,37 @@ __device__ __forceinline__ double dt_block(const DtParams& p, int64_t blk) {
             if (task >= NF || lk[r] == LINK_BOUNDARY || nh[r] <= p.h_min) continue;
             int f = task / N1, fi = task - f * N1;
             int li = eb + face_node<N>(f, fi);
-            rate = fmax_(rate, node_rate(nh[r], nhu[r], nhv[r], mrx[li], mry[li], msx[li], msy[li], p.g, p.h_min));
+            rate = fmax_(rate, node_rate(nh[r], nhu[r], nhv[r], mrx[li], mry[li], msx[li], msy[li], mnr[li], mns[li],
+                                         p.g, p.h_min));
         }
         sr[t] = rate;
+        // a node at or below the shallow depth (an order-free "any")
+        if (h <= 100.0 * p.h_dry) sshallow[e] = 1;
     }
     lds_sync();
+    // the element's largest rate: a tree over its lanes (a maximum does not depend on the order)
+    constexpr int P = NN <= 4 ? 4 : NN <= 16 ? 16 : 32;  // a power of two >= NN
+#pragma unroll
+    for (int w = P / 2; w > 0; w >>= 1) {
+        if (active && i < w && i + w < NN) sr[t] = fmax_(sr[t], sr[t + w]);
+        lds_sync();
+    }
     if (active && i == 0) {
-        double rmax = 0.0;
-#pragma unroll 4
-        for (int q = 0; q < NN; ++q) rmax = fmax_(rmax, sr[eb + q]);
+        const double rmax = sr[t];
         double cfl = p.cfl;
         if (p.positivity) {
-            double water = 0.0, water_b = 0.0, weight = 0.0, weight_b = 0.0;
-            bool shallow = false;
-            const double shallow_depth = 100.0 * p.h_dry;
-#pragma unroll 4
+            // the water over the element and over its boundary nodes, summed in node order; the
+            // masses' sums are static (geometry_kernel)
+            double water = 0.0, water_b = 0.0;
+#pragma unroll
             for (int q = 0; q < NN; ++q) {
-                int qa = q % N1, qj = q / N1;
-                double mass = smass[eb + q], wi = swater[eb + q];
-                shallow = shallow || (shh[eb + q] <= shallow_depth);
+                const int qa = q % N1, qj = q / N1;
+                const double wi = swater[eb + q];
                 water += wi;
-                weight += mass;
-                if (qa == 0 || qa == N1 - 1 || qj == 0 || qj == N1 - 1) { water_b += wi; weight_b += mass; }
+                if (qa == 0 || qa == N1 - 1 || qj == 0 || qj == N1 - 1) water_b += wi;
             }
+            const bool shallow = sshallow[e] != 0;
+            const double weight = ld32(p.emass, k32 * 16u), weight_b = ld32(p.emass, k32 * 16u + 8u);
             double ratio = fmin_(water / water_b, weight / weight_b);
             double rho = fmax_(0.9 * ratio, 1.0);
             const double cap_sub = p.max_cfl_sub, cap_lin = p.max_cfl;

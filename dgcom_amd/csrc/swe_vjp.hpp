@@ -140,7 +140,7 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
                 if (friction) {
                     if (wet_dry) desing_velocity(hin, huin, hvin, p.h_dry, uf, vf);
                     else if (hin > p.h_min) { double inv = 1.0 / hin; uf = huin * inv; vf = hvin * inv; }
-                    speed = sqrt(uf * uf + vf * vf);
+                    speed = sqrt_rn(uf * uf + vf * vf);
                     if (speed > 0.0) {
                         fric = true;
                         mn = pre.mn;

@@ -367,9 +367,10 @@ def gate_parent(projection) -> ParentModel:
         for y in ys:
             for x in xs:
                 phase = omega * t - kappa * x - 0.15 * y / 4000.0
-                zeta[it, q] = 0.9 * math.cos(phase) + 0.2 * math.cos(2.0 * phase + 0.4) + 0.04
+                sp, cp = T.sincos(phase)
+                zeta[it, q] = 0.9 * cp + 0.2 * math.cos(2.0 * phase + 0.4) + 0.04
                 east[it, q] = 0.32 * math.cos(phase + 0.1) + 0.03
-                north[it, q] = 0.05 * math.sin(phase) - 0.02 * x / 6000.0
+                north[it, q] = 0.05 * sp - 0.02 * x / 6000.0
                 q += 1
     depth = np.array([36.0 + 0.0006 * x for _ in ys for x in xs])
     return ParentModel(GeoGrid(lon, lat), np.array(times), zeta, east, north, depth)

@@ -29,8 +29,21 @@ _libm.hypot.restype = ctypes.c_double
 _libm.hypot.argtypes = [ctypes.c_double, ctypes.c_double]
 
 
+_libm.sincos.restype = None
+_libm.sincos.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+
+
 def hypot(x: float, y: float) -> float:
     return _libm.hypot(x, y)
+
+
+def sincos(x: float):
+    """(sin x, cos x) by the C library's sincos. The reference's `sin_cos()`, and a sine and
+    cosine of one argument in one block, compile to this one call (LLVM merges the pair), and
+    glibc's sincos differs from its separate sin and cos by an ulp for some arguments."""
+    sn, cs = ctypes.c_double(), ctypes.c_double()
+    _libm.sincos(x, ctypes.byref(sn), ctypes.byref(cs))
+    return sn.value, cs.value
 
 
 def rem_euclid(x: float, m: float) -> float:
@@ -130,8 +143,9 @@ class Doodson:
 
 def nodal_m2(n_rad: float):
     """Schureman's f, u (degrees) of the M2 family (astronomy.rs:471-475)."""
-    f = 1.0004 - 0.0373 * math.cos(n_rad) + 0.0002 * math.cos(2.0 * n_rad)
-    u = -2.14 * math.sin(n_rad)
+    sn, cs = sincos(n_rad)
+    f = 1.0004 - 0.0373 * cs + 0.0002 * sincos(2.0 * n_rad)[1]
+    u = -2.14 * sn
     return f, u
 
 
@@ -159,7 +173,8 @@ def nodal_l2_relative(n_rad: float, p_rad: float):
     tq = math.tan(0.5 * incl)
     k = 6.0 * (tq * tq)
     two_p = 2.0 * (p_rad - xi)
-    re, im = 1.0 - k * math.cos(two_p), -k * math.sin(two_p)
+    sn, cs = sincos(two_p)
+    re, im = 1.0 - k * cs, -k * sn
     return hypot(re, im), to_degrees(math.atan2(im, re))
 
 
@@ -172,8 +187,7 @@ def _powi(x: float, k: int) -> float:
 
 def nodal_factors(kind, n_rad: float, p_rad: float):
     """f and u (degrees) of a nodal family (astronomy.rs:520-566)."""
-    c, c2, c3 = math.cos(n_rad), math.cos(2.0 * n_rad), math.cos(3.0 * n_rad)
-    s, s2, s3 = math.sin(n_rad), math.sin(2.0 * n_rad), math.sin(3.0 * n_rad)
+    (s, c), (s2, c2), (s3, c3) = sincos(n_rad), sincos(2.0 * n_rad), sincos(3.0 * n_rad)
     if kind is None:
         return 1.0, 0.0
     if kind == "M2":
@@ -358,8 +372,8 @@ class AdmittanceInference:
 
         def complex_(hg):
             h, g = hg
-            g = to_radians(g)
-            return h * math.cos(g), -h * math.sin(g)
+            sn, cs = sincos(to_radians(g))
+            return h * cs, -h * sn
 
         a, b = complex_(lower), complex_(upper)
         re, im = wa * a[0] + wb * b[0], wa * a[1] + wb * b[1]

@@ -104,7 +104,7 @@ int dg_set_tide_nodal(dg_ctx* ctx, const double* f0, const double* f_rate, const
                       const double* u0_deg, const double* u_rate_deg);
 
 /* The open boundary's tides per boundary face node (BoundaryTides, src/boundary/tidal_atlas.rs:858-1140; the
- * reference device's moment-and-sums form, notes/gpu.md:191): n_c (<= 8) constituents with the angular speed
+ * reference device's moment-and-sums form, notes/gpu.md:191): n_c (<= 21) constituents with the angular speed
  * omega' (rad/s, the nodal phase drift included) and the nodal factor's drift f_rate (1/s); per boundary face
  * node [K][4][N+1] its slot (UINT32_MAX: the uniform tide of dg_set_tide); per slot coef[n_slots][n_c][6]
  * (C, S of eta, u, v), mean[n_slots] (added unramped; zeros for none) and source_depth[n_slots]; the ramp (s,
@@ -212,7 +212,10 @@ int dg_begin_coarse_step(dg_ctx* ctx, const uint8_t* level, const uint8_t* rates
  * every element with rates[plan->stage][k] >= plan->depth, its RHS from `input` at
  * its own stage time, out = start*base + input*input + c*F, the damping over c,
  * then either acc <- w*out + keep*acc (plan->last; `out` is then not written) or
- * the wet/dry correction into out. Asynchronous on the context's stream. */
+ * the wet/dry correction into out. Asynchronous on the context's stream. With a
+ * forcing per node (boundary tides, nesting, weather) its per-level moments go up
+ * through a ring of pinned host slots, so such a stage refuses to be captured into
+ * a graph (an error, by name, when the stream is capturing). */
 int dg_stage_where(dg_ctx* ctx, const dg_state* base, const dg_state* input, const dg_stage_plan* plan,
                    dg_state* out, dg_state* acc);
 

@@ -313,8 +313,10 @@ struct Solver {
     void atlas_state(uint32_t slot, double time, double bed, double& eta, double& u, double& v) const {
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
         for (int j = 0; j < atlas.n_c; ++j) {
-            double arg = atlas.omega[j] * time;
-            double f = 1.0 + atlas.f_rate[j] * time, cs = std::cos(arg), sn = std::sin(arg);
+            // (omega t).sin_cos() (tidal_atlas.rs:966-970): one call of the C library's sincos,
+            // as the reference's compiler makes of the pair
+            double f = 1.0 + atlas.f_rate[j] * time, cs, sn;
+            sincos(atlas.omega[j] * time, &sn, &cs);
             const double* c = atlas.coef.data() + ((size_t)slot * atlas.n_c + j) * 6;
             s0 += f * (c[0] * cs + c[1] * sn);
             s1 += f * (c[2] * cs + c[3] * sn);

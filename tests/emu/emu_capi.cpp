@@ -15,16 +15,18 @@ void emu_init(void) {
 }
 
 // The static geometry cache (geometry_kernel) of a form
+static std::vector<double> g_emass;  // the positivity bound's static mass sums of the last form
 static void run_geometry(int order, int64_t K, StageParams& p, std::vector<double>& geo, std::vector<double>& fgeo) {
+    g_emass.assign((size_t)2 * K, 0.0);
     size_t nn = (size_t)(order + 1) * (order + 1);
     geo.assign((size_t)GEO_COUNT * K * nn, 0.0);
     fgeo.assign((size_t)(3 * 4 + 4 * (order + 1)) * K, 0.0);
     int epb = 64 / (int)nn;
     unsigned grid = (unsigned)((K + epb - 1) / epb);
     switch (order) {
-        case 1: emu_launch(grid, 64, [&] { geometry_kernel<1>(p.corners, p.affine, p.links, K, geo.data(), fgeo.data()); }); break;
-        case 2: emu_launch(grid, 64, [&] { geometry_kernel<2>(p.corners, p.affine, p.links, K, geo.data(), fgeo.data()); }); break;
-        default: emu_launch(grid, 64, [&] { geometry_kernel<3>(p.corners, p.affine, p.links, K, geo.data(), fgeo.data()); }); break;
+        case 1: emu_launch(grid, 64, [&] { geometry_kernel<1>(p.corners, p.affine, p.links, K, geo.data(), fgeo.data(), g_emass.data()); }); break;
+        case 2: emu_launch(grid, 64, [&] { geometry_kernel<2>(p.corners, p.affine, p.links, K, geo.data(), fgeo.data(), g_emass.data()); }); break;
+        default: emu_launch(grid, 64, [&] { geometry_kernel<3>(p.corners, p.affine, p.links, K, geo.data(), fgeo.data(), g_emass.data()); }); break;
     }
     p.geo = geo.data();
     p.fgeo = fgeo.data();
@@ -261,7 +263,7 @@ double emu_dt(int order, int64_t K, const double* state, const double* corners, 
     size_t nn = (size_t)(order + 1) * (order + 1), nt = (size_t)K * nn;
     int epb = 64 / (int)nn;
     unsigned nblk = (unsigned)((K + epb - 1) / epb);
-    // the geometry cache of the form (the time step reads its metric and J)
+    // the form's static geometry (the time step reads the mass sums)
     StageParams sp{};
     std::memset(&sp, 0, sizeof(sp));
     std::vector<double> bed(nt, 0.0), geo, fgeo;
@@ -270,7 +272,7 @@ double emu_dt(int order, int64_t K, const double* state, const double* corners, 
     std::vector<unsigned long long> slots(DT_SLOTS, DT_SLOT_EMPTY);
     DtParams p;
     p.h = state; p.hu = state + nt; p.hv = state + 2 * nt;
-    p.geo = geo.data(); p.links = links; p.affine = affine; p.n_elements = K;
+    p.corners = corners; p.emass = g_emass.data(); p.links = links; p.affine = affine; p.n_elements = K;
     p.cfl = cfl; p.g = g; p.h_min = h_min; p.h_dry = h_dry; p.pcfl = pcfl; p.max_cfl = max_cfl;
     p.max_cfl_sub = max_cfl_sub; p.positivity = positivity; p.element_dt = element_dt; p.slots = slots.data();
     p.xcd_stride = (nblk + 7) / 8;
