@@ -136,6 +136,79 @@ unsigned long long emu_stage_where(int order, int64_t K, const double* corners, 
     return clips;
 }
 
+// The same pass with the forcing given per node compiled in (the kernels' FORCE
+// instantiations): tides per boundary face node (slot [K][4][N+1], per slot the
+// constants [n_c][6], mean level and source depth) with each level's moment row
+// (rows [DG_MAX_LEVELS + 1][LVL_MOMENT]), Manning n per node and Coriolis f per node.
+unsigned long long emu_stage_where_forced(int order, int64_t K, const double* corners, const uint32_t* links,
+                                          const uint8_t* tags, const uint8_t* affine, const double* bed,
+                                          const double* manning, const double* base, const double* input, double* out,
+                                          double* acc, const uint32_t* elist, int64_t n_list, const uint8_t* level,
+                                          const uint8_t* lrate, int depth, double a, double b, const double* lvl_c,
+                                          const double* lvl_eta, double g, double h_min, double h_dry, double u_max,
+                                          double tau, double f0, int flags, const uint32_t* tide_slot,
+                                          const double* tide_coef, const double* tide_mean, const double* tide_src,
+                                          int atlas_nc, int atlas_velocity, double atlas_max_ratio,
+                                          const double* lvl_moment, const double* manning_node,
+                                          const double* coriolis_node) {
+    size_t nn = (size_t)(order + 1) * (order + 1), nt = (size_t)K * nn;
+    unsigned long long clips = 0;
+    StageParams p{};
+    std::memset(&p, 0, sizeof(p));
+    p.bh = base; p.bhu = base + nt; p.bhv = base + 2 * nt;
+    p.ih = input; p.ihu = input + nt; p.ihv = input + 2 * nt;
+    p.oh = out; p.ohu = out + nt; p.ohv = out + 2 * nt;
+    p.bed = bed; p.corners = corners; p.manning = manning; p.links = links; p.tags = tags; p.affine = affine;
+    p.n_elements = K; p.a = a; p.b = b;
+    p.g = g; p.h_min = h_min; p.h_dry = h_dry; p.u_max = u_max; p.tau = tau; p.f0 = f0; p.sqrt_g = std::sqrt(g);
+    p.flags = flags; p.clips = &clips;
+    p.elist = elist; p.n_list = n_list; p.level = level; p.lrate = lrate; p.depth = depth;
+    if (acc) { p.ah = acc; p.ahu = acc + nt; p.ahv = acc + 2 * nt; }
+    for (int l = 0; l <= DG_MAX_LEVELS; ++l) { p.lvl_c[l] = lvl_c[l]; p.lvl_eta[l] = lvl_eta[l]; }
+    p.tide_slot = tide_slot; p.tide_coef = tide_coef; p.tide_mean = tide_mean; p.tide_src = tide_src;
+    p.atlas_nc = atlas_nc; p.atlas_velocity = atlas_velocity; p.atlas_max_ratio = atlas_max_ratio;
+    p.lvl_moment = lvl_moment; p.manning_node = manning_node; p.coriolis_node = coriolis_node;
+    std::vector<double> geo, fgeo;
+    run_geometry(order, K, p, geo, fgeo);
+    int epb = 64 / (int)nn;
+    unsigned nblk = (unsigned)((n_list + epb - 1) / epb);
+    unsigned nchunk = (nblk + 1023) / 1024;
+    std::vector<uint32_t> queue((size_t)nblk * epb), list((size_t)K);
+    std::vector<uint8_t> qcount(nblk);
+    std::vector<unsigned int> ctot(nchunk);
+    unsigned int lcount = 0;
+    p.queue = queue.data();
+    p.queue_count = qcount.data();
+    p.xcd_stride = (nblk + 7) / 8;
+    unsigned grid = 8 * p.xcd_stride, sub_grid = std::min(nblk, 5u);
+    switch (order) {
+        case 1: emu_launch(grid, 64, [&] { wet_lts_kernel<1, true>(p); }); break;
+        case 2: emu_launch(grid, 64, [&] { wet_lts_kernel<2, true>(p); }); break;
+        default: emu_launch(grid, 64, [&] { wet_lts_kernel<3, true>(p); }); break;
+    }
+    emu_launch(nchunk, 256, [&] { chunk_count_kernel(qcount.data(), nblk, ctot.data()); });
+    emu_launch(nchunk, 256, [&] { compact_kernel(qcount.data(), queue.data(), nblk, ctot.data(), epb, list.data(), &lcount); });
+    switch (order) {
+        case 1: emu_launch(sub_grid, 64, [&] { subcell_stage_kernel<1, true, true>(p, list.data(), &lcount); }); break;
+        case 2: emu_launch(sub_grid, 64, [&] { subcell_stage_kernel<2, true, true>(p, list.data(), &lcount); }); break;
+        default: emu_launch(sub_grid, 64, [&] { subcell_stage_kernel<3, true, true>(p, list.data(), &lcount); }); break;
+    }
+    return clips;
+}
+
+// f0 + beta (y - y0) at every node (coriolis_node_kernel)
+void emu_coriolis_nodes(int order, int64_t K, const double* corners, double f0, double beta, double y0, double* f) {
+    size_t nt = (size_t)K * (order + 1) * (order + 1);
+    unsigned grid = (unsigned)((nt + 255) / 256);
+    switch (order) {
+        case 1: emu_launch(grid, 256, [&] { coriolis_node_kernel<1>(corners, K, f0, beta, y0, f); }); break;
+        case 2: emu_launch(grid, 256, [&] { coriolis_node_kernel<2>(corners, K, f0, beta, y0, f); }); break;
+        default: emu_launch(grid, 256, [&] { coriolis_node_kernel<3>(corners, K, f0, beta, y0, f); }); break;
+    }
+}
+
+int emu_lvl_moment(void) { return LVL_MOMENT; }
+
 unsigned long long emu_finish_where(int order, int64_t K, const double* corners, const uint8_t* affine,
                                     double* state, const double* acc, const uint32_t* elist, int64_t n_list,
                                     double h_dry, double u_max, int wet_dry) {

@@ -32,6 +32,16 @@ def lib():
         L.emu_stage_where.argtypes = ([ctypes.c_int, ctypes.c_int64, _dp, vp, vp, vp, _dp, _dp, _dp, _dp, _dp, _dp,
                                        vp, ctypes.c_int64, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                        _dp, _dp] + [ctypes.c_double] * 6 + [ctypes.c_int])
+        L.emu_stage_where_forced.restype = ctypes.c_ulonglong
+        L.emu_stage_where_forced.argtypes = ([ctypes.c_int, ctypes.c_int64, _dp, vp, vp, vp, _dp, _dp, _dp, _dp, _dp,
+                                              _dp, vp, ctypes.c_int64, vp, vp, ctypes.c_int, ctypes.c_double,
+                                              ctypes.c_double, _dp, _dp] + [ctypes.c_double] * 6 +
+                                             [ctypes.c_int, vp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_double, _dp, _dp, _dp])
+        L.emu_coriolis_nodes.restype = None
+        L.emu_coriolis_nodes.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, _dp]
+        L.emu_lvl_moment.restype = ctypes.c_int
         L.emu_finish_where.restype = ctypes.c_ulonglong
         L.emu_finish_where.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, vp, _dp, _dp, vp, ctypes.c_int64,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_int]
@@ -138,6 +148,66 @@ class EmulatedKernels:
         if acc is not None:
             for dst, src in zip(acc, np.split(A, 3)):
                 dst[...] = src
+        return clips
+
+    def coriolis_nodes(self, beta, y0):
+        """f0 + beta (y - y0) at every node, by the library's coriolis_node_kernel."""
+        f = np.empty(self.bed.size)
+        lib().emu_coriolis_nodes(self.order, self.K, self._p(self.corners), self.case.f0, beta, y0, self._p(f))
+        return f
+
+    def stage_where_forced(self, base, inp, out, acc, elements, plan, bt, manning_node, coriolis_node):
+        """stage_where through the kernels' forced instantiations: tides per boundary
+        face node `bt` (a cases.BoundaryTides) with each level's moment row evaluated here
+        as the library's host does (the ramp, and f(t)/f0 and the C library's sincos of
+        omega' t per constituent, at the level's stage time), Manning n and Coriolis f per node."""
+        from dgcom_amd import multirate as M, tides as T
+        c_ = self.case
+        B = np.concatenate(base)
+        I = np.concatenate(inp)
+        O = np.concatenate(out)
+        A = np.concatenate(acc) if acc is not None else None
+        ROW = lib().emu_lvl_moment()
+        lc = np.zeros(M_LEVELS + 1)
+        le = np.zeros(M_LEVELS + 1)
+        rows = np.zeros((M_LEVELS + 1, ROW))
+        for l in range(M_LEVELS + 1):
+            lv = min(l, plan.finest)
+            es = M.element_stage((plan.t, plan.dt), plan.finest, plan.substep, plan.depth, plan.weights, False, lv, lv)
+            lc[l] = es.c
+            le[l] = c_.tide.elevation(es.time)
+            time = es.time
+            if bt.ramp > 0.0:
+                tau = min(max(time / bt.ramp, 0.0), 1.0)
+                rows[l, 0] = tau * tau * (3.0 - 2.0 * tau)
+            else:
+                rows[l, 0] = 1.0
+            for j in range(bt.omega.size):
+                sn, cs = T.sincos(bt.omega[j] * time)
+                rows[l, 1 + 3 * j] = 1.0 + bt.f_rate[j] * time
+                rows[l, 2 + 3 * j] = cs
+                rows[l, 3 + 3 * j] = sn
+        el = np.ascontiguousarray(elements, dtype=np.uint32)
+        level = np.ascontiguousarray(plan.level, dtype=np.uint8)
+        lrate = np.ascontiguousarray(plan.rate, dtype=np.uint8)
+        slot = np.ascontiguousarray(bt.slot, dtype=np.uint32).ravel()
+        coef, mean, src = (np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (bt.coef, bt.mean,
+                                                                                         bt.source_depth))
+        mn = np.ascontiguousarray(manning_node, dtype=np.float64)
+        cf = np.ascontiguousarray(coriolis_node, dtype=np.float64)
+        w = plan.weights
+        clips = lib().emu_stage_where_forced(
+            self.order, self.K, self._p(self.corners), self.links.ctypes.data, self.tags.ctypes.data,
+            self.affine.ctypes.data, self._p(self.bed), self._p(self.manning), self._p(B), self._p(I), self._p(O),
+            None if A is None else self._p(A), el.ctypes.data, el.size, level.ctypes.data, lrate.ctypes.data,
+            plan.depth, w.start, w.input, self._p(lc), self._p(le), c_.g, c_.h_min, c_.h_dry, 20.0, 1.0, c_.f0,
+            self.flags, slot.ctypes.data, self._p(coef), self._p(mean), self._p(src), int(bt.omega.size),
+            int(bt.velocity), bt.max_ratio, self._p(rows), self._p(mn), self._p(cf))
+        for dst, src_ in zip(out, np.split(O, 3)):
+            dst[...] = src_
+        if acc is not None:
+            for dst, src_ in zip(acc, np.split(A, 3)):
+                dst[...] = src_
         return clips
 
     def finish_where(self, state, acc, elements):

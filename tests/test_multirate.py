@@ -245,6 +245,70 @@ def test_emulated_lts_passes_equal_the_oracle(scheme):
     assert n_sub > 0, "no subcell element: the subcell LTS kernel went untested"
 
 
+def test_emulated_forced_lts_passes_equal_the_oracle():
+    """The forced instantiations of the LTS kernels (wet_lts_kernel<N, true>, the
+    subcell kernel's), emulated, under tides per boundary face node with a velocity
+    and transport scaling (11 constituents: each level's moment row at its own stage
+    time), Manning n per node and a beta-plane: every pass of a coarse step with
+    levels 0-3 on the drying beach equals the restatement bit for bit."""
+    from emu_bridge import EmulatedKernels
+    from dgcom_amd import tides as T
+    c = cases.drying_beach(nx=12, ny=4)
+    rng = np.random.default_rng(11)
+    omega = 2.0 * np.pi / np.array([T.constituent_period(n) for n in ("M2", "S2", "N2", "K1", "O1", "2N2", "MU2",
+                                                                      "NU2", "LDA2", "L2", "T2")])
+    a = rng.uniform(0.01, 0.2, size=(omega.size, 6))
+    bt = cases.boundary_tides(c, omega, rng.uniform(-1e-9, 1e-9, omega.size),
+                              lambda x, y: a * (1.0 + 0.2 * np.sin(y / 120.0)), ramp=600.0, velocity=True,
+                              max_ratio=1.5, mean=lambda x, y: 0.02 * np.cos(y / 150.0),
+                              source_depth=lambda x, y: 4.0 + 0.002 * y)
+    mn = np.repeat(c.manning, c.nn) * rng.uniform(0.6, 1.4, c.nn * c.n_elements)
+    orc = O.Oracle.from_case(c)
+    orc.set_boundary_tides(bt)
+    orc.set_manning_nodal(mn)
+    orc.set_beta(2e-8, 300.0)
+    emu = EmulatedKernels(c)
+    cf = emu.coriolis_nodes(2e-8, 300.0)
+    x = [c.h.copy(), c.hu.copy(), c.hv.copy()]
+    orc.run(*x, 0.0, 40, c.cfl)
+    t0 = 40 * 0.8
+    nb = M.neighbours(c.mesh.links)
+    edt = 2.0 ** -np.random.default_rng(3).integers(0, 4, size=c.n_elements).astype(float)
+    st = M.MultirateStepper(8, 1)
+    st.assign(edt, nb)
+    assert st.finest == 3
+    ns = len(st.stages)
+    ov = [orc.clone_state(x) for _ in range(ns - 1)]
+    ev = [orc.clone_state(x) for _ in range(ns - 1)]
+    olast, elast = orc.clone_state(x), orc.clone_state(x)
+    oacc, eacc = orc.clone_state(x), orc.clone_state(x)
+    ostate, estate = orc.clone_state(x), orc.clone_state(x)
+    orc.begin_coarse_step(st.level, st.rate)
+    n_stage = 0
+
+    def visit(kind, s, elements, plan):
+        nonlocal n_stage
+        if kind == "stage":
+            oin = ostate if s == 0 else ov[s - 1]
+            ein = estate if s == 0 else ev[s - 1]
+            last = s + 1 == ns
+            orc.stage_where(ostate, oin, elements, plan, olast if last else ov[s], oacc if last else None)
+            emu.stage_where_forced(estate, ein, elast if last else ev[s], eacc if last else None, elements, plan,
+                                   bt, mn, cf)
+            oo, eo = (oacc, eacc) if last else (ov[s], ev[s])
+            for a_, b_ in zip(oo, eo):
+                assert np.array_equal(a_, b_), (kind, s, plan.substep)
+            n_stage += 1
+        else:
+            orc.finish_where(ostate, oacc, elements, s)
+            emu.finish_where(estate, eacc, elements)
+            for a_, b_ in zip(ostate, estate):
+                assert np.array_equal(a_, b_), (kind, s)
+
+    st.schedule(t0, 1.6, visit)
+    assert n_stage > 0
+
+
 def test_line_local_rates_cross_an_element_in_two_stages():
     # multirate.rs:1511-1523: a row of line-local elements, the last on level 1:
     # the finer rate reaches one element further every two stages
