@@ -8,8 +8,8 @@
 //   -> Coriolis, 1/J -> volume (or subcell) term and surface term -> node states.
 // Cotangents of this element's own nodes are added to the input's cotangent and
 // to the bed's, except those through the face Riemann fluxes F*: the element
-// writes the cotangent of F* per face node, face_vjp_kernel runs each Riemann
-// solve backwards (one thread per face node, which keeps the element kernels'
+// writes the cotangent of F* per face node, edge_vjp_kernel runs each Riemann
+// solve backwards (one thread per edge node, which keeps the element kernels'
 // registers low) and writes the cotangents of both sides per face node, and
 // vjp_gather_kernel adds them, with the neighbour's inner node that a subcell
 // element reconstructs from, to the nodes afterwards. No two threads write one
@@ -17,9 +17,11 @@
 // swe_kernels.hpp before stage_block.
 #pragma once
 
+#include <vector>
+
 #include "swe_adjoint.hpp"
 
-// Three wavefronts per SIMD for the face kernel (it needs a few registers more)
+// Three wavefronts per SIMD for the edge kernel (it needs a few registers more)
 #ifdef DGSWE_EMULATE
 #define DG_WAVES_PER_EU(n)
 #else
@@ -520,7 +522,7 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
     lds_sync();  // the region's next use (tcot) overlays the line cotangents
     // Faces: rhs += sJ LIFT (F(q).n - F*) at the element's face nodes, spread
     // over its lanes. The F(q).n part is this node's own; the cotangent of F*
-    // goes to the face buffer for face_vjp_kernel, which runs the Riemann
+    // goes to the face buffer for edge_vjp_kernel, which runs the Riemann
     // solve backwards for both sides of the face.
     if (active) {
 #pragma unroll
@@ -572,83 +574,126 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
     (void)bin;
 }
 
-// The Riemann solves backwards, one lane per face node, 64 / 4(N+1) elements per
-// workgroup-wavefront, from the cotangent of F* that the element kernels wrote.
-// The cotangent of the neighbour's face node (h, hu, hv, B) goes to facebuf (per
-// face node: no two lanes write one address); those of the element's own face
-// nodes are summed in LDS over the faces a node lies on (two at a corner, in
-// face order) and added to the input's and the bed's cotangents by one lane
-// per node.
+// The faces whose Riemann solves the edge kernel runs backwards, as
+// k << 3 | alone << 2 | f: every boundary face; every interior edge once, from
+// its left side (the side whose link has the left bit), where the normal its
+// right side solved with equals the left's own (the right's own reversed on a
+// shared face, equal to the left's up to the sign of a zero component on a
+// conforming edge; the left's on a periodic pair); and an edge whose two
+// normals differ from each side alone. In element order. nx, ny: the faces'
+// unit normals [K][4].
+inline std::vector<uint32_t> vjp_edge_list(const uint32_t* links, const double* nx, const double* ny, int64_t K) {
+    std::vector<uint32_t> e;
+    e.reserve((size_t)K * 2 + 16);
+    for (int64_t k = 0; k < K; ++k)
+        for (int f = 0; f < 4; ++f) {
+            const size_t fk = (size_t)k * 4 + f;
+            const uint32_t l = links[fk];
+            const uint32_t own = (uint32_t)k << 3 | (uint32_t)f;
+            if (l == LINK_BOUNDARY) { e.push_back(own); continue; }
+            const size_t fo = (size_t)(l >> 4) * 4 + (l & 3u);
+            const bool left = (l & 4u) != 0;
+            const size_t fl = left ? fk : fo, fr = left ? fo : fk;
+            const bool agree = !(links[fr] & 8u) || (-nx[fr] == nx[fl] && -ny[fr] == ny[fl]);
+            if (agree) {
+                if (left) e.push_back(own);
+            } else {
+                e.push_back(own | 4u);
+            }
+        }
+    return e;
+}
+
+// The Riemann solves backwards, one lane per node of a listed face, from the
+// cotangents of F* that the element kernels wrote per face node. The forward
+// solved an interior edge once in each of its elements, both times from the
+// left side with the same states and (for a listed edge) the same normal, and
+// took F*_a on the left and -F*_b on the right: one reverse solve with both
+// cotangents (that of F*_a from the left, minus the right's of F*_b) carries
+// the edge, and writes the cotangents of both face nodes. A face listed alone
+// (its edge's two normals differ in a bit) carries its own element's
+// cotangent only, as that element solved it; a boundary face (wall ghost, open
+// boundary) is its element's alone. The (h, hu, hv, B) cotangent of every
+// element face node goes to that node's own slot of facebuf: each slot is
+// written by one lane.
 template <int N>
-__global__ __launch_bounds__(64) DG_WAVES_PER_EU(3) void face_vjp_kernel(VjpParams P) {
-    constexpr int N1 = N + 1, NN = N1 * N1, NF = 4 * N1, EPF = 64 / NF;
+__global__ __launch_bounds__(64) DG_WAVES_PER_EU(3) void edge_vjp_kernel(VjpParams P, const uint32_t* edges,
+                                                                         uint32_t n_edges) {
+    constexpr int N1 = N + 1, NN = N1 * N1, NF = 4 * N1;
     const StageParams& p = P.s;
-    __shared__ double own[EPF * NF][4];
-    const int t = threadIdx.x;
-    const int e = t / NF;
-    const int task = t - e * NF;
-    const int64_t k = (int64_t)blockIdx.x * EPF + e;
-    const bool active = e < EPF && k < p.n_elements;
-    if (active) {
-        const size_t gid = (size_t)k * NF + task;
-        const int f = task / N1, fi = task - f * N1;
-        const size_t n = (size_t)k * NN + face_node<N>(f, fi);
-        const double h = p.ih[n], hu = p.ihu[n], hv = p.ihv[n], bd = p.bed[n];
-        const Node q = make_node(h, hu, hv, bd, p.h_min);
-        const size_t fk = (size_t)k * 4 + f, nf4 = (size_t)p.n_elements * 4;
-        const double nx = p.fgeo[fk], ny = p.fgeo[nf4 + fk];
-        const uint32_t l = p.links[4 * k + f];
-        const double* fc = P.facecot + gid * 3;
-        const Cons fsb = {fc[0], fc[1], fc[2]};
-        NodeBar qb = nodebar_zero();
-        if (l == LINK_BOUNDARY && p.tags[4 * k + f] == DG_BC_TIDE) {
-            const bool incoming = (p.flags & DG_INCOMING_WAVE) != 0;
-            const Cons qbd = characteristic(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min, incoming);
-            const Node nbn = make_node(qbd.h, qbd.hu, qbd.hv, q.b, p.h_min);
-            NodeBar nbb = nodebar_zero();
-            ec_flux_adj(nbn, nbn, nx, ny, p.g, fsb, nbb, nbb);
-            double bh = 0.0, bhu = 0.0, bhv = 0.0, bbb = 0.0;
-            make_node_adj(qbd.h, qbd.hu, qbd.hv, p.h_min, nbb, bh, bhu, bhv, bbb);
-            qb.b += bbb;
-            characteristic_adj(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min, incoming, Cons{bh, bhu, bhv}, qb.h,
-                               qb.hu, qb.hv, qb.b);
+    const size_t gid = (size_t)blockIdx.x * 64 + threadIdx.x;
+    if (gid >= (size_t)n_edges * N1) return;
+    const uint32_t ed = edges[gid / N1];
+    const int fi = (int)(gid % N1);
+    const int64_t k = (int64_t)(ed >> 3);
+    const int f = (int)(ed & 3u);
+    const bool alone = (ed & 4u) != 0;
+    const size_t n = (size_t)k * NN + face_node<N>(f, fi);
+    const double h = p.ih[n], hu = p.ihu[n], hv = p.ihv[n], bd = p.bed[n];
+    const Node q = make_node(h, hu, hv, bd, p.h_min);
+    const size_t fk = (size_t)k * 4 + f, nf4 = (size_t)p.n_elements * 4;
+    const double nx = p.fgeo[fk], ny = p.fgeo[nf4 + fk];
+    const uint32_t l = p.links[fk];
+    const double* fc = P.facecot + ((size_t)k * NF + f * N1 + fi) * 3;
+    const Cons fsb = {fc[0], fc[1], fc[2]};
+    NodeBar qb = nodebar_zero();
+    if (l == LINK_BOUNDARY && p.tags[fk] == DG_BC_TIDE) {
+        const bool incoming = (p.flags & DG_INCOMING_WAVE) != 0;
+        const Cons qbd = characteristic(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min, incoming);
+        const Node nbn = make_node(qbd.h, qbd.hu, qbd.hv, q.b, p.h_min);
+        NodeBar nbb = nodebar_zero();
+        ec_flux_adj(nbn, nbn, nx, ny, p.g, fsb, nbb, nbb);
+        double bh = 0.0, bhu = 0.0, bhv = 0.0, bbb = 0.0;
+        make_node_adj(qbd.h, qbd.hu, qbd.hv, p.h_min, nbb, bh, bhu, bhv, bbb);
+        qb.b += bbb;
+        characteristic_adj(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min, incoming, Cons{bh, bhu, bhv}, qb.h,
+                           qb.hu, qb.hv, qb.b);
+    } else {
+        const bool wall = l == LINK_BOUNDARY;
+        const bool left = wall || (l & 4u);
+        const bool both = !wall && !alone;  // an edge carried from its left side for both elements
+        // the other side: the neighbour's face node, or a wall's reflected ghost
+        size_t slot_o = 0;
+        double xh = 0.0, xhu = 0.0, xhv = 0.0, xb = 0.0;
+        Cons fso = {0.0, 0.0, 0.0};
+        double lx = nx, ly = ny;
+        if (wall) {
+            const Cons gs = wall_ghost(q.h, q.hu, q.hv, nx, ny, p.h_min);
+            xh = gs.h; xhu = gs.hu; xhv = gs.hv; xb = q.b;
         } else {
-            const bool wall = l == LINK_BOUNDARY;
-            const bool left = wall || (l & 4u);
-            double nhr = 0.0, nhur = 0.0, nhvr = 0.0, nbdr = 0.0;
-            if (!wall) {
-                const size_t m = (size_t)(l >> 4) * NN + face_node<N>((int)(l & 3u), N1 - 1 - fi);
-                nhr = p.ih[m]; nhur = p.ihu[m]; nhvr = p.ihv[m]; nbdr = p.bed[m];
+            const int fo = (int)(l & 3u), fio = N1 - 1 - fi;
+            const size_t m = (size_t)(l >> 4) * NN + face_node<N>(fo, fio);
+            xh = p.ih[m]; xhu = p.ihu[m]; xhv = p.ihv[m]; xb = p.bed[m];
+            slot_o = (size_t)(l >> 4) * NF + fo * N1 + fio;
+            if (both) {
+                const double* fco = P.facecot + slot_o * 3;
+                fso = {fco[0], fco[1], fco[2]};
             }
-            // the outside state: the neighbour's face node, or a wall's reflected
-            // ghost (formed only where a lane has a wall face); one make_node
-            Cons gs = {0.0, 0.0, 0.0};
-            double xh = nhr, xhu = nhur, xhv = nhvr, xb = nbdr;
-            if (wall) {
-                gs = wall_ghost(q.h, q.hu, q.hv, nx, ny, p.h_min);
-                xh = gs.h; xhu = gs.hu; xhv = gs.hv; xb = q.b;
-            }
-            const Node qo = make_node(xh, xhu, xhv, xb, p.h_min);
-            double lx = nx, ly = ny;
             if (!left) {
+                // the right side's normal: its own reversed, or on a periodic pair the left's own
                 lx = -nx; ly = -ny;
                 if (!(l & 8u)) {
-                    const size_t fo = (size_t)(l >> 4) * 4 + (l & 3u);
-                    lx = p.fgeo[fo]; ly = p.fgeo[nf4 + fo];
+                    const size_t fq = (size_t)(l >> 4) * 4 + (l & 3u);
+                    lx = p.fgeo[fq]; ly = p.fgeo[nf4 + fq];
                 }
             }
-            const Side sa = left ? side(q) : side(qo), sbs = left ? side(qo) : side(q);
-            // per-field selects (a conditional between two struct lvalues would put them in memory)
-            const double fl = left ? 1.0 : 0.0, fr = left ? 0.0 : -1.0;
-            const Cons fab = {fl * fsb.h, fl * fsb.hu, fl * fsb.hv};
-            const Cons fbb = {fr * fsb.h, fr * fsb.hu, fr * fsb.hv};
-            SideBar sab = sbar_zero(), sbb = sbar_zero();
-            hydro_hll_adj(sa, sbs, lx, ly, p.g, p.h_min, fab, fbb, sab, sbb);
-            const SideBar mine = {left ? sab.h : sbb.h, left ? sab.u : sbb.u, left ? sab.v : sbb.v,
-                                  left ? sab.b : sbb.b};
-            const SideBar os = {left ? sbb.h : sab.h, left ? sbb.u : sab.u, left ? sbb.v : sab.v,
-                                left ? sbb.b : sab.b};
-            nbar_add_side(qb, mine);
+        }
+        const Node qo = make_node(xh, xhu, xhv, xb, p.h_min);
+        const Side sa = left ? side(q) : side(qo), sbs = left ? side(qo) : side(q);
+        // the cotangents of F*_a and F*_b (per-field selects: a conditional
+        // between two struct lvalues would put them in memory)
+        const Cons ca = {left ? fsb.h : fso.h, left ? fsb.hu : fso.hu, left ? fsb.hv : fso.hv};
+        const Cons cb = {left ? fso.h : fsb.h, left ? fso.hu : fsb.hu, left ? fso.hv : fsb.hv};
+        const Cons fab = ca;
+        const Cons fbb = {-cb.h, -cb.hu, -cb.hv};
+        SideBar sab = sbar_zero(), sbb = sbar_zero();
+        hydro_hll_adj(sa, sbs, lx, ly, p.g, p.h_min, fab, fbb, sab, sbb);
+        const SideBar mine = {left ? sab.h : sbb.h, left ? sab.u : sbb.u, left ? sab.v : sbb.v,
+                              left ? sab.b : sbb.b};
+        const SideBar os = {left ? sbb.h : sab.h, left ? sbb.u : sab.u, left ? sbb.v : sab.v,
+                            left ? sbb.b : sab.b};
+        nbar_add_side(qb, mine);
+        if (wall || both) {
             const NodeBar ob = {os.h, 0.0, 0.0, os.u, os.v, os.b};
             double oh = 0.0, ohu = 0.0, ohv = 0.0, obb = 0.0;
             make_node_adj(xh, xhu, xhv, p.h_min, ob, oh, ohu, ohv, obb);
@@ -656,37 +701,20 @@ __global__ __launch_bounds__(64) DG_WAVES_PER_EU(3) void face_vjp_kernel(VjpPara
                 qb.b += obb;
                 wall_ghost_adj(q.h, q.hu, q.hv, nx, ny, p.h_min, Cons{oh, ohu, ohv}, qb.h, qb.hu, qb.hv);
             } else {
-                double* o = P.facebuf + gid * 4;
+                double* o = P.facebuf + slot_o * 4;
                 o[0] = oh; o[1] = ohu; o[2] = ohv; o[3] = obb;
             }
         }
-        double gh = 0.0, ghu = 0.0, ghv = 0.0, gb = 0.0;
-        make_node_adj(h, hu, hv, p.h_min, qb, gh, ghu, ghv, gb);
-        own[t][0] = gh; own[t][1] = ghu; own[t][2] = ghv; own[t][3] = gb;
     }
-    lds_sync();
-    // the element's 4N boundary nodes: node (f, fi < N) of face f; a corner
-    // (fi = 0, vertex f) also ends the face before it
-    if (active && task < 4 * N) {
-        const int f = task / N, fi = task - f * N;
-        const int tf = e * NF + f * N1 + fi;
-        double s0 = own[tf][0], s1 = own[tf][1], s2 = own[tf][2], s3 = own[tf][3];
-        if (fi == 0) {
-            const int tp = e * NF + ((f + 3) & 3) * N1 + N;  // face f-1 (face 3 for vertex 0), last node
-            if (f == 0) { s0 = s0 + own[tp][0]; s1 = s1 + own[tp][1]; s2 = s2 + own[tp][2]; s3 = s3 + own[tp][3]; }
-            else { s0 = own[tp][0] + s0; s1 = own[tp][1] + s1; s2 = own[tp][2] + s2; s3 = own[tp][3] + s3; }
-        }
-        const size_t n = (size_t)k * NN + face_node<N>(f, fi);
-        P.gih[n] += s0;
-        P.gihu[n] += s1;
-        P.gihv[n] += s2;
-        P.gbed[n] += s3;
-    }
+    double gh = 0.0, ghu = 0.0, ghv = 0.0, gb = 0.0;
+    make_node_adj(h, hu, hv, p.h_min, qb, gh, ghu, ghv, gb);
+    double* o = P.facebuf + ((size_t)k * NF + f * N1 + fi) * 4;
+    o[0] = gh; o[1] = ghu; o[2] = ghv; o[3] = gb;
 }
 
-// The neighbour parts of the input's and the bed's cotangents: every node adds,
-// face by face in face order, what the face kernel wrote for it from the
-// neighbour's side of the neighbour's face, and what a neighbour that took the
+// The face parts of the input's and the bed's cotangents: every node adds,
+// face by face in face order, its own slot of facebuf on that face (the edge
+// kernel's cotangent of this face node), and what a neighbour that took the
 // subcell volume term wrote for it as the inner node behind that neighbour's
 // face. One thread per node.
 template <int N>
@@ -713,13 +741,13 @@ __global__ __launch_bounds__(256) void vjp_gather_kernel(const uint32_t* links, 
                      : f == 1 ? (a == N1 - 2 ? j : -1)
                      : f == 2 ? (j == N1 - 2 ? N1 - 1 - a : -1)
                               : (a == 1 ? N1 - 1 - j : -1);
+        if (fi >= 0) {
+            const double* o = facebuf + ((size_t)k * NF + f * N1 + fi) * 4;
+            sh += o[0]; shu += o[1]; shv += o[2]; sb += o[3];
+        }
         if (l == LINK_BOUNDARY) continue;
         const int64_t nb = (int64_t)(l >> 4);
         const int nbf = (int)(l & 3u);
-        if (fi >= 0) {
-            const double* o = facebuf + ((size_t)nb * NF + nbf * N1 + (N1 - 1 - fi)) * 4;
-            sh += o[0]; shu += o[1]; shv += o[2]; sb += o[3];
-        }
         if (gi >= 0 && kind[nb]) {
             const double* o = innerbuf + ((size_t)nb * NF + nbf * N1 + (N1 - 1 - gi)) * 4;
             sh += o[0]; shu += o[1]; shv += o[2]; sb += o[3];

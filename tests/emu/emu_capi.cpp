@@ -315,12 +315,13 @@ void emu_stage_vjp(int order, int64_t K, const double* corners, const uint32_t* 
         case 2: emu_launch(sub_grid, 64, [&] { subcell_vjp_kernel<2>(P, list.data(), &lcount); }); break;
         default: emu_launch(sub_grid, 64, [&] { subcell_vjp_kernel<3>(P, list.data(), &lcount); }); break;
     }
-    const int epf = 64 / (int)nf;
-    unsigned fgrid = (unsigned)((K + epf - 1) / epf);
+    const std::vector<uint32_t> edges = vjp_edge_list(links, fgeo.data(), fgeo.data() + (size_t)K * 4, K);
+    const uint32_t ne = (uint32_t)edges.size();
+    unsigned fgrid = (unsigned)(((size_t)ne * (order + 1) + 63) / 64);
     switch (order) {
-        case 1: emu_launch(fgrid, 64, [&] { face_vjp_kernel<1>(P); }); break;
-        case 2: emu_launch(fgrid, 64, [&] { face_vjp_kernel<2>(P); }); break;
-        default: emu_launch(fgrid, 64, [&] { face_vjp_kernel<3>(P); }); break;
+        case 1: emu_launch(fgrid, 64, [&] { edge_vjp_kernel<1>(P, edges.data(), ne); }); break;
+        case 2: emu_launch(fgrid, 64, [&] { edge_vjp_kernel<2>(P, edges.data(), ne); }); break;
+        default: emu_launch(fgrid, 64, [&] { edge_vjp_kernel<3>(P, edges.data(), ne); }); break;
     }
     unsigned ggrid = (unsigned)((nt + 255) / 256);
     switch (order) {

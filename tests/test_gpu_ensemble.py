@@ -1,10 +1,12 @@
 """Ensembles through the C ABI: members of orders 1-3 and different meshes,
 forcings and time steps advanced together equal, bit for bit, each member's
-own run (times included), with the stages launched directly, replayed from HIP
-graphs, and after the launch-shape tuner."""
+own run (times included) and the restatement's adaptive run of that member
+(state, time and wet/dry masks), with the stages launched directly, replayed
+from HIP graphs, and after the launch-shape tuner."""
 import numpy as np
 import pytest
 
+import oracle as O
 from dgcom_amd import cases, swe2d
 from cases_for_tests import perturbed, small_unstructured
 
@@ -40,6 +42,18 @@ def test_ensemble_equals_each_members_own_run(graph, tune):
         assert times[m] == t, (m, times[m], t)
         for g, r in zip(got, x):
             assert np.array_equal(g, r), (m, np.abs(g - r).max())
+    # the restatement's own run of each member
+    for m, (c, st) in enumerate(zip(cs, states)):
+        orc = O.Oracle.from_case(c)
+        h, hu, hv = (np.array(a, dtype=np.float64) for a in st)
+        t_o = orc.run(h, hu, hv, 0.0, n_steps, c.cfl)
+        assert times[m] == t_o, (m, times[m], t_o)
+        got = ens.download(m)
+        for g, r in zip(got, (h, hu, hv)):
+            assert np.array_equal(g, r), (m, np.abs(g - r).max())
+        nd_g, es_g = phys[m].masks(phys[m].state(*got))
+        nd_o, es_o = orc.masks(h)
+        assert np.array_equal(nd_g, nd_o) and np.array_equal(es_g, es_o)
     if tune:
         for o in (1, 2, 3):
             cfg = ens.tuning(o)
