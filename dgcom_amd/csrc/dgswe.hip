@@ -483,7 +483,9 @@ static int launch_stage(dg_ctx* c, const double* base, const double* input, doub
     return 0;
 }
 
-static int launch_dt(dg_ctx* c, const double* state, double cfl, int scheme, bool element_out, double* dt) {
+// The time step's kernels (element steps to c->element_dt with element_out,
+// the minimum to c->dt_value), not waited for
+static int launch_dt_kernels(dg_ctx* c, const double* state, double cfl, int scheme, bool element_out) {
     size_t nt = (size_t)c->K * c->nn;
     DtParams p;
     p.h = state; p.hu = state + nt; p.hv = state + 2 * nt;
@@ -504,7 +506,6 @@ static int launch_dt(dg_ctx* c, const double* state, double cfl, int scheme, boo
     const int64_t epb = 64 / c->nn, nblk = (c->K + epb - 1) / epb;
     p.xcd_stride = (unsigned int)((nblk + 7) / 8);
     dim3 grid(8 * p.xcd_stride);
-    if (c->timing) DG_TRY(hipEventRecord(c->ev[0], c->stream));
     switch (c->order) {
         case 1: hipLaunchKernelGGL(dt_kernel<1>, grid, dim3(64), 0, c->stream, p); break;
         case 2: hipLaunchKernelGGL(dt_kernel<2>, grid, dim3(64), 0, c->stream, p); break;
@@ -512,6 +513,12 @@ static int launch_dt(dg_ctx* c, const double* state, double cfl, int scheme, boo
     }
     hipLaunchKernelGGL(slots_min_kernel, dim3(1), dim3(DT_SLOTS), 0, c->stream, c->dt_slots, c->dt_value);
     DG_TRY(hipGetLastError());
+    return 0;
+}
+
+static int launch_dt(dg_ctx* c, const double* state, double cfl, int scheme, bool element_out, double* dt) {
+    if (c->timing) DG_TRY(hipEventRecord(c->ev[0], c->stream));
+    if (int r = launch_dt_kernels(c, state, cfl, scheme, element_out)) return r;
     if (c->timing) DG_TRY(hipEventRecord(c->ev[1], c->stream));
     double value = 0.0;
     DG_TRY(hipMemcpyAsync(&value, c->dt_value, 8, hipMemcpyDeviceToHost, c->stream));
@@ -1503,18 +1510,24 @@ int dg_finish_where(dg_ctx* c, dg_state* state, const dg_state* acc, int depth) 
     return 0;
 }
 
+// The depth flags into c->depth_flags, not waited for
+static int launch_depth_flags(dg_ctx* c, const double* h, double d0, double d1, double d2) {
+    if (!c->depth_flags) DG_TRY(hipMalloc(&c->depth_flags, (size_t)c->K));
+    dim3 grid((unsigned)((c->K + 255) / 256));
+    switch (c->order) {
+        case 1: hipLaunchKernelGGL(depth_flags_kernel<1>, grid, dim3(256), 0, c->stream, h, c->K, d0, d1, d2, c->depth_flags); break;
+        case 2: hipLaunchKernelGGL(depth_flags_kernel<2>, grid, dim3(256), 0, c->stream, h, c->K, d0, d1, d2, c->depth_flags); break;
+        default: hipLaunchKernelGGL(depth_flags_kernel<3>, grid, dim3(256), 0, c->stream, h, c->K, d0, d1, d2, c->depth_flags); break;
+    }
+    DG_TRY(hipGetLastError());
+    return 0;
+}
+
 int dg_depth_flags(dg_ctx* c, const dg_state* s, double d0, double d1, double d2, uint8_t* out) {
     DG_LOCK(c);
     DG_REQUIRE(c && s && out && c->K > 0, "dg_depth_flags: null argument or no form");
     DG_TRY(hipSetDevice(c->device));
-    if (!c->depth_flags) DG_TRY(hipMalloc(&c->depth_flags, (size_t)c->K));
-    dim3 grid((unsigned)((c->K + 255) / 256));
-    switch (c->order) {
-        case 1: hipLaunchKernelGGL(depth_flags_kernel<1>, grid, dim3(256), 0, c->stream, s->d, c->K, d0, d1, d2, c->depth_flags); break;
-        case 2: hipLaunchKernelGGL(depth_flags_kernel<2>, grid, dim3(256), 0, c->stream, s->d, c->K, d0, d1, d2, c->depth_flags); break;
-        default: hipLaunchKernelGGL(depth_flags_kernel<3>, grid, dim3(256), 0, c->stream, s->d, c->K, d0, d1, d2, c->depth_flags); break;
-    }
-    DG_TRY(hipGetLastError());
+    if (int r = launch_depth_flags(c, s->d, d0, d1, d2)) return r;
     DG_TRY(hipMemcpyAsync(out, c->depth_flags, (size_t)c->K, hipMemcpyDeviceToHost, c->stream));
     DG_TRY(hipStreamSynchronize(c->stream));
     return 0;
@@ -1529,8 +1542,15 @@ struct dg_multirate {
     int scheme = 0;
     dgswe::multirate::Stage st[4];
     dgswe::multirate::Assigner as;
-    double* edt = nullptr;     // pinned host copies of the element time steps and depth flags
-    uint8_t* flags = nullptr;
+    // the device half of an assignment (mr_*_kernel): the extremes' slots, the
+    // plan, the levels and line-local flags, the kept assignment's, and the
+    // differs / largest-level pair; pinned host copies of the plan and the pair
+    unsigned long long* d_slots = nullptr;
+    dgswe::multirate::Plan* d_plan = nullptr;
+    uint8_t *d_level = nullptr, *d_ll = nullptr, *d_memo_given = nullptr, *d_memo_ll = nullptr;
+    int* d_out = nullptr;
+    dgswe::multirate::Plan* h_plan = nullptr;
+    int* h_out = nullptr;
     dg_state* values[3] = {nullptr, nullptr, nullptr};
     dg_state *last = nullptr, *acc = nullptr;
     uint64_t generation = ~0ull;   // the context's list generation this stepper made
@@ -1542,6 +1562,28 @@ static int mr_free_states(dg_multirate* m) {
     for (dg_state* x : all)
         if (x) { if (int r = dg_state_free(m->c, x)) return r; }
     m->values[0] = m->values[1] = m->values[2] = m->last = m->acc = nullptr;
+    return 0;
+}
+
+static int mr_free_device(dg_multirate* m) {
+    void* dev[] = {m->d_slots, m->d_plan, m->d_level, m->d_ll, m->d_memo_given, m->d_memo_ll, m->d_out};
+    for (void* p : dev)
+        if (p) DG_TRY(hipFree(p));
+    if (m->h_plan) DG_TRY(hipHostFree(m->h_plan));
+    if (m->h_out) DG_TRY(hipHostFree(m->h_out));
+    m->d_slots = nullptr; m->d_plan = nullptr; m->d_out = nullptr;
+    m->d_level = m->d_ll = m->d_memo_given = m->d_memo_ll = nullptr;
+    m->h_plan = nullptr; m->h_out = nullptr;
+    return 0;
+}
+
+static int mr_alloc(dg_multirate* m, size_t K) {
+    DG_TRY(hipMalloc(&m->d_slots, 4 * sizeof(unsigned long long)));
+    DG_TRY(hipMalloc(&m->d_plan, sizeof(dgswe::multirate::Plan)));
+    DG_TRY(hipMalloc(&m->d_out, 2 * sizeof(int)));
+    for (uint8_t** p : {&m->d_level, &m->d_ll, &m->d_memo_given, &m->d_memo_ll}) DG_TRY(hipMalloc(p, K));
+    DG_TRY(hipHostMalloc((void**)&m->h_plan, sizeof(dgswe::multirate::Plan), hipHostMallocDefault));
+    DG_TRY(hipHostMalloc((void**)&m->h_out, 2 * sizeof(int), hipHostMallocDefault));
     return 0;
 }
 
@@ -1562,12 +1604,10 @@ int dg_multirate_create(dg_ctx* c, int max_levels, int scheme, int line_rates, d
     m->as.n1 = c->order + 1;
     m->as.n_stages = dgswe::multirate::stages(scheme == DG_SSP_RK3 ? 0 : 1, m->st);
     m->as.init(links.data(), K);
-    if (hipHostMalloc((void**)&m->edt, K * sizeof(double), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&m->flags, K, hipHostMallocDefault) != hipSuccess) {
-        if (m->edt) (void)hipHostFree(m->edt);
+    if (int r = mr_alloc(m, K)) {
+        (void)mr_free_device(m);
         delete m;
-        g_err = "dg_multirate_create: pinned host memory";
-        return 1;
+        return r;
     }
     *out = m;
     return 0;
@@ -1578,8 +1618,7 @@ int dg_multirate_destroy(dg_multirate* m) {
     DG_LOCK(m->c);
     DG_TRY(hipSetDevice(m->c->device));
     if (int r = mr_free_states(m)) return r;
-    DG_TRY(hipHostFree(m->edt));
-    DG_TRY(hipHostFree(m->flags));
+    if (int r = mr_free_device(m)) return r;
     delete m;
     return 0;
 }
@@ -1594,14 +1633,42 @@ int dg_multirate_assign_levels(dg_multirate* m, const dg_state* s, double cfl, d
     const size_t K = (size_t)c->K;
     DG_TRY(hipSetDevice(c->device));
     // LocalTimeStepping::element_dt on the device, and the depth flags of the
-    // coarse-step bound and the line-local elements
-    double dtmin = 0.0;
-    if (int r = launch_dt(c, s->d, cfl, m->scheme, true, &dtmin)) return r;
-    DG_TRY(hipMemcpyAsync(m->edt, c->element_dt, K * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    if (int r = dg_depth_flags(c, s, mr::SUBCELL_CAP_DEPTH_FACTOR * c->h_dry, c->h_dry, c->h_dry + mr::LINE_LOCAL_HEADROOM,
-                               m->flags)) return r;   // (synchronises the stream)
-    *dt_out = m->as.assign(m->edt, K, m->flags, (c->flags & DG_WET_DRY) != 0, dt_cap, landing);
-    if (stable_out) *stable_out = m->as.stable;
+    // coarse-step bound and the line-local elements; then the extremes, the
+    // plan and the levels there too, with whether they differ from the kept
+    // assignment's: only the plan and that pair come back
+    if (int r = launch_dt_kernels(c, s->d, cfl, m->scheme, true)) return r;
+    if (int r = launch_depth_flags(c, s->d, mr::SUBCELL_CAP_DEPTH_FACTOR * c->h_dry, c->h_dry,
+                                   c->h_dry + mr::LINE_LOCAL_HEADROOM)) return r;
+    const unsigned rgrid = (unsigned)std::min<size_t>((K + 255) / 256, 1024);
+    const bool lines = m->as.line_rates && m->as.n1 >= 2;
+    const bool kept = m->as.memo;
+    hipLaunchKernelGGL(mr::mr_clear_kernel, dim3(1), dim3(64), 0, c->stream, m->d_slots, m->d_out);
+    hipLaunchKernelGGL(mr::mr_extremes_kernel, dim3(rgrid), dim3(256), 0, c->stream, c->element_dt, c->depth_flags,
+                       c->K, m->d_slots);
+    hipLaunchKernelGGL(mr::mr_plan_kernel, dim3(1), dim3(64), 0, c->stream, m->d_slots, (c->flags & DG_WET_DRY) ? 1 : 0,
+                       dt_cap, landing, m->as.max_levels, m->d_plan);
+    hipLaunchKernelGGL(mr::mr_levels_kernel, dim3(rgrid), dim3(256), 0, c->stream, c->element_dt, c->depth_flags, c->K,
+                       m->d_plan, lines ? 1 : 0, kept ? m->d_memo_given : nullptr, kept ? m->d_memo_ll : nullptr,
+                       m->d_level, m->d_ll, m->d_out);
+    DG_TRY(hipGetLastError());
+    DG_TRY(hipMemcpyAsync(m->h_plan, m->d_plan, sizeof(mr::Plan), hipMemcpyDeviceToHost, c->stream));
+    DG_TRY(hipMemcpyAsync(m->h_out, m->d_out, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    DG_TRY(hipStreamSynchronize(c->stream));
+    const mr::Plan p = *m->h_plan;
+    if (!m->as.device_kept(p, m->h_out[0], m->h_out[1])) {
+        m->as.level.resize(K);
+        m->as.ll.resize(K);
+        DG_TRY(hipMemcpyAsync(m->as.level.data(), m->d_level, K, hipMemcpyDeviceToHost, c->stream));
+        DG_TRY(hipMemcpyAsync(m->as.ll.data(), m->d_ll, K, hipMemcpyDeviceToHost, c->stream));
+        DG_TRY(hipStreamSynchronize(c->stream));
+        m->as.apply(m->as.finest);
+        // the kept assignment's inputs for the next comparison on the device
+        DG_TRY(hipMemcpyAsync(m->d_memo_given, m->as.memo_given.data(), K, hipMemcpyHostToDevice, c->stream));
+        DG_TRY(hipMemcpyAsync(m->d_memo_ll, m->as.memo_ll.data(), K, hipMemcpyHostToDevice, c->stream));
+        DG_TRY(hipStreamSynchronize(c->stream));
+    }
+    *dt_out = p.dt;
+    if (stable_out) *stable_out = p.stable;
     return 0;
 }
 

@@ -12,6 +12,14 @@
 #include <limits>
 #include <vector>
 
+// The coarse step's plan and an element's level are evaluated on the device
+// too (the same IEEE operations, so the same bits)
+#if defined(__HIPCC__) || defined(DGSWE_EMULATE)
+#define MR_HD __host__ __device__ inline
+#else
+#define MR_HD inline
+#endif
+
 namespace dgswe {
 namespace multirate {
 
@@ -31,43 +39,56 @@ inline int stages(int scheme, Stage* out) {
     return n;
 }
 
-// assign_levels (multirate.rs:526-568): returns the coarse step, `finest` and
-// the levels; (inf, 0) when no element limits the step
-inline double assign_levels(const std::vector<double>& edt, int max_levels, double cap, int& finest,
-                            std::vector<uint8_t>& level) {
-    const size_t K = edt.size();
-    level.assign(K, 0);
-    double lo = std::numeric_limits<double>::infinity(), hi = 0.0;
-    for (double d : edt)
-        if (std::isfinite(d)) { lo = std::fmin(lo, d); hi = std::fmax(hi, d); }
-    finest = 0;
-    if (!std::isfinite(lo)) return std::numeric_limits<double>::infinity();
+// The coarse step from the extremes of the finite element time steps: lo, hi
+// the least and the largest (lo = inf: none is finite), long1, long2 the largest
+// of an element whose depth flags have bit 0, bit 1 (0: none) for the module's
+// bound (builder.rs:614-637); assign_levels' coarsening (multirate.rs:526-568)
+// and the fit to the landing (:1014-1043). The levels are element_level(edt, dt,
+// bound) for every element; with `fitted` the finest level is their largest
+// (levels_for_step, :574-586), else `finest`.
+struct Plan {
+    double dt, stable;  // the coarse step (fitted to the landing), and the one the elements allow
+    int finest, bound;  // assign_levels' finest level; the level loop's limit
+    int fitted;
+};
+MR_HD Plan plan_coarse_step(double lo, double hi, double long1, double long2, bool wet_dry, double dt_cap,
+                            double landing, int max_levels) {
+    const double inf = __builtin_inf();
+    double bound = inf;
+    if (wet_dry) {
+        if (long1 > 0.0) bound = long1;
+        else if (long2 > 0.0) bound = long2;
+    }
+    const double cap = fmin(dt_cap > 0.0 ? dt_cap : inf, bound);
+    Plan p = {inf, inf, 0, 0, 0};
+    if (!(lo < inf)) return p;
+    int finest = 0;
     while (finest < max_levels) {
         const double coarser = lo * (double)(2ull << finest);
         if (coarser > hi || coarser > cap) break;
         ++finest;
     }
     const double dt = lo * (double)(1ull << finest);
-    for (size_t k = 0; k < K; ++k) {
-        int l = 0;
-        double dl = dt;
-        while (l < finest && dl > edt[k]) { dl *= 0.5; ++l; }
-        level[k] = (uint8_t)l;
+    p.dt = p.stable = dt;
+    p.finest = p.bound = finest;
+    if (landing > 0.0 && finest > 0 && dt < inf) {
+        const double steps = fmax(ceil(landing / (dt * (1.0 + LANDING_SLACK))), 1.0);
+        const double fitted = landing / steps;
+        if (fitted < dt) {
+            p.dt = fitted;
+            p.bound = max_levels;
+            p.fitted = 1;
+        }
     }
-    return dt;
+    return p;
 }
 
-// levels_for_step (multirate.rs:574-586)
-inline int levels_for_step(const std::vector<double>& edt, double dt, int max_levels, std::vector<uint8_t>& level) {
-    int finest = 0;
-    for (size_t k = 0; k < edt.size(); ++k) {
-        int l = 0;
-        double dl = dt;
-        while (l < max_levels && dl > edt[k]) { dl *= 0.5; ++l; }
-        level[k] = (uint8_t)l;
-        finest = finest > l ? finest : l;
-    }
-    return finest;
+// An element's level: halve dt until it is at most the element's step, at most `bound` times
+MR_HD int element_level(double edt, double dt, int bound) {
+    int l = 0;
+    double dl = dt;
+    while (l < bound && dl > edt) { dl *= 0.5; ++l; }
+    return l;
 }
 
 // spread_to_neighbours (multirate.rs:592-613): to[k] = max(from[k], from[j] - drop)
@@ -189,44 +210,69 @@ struct Assigner {
     }
     std::vector<double> edt;
 
-  private:
+    // The levels of an element step sequence under its plan (the host's own
+    // pass; the library evaluates the extremes and the levels on the device)
     double assign_kept(const uint8_t* flags, bool wet_dry, double dt_cap, double landing) {
         const size_t K = edt.size();
-        double bound = std::numeric_limits<double>::infinity();
-        if (wet_dry) {
-            for (int bit : {1, 2}) {
-                double longest = 0.0;
-                for (size_t k = 0; k < K; ++k)
-                    if (std::isfinite(edt[k]) && (flags[k] & bit)) longest = std::fmax(longest, edt[k]);
-                if (longest > 0.0) { bound = longest; break; }
+        const double inf = std::numeric_limits<double>::infinity();
+        double lo = inf, hi = 0.0, long1 = 0.0, long2 = 0.0;
+        for (size_t k = 0; k < K; ++k)
+            if (std::isfinite(edt[k])) {
+                lo = std::fmin(lo, edt[k]);
+                hi = std::fmax(hi, edt[k]);
+                if (flags[k] & 1) long1 = std::fmax(long1, edt[k]);
+                if (flags[k] & 2) long2 = std::fmax(long2, edt[k]);
             }
+        const Plan p = plan_coarse_step(lo, hi, long1, long2, wet_dry, dt_cap, landing, max_levels);
+        level.resize(K);
+        int top = 0;
+        for (size_t k = 0; k < K; ++k) {
+            const int l = element_level(edt[k], p.dt, p.bound);
+            level[k] = (uint8_t)l;
+            top = top > l ? top : l;
         }
-        const double cap = std::fmin(dt_cap > 0.0 ? dt_cap : std::numeric_limits<double>::infinity(), bound);
-        int fin = 0;
-        double dt = assign_levels(edt, max_levels, cap, fin, level);
-        stable = dt;
-        if (landing > 0.0 && fin > 0 && std::isfinite(dt)) {
-            const double steps = std::fmax(std::ceil(landing / (dt * (1.0 + LANDING_SLACK))), 1.0);
-            const double fitted = landing / steps;
-            if (fitted < dt) {
-                dt = fitted;
-                fin = levels_for_step(edt, dt, max_levels, level);
-            }
-        }
-        finest = fin;
         const bool lines = line_rates && n1 >= 2;
         ll.assign(K, 0);
         if (lines)
             for (size_t k = 0; k < K; ++k) ll[k] = (flags[k] & 4) ? 1 : 0;
-        if (memo && memo_finest == fin && memo_given == level && memo_ll == ll) {
+        const int fin = p.fitted ? top : p.finest;
+        stable = p.stable;
+        if (kept(fin, level, ll)) return p.dt;
+        apply(fin);
+        return p.dt;
+    }
+
+    // After the device half of an assignment (the plan, whether some element's
+    // level or line-local flag differs from the kept assignment's inputs, the
+    // largest level): the finest level, and whether the kept assignment holds
+    // (then `level` is its limited levels already); else the caller brings the
+    // levels and flags back into `level` and `ll` and calls apply(finest)
+    bool device_kept(const Plan& p, int differs, int top) {
+        stable = p.stable;
+        finest = p.fitted ? top : p.finest;
+        return memo && memo_finest == finest && !differs;
+    }
+
+    // The last assignment holds when its finest level and its levels before the
+    // limit and line-local flags are the same (multirate.rs:1114-1121)
+    bool kept(int fin, const std::vector<uint8_t>& given, const std::vector<uint8_t>& given_ll) {
+        finest = fin;
+        if (memo && memo_finest == fin && memo_given == given && memo_ll == given_ll) {
             level = memo_limited;
-            return dt;
+            return true;
         }
+        return false;
+    }
+    // A new assignment from `level` (before the limit) and `ll`: the level-jump
+    // limit, the stage rates, kept for the next coarse steps
+    void apply(int fin) {
+        const size_t K = level.size();
+        finest = fin;
         changed = true;
         memo_given = level;
         while (spread_to_neighbours(nb, level, scratch, 1)) level.swap(scratch);
         rates.resize((size_t)n_stages * K);
-        if (lines) {
+        if (line_rates && n1 >= 2) {
             const size_t nn = (size_t)n1 * n1;
             node_from.resize(K * nn);
             for (size_t k = 0; k < K; ++k) std::memset(node_from.data() + k * nn, level[k], nn);
@@ -249,9 +295,98 @@ struct Assigner {
         memo_finest = fin;
         memo_ll = ll;
         memo_limited = level;
-        return dt;
     }
 };
+
+#if defined(__HIPCC__) || defined(DGSWE_EMULATE)
+// The device half of an assignment: the extremes of the element time steps,
+// the plan, every element's level and line-local flag, and whether they differ
+// from the kept assignment's (so that a coarse step whose levels are the last
+// one's brings back a few bytes instead of the element time steps).
+
+// slots: the bits of lo, hi, long1, long2 (non-negative doubles order like
+// their bits); out: [0] some element differs from the kept assignment, [1]
+// the largest level
+__global__ void mr_clear_kernel(unsigned long long* slots, int* out) {
+    if (threadIdx.x == 0) {
+        slots[0] = 0x7FF0000000000000ull;  // +inf
+        slots[1] = slots[2] = slots[3] = 0ull;
+        out[0] = out[1] = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void mr_extremes_kernel(const double* edt, const uint8_t* flags, int64_t K,
+                                                          unsigned long long* slots) {
+    __shared__ double red[4][256];
+    const int t = threadIdx.x;
+    const double inf = __builtin_inf();
+    double lo = inf, hi = 0.0, l1 = 0.0, l2 = 0.0;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + t; k < K; k += (int64_t)gridDim.x * 256) {
+        const double d = edt[k];
+        if (!(d < inf)) continue;  // not finite (the steps are non-negative)
+        const uint8_t f = flags[k];
+        lo = d < lo ? d : lo;
+        hi = d > hi ? d : hi;
+        if ((f & 1) && d > l1) l1 = d;
+        if ((f & 2) && d > l2) l2 = d;
+    }
+    red[0][t] = lo; red[1][t] = hi; red[2][t] = l1; red[3][t] = l2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            red[0][t] = red[0][t + w] < red[0][t] ? red[0][t + w] : red[0][t];
+            for (int q = 1; q < 4; ++q) red[q][t] = red[q][t + w] > red[q][t] ? red[q][t + w] : red[q][t];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        atomicMin(slots + 0, (unsigned long long)__double_as_longlong(red[0][0]));
+        for (int q = 1; q < 4; ++q) atomicMax(slots + q, (unsigned long long)__double_as_longlong(red[q][0]));
+    }
+}
+
+__global__ void mr_plan_kernel(const unsigned long long* slots, int wet_dry, double dt_cap, double landing,
+                               int max_levels, Plan* plan) {
+    if (threadIdx.x != 0) return;
+    *plan = plan_coarse_step(__longlong_as_double((long long)slots[0]), __longlong_as_double((long long)slots[1]),
+                             __longlong_as_double((long long)slots[2]), __longlong_as_double((long long)slots[3]),
+                             wet_dry != 0, dt_cap, landing, max_levels);
+}
+
+// memo_given, memo_ll: the kept assignment's levels before the limit and its
+// line-local flags (null: none kept)
+__global__ __launch_bounds__(256) void mr_levels_kernel(const double* edt, const uint8_t* flags, int64_t K,
+                                                        const Plan* plan, int lines, const uint8_t* memo_given,
+                                                        const uint8_t* memo_ll, uint8_t* level, uint8_t* ll, int* out) {
+    __shared__ int red[2][256];
+    const int t = threadIdx.x;
+    const double dt = plan->dt;
+    const int bound = plan->bound;
+    int diff = memo_given ? 0 : 1, top = 0;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + t; k < K; k += (int64_t)gridDim.x * 256) {
+        const int l = element_level(edt[k], dt, bound);
+        const uint8_t b = lines && (flags[k] & 4) ? 1 : 0;
+        level[k] = (uint8_t)l;
+        ll[k] = b;
+        if (memo_given && (memo_given[k] != (uint8_t)l || memo_ll[k] != b)) diff = 1;
+        top = l > top ? l : top;
+    }
+    red[0][t] = diff;
+    red[1][t] = top;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            red[0][t] |= red[0][t + w];
+            red[1][t] = red[1][t + w] > red[1][t] ? red[1][t + w] : red[1][t];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        if (red[0][0]) atomicOr(out, 1);
+        atomicMax(out + 1, red[1][0]);
+    }
+}
+#endif
 
 // The coarsest level whose substeps start at finest substep p (multirate.rs:842-849)
 inline int depth(int p, int finest) {

@@ -331,6 +331,51 @@ void emu_stage_vjp(int order, int64_t K, const double* corners, const uint32_t* 
     }
 }
 
+// The same sequence through the library's device half (the mr_*_kernel
+// launches of dg_multirate_assign_levels, emulated) and its host remainder
+void emu_multirate_assign_device(int64_t K, int n1, const uint32_t* links, int max_levels, int n_stages,
+                                 int line_rates, int calls, const double* edt, const uint8_t* flags, int wet_dry,
+                                 const double* dt_cap, const double* landing, double* dt_out, double* stable_out,
+                                 int* finest_out, uint8_t* level_out, uint8_t* rates_out) {
+    namespace mr = dgswe::multirate;
+    mr::Assigner as;
+    as.max_levels = max_levels;
+    as.n_stages = n_stages;
+    as.n1 = n1;
+    as.line_rates = line_rates != 0;
+    as.init(links, (size_t)K);
+    std::vector<unsigned long long> slots(4);
+    std::vector<uint8_t> dlevel((size_t)K), dll((size_t)K), memo_given((size_t)K), memo_ll((size_t)K);
+    mr::Plan plan{};
+    int out[2] = {0, 0};
+    const unsigned rgrid = (unsigned)std::min<int64_t>((K + 255) / 256, 1024);
+    const bool lines = as.line_rates && n1 >= 2;
+    for (int q = 0; q < calls; ++q) {
+        const double* e = edt + (size_t)q * K;
+        const uint8_t* f = flags + (size_t)q * K;
+        emu_launch(1, 64, [&] { mr::mr_clear_kernel(slots.data(), out); });
+        emu_launch(rgrid, 256, [&] { mr::mr_extremes_kernel(e, f, K, slots.data()); });
+        emu_launch(1, 64, [&] { mr::mr_plan_kernel(slots.data(), wet_dry, dt_cap[q], landing[q], max_levels, &plan); });
+        const bool kept = as.memo;
+        emu_launch(rgrid, 256, [&] {
+            mr::mr_levels_kernel(e, f, K, &plan, lines ? 1 : 0, kept ? memo_given.data() : nullptr,
+                                 kept ? memo_ll.data() : nullptr, dlevel.data(), dll.data(), out);
+        });
+        if (!as.device_kept(plan, out[0], out[1])) {
+            as.level = dlevel;
+            as.ll = dll;
+            as.apply(as.finest);
+            memo_given = as.memo_given;
+            memo_ll = as.memo_ll;
+        }
+        dt_out[q] = plan.dt;
+        stable_out[q] = plan.stable;
+        finest_out[q] = as.finest;
+        std::memcpy(level_out + (size_t)q * K, as.level.data(), (size_t)K);
+        std::memcpy(rates_out + (size_t)q * n_stages * K, as.rates.data(), (size_t)n_stages * K);
+    }
+}
+
 double emu_dt(int order, int64_t K, const double* state, const double* corners, const uint32_t* links,
               const uint8_t* affine, double cfl, double g, double h_min, double h_dry, double pcfl, double max_cfl,
               double max_cfl_sub, int positivity, double* element_dt) {

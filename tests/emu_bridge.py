@@ -45,9 +45,10 @@ def lib():
         L.emu_finish_where.restype = ctypes.c_ulonglong
         L.emu_finish_where.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, vp, _dp, _dp, vp, ctypes.c_int64,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_int]
-        L.emu_multirate_assign.restype = None
-        L.emu_multirate_assign.argtypes = [ctypes.c_int64, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                           ctypes.c_int, _dp, vp, ctypes.c_int, _dp, _dp, _dp, _dp, vp, vp, vp]
+        for fn in (L.emu_multirate_assign, L.emu_multirate_assign_device):
+            fn.restype = None
+            fn.argtypes = [ctypes.c_int64, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                           ctypes.c_int, _dp, vp, ctypes.c_int, _dp, _dp, _dp, _dp, vp, vp, vp]
         L.emu_init()
         _lib = L
     return _lib
@@ -239,9 +240,11 @@ class EmulatedKernels:
         return np.split(gb, 3), np.split(gi, 3), gbed, gman
 
 
-def multirate_assign(links, n1, max_levels, n_stages, line_rates, edts, flags, wet_dry, dt_caps, landings):
+def multirate_assign(links, n1, max_levels, n_stages, line_rates, edts, flags, wet_dry, dt_caps, landings,
+                     device=False):
     """The library's level assignment (multirate::Assigner) over a sequence of
-    calls: per call (dt, stable, finest, level[K], rates[n_stages][K])."""
+    calls: per call (dt, stable, finest, level[K], rates[n_stages][K]); with
+    `device`, through the device half the library runs (emulated kernels)."""
     links = np.ascontiguousarray(links, dtype=np.uint32).ravel()
     K = links.size // 4
     calls = len(edts)
@@ -254,7 +257,8 @@ def multirate_assign(links, n1, max_levels, n_stages, line_rates, edts, flags, w
     level = np.zeros(calls * K, dtype=np.uint8)
     rates = np.zeros(calls * n_stages * K, dtype=np.uint8)
     P = lambda a: a.ctypes.data_as(_dp)
-    lib().emu_multirate_assign(K, n1, links.ctypes.data, max_levels, n_stages, int(line_rates), calls, P(E),
+    fn = lib().emu_multirate_assign_device if device else lib().emu_multirate_assign
+    fn(K, n1, links.ctypes.data, max_levels, n_stages, int(line_rates), calls, P(E),
                                F.ctypes.data, int(wet_dry), P(cap), P(land), P(dt), P(stable), finest.ctypes.data,
                                level.ctypes.data, rates.ctypes.data)
     return [(dt[q], stable[q], int(finest[q]), level[q * K:(q + 1) * K],

@@ -437,6 +437,13 @@ def test_library_assigner_equals_the_stepper(line_rates, scheme):
             assert g[0] == want[0] and g[1] == want[1] and g[2] == want[2]
             assert np.array_equal(g[3], want[3])
             assert np.array_equal(g[4], want[4])
+        # the library's device half (extremes, plan, levels and the comparison
+        # with the kept assignment, emulated) gives the same sequence
+        dev = multirate_assign(mesh.links, n1, 8, ns, line_rates, [c[0] for c in calls], [c[1] for c in calls],
+                               wet_dry, [c[2] for c in calls], [c[3] for c in calls], device=True)
+        for g, d in zip(got, dev):
+            assert g[0] == d[0] and g[1] == d[1] and g[2] == d[2]
+            assert np.array_equal(g[3], d[3]) and np.array_equal(g[4], d[4])
 
 
 def test_library_assigner_on_the_refined_tidal_channel():
