@@ -1639,11 +1639,14 @@ int dg_multirate_assign_levels(dg_multirate* m, const dg_state* s, double cfl, d
     if (int r = launch_dt_kernels(c, s->d, cfl, m->scheme, true)) return r;
     if (int r = launch_depth_flags(c, s->d, mr::SUBCELL_CAP_DEPTH_FACTOR * c->h_dry, c->h_dry,
                                    c->h_dry + mr::LINE_LOCAL_HEADROOM)) return r;
+    // few workgroups for the extremes (each folds its minimum and maxima in with
+    // atomics on four addresses), up to 1024 for the levels
+    const unsigned egrid = (unsigned)std::min<size_t>((K + 4095) / 4096, 128);
     const unsigned rgrid = (unsigned)std::min<size_t>((K + 255) / 256, 1024);
     const bool lines = m->as.line_rates && m->as.n1 >= 2;
     const bool kept = m->as.memo;
     hipLaunchKernelGGL(mr::mr_clear_kernel, dim3(1), dim3(64), 0, c->stream, m->d_slots, m->d_out);
-    hipLaunchKernelGGL(mr::mr_extremes_kernel, dim3(rgrid), dim3(256), 0, c->stream, c->element_dt, c->depth_flags,
+    hipLaunchKernelGGL(mr::mr_extremes_kernel, dim3(egrid), dim3(256), 0, c->stream, c->element_dt, c->depth_flags,
                        c->K, m->d_slots);
     hipLaunchKernelGGL(mr::mr_plan_kernel, dim3(1), dim3(64), 0, c->stream, m->d_slots, (c->flags & DG_WET_DRY) ? 1 : 0,
                        dt_cap, landing, m->as.max_levels, m->d_plan);

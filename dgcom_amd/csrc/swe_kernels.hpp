@@ -336,23 +336,24 @@ __device__ __forceinline__ void interface_pair(int p, int lb, int ls, const doub
 // The subcell update's contributions to the node at position p of a line
 // (swe_2d_split_form.rs:938-1035), from the reconstructed face states of the
 // line's subcells (rl, rr) and its interface fluxes (fa, fb) in LDS.
-template <int N>
+template <int N, int M>
 __device__ __forceinline__ void subcell_line(Cons& rhs, int p, int lb, int ls, const double* sh, const double* shu,
                                              const double* shv, const double* su, const double* sv, const double* sb,
                                              const double* smx, const double* smy, bool affine, double g,
-                                             const Side* rl, const Side* rr, const Cons* ifa, const Cons* ifb) {
+                                             const SideRows<M>& rl, const SideRows<M>& rr, const ConsRows<M>& ifa,
+                                             const ConsRows<M>& ifb) {
     constexpr int N1 = N + 1;
     const GllTables& T = c_gll[N];
     const int ip = lb + p * ls;
     Node me = {sh[ip], shu[ip], shv[ip], su[ip], sv[ip], sb[ip]};
     if (p == 0) rhs = cadd(rhs, cscale(1.0 / T.w1[0], ec_flux(me, me, smx[ip], smy[ip], g)));
     if (p == N1 - 1) rhs = csub(rhs, cscale(1.0 / T.w1[N1 - 1], ec_flux(me, me, smx[ip], smy[ip], g)));
-    if (p > 0) rhs = cadd(rhs, cscale(1.0 / T.w1[p], ifb[ip - ls]));
-    if (p < N1 - 1) rhs = csub(rhs, cscale(1.0 / T.w1[p], ifa[ip]));
+    if (p > 0) rhs = cadd(rhs, cscale(1.0 / T.w1[p], ifb.get(ip - ls)));
+    if (p < N1 - 1) rhs = csub(rhs, cscale(1.0 / T.w1[p], ifa.get(ip)));
     double mlx, mly, mrx, mry;
     interface_pair<N>(p, lb, ls, smx, smy, affine, mlx, mly, mrx, mry);
-    const Side& left = rl[ip];
-    const Side& right = rr[ip];
+    const Side left = rl.get(ip);
+    const Side right = rr.get(ip);
     double bed = -0.5 * g * (left.h + right.h) * (right.b - left.b) / T.w1[p];
     double fx, fy;
     if (affine) { fx = bed * smx[lb]; fy = bed * smy[lb]; }
@@ -498,13 +499,13 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
     const GllTables& T = c_gll[N];
     __shared__ double sh[64], shu[64], shv[64], su[64], sv[64], sb[64];
     __shared__ double srx[64], sry[64], ssx[64], ssy[64];
-    __shared__ Cons sfd[EPB * NF];
+    __shared__ ConsRows<EPB * NF> sfd;
     __shared__ double ssj[EPB * 4];
     __shared__ double pw[64];
     __shared__ int sflag[EPB], wflag[EPB];
     __shared__ double savg[EPB * 4];
-    __shared__ Side rl[2][SUB ? 64 : 1], rr[2][SUB ? 64 : 1];
-    __shared__ Cons ifa[2][SUB ? 64 : 1], ifb[2][SUB ? 64 : 1];
+    __shared__ SideRows<SUB ? 64 : 1> rl[2], rr[2];
+    __shared__ ConsRows<SUB ? 64 : 1> ifa[2], ifb[2];
     __shared__ int64_t skid[LTS && !SUB ? EPB : 1];
 
     const int t = threadIdx.x;
@@ -529,6 +530,10 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
     double bxr = 0.0, byr = 0.0, bxs = 0.0, bys = 0.0;
     double bh0 = 0.0, bhu0 = 0.0, bhv0 = 0.0, mann = 0.0;
     VjpPre vpre = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    // SUB: the outer nodes behind this node's line ends (r-line, s-line), read with the other loads
+    Side o_side[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+    double o_dist[2] = {0.0, 0.0};
+    bool o_has[2] = {false, false};
     bool affine = false;
     int lv = 0, lr = 0;  // LTS: the element's level and last-stage rate
     if (active) {
@@ -590,6 +595,19 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             vpre.wh = vp->wh[n]; vpre.whu = vp->whu[n]; vpre.whv = vp->whv[n];
             if (!p.rhs_only && p.a != 0.0 && vp->gbh) { vpre.gbh = vp->gbh[n]; vpre.gbhu = vp->gbhu[n]; vpre.gbhv = vp->gbhv[n]; }
             vpre.gbed = vp->gbed[n];
+        }
+        if constexpr (SUB && !VJP) {
+            // a node is at one end of its line at most (N >= 1): one outer node per direction
+#pragma unroll
+            for (int dir = 0; dir < 2; ++dir) {
+                const int pos = dir == 0 ? a : j;
+                const bool at_l = pos == 0;
+                if (at_l || pos == N1 - 1) {
+                    const int f = dir == 0 ? (at_l ? 3 : 1) : (at_l ? 0 : 2);
+                    const int fi = dir == 0 ? (at_l ? N1 - 1 - j : j) : (at_l ? a : N1 - 1 - a);
+                    o_has[dir] = outer_node<N>(p, k, f, fi, o_side[dir], o_dist[dir]);
+                }
+            }
         }
     }
     if (active) {
@@ -769,7 +787,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
                 double* o = p.fstar_out + ((size_t)k * NF + task) * 3;
                 o[0] = fstar.h; o[1] = fstar.hu; o[2] = fstar.hv;
             }
-            sfd[e * NF + task] = csub(ec_flux(q, q, nx, ny, p.g), fstar);
+            sfd.put(e * NF + task, csub(ec_flux(q, q, nx, ny, p.g), fstar));
             if (fi == 0) ssj[e * 4 + f] = sj;
         }
     }
@@ -796,13 +814,17 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
                     const int fi = dir == 0 ? (at_l ? N1 - 1 - j : j) : (at_l ? a : N1 - 1 - a);
                     Side o = me;
                     double d = 0.0;
-                    const bool has = outer_node<N>(p, k, f, fi, o, d);
+                    bool has;
+                    if constexpr (!VJP) { o = o_side[dir]; d = o_dist[dir]; has = o_has[dir]; }
+                    else has = outer_node<N>(p, k, f, fi, o, d);
                     if (at_l) { lft = o; xlf = -1.0 - d; has_l = has; }
                     else { rgt = o; xrt = 1.0 + d; has_r = has; }
                 }
+                Side sl = me, sr = me;
                 if (has_l && has_r)
-                    reconstruct(lft, me, rgt, xlf, T.x1[pos], xrt, T.xl[pos], T.xr[pos], p.h_dry, rl[dir][t], rr[dir][t]);
-                else { rl[dir][t] = me; rr[dir][t] = me; }
+                    reconstruct(lft, me, rgt, xlf, T.x1[pos], xrt, T.xl[pos], T.xr[pos], p.h_dry, sl, sr);
+                rl[dir].put(t, sl);
+                rr[dir].put(t, sr);
             }
         }
         lds_sync();
@@ -814,7 +836,10 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
                 if (pos < N1 - 1) {
                     double mlx, mly, mx, my;  // the interface to the right of this subcell: im[pos + 1]
                     interface_pair<N>(pos, lb, ls, dir == 0 ? srx : ssx, dir == 0 ? sry : ssy, affine, mlx, mly, mx, my);
-                    hydro_hll(rr[dir][t], rl[dir][t + ls], mx, my, p.g, p.sqrt_g, p.h_min, ifa[dir][t], ifb[dir][t]);
+                    Cons fa, fb;
+                    hydro_hll(rr[dir].get(t), rl[dir].get(t + ls), mx, my, p.g, p.sqrt_g, p.h_min, fa, fb);
+                    ifa[dir].put(t, fa);
+                    ifb[dir].put(t, fb);
                 }
             }
         }
@@ -849,10 +874,10 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             for (int pass = 0; pass < 2; ++pass) line_pass(pass);
         }
         // Surface terms in face order: LIFT sJ (F(q).n - F*)
-        if (j == 0) rhs = cadd(rhs, cscale(ssj[e * 4 + 0] * T.lift[0][a], sfd[e * NF + 0 * N1 + a]));
-        if (a == N1 - 1) rhs = cadd(rhs, cscale(ssj[e * 4 + 1] * T.lift[1][j], sfd[e * NF + 1 * N1 + j]));
-        if (j == N1 - 1) rhs = cadd(rhs, cscale(ssj[e * 4 + 2] * T.lift[2][N1 - 1 - a], sfd[e * NF + 2 * N1 + (N1 - 1 - a)]));
-        if (a == 0) rhs = cadd(rhs, cscale(ssj[e * 4 + 3] * T.lift[3][N1 - 1 - j], sfd[e * NF + 3 * N1 + (N1 - 1 - j)]));
+        if (j == 0) rhs = cadd(rhs, cscale(ssj[e * 4 + 0] * T.lift[0][a], sfd.get(e * NF + 0 * N1 + a)));
+        if (a == N1 - 1) rhs = cadd(rhs, cscale(ssj[e * 4 + 1] * T.lift[1][j], sfd.get(e * NF + 1 * N1 + j)));
+        if (j == N1 - 1) rhs = cadd(rhs, cscale(ssj[e * 4 + 2] * T.lift[2][N1 - 1 - a], sfd.get(e * NF + 2 * N1 + (N1 - 1 - a))));
+        if (a == 0) rhs = cadd(rhs, cscale(ssj[e * 4 + 3] * T.lift[3][N1 - 1 - j], sfd.get(e * NF + 3 * N1 + (N1 - 1 - j))));
         rhs = {det_inv * rhs.h, det_inv * rhs.hu, det_inv * rhs.hv};
         if (FORCE && p.coriolis_node) {
             // a beta-plane: f0 + beta (y - y0) at the node (CoriolisSource2D::add_element, coriolis.rs:157-169)

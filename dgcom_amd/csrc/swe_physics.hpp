@@ -19,6 +19,20 @@ struct Cons { double h, hu, hv; };            // SWEState2D
 struct Node { double h, hu, hv, u, v, b; };   // SWENodeState2D
 struct Side { double h, u, v, b; };           // SWESide2D
 
+// Per-lane Side and Cons values in LDS as one array per field: a lane stride of
+// 8 bytes keeps a wavefront's 64-bit accesses off each other's banks (a struct
+// per lane, 24 or 32 bytes apart, puts several lanes on one bank)
+template <int M> struct SideRows {
+    double h[M], u[M], v[M], b[M];
+    DG_HD Side get(int i) const { return {h[i], u[i], v[i], b[i]}; }
+    DG_HD void put(int i, const Side& s) { h[i] = s.h; u[i] = s.u; v[i] = s.v; b[i] = s.b; }
+};
+template <int M> struct ConsRows {
+    double h[M], hu[M], hv[M];
+    DG_HD Cons get(int i) const { return {h[i], hu[i], hv[i]}; }
+    DG_HD void put(int i, const Cons& c) { h[i] = c.h; hu[i] = c.hu; hv[i] = c.hv; }
+};
+
 DG_HD Cons cadd(Cons a, Cons b) { return {a.h + b.h, a.hu + b.hu, a.hv + b.hv}; }
 DG_HD Cons csub(Cons a, Cons b) { return {a.h - b.h, a.hu - b.hu, a.hv - b.hv}; }
 DG_HD Cons cneg(Cons a) { return {-a.h, -a.hu, -a.hv}; }

@@ -87,8 +87,8 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
                                          const Cons& rhs, const VjpPre& pre, const double* sh,
                                          const double* shu, const double* shv, const double* su, const double* sv,
                                          const double* sb, const double* srx, const double* sry, const double* ssx,
-                                         const double* ssy, const Side (*rl)[SUB ? 64 : 1],
-                                         const Side (*rr)[SUB ? 64 : 1]) {
+                                         const double* ssy, const SideRows<SUB ? 64 : 1>* rl,
+                                         const SideRows<SUB ? 64 : 1>* rr) {
     const StageParams& p = P.s;
     constexpr int N1 = N + 1, NN = N1 * N1, EPB = 64 / NN, NF = 4 * N1;
     constexpr int NT = (NF + NN - 1) / NN;
@@ -421,7 +421,7 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
                 ifab[dir][t] = pp < N1 - 1 ? cscale(-iw, Wp) : Cons{0.0, 0.0, 0.0};
                 double mlx, mly, mrx, mry;
                 interface_pair<N>(pp, lb, ls, smx, smy, affine, mlx, mly, mrx, mry);
-                const Side L = rl[dir][t], R = rr[dir][t];
+                const Side L = rl[dir].get(t), R = rr[dir].get(t);
                 double bedb, metb;
                 if (affine) { bedb = Wp.hu * smx[lb] + Wp.hv * smy[lb]; metb = 0.0; }
                 else {
@@ -454,7 +454,8 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
                     double mlx, mly, mx, my;  // im[pp + 1]
                     interface_pair<N>(pp, lb, ls, dir == 0 ? srx : ssx, dir == 0 ? sry : ssy, affine, mlx, mly, mx, my);
                     SideBar sab = sbar_zero(), sbb = sbar_zero();
-                    hydro_hll_adj(rr[dir][t], rl[dir][t + ls], mx, my, p.g, p.h_min, ifab[dir][t], ifbb[dir][t], sab, sbb);
+                    hydro_hll_adj(rr[dir].get(t), rl[dir].get(t + ls), mx, my, p.g, p.h_min, ifab[dir][t], ifbb[dir][t], sab,
+                                  sbb);
                     SideBar& own = rrb[dir][t];
                     own.h += sab.h; own.u += sab.u; own.v += sab.v; own.b += sab.b;
                     rlx[dir][t + ls] = sbb;

@@ -348,13 +348,14 @@ void emu_multirate_assign_device(int64_t K, int n1, const uint32_t* links, int m
     std::vector<uint8_t> dlevel((size_t)K), dll((size_t)K), memo_given((size_t)K), memo_ll((size_t)K);
     mr::Plan plan{};
     int out[2] = {0, 0};
+    const unsigned egrid = (unsigned)std::min<int64_t>((K + 4095) / 4096, 128);
     const unsigned rgrid = (unsigned)std::min<int64_t>((K + 255) / 256, 1024);
     const bool lines = as.line_rates && n1 >= 2;
     for (int q = 0; q < calls; ++q) {
         const double* e = edt + (size_t)q * K;
         const uint8_t* f = flags + (size_t)q * K;
         emu_launch(1, 64, [&] { mr::mr_clear_kernel(slots.data(), out); });
-        emu_launch(rgrid, 256, [&] { mr::mr_extremes_kernel(e, f, K, slots.data()); });
+        emu_launch(egrid, 256, [&] { mr::mr_extremes_kernel(e, f, K, slots.data()); });
         emu_launch(1, 64, [&] { mr::mr_plan_kernel(slots.data(), wet_dry, dt_cap[q], landing[q], max_levels, &plan); });
         const bool kept = as.memo;
         emu_launch(rgrid, 256, [&] {
