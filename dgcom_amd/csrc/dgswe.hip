@@ -1306,9 +1306,17 @@ int dg_begin_coarse_step(dg_ctx* c, const uint8_t* level, const uint8_t* rates, 
     int finest = 0;
     for (uint8_t v : keys) finest = std::max(finest, (int)v);
     DG_REQUIRE(finest <= DG_MAX_LEVELS, "dg_begin_coarse_step: more than DG_MAX_LEVELS levels");
-    for (int64_t k = 0; k < K; ++k)
-        for (int s = 0; s < n_stages; ++s)
-            DG_REQUIRE(rates[(size_t)s * K + k] >= level[k], "dg_begin_coarse_step: a rate below its element's level");
+    {
+        namespace mr = dgswe::multirate;
+        const int parts = mr::host_parts((size_t)K);
+        std::vector<uint8_t> bad((size_t)parts, 0);
+        mr::host_parallel((size_t)K, parts, [&](size_t b, size_t e, int q) {
+            for (size_t k = b; k < e; ++k)
+                for (int s = 0; s < n_stages; ++s)
+                    if (rates[(size_t)s * K + k] < level[k]) bad[(size_t)q] = 1;
+        });
+        for (uint8_t x : bad) DG_REQUIRE(!x, "dg_begin_coarse_step: a rate below its element's level");
+    }
     DG_TRY(hipSetDevice(c->device));
     if (!c->lts_order || c->lts_stages != n_stages) {
         if (c->lts_order) DG_TRY(hipFree(c->lts_order));
@@ -1320,18 +1328,21 @@ int dg_begin_coarse_step(dg_ctx* c, const uint8_t* level, const uint8_t* rates, 
     // (counting sort), and how many have a key of at least d
     std::vector<uint32_t> order((size_t)nkey * K);
     std::vector<int64_t> count((size_t)nkey * (DG_MAX_LEVELS + 1), 0);
-    for (int q = 0; q < nkey; ++q) {
-        const uint8_t* key = keys.data() + (size_t)q * K;
-        int64_t hist[DG_MAX_LEVELS + 2] = {0};
-        for (int64_t k = 0; k < K; ++k) ++hist[key[k]];
-        int64_t at_least[DG_MAX_LEVELS + 2] = {0};
-        for (int d = DG_MAX_LEVELS; d >= 0; --d) at_least[d] = at_least[d + 1] + hist[d];
-        int64_t next[DG_MAX_LEVELS + 1];
-        for (int v = 0; v <= DG_MAX_LEVELS; ++v) next[v] = at_least[v + 1];
-        uint32_t* o = order.data() + (size_t)q * K;
-        for (int64_t k = 0; k < K; ++k) o[next[key[k]]++] = (uint32_t)k;
-        for (int d = 0; d <= DG_MAX_LEVELS; ++d) count[(size_t)q * (DG_MAX_LEVELS + 1) + d] = at_least[d];
-    }
+    // one host thread per key
+    dgswe::multirate::host_parallel((size_t)nkey, nkey, [&](size_t q0, size_t q1, int) {
+        for (int q = (int)q0; q < (int)q1; ++q) {
+            const uint8_t* key = keys.data() + (size_t)q * K;
+            int64_t hist[DG_MAX_LEVELS + 2] = {0};
+            for (int64_t k = 0; k < K; ++k) ++hist[key[k]];
+            int64_t at_least[DG_MAX_LEVELS + 2] = {0};
+            for (int d = DG_MAX_LEVELS; d >= 0; --d) at_least[d] = at_least[d + 1] + hist[d];
+            int64_t next[DG_MAX_LEVELS + 1];
+            for (int v = 0; v <= DG_MAX_LEVELS; ++v) next[v] = at_least[v + 1];
+            uint32_t* o = order.data() + (size_t)q * K;
+            for (int64_t k = 0; k < K; ++k) o[next[key[k]]++] = (uint32_t)k;
+            for (int d = 0; d <= DG_MAX_LEVELS; ++d) count[(size_t)q * (DG_MAX_LEVELS + 1) + d] = at_least[d];
+        }
+    });
     DG_TRY(hipMemcpyAsync(c->lts_order, order.data(), order.size() * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
     DG_TRY(hipMemcpyAsync(c->lts_level, level, (size_t)K, hipMemcpyHostToDevice, c->stream));
     DG_TRY(hipMemcpyAsync(c->lts_lrate, rates + (size_t)(n_stages - 1) * K, (size_t)K, hipMemcpyHostToDevice, c->stream));

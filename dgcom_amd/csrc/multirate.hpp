@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <thread>
 #include <vector>
 
 // The coarse step's plan and an element's level are evaluated on the device
@@ -91,22 +92,49 @@ MR_HD int element_level(double edt, double dt, int bound) {
     return l;
 }
 
+// body(begin, end, part) over contiguous parts of [0, n) on up to 16 host
+// threads (one below 32768 items); part in [0, parts())
+inline int host_parts(size_t n) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t by_size = n / 32768 + 1;
+    size_t t = std::min<size_t>(std::min<size_t>(hw ? hw : 1, 16), by_size);
+    return (int)(t < 1 ? 1 : t);
+}
+template <class F>
+inline void host_parallel(size_t n, int parts, F body) {
+    if (parts <= 1) { body((size_t)0, n, 0); return; }
+    std::vector<std::thread> th;
+    th.reserve((size_t)parts);
+    for (int q = 0; q < parts; ++q) {
+        const size_t b = n * (size_t)q / (size_t)parts, e = n * (size_t)(q + 1) / (size_t)parts;
+        th.emplace_back([&body, b, e, q] { body(b, e, q); });
+    }
+    for (auto& t : th) t.join();
+}
+
 // spread_to_neighbours (multirate.rs:592-613): to[k] = max(from[k], from[j] - drop)
 inline bool spread_to_neighbours(const std::vector<int64_t>& nb, const std::vector<uint8_t>& from,
                                  std::vector<uint8_t>& to, int drop) {
     const size_t K = from.size();
     to.resize(K);
-    bool changed = false;
-    for (size_t k = 0; k < K; ++k) {
-        int m = from[k];
-        for (int f = 0; f < 4; ++f) {
-            const int64_t j = nb[4 * k + f];
-            if (j >= 0) { const int v = (int)from[(size_t)j] - drop; m = v > m ? v : m; }
+    const int parts = host_parts(K);
+    std::vector<uint8_t> changed((size_t)parts, 0);
+    host_parallel(K, parts, [&](size_t b, size_t e, int q) {
+        bool c = false;
+        for (size_t k = b; k < e; ++k) {
+            int m = from[k];
+            for (int f = 0; f < 4; ++f) {
+                const int64_t j = nb[4 * k + f];
+                if (j >= 0) { const int v = (int)from[(size_t)j] - drop; m = v > m ? v : m; }
+            }
+            to[k] = (uint8_t)m;
+            c = c || to[k] != from[k];
         }
-        to[k] = (uint8_t)m;
-        changed = changed || to[k] != from[k];
-    }
-    return changed;
+        changed[(size_t)q] = c ? 1 : 0;
+    });
+    for (uint8_t c : changed)
+        if (c) return true;
+    return false;
 }
 
 // The node of face f at position fi (node j*n1 + i on r-line j, s-line i),
@@ -118,58 +146,71 @@ inline int inner_offset(int n1, int f) { return f == 0 ? n1 : f == 1 ? -1 : f ==
 
 // spread_over_lines (multirate.rs:705-767): node rates of a stage's value from
 // those of its input, and each element's RHS rate
-inline void spread_over_lines(const std::vector<int64_t>& nb, const std::vector<uint32_t>& links, int n1,
-                              const std::vector<uint8_t>& line_local, const std::vector<uint8_t>& from,
-                              std::vector<uint8_t>& to, std::vector<uint8_t>& rate) {
-    const size_t nn = (size_t)n1 * n1, K = line_local.size();
+template <int n1>
+inline void spread_over_lines_n(const std::vector<int64_t>& nb, const std::vector<uint32_t>& links,
+                                const std::vector<uint8_t>& line_local, const std::vector<uint8_t>& from,
+                                std::vector<uint8_t>& to, std::vector<uint8_t>& rate) {
+    constexpr size_t nn = (size_t)n1 * n1;
+    const size_t K = line_local.size();
     to.resize(K * nn);
     rate.resize(K);
-    for (size_t k = 0; k < K; ++k) {
-        const uint8_t* own = from.data() + k * nn;
-        uint8_t* t = to.data() + k * nn;
-        if (!line_local[k]) {
-            uint8_t m = 0;
-            for (size_t i = 0; i < nn; ++i) m = own[i] > m ? own[i] : m;
+    host_parallel(K, host_parts(K), [&](size_t kb, size_t ke, int) {
+        for (size_t k = kb; k < ke; ++k) {
+            const uint8_t* own = from.data() + k * nn;
+            uint8_t* t = to.data() + k * nn;
+            if (!line_local[k]) {
+                uint8_t m = 0;
+                for (size_t i = 0; i < nn; ++i) m = own[i] > m ? own[i] : m;
+                for (int f = 0; f < 4; ++f) {
+                    const int64_t j = nb[4 * k + f];
+                    if (j < 0) continue;
+                    const int g = (int)(links[4 * k + f] & 3u);
+                    const uint8_t* other = from.data() + (size_t)j * nn;
+                    for (int fi = 0; fi < n1; ++fi) {
+                        const int node = face_node(n1, g, fi);
+                        m = other[node] > m ? other[node] : m;
+                        const uint8_t in = other[node + inner_offset(n1, g)];
+                        m = in > m ? in : m;
+                    }
+                }
+                std::memset(t, m, nn);
+                rate[k] = m;
+                continue;
+            }
+            for (int j = 0; j < n1; ++j) {
+                uint8_t m = 0;
+                for (int i = 0; i < n1; ++i) m = own[j * n1 + i] > m ? own[j * n1 + i] : m;
+                for (int i = 0; i < n1; ++i) t[j * n1 + i] = m;
+            }
+            for (int i = 0; i < n1; ++i) {
+                uint8_t m = 0;
+                for (int j = 0; j < n1; ++j) m = own[j * n1 + i] > m ? own[j * n1 + i] : m;
+                for (int j = 0; j < n1; ++j) t[j * n1 + i] = t[j * n1 + i] > m ? t[j * n1 + i] : m;
+            }
             for (int f = 0; f < 4; ++f) {
                 const int64_t j = nb[4 * k + f];
                 if (j < 0) continue;
                 const int g = (int)(links[4 * k + f] & 3u);
                 const uint8_t* other = from.data() + (size_t)j * nn;
                 for (int fi = 0; fi < n1; ++fi) {
-                    const int node = face_node(n1, g, fi);
-                    m = other[node] > m ? other[node] : m;
-                    const uint8_t in = other[node + inner_offset(n1, g)];
-                    m = in > m ? in : m;
+                    const int node = face_node(n1, f, fi);
+                    const uint8_t v = other[face_node(n1, g, n1 - 1 - fi)];
+                    t[node] = t[node] > v ? t[node] : v;
                 }
             }
-            std::memset(t, m, nn);
+            uint8_t m = 0;
+            for (size_t i = 0; i < nn; ++i) m = t[i] > m ? t[i] : m;
             rate[k] = m;
-            continue;
         }
-        for (int j = 0; j < n1; ++j) {
-            uint8_t m = 0;
-            for (int i = 0; i < n1; ++i) m = own[j * n1 + i] > m ? own[j * n1 + i] : m;
-            for (int i = 0; i < n1; ++i) t[j * n1 + i] = m;
-        }
-        for (int i = 0; i < n1; ++i) {
-            uint8_t m = 0;
-            for (int j = 0; j < n1; ++j) m = own[j * n1 + i] > m ? own[j * n1 + i] : m;
-            for (int j = 0; j < n1; ++j) t[j * n1 + i] = t[j * n1 + i] > m ? t[j * n1 + i] : m;
-        }
-        for (int f = 0; f < 4; ++f) {
-            const int64_t j = nb[4 * k + f];
-            if (j < 0) continue;
-            const int g = (int)(links[4 * k + f] & 3u);
-            const uint8_t* other = from.data() + (size_t)j * nn;
-            for (int fi = 0; fi < n1; ++fi) {
-                const int node = face_node(n1, f, fi);
-                const uint8_t v = other[face_node(n1, g, n1 - 1 - fi)];
-                t[node] = t[node] > v ? t[node] : v;
-            }
-        }
-        uint8_t m = 0;
-        for (size_t i = 0; i < nn; ++i) m = t[i] > m ? t[i] : m;
-        rate[k] = m;
+    });
+}
+inline void spread_over_lines(const std::vector<int64_t>& nb, const std::vector<uint32_t>& links, int n1,
+                              const std::vector<uint8_t>& line_local, const std::vector<uint8_t>& from,
+                              std::vector<uint8_t>& to, std::vector<uint8_t>& rate) {
+    switch (n1) {
+        case 2: spread_over_lines_n<2>(nb, links, line_local, from, to, rate); break;
+        case 3: spread_over_lines_n<3>(nb, links, line_local, from, to, rate); break;
+        default: spread_over_lines_n<4>(nb, links, line_local, from, to, rate); break;
     }
 }
 
