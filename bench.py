@@ -10,6 +10,8 @@ at CFL 0.5 with the positivity-relaxed time step, from still water.
 A step is one adaptive SSP-RK3 time step (the time-step kernel, its global
 minimum read back to the host, three fused stage kernels), inputs resident in
 HBM. value = DOF-updates/s = 3 * K * (N+1)^2 * 3 stages * steps / wall time.
+Before the warm-up the library's autotuner (dg_tune) picks the stage's launch
+shape for this order and size, untimed (--no-tune keeps the default).
 
 Besides the timed loop: the stage kernel's average launch duration from HIP
 events on the library's stream gives the roofline point (algorithmic bytes per
@@ -341,6 +343,7 @@ def main():
     ap.add_argument("--n", type=int, default=1000, help="grid side (1000: the 1M-element workload)")
     ap.add_argument("--fma", action="store_true", help="the FMA-contracted variant of the library")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no dg_tune)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--adjoint-n", type=int, default=500, help="grid side of the adjoint workload (500: 250,000 quads)")
@@ -373,6 +376,8 @@ def main():
     K, nn = case.n_elements, case.nn
     phys = swe2d.SWEPhysics2D.from_case(case, fma=args.fma)
     state = phys.state(case.h, case.hu, case.hv)
+    # the stage's launch shape for this order and size (dg_tune, untimed; no result changes)
+    shape = phys.tune(state) if not args.no_tune else phys.launch_shape()
 
     t = phys.run(state, 0.0, args.warmup, case.cfl) if args.warmup > 0 else 0.0
     phys.sync()
@@ -425,7 +430,8 @@ def main():
                      "avg_launch_ms": stage_ms / max(launches, 1),
                      "bytes_per_launch_avg": bytes_total / max(launches, 1),
                      "measured_copy_GBps": copy_gbs, "frac_of_copy": achieved / copy_gbs,
-                     "dt_kernel_avg_ms": dt_ms / max(dt_launches, 1)},
+                     "dt_kernel_avg_ms": dt_ms / max(dt_launches, 1),
+                     "launch_shape": {**shape, "tuned": not args.no_tune}},
         "setup_s": gen_s,
     }
     if fp64 is not None:

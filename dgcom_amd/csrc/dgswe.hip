@@ -17,7 +17,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -76,8 +78,15 @@ struct dg_state {
     bool owned = true;    // false: caller memory (dg_state_wrap)
 };
 
+// The stage's launch shape, per (device, order, size class): the wet kernel's
+// block mapping (XCD-aware or linear) and the subcell kernel's grid
+struct LaunchShape {
+    unsigned int xcd = 1, sub_groups = SUB_GROUPS;
+};
+
 struct dg_ctx {
     int device = 0;
+    LaunchShape shape;
     hipStream_t stream = nullptr;      // where the work runs (the context's own, or the caller's)
     hipStream_t own_stream = nullptr;
     int order = 0, nn = 0;
@@ -451,8 +460,8 @@ static int launch_stage(dg_ctx* c, const double* base, const double* input, doub
     const int epb = 64 / c->nn;
     const unsigned int nblk = (unsigned int)((c->K + epb - 1) / epb);
     const unsigned int nchunk = (nblk + 1023) / 1024;
-    p.xcd_stride = (nblk + 7) / 8;
-    dim3 grid(8 * p.xcd_stride), sub_grid(std::min(nblk, SUB_GROUPS));
+    p.xcd_stride = c->shape.xcd ? (nblk + 7) / 8 : 0;
+    dim3 grid(p.xcd_stride ? 8 * p.xcd_stride : nblk), sub_grid(std::min(nblk, c->shape.sub_groups));
     // the forcing given per node compiled in only where some is set
     const bool force = c->tide_slot || c->nest_slot || c->wx_series || c->manning_node || c->coriolis_node;
     p.manning_node = c->manning_node;
@@ -548,6 +557,67 @@ static int ensure_work(dg_ctx* c) {
 static const double RK3[3][4] = {{1.0, 0.0, 1.0, 0.0}, {0.75, 0.25, 0.25, 1.0}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, 0.5}};
 static const double RK43[4][4] = {{1.0, 0.0, 0.5, 0.0}, {0.0, 1.0, 0.5, 0.5}, {2.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0, 1.0},
                                   {0.0, 1.0, 0.5, 0.5}};
+
+// The launch shapes tuned so far, per (device, order, size class: floor(log2 K));
+// a form uploaded in a class that is here starts from its shape
+static std::mutex g_shape_mutex;
+static std::map<std::tuple<int, int, int>, LaunchShape> g_shapes;
+
+static int size_class(int64_t K) {
+    int q = 0;
+    while (q < 62 && (int64_t(1) << (q + 1)) <= K) ++q;
+    return q;
+}
+
+static LaunchShape shape_for(int device, int order, int64_t K) {
+    std::lock_guard<std::mutex> lock(g_shape_mutex);
+    const auto it = g_shapes.find(std::make_tuple(device, order, size_class(K)));
+    return it == g_shapes.end() ? LaunchShape{} : it->second;
+}
+
+// The autotuner of the single-run stage: every launch shape (block mapping,
+// subcell grid) timed on the caller's state, three stages each after one
+// warm-up, into work buffers; the fastest is kept for the context and for its
+// (device, order, size class). The shapes change no arithmetic.
+static int tune_impl(dg_ctx* c, const dg_state* s) {
+    if (int r = ensure_work(c)) return r;
+    const bool timing = c->timing;
+    c->timing = false;
+    unsigned long long clips = 0;  // the caller's count, kept across the timing launches
+    DG_TRY(hipMemcpyAsync(&clips, c->clips, sizeof(clips), hipMemcpyDeviceToHost, c->stream));
+    DG_TRY(hipStreamSynchronize(c->stream));
+    static const unsigned int subs[3] = {2048, SUB_GROUPS, 65536};
+    // two passes over the shapes, five stages each after a warm-up; a shape's
+    // time is its better pass (the passes interleave the shapes against drift)
+    float ms_of[2][3];
+    for (int pass = 0; pass < 2; ++pass)
+        for (unsigned int x = 0; x < 2; ++x)
+            for (int q = 0; q < 3; ++q) {
+                c->shape = LaunchShape{1u - x, subs[q]};
+                // stage 1 of SSP-RK3 at a small step, into a work buffer
+                if (int r = launch_stage(c, s->d, s->d, c->work[0]->d, 1.0, 0.0, 1e-3, 0.0, 0)) return r;
+                DG_TRY(hipEventRecord(c->ev[0], c->stream));
+                for (int rep = 0; rep < 5; ++rep)
+                    if (int r = launch_stage(c, s->d, s->d, c->work[0]->d, 1.0, 0.0, 1e-3, 0.0, 0)) return r;
+                DG_TRY(hipEventRecord(c->ev[1], c->stream));
+                DG_TRY(hipEventSynchronize(c->ev[1]));
+                float ms = 0.f;
+                DG_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+                ms_of[x][q] = pass == 0 ? ms : std::min(ms_of[x][q], ms);
+            }
+    LaunchShape best = c->shape;
+    float best_ms = 1e30f;
+    for (unsigned int x = 0; x < 2; ++x)
+        for (int q = 0; q < 3; ++q)
+            if (ms_of[x][q] < best_ms) { best_ms = ms_of[x][q]; best = LaunchShape{1u - x, subs[q]}; }
+    c->shape = best;
+    c->timing = timing;
+    DG_TRY(hipMemcpyAsync(c->clips, &clips, sizeof(clips), hipMemcpyHostToDevice, c->stream));
+    DG_TRY(hipStreamSynchronize(c->stream));
+    std::lock_guard<std::mutex> lock(g_shape_mutex);
+    g_shapes[std::make_tuple(c->device, c->order, size_class(c->K))] = best;
+    return 0;
+}
 
 static int step_impl(dg_ctx* c, dg_state* s, double t, double dt, int scheme) {
     // stage values in their own buffers, the last one the new state
@@ -996,6 +1066,12 @@ static int batch_drop_graphs(dg_batch* b) {
 static int batch_tune(dg_batch* b) {
     hipStream_t st = b->owner->stream;
     hipEvent_t e0 = b->owner->ev[0], e1 = b->owner->ev[1];
+    // the members' clip counts, kept across the timing launches
+    const size_t M = b->members.size();
+    std::vector<unsigned long long> clips(M, 0);
+    for (size_t m = 0; m < M; ++m)
+        DG_TRY(hipMemcpyAsync(&clips[m], b->members[m]->clips, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    DG_TRY(hipStreamSynchronize(st));
     static const unsigned int subs[3] = {2048, SUB_GROUPS, 65536};
     for (int o = 1; o <= 3; ++o) {
         BatchOrder& B = b->ord[o];
@@ -1019,6 +1095,9 @@ static int batch_tune(dg_batch* b) {
         B.xcd = bx;
         B.sub_groups = bs;
     }
+    for (size_t m = 0; m < M; ++m)
+        DG_TRY(hipMemcpyAsync(b->members[m]->clips, &clips[m], sizeof(unsigned long long), hipMemcpyHostToDevice, st));
+    DG_TRY(hipStreamSynchronize(st));
     b->tuned = true;
     return batch_drop_graphs(b);
 }
@@ -1114,6 +1193,7 @@ int dg_upload_form(dg_ctx* c, const dg_form* f) {
                "dg_upload_form: the EntropyStable faces take no subcells (DG_SUBCELLS off)");
     DG_TRY(hipSetDevice(c->device));
     if (int r = free_statics(c)) return r;
+    c->shape = shape_for(c->device, f->order, f->n_elements);
     c->order = f->order;
     c->nn = (f->order + 1) * (f->order + 1);
     c->K = f->n_elements;
@@ -2011,6 +2091,22 @@ int dg_run(dg_ctx* c, dg_state* s, double t, int n_steps, double cfl, int scheme
     }
     DG_TRY(hipStreamSynchronize(c->stream));
     if (t_end) *t_end = t;
+    return 0;
+}
+
+int dg_tune(dg_ctx* c, const dg_state* s) {
+    DG_REQUIRE(c != nullptr, "dg_tune: null context");
+    DG_LOCK(c);
+    DG_REQUIRE(s && c->K > 0, "dg_tune: null state or no form");
+    DG_TRY(hipSetDevice(c->device));
+    return tune_impl(c, s);
+}
+
+int dg_launch_shape(dg_ctx* c, int* xcd_mapping, int* subcell_groups) {
+    DG_REQUIRE(c != nullptr, "dg_launch_shape: null context");
+    DG_LOCK(c);
+    if (xcd_mapping) *xcd_mapping = (int)c->shape.xcd;
+    if (subcell_groups) *subcell_groups = (int)c->shape.sub_groups;
     return 0;
 }
 

@@ -41,7 +41,7 @@ EXPORTS = (
     "dg_multirate_create", "dg_multirate_destroy", "dg_multirate_assign_levels", "dg_multirate_step",
     "dg_multirate_levels", "dg_multirate_stats_read", "dg_multirate_run", "dg_fnv1a", "dg_set_boundary_tides",
     "dg_set_nesting", "dg_set_weather", "dg_set_inverse_barometer", "dg_set_nesting_boundary", "dg_masks_noalloc",
-    "dg_set_manning_nodal", "dg_set_coriolis_beta",
+    "dg_set_manning_nodal", "dg_set_coriolis_beta", "dg_tune", "dg_launch_shape",
 )
 BATCH_GRAPH, BATCH_TUNE = 1, 2
 
@@ -112,6 +112,8 @@ def load(fma: bool = False):
     L.dg_step.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int]
     L.dg_run.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp]
     L.dg_is_finite.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int)]
+    L.dg_tune.argtypes = [vp, vp]
+    L.dg_launch_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.dg_clip_count.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
     L.dg_masks.argtypes = [vp, vp, _u8p, _u8p]
     L.dg_masks_noalloc.argtypes = [vp, vp, _u8p, _u8p]
@@ -412,6 +414,17 @@ class SWEPhysics2D:
         _check(self.lib, self.lib.dg_run(self.ctx, state.handle, t, n_steps, cfl, scheme, ctypes.byref(te),
                                          None if dts is None else _p(dts)), "dg_run")
         return (te.value, dts) if record_dt else te.value
+
+    def tune(self, state: DeviceState) -> dict:
+        """The stage's launch shape timed on `state` (dg_tune): kept for this context and for
+        later forms of its order and size class in this process."""
+        _check(self.lib, self.lib.dg_tune(self.ctx, state.handle), "dg_tune")
+        return self.launch_shape()
+
+    def launch_shape(self) -> dict:
+        x, g = ctypes.c_int(), ctypes.c_int()
+        _check(self.lib, self.lib.dg_launch_shape(self.ctx, ctypes.byref(x), ctypes.byref(g)), "dg_launch_shape")
+        return {"xcd_mapping": bool(x.value), "subcell_groups": g.value}
 
     def negative_depth_clips(self) -> int:
         v = ctypes.c_uint64()

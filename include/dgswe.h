@@ -281,6 +281,17 @@ int dg_step(dg_ctx* ctx, dg_state* state, double t, double dt, int scheme);
  * end time, dts (optional, n_steps) each step's dt. */
 int dg_run(dg_ctx* ctx, dg_state* state, double t, int n_steps, double cfl, int scheme, double* t_end, double* dts);
 
+/* The stage's launch shape (no counterpart in the reference, which launches one
+ * thread per element): dg_tune times every shape (the wet kernel's block mapping,
+ * XCD-aware or linear; the subcell kernel's grid, 2048, 16384 or 65536
+ * workgroups) on the state given, three stages each, and keeps the fastest for
+ * the context and for its (device, order, size class floor(log2 K)): a form
+ * uploaded later in that class starts from it. The shapes change no result; the
+ * clip count is kept. dg_launch_shape reports the context's shape (by default
+ * XCD-aware, 16384 subcell workgroups). */
+int dg_tune(dg_ctx* ctx, const dg_state* state);
+int dg_launch_shape(dg_ctx* ctx, int* xcd_mapping, int* subcell_groups);
+
 /* Integrable::is_finite (integrator.rs:54-106) and the negative-depth clips. */
 int dg_is_finite(dg_ctx* ctx, const dg_state* state, int* finite);
 int dg_clip_count(dg_ctx* ctx, uint64_t* clips);

@@ -25,12 +25,13 @@ def _members():
 def test_ensemble_equals_each_members_own_run(graph, tune):
     cs, states = _members()
     n_steps = 12
-    want = []
+    want, clips = [], []
     for c, st in zip(cs, states):
         phys = swe2d.SWEPhysics2D.from_case(c)
         s = phys.state(*st)
         t = phys.run(s, 0.0, n_steps, c.cfl)
         want.append((t, s.download()))
+        clips.append(phys.negative_depth_clips())
     phys = [swe2d.SWEPhysics2D.from_case(c) for c in cs]
     ens = swe2d.Ensemble(phys)
     for m, st in enumerate(states):
@@ -40,6 +41,7 @@ def test_ensemble_equals_each_members_own_run(graph, tune):
     for m, (t, x) in enumerate(want):
         got = ens.download(m)
         assert times[m] == t, (m, times[m], t)
+        assert phys[m].negative_depth_clips() == clips[m]   # the tuner's timing launches count none
         for g, r in zip(got, x):
             assert np.array_equal(g, r), (m, np.abs(g - r).max())
     # the restatement's own run of each member
