@@ -122,8 +122,8 @@ struct dg_ctx {
     double *gbed = nullptr, *gman = nullptr;          // parameter cotangents [K*NN], [K]
     double *facebuf = nullptr, *innerbuf = nullptr;  // [K][4(N+1)][4] neighbour cotangents
     double* facecot = nullptr;                       // [K][4(N+1)][3] F* cotangents
-    uint32_t* vjp_edges = nullptr;                   // the edge kernel's faces (vjp_edge_list)
-    uint32_t n_vjp_edges = 0;
+    uint32_t *vjp_edges = nullptr, *vjp_split = nullptr;  // the edge kernels' faces (vjp_edge_lists)
+    uint32_t n_vjp_edges = 0, n_vjp_split = 0;
     uint8_t* kind = nullptr;                         // [K] subcell elements of the last VJP
     dg_state* adj[2] = {nullptr, nullptr};           // stage cotangents of dg_step_vjp
     dg_state* rhsbuf[4] = {nullptr, nullptr, nullptr, nullptr};  // stage RHS saved by the forward for the VJP
@@ -243,15 +243,15 @@ static double tide_elevation(const dg_tide& t, const double (&nod)[5][8], double
 static int free_statics(dg_ctx* c) {
     void* ptrs[] = {c->corners, c->bed, c->manning, c->element_dt, c->links, c->tags, c->affine, c->queue,
                     c->queue_count, c->dt_slots, c->sub_list, c->chunk_total, c->gbed, c->gman, c->facebuf,
-                    c->innerbuf, c->facecot, c->vjp_edges, c->kind, c->st_elem, c->st_w, c->st_eta, c->geo, c->fgeo,
+                    c->innerbuf, c->facecot, c->vjp_edges, c->vjp_split, c->kind, c->st_elem, c->st_w, c->st_eta, c->geo, c->fgeo,
                     c->lts_level, c->lts_lrate, c->lts_order, c->depth_flags, c->tide_slot, c->tide_coef,
                     c->tide_mean, c->tide_src, c->nest_slot, c->nest_series, c->nest_weight, c->nest_depth,
                     c->wx_series, c->manning_node, c->coriolis_node, c->emass};
     for (void* p : ptrs)
         if (p) DG_TRY(hipFree(p));
     c->gbed = c->gman = c->facebuf = c->innerbuf = c->facecot = nullptr;
-    c->vjp_edges = nullptr;
-    c->n_vjp_edges = 0;
+    c->vjp_edges = c->vjp_split = nullptr;
+    c->n_vjp_edges = c->n_vjp_split = 0;
     c->geo = c->fgeo = c->emass = nullptr;
     c->lts_level = c->lts_lrate = c->depth_flags = nullptr;
     c->tide_slot = nullptr;
@@ -654,11 +654,16 @@ static int ensure_adjoint(dg_ctx* c) {
         DG_TRY(hipStreamSynchronize(c->stream));
         DG_TRY(hipMemcpy(links.data(), c->links, links.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
         DG_TRY(hipMemcpy(normals.data(), c->fgeo, normals.size() * sizeof(double), hipMemcpyDeviceToHost));
-        const std::vector<uint32_t> edges =
-            vjp_edge_list(links.data(), normals.data(), normals.data() + (size_t)c->K * 4, c->K);
+        std::vector<uint32_t> edges, split;
+        vjp_edge_lists(links.data(), normals.data(), normals.data() + (size_t)c->K * 4, c->K, edges, split);
         c->n_vjp_edges = (uint32_t)edges.size();
+        c->n_vjp_split = (uint32_t)split.size();
         DG_TRY(hipMalloc(&c->vjp_edges, edges.size() * sizeof(uint32_t)));
         DG_TRY(hipMemcpy(c->vjp_edges, edges.data(), edges.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        if (!split.empty()) {
+            DG_TRY(hipMalloc(&c->vjp_split, split.size() * sizeof(uint32_t)));
+            DG_TRY(hipMemcpy(c->vjp_split, split.data(), split.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        }
     }
     DG_TRY(hipMalloc(&c->kind, (size_t)c->K));
     DG_TRY(hipMalloc(&c->gbed, nt * sizeof(double)));
@@ -746,6 +751,14 @@ static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, 
         case 1: hipLaunchKernelGGL(edge_vjp_kernel<1>, fgrid, dim3(64), 0, c->stream, P, c->vjp_edges, c->n_vjp_edges); break;
         case 2: hipLaunchKernelGGL(edge_vjp_kernel<2>, fgrid, dim3(64), 0, c->stream, P, c->vjp_edges, c->n_vjp_edges); break;
         default: hipLaunchKernelGGL(edge_vjp_kernel<3>, fgrid, dim3(64), 0, c->stream, P, c->vjp_edges, c->n_vjp_edges); break;
+    }
+    if (c->n_vjp_split) {
+        const dim3 sgrid((unsigned)(((size_t)c->n_vjp_split * (c->order + 1) + 63) / 64));
+        switch (c->order) {
+            case 1: hipLaunchKernelGGL(edge_vjp_split_kernel<1>, sgrid, dim3(64), 0, c->stream, P, c->vjp_split, c->n_vjp_split); break;
+            case 2: hipLaunchKernelGGL(edge_vjp_split_kernel<2>, sgrid, dim3(64), 0, c->stream, P, c->vjp_split, c->n_vjp_split); break;
+            default: hipLaunchKernelGGL(edge_vjp_split_kernel<3>, sgrid, dim3(64), 0, c->stream, P, c->vjp_split, c->n_vjp_split); break;
+        }
     }
     double *gh = ginput, *ghu = ginput + nt, *ghv = ginput + 2 * nt;
     switch (c->order) {

@@ -575,34 +575,30 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
     (void)bin;
 }
 
-// The faces whose Riemann solves the edge kernel runs backwards, as
-// k << 3 | alone << 2 | f: every boundary face; every interior edge once, from
-// its left side (the side whose link has the left bit), where the normal its
-// right side solved with equals the left's own (the right's own reversed on a
-// shared face, equal to the left's up to the sign of a zero component on a
-// conforming edge; the left's on a periodic pair); and an edge whose two
-// normals differ from each side alone. In element order. nx, ny: the faces'
-// unit normals [K][4].
-inline std::vector<uint32_t> vjp_edge_list(const uint32_t* links, const double* nx, const double* ny, int64_t K) {
-    std::vector<uint32_t> e;
-    e.reserve((size_t)K * 2 + 16);
+// The faces whose Riemann solves the edge kernels run backwards, as k << 2 | f,
+// in element order. `edges`: every boundary face, and every interior edge once,
+// from its left side (the side whose link has the left bit), where the normal
+// its right side solved with equals the left's own (the right's own reversed on
+// a shared face: equal to the left's up to the sign of a zero component on a
+// conforming edge; the left's own on a periodic pair). `split`: the edges whose
+// two normals differ, from their left side (for edge_vjp_split_kernel). nx, ny:
+// the faces' unit normals [K][4].
+inline void vjp_edge_lists(const uint32_t* links, const double* nx, const double* ny, int64_t K,
+                           std::vector<uint32_t>& edges, std::vector<uint32_t>& split) {
+    edges.clear();
+    split.clear();
+    edges.reserve((size_t)K * 2 + 16);
     for (int64_t k = 0; k < K; ++k)
         for (int f = 0; f < 4; ++f) {
             const size_t fk = (size_t)k * 4 + f;
             const uint32_t l = links[fk];
-            const uint32_t own = (uint32_t)k << 3 | (uint32_t)f;
-            if (l == LINK_BOUNDARY) { e.push_back(own); continue; }
+            const uint32_t own = (uint32_t)k << 2 | (uint32_t)f;
+            if (l == LINK_BOUNDARY) { edges.push_back(own); continue; }
+            if (!(l & 4u)) continue;  // listed from its left side
             const size_t fo = (size_t)(l >> 4) * 4 + (l & 3u);
-            const bool left = (l & 4u) != 0;
-            const size_t fl = left ? fk : fo, fr = left ? fo : fk;
-            const bool agree = !(links[fr] & 8u) || (-nx[fr] == nx[fl] && -ny[fr] == ny[fl]);
-            if (agree) {
-                if (left) e.push_back(own);
-            } else {
-                e.push_back(own | 4u);
-            }
+            const bool agree = !(links[fo] & 8u) || (-nx[fo] == nx[fk] && -ny[fo] == ny[fk]);
+            (agree ? edges : split).push_back(own);
         }
-    return e;
 }
 
 // The Riemann solves backwards, one lane per node of a listed face, from the
@@ -611,12 +607,11 @@ inline std::vector<uint32_t> vjp_edge_list(const uint32_t* links, const double* 
 // left side with the same states and (for a listed edge) the same normal, and
 // took F*_a on the left and -F*_b on the right: one reverse solve with both
 // cotangents (that of F*_a from the left, minus the right's of F*_b) carries
-// the edge, and writes the cotangents of both face nodes. A face listed alone
-// (its edge's two normals differ in a bit) carries its own element's
-// cotangent only, as that element solved it; a boundary face (wall ghost, open
-// boundary) is its element's alone. The (h, hu, hv, B) cotangent of every
-// element face node goes to that node's own slot of facebuf: each slot is
-// written by one lane.
+// the edge, and writes the cotangents of both face nodes; a boundary face (wall
+// ghost, open boundary) is its element's alone. The (h, hu, hv, B) cotangent
+// of every element face node goes to that node's own slot of facebuf: each
+// slot is written by one lane (an edge whose normals differ is
+// edge_vjp_split_kernel's).
 template <int N>
 __global__ __launch_bounds__(64) DG_WAVES_PER_EU(3) void edge_vjp_kernel(VjpParams P, const uint32_t* edges,
                                                                          uint32_t n_edges) {
@@ -626,9 +621,8 @@ __global__ __launch_bounds__(64) DG_WAVES_PER_EU(3) void edge_vjp_kernel(VjpPara
     if (gid >= (size_t)n_edges * N1) return;
     const uint32_t ed = edges[gid / N1];
     const int fi = (int)(gid % N1);
-    const int64_t k = (int64_t)(ed >> 3);
+    const int64_t k = (int64_t)(ed >> 2);
     const int f = (int)(ed & 3u);
-    const bool alone = (ed & 4u) != 0;
     const size_t n = (size_t)k * NN + face_node<N>(f, fi);
     const double h = p.ih[n], hu = p.ihu[n], hv = p.ihv[n], bd = p.bed[n];
     const Node q = make_node(h, hu, hv, bd, p.h_min);
@@ -650,14 +644,11 @@ __global__ __launch_bounds__(64) DG_WAVES_PER_EU(3) void edge_vjp_kernel(VjpPara
         characteristic_adj(q.h, q.hu, q.hv, q.b, p.eta_e, nx, ny, p.g, p.h_min, incoming, Cons{bh, bhu, bhv}, qb.h,
                            qb.hu, qb.hv, qb.b);
     } else {
+        // this lane's face is the edge's left side (or a wall's, the ghost on the right)
         const bool wall = l == LINK_BOUNDARY;
-        const bool left = wall || (l & 4u);
-        const bool both = !wall && !alone;  // an edge carried from its left side for both elements
-        // the other side: the neighbour's face node, or a wall's reflected ghost
         size_t slot_o = 0;
         double xh = 0.0, xhu = 0.0, xhv = 0.0, xb = 0.0;
         Cons fso = {0.0, 0.0, 0.0};
-        double lx = nx, ly = ny;
         if (wall) {
             const Cons gs = wall_ghost(q.h, q.hu, q.hv, nx, ny, p.h_min);
             xh = gs.h; xhu = gs.hu; xhv = gs.hv; xb = q.b;
@@ -666,50 +657,70 @@ __global__ __launch_bounds__(64) DG_WAVES_PER_EU(3) void edge_vjp_kernel(VjpPara
             const size_t m = (size_t)(l >> 4) * NN + face_node<N>(fo, fio);
             xh = p.ih[m]; xhu = p.ihu[m]; xhv = p.ihv[m]; xb = p.bed[m];
             slot_o = (size_t)(l >> 4) * NF + fo * N1 + fio;
-            if (both) {
-                const double* fco = P.facecot + slot_o * 3;
-                fso = {fco[0], fco[1], fco[2]};
-            }
-            if (!left) {
-                // the right side's normal: its own reversed, or on a periodic pair the left's own
-                lx = -nx; ly = -ny;
-                if (!(l & 8u)) {
-                    const size_t fq = (size_t)(l >> 4) * 4 + (l & 3u);
-                    lx = p.fgeo[fq]; ly = p.fgeo[nf4 + fq];
-                }
-            }
+            const double* fco = P.facecot + slot_o * 3;
+            fso = {fco[0], fco[1], fco[2]};
         }
         const Node qo = make_node(xh, xhu, xhv, xb, p.h_min);
-        const Side sa = left ? side(q) : side(qo), sbs = left ? side(qo) : side(q);
-        // the cotangents of F*_a and F*_b (per-field selects: a conditional
-        // between two struct lvalues would put them in memory)
-        const Cons ca = {left ? fsb.h : fso.h, left ? fsb.hu : fso.hu, left ? fsb.hv : fso.hv};
-        const Cons cb = {left ? fso.h : fsb.h, left ? fso.hu : fsb.hu, left ? fso.hv : fsb.hv};
-        const Cons fab = ca;
-        const Cons fbb = {-cb.h, -cb.hu, -cb.hv};
+        const Cons fbb = {-fso.h, -fso.hu, -fso.hv};
         SideBar sab = sbar_zero(), sbb = sbar_zero();
-        hydro_hll_adj(sa, sbs, lx, ly, p.g, p.h_min, fab, fbb, sab, sbb);
-        const SideBar mine = {left ? sab.h : sbb.h, left ? sab.u : sbb.u, left ? sab.v : sbb.v,
-                              left ? sab.b : sbb.b};
-        const SideBar os = {left ? sbb.h : sab.h, left ? sbb.u : sab.u, left ? sbb.v : sab.v,
-                            left ? sbb.b : sab.b};
-        nbar_add_side(qb, mine);
-        if (wall || both) {
-            const NodeBar ob = {os.h, 0.0, 0.0, os.u, os.v, os.b};
-            double oh = 0.0, ohu = 0.0, ohv = 0.0, obb = 0.0;
-            make_node_adj(xh, xhu, xhv, p.h_min, ob, oh, ohu, ohv, obb);
-            if (wall) {
-                qb.b += obb;
-                wall_ghost_adj(q.h, q.hu, q.hv, nx, ny, p.h_min, Cons{oh, ohu, ohv}, qb.h, qb.hu, qb.hv);
-            } else {
-                double* o = P.facebuf + slot_o * 4;
-                o[0] = oh; o[1] = ohu; o[2] = ohv; o[3] = obb;
-            }
+        hydro_hll_adj(side(q), side(qo), nx, ny, p.g, p.h_min, fsb, fbb, sab, sbb);
+        nbar_add_side(qb, sab);
+        const NodeBar ob = {sbb.h, 0.0, 0.0, sbb.u, sbb.v, sbb.b};
+        double oh = 0.0, ohu = 0.0, ohv = 0.0, obb = 0.0;
+        make_node_adj(xh, xhu, xhv, p.h_min, ob, oh, ohu, ohv, obb);
+        if (wall) {
+            qb.b += obb;
+            wall_ghost_adj(q.h, q.hu, q.hv, nx, ny, p.h_min, Cons{oh, ohu, ohv}, qb.h, qb.hu, qb.hv);
+        } else {
+            double* o = P.facebuf + slot_o * 4;
+            o[0] = oh; o[1] = ohu; o[2] = ohv; o[3] = obb;
         }
     }
     double gh = 0.0, ghu = 0.0, ghv = 0.0, gb = 0.0;
     make_node_adj(h, hu, hv, p.h_min, qb, gh, ghu, ghv, gb);
     double* o = P.facebuf + ((size_t)k * NF + f * N1 + fi) * 4;
+    o[0] = gh; o[1] = ghu; o[2] = ghv; o[3] = gb;
+}
+
+// The edges whose two normals differ (each element solved the edge with its
+// own: the left side with its normal for F*_a, the right side with its own
+// reversed for F*_b): both solves backwards, one lane per edge node, each with
+// its own element's cotangent, the two sides' parts summed, then both face
+// nodes' slots written. Not on a conforming mesh.
+template <int N>
+__global__ __launch_bounds__(64) void edge_vjp_split_kernel(VjpParams P, const uint32_t* edges, uint32_t n_edges) {
+    constexpr int N1 = N + 1, NN = N1 * N1, NF = 4 * N1;
+    const StageParams& p = P.s;
+    const size_t gid = (size_t)blockIdx.x * 64 + threadIdx.x;
+    if (gid >= (size_t)n_edges * N1) return;
+    const uint32_t ed = edges[gid / N1];
+    const int fi = (int)(gid % N1);
+    const int64_t k = (int64_t)(ed >> 2);
+    const int f = (int)(ed & 3u);
+    const uint32_t l = p.links[(size_t)k * 4 + f];
+    const int64_t kr = (int64_t)(l >> 4);
+    const int fr = (int)(l & 3u), fir = N1 - 1 - fi;
+    const size_t nl = (size_t)k * NN + face_node<N>(f, fi), nr = (size_t)kr * NN + face_node<N>(fr, fir);
+    const size_t sl = (size_t)k * NF + f * N1 + fi, sr = (size_t)kr * NF + fr * N1 + fir;
+    const size_t nf4 = (size_t)p.n_elements * 4, fl = (size_t)k * 4 + f, fq = (size_t)kr * 4 + fr;
+    const double lh = p.ih[nl], lhu = p.ihu[nl], lhv = p.ihv[nl], rh = p.ih[nr], rhu = p.ihu[nr], rhv = p.ihv[nr];
+    const Side sa = side(make_node(lh, lhu, lhv, p.bed[nl], p.h_min));
+    const Side sb = side(make_node(rh, rhu, rhv, p.bed[nr], p.h_min));
+    const double* cl = P.facecot + sl * 3;
+    const double* cr = P.facecot + sr * 3;
+    const Cons zero = {0.0, 0.0, 0.0};
+    SideBar a1 = sbar_zero(), b1 = sbar_zero(), a2 = sbar_zero(), b2 = sbar_zero();
+    hydro_hll_adj(sa, sb, p.fgeo[fl], p.fgeo[nf4 + fl], p.g, p.h_min, Cons{cl[0], cl[1], cl[2]}, zero, a1, b1);
+    hydro_hll_adj(sa, sb, -p.fgeo[fq], -p.fgeo[nf4 + fq], p.g, p.h_min, zero, Cons{-cr[0], -cr[1], -cr[2]}, a2, b2);
+    const NodeBar ql = {a1.h + a2.h, 0.0, 0.0, a1.u + a2.u, a1.v + a2.v, a1.b + a2.b};
+    const NodeBar qr = {b1.h + b2.h, 0.0, 0.0, b1.u + b2.u, b1.v + b2.v, b1.b + b2.b};
+    double gh = 0.0, ghu = 0.0, ghv = 0.0, gb = 0.0;
+    make_node_adj(lh, lhu, lhv, p.h_min, ql, gh, ghu, ghv, gb);
+    double* o = P.facebuf + sl * 4;
+    o[0] = gh; o[1] = ghu; o[2] = ghv; o[3] = gb;
+    gh = ghu = ghv = gb = 0.0;
+    make_node_adj(rh, rhu, rhv, p.h_min, qr, gh, ghu, ghv, gb);
+    o = P.facebuf + sr * 4;
     o[0] = gh; o[1] = ghu; o[2] = ghv; o[3] = gb;
 }
 

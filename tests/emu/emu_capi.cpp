@@ -315,13 +315,22 @@ void emu_stage_vjp(int order, int64_t K, const double* corners, const uint32_t* 
         case 2: emu_launch(sub_grid, 64, [&] { subcell_vjp_kernel<2>(P, list.data(), &lcount); }); break;
         default: emu_launch(sub_grid, 64, [&] { subcell_vjp_kernel<3>(P, list.data(), &lcount); }); break;
     }
-    const std::vector<uint32_t> edges = vjp_edge_list(links, fgeo.data(), fgeo.data() + (size_t)K * 4, K);
-    const uint32_t ne = (uint32_t)edges.size();
+    std::vector<uint32_t> edges, split;
+    vjp_edge_lists(links, fgeo.data(), fgeo.data() + (size_t)K * 4, K, edges, split);
+    const uint32_t ne = (uint32_t)edges.size(), ns = (uint32_t)split.size();
     unsigned fgrid = (unsigned)(((size_t)ne * (order + 1) + 63) / 64);
     switch (order) {
         case 1: emu_launch(fgrid, 64, [&] { edge_vjp_kernel<1>(P, edges.data(), ne); }); break;
         case 2: emu_launch(fgrid, 64, [&] { edge_vjp_kernel<2>(P, edges.data(), ne); }); break;
         default: emu_launch(fgrid, 64, [&] { edge_vjp_kernel<3>(P, edges.data(), ne); }); break;
+    }
+    if (ns) {
+        unsigned sgrid = (unsigned)(((size_t)ns * (order + 1) + 63) / 64);
+        switch (order) {
+            case 1: emu_launch(sgrid, 64, [&] { edge_vjp_split_kernel<1>(P, split.data(), ns); }); break;
+            case 2: emu_launch(sgrid, 64, [&] { edge_vjp_split_kernel<2>(P, split.data(), ns); }); break;
+            default: emu_launch(sgrid, 64, [&] { edge_vjp_split_kernel<3>(P, split.data(), ns); }); break;
+        }
     }
     unsigned ggrid = (unsigned)((nt + 255) / 256);
     switch (order) {

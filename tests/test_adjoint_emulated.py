@@ -109,3 +109,29 @@ def test_theta_limiter_adjoint():
     res = dot_test(c, base, base, 1.0, 0.0, cc, 0.0, False, 5)
     for which, (lhs, rhs) in res.items():
         assert abs(lhs - rhs) <= TOL * abs(lhs), (which, lhs, rhs)
+
+
+
+def test_edges_whose_normals_differ_against_the_kernels_differences():
+    """An edge whose two elements hold its shared vertex at slightly different
+    places: their face normals differ, each element solves the edge's Riemann
+    problem with its own normal, and the edge kernel runs that edge from each
+    side alone (the others once for both sides). The restatement takes the
+    left side's normal for both (the reference's one flux per edge), so the
+    check here is against central differences of the emulated stage itself: the
+    stage's RHS along a random input direction on the wet estuary."""
+    c = cases.tidal_estuary(nx=6, ny=3)
+    c.mesh.corners[7, 2] += (3.0, -2.0)            # an interior element's third vertex, moved for it alone
+    emu = EmulatedKernels(c)
+    base = perturbed(c, seed=4)
+    n = base[0].size
+    rng = np.random.default_rng(21)
+    w = [rng.standard_normal(n) for _ in range(3)]
+    d = [rng.standard_normal(n) * s for s in (1e-3, 1e-2, 1e-2)]
+    _, gi, _, _ = emu.stage_vjp(base, base, w, 0.0, 0.0, 0.0, 0.0, rhs_only=True)
+    eps = 1e-4
+    plus = emu.stage(base, [x + eps * y for x, y in zip(base, d)], 0.0, 0.0, 0.0, 0.0, rhs_only=True)
+    minus = emu.stage(base, [x - eps * y for x, y in zip(base, d)], 0.0, 0.0, 0.0, 0.0, rhs_only=True)
+    fd = _dot(w, [(p - m) / (2 * eps) for p, m in zip(plus, minus)])
+    adj = _dot(gi, d)
+    assert abs(fd - adj) <= 1e-6 * abs(adj), (fd, adj)

@@ -46,7 +46,15 @@ def _gpu_stage_vjp(phys, base, inp, w, a, b, cc, time, rhs_only=False):
     return gb.download(), gi.download(), gbed, gman
 
 
-@pytest.mark.parametrize("name,make", STAGE_CASES)
+def _estuary_with_unmatched_normals():
+    # an interior element's vertex moved for it alone: the edges at that vertex have two
+    # normals, and the library runs them through its split-edge kernel
+    c = cases.tidal_estuary(nx=6, ny=3)
+    c.mesh.corners[7, 2] += (3.0, -2.0)
+    return c
+
+
+@pytest.mark.parametrize("name,make", STAGE_CASES + [("estuary_unmatched_normals", _estuary_with_unmatched_normals)])
 def test_stage_vjp_matches_emulated_kernels_bitwise(name, make):
     c = make()
     base, inp, dt = _stage_inputs(c)
