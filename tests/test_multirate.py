@@ -460,3 +460,8 @@ def test_library_assigner_on_the_refined_tidal_channel():
     st = M.MultirateStepper(8, c.scheme)
     assert st.assign_levels(orc, nb, x, c.cfl, None, 900.0) == dt
     assert np.array_equal(st.level, level) and np.array_equal(np.stack(st.rate), rates)
+    # through the device half (emulated) and the host remainder on its threads (64,000 elements: two parts)
+    (d_dt, d_stable, d_finest, d_level, d_rates), = multirate_assign(c.mesh.links, 3, 8, 4, True, [edt], [flags], True,
+                                                                     [None], [900.0], device=True)
+    assert (d_dt, d_stable, d_finest) == (dt, stable, finest)
+    assert np.array_equal(d_level, level) and np.array_equal(d_rates, rates)
