@@ -13,13 +13,14 @@ import pytest
 
 from dgcom_amd import cases
 import oracle as O
-from cases_for_tests import perturbed, small_unstructured
+from cases_for_tests import periodic_distorted, perturbed, small_unstructured
 from emu_bridge import EmulatedKernels
 
 TOL = 1e-12
 
 STAGE_CASES = [
     ("estuary_n2", lambda: cases.tidal_estuary(nx=6, ny=3)),
+    ("periodic_n2", lambda: periodic_distorted(2)),
     ("beach_n1", lambda: cases.drying_beach(order=1, nx=8, ny=3)),
     ("beach_n2", lambda: cases.drying_beach(order=2, nx=8, ny=3)),
     ("beach_n3_jitter", lambda: cases.drying_beach(order=3, nx=8, ny=3, jitter=0.2)),
