@@ -6,6 +6,7 @@
 // over the device's element-subset stages. Included by dgswe.hip (host code).
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -92,8 +93,10 @@ MR_HD int element_level(double edt, double dt, int bound) {
     return l;
 }
 
-// body(begin, end, part) over contiguous parts of [0, n) on up to 16 host
-// threads (one below 32768 items); part in [0, parts())
+// The host loops over elements on threads: host_parts(n) parts of [0, n) (up to
+// 16, one per 32768 items, at most the hardware's threads), and host_parallel
+// runs body(begin, end, part) on each contiguous part, one thread per part.
+// Every loop run this way writes each element's own results only.
 inline int host_parts(size_t n) {
     const unsigned hw = std::thread::hardware_concurrency();
     const size_t by_size = n / 32768 + 1;
