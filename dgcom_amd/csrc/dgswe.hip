@@ -107,7 +107,8 @@ struct dg_ctx {
     unsigned long long* clips = nullptr;
     uint32_t* queue = nullptr;
     uint8_t* queue_count = nullptr;
-    double* dt_value = nullptr;
+    double* dt_host = nullptr;   // pinned, mapped: the time step, written by slots_min_kernel itself
+    double* dt_value = nullptr;  // its device address
     unsigned long long* dt_slots = nullptr;  // [DT_SLOTS]: the time step's running minima (dt_kernel)
     uint32_t* sub_list = nullptr;
     unsigned int *chunk_total = nullptr, *sub_count = nullptr;
@@ -472,7 +473,9 @@ static int launch_stage(dg_ctx* c, const double* base, const double* input, doub
         case 2: hipLaunchKernelGGL((force ? wet_stage_kernel<2, true> : wet_stage_kernel<2>), grid, dim3(64), 0, c->stream, p); break;
         default: hipLaunchKernelGGL((force ? wet_stage_kernel<3, true> : wet_stage_kernel<3>), grid, dim3(64), 0, c->stream, p); break;
     }
-    hipLaunchKernelGGL(chunk_count_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, nblk, c->chunk_total);
+    // (one chunk: the compaction's own scan is the whole offset, no chunk totals)
+    if (nchunk > 1)
+        hipLaunchKernelGGL(chunk_count_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, nblk, c->chunk_total);
     hipLaunchKernelGGL(compact_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, c->queue, nblk,
                        c->chunk_total, epb, c->sub_list, c->sub_count);
     switch (c->order) {
@@ -529,9 +532,8 @@ static int launch_dt(dg_ctx* c, const double* state, double cfl, int scheme, boo
     if (c->timing) DG_TRY(hipEventRecord(c->ev[0], c->stream));
     if (int r = launch_dt_kernels(c, state, cfl, scheme, element_out)) return r;
     if (c->timing) DG_TRY(hipEventRecord(c->ev[1], c->stream));
-    double value = 0.0;
-    DG_TRY(hipMemcpyAsync(&value, c->dt_value, 8, hipMemcpyDeviceToHost, c->stream));
     DG_TRY(hipStreamSynchronize(c->stream));
+    const double value = *(volatile double*)c->dt_host;
     if (c->timing) {
         float ms = 0.f;
         DG_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
@@ -739,7 +741,9 @@ static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, 
         case 2: hipLaunchKernelGGL(wet_vjp_kernel<2>, grid, dim3(64), 0, c->stream, P); break;
         default: hipLaunchKernelGGL(wet_vjp_kernel<3>, grid, dim3(64), 0, c->stream, P); break;
     }
-    hipLaunchKernelGGL(chunk_count_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, nblk, c->chunk_total);
+    // (one chunk: the compaction's own scan is the whole offset, no chunk totals)
+    if (nchunk > 1)
+        hipLaunchKernelGGL(chunk_count_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, nblk, c->chunk_total);
     hipLaunchKernelGGL(compact_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, c->queue, nblk,
                        c->chunk_total, epb, c->sub_list, c->sub_count);
     switch (c->order) {
@@ -1024,7 +1028,9 @@ static int batch_stage_order(dg_batch* b, int o, int q, int parity) {
         case 2: hipLaunchKernelGGL(batch_wet_kernel<2>, grid, dim3(64), 0, st, A); break;
         default: hipLaunchKernelGGL(batch_wet_kernel<3>, grid, dim3(64), 0, st, A); break;
     }
-    hipLaunchKernelGGL(chunk_count_kernel, dim3(nchunk), dim3(256), 0, st, B.queue_count, B.n_blocks, B.chunk_total);
+    // (one chunk: the compaction's own scan is the whole offset, no chunk totals)
+    if (nchunk > 1)
+        hipLaunchKernelGGL(chunk_count_kernel, dim3(nchunk), dim3(256), 0, st, B.queue_count, B.n_blocks, B.chunk_total);
     hipLaunchKernelGGL(batch_compact_kernel, dim3(nchunk), dim3(256), 0, st, B.queue_count, B.queue, B.block_member,
                        B.n_blocks, B.chunk_total, epb, B.list, B.list_member, B.list_count);
     switch (o) {
@@ -1159,7 +1165,10 @@ int dg_create(int device, dg_ctx** out) {
         tables = true;
     }
     DG_TRY(hipMalloc(&c->clips, sizeof(unsigned long long)));
-    DG_TRY(hipMalloc(&c->dt_value, sizeof(double)));
+    // (the time step's one value goes to the host without a copy launch: the
+    // reduction kernel stores it in pinned memory the host reads after the sync)
+    DG_TRY(hipHostMalloc((void**)&c->dt_host, sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+    DG_TRY(hipHostGetDevicePointer((void**)&c->dt_value, c->dt_host, 0));
     DG_TRY(hipMalloc(&c->sub_count, sizeof(unsigned int)));
     DG_TRY(hipMalloc(&c->flag, sizeof(int)));
     DG_TRY(hipMemset(c->clips, 0, sizeof(unsigned long long)));
@@ -1176,7 +1185,7 @@ int dg_destroy(dg_ctx* c) {
     DG_TRY(hipStreamSynchronize(c->stream));
     if (int r = free_statics(c)) return r;
     DG_TRY(hipFree(c->clips));
-    DG_TRY(hipFree(c->dt_value));
+    DG_TRY(hipHostFree(c->dt_host));
     DG_TRY(hipFree(c->sub_count));
     DG_TRY(hipFree(c->flag));
     DG_TRY(hipEventDestroy(c->ev[0]));
@@ -1571,7 +1580,9 @@ int dg_stage_where(dg_ctx* c, const dg_state* base, const dg_state* input, const
         case 2: hipLaunchKernelGGL((force ? wet_lts_kernel<2, true> : wet_lts_kernel<2>), grid, dim3(64), 0, c->stream, p); break;
         default: hipLaunchKernelGGL((force ? wet_lts_kernel<3, true> : wet_lts_kernel<3>), grid, dim3(64), 0, c->stream, p); break;
     }
-    hipLaunchKernelGGL(chunk_count_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, nblk, c->chunk_total);
+    // (one chunk: the compaction's own scan is the whole offset, no chunk totals)
+    if (nchunk > 1)
+        hipLaunchKernelGGL(chunk_count_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, nblk, c->chunk_total);
     hipLaunchKernelGGL(compact_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, c->queue, nblk,
                        c->chunk_total, epb, c->sub_list, c->sub_count);
     switch (c->order) {

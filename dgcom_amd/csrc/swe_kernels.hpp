@@ -1144,7 +1144,8 @@ __global__ __launch_bounds__(256) void chunk_count_kernel(const uint8_t* counts,
 
 // Exclusive prefix sum over the 256 threads of a workgroup (Hillis-Steele in
 // LDS) and the sum of chunk_total[0..blockIdx.x), each thread a share: the
-// list offsets of a compaction chunk (integer sums: any order is exact)
+// list offsets of a compaction chunk (integer sums: any order is exact). The
+// first chunk reads no totals: a one-chunk compaction needs no count launch.
 __device__ __forceinline__ void chunk_offsets(const unsigned int* chunk_total, unsigned int s, unsigned int& base,
                                               unsigned int& excl, unsigned int& chunk_sum) {
     __shared__ unsigned int scan[256], part[256];

@@ -40,7 +40,7 @@ static void run_stage(int order, int64_t K, StageParams& p) {
     unsigned nchunk = (nblk + 1023) / 1024;
     std::vector<uint32_t> queue((size_t)nblk * epb), list((size_t)K);
     std::vector<uint8_t> qcount(nblk);
-    std::vector<unsigned int> ctot(nchunk);
+    std::vector<unsigned int> ctot(nchunk, ~0u);  // unread by a one-chunk compaction
     unsigned int lcount = 0;
     p.queue = queue.data();
     p.queue_count = qcount.data();
@@ -51,7 +51,7 @@ static void run_stage(int order, int64_t K, StageParams& p) {
         case 2: emu_launch(grid, 64, [&] { wet_stage_kernel<2>(p); }); break;
         default: emu_launch(grid, 64, [&] { wet_stage_kernel<3>(p); }); break;
     }
-    emu_launch(nchunk, 256, [&] { chunk_count_kernel(qcount.data(), nblk, ctot.data()); });
+    if (nchunk > 1) emu_launch(nchunk, 256, [&] { chunk_count_kernel(qcount.data(), nblk, ctot.data()); });
     emu_launch(nchunk, 256, [&] { compact_kernel(qcount.data(), queue.data(), nblk, ctot.data(), epb, list.data(), &lcount); });
     switch (order) {
         case 1: emu_launch(sub_grid, 64, [&] { subcell_stage_kernel<1>(p, list.data(), &lcount); }); break;
@@ -115,7 +115,7 @@ unsigned long long emu_stage_where(int order, int64_t K, const double* corners, 
     unsigned nchunk = (nblk + 1023) / 1024;
     std::vector<uint32_t> queue((size_t)nblk * epb), list((size_t)K);
     std::vector<uint8_t> qcount(nblk);
-    std::vector<unsigned int> ctot(nchunk);
+    std::vector<unsigned int> ctot(nchunk, ~0u);  // unread by a one-chunk compaction
     unsigned int lcount = 0;
     p.queue = queue.data();
     p.queue_count = qcount.data();
@@ -126,7 +126,7 @@ unsigned long long emu_stage_where(int order, int64_t K, const double* corners, 
         case 2: emu_launch(grid, 64, [&] { wet_lts_kernel<2>(p); }); break;
         default: emu_launch(grid, 64, [&] { wet_lts_kernel<3>(p); }); break;
     }
-    emu_launch(nchunk, 256, [&] { chunk_count_kernel(qcount.data(), nblk, ctot.data()); });
+    if (nchunk > 1) emu_launch(nchunk, 256, [&] { chunk_count_kernel(qcount.data(), nblk, ctot.data()); });
     emu_launch(nchunk, 256, [&] { compact_kernel(qcount.data(), queue.data(), nblk, ctot.data(), epb, list.data(), &lcount); });
     switch (order) {
         case 1: emu_launch(sub_grid, 64, [&] { subcell_stage_kernel<1, true>(p, list.data(), &lcount); }); break;
@@ -175,7 +175,7 @@ unsigned long long emu_stage_where_forced(int order, int64_t K, const double* co
     unsigned nchunk = (nblk + 1023) / 1024;
     std::vector<uint32_t> queue((size_t)nblk * epb), list((size_t)K);
     std::vector<uint8_t> qcount(nblk);
-    std::vector<unsigned int> ctot(nchunk);
+    std::vector<unsigned int> ctot(nchunk, ~0u);  // unread by a one-chunk compaction
     unsigned int lcount = 0;
     p.queue = queue.data();
     p.queue_count = qcount.data();
@@ -186,7 +186,7 @@ unsigned long long emu_stage_where_forced(int order, int64_t K, const double* co
         case 2: emu_launch(grid, 64, [&] { wet_lts_kernel<2, true>(p); }); break;
         default: emu_launch(grid, 64, [&] { wet_lts_kernel<3, true>(p); }); break;
     }
-    emu_launch(nchunk, 256, [&] { chunk_count_kernel(qcount.data(), nblk, ctot.data()); });
+    if (nchunk > 1) emu_launch(nchunk, 256, [&] { chunk_count_kernel(qcount.data(), nblk, ctot.data()); });
     emu_launch(nchunk, 256, [&] { compact_kernel(qcount.data(), queue.data(), nblk, ctot.data(), epb, list.data(), &lcount); });
     switch (order) {
         case 1: emu_launch(sub_grid, 64, [&] { subcell_stage_kernel<1, true, true>(p, list.data(), &lcount); }); break;
@@ -297,7 +297,7 @@ void emu_stage_vjp(int order, int64_t K, const double* corners, const uint32_t* 
     unsigned nchunk = (nblk + 1023) / 1024;
     std::vector<uint32_t> queue((size_t)nblk * epb), list((size_t)K);
     std::vector<uint8_t> qcount(nblk);
-    std::vector<unsigned int> ctot(nchunk);
+    std::vector<unsigned int> ctot(nchunk, ~0u);  // unread by a one-chunk compaction
     unsigned int lcount = 0;
     p.queue = queue.data();
     p.queue_count = qcount.data();
@@ -308,7 +308,7 @@ void emu_stage_vjp(int order, int64_t K, const double* corners, const uint32_t* 
         case 2: emu_launch(grid, 64, [&] { wet_vjp_kernel<2>(P); }); break;
         default: emu_launch(grid, 64, [&] { wet_vjp_kernel<3>(P); }); break;
     }
-    emu_launch(nchunk, 256, [&] { chunk_count_kernel(qcount.data(), nblk, ctot.data()); });
+    if (nchunk > 1) emu_launch(nchunk, 256, [&] { chunk_count_kernel(qcount.data(), nblk, ctot.data()); });
     emu_launch(nchunk, 256, [&] { compact_kernel(qcount.data(), queue.data(), nblk, ctot.data(), epb, list.data(), &lcount); });
     switch (order) {
         case 1: emu_launch(sub_grid, 64, [&] { subcell_vjp_kernel<1>(P, list.data(), &lcount); }); break;
