@@ -92,8 +92,10 @@ def cpu_baseline(case, seconds_target: float):
 def adjoint_bench(n, n_steps, snapshots, every, n_stations):
     """The adjoint workload: wall time of the station-misfit gradient (forward
     sweep + reverse sweep with recomputation from `snapshots` checkpoints) over
-    the wall time of one plain forward run of the same steps, and the reverse
-    sweep alone over the forward when every state is kept (per-step VJP cost)."""
+    the wall time of one plain forward run of the same steps, the reverse
+    sweep alone over the forward when every state is kept (per-step VJP cost),
+    and the same with step records (every step's stage values and RHS kept: no
+    stage recomputed) over the first 400 steps against their plain forward."""
     from dgcom_amd import inversion
 
     t0 = time.time()
@@ -128,6 +130,21 @@ def adjoint_bench(n, n_steps, snapshots, every, n_stations):
         full.close()
         per_step = {"forward_with_all_checkpoints_s": f2, "reverse_sweep_s": b2, "ratio": b2 / fwd_s,
                     "gradient_over_forward_keeping_every_state": (f2 + b2) / fwd_s}
+    # step records (every step's stage values and RHS kept, nothing recomputed) over
+    # the first steps that fit HBM at 6 states per step
+    n_rec = min(n_steps, 400)
+    dts_r, obs_r = inv.dts[:n_rec], inv.obs[: n_rec // every]
+    short = phys.tape(1)
+    _, fr = timed(lambda: short.forward(x0, 0.0, dts_r, every, obs_r))
+    _, fr = timed(lambda: short.forward(x0, 0.0, dts_r, every, obs_r))
+    short.close()
+    rec = phys.tape(n_rec + 1)
+    rec.keep_stages()
+    _, f3 = timed(lambda: rec.forward(x0, 0.0, dts_r, every, obs_r))
+    _, b3 = timed(lambda: rec.backward(1.0))
+    rec.close()
+    records = {"steps": n_rec, "forward_s": fr, "forward_keeping_stages_s": f3, "reverse_sweep_s": b3,
+               "ratio": b3 / fr, "gradient_over_forward": (f3 + b3) / fr}
     K, nn = c.n_elements, c.nn
     return {"workload": f"bathymetry inversion: {K} jittered quads ({n}x{n} over 50 km), N={c.order}, GRF truth bed, "
                         f"prior = truth + GRF(sd 2 m), {n_stations} stations every {every} steps, {n_steps} SSP-RK3 "
@@ -137,7 +154,7 @@ def adjoint_bench(n, n_steps, snapshots, every, n_stations):
             "gradient_forward_sweep_s": f1, "gradient_reverse_sweep_s": b1,
             "forward_steps": fsteps, "step_vjps": vjps, "misfit": Jg,
             "grad_norms": {"bed": float(np.linalg.norm(gbed)), "manning": float(np.linalg.norm(gman))},
-            "per_step": per_step, "setup_s": setup}
+            "per_step": per_step, "step_records": records, "setup_s": setup}
 
 
 def adjoint_ensemble_bench(n_members, n_steps, snapshots):

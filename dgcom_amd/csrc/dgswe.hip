@@ -775,29 +775,35 @@ static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, 
 }
 
 // The VJP of one SSP step from state X: the stages are recomputed, saving
-// their RHS, then reversed; lam = (dX'/dX)^T lam_next (overwritten).
+// their RHS, then reversed; lam = (dX'/dX)^T lam_next (overwritten). With a
+// step record (the step's stage values 0..ns-2, then its ns stage RHS, as the
+// forward kept them) nothing is recomputed.
 static int step_vjp_impl(dg_ctx* c, const double* X, double t, double dt, int scheme, const double* lam_next,
-                         double* lam) {
+                         double* lam, const double* record = nullptr) {
     if (int r = ensure_work(c)) return r;
     if (int r = ensure_adjoint(c)) return r;
     const int ns = scheme == DG_SSP_RK3 ? 3 : 4;
     const double(*st)[4] = scheme == DG_SSP_RK3 ? RK3 : RK43;
     const size_t n3 = (size_t)3 * c->K * c->nn;
     const double* input = X;
-    for (int q = 0; q < ns; ++q) {
-        if (int r = launch_stage(c, X, input, c->work[q]->d, st[q][0], st[q][1], st[q][2] * dt, t + st[q][3] * dt, 0,
-                                 nullptr, c->rhsbuf[q]->d))
-            return r;
-        input = c->work[q]->d;
+    if (!record) {
+        for (int q = 0; q < ns; ++q) {
+            if (int r = launch_stage(c, X, input, c->work[q]->d, st[q][0], st[q][1], st[q][2] * dt, t + st[q][3] * dt,
+                                     0, nullptr, c->rhsbuf[q]->d))
+                return r;
+            input = c->work[q]->d;
+        }
     }
+    auto stage_value = [&](int q) -> const double* { return record ? record + (size_t)q * n3 : c->work[q]->d; };
+    auto stage_rhs = [&](int q) -> const double* { return record ? record + (size_t)(ns - 1 + q) * n3 : c->rhsbuf[q]->d; };
     if (int r = zero_async(c, lam, n3)) return r;
     const double* w = lam_next;
     int tb = 0;
     for (int q = ns - 1; q >= 0; --q) {
         // the stage values' cotangents are fresh buffers, written (not added to)
-        const double* inq = q == 0 ? X : c->work[q - 1]->d;
+        const double* inq = q == 0 ? X : stage_value(q - 1);
         double* gin = q > 0 ? c->adj[tb]->d : lam;
-        if (int r = launch_stage_vjp(c, X, inq, c->rhsbuf[q]->d, w, lam, gin, st[q][0], st[q][1], st[q][2] * dt,
+        if (int r = launch_stage_vjp(c, X, inq, stage_rhs(q), w, lam, gin, st[q][0], st[q][1], st[q][2] * dt,
                                      t + st[q][3] * dt, 0, q > 0 ? 1 : 0))
             return r;
         w = gin;
@@ -874,7 +880,34 @@ struct dg_tape {
     std::vector<std::pair<int, int>> pre;  // (step, slot) kept by the forward sweep
     int lam_cur = 0;
     double scale = 1.0;
+    // Step records (dg_tape_keep_stages): per step its stage values 0..ns-2 and
+    // its ns stage RHS, kept by the forward so that the reverse recomputes nothing
+    bool keep_stages = false, records = false;
+    double* rec = nullptr;
+    size_t rec_bytes = 0;
+    int rec_per_step = 0;
 };
+
+// One forward step of the tape keeping its record: step_impl's stages with
+// the stage values and RHS written into step i's record
+static int record_step(dg_ctx* c, dg_tape* tp, int i) {
+    if (int r = ensure_work(c)) return r;
+    const int ns = tp->scheme == DG_SSP_RK3 ? 3 : 4;
+    const double(*st)[4] = tp->scheme == DG_SSP_RK3 ? RK3 : RK43;
+    const size_t n3 = (size_t)3 * c->K * c->nn;
+    double* rec = tp->rec + (size_t)i * tp->rec_per_step * n3;
+    const double t = tp->t_at[i], dt = tp->dts[i];
+    const double* input = tp->cur->d;
+    for (int q = 0; q < ns; ++q) {
+        double* out = q < ns - 1 ? rec + (size_t)q * n3 : c->work[ns - 1]->d;
+        if (int r = launch_stage(c, tp->cur->d, input, out, st[q][0], st[q][1], st[q][2] * dt, t + st[q][3] * dt, 0,
+                                 nullptr, rec + (size_t)(ns - 1 + q) * n3))
+            return r;
+        input = out;
+    }
+    std::swap(tp->cur->d, c->work[ns - 1]->d);  // the tape's own state buffer
+    return 0;
+}
 
 static int tape_free_slot(dg_tape* tp) {
     for (size_t i = 0; i < tp->slot.size(); ++i)
@@ -911,7 +944,8 @@ static int tape_step_back(dg_ctx* c, dg_tape* tp, const dg_state* xs, int i) {
         }
     }
     double* out = tp->lam[tp->lam_cur ^ 1]->d;
-    if (int r = step_vjp_impl(c, xs->d, tp->t_at[i], tp->dts[i], tp->scheme, lam, out)) return r;
+    const double* record = tp->records ? tp->rec + (size_t)i * tp->rec_per_step * 3 * c->K * c->nn : nullptr;
+    if (int r = step_vjp_impl(c, xs->d, tp->t_at[i], tp->dts[i], tp->scheme, lam, out, record)) return r;
     tp->lam_cur ^= 1;
     tp->vjps += 1;
     return 0;
@@ -2490,7 +2524,23 @@ int dg_tape_destroy(dg_ctx* c, dg_tape* tp) {
     if (int r = fr(tp->lam[1])) return r;
     if (tp->eta) DG_TRY(hipFree(tp->eta));
     if (tp->obs) DG_TRY(hipFree(tp->obs));
+    if (tp->rec) DG_TRY(hipFree(tp->rec));
     delete tp;
+    return 0;
+}
+
+int dg_tape_keep_stages(dg_ctx* c, dg_tape* tp, int on) {
+    DG_LOCK(c);
+    DG_REQUIRE(c && tp, "dg_tape_keep_stages: null argument");
+    DG_TRY(hipSetDevice(c->device));
+    tp->keep_stages = on != 0;
+    tp->ready = false;
+    if (!tp->keep_stages && tp->rec) {
+        DG_TRY(hipStreamSynchronize(c->stream));
+        DG_TRY(hipFree(tp->rec));
+        tp->rec = nullptr;
+        tp->rec_bytes = 0;
+    }
     return 0;
 }
 
@@ -2519,6 +2569,25 @@ int dg_misfit_forward(dg_ctx* c, dg_tape* tp, const dg_state* x0, double t0, int
     if (tp->eta) DG_TRY(hipFree(tp->eta));
     if (tp->obs) DG_TRY(hipFree(tp->obs));
     tp->eta = tp->obs = nullptr;
+    tp->records = false;
+    if (tp->keep_stages) {
+        DG_REQUIRE(tp->snapshots >= n_steps + 1,
+                   "dg_misfit_forward: keeping the stages needs a snapshot per step (snapshots >= n_steps + 1)");
+        const int per = scheme == DG_SSP_RK3 ? 5 : 7;
+        const size_t need = (size_t)n_steps * per * 3 * c->K * c->nn * sizeof(double);
+        if (tp->rec_bytes != need) {
+            if (tp->rec) {
+                DG_TRY(hipStreamSynchronize(c->stream));
+                DG_TRY(hipFree(tp->rec));
+                tp->rec = nullptr;
+                tp->rec_bytes = 0;
+            }
+            DG_TRY(hipMalloc(&tp->rec, need));
+            tp->rec_bytes = need;
+        }
+        tp->rec_per_step = per;
+        tp->records = true;
+    }
     const size_t ns = (size_t)tp->n_samples * tp->n_st;
     DG_TRY(hipMalloc(&tp->eta, ns * sizeof(double)));
     DG_TRY(hipMalloc(&tp->obs, ns * sizeof(double)));
@@ -2545,7 +2614,8 @@ int dg_misfit_forward(dg_ctx* c, dg_tape* tp, const dg_state* x0, double t0, int
     tp->vjps = 0;
     size_t pi = 0;
     for (int i = 0; i < n_steps; ++i) {
-        if (int r = step_impl(c, tp->cur, tp->t_at[i], tp->dts[i], scheme)) return r;
+        if (int r = tp->records ? record_step(c, tp, i) : step_impl(c, tp->cur, tp->t_at[i], tp->dts[i], scheme))
+            return r;
         tp->fwd += 1;
         const int after = i + 1;
         if (after % every == 0 && after / every <= tp->n_samples) {

@@ -36,6 +36,7 @@ EXPORTS = (
     "dg_state_zero", "dg_state_wrap", "dg_set_stream", "dg_set_bed", "dg_set_manning", "dg_param_grads_zero",
     "dg_param_grads_read", "dg_rhs_vjp", "dg_stage_vjp", "dg_step_vjp", "dg_set_stations", "dg_sample_stations",
     "dg_tape_create", "dg_tape_destroy", "dg_misfit_forward", "dg_misfit_backward", "dg_tape_stats",
+    "dg_tape_keep_stages",
     "dg_revolve_count", "dg_batch_create", "dg_batch_destroy", "dg_batch_upload", "dg_batch_download", "dg_batch_run",
     "dg_batch_tuning", "dg_begin_coarse_step", "dg_stage_where", "dg_finish_where", "dg_depth_flags",
     "dg_multirate_create", "dg_multirate_destroy", "dg_multirate_assign_levels", "dg_multirate_step",
@@ -142,6 +143,7 @@ def load(fma: bool = False):
                                     _dp]
     L.dg_misfit_backward.argtypes = [vp, vp, ctypes.c_double, vp]
     L.dg_tape_stats.argtypes = [vp, i64p, i64p]
+    L.dg_tape_keep_stages.argtypes = [vp, vp, ctypes.c_int]
     L.dg_revolve_count.argtypes = [ctypes.c_int, ctypes.c_int, i64p]
     L.dg_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
     L.dg_batch_destroy.argtypes = [vp]
@@ -584,6 +586,13 @@ class Tape:
         _check(L, L.dg_misfit_backward(self.physics.ctx, self.handle, scale, None if cot_x0 is None else cot_x0.handle),
                "dg_misfit_backward")
         return self.physics.param_grads()
+
+    def keep_stages(self, on: bool = True):
+        """Keep every step's stage values and RHS in the next forward (the tape needs a
+        snapshot per step: 5 more states per SSP-RK3 step); the backward then
+        recomputes no stage and gives the same gradient bit for bit."""
+        _check(self.physics.lib, self.physics.lib.dg_tape_keep_stages(self.physics.ctx, self.handle, int(on)),
+               "dg_tape_keep_stages")
 
     def stats(self):
         f, v = ctypes.c_int64(), ctypes.c_int64()

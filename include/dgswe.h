@@ -382,6 +382,14 @@ int dg_tape_destroy(dg_ctx* ctx, dg_tape* tape);
 int dg_misfit_forward(dg_ctx* ctx, dg_tape* tape, const dg_state* x0, double t0, int n_steps, const double* dts,
                       int scheme, int every, const double* obs /* [n_steps/every][stations] */, double* misfit);
 int dg_misfit_backward(dg_ctx* ctx, dg_tape* tape, double scale, dg_state* cot_x0);
+/* Step records (no counterpart in the reference, which has no adjoint):
+ * with on != 0 the tape's next dg_misfit_forward keeps, besides
+ * a state per step, every step's stage values and stage RHS (2 stages - 1 more
+ * states per step: 5 for SSP-RK3, 7 for SSP-RK(4,3), allocated by the forward),
+ * and dg_misfit_backward reverses each step from its record without
+ * recomputing its stages. Needs snapshots >= n_steps + 1 (refused otherwise);
+ * the same gradient bit for bit. on = 0 frees the records. */
+int dg_tape_keep_stages(dg_ctx* ctx, dg_tape* tape, int on);
 /* Forward steps and step VJPs the last forward/backward pair took, and the
  * forward steps a schedule takes (host only). */
 int dg_tape_stats(const dg_tape* tape, int64_t* forward_steps, int64_t* step_vjps);

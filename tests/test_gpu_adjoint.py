@@ -175,9 +175,11 @@ class Problem:
         nd, es = phys.masks(tape_final_state(phys, self, bed, manning))
         return J, (nd, es)
 
-    def gradient(self, snapshots):
+    def gradient(self, snapshots, keep_stages=False):
         phys = self.physics()
         tape = phys.tape(snapshots)
+        if keep_stages:
+            tape.keep_stages()
         J = tape.forward(phys.state(*self.x0), 0.0, self.dts, EVERY, self.obs)
         gx0 = phys.state()
         gbed, gman = tape.backward(1.0, gx0)
@@ -207,6 +209,24 @@ def test_misfit_gradient_is_independent_of_the_checkpoint_schedule(problem):
         assert np.array_equal(gb, gb0) and np.array_equal(gm, gm0) and all(np.array_equal(a, b) for a, b in zip(gx, gx0))
         assert vjp == N_STEPS
         assert fwd == swe2d.revolve_count(N_STEPS, snaps)
+
+
+def test_step_records_give_the_same_gradient_without_recomputing(problem):
+    """A tape keeping every step's stage values and RHS reverses each step from its
+    record: the same misfit and gradients bit for bit as the tape of states, no
+    forward step beyond the first sweep; a tape without a snapshot per step is
+    refused by name."""
+    J0, gb0, gm0, gx0, _ = problem.gradient(N_STEPS + 1)
+    J, gb, gm, gx, (fwd, vjp) = problem.gradient(N_STEPS + 1, keep_stages=True)
+    assert J == J0 and fwd == N_STEPS and vjp == N_STEPS
+    assert np.array_equal(gb, gb0) and np.array_equal(gm, gm0) and all(np.array_equal(a, b) for a, b in zip(gx, gx0))
+    phys = problem.physics()
+    tape = phys.tape(N_STEPS)
+    tape.keep_stages()
+    with pytest.raises(RuntimeError, match="snapshot per step"):
+        tape.forward(phys.state(*problem.x0), 0.0, problem.dts, EVERY, problem.obs)
+    tape.keep_stages(False)
+    assert tape.forward(phys.state(*problem.x0), 0.0, problem.dts, EVERY, problem.obs) == J0
 
 
 def test_misfit_gradient_matches_tangent_linear(problem):
