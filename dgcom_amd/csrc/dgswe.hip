@@ -888,24 +888,26 @@ struct dg_tape {
     int rec_per_step = 0;
 };
 
-// One forward step of the tape keeping its record: step_impl's stages with
-// the stage values and RHS written into step i's record
-static int record_step(dg_ctx* c, dg_tape* tp, int i) {
+// One step of the tape's forward sweep from state x: step_impl's stages, the
+// last one written into dst (a checkpoint slot, so the state needs no copy
+// there) or else swapped into the tape's own state buffer; with step records the
+// stage values and RHS go into step i's record
+static int sweep_step(dg_ctx* c, dg_tape* tp, int i, const double* x, double* dst) {
     if (int r = ensure_work(c)) return r;
     const int ns = tp->scheme == DG_SSP_RK3 ? 3 : 4;
     const double(*st)[4] = tp->scheme == DG_SSP_RK3 ? RK3 : RK43;
     const size_t n3 = (size_t)3 * c->K * c->nn;
-    double* rec = tp->rec + (size_t)i * tp->rec_per_step * n3;
+    double* rec = tp->records ? tp->rec + (size_t)i * tp->rec_per_step * n3 : nullptr;
     const double t = tp->t_at[i], dt = tp->dts[i];
-    const double* input = tp->cur->d;
+    const double* input = x;
     for (int q = 0; q < ns; ++q) {
-        double* out = q < ns - 1 ? rec + (size_t)q * n3 : c->work[ns - 1]->d;
-        if (int r = launch_stage(c, tp->cur->d, input, out, st[q][0], st[q][1], st[q][2] * dt, t + st[q][3] * dt, 0,
-                                 nullptr, rec + (size_t)(ns - 1 + q) * n3))
+        double* out = q < ns - 1 ? (rec ? rec + (size_t)q * n3 : c->work[q]->d) : (dst ? dst : c->work[ns - 1]->d);
+        if (int r = launch_stage(c, x, input, out, st[q][0], st[q][1], st[q][2] * dt, t + st[q][3] * dt, 0, nullptr,
+                                 rec ? rec + (size_t)(ns - 1 + q) * n3 : nullptr))
             return r;
         input = out;
     }
-    std::swap(tp->cur->d, c->work[ns - 1]->d);  // the tape's own state buffer
+    if (!dst) std::swap(tp->cur->d, c->work[ns - 1]->d);  // the tape's own state buffer
     return 0;
 }
 
@@ -2613,20 +2615,18 @@ int dg_misfit_forward(dg_ctx* c, dg_tape* tp, const dg_state* x0, double t0, int
     tp->fwd = 0;
     tp->vjps = 0;
     size_t pi = 0;
+    const double* x = tp->cur->d;  // the current state: the tape's own buffer or the last checkpoint's slot
     for (int i = 0; i < n_steps; ++i) {
-        if (int r = tp->records ? record_step(c, tp, i) : step_impl(c, tp->cur, tp->t_at[i], tp->dts[i], scheme))
-            return r;
-        tp->fwd += 1;
         const int after = i + 1;
+        double* dst = nullptr;
+        if (pi < tp->pre.size() && tp->pre[pi].first == after) dst = tp->slot[tp->pre[pi++].second]->d;
+        if (int r = sweep_step(c, tp, i, x, dst)) return r;
+        x = dst ? dst : tp->cur->d;
+        tp->fwd += 1;
         if (after % every == 0 && after / every <= tp->n_samples) {
-            hipLaunchKernelGGL(station_sample_kernel, dim3((tp->n_st + 63) / 64), dim3(64), 0, c->stream, tp->cur->d,
-                               c->bed, c->st_elem, c->st_w, tp->n_st, c->nn,
-                               tp->eta + (size_t)(after / every - 1) * tp->n_st);
+            hipLaunchKernelGGL(station_sample_kernel, dim3((tp->n_st + 63) / 64), dim3(64), 0, c->stream, x, c->bed,
+                               c->st_elem, c->st_w, tp->n_st, c->nn, tp->eta + (size_t)(after / every - 1) * tp->n_st);
             DG_TRY(hipGetLastError());
-        }
-        if (pi < tp->pre.size() && tp->pre[pi].first == after) {
-            if (int r = tape_copy(c, tp->slot[tp->pre[pi].second], tp->cur)) return r;
-            ++pi;
         }
     }
     std::vector<double> eta(ns);
