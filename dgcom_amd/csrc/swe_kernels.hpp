@@ -535,13 +535,10 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
     double o_dist[2] = {0.0, 0.0};
     bool o_has[2] = {false, false};
     bool affine = false;
-    int lv = 0, lr = 0;  // LTS: the element's level and last-stage rate
+    int lv = 0;  // LTS: the element's level
     if (active) {
         affine = ld32(p.affine, k32) != 0;
-        if constexpr (LTS) {
-            lv = p.level[k];
-            if (p.ah) lr = p.lrate[k];
-        }
+        if constexpr (LTS) lv = p.level[k];
 #pragma unroll
         for (int r = 0; r < NT; ++r) {
             int task = i + r * NN;
@@ -965,7 +962,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
                 double rate = 0.0;
                 if (friction) {
                     double u, v;
-                    if (wet_dry && !SUB && !LTS && hin >= p.h_dry) {
+                    if (wet_dry && !SUB && hin >= p.h_dry) {
                         // desingularized_velocity of a node at or above h_dry (> h_min) is make_node's
                         // hu (1/h), already in LDS
                         u = su[t]; v = sv[t];
@@ -995,7 +992,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             // The scheme's last stage: summed into the substep's result, which
             // finish_kernel post-processes (builder.rs:716-728)
             if (active) {
-                const double w = ldexp(1.0, lv - lr);
+                const double w = ldexp(1.0, lv - (int)p.lrate[k]);  // 2^(level - last-stage rate)
                 if (lv >= p.depth) { p.ah[n] = w * vh; p.ahu[n] = w * vhu; p.ahv[n] = w * vhv; }
                 else { p.ah[n] = w * vh + p.ah[n]; p.ahu[n] = w * vhu + p.ahu[n]; p.ahv[n] = w * vhv + p.ahv[n]; }
             }
