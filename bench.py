@@ -94,8 +94,10 @@ def adjoint_bench(n, n_steps, snapshots, every, n_stations):
     sweep + reverse sweep with recomputation from `snapshots` checkpoints) over
     the wall time of one plain forward run of the same steps, the reverse
     sweep alone over the forward when every state is kept (per-step VJP cost),
-    and the same with step records (every step's stage values and RHS kept: no
-    stage recomputed) over the first 400 steps against their plain forward."""
+    the gradient of the same schedule with step records (snapshots + 1 records of
+    stage values and RHS: no reverse step recomputes its stages), and the
+    reverse sweep with every step recorded over the first 400 steps against their
+    plain forward."""
     from dgcom_amd import inversion
 
     t0 = time.time()
@@ -121,6 +123,13 @@ def adjoint_bench(n, n_steps, snapshots, every, n_stations):
     Jg, f1 = timed(lambda: tape.forward(x0, 0.0, inv.dts, every, inv.obs))
     (gbed, gman), b1 = timed(lambda: tape.backward(1.0))
     fsteps, vjps = tape.stats()
+    tape.close()
+    # the same schedule with step records (snapshots + 1 of them): no reverse step recomputes its stages
+    tape = phys.tape(snapshots)
+    tape.keep_stages()
+    Jr, f4 = timed(lambda: tape.forward(x0, 0.0, inv.dts, every, inv.obs))
+    _, b4 = timed(lambda: tape.backward(1.0))
+    used = tape.records_used()
     tape.close()
     per_step = None
     if n_steps <= 4000:
@@ -151,6 +160,9 @@ def adjoint_bench(n, n_steps, snapshots, every, n_stations):
                         f"steps, {snapshots} checkpoints",
             "elements": K, "order": c.order, "steps": n_steps, "snapshots": snapshots,
             "forward_s": fwd_s, "gradient_s": f1 + b1, "ratio": (f1 + b1) / fwd_s,
+            "with_step_records": {"gradient_s": f4 + b4, "ratio": (f4 + b4) / fwd_s, "forward_sweep_s": f4,
+                                  "reverse_sweep_s": b4, "reverse_steps_from_records": used,
+                                  "misfit_equal": Jr == Jg},
             "gradient_forward_sweep_s": f1, "gradient_reverse_sweep_s": b1,
             "forward_steps": fsteps, "step_vjps": vjps, "misfit": Jg,
             "grad_norms": {"bed": float(np.linalg.norm(gbed)), "manning": float(np.linalg.norm(gman))},
@@ -191,6 +203,13 @@ def adjoint_ensemble_bench(n_members, n_steps, snapshots):
     for phys in physics:
         phys.sync()
     grad = time.perf_counter() - w0
+    w0 = time.perf_counter()
+    grads_r = inversion.ensemble_gradients(members, physics, snapshots, keep_stages=True)
+    for phys in physics:
+        phys.sync()
+    grad_r = time.perf_counter() - w0
+    same = all(a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+               for a, b in zip(grads, grads_r))
     orders = [inv.case.order for inv in members]
     return {"workload": f"inversion ensemble: {n_members} members, K log-uniform 20000-250000 jittered quads, N in "
                         f"{{2,3}}, own GRF truth/prior beds (land in each), 12 stations every 10 steps, {n_steps} "
@@ -198,6 +217,7 @@ def adjoint_ensemble_bench(n_members, n_steps, snapshots):
             "members": n_members, "elements_total": int(sum(inv.case.n_elements for inv in members)),
             "members_per_order": {str(o): orders.count(o) for o in (2, 3)},
             "forward_runs_s": fwd, "gradients_s": grad, "ratio": grad / fwd,
+            "with_step_records": {"gradients_s": grad_r, "ratio": grad_r / fwd, "same_gradients": bool(same)},
             "misfits": [float(J) for J, _, _ in grads], "setup_s": setup}
 
 

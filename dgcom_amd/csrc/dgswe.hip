@@ -880,24 +880,42 @@ struct dg_tape {
     std::vector<std::pair<int, int>> pre;  // (step, slot) kept by the forward sweep
     int lam_cur = 0;
     double scale = 1.0;
-    // Step records (dg_tape_keep_stages): per step its stage values 0..ns-2 and
-    // its ns stage RHS, kept by the forward so that the reverse recomputes nothing
+    // Step records (dg_tape_keep_stages): a step's stage values 0..ns-2 and its ns
+    // stage RHS, so that its reverse recomputes nothing. A pool of snapshots + 1
+    // records (at most one per step): the forward sweep records the step before
+    // each checkpoint it keeps and its last step, every advance of the schedule its
+    // last step, and a reverse step takes its step's record and frees it. The
+    // schedule's reverse steps always find theirs: a record is made for the step
+    // whose state the advance (or the sweep) leaves in a slot or in the tape's
+    // state, and each level of the schedule holds at most one outstanding.
     bool keep_stages = false, records = false;
     double* rec = nullptr;
     size_t rec_bytes = 0;
     int rec_per_step = 0;
+    std::vector<int> rec_step;  // the step each record holds (-1: free)
+    int64_t rec_used = 0;       // reverse steps of the last backward taken from a record
 };
 
-// One step of the tape's forward sweep from state x: step_impl's stages, the
-// last one written into dst (a checkpoint slot, so the state needs no copy
-// there) or else swapped into the tape's own state buffer; with step records the
-// stage values and RHS go into step i's record
-static int sweep_step(dg_ctx* c, dg_tape* tp, int i, const double* x, double* dst) {
+// A free record for step i (nullptr when none is free or records are off)
+static double* rec_take(dg_ctx* c, dg_tape* tp, int i) {
+    if (!tp->records) return nullptr;
+    for (size_t r = 0; r < tp->rec_step.size(); ++r)
+        if (tp->rec_step[r] < 0) {
+            tp->rec_step[r] = i;
+            return tp->rec + r * tp->rec_per_step * (size_t)3 * c->K * c->nn;
+        }
+    return nullptr;
+}
+
+// One step i of the tape from state x: step_impl's stages, the last one written
+// into dst (a checkpoint slot, so the state needs no copy there) or else swapped
+// into the tape's own state buffer; with a record (rec) the stage values and RHS
+// go into it
+static int sweep_step(dg_ctx* c, dg_tape* tp, int i, const double* x, double* dst, double* rec) {
     if (int r = ensure_work(c)) return r;
     const int ns = tp->scheme == DG_SSP_RK3 ? 3 : 4;
     const double(*st)[4] = tp->scheme == DG_SSP_RK3 ? RK3 : RK43;
     const size_t n3 = (size_t)3 * c->K * c->nn;
-    double* rec = tp->records ? tp->rec + (size_t)i * tp->rec_per_step * n3 : nullptr;
     const double t = tp->t_at[i], dt = tp->dts[i];
     const double* input = x;
     for (int q = 0; q < ns; ++q) {
@@ -922,10 +940,14 @@ static int tape_copy(dg_ctx* c, dg_state* dst, const dg_state* src) {
     return 0;
 }
 
-// Advance tp->cur from step a to step b
+// Advance tp->cur from step a to step b (with step records, the last step
+// recorded: it is the next one the schedule reverses from this state or its slot)
 static int tape_advance(dg_ctx* c, dg_tape* tp, int a, int b) {
     for (int i = a; i < b; ++i) {
-        if (int r = step_impl(c, tp->cur, tp->t_at[i], tp->dts[i], tp->scheme)) return r;
+        double* rec = i == b - 1 ? rec_take(c, tp, i) : nullptr;
+        if (int r = rec ? sweep_step(c, tp, i, tp->cur->d, nullptr, rec)
+                        : step_impl(c, tp->cur, tp->t_at[i], tp->dts[i], tp->scheme))
+            return r;
         tp->fwd += 1;
     }
     return 0;
@@ -946,7 +968,14 @@ static int tape_step_back(dg_ctx* c, dg_tape* tp, const dg_state* xs, int i) {
         }
     }
     double* out = tp->lam[tp->lam_cur ^ 1]->d;
-    const double* record = tp->records ? tp->rec + (size_t)i * tp->rec_per_step * 3 * c->K * c->nn : nullptr;
+    const double* record = nullptr;
+    for (size_t r = 0; r < tp->rec_step.size(); ++r)
+        if (tp->rec_step[r] == i) {
+            record = tp->rec + r * tp->rec_per_step * (size_t)3 * c->K * c->nn;
+            tp->rec_step[r] = -1;  // reusable by later records (stream order: after this VJP's reads)
+            tp->rec_used += 1;
+            break;
+        }
     if (int r = step_vjp_impl(c, xs->d, tp->t_at[i], tp->dts[i], tp->scheme, lam, out, record)) return r;
     tp->lam_cur ^= 1;
     tp->vjps += 1;
@@ -2575,11 +2604,11 @@ int dg_misfit_forward(dg_ctx* c, dg_tape* tp, const dg_state* x0, double t0, int
     if (tp->obs) DG_TRY(hipFree(tp->obs));
     tp->eta = tp->obs = nullptr;
     tp->records = false;
+    tp->rec_step.clear();
     if (tp->keep_stages) {
-        DG_REQUIRE(tp->snapshots >= n_steps + 1,
-                   "dg_misfit_forward: keeping the stages needs a snapshot per step (snapshots >= n_steps + 1)");
         const int per = scheme == DG_SSP_RK3 ? 5 : 7;
-        const size_t need = (size_t)n_steps * per * 3 * c->K * c->nn * sizeof(double);
+        const int n_rec = std::min(tp->snapshots + 1, n_steps);
+        const size_t need = (size_t)n_rec * per * 3 * c->K * c->nn * sizeof(double);
         if (tp->rec_bytes != need) {
             if (tp->rec) {
                 DG_TRY(hipStreamSynchronize(c->stream));
@@ -2591,6 +2620,7 @@ int dg_misfit_forward(dg_ctx* c, dg_tape* tp, const dg_state* x0, double t0, int
             tp->rec_bytes = need;
         }
         tp->rec_per_step = per;
+        tp->rec_step.assign(n_rec, -1);
         tp->records = true;
     }
     const size_t ns = (size_t)tp->n_samples * tp->n_st;
@@ -2623,7 +2653,8 @@ int dg_misfit_forward(dg_ctx* c, dg_tape* tp, const dg_state* x0, double t0, int
         const int after = i + 1;
         double* dst = nullptr;
         if (pi < tp->pre.size() && tp->pre[pi].first == after) dst = tp->slot[tp->pre[pi++].second]->d;
-        if (int r = sweep_step(c, tp, i, x, dst)) return r;
+        double* rec = dst || after == n_steps ? rec_take(c, tp, i) : nullptr;
+        if (int r = sweep_step(c, tp, i, x, dst, rec)) return r;
         x = dst ? dst : tp->cur->d;
         tp->fwd += 1;
         if (after % every == 0 && after / every <= tp->n_samples) {
@@ -2656,6 +2687,7 @@ int dg_misfit_backward(dg_ctx* c, dg_tape* tp, double scale, dg_state* cot_x0) {
     if (int r = zero_async(c, c->gbed, (size_t)c->K * c->nn)) return r;
     if (int r = zero_async(c, c->gman, (size_t)c->K)) return r;
     tp->lam_cur = 0;
+    tp->rec_used = 0;
     if (int r = zero_async(c, tp->lam[0]->d, (size_t)3 * c->K * c->nn)) return r;
     tp->scale = scale;
     if (int r = tape_reverse(c, tp, 0, tp->n, 0, tp->snapshots - 1, true)) return r;
@@ -2663,6 +2695,12 @@ int dg_misfit_backward(dg_ctx* c, dg_tape* tp, double scale, dg_state* cot_x0) {
         DG_TRY(hipMemcpyAsync(cot_x0->d, tp->lam[tp->lam_cur]->d, (size_t)3 * c->K * c->nn * 8,
                               hipMemcpyDeviceToDevice, c->stream));
     DG_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dg_tape_records_used(const dg_tape* tp, int64_t* used) {
+    DG_REQUIRE(tp && used, "dg_tape_records_used: null argument");
+    *used = tp->rec_used;
     return 0;
 }
 

@@ -130,16 +130,19 @@ def ensemble_workload(n_members=32, k_lo=20_000, k_hi=250_000, orders=(2, 3), n_
     return out
 
 
-def ensemble_gradients(members, physics, snapshots):
+def ensemble_gradients(members, physics, snapshots, keep_stages=False):
     """Every member's station misfit and its gradients with respect to the nodal
     bed and the per-element Manning n, from its own binomial tape of `snapshots`
-    states: all the members' forward sweeps, then all their reverse sweeps, on the
-    library's stream. `members` are observed Inversions, `physics` their
+    states (with `keep_stages`, also its step records: no reverse step recomputes
+    its stages): all the members' forward sweeps, then all their reverse sweeps, on
+    the library's stream. `members` are observed Inversions, `physics` their
     SWEPhysics2D (bed at the prior, stations set). Returns [(J, dJ/dB, dJ/dn)]."""
     tapes, Js = [], []
     for inv, phys in zip(members, physics):
         c = inv.case
         tape = phys.tape(snapshots)
+        if keep_stages:
+            tape.keep_stages()
         Js.append(tape.forward(phys.state(c.h, c.hu, c.hv), 0.0, inv.dts, inv.every, inv.obs))
         tapes.append(tape)
     out = []

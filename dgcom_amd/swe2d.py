@@ -36,7 +36,7 @@ EXPORTS = (
     "dg_state_zero", "dg_state_wrap", "dg_set_stream", "dg_set_bed", "dg_set_manning", "dg_param_grads_zero",
     "dg_param_grads_read", "dg_rhs_vjp", "dg_stage_vjp", "dg_step_vjp", "dg_set_stations", "dg_sample_stations",
     "dg_tape_create", "dg_tape_destroy", "dg_misfit_forward", "dg_misfit_backward", "dg_tape_stats",
-    "dg_tape_keep_stages",
+    "dg_tape_keep_stages", "dg_tape_records_used",
     "dg_revolve_count", "dg_batch_create", "dg_batch_destroy", "dg_batch_upload", "dg_batch_download", "dg_batch_run",
     "dg_batch_tuning", "dg_begin_coarse_step", "dg_stage_where", "dg_finish_where", "dg_depth_flags",
     "dg_multirate_create", "dg_multirate_destroy", "dg_multirate_assign_levels", "dg_multirate_step",
@@ -144,6 +144,7 @@ def load(fma: bool = False):
     L.dg_misfit_backward.argtypes = [vp, vp, ctypes.c_double, vp]
     L.dg_tape_stats.argtypes = [vp, i64p, i64p]
     L.dg_tape_keep_stages.argtypes = [vp, vp, ctypes.c_int]
+    L.dg_tape_records_used.argtypes = [vp, i64p]
     L.dg_revolve_count.argtypes = [ctypes.c_int, ctypes.c_int, i64p]
     L.dg_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
     L.dg_batch_destroy.argtypes = [vp]
@@ -588,11 +589,18 @@ class Tape:
         return self.physics.param_grads()
 
     def keep_stages(self, on: bool = True):
-        """Keep every step's stage values and RHS in the next forward (the tape needs a
-        snapshot per step: 5 more states per SSP-RK3 step); the backward then
-        recomputes no stage and gives the same gradient bit for bit."""
+        """Keep step records (a step's stage values and RHS: 5 states per SSP-RK3
+        step, snapshots + 1 records at most) from the next forward on, so that no
+        reverse step recomputes its stages; the same gradient bit for bit."""
         _check(self.physics.lib, self.physics.lib.dg_tape_keep_stages(self.physics.ctx, self.handle, int(on)),
                "dg_tape_keep_stages")
+
+    def records_used(self) -> int:
+        """Reverse steps of the last backward that took their step's record."""
+        u = ctypes.c_int64()
+        _check(self.physics.lib, self.physics.lib.dg_tape_records_used(self.handle, ctypes.byref(u)),
+               "dg_tape_records_used")
+        return int(u.value)
 
     def stats(self):
         f, v = ctypes.c_int64(), ctypes.c_int64()

@@ -383,13 +383,16 @@ int dg_misfit_forward(dg_ctx* ctx, dg_tape* tape, const dg_state* x0, double t0,
                       int scheme, int every, const double* obs /* [n_steps/every][stations] */, double* misfit);
 int dg_misfit_backward(dg_ctx* ctx, dg_tape* tape, double scale, dg_state* cot_x0);
 /* Step records (no counterpart in the reference, which has no adjoint):
- * with on != 0 the tape's next dg_misfit_forward keeps, besides
- * a state per step, every step's stage values and stage RHS (2 stages - 1 more
- * states per step: 5 for SSP-RK3, 7 for SSP-RK(4,3), allocated by the forward),
- * and dg_misfit_backward reverses each step from its record without
- * recomputing its stages. Needs snapshots >= n_steps + 1 (refused otherwise);
- * the same gradient bit for bit. on = 0 frees the records. */
+ * with on != 0 the tape keeps step records — a step's stage values and stage
+ * RHS, 5 states for SSP-RK3 and 7 for SSP-RK(4,3) — so that no reverse step
+ * recomputes its stages: snapshots + 1 of them at most (one per step), made by
+ * the forward sweep (the step before each checkpoint it keeps, and its last
+ * step) and by the last step of each advance of the schedule, each taken by the
+ * reverse of its step. The same gradient bit for bit; the records are allocated
+ * by dg_misfit_forward, and on = 0 frees them. dg_tape_records_used: the reverse
+ * steps of the last backward that took their step's record. */
 int dg_tape_keep_stages(dg_ctx* ctx, dg_tape* tape, int on);
+int dg_tape_records_used(const dg_tape* tape, int64_t* used);
 /* Forward steps and step VJPs the last forward/backward pair took, and the
  * forward steps a schedule takes (host only). */
 int dg_tape_stats(const dg_tape* tape, int64_t* forward_steps, int64_t* step_vjps);

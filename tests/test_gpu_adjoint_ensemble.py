@@ -93,3 +93,13 @@ def test_each_gradient_equals_its_own_run(ensemble):
         gb1, gm1 = tape.backward()
         tape.close()
         assert J1 == J and np.array_equal(gb1, gbed) and np.array_equal(gm1, gman)
+
+
+@pytest.mark.gpu
+def test_step_records_give_the_same_member_gradients(ensemble):
+    """The members' tapes with step records (no reverse step recomputes its stages):
+    every member's misfit and gradients bit for bit those of its tape without."""
+    members, physics, grads = ensemble
+    again = inversion.ensemble_gradients(members, physics, snapshots=4, keep_stages=True)
+    for (J, gb, gm), (Jr, gbr, gmr) in zip(grads, again):
+        assert J == Jr and np.array_equal(gb, gbr) and np.array_equal(gm, gmr)
