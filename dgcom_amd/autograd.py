@@ -95,9 +95,11 @@ class _Step(torch.autograd.Function):
 
 class _StationMisfit(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, phys, bed, manning, x0, t0, dts, every, obs, snapshots, scheme):
+    def forward(ctx, phys, bed, manning, x0, t0, dts, every, obs, snapshots, scheme, keep_stages):
         _prepare(phys, bed, manning)
         tape = phys.tape(snapshots)
+        if keep_stages:
+            tape.keep_stages()
         J = tape.forward(_state_tensor(phys, x0), t0, dts, every, obs, scheme)
         ctx.tape = tape
         ctx.phys = phys
@@ -114,7 +116,7 @@ class _StationMisfit(torch.autograd.Function):
         ctx.tape.backward(float(gJ), _state_tensor(phys, gx0))
         phys.param_grads_to_device(gbed.data_ptr(), gman.data_ptr())
         ctx.tape.close()
-        return None, gbed, gman, gx0, None, None, None, None, None, None
+        return None, gbed, gman, gx0, None, None, None, None, None, None, None
 
 
 def stage(phys, base, inp, bed, manning, a, b, c, time):
@@ -125,7 +127,9 @@ def step(phys, x, bed, manning, t, dt, scheme=swe2d.SSP_RK3):
     return _Step.apply(phys, x, bed, manning, t, dt, scheme)
 
 
-def station_misfit(phys, bed, manning, x0, t0, dts, every, obs, snapshots, scheme=swe2d.SSP_RK3):
-    """Requires `phys.set_stations` first; obs is [len(dts) // every][stations]."""
+def station_misfit(phys, bed, manning, x0, t0, dts, every, obs, snapshots, scheme=swe2d.SSP_RK3, keep_stages=False):
+    """Requires `phys.set_stations` first; obs is [len(dts) // every][stations].
+    keep_stages: the tape keeps step records (no reverse step recomputes its
+    stages; 5 states per SSP-RK3 record, snapshots + 1 of them)."""
     return _StationMisfit.apply(phys, bed, manning, x0, t0, np.asarray(dts, dtype=np.float64), every,
-                                np.asarray(obs, dtype=np.float64), snapshots, scheme)
+                                np.asarray(obs, dtype=np.float64), snapshots, scheme, bool(keep_stages))

@@ -90,6 +90,13 @@ def test_step_chain_matches_the_checkpointed_misfit_gradient():
     for a, b in ((bed.grad, bed2.grad), (man.grad, man2.grad), (x.grad, x2.grad)):
         a, b = a.cpu().numpy(), b.cpu().numpy()
         assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+    # the tape with step records: the same gradients bit for bit
+    bed3 = _t(c.bed).requires_grad_(True)
+    man3 = _t(c.manning).requires_grad_(True)
+    x3 = _t(np.stack(x0)).requires_grad_(True)
+    autograd.station_misfit(phys, bed3, man3, x3, 0.0, dts, 4, obs, 3, keep_stages=True).backward()
+    for a, b in ((bed3.grad, bed2.grad), (man3.grad, man2.grad), (x3.grad, x2.grad)):
+        assert torch.equal(a, b)
 
 
 def test_gradient_step_lowers_the_misfit():
