@@ -48,3 +48,14 @@ def test_missing_library_raises(tmp_path, monkeypatch):
     monkeypatch.setattr(swe2d, "_libs", {})
     with pytest.raises(RuntimeError, match="missing"):
         swe2d.load()
+
+
+def test_host_entry_points_refuse_null_arguments():
+    """Entry points that touch no device refuse a null tape by name (no GPU needed)."""
+    L = swe2d.load()
+    v, w = ctypes.c_int64(), ctypes.c_int64()
+    assert L.dg_tape_records_used(None, ctypes.byref(v)) != 0
+    assert b"dg_tape_records_used" in L.dg_last_error()
+    assert L.dg_tape_stats(None, ctypes.byref(v), ctypes.byref(w)) != 0
+    assert b"dg_tape_stats" in L.dg_last_error()
+    assert L.dg_revolve_count(40, 5, ctypes.byref(v)) == 0 and v.value == swe2d.revolve_count(40, 5) > 40
