@@ -1639,9 +1639,9 @@ int dg_stage_where(dg_ctx* c, const dg_state* base, const dg_state* input, const
     const unsigned int nblk = (unsigned int)((n_list + epb - 1) / epb);
     const unsigned int nchunk = (nblk + 1023) / 1024;
     p.xcd_stride = (nblk + 7) / 8;
-    // (most passes list no subcell element: shallow elements sit on coarse levels, so a
-    // smaller subcell grid makes their empty launches cheaper; 2048 measured best of
-    // 2048 / 4096 / 16384 workgroups on the 400,000-element channel)
+    // (most passes list no subcell element: shallow elements sit on coarse levels; a
+    // subcell grid of 2048 workgroups measured best of 2048 / 4096 / 16384 on the
+    // 400,000-element channel, by the wall time of the run)
     dim3 grid(8 * p.xcd_stride), sub_grid(std::min(nblk, 2048u));
     switch (c->order) {
         case 1: hipLaunchKernelGGL((force ? wet_lts_kernel<1, true> : wet_lts_kernel<1>), grid, dim3(64), 0, c->stream, p); break;
