@@ -101,7 +101,7 @@ struct dg_ctx {
     // static geometry cache (geometry_kernel): geo per node; fgeo per face [3][K*4], then per face node
     // [K][4][N+1] the outer node's distance
     double *geo = nullptr, *fgeo = nullptr;
-    double* emass = nullptr;  // [K][2]: the positivity bound's static mass sums (geometry_kernel)
+    double* emass = nullptr;  // [K]: the positivity bound's static mass ratio (geometry_kernel)
     uint32_t* links = nullptr;
     uint8_t *tags = nullptr, *affine = nullptr;
     unsigned long long* clips = nullptr;
@@ -1338,7 +1338,7 @@ int dg_upload_form(dg_ctx* c, const dg_form* f) {
     DG_TRY(hipMemcpy(c->affine, aff.data(), K, hipMemcpyHostToDevice));
     DG_TRY(hipMalloc(&c->geo, (size_t)GEO_COUNT * K * nn * sizeof(double)));
     DG_TRY(hipMalloc(&c->fgeo, (size_t)(3 * 4 + 4 * (f->order + 1)) * K * sizeof(double)));
-    DG_TRY(hipMalloc(&c->emass, (size_t)2 * K * sizeof(double)));
+    DG_TRY(hipMalloc(&c->emass, (size_t)K * sizeof(double)));
     if (int r = launch_geometry(c)) return r;
     DG_TRY(hipStreamSynchronize(c->stream));
     return 0;

@@ -1026,16 +1026,15 @@ __global__ __launch_bounds__(64) void geometry_kernel(const double* corners, con
     if (active) smass[t] = T.w2[i] * bilinear(load_quad(corners, k), T.x1[a], T.x1[j]).det;
     lds_sync();
     if (active && i == 0) {
-        // the positivity bound's static sums (swe_2d.rs:2165-2203): the node masses w_i J_i over
-        // the element and over its boundary nodes, in node order
+        // the positivity bound's static ratio (swe_2d.rs:2165-2203): the node masses w_i J_i
+        // summed over the element and over its boundary nodes, in node order, and their quotient
         double weight = 0.0, weight_b = 0.0;
         for (int q = 0; q < NN; ++q) {
             const int qa = q % N1, qj = q / N1;
             weight += smass[t + q];
             if (qa == 0 || qa == N1 - 1 || qj == 0 || qj == N1 - 1) weight_b += smass[t + q];
         }
-        emass[2 * k] = weight;
-        emass[2 * k + 1] = weight_b;
+        emass[k] = weight / weight_b;
     }
     if (active) {
         const bool affine = affine_flags[k] != 0;
@@ -1271,7 +1270,7 @@ constexpr unsigned long long DT_SLOT_EMPTY = 0x7FF0000000000000ull;  // +infinit
 struct DtParams {
     const double *h, *hu, *hv;
     const double* corners;              // [K][4][2]: grad r, grad s and J of each node from its element's map
-    const double* emass;                // [K][2]: the element's node masses summed, all and on its boundary
+    const double* emass;                // [K]: the element's node masses summed, over those of its boundary nodes
     const uint32_t* links;
     const uint8_t* affine;
     int64_t n_elements;
@@ -1394,8 +1393,7 @@ __device__ __forceinline__ double dt_block(const DtParams& p, int64_t blk) {
                 if (qa == 0 || qa == N1 - 1 || qj == 0 || qj == N1 - 1) water_b += wi;
             }
             const bool shallow = sshallow[e] != 0;
-            const double weight = ld32(p.emass, k32 * 16u), weight_b = ld32(p.emass, k32 * 16u + 8u);
-            double ratio = fmin_(water / water_b, weight / weight_b);
+            double ratio = fmin_(water / water_b, ld32(p.emass, k32 * 8u));
             double rho = fmax_(0.9 * ratio, 1.0);
             const double cap_sub = p.max_cfl_sub, cap_lin = p.max_cfl;
             double cap = shallow ? cap_sub : cap_lin;  // values: selecting a field's address would put p in memory

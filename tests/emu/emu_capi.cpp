@@ -15,9 +15,9 @@ void emu_init(void) {
 }
 
 // The static geometry cache (geometry_kernel) of a form
-static std::vector<double> g_emass;  // the positivity bound's static mass sums of the last form
+static std::vector<double> g_emass;  // the positivity bound's static mass ratios of the last form
 static void run_geometry(int order, int64_t K, StageParams& p, std::vector<double>& geo, std::vector<double>& fgeo) {
-    g_emass.assign((size_t)2 * K, 0.0);
+    g_emass.assign((size_t)K, 0.0);
     size_t nn = (size_t)(order + 1) * (order + 1);
     geo.assign((size_t)GEO_COUNT * K * nn, 0.0);
     fgeo.assign((size_t)(3 * 4 + 4 * (order + 1)) * K, 0.0);
