@@ -692,8 +692,7 @@ static int ensure_adjoint(dg_ctx* c) {
 // saved it (not read for rhs_only).
 static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, const double* rhs, const double* w,
                             double* gbase, double* ginput, double a, double b, double cc, double time, int rhs_only,
-                            int input_write = 0, const uint32_t* list_in = nullptr, const unsigned int* count_in = nullptr,
-                            int base_write = 0) {
+                            int input_write = 0, const uint32_t* list_in = nullptr, const unsigned int* count_in = nullptr) {
     DG_REQUIRE(!(c->flags & DG_ENTROPY_STABLE), "the EntropyStable faces have no VJP here (the adjoint is of the WetDry faces)");
     DG_REQUIRE(!c->tide_slot, "the boundary tides per node have no VJP here (the adjoint is of the uniform tide)");
     DG_REQUIRE(!c->nest_slot, "the nesting band has no VJP here");
@@ -729,7 +728,6 @@ static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, 
     P.gbh = gbase; P.gbhu = gbase ? gbase + nt : nullptr; P.gbhv = gbase ? gbase + 2 * nt : nullptr;
     P.gih = ginput; P.gihu = ginput + nt; P.gihv = ginput + 2 * nt;
     P.input_write = input_write;
-    P.base_write = base_write;
     P.gbed = c->gbed;
     P.gman = c->gman;
     P.facebuf = c->facebuf;
@@ -808,13 +806,7 @@ static int step_vjp_impl(dg_ctx* c, const double* X, double t, double dt, int sc
     }
     auto stage_value = [&](int q) -> const double* { return record ? record + (size_t)q * n3 : c->work[q]->d; };
     auto stage_rhs = [&](int q) -> const double* { return record ? record + (size_t)(ns - 1 + q) * n3 : c->rhsbuf[q]->d; };
-    // lam gathers the stages' base cotangents a_q (...) and then stage 0's input cotangent: the
-    // last stage with a != 0, the first reversed, writes its part instead of adding it to zeros
-    int first_base = -1;
-    for (int q = ns - 1; q >= 1 && first_base < 0; --q)
-        if (st[q][0] != 0.0) first_base = q;
-    if (first_base < 0)
-        if (int r = zero_async(c, lam, n3)) return r;
+    if (int r = zero_async(c, lam, n3)) return r;
     const double* w = lam_next;
     int tb = 0;
     for (int q = ns - 1; q >= 0; --q) {
@@ -824,7 +816,7 @@ static int step_vjp_impl(dg_ctx* c, const double* X, double t, double dt, int sc
         const uint32_t* lists = record ? reinterpret_cast<const uint32_t*>(record + (size_t)(2 * ns - 1) * n3) : nullptr;
         if (int r = launch_stage_vjp(c, X, inq, stage_rhs(q), w, lam, gin, st[q][0], st[q][1], st[q][2] * dt,
                                      t + st[q][3] * dt, 0, q > 0 ? 1 : 0, record ? lists + (size_t)q * c->K : nullptr,
-                                     record ? lists + (size_t)ns * c->K + q : nullptr, q == first_base ? 1 : 0))
+                                     record ? lists + (size_t)ns * c->K + q : nullptr))
             return r;
         w = gin;
         tb ^= 1;

@@ -590,9 +590,7 @@ __device__ __forceinline__ void stage_block(const StageParams& p, int64_t k, boo
             if (!p.rhs_only && p.a != 0.0) { vpre.bh = p.bh[n]; vpre.bhu = p.bhu[n]; vpre.bhv = p.bhv[n]; }
             if (p.flags & DG_FRICTION) vpre.mn = p.manning[k];
             vpre.wh = vp->wh[n]; vpre.whu = vp->whu[n]; vpre.whv = vp->whv[n];
-            if (!p.rhs_only && p.a != 0.0 && vp->gbh && !vp->base_write) {
-                vpre.gbh = vp->gbh[n]; vpre.gbhu = vp->gbhu[n]; vpre.gbhv = vp->gbhv[n];
-            }
+            if (!p.rhs_only && p.a != 0.0 && vp->gbh) { vpre.gbh = vp->gbh[n]; vpre.gbhu = vp->gbhu[n]; vpre.gbhv = vp->gbhv[n]; }
             vpre.gbed = vp->gbed[n];
         }
         if constexpr (SUB && !VJP) {

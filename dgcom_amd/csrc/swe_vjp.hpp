@@ -42,7 +42,6 @@ struct VjpParams {
     double *gbh, *gbhu, *gbhv;     // += a * (cotangent of the combination); null: not wanted
     double *gih, *gihu, *gihv;     // += the element's own part of the input's cotangent (= with input_write)
     int input_write;               // the input's cotangent is a fresh buffer: write it instead of adding
-    int base_write;                // the base's cotangent is a fresh buffer: write a * (cotangent) instead of adding
     double* gbed;                  // += the element's own part of the bed's cotangent
     double* gman;                  // += per element Manning n cotangent (null: not wanted)
     double* facecot;               // [K][4(N+1)][3]: cotangent of F* at each face node (element kernels)
@@ -320,8 +319,7 @@ __device__ __forceinline__ void vjp_tail(const VjpParams& P, int64_t k, bool act
         // combination
         if (active) {
             if (p.a != 0.0 && P.gbh) {
-                if (P.base_write) { P.gbh[n] = p.a * gh; P.gbhu[n] = p.a * ghu; P.gbhv[n] = p.a * ghv; }
-                else { P.gbh[n] = pre.gbh + p.a * gh; P.gbhu[n] = pre.gbhu + p.a * ghu; P.gbhv[n] = pre.gbhv + p.a * ghv; }
+                P.gbh[n] = pre.gbh + p.a * gh; P.gbhu[n] = pre.gbhu + p.a * ghu; P.gbhv[n] = pre.gbhv + p.a * ghv;
             }
             if (p.b != 0.0) { gi_h += p.b * gh; gi_hu += p.b * ghu; gi_hv += p.b * ghv; }
             if (p.c != 0.0) rb = {p.c * gh, p.c * ghu, p.c * ghv};
