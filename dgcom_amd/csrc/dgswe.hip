@@ -431,8 +431,7 @@ static int set_weather(const dg_ctx* c, StageParams& p, double time) {
 }
 
 static int launch_stage(dg_ctx* c, const double* base, const double* input, double* out, double a, double b, double cc,
-                        double time, int rhs_only, double* fstar_out = nullptr, double* rhs_save = nullptr,
-                        uint32_t* list_out = nullptr, unsigned int* count_out = nullptr) {
+                        double time, int rhs_only, double* fstar_out = nullptr, double* rhs_save = nullptr) {
     size_t nt = (size_t)c->K * c->nn;
     StageParams p{};
     p.bh = base; p.bhu = base + nt; p.bhv = base + 2 * nt;
@@ -477,15 +476,12 @@ static int launch_stage(dg_ctx* c, const double* base, const double* input, doub
     // (one chunk: the compaction's own scan is the whole offset, no chunk totals)
     if (nchunk > 1)
         hipLaunchKernelGGL(chunk_count_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, nblk, c->chunk_total);
-    // (the subcell list into the context's buffer, or into a step record's, which the reverse reuses)
-    uint32_t* list = list_out ? list_out : c->sub_list;
-    unsigned int* count = list_out ? count_out : c->sub_count;
     hipLaunchKernelGGL(compact_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, c->queue, nblk,
-                       c->chunk_total, epb, list, count);
+                       c->chunk_total, epb, c->sub_list, c->sub_count);
     switch (c->order) {
-        case 1: hipLaunchKernelGGL((force ? subcell_stage_kernel<1, false, true> : subcell_stage_kernel<1>), sub_grid, dim3(64), 0, c->stream, p, list, count); break;
-        case 2: hipLaunchKernelGGL((force ? subcell_stage_kernel<2, false, true> : subcell_stage_kernel<2>), sub_grid, dim3(64), 0, c->stream, p, list, count); break;
-        default: hipLaunchKernelGGL((force ? subcell_stage_kernel<3, false, true> : subcell_stage_kernel<3>), sub_grid, dim3(64), 0, c->stream, p, list, count); break;
+        case 1: hipLaunchKernelGGL((force ? subcell_stage_kernel<1, false, true> : subcell_stage_kernel<1>), sub_grid, dim3(64), 0, c->stream, p, c->sub_list, c->sub_count); break;
+        case 2: hipLaunchKernelGGL((force ? subcell_stage_kernel<2, false, true> : subcell_stage_kernel<2>), sub_grid, dim3(64), 0, c->stream, p, c->sub_list, c->sub_count); break;
+        default: hipLaunchKernelGGL((force ? subcell_stage_kernel<3, false, true> : subcell_stage_kernel<3>), sub_grid, dim3(64), 0, c->stream, p, c->sub_list, c->sub_count); break;
     }
     DG_TRY(hipGetLastError());
     if (c->timing) {
@@ -692,7 +688,7 @@ static int ensure_adjoint(dg_ctx* c) {
 // saved it (not read for rhs_only).
 static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, const double* rhs, const double* w,
                             double* gbase, double* ginput, double a, double b, double cc, double time, int rhs_only,
-                            int input_write = 0, const uint32_t* list_in = nullptr, const unsigned int* count_in = nullptr) {
+                            int input_write = 0) {
     DG_REQUIRE(!(c->flags & DG_ENTROPY_STABLE), "the EntropyStable faces have no VJP here (the adjoint is of the WetDry faces)");
     DG_REQUIRE(!c->tide_slot, "the boundary tides per node have no VJP here (the adjoint is of the uniform tide)");
     DG_REQUIRE(!c->nest_slot, "the nesting band has no VJP here");
@@ -745,21 +741,15 @@ static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, 
         case 2: hipLaunchKernelGGL(wet_vjp_kernel<2>, grid, dim3(64), 0, c->stream, P); break;
         default: hipLaunchKernelGGL(wet_vjp_kernel<3>, grid, dim3(64), 0, c->stream, P); break;
     }
-    // The subcell elements: a step record's list of the forward stage (the same elements: the
-    // same input and test), or compacted from the wet VJP kernel's queue
-    const uint32_t* list = list_in ? list_in : c->sub_list;
-    const unsigned int* count = list_in ? count_in : c->sub_count;
-    if (!list_in) {
-        // (one chunk: the compaction's own scan is the whole offset, no chunk totals)
-        if (nchunk > 1)
-            hipLaunchKernelGGL(chunk_count_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, nblk, c->chunk_total);
-        hipLaunchKernelGGL(compact_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, c->queue, nblk,
-                           c->chunk_total, epb, c->sub_list, c->sub_count);
-    }
+    // (one chunk: the compaction's own scan is the whole offset, no chunk totals)
+    if (nchunk > 1)
+        hipLaunchKernelGGL(chunk_count_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, nblk, c->chunk_total);
+    hipLaunchKernelGGL(compact_kernel, dim3(nchunk), dim3(256), 0, c->stream, c->queue_count, c->queue, nblk,
+                       c->chunk_total, epb, c->sub_list, c->sub_count);
     switch (c->order) {
-        case 1: hipLaunchKernelGGL(subcell_vjp_kernel<1>, sub_grid, dim3(64), 0, c->stream, P, list, count); break;
-        case 2: hipLaunchKernelGGL(subcell_vjp_kernel<2>, sub_grid, dim3(64), 0, c->stream, P, list, count); break;
-        default: hipLaunchKernelGGL(subcell_vjp_kernel<3>, sub_grid, dim3(64), 0, c->stream, P, list, count); break;
+        case 1: hipLaunchKernelGGL(subcell_vjp_kernel<1>, sub_grid, dim3(64), 0, c->stream, P, c->sub_list, c->sub_count); break;
+        case 2: hipLaunchKernelGGL(subcell_vjp_kernel<2>, sub_grid, dim3(64), 0, c->stream, P, c->sub_list, c->sub_count); break;
+        default: hipLaunchKernelGGL(subcell_vjp_kernel<3>, sub_grid, dim3(64), 0, c->stream, P, c->sub_list, c->sub_count); break;
     }
     switch (c->order) {
         case 1: hipLaunchKernelGGL(edge_vjp_kernel<1>, fgrid, dim3(64), 0, c->stream, P, c->vjp_edges, c->n_vjp_edges); break;
@@ -787,7 +777,7 @@ static int launch_stage_vjp(dg_ctx* c, const double* base, const double* input, 
 // The VJP of one SSP step from state X: the stages are recomputed, saving
 // their RHS, then reversed; lam = (dX'/dX)^T lam_next (overwritten). With a
 // step record (the step's stage values 0..ns-2, then its ns stage RHS, as the
-// forward kept them, with its stages' subcell lists) nothing is recomputed.
+// forward kept them) nothing is recomputed.
 static int step_vjp_impl(dg_ctx* c, const double* X, double t, double dt, int scheme, const double* lam_next,
                          double* lam, const double* record = nullptr) {
     if (int r = ensure_work(c)) return r;
@@ -813,10 +803,8 @@ static int step_vjp_impl(dg_ctx* c, const double* X, double t, double dt, int sc
         // the stage values' cotangents are fresh buffers, written (not added to)
         const double* inq = q == 0 ? X : stage_value(q - 1);
         double* gin = q > 0 ? c->adj[tb]->d : lam;
-        const uint32_t* lists = record ? reinterpret_cast<const uint32_t*>(record + (size_t)(2 * ns - 1) * n3) : nullptr;
         if (int r = launch_stage_vjp(c, X, inq, stage_rhs(q), w, lam, gin, st[q][0], st[q][1], st[q][2] * dt,
-                                     t + st[q][3] * dt, 0, q > 0 ? 1 : 0, record ? lists + (size_t)q * c->K : nullptr,
-                                     record ? lists + (size_t)ns * c->K + q : nullptr))
+                                     t + st[q][3] * dt, 0, q > 0 ? 1 : 0))
             return r;
         w = gin;
         tb ^= 1;
@@ -892,8 +880,8 @@ struct dg_tape {
     std::vector<std::pair<int, int>> pre;  // (step, slot) kept by the forward sweep
     int lam_cur = 0;
     double scale = 1.0;
-    // Step records (dg_tape_keep_stages): a step's stage values 0..ns-2, its ns
-    // stage RHS and its stages' subcell lists, so that its reverse recomputes nothing. A pool of snapshots + 1
+    // Step records (dg_tape_keep_stages): a step's stage values 0..ns-2 and its ns
+    // stage RHS, so that its reverse recomputes nothing. A pool of snapshots + 1
     // records (at most one per step): the forward sweep records the step before
     // each checkpoint it keeps and its last step, every advance of the schedule its
     // last step, and a reverse step takes its step's record and frees it. The
@@ -903,8 +891,7 @@ struct dg_tape {
     bool keep_stages = false, records = false;
     double* rec = nullptr;
     size_t rec_bytes = 0;
-    int rec_per_step = 0;       // states per record (2 stages - 1)
-    size_t rec_stride = 0;      // doubles per record: its states, then the stages' subcell lists and counts
+    int rec_per_step = 0;
     std::vector<int> rec_step;  // the step each record holds (-1: free)
     int64_t rec_used = 0;       // reverse steps of the last backward taken from a record
 };
@@ -915,7 +902,7 @@ static double* rec_take(dg_ctx* c, dg_tape* tp, int i) {
     for (size_t r = 0; r < tp->rec_step.size(); ++r)
         if (tp->rec_step[r] < 0) {
             tp->rec_step[r] = i;
-            return tp->rec + r * tp->rec_stride;
+            return tp->rec + r * tp->rec_per_step * (size_t)3 * c->K * c->nn;
         }
     return nullptr;
 }
@@ -931,13 +918,10 @@ static int sweep_step(dg_ctx* c, dg_tape* tp, int i, const double* x, double* ds
     const size_t n3 = (size_t)3 * c->K * c->nn;
     const double t = tp->t_at[i], dt = tp->dts[i];
     const double* input = x;
-    // a record's subcell lists and counts follow its 2 ns - 1 states
-    uint32_t* lists = rec ? reinterpret_cast<uint32_t*>(rec + (size_t)(2 * ns - 1) * n3) : nullptr;
     for (int q = 0; q < ns; ++q) {
         double* out = q < ns - 1 ? (rec ? rec + (size_t)q * n3 : c->work[q]->d) : (dst ? dst : c->work[ns - 1]->d);
         if (int r = launch_stage(c, x, input, out, st[q][0], st[q][1], st[q][2] * dt, t + st[q][3] * dt, 0, nullptr,
-                                 rec ? rec + (size_t)(ns - 1 + q) * n3 : nullptr,
-                                 rec ? lists + (size_t)q * c->K : nullptr, rec ? lists + (size_t)ns * c->K + q : nullptr))
+                                 rec ? rec + (size_t)(ns - 1 + q) * n3 : nullptr))
             return r;
         input = out;
     }
@@ -987,7 +971,7 @@ static int tape_step_back(dg_ctx* c, dg_tape* tp, const dg_state* xs, int i) {
     const double* record = nullptr;
     for (size_t r = 0; r < tp->rec_step.size(); ++r)
         if (tp->rec_step[r] == i) {
-            record = tp->rec + r * tp->rec_stride;
+            record = tp->rec + r * tp->rec_per_step * (size_t)3 * c->K * c->nn;
             tp->rec_step[r] = -1;  // reusable by later records (stream order: after this VJP's reads)
             tp->rec_used += 1;
             break;
@@ -2622,12 +2606,9 @@ int dg_misfit_forward(dg_ctx* c, dg_tape* tp, const dg_state* x0, double t0, int
     tp->records = false;
     tp->rec_step.clear();
     if (tp->keep_stages) {
-        const int per = scheme == DG_SSP_RK3 ? 5 : 7, ns = scheme == DG_SSP_RK3 ? 3 : 4;
+        const int per = scheme == DG_SSP_RK3 ? 5 : 7;
         const int n_rec = std::min(tp->snapshots + 1, n_steps);
-        // a record: its states, then per stage the subcell list (K entries) and, after the lists, the counts
-        const size_t list_doubles = (((size_t)ns * c->K + ns) * sizeof(uint32_t) + 255) / 256 * 32;
-        const size_t stride = (size_t)per * 3 * c->K * c->nn + list_doubles;
-        const size_t need = (size_t)n_rec * stride * sizeof(double);
+        const size_t need = (size_t)n_rec * per * 3 * c->K * c->nn * sizeof(double);
         if (tp->rec_bytes != need) {
             if (tp->rec) {
                 DG_TRY(hipStreamSynchronize(c->stream));
@@ -2638,7 +2619,6 @@ int dg_misfit_forward(dg_ctx* c, dg_tape* tp, const dg_state* x0, double t0, int
             DG_TRY(hipMalloc(&tp->rec, need));
             tp->rec_bytes = need;
         }
-        tp->rec_stride = stride;
         tp->rec_per_step = per;
         tp->rec_step.assign(n_rec, -1);
         tp->records = true;

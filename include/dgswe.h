@@ -384,8 +384,7 @@ int dg_misfit_forward(dg_ctx* ctx, dg_tape* tape, const dg_state* x0, double t0,
 int dg_misfit_backward(dg_ctx* ctx, dg_tape* tape, double scale, dg_state* cot_x0);
 /* Step records (no counterpart in the reference, which has no adjoint):
  * with on != 0 the tape keeps step records — a step's stage values and stage
- * RHS, 5 states for SSP-RK3 and 7 for SSP-RK(4,3), and its stages' subcell
- * element lists — so that no reverse step
+ * RHS, 5 states for SSP-RK3 and 7 for SSP-RK(4,3) — so that no reverse step
  * recomputes its stages: snapshots + 1 of them at most (one per step), made by
  * the forward sweep (the step before each checkpoint it keeps, and its last
  * step) and by the last step of each advance of the schedule, each taken by the
